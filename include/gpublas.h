@@ -1,0 +1,164 @@
+/*
+ * gpublas.h -- the drop-in surface of libgpublas (MI355X / gfx950).
+ *
+ * Load it ahead of the host BLAS (LD_PRELOAD=libgpublas_preload.so, or link
+ * libgpublas.so before -lblas).  It exports:
+ *
+ *  1. Fortran-ABI BLAS entry points (all arguments by reference, 32-bit
+ *     integers, hidden Fortran string lengths ignored).  Each replaces the
+ *     reference's interposed symbol or its CBLAS-only spec:
+ *       dgemm_/sgemm_  blas_level3/gemm.cc:143-179 (prototype blas.h:202-215)
+ *       dsyrk_/ssyrk_  blas_level3/syrk.cc:148-180 (blas.h:268-274)
+ *       dtrsm_/strsm_  blas_level3/trsm.cc:134-172 (blas.h:303-314)
+ *       dsymm_/ssymm_  blas_level3/symm.cc:124-156 (blas.h:254-266)
+ *       dgemv_         blas_level2/gemv.cc:90-98 (blas.h:85-92)
+ *       dger_          blas_level2/ger.cc:86-94
+ *       dtrsv_         blas_level2/trsv.cc:81-88
+ *       daxpy_         blas_level1/axpy.cc:48-50 (blas.h:20-31)
+ *       ddot_          blas_level1/dot.cc:44-48 (returns double by value,
+ *                      not the reference's void macro blas.h:45-51)
+ *       dnrm2_         blas_level1/nrm2.cc:37-41 (returns double)
+ *       idamax_        blas_level1/amax.cc:38-43 (returns the 1-based index)
+ *     Illegal arguments call xerbla_("DGEMM ", &info, 6) of the process
+ *     (runtime-blas.c:34-52) with netlib INFO codes and return.
+ *
+ *  2. CBLAS twins cblas_dgemm ... cblas_idamax (cblas.h:57-822), row-major
+ *     handled by operand/flag swaps; cblas_idamax returns the 0-based index
+ *     (the reference returns cuBLAS's 1-based value, amax.cc:38-42).
+ *
+ *  3. malloc/calloc/realloc/free (libgpublas_preload.so only): allocations
+ *     selected by GPUBLAS_OPTIONS (heuristic=...) become tracked objects whose
+ *     contents stay resident in HBM across BLAS calls; replaces the
+ *     reference's interposer lib/obj_tracker.c:789-982.
+ *
+ *  4. The gpublas_* extension API below (options, statistics, explicit
+ *     tracked allocation, call batching).
+ */
+#ifndef GPUBLAS_H
+#define GPUBLAS_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- Fortran ABI ---------------- */
+void dgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+            const double *beta, double *c, const int *ldc);
+void sgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
+            const float *beta, float *c, const int *ldc);
+void dsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+            const double *a, const int *lda, const double *beta, double *c, const int *ldc);
+void ssyrk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+            const float *a, const int *lda, const float *beta, float *c, const int *ldc);
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const double *alpha, const double *a, const int *lda, double *b,
+            const int *ldb);
+void strsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const float *alpha, const float *a, const int *lda, float *b,
+            const int *ldb);
+void dsymm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+            double *c, const int *ldc);
+void ssymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+            float *c, const int *ldc);
+void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, const double *a,
+            const int *lda, const double *x, const int *incx, const double *beta, double *y,
+            const int *incy);
+void dger_(const int *m, const int *n, const double *alpha, const double *x, const int *incx,
+           const double *y, const int *incy, double *a, const int *lda);
+void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *a,
+            const int *lda, double *x, const int *incx);
+void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
+            const int *incy);
+double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
+double dnrm2_(const int *n, const double *x, const int *incx);
+int idamax_(const int *n, const double *x, const int *incx);
+int lsame_(const char *ca, const char *cb);
+
+/* ---------------- CBLAS (cblas.h:21-25 enum values) ---------------- */
+typedef enum { CblasRowMajor = 101, CblasColMajor = 102 } CBLAS_LAYOUT;
+typedef enum { CblasNoTrans = 111, CblasTrans = 112, CblasConjTrans = 113 } CBLAS_TRANSPOSE;
+typedef enum { CblasUpper = 121, CblasLower = 122 } CBLAS_UPLO;
+typedef enum { CblasNonUnit = 131, CblasUnit = 132 } CBLAS_DIAG;
+typedef enum { CblasLeft = 141, CblasRight = 142 } CBLAS_SIDE;
+
+void cblas_dgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
+                 double alpha, const double *a, int lda, const double *b, int ldb, double beta,
+                 double *c, int ldc);
+void cblas_sgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
+                 float alpha, const float *a, int lda, const float *b, int ldb, float beta,
+                 float *c, int ldc);
+void cblas_dsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 double alpha, const double *a, int lda, double beta, double *c, int ldc);
+void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
+                 int ldb);
+void cblas_dsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n, double alpha,
+                 const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                 int ldc);
+void cblas_dgemv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, double alpha,
+                 const double *a, int lda, const double *x, int incx, double beta, double *y,
+                 int incy);
+void cblas_dger(CBLAS_LAYOUT layout, int m, int n, double alpha, const double *x, int incx,
+                const double *y, int incy, double *a, int lda);
+void cblas_dtrsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *a, int lda, double *x, int incx);
+void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy);
+double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
+double cblas_dnrm2(int n, const double *x, int incx);
+int cblas_idamax(int n, const double *x, int incx);
+
+/* ---------------- extension API ---------------- */
+/* Statistics (the reference keeps hits/misses, runtime-mem.hpp:83,112). */
+typedef struct {
+    uint64_t calls;          /* BLAS calls entered */
+    uint64_t device_calls;   /* calls executed on the GPU */
+    uint64_t fallthrough;    /* calls handed to the host BLAS (cutoff) */
+    uint64_t hits;           /* operands found resident (tracked) */
+    uint64_t misses;         /* operands staged through a temporary device buffer */
+    uint64_t h2d_bytes, d2h_bytes;
+    uint64_t tracked_objects;/* live tracked objects */
+    uint64_t tracked_bytes;
+    uint64_t faults;         /* host accesses that pulled a tracked object back */
+    uint64_t uploads;        /* tracked objects (re)copied to HBM */
+    uint64_t allocs_seen;    /* malloc-family calls counted for ordinals */
+} gpublas_stats_t;
+void gpublas_get_stats(gpublas_stats_t *out);
+void gpublas_reset_stats(void);
+
+/* Options string (same syntax as GPUBLAS_OPTIONS); returns 0 if all parsed. */
+int gpublas_set_options(const char *opts);
+/* GEMM host fall-through cutoff: leading dimensions below it go to the host
+ * BLAS (reference rule gemm.cc:129-141, default 512); 0 = always the GPU. */
+void gpublas_set_cutoff(int cutoff);
+int gpublas_get_cutoff(void);
+
+/* Explicit tracked allocation: host-visible memory whose contents stay
+ * resident in HBM between BLAS calls.  Host reads/writes are coherent (a
+ * host touch after a GPU write pulls the object back, a host write after an
+ * upload invalidates the device copy). */
+void *gpublas_malloc(size_t bytes);
+void *gpublas_calloc(size_t nmemb, size_t size);
+void gpublas_free(void *p);
+int gpublas_is_tracked(const void *p);
+/* Device address of a tracked pointer after making the device copy valid
+ * (for callers that launch their own kernels); NULL if untracked. */
+void *gpublas_device_ptr(const void *p);
+
+/* Number of the last xerbla report on this thread and its routine name. */
+int gpublas_last_xerbla(char name[8]);
+
+/* Kernel timing with HIP events (profile option): summed ms, launch count. */
+int gpublas_kernel_time(const char *kernel, double *total_ms);
+
+/* 1 when a gfx950 device is usable. */
+int gpublas_device_ready(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+/*
+ * gpublas_dev.h -- the thin C ABI between the C host layer and the gfx950
+ * HIP kernels.  It replaces the reference's vendor-BLAS shim: the cuBLAS /
+ * clBLAS handle and calls behind runtime-blas.h:73-83 and the per-routine
+ * call sites (blas_level3/gemm.cc:62-82, syrk.cc:57-75, trsm.cc:52-72,
+ * symm.cc:57-78, and the unbuilt blas_level1/2 wrappers).
+ *
+ * Conventions: column-major, 0-based device pointers (names start with d_),
+ * Fortran leading dimensions, trans/uplo/side/diag as the netlib characters
+ * (case-insensitive).  `stream` is a hipStream_t or NULL for the calling
+ * thread's own stream.  Arguments are assumed legal (the host layer checks
+ * them netlib-style first).  Every function returns 0 on success or a
+ * hipError_t code; launches are asynchronous on the stream.
+ */
+#ifndef GPUBLAS_DEV_H
+#define GPUBLAS_DEV_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device, streams, memory (replaces runtime.h:78-125) ---- */
+int gpublas_dev_init(void);              /* 0 = a gfx950 device is ready */
+int gpublas_dev_count(void);
+const char *gpublas_dev_arch(void);      /* e.g. "gfx950:sramecc+:xnack-" */
+size_t gpublas_dev_total_mem(void);
+void *gpublas_dev_stream(void);          /* calling thread's stream */
+int gpublas_dev_sync(void *stream);
+void *gpublas_dev_alloc(size_t bytes);   /* caching pool, 256-B aligned; NULL on OOM */
+void gpublas_dev_release(void *d_ptr);   /* back to the pool; caller has synced users */
+int gpublas_dev_trim(void);              /* return cached blocks to the driver */
+size_t gpublas_dev_pool_bytes(void);     /* bytes held by the pool (in use + cached) */
+int gpublas_dev_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int gpublas_dev_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int gpublas_dev_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* 2-D copies of an m x n column-major block (element size es). */
+int gpublas_dev_h2d_2d(void *d_dst, size_t ldd, const void *h_src, size_t lds, size_t m,
+                       size_t n, size_t es, void *stream);
+int gpublas_dev_d2h_2d(void *h_dst, size_t ldd, const void *d_src, size_t lds, size_t m,
+                       size_t n, size_t es, void *stream);
+void *gpublas_dev_host_alloc(size_t bytes); /* pinned host memory */
+void gpublas_dev_host_free(void *p);
+
+/* Deterministic inputs: element i = 2*u-1, u = (splitmix64(seed,i)>>11)*2^-53. */
+int gpublas_dev_fill_uniform(void *stream, uint64_t seed, int64_t n, double *d_out);
+int gpublas_dev_fill_uniform_f32(void *stream, uint64_t seed, int64_t n, float *d_out);
+
+/* Kernel timing with HIP events on the launch stream. */
+void gpublas_dev_prof_enable(int on);
+void gpublas_dev_prof_reset(void);
+/* Resolve pending events (syncs their streams); returns the number of
+ * launches of `kernel` recorded and writes the summed / min / max durations. */
+int gpublas_dev_prof_query(const char *kernel, double *total_ms, double *min_ms, double *max_ms);
+int gpublas_dev_prof_dump(char *buf, size_t len); /* "name count total_ms\n" lines */
+
+/* ---- Level 3 ---- */
+int gpublas_dev_dgemm(void *stream, char transa, char transb, int m, int n, int k, double alpha,
+                      const double *d_A, int lda, const double *d_B, int ldb, double beta,
+                      double *d_C, int ldc);
+int gpublas_dev_sgemm(void *stream, char transa, char transb, int m, int n, int k, float alpha,
+                      const float *d_A, int lda, const float *d_B, int ldb, float beta,
+                      float *d_C, int ldc);
+int gpublas_dev_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha,
+                      const double *d_A, int lda, double beta, double *d_C, int ldc);
+int gpublas_dev_ssyrk(void *stream, char uplo, char trans, int n, int k, float alpha,
+                      const float *d_A, int lda, float beta, float *d_C, int ldc);
+int gpublas_dev_dsymm(void *stream, char side, char uplo, int m, int n, double alpha,
+                      const double *d_A, int lda, const double *d_B, int ldb, double beta,
+                      double *d_C, int ldc);
+int gpublas_dev_ssymm(void *stream, char side, char uplo, int m, int n, float alpha,
+                      const float *d_A, int lda, const float *d_B, int ldb, float beta,
+                      float *d_C, int ldc);
+int gpublas_dev_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      double alpha, const double *d_A, int lda, double *d_B, int ldb);
+int gpublas_dev_strsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      float alpha, const float *d_A, int lda, float *d_B, int ldb);
+
+/* ---- Level 2 ---- */
+int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
+                      int lda, const double *d_x, int incx, double beta, double *d_y, int incy);
+int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x, int incx,
+                     const double *d_y, int incy, double *d_A, int lda);
+int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,
+                      int lda, double *d_x, int incx);
+
+/* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
+ * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */
+int gpublas_dev_daxpy(void *stream, int n, double alpha, const double *d_x, int incx,
+                      double *d_y, int incy);
+int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const double *d_y,
+                     int incy, double *h_result);
+int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *h_result);
+int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result);
+/* Pinned result slot of the calling thread (256 bytes). */
+void *gpublas_dev_result_slot(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

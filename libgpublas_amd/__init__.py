@@ -1,0 +1,266 @@
+"""libgpublas_amd -- Python handle on the MI355X drop-in BLAS library.
+
+The product is the C-ABI shared library ``lib/libgpublas.so`` (and
+``lib/libgpublas_preload.so`` for ``LD_PRELOAD``).  This module loads it with
+ctypes and mirrors its Fortran-ABI entry points with the reference's
+argument order and meaning (``blas.h:202-314`` of Prince781/libgpublas; the
+Level-1/2 entries follow the netlib definitions the reference's CBLAS specs
+forward to), so Python callers and the test-suite drive exactly the symbols an
+interposed application calls.  Arrays are NumPy arrays in column-major
+(Fortran) layout; outputs are updated in place, as in Fortran.
+
+Loading fails loudly when the library has not been built: nothing here falls
+back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+BIN_DIR = os.path.join(_HERE, "bin")
+LIB_PATH = os.path.join(LIB_DIR, "libgpublas.so")
+PRELOAD_PATH = os.path.join(LIB_DIR, "libgpublas_preload.so")
+CHAIN_BENCH = os.path.join(BIN_DIR, "chain_bench")
+
+_lib = None
+
+
+class GpublasError(RuntimeError):
+    pass
+
+
+def library():
+    """The loaded libgpublas.so (ctypes.CDLL); raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpublasError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C libgpublas_amd/csrc)")
+        lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        _declare(lib)
+        _lib = lib
+    return _lib
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "calls", "device_calls", "fallthrough", "hits", "misses", "h2d_bytes", "d2h_bytes",
+        "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+def _declare(lib):
+    vp, ci, cd, cf, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
+    lib.gpublas_malloc.restype = vp
+    lib.gpublas_malloc.argtypes = [sz]
+    lib.gpublas_calloc.restype = vp
+    lib.gpublas_calloc.argtypes = [sz, sz]
+    lib.gpublas_free.argtypes = [vp]
+    lib.gpublas_is_tracked.argtypes = [vp]
+    lib.gpublas_device_ptr.restype = vp
+    lib.gpublas_device_ptr.argtypes = [vp]
+    lib.gpublas_set_options.argtypes = [ctypes.c_char_p]
+    lib.gpublas_set_cutoff.argtypes = [ci]
+    lib.gpublas_get_stats.argtypes = [ctypes.POINTER(Stats)]
+    lib.gpublas_kernel_time.argtypes = [ctypes.c_char_p, ctypes.POINTER(cd)]
+    lib.gpublas_last_xerbla.argtypes = [ctypes.c_char_p]
+    lib.ddot_.restype = cd
+    lib.dnrm2_.restype = cd
+    lib.idamax_.restype = ci
+    lib.cblas_ddot.restype = cd
+    lib.cblas_ddot.argtypes = [ci, vp, ci, vp, ci]
+    lib.cblas_dnrm2.restype = cd
+    lib.cblas_dnrm2.argtypes = [ci, vp, ci]
+    lib.cblas_idamax.restype = ci
+    lib.cblas_idamax.argtypes = [ci, vp, ci]
+    lib.cblas_daxpy.argtypes = [ci, cd, vp, ci, vp, ci]
+    lib.cblas_dgemm.argtypes = [ci, ci, ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.cblas_sgemm.argtypes = [ci, ci, ci, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci]
+    lib.cblas_dgemv.argtypes = [ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.cblas_dger.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, vp, ci]
+    lib.cblas_dsyrk.argtypes = [ci, ci, ci, ci, ci, cd, vp, ci, cd, vp, ci]
+    lib.cblas_dtrsm.argtypes = [ci, ci, ci, ci, ci, ci, ci, cd, vp, ci, vp, ci]
+    lib.cblas_dsymm.argtypes = [ci, ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.cblas_dtrsv.argtypes = [ci, ci, ci, ci, ci, vp, ci, vp, ci]
+    lib.gpublas_dev_alloc.restype = vp
+    lib.gpublas_dev_alloc.argtypes = [sz]
+    lib.gpublas_dev_release.argtypes = [vp]
+    lib.gpublas_dev_h2d.argtypes = [vp, vp, sz, vp]
+    lib.gpublas_dev_d2h.argtypes = [vp, vp, sz, vp]
+    lib.gpublas_dev_sync.argtypes = [vp]
+    lib.gpublas_dev_stream.restype = vp
+    lib.gpublas_dev_arch.restype = ctypes.c_char_p
+    lib.gpublas_dev_fill_uniform.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, vp]
+    lib.gpublas_dev_prof_enable.argtypes = [ci]
+    lib.gpublas_dev_prof_query.argtypes = [ctypes.c_char_p, ctypes.POINTER(cd),
+                                           ctypes.POINTER(cd), ctypes.POINTER(cd)]
+
+
+# ---------------------------------------------------------------------------
+# Fortran-ABI mirrors: every scalar by reference, arrays by address.
+# ---------------------------------------------------------------------------
+def _c(ch):
+    return ctypes.byref(ctypes.c_char(ch.encode() if isinstance(ch, str) else ch))
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _f(v):
+    return ctypes.byref(ctypes.c_float(float(v)))
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().dgemm_(_c(transa), _c(transb), _i(m), _i(n), _i(k), _d(alpha), _p(a), _i(lda),
+                     _p(b), _i(ldb), _d(beta), _p(c), _i(ldc))
+
+
+def sgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().sgemm_(_c(transa), _c(transb), _i(m), _i(n), _i(k), _f(alpha), _p(a), _i(lda),
+                     _p(b), _i(ldb), _f(beta), _p(c), _i(ldc))
+
+
+def dsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
+    library().dsyrk_(_c(uplo), _c(trans), _i(n), _i(k), _d(alpha), _p(a), _i(lda), _d(beta),
+                     _p(c), _i(ldc))
+
+
+def ssyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
+    library().ssyrk_(_c(uplo), _c(trans), _i(n), _i(k), _f(alpha), _p(a), _i(lda), _f(beta),
+                     _p(c), _i(ldc))
+
+
+def dtrsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    library().dtrsm_(_c(side), _c(uplo), _c(transa), _c(diag), _i(m), _i(n), _d(alpha), _p(a),
+                     _i(lda), _p(b), _i(ldb))
+
+
+def strsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    library().strsm_(_c(side), _c(uplo), _c(transa), _c(diag), _i(m), _i(n), _f(alpha), _p(a),
+                     _i(lda), _p(b), _i(ldb))
+
+
+def dsymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().dsymm_(_c(side), _c(uplo), _i(m), _i(n), _d(alpha), _p(a), _i(lda), _p(b),
+                     _i(ldb), _d(beta), _p(c), _i(ldc))
+
+
+def ssymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().ssymm_(_c(side), _c(uplo), _i(m), _i(n), _f(alpha), _p(a), _i(lda), _p(b),
+                     _i(ldb), _f(beta), _p(c), _i(ldc))
+
+
+def dgemv(trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    library().dgemv_(_c(trans), _i(m), _i(n), _d(alpha), _p(a), _i(lda), _p(x), _i(incx),
+                     _d(beta), _p(y), _i(incy))
+
+
+def dger(m, n, alpha, x, incx, y, incy, a, lda):
+    library().dger_(_i(m), _i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy), _p(a), _i(lda))
+
+
+def dtrsv(uplo, trans, diag, n, a, lda, x, incx):
+    library().dtrsv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(a), _i(lda), _p(x), _i(incx))
+
+
+def daxpy(n, alpha, x, incx, y, incy):
+    library().daxpy_(_i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy))
+
+
+def ddot(n, x, incx, y, incy):
+    return library().ddot_(_i(n), _p(x), _i(incx), _p(y), _i(incy))
+
+
+def dnrm2(n, x, incx):
+    return library().dnrm2_(_i(n), _p(x), _i(incx))
+
+
+def idamax(n, x, incx):
+    return library().idamax_(_i(n), _p(x), _i(incx))
+
+
+# ---------------------------------------------------------------------------
+# Extension API
+# ---------------------------------------------------------------------------
+def set_options(opts: str) -> int:
+    return library().gpublas_set_options(opts.encode())
+
+
+def set_cutoff(cutoff: int) -> None:
+    library().gpublas_set_cutoff(int(cutoff))
+
+
+def get_cutoff() -> int:
+    return library().gpublas_get_cutoff()
+
+
+def stats() -> dict:
+    s = Stats()
+    library().gpublas_get_stats(ctypes.byref(s))
+    return s.as_dict()
+
+
+def reset_stats() -> None:
+    library().gpublas_reset_stats()
+
+
+def last_xerbla():
+    buf = ctypes.create_string_buffer(8)
+    info = library().gpublas_last_xerbla(buf)
+    return buf.value.decode(), info
+
+
+def device_ready() -> bool:
+    return bool(library().gpublas_device_ready())
+
+
+def kernel_time(name: str):
+    ms = ctypes.c_double(0)
+    n = library().gpublas_kernel_time(name.encode(), ctypes.byref(ms))
+    return n, ms.value
+
+
+def tracked_empty(shape, dtype=np.float64, order="F"):
+    """NumPy array backed by a tracked (HBM-resident) object."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape)) if shape else 1
+    nbytes = max(count * dtype.itemsize, 1)
+    lib = library()
+    addr = lib.gpublas_malloc(nbytes)
+    if not addr:
+        raise MemoryError("gpublas_malloc failed")
+    buf = (ctypes.c_char * nbytes).from_address(addr)
+    arr = np.frombuffer(buf, dtype=dtype, count=count).reshape(shape, order=order)
+    weakref.finalize(buf, lib.gpublas_free, ctypes.c_void_p(addr))
+    return arr
+
+
+def tracked_copy(a, order="F"):
+    t = tracked_empty(a.shape, a.dtype, order=order)
+    t[...] = a
+    return t
+
+
+def is_tracked(a) -> bool:
+    return bool(library().gpublas_is_tracked(_p(a)))

@@ -1,0 +1,357 @@
+// blas1.hip -- Level-1 streaming kernels on gfx950: DAXPY, DDOT, DNRM2,
+// IDAMAX.
+//
+// Replaces the vendor calls the reference's (unbuilt) CBLAS wrappers make:
+// cublasDaxpy (blas_level1/axpy.cc:49), cublasDdot (dot.cc:46), cublasDnrm2
+// (nrm2.cc:39), cublasIdamax (amax.cc:40).
+//
+// The reductions are single launches: every workgroup reduces its
+// grid-stride share in a fixed order (per-lane sequential, then a wave64
+// shuffle tree, then the 4 waves in order), writes one partial, and the
+// workgroup that draws the last ticket folds the partials in index order and
+// writes the scalar straight into pinned host memory.  The grid depends only
+// on n, so the result is bitwise reproducible run to run and independent of
+// what else is on the GPU.  Tickets reset themselves, so no memset is needed
+// between calls.
+#include "devcommon.hpp"
+#include "gpublas_dev.h"
+#include <math.h>
+
+namespace {
+
+constexpr int kSlots = 64;              // reduction slots per host thread
+constexpr int kMaxBlocks = 1024;        // partials per slot
+constexpr size_t kSlotBytes = kMaxBlocks * 16;
+thread_local unsigned t_slot = 0;
+
+struct RedSlot {
+    void *partials;
+    unsigned *ticket;
+};
+
+RedSlot next_slot()
+{
+    DevCtx *c = dev_ctx();
+    char *ws = (char *)dev_workspace(kSlots * kSlotBytes);
+    unsigned s = t_slot++ % kSlots;
+    return RedSlot{ws + s * kSlotBytes, c->tickets + s};
+}
+
+int red_blocks(int n)
+{
+    long b = ((long)n + 2047) / 2048;  // 8 elements per thread
+    if (b < 1) b = 1;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    return (int)b;
+}
+
+__device__ __forceinline__ bool last_block(unsigned *ticket)
+{
+    __shared__ bool is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return r;
+}
+
+// ---- DAXPY ------------------------------------------------------------------
+__global__ void axpy_unit_v2(long n2, double a, const double2 *__restrict__ x, double2 *y)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n2; i += stride) {
+        double2 xv = x[i], yv = y[i];
+        yv.x = fma(a, xv.x, yv.x);
+        yv.y = fma(a, xv.y, yv.y);
+        y[i] = yv;
+    }
+}
+
+__global__ void axpy_strided(int n, double a, const double *x, long incx, long x0, double *y,
+                             long incy, long y0)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[y0 + i * incy] = fma(a, x[x0 + i * incx], y[y0 + i * incy]);
+}
+
+// ---- DDOT -------------------------------------------------------------------
+__global__ __launch_bounds__(256) void dot_kernel(int n, const double *x, long incx, long x0,
+                                                  const double *y, long incy, long y0,
+                                                  double *partials, unsigned *ticket,
+                                                  double *result)
+{
+    __shared__ double red[4];
+    double s = 0;
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long stride = (long)gridDim.x * 256;
+    if (incx == 1 && incy == 1 && x0 == 0 && y0 == 0) {
+        for (; i < n; i += stride) s = fma(x[i], y[i], s);
+    } else {
+        for (; i < n; i += stride) s = fma(x[x0 + i * incx], y[y0 + i * incy], s);
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (!last_block(ticket)) return;
+    double t = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) t += partials[b];
+    t = block_sum(t, red);
+    if (threadIdx.x == 0) {
+        *result = t;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- DNRM2: Blue's three-accumulator sum of squares ---------------------------
+// Thresholds and scalings for IEEE double (Anderson, "Algorithm 978: Safe
+// Scaling in the Level 1 BLAS", ACM TOMS 44(1), 2017): values above tbig are
+// summed scaled down by sbig, values below tsml scaled up by ssml, the rest
+// unscaled.  The three sums are associative, so they reduce like a dot.
+__constant__ double kTsml = 0x1p-511, kTbig = 0x1p486, kSsml = 0x1p537, kSbig = 0x1p-538;
+
+struct Acc3 {
+    double sml, med, big;
+};
+
+__device__ __forceinline__ void acc3_add(Acc3 &a, double v)
+{
+    const double ax = fabs(v);
+    if (ax > kTbig) {
+        const double t = ax * kSbig;
+        a.big = fma(t, t, a.big);
+    } else if (ax < kTsml) {
+        const double t = ax * kSsml;
+        a.sml = fma(t, t, a.sml);
+    } else {
+        a.med = fma(ax, ax, a.med);
+    }
+}
+
+__device__ double nrm2_finish(double asml, double amed, double abig)
+{
+    double scl, sumsq;
+    if (abig > 0) {
+        if (amed > 0 || isnan(amed)) abig += (amed * kSbig) * kSbig;
+        scl = 1.0 / kSbig;
+        sumsq = abig;
+    } else if (asml > 0) {
+        if (amed > 0 || isnan(amed)) {
+            const double ymed = sqrt(amed), ysml = sqrt(asml) / kSsml;
+            const double ymin = ysml > ymed ? ymed : ysml, ymax = ysml > ymed ? ysml : ymed;
+            const double q = ymin / ymax;
+            scl = 1.0;
+            sumsq = ymax * ymax * (1.0 + q * q);
+        } else {
+            scl = 1.0 / kSsml;
+            sumsq = asml;
+        }
+    } else {
+        scl = 1.0;
+        sumsq = amed;
+    }
+    return scl * sqrt(sumsq);
+}
+
+__global__ __launch_bounds__(256) void nrm2_kernel(int n, const double *x, long incx, long x0,
+                                                   double *partials, unsigned *ticket,
+                                                   double *result)
+{
+    __shared__ double red[4];
+    Acc3 a{0, 0, 0};
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long stride = (long)gridDim.x * 256;
+    for (; i < n; i += stride) acc3_add(a, x[x0 + i * incx]);
+    a.sml = block_sum(a.sml, red);
+    a.med = block_sum(a.med, red);
+    a.big = block_sum(a.big, red);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = a.sml;
+        partials[3 * blockIdx.x + 1] = a.med;
+        partials[3 * blockIdx.x + 2] = a.big;
+    }
+    if (!last_block(ticket)) return;
+    Acc3 t{0, 0, 0};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) {
+        t.sml += partials[3 * b + 0];
+        t.med += partials[3 * b + 1];
+        t.big += partials[3 * b + 2];
+    }
+    t.sml = block_sum(t.sml, red);
+    t.med = block_sum(t.med, red);
+    t.big = block_sum(t.big, red);
+    if (threadIdx.x == 0) {
+        *result = nrm2_finish(t.sml, t.med, t.big);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- IDAMAX -------------------------------------------------------------------
+// Candidate (value, index): larger |x| wins, equal |x| keeps the smaller
+// index, NaN is mapped below every number (netlib's strict '>' never moves to
+// a NaN).  If x(1) itself is NaN netlib returns 1; the final block checks it.
+struct Cand {
+    double v;
+    long i;
+};
+
+__device__ __forceinline__ Cand better(Cand a, Cand b)
+{
+    if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+    return a;
+}
+
+__device__ __forceinline__ Cand wave_best(Cand c)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Cand d;
+        d.v = __shfl_xor(c.v, o, 64);
+        d.i = __shfl_xor(c.i, o, 64);
+        c = better(c, d);
+    }
+    return c;
+}
+
+__device__ Cand block_best(Cand c, Cand *red)
+{
+    c = wave_best(c);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = c;
+    __syncthreads();
+    Cand r = red[0];
+    r = better(r, red[1]);
+    r = better(r, red[2]);
+    r = better(r, red[3]);
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long incx,
+                                                    double *partials, unsigned *ticket,
+                                                    int *result)
+{
+    __shared__ Cand red[4];
+    Cand c{-1.0, 0x7fffffffffffffffL};
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long stride = (long)gridDim.x * 256;
+    for (; i < n; i += stride) {
+        double a = fabs(x[i * incx]);
+        if (isnan(a)) a = -1.0;
+        c = better(c, Cand{a, i});
+    }
+    c = block_best(c, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = c.v;
+        ((long *)partials)[2 * blockIdx.x + 1] = c.i;
+    }
+    if (!last_block(ticket)) return;
+    Cand t{-1.0, 0x7fffffffffffffffL};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
+        t = better(t, Cand{partials[2 * b], ((long *)partials)[2 * b + 1]});
+    t = block_best(t, red);
+    if (threadIdx.x == 0) {
+        long r = t.i + 1;
+        if (isnan(x[0]) || t.v < 0) r = 1;
+        *result = (int)r;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
+
+}  // namespace
+
+extern "C" {
+
+int gpublas_dev_daxpy(void *stream, int n, double alpha, const double *d_x, int incx,
+                      double *d_y, int incy)
+{
+    if (n <= 0 || alpha == 0.0) return 0;
+    hipStream_t s = pick_stream(stream);
+    const bool unit = incx == 1 && incy == 1 && (n % 2 == 0) && (((uintptr_t)d_x & 15) == 0) &&
+                      (((uintptr_t)d_y & 15) == 0);
+    PROF_BEGIN(s, "daxpy");
+    if (unit) {
+        long n2 = n / 2;
+        int blocks = (int)((n2 + 255) / 256);
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(axpy_unit_v2, dim3(blocks), dim3(256), 0, s, n2, alpha,
+                           (const double2 *)d_x, (double2 *)d_y);
+    } else {
+        int blocks = (int)(((long)n + 255) / 256);
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(axpy_strided, dim3(blocks), dim3(256), 0, s, n, alpha, d_x, (long)incx,
+                           start_of(n, incx), d_y, (long)incy, start_of(n, incy));
+    }
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const double *d_y, int incy,
+                     double *h_result)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) {
+        *h_result = 0.0;
+        return 0;
+    }
+    RedSlot r = next_slot();
+    int blocks = red_blocks(n);
+    PROF_BEGIN(s, "ddot");
+    hipLaunchKernelGGL(dot_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
+                       start_of(n, incx), d_y, (long)incy, start_of(n, incy),
+                       (double *)r.partials, r.ticket, h_result);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *h_result)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) {
+        *h_result = 0.0;
+        return 0;
+    }
+    RedSlot r = next_slot();
+    int blocks = red_blocks(n);
+    if (blocks > kMaxBlocks / 2) blocks = kMaxBlocks / 2;  // 3 partials per block
+    PROF_BEGIN(s, "dnrm2");
+    hipLaunchKernelGGL(nrm2_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
+                       start_of(n, incx), (double *)r.partials, r.ticket, h_result);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n < 1 || incx <= 0) {
+        *h_result = 0;
+        return 0;
+    }
+    RedSlot r = next_slot();
+    int blocks = red_blocks(n);
+    PROF_BEGIN(s, "idamax");
+    hipLaunchKernelGGL(iamax_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
+                       (double *)r.partials, r.ticket, h_result);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// devcommon.hpp -- shared pieces of the gfx950 device layer.
+//
+// The device layer is a thin C ABI (include/gpublas_dev.h) over hand-written
+// HIP kernels.  Every entry point takes device addresses, a stream and plain
+// sizes; none of them allocates on the host's behalf except through the
+// caching pool in runtime.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define GPUBLAS_WAVE 64
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Per-thread device context: the stream every call of this host thread is
+// ordered on, the reduction workspace that lives on that stream, and a
+// pinned slot that kernels write scalar results into.
+struct DevCtx {
+    hipStream_t stream;
+    void *ws;              // device workspace (reduction partials)
+    size_t ws_bytes;
+    unsigned *tickets;     // device ticket counters, kept zero between launches
+    void *pinned;          // 256 B pinned host slot for scalar results
+};
+
+// Returns the calling thread's context (creates it on first use).  The stream
+// argument of the C ABI may be NULL (use this thread's stream) or a stream of
+// the caller's; workspace and tickets always come from the calling thread's
+// context, so two threads never share a workspace.
+DevCtx *dev_ctx();
+// Workspace of at least `bytes` on the calling thread's context (grows).
+void *dev_workspace(size_t bytes);
+
+static constexpr int kTicketSlots = 4096;  // ticket counters per context
+
+// Kernel timing (HIP events on the launch stream), enabled by
+// gpublas_dev_prof_enable.  Wrap each launch as
+//   PROF_BEGIN(stream, "name"); hipLaunchKernelGGL(...); PROF_END(stream);
+void prof_begin(hipStream_t s, const char *name);
+void prof_end(hipStream_t s);
+extern int g_prof_on;
+#define PROF_BEGIN(s, name) do { if (g_prof_on) prof_begin((s), (name)); } while (0)
+#define PROF_END(s) do { if (g_prof_on) prof_end((s)); } while (0)
+
+static inline hipStream_t pick_stream(void *s) { return s ? (hipStream_t)s : dev_ctx()->stream; }
+
+static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Wave-level reductions with DPP/ds_swizzle via __shfl_xor (wave64).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}

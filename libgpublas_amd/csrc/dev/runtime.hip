@@ -1,0 +1,433 @@
+// runtime.hip -- device bring-up, per-thread streams, the caching device
+// pool, host<->device copies, deterministic input generation and kernel
+// timing for the gfx950 device layer.
+//
+// Replaces the reference's device runtime shim (runtime.h:78-125,
+// runtime.c:195-260): backend-neutral alloc / memcpy there, one HIP device,
+// one stream per host thread and a size-class cache of device blocks here.
+#include "devcommon.hpp"
+#include "gpublas_dev.h"
+
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+#include <string.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+namespace {
+
+std::once_flag g_init_once;
+int g_init_status = -1;
+hipDeviceProp_t g_prop;
+
+// ---- caching pool --------------------------------------------------------
+// Device blocks are rounded to a size class (powers of two up to 2 MiB,
+// multiples of 2 MiB above) and kept on per-class free lists after release,
+// so steady-state BLAS calls on untracked operands do not hit hipMalloc /
+// hipFree (hipFree synchronises the whole device).
+std::mutex g_pool_mu;
+std::unordered_map<size_t, std::vector<void *>> *g_free;
+std::unordered_map<void *, size_t> *g_live;
+size_t g_pool_bytes = 0;
+
+size_t size_class(size_t b)
+{
+    if (b <= 256) return 256;
+    const size_t two_mib = 2u << 20;
+    if (b <= two_mib) {
+        size_t c = 256;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    return (b + two_mib - 1) / two_mib * two_mib;
+}
+
+void pool_trim_locked()
+{
+    for (auto &kv : *g_free) {
+        for (void *p : kv.second) {
+            (void)hipFree(p);
+            g_pool_bytes -= kv.first;
+        }
+        kv.second.clear();
+    }
+}
+
+// ---- timing ----------------------------------------------------------------
+struct Pending {
+    hipEvent_t a, b;
+    int name;
+};
+std::mutex g_prof_mu;
+std::vector<Pending> *g_pending;
+std::vector<hipEvent_t> *g_event_cache;
+const int kMaxNames = 128;
+const char *g_names[kMaxNames];
+int g_nnames = 0;
+struct Stat {
+    long count;
+    double total, mn, mx;
+};
+Stat g_stats[kMaxNames];
+thread_local hipEvent_t t_begin_ev = nullptr;
+thread_local int t_begin_name = -1;
+
+hipEvent_t ev_get()
+{
+    hipEvent_t e = nullptr;
+    if (!g_event_cache->empty()) {
+        e = g_event_cache->back();
+        g_event_cache->pop_back();
+        return e;
+    }
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+int name_id(const char *n)
+{
+    for (int i = 0; i < g_nnames; ++i)
+        if (g_names[i] == n || strcmp(g_names[i], n) == 0) return i;
+    if (g_nnames == kMaxNames) return kMaxNames - 1;
+    g_names[g_nnames] = n;
+    g_stats[g_nnames] = Stat{0, 0, 1e300, 0};
+    return g_nnames++;
+}
+
+void prof_resolve_locked()
+{
+    for (auto &p : *g_pending) {
+        (void)hipEventSynchronize(p.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        Stat &s = g_stats[p.name];
+        s.count++;
+        s.total += ms;
+        if (ms < s.mn) s.mn = ms;
+        if (ms > s.mx) s.mx = ms;
+        g_event_cache->push_back(p.a);
+        g_event_cache->push_back(p.b);
+    }
+    g_pending->clear();
+}
+
+// ---- deterministic inputs ------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void fill_uniform_kernel(uint64_t seed, int64_t n, double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        uint64_t z = splitmix_at(seed, (uint64_t)i);
+        double u = (double)(z >> 11) * 0x1.0p-53;
+        out[i] = 2.0 * u - 1.0;
+    }
+}
+
+__global__ void fill_uniform_f32_kernel(uint64_t seed, int64_t n, float *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        uint64_t z = splitmix_at(seed, (uint64_t)i);
+        double u = (double)(z >> 11) * 0x1.0p-53;
+        out[i] = (float)(2.0 * u - 1.0);
+    }
+}
+
+void do_init()
+{
+    g_free = new std::unordered_map<size_t, std::vector<void *>>();
+    g_live = new std::unordered_map<void *, size_t>();
+    g_pending = new std::vector<Pending>();
+    g_event_cache = new std::vector<hipEvent_t>();
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        g_init_status = -1;
+        return;
+    }
+    if (hipSetDevice(0) != hipSuccess || hipGetDeviceProperties(&g_prop, 0) != hipSuccess) {
+        g_init_status = -2;
+        return;
+    }
+    if (strncmp(g_prop.gcnArchName, "gfx950", 6) != 0) {
+        g_init_status = -3;  // code objects are built for gfx950 only
+        return;
+    }
+    g_init_status = 0;
+}
+
+}  // namespace
+
+int g_prof_on = 0;
+
+// ---- per-thread context ------------------------------------------------------
+namespace {
+struct CtxHolder {
+    DevCtx c{};
+    ~CtxHolder()
+    {
+        // Thread exit: the stream and buffers are released only if the
+        // runtime is still up (process teardown order is not guaranteed).
+        if (c.stream && g_init_status == 0) {
+            (void)hipStreamSynchronize(c.stream);
+            if (c.ws) gpublas_dev_release(c.ws);
+            if (c.tickets) (void)hipFree(c.tickets);
+            if (c.pinned) (void)hipHostFree(c.pinned);
+            (void)hipStreamDestroy(c.stream);
+        }
+    }
+};
+thread_local CtxHolder t_ctx;
+}  // namespace
+
+DevCtx *dev_ctx()
+{
+    DevCtx &c = t_ctx.c;
+    if (!c.stream) {
+        if (gpublas_dev_init() != 0) return &c;
+        (void)hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+        (void)hipMalloc(&c.tickets, kTicketSlots * sizeof(unsigned));
+        (void)hipMemsetAsync(c.tickets, 0, kTicketSlots * sizeof(unsigned), c.stream);
+        (void)hipHostMalloc(&c.pinned, 256, hipHostMallocDefault);
+        (void)hipStreamSynchronize(c.stream);
+    }
+    return &c;
+}
+
+void *dev_workspace(size_t bytes)
+{
+    DevCtx *c = dev_ctx();
+    if (c->ws_bytes < bytes) {
+        if (c->ws) {
+            (void)hipStreamSynchronize(c->stream);
+            gpublas_dev_release(c->ws);
+        }
+        size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+        c->ws = gpublas_dev_alloc(want);
+        c->ws_bytes = c->ws ? want : 0;
+    }
+    return c->ws;
+}
+
+void prof_begin(hipStream_t s, const char *name)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    t_begin_ev = ev_get();
+    t_begin_name = name_id(name);
+    (void)hipEventRecord(t_begin_ev, s);
+}
+
+void prof_end(hipStream_t s)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!t_begin_ev) return;
+    hipEvent_t b = ev_get();
+    (void)hipEventRecord(b, s);
+    g_pending->push_back(Pending{t_begin_ev, b, t_begin_name});
+    t_begin_ev = nullptr;
+}
+
+extern "C" {
+
+int gpublas_dev_init(void)
+{
+    std::call_once(g_init_once, do_init);
+    return g_init_status;
+}
+
+int gpublas_dev_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *gpublas_dev_arch(void)
+{
+    if (gpublas_dev_init() != 0) return "none";
+    return g_prop.gcnArchName;
+}
+
+size_t gpublas_dev_total_mem(void)
+{
+    if (gpublas_dev_init() != 0) return 0;
+    return g_prop.totalGlobalMem;
+}
+
+void *gpublas_dev_stream(void) { return (void *)dev_ctx()->stream; }
+
+int gpublas_dev_sync(void *stream) { return (int)hipStreamSynchronize(pick_stream(stream)); }
+
+void *gpublas_dev_alloc(size_t bytes)
+{
+    if (gpublas_dev_init() != 0) return nullptr;
+    size_t c = size_class(bytes ? bytes : 1);
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_free->find(c);
+    void *p = nullptr;
+    if (it != g_free->end() && !it->second.empty()) {
+        p = it->second.back();
+        it->second.pop_back();
+    } else {
+        if (hipMalloc(&p, c) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipDeviceSynchronize();
+            pool_trim_locked();
+            if (hipMalloc(&p, c) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        g_pool_bytes += c;
+    }
+    (*g_live)[p] = c;
+    return p;
+}
+
+void gpublas_dev_release(void *d_ptr)
+{
+    if (!d_ptr) return;
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_live->find(d_ptr);
+    if (it == g_live->end()) return;
+    (*g_free)[it->second].push_back(d_ptr);
+    g_live->erase(it);
+}
+
+int gpublas_dev_trim(void)
+{
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    (void)hipDeviceSynchronize();
+    pool_trim_locked();
+    return 0;
+}
+
+size_t gpublas_dev_pool_bytes(void)
+{
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    return g_pool_bytes;
+}
+
+int gpublas_dev_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return (int)hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, pick_stream(stream));
+}
+
+int gpublas_dev_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return (int)hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, pick_stream(stream));
+}
+
+int gpublas_dev_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    return (int)hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, pick_stream(stream));
+}
+
+int gpublas_dev_h2d_2d(void *d_dst, size_t ldd, const void *h_src, size_t lds, size_t m, size_t n,
+                       size_t es, void *stream)
+{
+    if (!m || !n) return 0;
+    return (int)hipMemcpy2DAsync(d_dst, ldd * es, h_src, lds * es, m * es, n,
+                                 hipMemcpyHostToDevice, pick_stream(stream));
+}
+
+int gpublas_dev_d2h_2d(void *h_dst, size_t ldd, const void *d_src, size_t lds, size_t m, size_t n,
+                       size_t es, void *stream)
+{
+    if (!m || !n) return 0;
+    return (int)hipMemcpy2DAsync(h_dst, ldd * es, d_src, lds * es, m * es, n,
+                                 hipMemcpyDeviceToHost, pick_stream(stream));
+}
+
+void *gpublas_dev_host_alloc(size_t bytes)
+{
+    if (gpublas_dev_init() != 0) return nullptr;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void gpublas_dev_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int gpublas_dev_fill_uniform(void *stream, uint64_t seed, int64_t n, double *d_out)
+{
+    if (n <= 0) return 0;
+    hipStream_t s = pick_stream(stream);
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3(blocks), dim3(256), 0, s, seed, n, d_out);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_fill_uniform_f32(void *stream, uint64_t seed, int64_t n, float *d_out)
+{
+    if (n <= 0) return 0;
+    hipStream_t s = pick_stream(stream);
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_uniform_f32_kernel, dim3(blocks), dim3(256), 0, s, seed, n, d_out);
+    return (int)hipGetLastError();
+}
+
+void *gpublas_dev_result_slot(void) { return dev_ctx()->pinned; }
+
+void gpublas_dev_prof_enable(int on)
+{
+    gpublas_dev_init();
+    g_prof_on = on;
+}
+
+void gpublas_dev_prof_reset(void)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (g_pending) prof_resolve_locked();
+    for (int i = 0; i < g_nnames; ++i) g_stats[i] = Stat{0, 0, 1e300, 0};
+}
+
+int gpublas_dev_prof_query(const char *kernel, double *total_ms, double *min_ms, double *max_ms)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!g_pending) return 0;
+    prof_resolve_locked();
+    for (int i = 0; i < g_nnames; ++i)
+        if (strcmp(g_names[i], kernel) == 0) {
+            if (total_ms) *total_ms = g_stats[i].total;
+            if (min_ms) *min_ms = g_stats[i].count ? g_stats[i].mn : 0;
+            if (max_ms) *max_ms = g_stats[i].mx;
+            return (int)g_stats[i].count;
+        }
+    if (total_ms) *total_ms = 0;
+    return 0;
+}
+
+int gpublas_dev_prof_dump(char *buf, size_t len)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!g_pending) return 0;
+    prof_resolve_locked();
+    size_t off = 0;
+    for (int i = 0; i < g_nnames && off < len; ++i) {
+        int w = snprintf(buf + off, len - off, "%s %ld %.6f\n", g_names[i], g_stats[i].count,
+                         g_stats[i].total);
+        if (w < 0) break;
+        off += (size_t)w;
+    }
+    return (int)off;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+/*
+ * cblas.c -- CBLAS twins of the hot-path routines (cblas.h:57-822 of the
+ * reference; enum values cblas.h:21-25).
+ *
+ * Column-major calls forward to the Fortran entries; row-major calls are the
+ * same column-major problem on the transposed operands (swap the operand
+ * roles and flip trans / uplo / side), which is exact -- no data moves.
+ * Interposing cblas_* matters because a host CBLAS does not route its cblas_
+ * functions through the Fortran symbols.
+ *
+ * Errors are reported through xerbla with the CBLAS parameter position
+ * (layout = 1), the convention of the netlib CBLAS and OpenBLAS front ends.
+ */
+#include "internal.h"
+
+static void cbl_err(const char *name6, int pos) { gb_xerbla(name6, pos); }
+
+static char tr(CBLAS_TRANSPOSE t) { return t == CblasNoTrans ? 'N' : t == CblasTrans ? 'T' : 'C'; }
+static char trflip(CBLAS_TRANSPOSE t) { return t == CblasNoTrans ? 'T' : 'N'; }
+static int tr_ok(CBLAS_TRANSPOSE t)
+{
+    return t == CblasNoTrans || t == CblasTrans || t == CblasConjTrans;
+}
+static int up_ok(CBLAS_UPLO u) { return u == CblasUpper || u == CblasLower; }
+static char up(CBLAS_UPLO u) { return u == CblasUpper ? 'U' : 'L'; }
+static char upflip(CBLAS_UPLO u) { return u == CblasUpper ? 'L' : 'U'; }
+static int side_ok(CBLAS_SIDE s) { return s == CblasLeft || s == CblasRight; }
+static char sd(CBLAS_SIDE s) { return s == CblasLeft ? 'L' : 'R'; }
+static char sdflip(CBLAS_SIDE s) { return s == CblasLeft ? 'R' : 'L'; }
+static int diag_ok(CBLAS_DIAG d) { return d == CblasUnit || d == CblasNonUnit; }
+static char dg(CBLAS_DIAG d) { return d == CblasUnit ? 'U' : 'N'; }
+static int lay_ok(CBLAS_LAYOUT l) { return l == CblasRowMajor || l == CblasColMajor; }
+
+/* Map the Fortran INFO of the forwarded call to the CBLAS position.
+ * fmap[i] = CBLAS position for Fortran INFO i (0 = none). */
+static GB_TLS const int *t_map = 0;
+static GB_TLS int t_map_n = 0;
+extern GB_TLS int gb_cblas_active;
+GB_TLS int gb_cblas_active = 0;
+
+int gb_cblas_translate(int info)
+{
+    if (!gb_cblas_active || !t_map || info <= 0 || info >= t_map_n) return info;
+    return t_map[info] ? t_map[info] : info;
+}
+
+#define WITH_MAP(map, call)                                                                      \
+    do {                                                                                         \
+        t_map = map;                                                                             \
+        t_map_n = (int)(sizeof(map) / sizeof(map[0]));                                           \
+        gb_cblas_active = 1;                                                                     \
+        call;                                                                                    \
+        gb_cblas_active = 0;                                                                     \
+        t_map = 0;                                                                               \
+    } while (0)
+
+/* ---- GEMM: cblas positions 1 layout 2 ta 3 tb 4 m 5 n 6 k 7 alpha 8 A 9 lda
+ *      10 B 11 ldb 12 beta 13 C 14 ldc ---- */
+static const int gemm_col[14] = {0, 2, 3, 4, 5, 6, 0, 0, 9, 0, 11, 0, 0, 14};
+static const int gemm_row[14] = {0, 3, 2, 5, 4, 6, 0, 0, 11, 0, 9, 0, 0, 14};
+
+#define DEFINE_CBLAS_GEMM(PRE, T, NAME6)                                                         \
+    void cblas_##PRE##gemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m,    \
+                           int n, int k, T alpha, const T *a, int lda, const T *b, int ldb,      \
+                           T beta, T *c, int ldc)                                                \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                 \
+        if (!tr_ok(ta)) { cbl_err(NAME6, 2); return; }                                                \
+        if (!tr_ok(tb)) { cbl_err(NAME6, 3); return; }                                                \
+        char cta = tr(ta), ctb = tr(tb);                                                         \
+        if (layout == CblasColMajor)                                                             \
+            WITH_MAP(gemm_col, PRE##gemm_(&cta, &ctb, &m, &n, &k, &alpha, a, &lda, b, &ldb,      \
+                                          &beta, c, &ldc));                                      \
+        else                                                                                     \
+            WITH_MAP(gemm_row, PRE##gemm_(&ctb, &cta, &n, &m, &k, &alpha, b, &ldb, a, &lda,      \
+                                          &beta, c, &ldc));                                      \
+    }
+DEFINE_CBLAS_GEMM(d, double, "DGEMM ")
+DEFINE_CBLAS_GEMM(s, float, "SGEMM ")
+
+/* ---- SYRK: 1 layout 2 uplo 3 trans 4 n 5 k 6 alpha 7 A 8 lda 9 beta 10 C 11 ldc ---- */
+static const int syrk_map[11] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 0, 11};
+
+void cblas_dsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 double alpha, const double *a, int lda, double beta, double *c, int ldc)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYRK ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYRK ", 2); return; }
+    if (!tr_ok(trans)) { cbl_err("DSYRK ", 3); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    WITH_MAP(syrk_map, dsyrk_(&u, &t, &n, &k, &alpha, a, &lda, &beta, c, &ldc));
+}
+
+/* ---- TRSM: 1 layout 2 side 3 uplo 4 ta 5 diag 6 m 7 n 8 alpha 9 A 10 lda 11 B 12 ldb ---- */
+static const int trsm_col[12] = {0, 2, 3, 4, 5, 6, 7, 0, 0, 10, 0, 12};
+static const int trsm_row[12] = {0, 2, 3, 4, 5, 7, 6, 0, 0, 10, 0, 12};
+
+void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
+                 int ldb)
+{
+    if (!lay_ok(layout)) { cbl_err("DTRSM ", 1); return; }
+    if (!side_ok(side)) { cbl_err("DTRSM ", 2); return; }
+    if (!up_ok(uplo)) { cbl_err("DTRSM ", 3); return; }
+    if (!tr_ok(ta)) { cbl_err("DTRSM ", 4); return; }
+    if (!diag_ok(diag)) { cbl_err("DTRSM ", 5); return; }
+    char t = tr(ta), d = dg(diag);
+    if (layout == CblasColMajor) {
+        char s = sd(side), u = up(uplo);
+        WITH_MAP(trsm_col, dtrsm_(&s, &u, &t, &d, &m, &n, &alpha, a, &lda, b, &ldb));
+    } else {
+        char s = sdflip(side), u = upflip(uplo);
+        WITH_MAP(trsm_row, dtrsm_(&s, &u, &t, &d, &n, &m, &alpha, a, &lda, b, &ldb));
+    }
+}
+
+/* ---- SYMM: 1 layout 2 side 3 uplo 4 m 5 n 6 alpha 7 A 8 lda 9 B 10 ldb 11 beta 12 C 13 ldc ---- */
+static const int symm_col[13] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 10, 0, 0, 13};
+static const int symm_row[13] = {0, 2, 3, 5, 4, 0, 0, 8, 0, 10, 0, 0, 13};
+
+void cblas_dsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n, double alpha,
+                 const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                 int ldc)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYMM ", 1); return; }
+    if (!side_ok(side)) { cbl_err("DSYMM ", 2); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYMM ", 3); return; }
+    if (layout == CblasColMajor) {
+        char s = sd(side), u = up(uplo);
+        WITH_MAP(symm_col, dsymm_(&s, &u, &m, &n, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
+    } else {
+        char s = sdflip(side), u = upflip(uplo);
+        WITH_MAP(symm_row, dsymm_(&s, &u, &n, &m, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
+    }
+}
+
+/* ---- GEMV: 1 layout 2 trans 3 m 4 n 5 alpha 6 A 7 lda 8 x 9 incx 10 beta 11 y 12 incy ---- */
+static const int gemv_col[12] = {0, 2, 3, 4, 0, 0, 7, 0, 9, 0, 0, 12};
+static const int gemv_row[12] = {0, 2, 4, 3, 0, 0, 7, 0, 9, 0, 0, 12};
+
+void cblas_dgemv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, double alpha,
+                 const double *a, int lda, const double *x, int incx, double beta, double *y,
+                 int incy)
+{
+    if (!lay_ok(layout)) { cbl_err("DGEMV ", 1); return; }
+    if (!tr_ok(trans)) { cbl_err("DGEMV ", 2); return; }
+    if (layout == CblasColMajor) {
+        char t = tr(trans);
+        WITH_MAP(gemv_col, dgemv_(&t, &m, &n, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+    } else {
+        char t = trflip(trans);
+        WITH_MAP(gemv_row, dgemv_(&t, &n, &m, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+    }
+}
+
+/* ---- GER: 1 layout 2 m 3 n 4 alpha 5 x 6 incx 7 y 8 incy 9 A 10 lda ---- */
+static const int ger_col[10] = {0, 2, 3, 0, 0, 6, 0, 8, 0, 10};
+static const int ger_row[10] = {0, 3, 2, 0, 0, 8, 0, 6, 0, 10};
+
+void cblas_dger(CBLAS_LAYOUT layout, int m, int n, double alpha, const double *x, int incx,
+                const double *y, int incy, double *a, int lda)
+{
+    if (!lay_ok(layout)) { cbl_err("DGER  ", 1); return; }
+    if (layout == CblasColMajor)
+        WITH_MAP(ger_col, dger_(&m, &n, &alpha, x, &incx, y, &incy, a, &lda));
+    else
+        WITH_MAP(ger_row, dger_(&n, &m, &alpha, y, &incy, x, &incx, a, &lda));
+}
+
+/* ---- TRSV: 1 layout 2 uplo 3 trans 4 diag 5 n 6 A 7 lda 8 x 9 incx ---- */
+static const int trsv_map[9] = {0, 2, 3, 4, 5, 0, 7, 0, 9};
+
+void cblas_dtrsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *a, int lda, double *x, int incx)
+{
+    if (!lay_ok(layout)) { cbl_err("DTRSV ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DTRSV ", 2); return; }
+    if (!tr_ok(trans)) { cbl_err("DTRSV ", 3); return; }
+    if (!diag_ok(diag)) { cbl_err("DTRSV ", 4); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    char d = dg(diag);
+    WITH_MAP(trsv_map, dtrsv_(&u, &t, &d, &n, a, &lda, x, &incx));
+}
+
+/* ---- Level 1 (no layout argument) ---- */
+void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy)
+{
+    daxpy_(&n, &alpha, x, &incx, y, &incy);
+}
+
+double cblas_ddot(int n, const double *x, int incx, const double *y, int incy)
+{
+    return ddot_(&n, x, &incx, y, &incy);
+}
+
+double cblas_dnrm2(int n, const double *x, int incx) { return dnrm2_(&n, x, &incx); }
+
+int cblas_idamax(int n, const double *x, int incx)
+{
+    int r = idamax_(&n, x, &incx);
+    return r ? r - 1 : 0; /* CBLAS indices are 0-based */
+}

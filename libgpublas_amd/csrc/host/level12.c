@@ -1,0 +1,297 @@
+/*
+ * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
+ * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_.
+ *
+ * The reference specifies these only through CBLAS wrappers it never builds
+ * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
+ * blas_level2/gemv.cc:11-98, ger.cc:11-94, trsv.cc:11-88; meson.build:34-76).
+ * Their behaviour is the netlib definition those wrappers forward to; the
+ * argument checks are netlib's (INFO codes in the comments).  Scalar results
+ * are written by the kernel into a pinned per-thread slot and returned by
+ * value after the stream synchronises.
+ *
+ * Host fall-through: when none of a call's operands is a tracked
+ * (device-resident) object and its length is below the GEMM cutoff, the call
+ * goes to the host BLAS -- the reference's small-problem rule (gemm.cc:129-
+ * 141) applied to vectors, where staging 2 x 4 KB through the host link
+ * costs more than the arithmetic.  cutoff=0 sends everything to the GPU.
+ */
+#include "internal.h"
+#include <math.h>
+#include <string.h>
+
+#define MAXI(a, b) ((a) > (b) ? (a) : (b))
+
+static int small_untracked(int len, const void *p, const void *q, const void *r)
+{
+    const int cut = gb_opt.cutoff;
+    if (cut <= 0 || len >= cut) return 0;
+    return !gb_ptr_tracked(p) && !gb_ptr_tracked(q) && !gb_ptr_tracked(r);
+}
+
+/* ---- DAXPY: y := alpha*x + y ---- */
+void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
+            const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0 || *alpha == 0.0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, x, y, NULL)) {
+        void (*f)(const int *, const double *, const double *, const int *, double *,
+                  const int *) = gb_host_fn("daxpy_");
+        GB_COUNT(fallthrough, 1);
+        f(n, alpha, x, incx, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_daxpy(s, *n, *alpha, (const double *)ops[0].dev, *incx,
+                               (double *)ops[1].dev, *incy),
+             "daxpy launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DDOT ---- */
+double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0) {
+        gb_leave();
+        return 0.0;
+    }
+    if (small_untracked(*n, x, y, NULL)) {
+        double (*f)(const int *, const double *, const int *, const double *, const int *) =
+            gb_host_fn("ddot_");
+        GB_COUNT(fallthrough, 1);
+        double r = f(n, x, incx, y, incy);
+        gb_leave();
+        return r;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
+    double *slot = (double *)gpublas_dev_result_slot();
+    gb_check(gpublas_dev_ddot(s, *n, (const double *)ops[0].dev, *incx,
+                              (const double *)ops[1].dev, *incy, slot),
+             "ddot launch");
+    gb_finish(s, ops, 2);
+    double r = *(volatile double *)slot;
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+    return r;
+}
+
+/* ---- DNRM2 (n <= 0 -> 0; incx == 0 repeats x(1); incx < 0 same set) ---- */
+double dnrm2_(const int *n, const double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0) {
+        gb_leave();
+        return 0.0;
+    }
+    if (small_untracked(*n, x, NULL, NULL)) {
+        double (*f)(const int *, const double *, const int *) = gb_host_fn("dnrm2_");
+        GB_COUNT(fallthrough, 1);
+        double r = f(n, x, incx);
+        gb_leave();
+        return r;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand op;
+    gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    double *slot = (double *)gpublas_dev_result_slot();
+    gb_check(gpublas_dev_dnrm2(s, *n, (const double *)op.dev, *incx, slot), "dnrm2 launch");
+    gb_finish(s, &op, 1);
+    double r = *(volatile double *)slot;
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+    return r;
+}
+
+/* ---- IDAMAX (1-based, first maximal |x_i|; n < 1 or incx <= 0 -> 0) ---- */
+int idamax_(const int *n, const double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n < 1 || *incx <= 0) {
+        gb_leave();
+        return 0;
+    }
+    if (*n == 1) {
+        gb_leave();
+        return 1;
+    }
+    if (small_untracked(*n, x, NULL, NULL)) {
+        int (*f)(const int *, const double *, const int *) = gb_host_fn("idamax_");
+        GB_COUNT(fallthrough, 1);
+        int r = f(n, x, incx);
+        gb_leave();
+        return r;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand op;
+    gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    int *slot = (int *)gpublas_dev_result_slot();
+    gb_check(gpublas_dev_idamax(s, *n, (const double *)op.dev, *incx, slot), "idamax launch");
+    gb_finish(s, &op, 1);
+    int r = *(volatile int *)slot;
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+    return r;
+}
+
+/* ---- DGEMV: y := alpha*op(A)*x + beta*y
+ * INFO: 1 trans, 2 m, 3 n, 6 lda, 8 incx, 11 incy ---- */
+void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, const double *a,
+            const int *lda, const double *x, const int *incx, const double *beta, double *y,
+            const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    const int notr = gb_lsame(*trans, 'N');
+    int info = 0;
+    if (!notr && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 1;
+    else if (*m < 0) info = 2;
+    else if (*n < 0) info = 3;
+    else if (*lda < MAXI(1, *m)) info = 6;
+    else if (*incx == 0) info = 8;
+    else if (*incy == 0) info = 11;
+    if (info) {
+        gb_xerbla("DGEMV ", info);
+        gb_leave();
+        return;
+    }
+    if (*m == 0 || *n == 0 || (*alpha == 0.0 && *beta == 1.0)) {
+        gb_leave();
+        return;
+    }
+    const int lenx = notr ? *n : *m, leny = notr ? *m : *n;
+    if (small_untracked(MAXI(*m, *n), a, x, y)) {
+        void (*f)(const char *, const int *, const int *, const double *, const double *,
+                  const int *, const double *, const int *, const double *, double *,
+                  const int *) = gb_host_fn("dgemv_");
+        GB_COUNT(fallthrough, 1);
+        f(trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    if (*alpha != 0.0) {
+        gb_acquire_matrix(&ops[0], a, *lda, *m, *n, 8, GB_IN, s);
+        gb_acquire_vector(&ops[1], x, lenx, *incx, 8, GB_IN, s);
+    }
+    gb_acquire_vector(&ops[2], y, leny, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);
+    gb_check(gpublas_dev_dgemv(s, *trans, *m, *n, *alpha, (const double *)ops[0].dev, *lda,
+                               (const double *)ops[1].dev, *incx, *beta, (double *)ops[2].dev,
+                               *incy),
+             "dgemv launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DGER: A := alpha*x*y**T + A
+ * INFO: 1 m, 2 n, 5 incx, 7 incy, 9 lda ---- */
+void dger_(const int *m, const int *n, const double *alpha, const double *x, const int *incx,
+           const double *y, const int *incy, double *a, const int *lda)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (*m < 0) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*incx == 0) info = 5;
+    else if (*incy == 0) info = 7;
+    else if (*lda < MAXI(1, *m)) info = 9;
+    if (info) {
+        gb_xerbla("DGER  ", info);
+        gb_leave();
+        return;
+    }
+    if (*m == 0 || *n == 0 || *alpha == 0.0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(MAXI(*m, *n), a, x, y)) {
+        void (*f)(const int *, const int *, const double *, const double *, const int *,
+                  const double *, const int *, double *, const int *) = gb_host_fn("dger_");
+        GB_COUNT(fallthrough, 1);
+        f(m, n, alpha, x, incx, y, incy, a, lda);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    gb_acquire_vector(&ops[0], x, *m, *incx, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
+    gb_acquire_matrix(&ops[2], a, *lda, *m, *n, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dger(s, *m, *n, *alpha, (const double *)ops[0].dev, *incx,
+                              (const double *)ops[1].dev, *incy, (double *)ops[2].dev, *lda),
+             "dger launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DTRSV: solve op(A)*x = b in place
+ * INFO: 1 uplo, 2 trans, 3 diag, 4 n, 6 lda, 8 incx ---- */
+void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *a,
+            const int *lda, double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (!gb_lsame(*trans, 'N') && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 2;
+    else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 3;
+    else if (*n < 0) info = 4;
+    else if (*lda < MAXI(1, *n)) info = 6;
+    else if (*incx == 0) info = 8;
+    if (info) {
+        gb_xerbla("DTRSV ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, a, x, NULL)) {
+        void (*f)(const char *, const char *, const char *, const int *, const double *,
+                  const int *, double *, const int *) = gb_host_fn("dtrsv_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, trans, diag, n, a, lda, x, incx);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_matrix(&ops[0], a, *lda, *n, *n, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dtrsv(s, *uplo, *trans, *diag, *n, (const double *)ops[0].dev, *lda,
+                               (double *)ops[1].dev, *incx),
+             "dtrsv launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}

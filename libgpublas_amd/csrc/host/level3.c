@@ -1,0 +1,262 @@
+/*
+ * level3.c -- Fortran-ABI Level-3 entry points: ?gemm_, ?syrk_, ?trsm_,
+ * ?symm_ for real single and double precision.
+ *
+ * Each entry follows the reference's three steps (blas_level3/gemm.cc:87-179
+ * and the syrk/trsm/symm twins): netlib argument check with xerbla, quick
+ * return, then either the host fall-through (GEMM cutoff, gemm.cc:129-141)
+ * or the device path: stage the operands (tracked objects in place, others
+ * through pool buffers), launch the gfx950 kernel, copy results back, and
+ * synchronise before returning so results are visible to the caller (the
+ * reference skips that sync on some devices, runtime.h:33-34).
+ *
+ * The alpha == 0 / k == 0 paths follow netlib (C := beta*C, exact zeros for
+ * beta == 0), not the reference's C *= beta (gemm.cc:121-126) nor its double
+ * scaling in SYRK (syrk.cc:122-145).  They run on the host when C is
+ * untracked and on the GPU when C is device-resident.
+ */
+#include "internal.h"
+#include <stddef.h>
+
+#define MAXI(a, b) ((a) > (b) ? (a) : (b))
+#define IDX(i, j, ld) ((size_t)(j) * (size_t)(ld) + (size_t)(i))
+
+/* ------------------------------------------------------------------ */
+/* GEMM                                                                */
+/* ------------------------------------------------------------------ */
+#define DEFINE_GEMM(PRE, T, NAME6, DEVFN)                                                        \
+    void PRE##gemm_(const char *transa, const char *transb, const int *m, const int *n,          \
+                    const int *k, const T *alpha, const T *a, const int *lda, const T *b,        \
+                    const int *ldb, const T *beta, T *c, const int *ldc)                          \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int nota = gb_lsame(*transa, 'N'), notb = gb_lsame(*transb, 'N');                  \
+        const int nrowa = nota ? *m : *k, nrowb = notb ? *k : *n;                                \
+        int info = 0;                                                                            \
+        if (!nota && !gb_lsame(*transa, 'C') && !gb_lsame(*transa, 'T')) info = 1;               \
+        else if (!notb && !gb_lsame(*transb, 'C') && !gb_lsame(*transb, 'T')) info = 2;          \
+        else if (*m < 0) info = 3;                                                               \
+        else if (*n < 0) info = 4;                                                               \
+        else if (*k < 0) info = 5;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 8;                                                \
+        else if (*ldb < MAXI(1, nrowb)) info = 10;                                               \
+        else if (*ldc < MAXI(1, *m)) info = 13;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0 || ((*alpha == 0 || *k == 0) && *beta == 1)) {                    \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int cut = gb_opt.cutoff;                                                           \
+        if (cut > 0 && (*lda < cut || *ldb < cut || *ldc < cut) && !gb_ptr_tracked(a) &&         \
+            !gb_ptr_tracked(b) && !gb_ptr_tracked(c)) {                                          \
+            typedef void (*fn_t)(const char *, const char *, const int *, const int *,           \
+                                 const int *, const T *, const T *, const int *, const T *,      \
+                                 const int *, const T *, T *, const int *);                      \
+            fn_t f = (fn_t)gb_host_fn(#PRE "gemm_");                                             \
+            GB_COUNT(fallthrough, 1);                                                            \
+            f(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);                     \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int scale_only = (*alpha == 0 || *k == 0);                                         \
+        if (scale_only && !gb_ptr_tracked(c)) {                                                  \
+            for (int j = 0; j < *n; ++j)                                                         \
+                for (int i = 0; i < *m; ++i)                                                     \
+                    c[IDX(i, j, *ldc)] = *beta == 0 ? (T)0 : *beta * c[IDX(i, j, *ldc)];         \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (!scale_only) {                                                                       \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nota ? *k : *m, sizeof(T), GB_IN, s);     \
+            gb_acquire_matrix(&ops[1], b, *ldb, nrowb, notb ? *n : *k, sizeof(T), GB_IN, s);     \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *m, *n, sizeof(T), *beta == 0 ? GB_OUT : GB_INOUT,  \
+                          s);                                                                    \
+        gb_check(DEVFN(s, *transa, *transb, *m, *n, *k, *alpha, (const T *)ops[0].dev, *lda,     \
+                       (const T *)ops[1].dev, *ldb, *beta, (T *)ops[2].dev, *ldc),               \
+                 #PRE "gemm launch");                                                            \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_GEMM(d, double, "DGEMM ", gpublas_dev_dgemm)
+DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm)
+
+/* ------------------------------------------------------------------ */
+/* SYRK                                                                */
+/* ------------------------------------------------------------------ */
+#define DEFINE_SYRK(PRE, T, NAME6, DEVFN)                                                        \
+    void PRE##syrk_(const char *uplo, const char *trans, const int *n, const int *k,             \
+                    const T *alpha, const T *a, const int *lda, const T *beta, T *c,             \
+                    const int *ldc)                                                              \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int notr = gb_lsame(*trans, 'N');                                                  \
+        const int nrowa = notr ? *n : *k;                                                        \
+        const int upper = gb_lsame(*uplo, 'U');                                                  \
+        int info = 0;                                                                            \
+        if (!upper && !gb_lsame(*uplo, 'L')) info = 1;                                           \
+        else if (!notr && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 2;            \
+        else if (*n < 0) info = 3;                                                               \
+        else if (*k < 0) info = 4;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 7;                                                \
+        else if (*ldc < MAXI(1, *n)) info = 10;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*n == 0 || ((*alpha == 0 || *k == 0) && *beta == 1)) {                               \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int scale_only = (*alpha == 0 || *k == 0);                                         \
+        if (scale_only && !gb_ptr_tracked(c)) {                                                  \
+            for (int j = 0; j < *n; ++j) {                                                       \
+                const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : *n;                           \
+                for (int i = i0; i < i1; ++i)                                                    \
+                    c[IDX(i, j, *ldc)] = *beta == 0 ? (T)0 : *beta * c[IDX(i, j, *ldc)];         \
+            }                                                                                    \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[2] = {{0}};                                                               \
+        if (!scale_only)                                                                         \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, notr ? *k : *n, sizeof(T), GB_IN, s);     \
+        /* only one triangle is written: the other must round-trip unchanged */                  \
+        gb_acquire_matrix(&ops[1], c, *ldc, *n, *n, sizeof(T), GB_INOUT, s);                     \
+        gb_check(DEVFN(s, *uplo, *trans, *n, *k, *alpha, (const T *)ops[0].dev, *lda, *beta,     \
+                       (T *)ops[1].dev, *ldc),                                                   \
+                 #PRE "syrk launch");                                                            \
+        gb_finish(s, ops, 2);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_SYRK(d, double, "DSYRK ", gpublas_dev_dsyrk)
+DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
+
+/* ------------------------------------------------------------------ */
+/* TRSM                                                                */
+/* ------------------------------------------------------------------ */
+#define DEFINE_TRSM(PRE, T, NAME6, DEVFN)                                                        \
+    void PRE##trsm_(const char *side, const char *uplo, const char *transa, const char *diag,    \
+                    const int *m, const int *n, const T *alpha, const T *a, const int *lda,      \
+                    T *b, const int *ldb)                                                        \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int lside = gb_lsame(*side, 'L');                                                  \
+        const int nrowa = lside ? *m : *n;                                                       \
+        const int upper = gb_lsame(*uplo, 'U');                                                  \
+        int info = 0;                                                                            \
+        if (!lside && !gb_lsame(*side, 'R')) info = 1;                                           \
+        else if (!upper && !gb_lsame(*uplo, 'L')) info = 2;                                      \
+        else if (!gb_lsame(*transa, 'N') && !gb_lsame(*transa, 'T') && !gb_lsame(*transa, 'C'))  \
+            info = 3;                                                                            \
+        else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 4;                       \
+        else if (*m < 0) info = 5;                                                               \
+        else if (*n < 0) info = 6;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 9;                                                \
+        else if (*ldb < MAXI(1, *m)) info = 11;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0) {                                                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*alpha == 0 && !gb_ptr_tracked(b)) {                                                 \
+            for (int j = 0; j < *n; ++j)                                                         \
+                for (int i = 0; i < *m; ++i) b[IDX(i, j, *ldb)] = 0;                             \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[2] = {{0}};                                                               \
+        if (*alpha != 0)                                                                         \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nrowa, sizeof(T), GB_IN, s);              \
+        gb_acquire_matrix(&ops[1], b, *ldb, *m, *n, sizeof(T), *alpha == 0 ? GB_OUT : GB_INOUT,  \
+                          s);                                                                    \
+        gb_check(DEVFN(s, *side, *uplo, *transa, *diag, *m, *n, *alpha, (const T *)ops[0].dev,   \
+                       *lda, (T *)ops[1].dev, *ldb),                                             \
+                 #PRE "trsm launch");                                                            \
+        gb_finish(s, ops, 2);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_TRSM(d, double, "DTRSM ", gpublas_dev_dtrsm)
+DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm)
+
+/* ------------------------------------------------------------------ */
+/* SYMM                                                                */
+/* ------------------------------------------------------------------ */
+#define DEFINE_SYMM(PRE, T, NAME6, DEVFN)                                                        \
+    void PRE##symm_(const char *side, const char *uplo, const int *m, const int *n,              \
+                    const T *alpha, const T *a, const int *lda, const T *b, const int *ldb,      \
+                    const T *beta, T *c, const int *ldc)                                         \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int lside = gb_lsame(*side, 'L');                                                  \
+        const int nrowa = lside ? *m : *n;                                                       \
+        const int upper = gb_lsame(*uplo, 'U');                                                  \
+        int info = 0;                                                                            \
+        if (!lside && !gb_lsame(*side, 'R')) info = 1;                                           \
+        else if (!upper && !gb_lsame(*uplo, 'L')) info = 2;                                      \
+        else if (*m < 0) info = 3;                                                               \
+        else if (*n < 0) info = 4;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 7;                                                \
+        else if (*ldb < MAXI(1, *m)) info = 9;                                                   \
+        else if (*ldc < MAXI(1, *m)) info = 12;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0 || (*alpha == 0 && *beta == 1)) {                                 \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*alpha == 0 && !gb_ptr_tracked(c)) {                                                 \
+            for (int j = 0; j < *n; ++j)                                                         \
+                for (int i = 0; i < *m; ++i)                                                     \
+                    c[IDX(i, j, *ldc)] = *beta == 0 ? (T)0 : *beta * c[IDX(i, j, *ldc)];         \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (*alpha != 0) {                                                                       \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nrowa, sizeof(T), GB_IN, s);              \
+            gb_acquire_matrix(&ops[1], b, *ldb, *m, *n, sizeof(T), GB_IN, s);                    \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *m, *n, sizeof(T), *beta == 0 ? GB_OUT : GB_INOUT,  \
+                          s);                                                                    \
+        gb_check(DEVFN(s, *side, *uplo, *m, *n, *alpha, (const T *)ops[0].dev, *lda,             \
+                       (const T *)ops[1].dev, *ldb, *beta, (T *)ops[2].dev, *ldc),               \
+                 #PRE "symm launch");                                                            \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_SYMM(d, double, "DSYMM ", gpublas_dev_dsymm)
+DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm)

@@ -1,0 +1,613 @@
+/*
+ * oracle/refblas.c -- CPU restatement of the reference's BLAS semantics for
+ * the interposed hot path.  TEST INFRASTRUCTURE ONLY (see refblas.h).
+ *
+ * Argument checks follow the reference's netlib-style check macros:
+ *   gemm  gemm.cc:87-127   (info 1,2,3,4,5,8,10,13)
+ *   syrk  syrk.cc:78-146   (info 1,2,3,4,7,10)
+ *   trsm  trsm.cc:76-132   (info 1,2,3,4,5,6,9,11)
+ *   symm  symm.cc:81-122   (info 1,2,3,4,7,9,12)
+ * and, for routines the reference only specifies through its unbuilt CBLAS
+ * wrappers (blas_level1/axpy.cc, dot.cc, nrm2.cc, amax.cc,
+ * blas_level2/gemv.cc, ger.cc, trsv.cc), the netlib reference BLAS
+ * definition those wrappers forward to.  Quick returns follow netlib, not the
+ * reference's known defects (gemm.cc:121-126 multiplies C by beta==0, which
+ * propagates NaN; syrk.cc:122-145 scales C twice): netlib sets C = 0 when
+ * beta == 0 and returns after the alpha == 0 scaling.
+ *
+ * Arithmetic follows the netlib loop order (column sweeps with axpy inner
+ * loops for the 'N' forms, dot-product inner loops for the 'T' forms), which
+ * is what the fall-through host BLAS computes up to summation order.
+ */
+#include "refblas.h"
+#include <math.h>
+#include <stddef.h>
+#include <ctype.h>
+
+int ora_lsame(char a, char b) { return toupper((unsigned char)a) == toupper((unsigned char)b); }
+
+static int imax(int a, int b) { return a > b ? a : b; }
+
+#define IX(i, j, ld) ((size_t)(j) * (size_t)(ld) + (size_t)(i))
+
+/* ------------------------------------------------------------------ */
+/* INFO codes                                                          */
+/* ------------------------------------------------------------------ */
+int ora_gemm_info(char ta, char tb, int m, int n, int k, int lda, int ldb, int ldc)
+{
+    /* gemm.cc:87-116 */
+    int nota = ora_lsame(ta, 'N'), notb = ora_lsame(tb, 'N');
+    int nrowa = nota ? m : k, nrowb = notb ? k : n;
+    if (!nota && !ora_lsame(ta, 'C') && !ora_lsame(ta, 'T')) return 1;
+    if (!notb && !ora_lsame(tb, 'C') && !ora_lsame(tb, 'T')) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (k < 0) return 5;
+    if (lda < imax(1, nrowa)) return 8;
+    if (ldb < imax(1, nrowb)) return 10;
+    if (ldc < imax(1, m)) return 13;
+    return 0;
+}
+
+int ora_syrk_info(char uplo, char trans, int n, int k, int lda, int ldc)
+{
+    /* syrk.cc:87-115 */
+    int nrowa = ora_lsame(trans, 'N') ? n : k;
+    int upper = ora_lsame(uplo, 'U');
+    if (!upper && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    if (lda < imax(1, nrowa)) return 7;
+    if (ldc < imax(1, n)) return 10;
+    return 0;
+}
+
+int ora_trsm_info(char side, char uplo, char ta, char diag, int m, int n, int lda, int ldb)
+{
+    /* trsm.cc:84-116 */
+    int lside = ora_lsame(side, 'L');
+    int nrowa = lside ? m : n;
+    int upper = ora_lsame(uplo, 'U');
+    if (!lside && !ora_lsame(side, 'R')) return 1;
+    if (!upper && !ora_lsame(uplo, 'L')) return 2;
+    if (!ora_lsame(ta, 'N') && !ora_lsame(ta, 'T') && !ora_lsame(ta, 'C')) return 3;
+    if (!ora_lsame(diag, 'U') && !ora_lsame(diag, 'N')) return 4;
+    if (m < 0) return 5;
+    if (n < 0) return 6;
+    if (lda < imax(1, nrowa)) return 9;
+    if (ldb < imax(1, m)) return 11;
+    return 0;
+}
+
+int ora_symm_info(char side, char uplo, int m, int n, int lda, int ldb, int ldc)
+{
+    /* symm.cc:81-104 */
+    int nrowa = ora_lsame(side, 'L') ? m : n;
+    int upper = ora_lsame(uplo, 'U');
+    if (!ora_lsame(side, 'L') && !ora_lsame(side, 'R')) return 1;
+    if (!upper && !ora_lsame(uplo, 'L')) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (lda < imax(1, nrowa)) return 7;
+    if (ldb < imax(1, m)) return 9;
+    if (ldc < imax(1, m)) return 12;
+    return 0;
+}
+
+int ora_gemv_info(char trans, int m, int n, int lda, int incx, int incy)
+{
+    /* netlib DGEMV, the routine cblas_dgemv forwards to (gemv.cc:11-98) */
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 1;
+    if (m < 0) return 2;
+    if (n < 0) return 3;
+    if (lda < imax(1, m)) return 6;
+    if (incx == 0) return 8;
+    if (incy == 0) return 11;
+    return 0;
+}
+
+int ora_ger_info(int m, int n, int incx, int incy, int lda)
+{
+    /* netlib DGER (ger.cc:11-94) */
+    if (m < 0) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 5;
+    if (incy == 0) return 7;
+    if (lda < imax(1, m)) return 9;
+    return 0;
+}
+
+int ora_trsv_info(char uplo, char trans, char diag, int n, int lda, int incx)
+{
+    /* netlib DTRSV (trsv.cc:11-88) */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (!ora_lsame(diag, 'U') && !ora_lsame(diag, 'N')) return 3;
+    if (n < 0) return 4;
+    if (lda < imax(1, n)) return 6;
+    if (incx == 0) return 8;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Level 3, generated for double and float                            */
+/* ------------------------------------------------------------------ */
+#define DEFINE_GEMM(NAME, T)                                                                  \
+void NAME(char ta, char tb, int m, int n, int k, T alpha, const T *A, int lda, const T *B,    \
+          int ldb, T beta, T *C, int ldc)                                                     \
+{                                                                                             \
+    int nota = ora_lsame(ta, 'N'), notb = ora_lsame(tb, 'N');                                 \
+    if (ora_gemm_info(ta, tb, m, n, k, lda, ldb, ldc)) return;                                \
+    if (m == 0 || n == 0 || ((alpha == 0 || k == 0) && beta == 1)) return;                    \
+    if (alpha == 0) {                                                                         \
+        for (int j = 0; j < n; ++j)                                                           \
+            for (int i = 0; i < m; ++i)                                                       \
+                C[IX(i, j, ldc)] = beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)];                \
+        return;                                                                               \
+    }                                                                                         \
+    if (nota) {                                                                               \
+        /* C := alpha*A*op(B) + beta*C, column axpy form */                                   \
+        for (int j = 0; j < n; ++j) {                                                         \
+            if (beta == 0)                                                                    \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] = 0;                             \
+            else if (beta != 1)                                                               \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] *= beta;                         \
+            for (int l = 0; l < k; ++l) {                                                     \
+                T temp = alpha * (notb ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)]);                \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] += temp * A[IX(i, l, lda)];      \
+            }                                                                                 \
+        }                                                                                     \
+    } else {                                                                                  \
+        /* C := alpha*A**T*op(B) + beta*C, dot form */                                        \
+        for (int j = 0; j < n; ++j)                                                           \
+            for (int i = 0; i < m; ++i) {                                                     \
+                T temp = 0;                                                                   \
+                for (int l = 0; l < k; ++l)                                                   \
+                    temp += A[IX(l, i, lda)] * (notb ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)]);  \
+                C[IX(i, j, ldc)] = beta == 0 ? alpha * temp                                   \
+                                             : alpha * temp + beta * C[IX(i, j, ldc)];        \
+            }                                                                                 \
+    }                                                                                         \
+}
+DEFINE_GEMM(ora_dgemm, double)
+DEFINE_GEMM(ora_sgemm, float)
+
+#define DEFINE_SYRK(NAME, T)                                                                  \
+void NAME(char uplo, char trans, int n, int k, T alpha, const T *A, int lda, T beta, T *C,    \
+          int ldc)                                                                            \
+{                                                                                             \
+    int upper = ora_lsame(uplo, 'U');                                                         \
+    if (ora_syrk_info(uplo, trans, n, k, lda, ldc)) return;                                   \
+    if (n == 0 || ((alpha == 0 || k == 0) && beta == 1)) return;                              \
+    if (alpha == 0) {                                                                         \
+        for (int j = 0; j < n; ++j) {                                                         \
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;                                   \
+            for (int i = i0; i < i1; ++i)                                                     \
+                C[IX(i, j, ldc)] = beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)];                \
+        }                                                                                     \
+        return;                                                                               \
+    }                                                                                         \
+    if (ora_lsame(trans, 'N')) {                                                              \
+        for (int j = 0; j < n; ++j) {                                                         \
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;                                   \
+            if (beta == 0)                                                                    \
+                for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] = 0;                           \
+            else if (beta != 1)                                                               \
+                for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] *= beta;                       \
+            for (int l = 0; l < k; ++l) {                                                     \
+                T temp = alpha * A[IX(j, l, lda)];                                            \
+                for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] += temp * A[IX(i, l, lda)];    \
+            }                                                                                 \
+        }                                                                                     \
+    } else {                                                                                  \
+        for (int j = 0; j < n; ++j) {                                                         \
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;                                   \
+            for (int i = i0; i < i1; ++i) {                                                   \
+                T temp = 0;                                                                   \
+                for (int l = 0; l < k; ++l) temp += A[IX(l, i, lda)] * A[IX(l, j, lda)];      \
+                C[IX(i, j, ldc)] = beta == 0 ? alpha * temp                                   \
+                                             : alpha * temp + beta * C[IX(i, j, ldc)];        \
+            }                                                                                 \
+        }                                                                                     \
+    }                                                                                         \
+}
+DEFINE_SYRK(ora_dsyrk, double)
+DEFINE_SYRK(ora_ssyrk, float)
+
+#define DEFINE_TRSM(NAME, T)                                                                  \
+void NAME(char side, char uplo, char ta, char diag, int m, int n, T alpha, const T *A,        \
+          int lda, T *B, int ldb)                                                             \
+{                                                                                             \
+    int lside = ora_lsame(side, 'L'), upper = ora_lsame(uplo, 'U');                           \
+    int nounit = ora_lsame(diag, 'N'), notr = ora_lsame(ta, 'N');                             \
+    if (ora_trsm_info(side, uplo, ta, diag, m, n, lda, ldb)) return;                          \
+    if (m == 0 || n == 0) return;                                                             \
+    if (alpha == 0) {                                                                         \
+        for (int j = 0; j < n; ++j)                                                           \
+            for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] = 0;                                 \
+        return;                                                                               \
+    }                                                                                         \
+    if (lside) {                                                                              \
+        if (notr) {                                                                           \
+            /* B := alpha*inv(A)*B */                                                         \
+            for (int j = 0; j < n; ++j) {                                                     \
+                if (alpha != 1)                                                               \
+                    for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= alpha;                    \
+                if (upper) {                                                                  \
+                    for (int kk = m - 1; kk >= 0; --kk)                                       \
+                        if (B[IX(kk, j, ldb)] != 0) {                                         \
+                            if (nounit) B[IX(kk, j, ldb)] /= A[IX(kk, kk, lda)];              \
+                            for (int i = 0; i < kk; ++i)                                      \
+                                B[IX(i, j, ldb)] -= B[IX(kk, j, ldb)] * A[IX(i, kk, lda)];    \
+                        }                                                                     \
+                } else {                                                                      \
+                    for (int kk = 0; kk < m; ++kk)                                            \
+                        if (B[IX(kk, j, ldb)] != 0) {                                         \
+                            if (nounit) B[IX(kk, j, ldb)] /= A[IX(kk, kk, lda)];              \
+                            for (int i = kk + 1; i < m; ++i)                                  \
+                                B[IX(i, j, ldb)] -= B[IX(kk, j, ldb)] * A[IX(i, kk, lda)];    \
+                        }                                                                     \
+                }                                                                             \
+            }                                                                                 \
+        } else {                                                                              \
+            /* B := alpha*inv(A**T)*B */                                                      \
+            for (int j = 0; j < n; ++j) {                                                     \
+                if (upper) {                                                                  \
+                    for (int i = 0; i < m; ++i) {                                             \
+                        T temp = alpha * B[IX(i, j, ldb)];                                    \
+                        for (int kk = 0; kk < i; ++kk)                                        \
+                            temp -= A[IX(kk, i, lda)] * B[IX(kk, j, ldb)];                    \
+                        if (nounit) temp /= A[IX(i, i, lda)];                                 \
+                        B[IX(i, j, ldb)] = temp;                                              \
+                    }                                                                         \
+                } else {                                                                      \
+                    for (int i = m - 1; i >= 0; --i) {                                        \
+                        T temp = alpha * B[IX(i, j, ldb)];                                    \
+                        for (int kk = i + 1; kk < m; ++kk)                                    \
+                            temp -= A[IX(kk, i, lda)] * B[IX(kk, j, ldb)];                    \
+                        if (nounit) temp /= A[IX(i, i, lda)];                                 \
+                        B[IX(i, j, ldb)] = temp;                                              \
+                    }                                                                         \
+                }                                                                             \
+            }                                                                                 \
+        }                                                                                     \
+    } else {                                                                                  \
+        if (notr) {                                                                           \
+            /* B := alpha*B*inv(A) */                                                         \
+            if (upper) {                                                                      \
+                for (int j = 0; j < n; ++j) {                                                 \
+                    if (alpha != 1)                                                           \
+                        for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= alpha;                \
+                    for (int kk = 0; kk < j; ++kk)                                            \
+                        if (A[IX(kk, j, lda)] != 0)                                           \
+                            for (int i = 0; i < m; ++i)                                       \
+                                B[IX(i, j, ldb)] -= A[IX(kk, j, lda)] * B[IX(i, kk, ldb)];    \
+                    if (nounit) {                                                             \
+                        T temp = (T)1 / A[IX(j, j, lda)];                                     \
+                        for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= temp;                 \
+                    }                                                                         \
+                }                                                                             \
+            } else {                                                                          \
+                for (int j = n - 1; j >= 0; --j) {                                            \
+                    if (alpha != 1)                                                           \
+                        for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= alpha;                \
+                    for (int kk = j + 1; kk < n; ++kk)                                        \
+                        if (A[IX(kk, j, lda)] != 0)                                           \
+                            for (int i = 0; i < m; ++i)                                       \
+                                B[IX(i, j, ldb)] -= A[IX(kk, j, lda)] * B[IX(i, kk, ldb)];    \
+                    if (nounit) {                                                             \
+                        T temp = (T)1 / A[IX(j, j, lda)];                                     \
+                        for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= temp;                 \
+                    }                                                                         \
+                }                                                                             \
+            }                                                                                 \
+        } else {                                                                              \
+            /* B := alpha*B*inv(A**T) */                                                      \
+            if (upper) {                                                                      \
+                for (int kk = n - 1; kk >= 0; --kk) {                                         \
+                    if (nounit) {                                                             \
+                        T temp = (T)1 / A[IX(kk, kk, lda)];                                   \
+                        for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= temp;                \
+                    }                                                                         \
+                    for (int j = 0; j < kk; ++j)                                              \
+                        if (A[IX(j, kk, lda)] != 0) {                                         \
+                            T temp = A[IX(j, kk, lda)];                                       \
+                            for (int i = 0; i < m; ++i)                                       \
+                                B[IX(i, j, ldb)] -= temp * B[IX(i, kk, ldb)];                 \
+                        }                                                                     \
+                    if (alpha != 1)                                                           \
+                        for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= alpha;               \
+                }                                                                             \
+            } else {                                                                          \
+                for (int kk = 0; kk < n; ++kk) {                                              \
+                    if (nounit) {                                                             \
+                        T temp = (T)1 / A[IX(kk, kk, lda)];                                   \
+                        for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= temp;                \
+                    }                                                                         \
+                    for (int j = kk + 1; j < n; ++j)                                          \
+                        if (A[IX(j, kk, lda)] != 0) {                                         \
+                            T temp = A[IX(j, kk, lda)];                                       \
+                            for (int i = 0; i < m; ++i)                                       \
+                                B[IX(i, j, ldb)] -= temp * B[IX(i, kk, ldb)];                 \
+                        }                                                                     \
+                    if (alpha != 1)                                                           \
+                        for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= alpha;               \
+                }                                                                             \
+            }                                                                                 \
+        }                                                                                     \
+    }                                                                                         \
+}
+DEFINE_TRSM(ora_dtrsm, double)
+DEFINE_TRSM(ora_strsm, float)
+
+#define DEFINE_SYMM(NAME, T)                                                                  \
+void NAME(char side, char uplo, int m, int n, T alpha, const T *A, int lda, const T *B,       \
+          int ldb, T beta, T *C, int ldc)                                                     \
+{                                                                                             \
+    int upper = ora_lsame(uplo, 'U');                                                         \
+    if (ora_symm_info(side, uplo, m, n, lda, ldb, ldc)) return;                               \
+    if (m == 0 || n == 0 || (alpha == 0 && beta == 1)) return;                                \
+    if (alpha == 0) {                                                                         \
+        for (int j = 0; j < n; ++j)                                                           \
+            for (int i = 0; i < m; ++i)                                                       \
+                C[IX(i, j, ldc)] = beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)];                \
+        return;                                                                               \
+    }                                                                                         \
+    if (ora_lsame(side, 'L')) {                                                               \
+        /* C := alpha*A*B + beta*C, A symmetric m x m */                                      \
+        for (int j = 0; j < n; ++j) {                                                         \
+            if (upper) {                                                                      \
+                for (int i = 0; i < m; ++i) {                                                 \
+                    T temp1 = alpha * B[IX(i, j, ldb)], temp2 = 0;                            \
+                    for (int kk = 0; kk < i; ++kk) {                                          \
+                        C[IX(kk, j, ldc)] += temp1 * A[IX(kk, i, lda)];                       \
+                        temp2 += B[IX(kk, j, ldb)] * A[IX(kk, i, lda)];                       \
+                    }                                                                         \
+                    C[IX(i, j, ldc)] = (beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)]) +         \
+                                       temp1 * A[IX(i, i, lda)] + alpha * temp2;              \
+                }                                                                             \
+            } else {                                                                          \
+                for (int i = m - 1; i >= 0; --i) {                                            \
+                    T temp1 = alpha * B[IX(i, j, ldb)], temp2 = 0;                            \
+                    for (int kk = i + 1; kk < m; ++kk) {                                      \
+                        C[IX(kk, j, ldc)] += temp1 * A[IX(kk, i, lda)];                       \
+                        temp2 += B[IX(kk, j, ldb)] * A[IX(kk, i, lda)];                       \
+                    }                                                                         \
+                    C[IX(i, j, ldc)] = (beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)]) +         \
+                                       temp1 * A[IX(i, i, lda)] + alpha * temp2;              \
+                }                                                                             \
+            }                                                                                 \
+        }                                                                                     \
+    } else {                                                                                  \
+        /* C := alpha*B*A + beta*C, A symmetric n x n */                                      \
+        for (int j = 0; j < n; ++j) {                                                         \
+            T temp1 = alpha * A[IX(j, j, lda)];                                               \
+            for (int i = 0; i < m; ++i)                                                       \
+                C[IX(i, j, ldc)] = (beta == 0 ? (T)0 : beta * C[IX(i, j, ldc)]) +             \
+                                   temp1 * B[IX(i, j, ldb)];                                  \
+            for (int kk = 0; kk < n; ++kk) {                                                  \
+                if (kk == j) continue;                                                        \
+                T akj = (kk < j) == (upper != 0) ? A[IX(kk, j, lda)] : A[IX(j, kk, lda)];     \
+                temp1 = alpha * akj;                                                          \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] += temp1 * B[IX(i, kk, ldb)];    \
+            }                                                                                 \
+        }                                                                                     \
+    }                                                                                         \
+}
+DEFINE_SYMM(ora_dsymm, double)
+DEFINE_SYMM(ora_ssymm, float)
+
+/* ------------------------------------------------------------------ */
+/* Level 2                                                             */
+/* ------------------------------------------------------------------ */
+void ora_dgemv(char trans, int m, int n, double alpha, const double *A, int lda,
+               const double *x, int incx, double beta, double *y, int incy)
+{
+    if (ora_gemv_info(trans, m, n, lda, incx, incy)) return;
+    if (m == 0 || n == 0 || (alpha == 0 && beta == 1)) return;
+    int notr = ora_lsame(trans, 'N');
+    int lenx = notr ? n : m, leny = notr ? m : n;
+    int kx = incx > 0 ? 0 : -(lenx - 1) * incx;
+    int ky = incy > 0 ? 0 : -(leny - 1) * incy;
+    if (beta != 1) {
+        int iy = ky;
+        for (int i = 0; i < leny; ++i, iy += incy) y[iy] = beta == 0 ? 0.0 : beta * y[iy];
+    }
+    if (alpha == 0) return;
+    if (notr) {
+        int jx = kx;
+        for (int j = 0; j < n; ++j, jx += incx) {
+            double temp = alpha * x[jx];
+            int iy = ky;
+            for (int i = 0; i < m; ++i, iy += incy) y[iy] += temp * A[IX(i, j, lda)];
+        }
+    } else {
+        int jy = ky;
+        for (int j = 0; j < n; ++j, jy += incy) {
+            double temp = 0;
+            int ix = kx;
+            for (int i = 0; i < m; ++i, ix += incx) temp += A[IX(i, j, lda)] * x[ix];
+            y[jy] += alpha * temp;
+        }
+    }
+}
+
+void ora_dger(int m, int n, double alpha, const double *x, int incx, const double *y, int incy,
+              double *A, int lda)
+{
+    if (ora_ger_info(m, n, incx, incy, lda)) return;
+    if (m == 0 || n == 0 || alpha == 0) return;
+    int jy = incy > 0 ? 0 : -(n - 1) * incy;
+    int kx = incx > 0 ? 0 : -(m - 1) * incx;
+    for (int j = 0; j < n; ++j, jy += incy) {
+        if (y[jy] != 0) {
+            double temp = alpha * y[jy];
+            int ix = kx;
+            for (int i = 0; i < m; ++i, ix += incx) A[IX(i, j, lda)] += x[ix] * temp;
+        }
+    }
+}
+
+void ora_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x,
+               int incx)
+{
+    if (ora_trsv_info(uplo, trans, diag, n, lda, incx)) return;
+    if (n == 0) return;
+    int nounit = ora_lsame(diag, 'N'), upper = ora_lsame(uplo, 'U');
+    int kx = incx > 0 ? 0 : -(n - 1) * incx;
+#define XE(j) x[kx + (j) * incx]
+    if (ora_lsame(trans, 'N')) {
+        if (upper) {
+            for (int j = n - 1; j >= 0; --j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= A[IX(j, j, lda)];
+                    double temp = XE(j);
+                    for (int i = j - 1; i >= 0; --i) XE(i) -= temp * A[IX(i, j, lda)];
+                }
+        } else {
+            for (int j = 0; j < n; ++j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= A[IX(j, j, lda)];
+                    double temp = XE(j);
+                    for (int i = j + 1; i < n; ++i) XE(i) -= temp * A[IX(i, j, lda)];
+                }
+        }
+    } else {
+        if (upper) {
+            for (int j = 0; j < n; ++j) {
+                double temp = XE(j);
+                for (int i = 0; i < j; ++i) temp -= A[IX(i, j, lda)] * XE(i);
+                if (nounit) temp /= A[IX(j, j, lda)];
+                XE(j) = temp;
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                double temp = XE(j);
+                for (int i = n - 1; i > j; --i) temp -= A[IX(i, j, lda)] * XE(i);
+                if (nounit) temp /= A[IX(j, j, lda)];
+                XE(j) = temp;
+            }
+        }
+    }
+#undef XE
+}
+
+/* ------------------------------------------------------------------ */
+/* Level 1                                                             */
+/* ------------------------------------------------------------------ */
+void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy)
+{
+    if (n <= 0 || a == 0) return;
+    int ix = incx < 0 ? -(n - 1) * incx : 0;
+    int iy = incy < 0 ? -(n - 1) * incy : 0;
+    for (int i = 0; i < n; ++i, ix += incx, iy += incy) y[iy] += a * x[ix];
+}
+
+double ora_ddot(int n, const double *x, int incx, const double *y, int incy)
+{
+    double s = 0;
+    if (n <= 0) return 0;
+    int ix = incx < 0 ? -(n - 1) * incx : 0;
+    int iy = incy < 0 ? -(n - 1) * incy : 0;
+    for (int i = 0; i < n; ++i, ix += incx, iy += incy) s += x[ix] * y[iy];
+    return s;
+}
+
+double ora_dnrm2(int n, const double *x, int incx)
+{
+    /* Scaled sum of squares (the classic netlib DNRM2 recurrence): never
+     * overflows or flushes for finite input.  Stride conventions follow the
+     * current netlib dnrm2.f90: n <= 0 -> 0, a negative stride walks the
+     * vector from its far end (same set of elements), incx == 0 repeats x(1). */
+    if (n <= 0) return 0.0;
+    if (n == 1) return fabs(x[0]);
+    double scale = 0.0, ssq = 1.0;
+    long ix = incx < 0 ? -(long)(n - 1) * incx : 0;
+    for (int i = 0; i < n; ++i, ix += incx) {
+        double v = x[ix];
+        if (v != 0) {
+            double a = fabs(v);
+            if (scale < a) {
+                double r = scale / a;
+                ssq = 1.0 + ssq * r * r;
+                scale = a;
+            } else {
+                double r = a / scale;
+                ssq += r * r;
+            }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+int ora_idamax(int n, const double *x, int incx)
+{
+    /* netlib IDAMAX: 1-based index of the first maximal |x_i|, strict '>'
+     * so ties keep the smaller index and a NaN after position 1 is never
+     * selected.  n < 1 or incx <= 0 -> 0. */
+    if (n < 1 || incx <= 0) return 0;
+    if (n == 1) return 1;
+    int best = 1;
+    double dmax = fabs(x[0]);
+    for (int i = 1; i < n; ++i) {
+        double a = fabs(x[(size_t)i * incx]);
+        if (a > dmax) {
+            best = i + 1;
+            dmax = a;
+        }
+    }
+    return best;
+}
+
+/* ------------------------------------------------------------------ */
+/* Checkers                                                            */
+/* ------------------------------------------------------------------ */
+double ora_gemm_gauge_ratio(char ta, char tb, int m, int n, int k, double alpha,
+                            const double *A, int lda, const double *B, int ldb, double beta,
+                            const double *C0, const double *Cgot, int ldc, double eps)
+{
+    /* tests/netlib/dblat3.f:2496-2546 (DMMCH): expected value and gauge per
+     * entry, error ratio = |ct - cc| / (eps * g), 0 where g == 0. */
+    int nota = ora_lsame(ta, 'N'), notb = ora_lsame(tb, 'N');
+    double worst = 0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            long double ct = 0, g = 0;
+            for (int l = 0; l < k; ++l) {
+                double a = nota ? A[IX(i, l, lda)] : A[IX(l, i, lda)];
+                double b = notb ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)];
+                ct += (long double)a * b;
+                g += fabsl((long double)a * b);
+            }
+            ct = (long double)alpha * ct + (long double)beta * C0[IX(i, j, ldc)];
+            g = fabsl((long double)alpha) * g + fabsl((long double)beta) * fabs(C0[IX(i, j, ldc)]);
+            long double err = fabsl(ct - (long double)Cgot[IX(i, j, ldc)]);
+            if (g != 0) {
+                double r = (double)(err / ((long double)eps * g));
+                if (r > worst) worst = r;
+            } else if (err != 0) {
+                return INFINITY;
+            }
+        }
+    return worst;
+}
+
+uint64_t ora_splitmix64_at(uint64_t seed, uint64_t i)
+{
+    /* SplitMix64 (Steele, Lea & Flood, OOPSLA 2014): state advances by the
+     * golden gamma; element i is the mix of seed + (i+1)*gamma. */
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void ora_fill_uniform(uint64_t seed, int64_t n, double *out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t z = ora_splitmix64_at(seed, (uint64_t)i);
+        double u = (double)(z >> 11) * 0x1.0p-53; /* [0,1), exact */
+        out[i] = 2.0 * u - 1.0;                   /* [-1,1), exact */
+    }
+}

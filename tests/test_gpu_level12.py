@@ -1,0 +1,205 @@
+"""GPU parity of the Level-1/2 routines against the CPU oracle.
+
+Bars:
+  * idamax: bit-exact (1-based index, first-index tie rule, netlib NaN rule);
+  * integer-valued data: bit-exact for daxpy / ddot / dgemv / dger;
+  * random data: |got - ref| <= 2 * n * eps * sum|terms| (the standard
+    backward-error bound of a length-n inner product, with margin for the two
+    different summation orders) for ddot / dgemv / dtrsv, one rounding for daxpy
+    (the GPU fuses a*x+y), relative 4 * eps * sqrt(n) for dnrm2.
+"""
+import numpy as np
+import pytest
+
+import _oracle
+from _oracle import c, p, uniform
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _strided(vec, inc, fill=-7.0):
+    """Lay `vec` out with stride |inc| (BLAS order: inc < 0 starts at the end)."""
+    n = len(vec)
+    a = abs(inc)
+    buf = np.full(1 + (n - 1) * a, fill)
+    if inc > 0:
+        buf[::a] = vec
+    else:
+        buf[::a] = vec[::-1]
+    return buf
+
+
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (1000, 1, 1), (1001, 1, 1), (777, 2, 3),
+                                         (500, -1, 2), (300, -3, -2), (8192, 1, 1)])
+def test_daxpy(gpu, ora, n, incx, incy):
+    x = _strided(uniform(1, (n,)), incx)
+    y0 = _strided(uniform(2, (n,)), incy)
+    y = y0.copy()
+    r = y0.copy()
+    gpu.daxpy(n, 0.3, x, incx, y, incy)
+    ora.ora_daxpy(n, 0.3, p(x), incx, p(r), incy)
+    # one rounding (fma) vs two (mul, add): within a couple of ulps of the terms
+    assert np.all(np.abs(y - r) <= 2 * EPS * (np.abs(y0) + 0.3 * np.max(np.abs(x)) + np.abs(r)))
+
+
+def test_daxpy_integer_exact(gpu, ora):
+    rng = np.random.default_rng(1)
+    x = rng.integers(-100, 100, 5000).astype(float)
+    y = rng.integers(-100, 100, 5000).astype(float)
+    r = y.copy()
+    gpu.daxpy(5000, 3.0, x, 1, y, 1)
+    ora.ora_daxpy(5000, 3.0, p(x), 1, p(r), 1)
+    assert np.array_equal(y, r)
+
+
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (5, 1, 1), (2048, 1, 1), (8192, 1, 1),
+                                         (100000, 1, 1), (999, 3, -2), (4097, -1, -1)])
+def test_ddot(gpu, ora, n, incx, incy):
+    xv, yv = uniform(3, (n,)), uniform(4, (n,))
+    x, y = _strided(xv, incx), _strided(yv, incy)
+    got = gpu.ddot(n, x, incx, y, incy)
+    ref = ora.ora_ddot(n, p(x), incx, p(y), incy)
+    assert abs(got - ref) <= 2 * n * EPS * np.sum(np.abs(xv * yv))
+
+
+def test_ddot_integer_exact_and_repeatable(gpu):
+    rng = np.random.default_rng(2)
+    x = rng.integers(-1000, 1000, 300000).astype(float)
+    y = rng.integers(-1000, 1000, 300000).astype(float)
+    got = gpu.ddot(300000, x, 1, y, 1)
+    assert got == float(np.dot(x.astype(np.int64), y.astype(np.int64)))
+    z, w = uniform(5, (123457,)), uniform(6, (123457,))
+    vals = {gpu.ddot(123457, z, 1, w, 1) for _ in range(5)}
+    assert len(vals) == 1  # fixed reduction order: bitwise repeatable
+
+
+@pytest.mark.parametrize("n,incx", [(2, 1), (1000, 1), (8192, 1), (70001, 1), (500, 2), (400, -3)])
+def test_dnrm2(gpu, ora, n, incx):
+    xv = uniform(7, (n,))
+    x = _strided(xv, incx)
+    got = gpu.dnrm2(n, x, incx)
+    ref = ora.ora_dnrm2(n, p(x), incx)
+    assert abs(got - ref) <= 4 * EPS * np.sqrt(n) * ref
+
+
+@pytest.mark.parametrize("scale", [1e300, 1e-300, 1e-160, 1e160, 1.0])
+def test_dnrm2_no_overflow_underflow(gpu, scale):
+    """Closed form: ||c * 1_n|| = |c| sqrt(n), at magnitudes where a plain
+    sum of squares would overflow or flush to zero."""
+    n = 10000
+    x = np.full(n, scale)
+    got = gpu.dnrm2(n, x, 1)
+    assert got == pytest.approx(scale * np.sqrt(n), rel=1e-14)
+
+
+def test_dnrm2_special(gpu):
+    x = np.array([3.0, -4.0] + [0.0] * 1000)
+    assert gpu.dnrm2(1002, x, 1) == 5.0
+    x = uniform(8, (3000,))
+    x[1234] = np.inf
+    assert gpu.dnrm2(3000, x, 1) == np.inf
+    x[1234] = np.nan
+    assert np.isnan(gpu.dnrm2(3000, x, 1))
+    assert gpu.dnrm2(0, x, 1) == 0.0
+    # incx == 0 repeats x(1): |x1| sqrt(n)
+    y = np.array([2.0])
+    assert gpu.dnrm2(1000, y, 0) == pytest.approx(2.0 * np.sqrt(1000), rel=1e-15)
+
+
+@pytest.mark.parametrize("n,incx", [(1, 1), (2, 1), (1000, 1), (8192, 1), (100003, 1), (777, 3)])
+def test_idamax_random(gpu, ora, n, incx):
+    x = _strided(uniform(9, (n,)), incx)
+    assert gpu.idamax(n, x, incx) == ora.ora_idamax(n, p(x), incx)
+
+
+def test_idamax_ties_first_index(gpu, ora):
+    n = 50000
+    x = uniform(10, (n,)) * 0.5
+    for pos in (40000, 7, 12345, 49999):
+        x[pos] = 0.9 if pos % 2 else -0.9  # equal |x|, both signs
+    want = 8  # 1-based index of position 7
+    assert ora.ora_idamax(n, p(x), 1) == want
+    assert gpu.idamax(n, x, 1) == want
+    y = np.full(9000, 2.5)
+    y[::2] *= -1
+    assert gpu.idamax(9000, y, 1) == 1
+
+
+def test_idamax_nan_rule(gpu, ora):
+    """netlib: strict '>' never moves to a NaN; a NaN in x(1) gives 1."""
+    n = 5000
+    x = uniform(11, (n,))
+    x[100] = np.nan
+    x[4000] = 3.0
+    assert gpu.idamax(n, x, 1) == ora.ora_idamax(n, p(x), 1) == 4001
+    x[0] = np.nan
+    assert gpu.idamax(n, x, 1) == ora.ora_idamax(n, p(x), 1) == 1
+    assert gpu.idamax(0, x, 1) == 0
+    assert gpu.idamax(10, x, 0) == 0
+    assert gpu.idamax(10, x, -1) == 0
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m,n,incx,incy", [(1, 1, 1, 1), (100, 70, 1, 1), (513, 1025, 1, 1),
+                                           (300, 200, 2, -1), (129, 4000, -2, 3), (5000, 37, 1, 1)])
+def test_dgemv(gpu, ora, trans, m, n, incx, incy):
+    lda = m + 1
+    A = np.full((lda, n), -9.0, order="F")
+    A[:m, :] = uniform(12, (m, n))
+    lenx, leny = (n, m) if trans == "N" else (m, n)
+    xv, yv = uniform(13, (lenx,)), uniform(14, (leny,))
+    x, y0 = _strided(xv, incx), _strided(yv, incy)
+    y, r = y0.copy(), y0.copy()
+    gpu.dgemv(trans, m, n, 0.8, A, lda, x, incx, 0.6, y, incy)
+    ora.ora_dgemv(c(trans), m, n, 0.8, p(A), lda, p(x), incx, 0.6, p(r), incy)
+    absA = np.abs(A[:m, :]) if trans == "N" else np.abs(A[:m, :]).T
+    G = 0.8 * absA @ np.abs(xv) + 0.6 * np.abs(yv)
+    k = n if trans == "N" else m
+    got = y[:: abs(incy)] if incy > 0 else y[:: abs(incy)][::-1]
+    ref = r[:: abs(incy)] if incy > 0 else r[:: abs(incy)][::-1]
+    assert np.all(np.abs(got - ref) <= 2 * (k + 2) * EPS * G)
+    assert np.array_equal(y[np.arange(len(y)) % abs(incy) != 0], y0[np.arange(len(y)) % abs(incy) != 0])
+
+
+def test_dgemv_integer_exact(gpu, ora):
+    rng = np.random.default_rng(4)
+    m, n = 700, 900
+    A = np.asfortranarray(rng.integers(-5, 6, size=(m, n)).astype(float))
+    x = rng.integers(-5, 6, n).astype(float)
+    y = np.zeros(m)
+    r = np.zeros(m)
+    gpu.dgemv("N", m, n, 1.0, A, m, x, 1, 0.0, y, 1)
+    ora.ora_dgemv(b"N", m, n, 1.0, p(A), m, p(x), 1, 0.0, p(r), 1)
+    assert np.array_equal(y, r)
+
+
+@pytest.mark.parametrize("m,n,incx,incy", [(1, 1, 1, 1), (300, 200, 1, 1), (257, 33, -2, 3)])
+def test_dger(gpu, ora, m, n, incx, incy):
+    lda = m + 2
+    A0 = np.full((lda, n), 5.0, order="F")
+    A0[:m, :] = uniform(15, (m, n))
+    x, y = _strided(uniform(16, (m,)), incx), _strided(uniform(17, (n,)), incy)
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dger(m, n, 1.7, x, incx, y, incy, A, lda)
+    ora.ora_dger(m, n, 1.7, p(x), incx, p(y), incy, p(R), lda)
+    np.testing.assert_allclose(A, R, rtol=4 * EPS, atol=4 * EPS)
+    assert np.array_equal(A[m:, :], A0[m:, :])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("n,incx", [(1, 1), (50, 1), (300, 1), (200, -2)])
+def test_dtrsv(gpu, ora, uplo, trans, diag, n, incx):
+    A = uniform(18, (n, n))
+    if diag == "N":
+        A[np.diag_indices(n)] = n + 1.0
+    else:
+        A *= 1.0 / n
+    A = np.asfortranarray(A)
+    x0 = _strided(uniform(19, (n,)), incx)
+    x, r = x0.copy(), x0.copy()
+    gpu.dtrsv(uplo, trans, diag, n, A, n, x, incx)
+    ora.ora_dtrsv(c(uplo), c(trans), c(diag), n, p(A), n, p(r), incx)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
