@@ -65,7 +65,7 @@ int main(int argc, char **argv)
 {
     int n = 8192, iters = 200, warmup = 5;
     uint64_t seed = 20240601;
-    const char *blas = NULL, *prefix = "";
+    const char *blas = NULL, *prefix = "", *direct = NULL;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--n") && i + 1 < argc) n = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--iters") && i + 1 < argc) iters = atoi(argv[++i]);
@@ -73,18 +73,35 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--seed") && i + 1 < argc) seed = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--blas") && i + 1 < argc) blas = argv[++i];
         else if (!strcmp(argv[i], "--prefix") && i + 1 < argc) prefix = argv[++i];
+        else if (!strcmp(argv[i], "--lib") && i + 1 < argc) direct = argv[++i];
+    }
+    void *(*alloc)(size_t) = malloc;
+    void (*release)(void *) = free;
+    void *dh = NULL;
+    if (direct) {
+        dh = dlopen(direct, RTLD_NOW | RTLD_GLOBAL);
+        if (!dh) {
+            fprintf(stderr, "chain_bench: %s\n", dlerror());
+            return 2;
+        }
+        alloc = (void *(*)(size_t))dlsym(dh, "gpublas_malloc");
+        release = (void (*)(void *))dlsym(dh, "gpublas_free");
+        if (!alloc || !release) {
+            fprintf(stderr, "chain_bench: %s lacks gpublas_malloc\n", direct);
+            return 2;
+        }
     }
     /* The operands are this program's first four allocations. */
     const size_t nn = (size_t)n * n;
-    double *A = malloc(nn * sizeof(double));
-    double *x = malloc((size_t)n * sizeof(double));
-    double *y = malloc((size_t)n * sizeof(double));
-    double *z = malloc((size_t)n * sizeof(double));
+    double *A = alloc(nn * sizeof(double));
+    double *x = alloc((size_t)n * sizeof(double));
+    double *y = alloc((size_t)n * sizeof(double));
+    double *z = alloc((size_t)n * sizeof(double));
     if (!A || !x || !y || !z) {
         fprintf(stderr, "chain_bench: out of memory\n");
         return 2;
     }
-    void *h = RTLD_DEFAULT;
+    void *h = dh ? dh : RTLD_DEFAULT;
     if (blas) {
         h = dlopen(blas, RTLD_NOW | RTLD_LOCAL);
         if (!h) {
@@ -97,9 +114,10 @@ int main(int argc, char **argv)
     dot_t dot = (dot_t)sym(h, prefix, "ddot_");
     nrm2_t nrm2 = (nrm2_t)sym(h, prefix, "dnrm2_");
     iamax_t iamax = (iamax_t)sym(h, prefix, "idamax_");
-    int (*is_tracked)(const void *) = (int (*)(const void *))dlsym(RTLD_DEFAULT, "gpublas_is_tracked");
-    int (*ktime)(const char *, double *) = (int (*)(const char *, double *))dlsym(RTLD_DEFAULT, "gpublas_kernel_time");
-    void (*prof_reset)(void) = (void (*)(void))dlsym(RTLD_DEFAULT, "gpublas_dev_prof_reset");
+    void *gh = dh ? dh : RTLD_DEFAULT;
+    int (*is_tracked)(const void *) = (int (*)(const void *))dlsym(gh, "gpublas_is_tracked");
+    int (*ktime)(const char *, double *) = (int (*)(const char *, double *))dlsym(gh, "gpublas_kernel_time");
+    void (*prof_reset)(void) = (void (*)(void))dlsym(gh, "gpublas_dev_prof_reset");
 
     fill(A, nn, seed);
     fill(x, n, seed + 1);
@@ -143,9 +161,9 @@ int main(int argc, char **argv)
     double ysum = 0;
     for (int i = 0; i < n; ++i) ysum += y[i] + z[i];
     fprintf(stderr, "chain_bench: checksum %.17g\n", ysum);
-    free(A);
-    free(x);
-    free(y);
-    free(z);
+    release(A);
+    release(x);
+    release(y);
+    release(z);
     return 0;
 }

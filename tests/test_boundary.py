@@ -1,0 +1,274 @@
+"""The drop-in boundary on the CPU (no GPU needed): exported symbols, netlib
+argument checking and xerbla, quick returns, the host fall-through cutoff,
+CBLAS layout handling, options, tracked objects and the malloc interposer.
+
+Nothing here reaches a kernel: every call either fails its argument check,
+returns early, or is below the fall-through cutoff and runs on the host BLAS
+(scipy's OpenBLAS in this image), exactly as the reference's gemm_perf_check
+(blas_level3/gemm.cc:129-141) routes small problems."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+import scipy.linalg.blas as sb
+
+import libgpublas_amd as gb
+from _oracle import p, uniform
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", src)
+    skip = {"if", "for", "while", "return", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def _exports(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+@pytest.mark.parametrize("header", ["gpublas.h", "gpublas_dev.h"])
+def test_library_exports_every_declared_symbol(header):
+    gb.library()
+    syms = _exports(gb.LIB_PATH)
+    missing = [n for n in _declared(header) if n not in syms]
+    assert not missing, missing
+
+
+def test_preload_library_adds_the_malloc_family():
+    syms = _exports(gb.PRELOAD_PATH)
+    for s in ("malloc", "calloc", "realloc", "free", "malloc_usable_size", "dgemm_", "cblas_dgemm"):
+        assert s in syms
+    assert "malloc" not in _exports(gb.LIB_PATH)
+
+
+def _ba(*vals):
+    return [ctypes.byref(v) for v in vals]
+
+
+GEMM_CASES = [  # (transa, transb, m, n, k, lda, ldb, ldc) -> INFO
+    (("X", "N", 2, 2, 2, 2, 2, 2), 1), (("N", "Y", 2, 2, 2, 2, 2, 2), 2),
+    (("N", "N", -1, 2, 2, 2, 2, 2), 3), (("N", "N", 2, -1, 2, 2, 2, 2), 4),
+    (("N", "N", 2, 2, -1, 2, 2, 2), 5), (("N", "N", 3, 2, 2, 2, 2, 3), 8),
+    (("N", "N", 2, 2, 3, 2, 2, 2), 10), (("N", "N", 3, 2, 2, 3, 2, 2), 13)]
+
+
+@pytest.mark.parametrize("args,info", GEMM_CASES)
+def test_dgemm_xerbla(args, info):
+    ta, tb, m, n, k, lda, ldb, ldc = args
+    dummy = np.zeros(64)
+    gb.dgemm(ta, tb, m, n, k, 1.0, dummy, lda, dummy, ldb, 0.0, dummy, ldc)
+    assert gb.last_xerbla() == ("DGEMM ", info)
+
+
+def test_level3_xerbla_positions(ora):
+    d = np.zeros(64)
+    gb.dsyrk("L", "N", 3, 2, 1.0, d, 2, 0.0, d, 3)
+    assert gb.last_xerbla() == ("DSYRK ", ora.ora_syrk_info(b"L", b"N", 3, 2, 2, 3))
+    gb.dtrsm("L", "L", "N", "X", 2, 2, 1.0, d, 2, d, 2)
+    assert gb.last_xerbla() == ("DTRSM ", 4)
+    gb.dtrsm("R", "L", "N", "N", 2, 3, 1.0, d, 2, d, 2)
+    assert gb.last_xerbla() == ("DTRSM ", 9)
+    gb.dsymm("L", "L", 3, 2, 1.0, d, 3, d, 3, 0.0, d, 2)
+    assert gb.last_xerbla() == ("DSYMM ", 12)
+    gb.ssyrk("Q", "N", 3, 2, 1.0, d, 3, 0.0, d, 3)
+    assert gb.last_xerbla() == ("SSYRK ", 1)
+
+
+def test_level12_xerbla_positions():
+    d = np.zeros(64)
+    gb.dgemv("N", 3, 2, 1.0, d, 2, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DGEMV ", 6)
+    gb.dgemv("N", 2, 2, 1.0, d, 2, d, 0, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DGEMV ", 8)
+    gb.dger(2, 2, 1.0, d, 1, d, 0, d, 2)
+    assert gb.last_xerbla() == ("DGER  ", 7)
+    gb.dtrsv("L", "N", "N", 3, d, 2, d, 1)
+    assert gb.last_xerbla() == ("DTRSV ", 6)
+
+
+def test_cblas_error_positions():
+    lib = gb.library()
+    d = np.zeros(64)
+    lib.cblas_dgemm(999, 111, 111, 2, 2, 2, 1.0, p(d), 2, p(d), 2, 0.0, p(d), 2)
+    assert gb.last_xerbla() == ("DGEMM ", 1)
+    lib.cblas_dgemm(102, 111, 7, 2, 2, 2, 1.0, p(d), 2, p(d), 2, 0.0, p(d), 2)
+    assert gb.last_xerbla() == ("DGEMM ", 3)
+    # column-major m < 0 is CBLAS parameter 4; row-major it is the Fortran n -> CBLAS 4 too
+    lib.cblas_dgemm(102, 111, 111, -1, 2, 2, 1.0, p(d), 2, p(d), 2, 0.0, p(d), 2)
+    assert gb.last_xerbla() == ("DGEMM ", 4)
+    lib.cblas_dgemm(101, 111, 111, -1, 2, 2, 1.0, p(d), 2, p(d), 2, 0.0, p(d), 2)
+    assert gb.last_xerbla() == ("DGEMM ", 4)
+    lib.cblas_dgemm(101, 111, 111, 2, 3, 2, 1.0, p(d), 2, p(d), 3, 0.0, p(d), 2)
+    assert gb.last_xerbla() == ("DGEMM ", 14)  # row-major: ldc < n
+
+
+def test_quick_returns_touch_nothing():
+    gb.reset_stats()
+    # null operands: quick returns must not dereference them
+    gb.dgemm("N", "N", 0, 5, 5, 1.0, None, 1, None, 5, 0.0, None, 1)
+    gb.dgemm("N", "N", 5, 5, 0, 1.0, None, 5, None, 1, 1.0, None, 5)
+    gb.dsyrk("L", "N", 0, 5, 1.0, None, 1, 0.0, None, 1)
+    gb.dtrsm("L", "L", "N", "N", 0, 5, 1.0, None, 1, None, 1)
+    gb.dsymm("L", "L", 4, 0, 1.0, None, 4, None, 4, 0.0, None, 4)
+    gb.dgemv("N", 0, 4, 1.0, None, 1, None, 1, 0.0, None, 1)
+    gb.daxpy(0, 1.0, None, 1, None, 1)
+    assert gb.ddot(0, None, 1, None, 1) == 0.0
+    assert gb.dnrm2(-1, None, 1) == 0.0
+    assert gb.idamax(0, None, 1) == 0
+    s = gb.stats()
+    assert s["device_calls"] == 0 and s["fallthrough"] == 0
+
+
+def test_alpha_zero_host_path_is_netlib():
+    C = np.full((5, 4), np.nan, order="F")
+    A, B = uniform(1, (5, 3)), uniform(2, (3, 4))
+    gb.dgemm("N", "N", 5, 4, 3, 0.0, A, 5, B, 3, 0.0, C, 5)
+    assert np.all(C == 0.0)  # the reference's C *= beta would keep NaN (gemm.cc:121-126)
+    C = uniform(3, (5, 4))
+    want = 2.0 * C
+    gb.dgemm("N", "N", 5, 4, 0, 1.0, A, 5, B, 1, 2.0, C, 5)
+    assert np.array_equal(C, want)
+    S0 = uniform(4, (4, 4))
+    S = S0.copy(order="F")
+    gb.dsyrk("U", "N", 4, 3, 0.0, A, 4, 3.0, S, 4)
+    tri = np.triu(np.ones((4, 4), bool))
+    assert np.array_equal(S[tri], 3.0 * S0[tri]) and np.array_equal(S[~tri], S0[~tri])
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_small_gemm_falls_through_to_host_blas(ta, tb):
+    gb.set_cutoff(512)
+    m, n, k = 9, 7, 5
+    A = uniform(5, (m, k) if ta == "N" else (k, m))
+    B = uniform(6, (k, n) if tb == "N" else (n, k))
+    C = uniform(7, (m, n))
+    H = sb.dgemm(0.5, A, B, 1.5, C.copy(order="F"), trans_a=ta == "T", trans_b=tb == "T")
+    gb.reset_stats()
+    gb.dgemm(ta, tb, m, n, k, 0.5, A, A.shape[0], B, B.shape[0], 1.5, C, m)
+    assert gb.stats()["fallthrough"] == 1
+    assert np.array_equal(C, H)  # the same host routine: bit-identical
+
+
+def test_small_level12_fall_through():
+    gb.set_cutoff(512)
+    x, y = uniform(8, (100,)), uniform(9, (100,))
+    gb.reset_stats()
+    assert gb.ddot(100, x, 1, y, 1) == sb.ddot(x, y)
+    assert gb.dnrm2(100, x, 1) == sb.dnrm2(x)
+    assert gb.idamax(100, x, 1) == sb.idamax(x) + 1
+    lib = gb.library()
+    assert lib.cblas_idamax(100, p(x), 1) == sb.idamax(x)  # CBLAS is 0-based
+    assert gb.stats()["fallthrough"] == 4
+
+
+def test_cblas_row_major_gemm_via_fall_through():
+    gb.set_cutoff(512)
+    lib = gb.library()
+    m, n, k = 6, 5, 4
+    A = np.ascontiguousarray(uniform(10, (m, k)))  # row-major
+    B = np.ascontiguousarray(uniform(11, (k, n)))
+    C = np.zeros((m, n))
+    lib.cblas_dgemm(101, 111, 111, m, n, k, 1.0, p(A), k, p(B), n, 0.0, p(C), n)
+    np.testing.assert_allclose(C, A @ B, rtol=1e-14, atol=1e-15)
+    x = uniform(12, (k,))
+    y = np.zeros(m)
+    lib.cblas_dgemv(101, 111, m, k, 1.0, p(A), k, p(x), 1, 0.0, p(y), 1)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-14, atol=1e-15)
+
+
+def test_options_parsing():
+    assert gb.set_options("cutoff=128;min_track=64k") == 0
+    assert gb.get_cutoff() == 128
+    assert gb.set_options("bogus_option") != 0
+    assert gb.set_options("heuristic=objtrack:/nonexistent/file") != 0
+    gb.set_options("cutoff=512;heuristic=false")
+
+
+def test_tracked_object_host_side():
+    a = gb.tracked_empty((1000,))
+    a[:] = np.arange(1000.0)
+    assert gb.is_tracked(a) and gb.is_tracked(a[500:])
+    assert float(a.sum()) == 499500.0
+    assert not gb.is_tracked(np.zeros(10))
+    s = gb.stats()
+    assert s["tracked_objects"] >= 1
+
+
+def _run_preloaded(objtrack_text, extra=""):
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "ot.txt")
+        open(f, "w").write(objtrack_text)
+        env = dict(os.environ)
+        env["LD_PRELOAD"] = gb.PRELOAD_PATH
+        env["GPUBLAS_OPTIONS"] = f"heuristic=objtrack:{f};min_track=0{extra}"
+        out = subprocess.run([gb.CHAIN_BENCH, "--n", "64", "--iters", "0", "--warmup", "0"],
+                             env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out
+
+
+def test_objtrack_file_selects_allocations_by_ordinal():
+    """The reference's "%1[HD] #nth" format (lib/oracle.c:41): the driver's
+    first four mallocs are its operands A, x, y, z."""
+    r = _run_preloaded("D #0 A\nD #1 x\nD #2 y\nD #3 z\n")
+    assert '"tracked": 4' in r.stdout
+    r = _run_preloaded("D #0\nH #1\nD #2\n# comment line\n")
+    assert '"tracked": 2' in r.stdout
+    r = _run_preloaded("")
+    assert '"tracked": 0' in r.stdout
+
+
+def test_objtrack_trace_names_call_sites():
+    """The trace option logs every tracked allocation with its call site
+    (module+offset), the key a call-site objtrack line uses."""
+    with tempfile.TemporaryDirectory() as td:
+        tr = os.path.join(td, "trace.txt")
+        _run_preloaded("D #0\nD #1\n", extra=f";trace={tr}")
+        lines = open(tr).read().splitlines()
+    t = [ln for ln in lines if ln.startswith("T ")]
+    assert len(t) == 2 and all("site=[chain_bench+0x" in ln for ln in t)
+    site = re.search(r"site=\[(chain_bench\+0x[0-9a-f]+)\]", t[0]).group(1)
+    r = _run_preloaded(f"D @{site}\n")
+    assert '"tracked": 1' in r.stdout
+
+
+def test_device_required_message_without_gpu():
+    if gb.device_ready():
+        pytest.skip("a GPU is present")
+    code = ("import numpy as np, libgpublas_amd as gb; gb.set_cutoff(0); "
+            "a=np.ones((4,4),order='F'); gb.dgemm('N','N',4,4,4,1.0,a,4,a,4,0.0,a,4)")
+    out = subprocess.run(["python3", "-c", code], cwd=ROOT, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode != 0
+    assert "no usable gfx950 GPU" in out.stderr
+
+
+def test_netlib_dblat3_error_exits_and_fallthrough_gemm():
+    """The reference's own Level-3 tester (tests/netlib/dblat3.f, built from
+    the reference tree into oracle/_ref) run against libgpublas: error exits
+    of DGEMM (DCHKE) and DGEMM results through the host fall-through."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "dblat3")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/dblat3 not built")
+    inp = open(os.path.join(ROOT, "tests", "golden", "input.dblat3.gemm")).read()
+    with tempfile.TemporaryDirectory() as td:
+        env = dict(os.environ)
+        env["GPUBLAS_OPTIONS"] = "cutoff=512"
+        env["GPUBLAS_HOST_BLAS"] = gb.__dict__.get("SCIPY_OPENBLAS", "")
+        out = subprocess.run([exe], input=inp, cwd=td, env=env, capture_output=True, text=True,
+                             timeout=300)
+        summ = open(os.path.join(td, "DBLAT3.SUMM")).read()
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert "DGEMM  PASSED THE TESTS OF ERROR-EXITS" in summ
+    assert "DGEMM  PASSED THE COMPUTATIONAL TESTS" in summ
+    assert "FAIL" not in summ and "**" not in summ

@@ -1,0 +1,287 @@
+"""Pins of the CPU oracle (oracle/refblas.c) -- CPU only.
+
+The oracle restates the netlib semantics the reference's host fall-through
+computes.  It is checked here against
+  * the reference's own known-answer data: the DMMCH exact-data self-test of
+    tests/netlib/dblat3.f:195-256 (all four trans forms of GEMM, bit-exact);
+  * the host BLAS the reference falls through to on this machine (scipy's
+    OpenBLAS 0.3.29, through scipy.linalg.blas), within the DMMCH gauge;
+  * closed forms (nrm2 of constant vectors, idamax tie/NaN rules, identity
+    and integer-exact products, triangular solves with integer solutions);
+  * the published SplitMix64 test vector for the input generator.
+Each check is chosen so that a dropped term, a swapped index or a wrong
+transpose in the oracle would fail it.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg.blas as sb
+
+import _oracle
+from _oracle import c, p, uniform
+
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _oracle.load()
+
+
+def test_splitmix64_published_vector(L):
+    # Vigna's splitmix64.c, seed 1234567: the first five outputs
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+            4593380528125082431, 16408922859458223821]
+    assert [L.ora_splitmix64_at(1234567, i) for i in range(5)] == want
+
+
+def test_uniform_generator_exact_grid(L):
+    u = uniform(77, (100000,))
+    assert u.min() >= -1.0 and u.max() < 1.0
+    assert np.all(np.floor(u * 2.0 ** 52) == u * 2.0 ** 52)  # on the 2^-52 grid
+    assert abs(u.mean()) < 0.01
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_dmmch_exact_data_selftest(L, ta, tb):
+    """dblat3.f:195-256: AB(i,j) = max(i-j+1, 0), b_j = j (NN/NT) or N-j+1
+    (TN/TT); the exact product is CC(j) = j((j+1)j)/2 - ((j+1)j(j-1))/3,
+    reversed for the transposed-A forms."""
+    n = 32
+    i = np.arange(1, n + 1)[:, None]
+    j = np.arange(1, n + 1)[None, :]
+    AB = np.asfortranarray(np.maximum(i - j + 1, 0).astype(float))
+    jj = np.arange(1, n + 1)
+    cc = (jj * ((jj + 1) * jj) // 2 - ((jj + 1) * jj * (jj - 1)) // 3).astype(float)
+    if ta == "N":
+        bvec = jj.astype(float)
+        want = cc
+    else:
+        bvec = (n - jj + 1).astype(float)
+        want = cc[::-1].copy()
+    B = np.asfortranarray(bvec.reshape(n, 1) if tb == "N" else bvec.reshape(1, n))
+    C = np.zeros((n, 1), order="F")
+    L.ora_dgemm(c(ta), c(tb), n, 1, n, 1.0, p(AB), n, p(B), B.shape[0], 0.0, p(C), n)
+    assert np.array_equal(C[:, 0], want)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (3, 5, 2), (17, 9, 33)])
+def test_dgemm_vs_host_blas(L, ta, tb, m, n, k):
+    A = uniform(1, (m, k) if ta == "N" else (k, m))
+    B = uniform(2, (k, n) if tb == "N" else (n, k))
+    C0 = uniform(3, (m, n))
+    R = C0.copy(order="F")
+    L.ora_dgemm(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B), B.shape[0], 1.3, p(R), m)
+    H = sb.dgemm(0.7, A, B, 1.3, C0.copy(order="F"), trans_a=ta == "T", trans_b=tb == "T")
+    ratio = L.ora_gemm_gauge_ratio(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B),
+                                   B.shape[0], 1.3, p(C0), p(np.asfortranarray(H)), m, EPS)
+    assert ratio < 16
+    ratio = L.ora_gemm_gauge_ratio(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B),
+                                   B.shape[0], 1.3, p(C0), p(R), m, EPS)
+    assert ratio < 16
+
+
+def test_gauge_checker_catches_planted_error(L):
+    m = n = k = 20
+    A, B, C0 = uniform(4, (m, k)), uniform(5, (k, n)), uniform(6, (m, n))
+    R = C0.copy(order="F")
+    L.ora_dgemm(b"N", b"N", m, n, k, 1.0, p(A), m, p(B), k, 1.0, p(R), m)
+    G = np.abs(A) @ np.abs(B) + np.abs(C0)
+    bad = R.copy(order="F")
+    bad[3, 7] += 100 * EPS * G[3, 7]
+    assert L.ora_gemm_gauge_ratio(b"N", b"N", m, n, k, 1.0, p(A), m, p(B), k, 1.0, p(C0),
+                                  p(bad), m, EPS) > 16
+    # a transposed operand is a gross error
+    wrong = np.asfortranarray(A.T @ B + C0)
+    assert L.ora_gemm_gauge_ratio(b"N", b"N", m, n, k, 1.0, p(A), m, p(B), k, 1.0, p(C0),
+                                  p(wrong), m, EPS) > 1e6
+
+
+def test_dgemm_beta_zero_and_alpha_zero_netlib(L):
+    A, B = uniform(7, (4, 3)), uniform(8, (3, 5))
+    C = np.full((4, 5), np.nan, order="F")
+    L.ora_dgemm(b"N", b"N", 4, 5, 3, 1.0, p(A), 4, p(B), 3, 0.0, p(C), 4)
+    assert np.all(np.isfinite(C))  # beta == 0: C is not read
+    C = np.full((4, 5), np.nan, order="F")
+    L.ora_dgemm(b"N", b"N", 4, 5, 3, 0.0, p(A), 4, p(B), 3, 0.0, p(C), 4)
+    assert np.all(C == 0.0)  # alpha == beta == 0 gives exact zeros
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dsyrk_vs_host_blas(L, uplo, trans):
+    n, k = 13, 7
+    A = uniform(9, (n, k) if trans == "N" else (k, n))
+    C0 = uniform(10, (n, n))
+    R = C0.copy(order="F")
+    L.ora_dsyrk(c(uplo), c(trans), n, k, 0.5, p(A), A.shape[0], 2.0, p(R), n)
+    H = sb.dsyrk(0.5, A, 2.0, C0.copy(order="F"), trans=int(trans == "T"), lower=int(uplo == "L"))
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    np.testing.assert_allclose(R[tri], H[tri], rtol=1e-14, atol=1e-14)
+    assert np.array_equal(R[~tri], C0[~tri])
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_dtrsm_vs_host_blas(L, side, uplo, ta, diag):
+    m, n = 11, 6
+    na = m if side == "L" else n
+    A = uniform(11, (na, na))
+    A[np.diag_indices(na)] += 4.0
+    B0 = uniform(12, (m, n))
+    R = B0.copy(order="F")
+    L.ora_dtrsm(c(side), c(uplo), c(ta), c(diag), m, n, 0.5, p(A), na, p(R), m)
+    H = sb.dtrsm(0.5, A, B0.copy(order="F"), side=int(side == "R"), lower=int(uplo == "L"),
+                 trans_a=int(ta == "T"), diag=int(diag == "U"))
+    np.testing.assert_allclose(R, H, rtol=1e-13, atol=1e-14)
+
+
+def test_dtrsm_integer_solution(L):
+    rng = np.random.default_rng(0)
+    n, r = 12, 4
+    U = np.triu(rng.integers(-3, 4, (n, n)).astype(float), 1) + np.eye(n)
+    X = rng.integers(-5, 6, (n, r)).astype(float)
+    B = np.asfortranarray(U @ X)
+    Uf = np.asfortranarray(U)
+    L.ora_dtrsm(b"L", b"U", b"N", b"U", n, r, 1.0, p(Uf), n, p(B), n)
+    assert np.array_equal(B, X)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_dsymm_vs_host_blas(L, side, uplo):
+    m, n = 9, 7
+    na = m if side == "L" else n
+    A = uniform(13, (na, na))
+    B, C0 = uniform(14, (m, n)), uniform(15, (m, n))
+    R = C0.copy(order="F")
+    L.ora_dsymm(c(side), c(uplo), m, n, 1.5, p(A), na, p(B), m, 0.25, p(R), m)
+    H = sb.dsymm(1.5, A, B, 0.25, C0.copy(order="F"), side=int(side == "R"), lower=int(uplo == "L"))
+    np.testing.assert_allclose(R, H, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1), (-3, 2)])
+def test_dgemv_vs_host_blas(L, trans, incx, incy):
+    m, n = 15, 11
+    A = uniform(16, (m, n))
+    lx, ly = (n, m) if trans == "N" else (m, n)
+    x = uniform(17, (1 + (lx - 1) * abs(incx),))
+    y0 = uniform(18, (1 + (ly - 1) * abs(incy),))
+    R = y0.copy()
+    L.ora_dgemv(c(trans), m, n, 0.9, p(A), m, p(x), incx, 0.4, p(R), incy)
+    H = sb.dgemv(0.9, A, x, beta=0.4, y=y0.copy(), incx=abs(incx), incy=abs(incy),
+                 offx=0, offy=0, trans=int(trans == "T"))
+    if incx > 0 and incy > 0:
+        np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-15)
+    # independent closed form with BLAS stride conventions
+    xs = x[:: abs(incx)][:lx]
+    xs = xs if incx > 0 else xs[::-1]
+    ys = y0[:: abs(incy)][:ly]
+    ys = ys if incy > 0 else ys[::-1]
+    want = 0.9 * ((A if trans == "N" else A.T) @ xs) + 0.4 * ys
+    got = R[:: abs(incy)][:ly]
+    got = got if incy > 0 else got[::-1]
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14)
+
+
+def test_dger_and_axpy_closed_forms(L):
+    m, n = 8, 6
+    A0 = uniform(19, (m, n))
+    x, y = uniform(20, (m,)), uniform(21, (n,))
+    R = A0.copy(order="F")
+    L.ora_dger(m, n, 2.0, p(x), 1, p(y), 1, p(R), m)
+    np.testing.assert_allclose(R, A0 + 2.0 * np.outer(x, y), rtol=1e-15, atol=1e-15)
+    yy = uniform(22, (m,))
+    r = yy.copy()
+    L.ora_daxpy(m, -0.5, p(x), 1, p(r), 1)
+    assert np.array_equal(r, yy + (-0.5) * x)
+    r = yy.copy()
+    L.ora_daxpy(4, 2.0, p(x), -2, p(r), 1)  # x walked from its far end
+    assert np.array_equal(r[:4], yy[:4] + 2.0 * x[6::-2])  # x(7), x(5), x(3), x(1)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dtrsv_vs_host_blas(L, uplo, trans):
+    n = 10
+    A = uniform(23, (n, n))
+    A[np.diag_indices(n)] += 5.0
+    x0 = uniform(24, (n,))
+    R = x0.copy()
+    L.ora_dtrsv(c(uplo), c(trans), b"N", n, p(A), n, p(R), 1)
+    H = sb.dtrsv(A, x0.copy(), lower=int(uplo == "L"), trans=int(trans == "T"), diag=0)
+    np.testing.assert_allclose(R, H, rtol=1e-13, atol=1e-14)
+
+
+def test_ddot_nrm2_vs_host_blas(L):
+    x, y = uniform(25, (1000,)), uniform(26, (1000,))
+    assert abs(L.ora_ddot(1000, p(x), 1, p(y), 1) - sb.ddot(x, y)) <= 1e3 * EPS * np.abs(x * y).sum()
+    assert L.ora_dnrm2(1000, p(x), 1) == pytest.approx(sb.dnrm2(x), rel=1e-14)
+    assert L.ora_dnrm2(0, p(x), 1) == 0.0
+    assert L.ora_dnrm2(1, p(x), 1) == abs(x[0])
+
+
+@pytest.mark.parametrize("scale", [1e300, 1e-300, 1.0])
+def test_dnrm2_closed_form(L, scale):
+    x = np.full(400, scale)
+    assert L.ora_dnrm2(400, p(x), 1) == pytest.approx(scale * 20.0, rel=1e-15)
+
+
+def test_idamax_rules(L):
+    x = np.array([1.0, -3.0, 3.0, 2.0])
+    assert L.ora_idamax(4, p(x), 1) == 2  # first of the tie, 1-based
+    assert sb.idamax(x) + 1 == 2  # scipy returns 0-based
+    x = np.array([np.nan, 5.0, 1.0])
+    assert L.ora_idamax(3, p(x), 1) == 1
+    x = np.array([1.0, np.nan, 5.0])
+    assert L.ora_idamax(3, p(x), 1) == 3
+    assert L.ora_idamax(0, p(x), 1) == 0
+    assert L.ora_idamax(3, p(x), 0) == 0
+    y = uniform(27, (5000,))
+    assert L.ora_idamax(5000, p(y), 1) == int(np.argmax(np.abs(y))) + 1
+    assert L.ora_idamax(5000, p(y), 1) == sb.idamax(y) + 1
+
+
+def test_info_codes(L):
+    # gemm.cc:87-116: 1 transa, 2 transb, 3 m, 4 n, 5 k, 8 lda, 10 ldb, 13 ldc
+    g = L.ora_gemm_info
+    assert g(b"X", b"N", 2, 2, 2, 2, 2, 2) == 1
+    assert g(b"N", b"Q", 2, 2, 2, 2, 2, 2) == 2
+    assert g(b"N", b"N", -1, 2, 2, 2, 2, 2) == 3
+    assert g(b"N", b"N", 2, -1, 2, 2, 2, 2) == 4
+    assert g(b"N", b"N", 2, 2, -1, 2, 2, 2) == 5
+    assert g(b"N", b"N", 3, 2, 2, 2, 2, 3) == 8
+    assert g(b"T", b"N", 3, 2, 4, 3, 3, 3) == 8  # nrowa = k for 'T'
+    assert g(b"N", b"N", 2, 2, 3, 2, 2, 2) == 10
+    assert g(b"N", b"T", 2, 3, 2, 2, 2, 2) == 10  # nrowb = n for 'T'
+    assert g(b"N", b"N", 3, 2, 2, 3, 2, 2) == 13
+    assert g(b"c", b"t", 2, 2, 2, 2, 2, 2) == 0
+    s = L.ora_syrk_info  # syrk.cc:87-115
+    assert [s(b"X", b"N", 2, 2, 2, 2), s(b"L", b"X", 2, 2, 2, 2), s(b"L", b"N", -1, 2, 2, 2),
+            s(b"L", b"N", 2, -1, 2, 2), s(b"L", b"N", 3, 2, 2, 3), s(b"L", b"T", 2, 3, 2, 2),
+            s(b"U", b"N", 3, 2, 3, 2)] == [1, 2, 3, 4, 7, 7, 10]
+    t = L.ora_trsm_info  # trsm.cc:84-116
+    assert [t(b"X", b"L", b"N", b"N", 2, 2, 2, 2), t(b"L", b"X", b"N", b"N", 2, 2, 2, 2),
+            t(b"L", b"L", b"X", b"N", 2, 2, 2, 2), t(b"L", b"L", b"N", b"X", 2, 2, 2, 2),
+            t(b"L", b"L", b"N", b"N", -1, 2, 2, 2), t(b"L", b"L", b"N", b"N", 2, -1, 2, 2),
+            t(b"L", b"L", b"N", b"N", 3, 2, 2, 3), t(b"R", b"L", b"N", b"N", 2, 3, 2, 2),
+            t(b"L", b"L", b"N", b"N", 3, 2, 3, 2)] == [1, 2, 3, 4, 5, 6, 9, 9, 11]
+    y = L.ora_symm_info  # symm.cc:81-104
+    assert [y(b"X", b"L", 2, 2, 2, 2, 2), y(b"L", b"X", 2, 2, 2, 2, 2), y(b"L", b"L", -1, 2, 2, 2, 2),
+            y(b"L", b"L", 2, -1, 2, 2, 2), y(b"L", b"L", 3, 2, 2, 3, 3), y(b"R", b"L", 2, 3, 2, 2, 2),
+            y(b"L", b"L", 3, 2, 3, 2, 3), y(b"L", b"L", 3, 2, 3, 3, 2)] == [1, 2, 3, 4, 7, 7, 9, 12]
+    v = L.ora_gemv_info
+    assert [v(b"X", 2, 2, 2, 1, 1), v(b"N", -1, 2, 2, 1, 1), v(b"N", 2, -1, 2, 1, 1),
+            v(b"N", 3, 2, 2, 1, 1), v(b"N", 2, 2, 2, 0, 1), v(b"N", 2, 2, 2, 1, 0)] == [1, 2, 3, 6, 8, 11]
+    r = L.ora_ger_info
+    assert [r(-1, 2, 1, 1, 2), r(2, -1, 1, 1, 2), r(2, 2, 0, 1, 2), r(2, 2, 1, 0, 2),
+            r(3, 2, 1, 1, 2)] == [1, 2, 5, 7, 9]
+    tv = L.ora_trsv_info
+    assert [tv(b"X", b"N", b"N", 2, 2, 1), tv(b"L", b"X", b"N", 2, 2, 1), tv(b"L", b"N", b"X", 2, 2, 1),
+            tv(b"L", b"N", b"N", -1, 2, 1), tv(b"L", b"N", b"N", 3, 2, 1),
+            tv(b"L", b"N", b"N", 2, 2, 0)] == [1, 2, 3, 4, 6, 8]
