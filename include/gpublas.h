@@ -126,6 +126,8 @@ typedef struct {
     uint64_t faults;         /* host accesses that pulled a tracked object back */
     uint64_t uploads;        /* tracked objects (re)copied to HBM */
     uint64_t allocs_seen;    /* malloc-family calls counted for ordinals */
+    uint64_t async_returns;  /* void calls on resident objects that returned before
+                                their kernel finished (host access waits for it) */
 } gpublas_stats_t;
 void gpublas_get_stats(gpublas_stats_t *out);
 void gpublas_reset_stats(void);

@@ -27,6 +27,15 @@ const char *gpublas_dev_arch(void);      /* e.g. "gfx950:sramecc+:xnack-" */
 size_t gpublas_dev_total_mem(void);
 void *gpublas_dev_stream(void);          /* calling thread's stream */
 int gpublas_dev_sync(void *stream);
+/* Stream sequence numbers: mark() numbers the work enqueued so far on a
+ * stream; wait(seq) returns once that work is complete (no-op if a sync has
+ * already observed it).  Streams live for the whole process. */
+uint64_t gpublas_dev_stream_mark(void *stream);
+int gpublas_dev_stream_wait(void *stream, uint64_t seq);
+/* Scalar results: the next sequence number of the calling thread's result
+ * slot, and a poll for the reduction kernel's publication of that number. */
+uint64_t gpublas_dev_scalar_next(void);
+int gpublas_dev_scalar_wait(void *stream, uint64_t seq);
 void *gpublas_dev_alloc(size_t bytes);   /* caching pool, 256-B aligned; NULL on OOM */
 void gpublas_dev_release(void *d_ptr);   /* back to the pool; caller has synced users */
 int gpublas_dev_trim(void);              /* return cached blocks to the driver */

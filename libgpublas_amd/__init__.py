@@ -51,7 +51,8 @@ def library():
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "calls", "device_calls", "fallthrough", "hits", "misses", "h2d_bytes", "d2h_bytes",
-        "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen")]
+        "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen",
+        "async_returns")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

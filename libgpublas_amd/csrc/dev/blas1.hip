@@ -70,6 +70,29 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     return r;
 }
 
+// Scalar results land in pinned host memory; when the destination is the
+// calling thread's result slot, the kernel also publishes a sequence number
+// next to it (system-scope fence, then the flag) so the host can poll for
+// completion instead of synchronising the stream.
+__device__ __forceinline__ void publish(uint64_t *flag, uint64_t seq)
+{
+    if (!flag) return;
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+struct Pub {
+    uint64_t *flag;
+    uint64_t seq;
+};
+
+Pub pub_for(const void *h_result)
+{
+    DevCtx *c = dev_ctx();
+    if (h_result == c->pinned) return Pub{(uint64_t *)((char *)c->pinned + 8), c->scalar_seq};
+    return Pub{nullptr, 0};
+}
+
 // ---- DAXPY ------------------------------------------------------------------
 __global__ void axpy_unit_v2(long n2, double a, const double2 *__restrict__ x, double2 *y)
 {
@@ -95,7 +118,7 @@ __global__ void axpy_strided(int n, double a, const double *x, long incx, long x
 __global__ __launch_bounds__(256) void dot_kernel(int n, const double *x, long incx, long x0,
                                                   const double *y, long incy, long y0,
                                                   double *partials, unsigned *ticket,
-                                                  double *result)
+                                                  double *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ double red[4];
     double s = 0;
@@ -114,6 +137,7 @@ __global__ __launch_bounds__(256) void dot_kernel(int n, const double *x, long i
     t = block_sum(t, red);
     if (threadIdx.x == 0) {
         *result = t;
+        publish(flag, seq);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -170,7 +194,7 @@ __device__ double nrm2_finish(double asml, double amed, double abig)
 
 __global__ __launch_bounds__(256) void nrm2_kernel(int n, const double *x, long incx, long x0,
                                                    double *partials, unsigned *ticket,
-                                                   double *result)
+                                                   double *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ double red[4];
     Acc3 a{0, 0, 0};
@@ -197,6 +221,7 @@ __global__ __launch_bounds__(256) void nrm2_kernel(int n, const double *x, long 
     t.big = block_sum(t.big, red);
     if (threadIdx.x == 0) {
         *result = nrm2_finish(t.sml, t.med, t.big);
+        publish(flag, seq);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -244,7 +269,7 @@ __device__ Cand block_best(Cand c, Cand *red)
 
 __global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long incx,
                                                     double *partials, unsigned *ticket,
-                                                    int *result)
+                                                    int *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ Cand red[4];
     Cand c{-1.0, 0x7fffffffffffffffL};
@@ -269,6 +294,7 @@ __global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long
         long r = t.i + 1;
         if (isnan(x[0]) || t.v < 0) r = 1;
         *result = (int)r;
+        publish(flag, seq);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -312,11 +338,12 @@ int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const dou
         return 0;
     }
     RedSlot r = next_slot();
+    const Pub pb = pub_for(h_result);
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "ddot");
     hipLaunchKernelGGL(dot_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
                        start_of(n, incx), d_y, (long)incy, start_of(n, incy),
-                       (double *)r.partials, r.ticket, h_result);
+                       (double *)r.partials, r.ticket, h_result, pb.flag, pb.seq);
     PROF_END(s);
     return (int)hipGetLastError();
 }
@@ -329,11 +356,13 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
         return 0;
     }
     RedSlot r = next_slot();
+    const Pub pb = pub_for(h_result);
     int blocks = red_blocks(n);
     if (blocks > kMaxBlocks / 2) blocks = kMaxBlocks / 2;  // 3 partials per block
     PROF_BEGIN(s, "dnrm2");
     hipLaunchKernelGGL(nrm2_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
-                       start_of(n, incx), (double *)r.partials, r.ticket, h_result);
+                       start_of(n, incx), (double *)r.partials, r.ticket, h_result, pb.flag,
+                       pb.seq);
     PROF_END(s);
     return (int)hipGetLastError();
 }
@@ -346,10 +375,11 @@ int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_
         return 0;
     }
     RedSlot r = next_slot();
+    const Pub pb = pub_for(h_result);
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "idamax");
     hipLaunchKernelGGL(iamax_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
-                       (double *)r.partials, r.ticket, h_result);
+                       (double *)r.partials, r.ticket, h_result, pb.flag, pb.seq);
     PROF_END(s);
     return (int)hipGetLastError();
 }

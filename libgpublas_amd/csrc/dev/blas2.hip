@@ -5,16 +5,21 @@
 // cublasDtrsv (trsv.cc:86).
 //
 // DGEMV 'N' is the HBM stream of the Level-1/2 chain: y = alpha*A*x + beta*y
-// with A column-major.  A workgroup owns 128 rows x CB columns; each of its
-// 4 waves reads whole 1 KB column segments (64 lanes x 16 B) of every 4th
-// column, so each load instruction is one contiguous 1 KB burst.  The waves'
-// partial sums are folded through LDS in wave order; when the columns are
-// split over several workgroups, each writes its 128 partial sums and the
-// workgroup drawing the last ticket of that row block folds the column
-// blocks in index order and applies alpha / beta.  The reduction order is a
-// function of (m, n) only, so results are reproducible.
-// DGEMV 'T' runs one wave per output element (a column of A), 16-B loads and
-// a wave64 shuffle tree.
+// with A column-major.  Two launches:
+//  * gemv_n_stream: a workgroup owns a 512-row x CB-column panel of A; wave
+//    w reads rows [128w, 128w+128) of every column of the panel, 16 B per
+//    lane, so each load instruction is one contiguous 1 KB burst and the 4
+//    waves together stream 4 KB of each column.  Eight columns' loads are in
+//    flight per wave (non-temporal: A is read once), two workgroups per CU.
+//    Each workgroup writes its 512 partial sums to a per-panel slot;
+//  * gemv_n_fold: sums the column panels of each row in panel order and
+//    applies alpha / beta.
+// A single-launch variant that folds in the last workgroup of each row block
+// needs a device-scope release per workgroup, which writes back the XCD's L2
+// and measured slower at this grid size, so the fold is a second kernel.  The
+// summation order depends only on (m, n): results are reproducible.
+// DGEMV 'T' runs one wave per column of A (a contiguous dot product), eight
+// 16-B loads in flight per lane and a wave64 shuffle tree.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 
@@ -23,84 +28,85 @@ int gpublas_dev_dtrsm(void *stream, char side, char uplo, char transa, char diag
 
 namespace {
 
-constexpr int GEMV_ROWS = 128;
+typedef double d2 __attribute__((ext_vector_type(2)));
+constexpr int GN_ROWS = 512;  // rows per workgroup panel (256 threads x 2)
+constexpr int GN_U = 8;       // columns in flight per wave
 
-__global__ __launch_bounds__(256) void gemv_n_kernel(int m, int n, double alpha, const double *A,
+__global__ __launch_bounds__(256) void gemv_n_stream(int m, int n, double alpha, const double *A,
                                                      long lda, const double *x, long incx, long x0,
                                                      double beta, double *y, long incy, long y0,
-                                                     int nrb, int ncb, int cbw, int vec,
-                                                     double *partials, unsigned *tickets)
+                                                     int nrb, int cbw, int direct, int vec,
+                                                     double *partials)
 {
-    __shared__ double red[4][GEMV_ROWS];
-    __shared__ bool is_last;
     const int rb = blockIdx.x % nrb, cb = blockIdx.x / nrb;
-    const int r0 = rb * GEMV_ROWS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long row = (long)rb * GN_ROWS + wave * 128 + 2 * lane;
     const int c0 = cb * cbw;
     const int c1 = min(n, c0 + cbw);
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int row = r0 + 2 * lane;
     double s0 = 0, s1 = 0;
     if (vec && row + 1 < m) {
         const double *a = A + row;
-        int c = c0 + wave;
-        for (; c + 12 < c1; c += 16) {
-            const double2 v0 = *(const double2 *)(a + (long)c * lda);
-            const double2 v1 = *(const double2 *)(a + (long)(c + 4) * lda);
-            const double2 v2 = *(const double2 *)(a + (long)(c + 8) * lda);
-            const double2 v3 = *(const double2 *)(a + (long)(c + 12) * lda);
-            const double x0v = x[x0 + (long)c * incx], x1v = x[x0 + (long)(c + 4) * incx];
-            const double x2v = x[x0 + (long)(c + 8) * incx], x3v = x[x0 + (long)(c + 12) * incx];
-            s0 = fma(v0.x, x0v, s0); s1 = fma(v0.y, x0v, s1);
-            s0 = fma(v1.x, x1v, s0); s1 = fma(v1.y, x1v, s1);
-            s0 = fma(v2.x, x2v, s0); s1 = fma(v2.y, x2v, s1);
-            s0 = fma(v3.x, x3v, s0); s1 = fma(v3.y, x3v, s1);
+        int c = c0;
+        for (; c + GN_U <= c1; c += GN_U) {
+            d2 v[GN_U];
+#pragma unroll
+            for (int u = 0; u < GN_U; ++u)
+                v[u] = __builtin_nontemporal_load((const d2 *)(a + (long)(c + u) * lda));
+#pragma unroll
+            for (int u = 0; u < GN_U; ++u) {
+                const double xv = x[x0 + (long)(c + u) * incx];
+                s0 = fma(v[u].x, xv, s0);
+                s1 = fma(v[u].y, xv, s1);
+            }
         }
-        for (; c < c1; c += 4) {
-            const double2 v = *(const double2 *)(a + (long)c * lda);
+        for (; c < c1; ++c) {
+            const d2 v = *(const d2 *)(a + (long)c * lda);
             const double xv = x[x0 + (long)c * incx];
             s0 = fma(v.x, xv, s0);
             s1 = fma(v.y, xv, s1);
         }
     } else {
-        for (int c = c0 + wave; c < c1; c += 4) {
+        for (int c = c0; c < c1; ++c) {
             const double xv = x[x0 + (long)c * incx];
             if (row < m) s0 = fma(A[row + (long)c * lda], xv, s0);
             if (row + 1 < m) s1 = fma(A[row + 1 + (long)c * lda], xv, s1);
         }
     }
-    red[wave][2 * lane] = s0;
-    red[wave][2 * lane + 1] = s1;
-    __syncthreads();
-    const int t = threadIdx.x;
-    const int i = r0 + t;
-    double v = 0;
-    if (t < GEMV_ROWS) v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-    if (ncb == 1) {
-        if (t < GEMV_ROWS && i < m) {
-            double *yp = y + y0 + (long)i * incy;
-            *yp = beta == 0.0 ? alpha * v : fma(beta, *yp, alpha * v);
+    if (direct) {
+        if (row < m) {
+            double *yp = y + y0 + row * incy;
+            *yp = beta == 0.0 ? alpha * s0 : fma(beta, *yp, alpha * s0);
+        }
+        if (row + 1 < m) {
+            double *yp = y + y0 + (row + 1) * incy;
+            *yp = beta == 0.0 ? alpha * s1 : fma(beta, *yp, alpha * s1);
         }
         return;
     }
-    if (t < GEMV_ROWS && i < m) partials[(long)cb * m + i] = v;
-    __syncthreads();
-    if (t == 0) {
-        __threadfence();
-        unsigned k = __hip_atomic_fetch_add(&tickets[rb], 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (k == (unsigned)ncb - 1);
+    double *pp = partials + (long)cb * m;
+    if (row < m) pp[row] = s0;
+    if (row + 1 < m) pp[row + 1] = s1;
+}
+
+__global__ __launch_bounds__(256) void gemv_n_fold(int m, int ncb, const double *partials,
+                                                   double alpha, double beta, double *y, long incy,
+                                                   long y0)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double s = 0;
+    int b = 0;
+    for (; b + 4 <= ncb; b += 4) {
+        const double p0 = partials[(long)b * m + i], p1 = partials[(long)(b + 1) * m + i];
+        const double p2 = partials[(long)(b + 2) * m + i], p3 = partials[(long)(b + 3) * m + i];
+        s += p0;
+        s += p1;
+        s += p2;
+        s += p3;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    if (t < GEMV_ROWS && i < m) {
-        double acc = 0;
-        for (int b = 0; b < ncb; ++b) acc += partials[(long)b * m + i];
-        double *yp = y + y0 + (long)i * incy;
-        *yp = beta == 0.0 ? alpha * acc : fma(beta, *yp, alpha * acc);
-    }
-    if (t == 0) __hip_atomic_store(&tickets[rb], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (; b < ncb; ++b) s += partials[(long)b * m + i];
+    double *yp = y + y0 + i * incy;
+    *yp = beta == 0.0 ? alpha * s : fma(beta, *yp, alpha * s);
 }
 
 __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, double alpha, const double *A,
@@ -109,26 +115,39 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, int n, double alpha,
                                                      int vec)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long j = (long)blockIdx.x * 4 + wave;
-    if (j >= n) return;
-    const double *a = A + j * lda;
-    double s = 0;
-    if (vec && incx == 1 && x0 == 0) {
-        int i = 2 * lane;
-        for (; i + 1 < m; i += 128) {
-            const double2 av = *(const double2 *)(a + i);
-            const double2 xv = *(const double2 *)(x + i);
-            s = fma(av.x, xv.x, s);
-            s = fma(av.y, xv.y, s);
+    for (long j = (long)blockIdx.x * 4 + wave; j < n; j += (long)gridDim.x * 4) {
+        const double *a = A + j * lda;
+        double s = 0;
+        if (vec && incx == 1 && x0 == 0) {
+            int i = 2 * lane;
+            for (; i + 128 * 7 + 1 < m; i += 128 * 8) {
+                d2 av[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    av[u] = __builtin_nontemporal_load((const d2 *)(a + i + 128 * u));
+                    xv[u] = *(const d2 *)(x + i + 128 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s = fma(av[u].x, xv[u].x, s);
+                    s = fma(av[u].y, xv[u].y, s);
+                }
+            }
+            for (; i + 1 < m; i += 128) {
+                const d2 av = *(const d2 *)(a + i);
+                const d2 xv = *(const d2 *)(x + i);
+                s = fma(av.x, xv.x, s);
+                s = fma(av.y, xv.y, s);
+            }
+            if (i < m) s = fma(a[i], x[i], s);
+        } else {
+            for (int i = lane; i < m; i += 64) s = fma(a[i], x[x0 + (long)i * incx], s);
         }
-        if (i < m) s = fma(a[i], x[i], s);
-    } else {
-        for (int i = lane; i < m; i += 64) s = fma(a[i], x[x0 + (long)i * incx], s);
-    }
-    s = wave_sum(s);
-    if (lane == 0) {
-        double *yp = y + y0 + j * incy;
-        *yp = beta == 0.0 ? alpha * s : fma(beta, *yp, alpha * s);
+        s = wave_sum(s);
+        if (lane == 0) {
+            double *yp = y + y0 + j * incy;
+            *yp = beta == 0.0 ? alpha * s : fma(beta, *yp, alpha * s);
+        }
     }
 }
 
@@ -194,13 +213,12 @@ int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, cons
         return (int)hipGetLastError();
     }
     if (nt) {
-        const int nrb = ceil_div(m, GEMV_ROWS);
-        // Aim for ~4 workgroups per CU; each workgroup at least 64 columns.
-        int ncb = ceil_div(1024, nrb);
-        const int maxcb = ceil_div(n, 64);
+        // ~2 workgroups per CU, each panel at least 16 columns wide
+        const int nrb = ceil_div(m, GN_ROWS);
+        int ncb = ceil_div(512, nrb);
+        const int maxcb = ceil_div(n, 16);
         if (ncb > maxcb) ncb = maxcb;
         if (ncb < 1) ncb = 1;
-        if (nrb > kTicketSlots) ncb = 1;
         const int cbw = ceil_div(n, ncb);
         ncb = ceil_div(n, cbw);
         double *partials = nullptr;
@@ -210,16 +228,24 @@ int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, cons
         }
         const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
         PROF_BEGIN(s, "dgemv_n");
-        hipLaunchKernelGGL(gemv_n_kernel, dim3((unsigned)(nrb * ncb)), dim3(256), 0, s, m, n,
-                           alpha, d_A, (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0,
-                           nrb, ncb, cbw, vec, partials, dev_ctx()->tickets);
+        hipLaunchKernelGGL(gemv_n_stream, dim3((unsigned)nrb * ncb), dim3(256), 0, s, m, n, alpha,
+                           d_A, (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0, nrb,
+                           cbw, ncb == 1, vec, partials);
         PROF_END(s);
+        if (ncb > 1) {
+            PROF_BEGIN(s, "dgemv_n_fold");
+            hipLaunchKernelGGL(gemv_n_fold, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, m,
+                               ncb, partials, alpha, beta, d_y, (long)incy, y0);
+            PROF_END(s);
+        }
         return (int)hipGetLastError();
     }
     const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0) && (((uintptr_t)d_x & 15) == 0);
+    int blocks = ceil_div(n, 4);
+    if (blocks > 2048) blocks = 2048;
     PROF_BEGIN(s, "dgemv_t");
-    hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, s, m, n, alpha,
-                       d_A, (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0, vec);
+    hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)blocks), dim3(256), 0, s, m, n, alpha, d_A,
+                       (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0, vec);
     PROF_END(s);
     return (int)hipGetLastError();
 }

@@ -24,13 +24,20 @@ struct DevCtx {
     void *ws;              // device workspace (reduction partials)
     size_t ws_bytes;
     unsigned *tickets;     // device ticket counters, kept zero between launches
-    void *pinned;          // 256 B pinned host slot for scalar results
+    void *pinned;          // 256 B pinned host slot: [0] scalar result, [8] publish flag
+    uint64_t submitted;    // calls marked on this stream (gpublas_dev_stream_mark)
+    uint64_t completed;    // calls known complete (a sync observed them)
+    uint64_t scalar_seq;   // last sequence number published into the pinned slot
 };
+// Context owning `stream` (contexts and their streams live for the process).
+DevCtx *ctx_of(hipStream_t stream);
 
-// Returns the calling thread's context (creates it on first use).  The stream
-// argument of the C ABI may be NULL (use this thread's stream) or a stream of
-// the caller's; workspace and tickets always come from the calling thread's
-// context, so two threads never share a workspace.
+// Returns the calling thread's context (takes a pooled one, or creates it, on
+// first use; returns it to the pool at thread exit -- streams are never
+// destroyed, so a stream handle recorded by the host layer stays valid).  The
+// stream argument of the C ABI may be NULL (use this thread's stream) or a
+// stream of the caller's; workspace and tickets always come from the calling
+// thread's context, so two threads never share a workspace.
 DevCtx *dev_ctx();
 // Workspace of at least `bytes` on the calling thread's context (grows).
 void *dev_workspace(size_t bytes);

@@ -171,36 +171,66 @@ int g_prof_on = 0;
 
 // ---- per-thread context ------------------------------------------------------
 namespace {
+std::mutex g_ctx_mu;
+std::vector<DevCtx *> *g_ctx_all;   // every context ever made (stream registry)
+std::vector<DevCtx *> *g_ctx_free;  // contexts of exited threads
+
+DevCtx *ctx_create()
+{
+    DevCtx *c = new DevCtx();
+    (void)hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    (void)hipMalloc(&c->tickets, kTicketSlots * sizeof(unsigned));
+    (void)hipMemsetAsync(c->tickets, 0, kTicketSlots * sizeof(unsigned), c->stream);
+    (void)hipHostMalloc(&c->pinned, 256, hipHostMallocDefault);
+    memset(c->pinned, 0, 256);
+    (void)hipStreamSynchronize(c->stream);
+    return c;
+}
+
 struct CtxHolder {
-    DevCtx c{};
+    DevCtx *c = nullptr;
     ~CtxHolder()
     {
-        // Thread exit: the stream and buffers are released only if the
-        // runtime is still up (process teardown order is not guaranteed).
-        if (c.stream && g_init_status == 0) {
-            (void)hipStreamSynchronize(c.stream);
-            if (c.ws) gpublas_dev_release(c.ws);
-            if (c.tickets) (void)hipFree(c.tickets);
-            if (c.pinned) (void)hipHostFree(c.pinned);
-            (void)hipStreamDestroy(c.stream);
+        // Thread exit: finish the thread's work and park the context for the
+        // next thread (its stream may still be named by tracked objects).
+        if (c && g_init_status == 0) {
+            (void)hipStreamSynchronize(c->stream);
+            std::lock_guard<std::mutex> g(g_ctx_mu);
+            g_ctx_free->push_back(c);
         }
     }
 };
 thread_local CtxHolder t_ctx;
+DevCtx g_null_ctx{};
 }  // namespace
 
 DevCtx *dev_ctx()
 {
-    DevCtx &c = t_ctx.c;
-    if (!c.stream) {
-        if (gpublas_dev_init() != 0) return &c;
-        (void)hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
-        (void)hipMalloc(&c.tickets, kTicketSlots * sizeof(unsigned));
-        (void)hipMemsetAsync(c.tickets, 0, kTicketSlots * sizeof(unsigned), c.stream);
-        (void)hipHostMalloc(&c.pinned, 256, hipHostMallocDefault);
-        (void)hipStreamSynchronize(c.stream);
+    if (t_ctx.c) return t_ctx.c;
+    if (gpublas_dev_init() != 0) return &g_null_ctx;
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    if (!g_ctx_all) {
+        g_ctx_all = new std::vector<DevCtx *>();
+        g_ctx_free = new std::vector<DevCtx *>();
     }
-    return &c;
+    if (!g_ctx_free->empty()) {
+        t_ctx.c = g_ctx_free->back();
+        g_ctx_free->pop_back();
+    } else {
+        t_ctx.c = ctx_create();
+        g_ctx_all->push_back(t_ctx.c);
+    }
+    return t_ctx.c;
+}
+
+DevCtx *ctx_of(hipStream_t stream)
+{
+    DevCtx *mine = dev_ctx();
+    if (!stream || mine->stream == stream) return mine;
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    for (DevCtx *c : *g_ctx_all)
+        if (c->stream == stream) return c;
+    return nullptr;
 }
 
 void *dev_workspace(size_t bytes)
@@ -265,7 +295,62 @@ size_t gpublas_dev_total_mem(void)
 
 void *gpublas_dev_stream(void) { return (void *)dev_ctx()->stream; }
 
-int gpublas_dev_sync(void *stream) { return (int)hipStreamSynchronize(pick_stream(stream)); }
+int gpublas_dev_sync(void *stream)
+{
+    hipStream_t s = pick_stream(stream);
+    DevCtx *c = ctx_of(s);
+    const uint64_t sub = c ? __atomic_load_n(&c->submitted, __ATOMIC_ACQUIRE) : 0;
+    int err = (int)hipStreamSynchronize(s);
+    if (c && !err) {
+        uint64_t cur = __atomic_load_n(&c->completed, __ATOMIC_RELAXED);
+        while (cur < sub && !__atomic_compare_exchange_n(&c->completed, &cur, sub, false,
+                                                         __ATOMIC_RELEASE, __ATOMIC_RELAXED)) {
+        }
+    }
+    return err;
+}
+
+uint64_t gpublas_dev_stream_mark(void *stream)
+{
+    DevCtx *c = ctx_of(pick_stream(stream));
+    return c ? __atomic_add_fetch(&c->submitted, 1, __ATOMIC_ACQ_REL) : 0;
+}
+
+int gpublas_dev_stream_wait(void *stream, uint64_t seq)
+{
+    DevCtx *c = ctx_of(pick_stream(stream));
+    if (!c) return (int)hipStreamSynchronize(pick_stream(stream));
+    if (__atomic_load_n(&c->completed, __ATOMIC_ACQUIRE) >= seq) return 0;
+    return gpublas_dev_sync(c->stream);
+}
+
+uint64_t gpublas_dev_scalar_next(void)
+{
+    DevCtx *c = dev_ctx();
+    return ++c->scalar_seq;
+}
+
+int gpublas_dev_scalar_wait(void *stream, uint64_t seq)
+{
+    // The reduction kernels publish [value, seq] into this thread's pinned
+    // slot; poll the flag (a system-scope store after a system fence) rather
+    // than paying a full stream synchronisation.  Every 4096 polls the stream
+    // is queried so an error or a lost publication ends the wait.
+    DevCtx *c = dev_ctx();
+    volatile uint64_t *flag = (volatile uint64_t *)((char *)c->pinned + 8);
+    for (unsigned spin = 1;; ++spin) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return 0;
+        if ((spin & 4095) == 0) {
+            hipError_t q = hipStreamQuery(pick_stream(stream));
+            if (q == hipSuccess) {
+                // stream drained: the flag must be there now
+                if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return 0;
+                return (int)hipErrorUnknown;
+            }
+            if (q != hipErrorNotReady) return (int)q;
+        }
+    }
+}
 
 void *gpublas_dev_alloc(size_t bytes)
 {

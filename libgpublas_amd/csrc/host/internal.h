@@ -67,6 +67,8 @@ struct gb_obj {
     void *callsite;
     int state;
     int used;            /* seen as a BLAS operand */
+    void *use_stream;    /* stream of the last kernel that used the mirror */
+    uint64_t use_seq;    /* its sequence number on that stream */
     pthread_mutex_t mu;
 };
 void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero);
@@ -76,8 +78,11 @@ struct gb_obj *gb_obj_find(const void *p);   /* object containing p, or NULL */
 size_t gb_obj_usable_size(const void *p);    /* 0 if untracked */
 int gb_tracked_count(void);
 void gb_install_fault_handler(void);
-/* Make the device copy valid (upload if the host copy is newer). */
-void gb_obj_to_device(struct gb_obj *o, void *stream);
+/* Make the device copy valid (upload if the host copy is newer); returns 1 if
+ * it uploaded.  Orders `stream` after the object's last use elsewhere. */
+int gb_obj_to_device(struct gb_obj *o, void *stream);
+/* Wait until no kernel uses the object's device mirror any more. */
+void gb_obj_quiesce(struct gb_obj *o);
 /* Mark the device copy as the only valid one (after a kernel wrote it). */
 void gb_obj_device_written(struct gb_obj *o);
 
@@ -99,6 +104,7 @@ typedef struct {
     char *tmp;           /* untracked: pool block */
     size_t ld, rows, cols, es;
     int mode;
+    int uploaded;        /* this call copied host data in (object or staging) */
 } gb_operand;
 /* Matrix rows x cols with leading dimension ld (elements of es bytes). */
 void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows, size_t cols,
@@ -107,9 +113,16 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
 void gb_acquire_vector(gb_operand *op, const void *host, int n, int inc, size_t es, int mode,
                        void *stream);
 /* Copy outputs back (untracked) or mark them device-resident (tracked).
- * Call after the kernel launch; gb_release_all syncs the stream first. */
+ * Call after the kernel launch. */
 void gb_release(gb_operand *op, void *stream);
+/* End of a void BLAS call.  When every operand is a tracked object that was
+ * already resident, the call returns without synchronising: nothing the host
+ * can see is pending (a host touch of a written object faults and waits for
+ * the kernel first).  Otherwise it synchronises and frees staging buffers. */
 void gb_finish(void *stream, gb_operand *ops, int nops);
+/* End of a scalar-returning call: wait for the kernel's publication of `seq`
+ * in the result slot, then release staging buffers. */
+void gb_finish_scalar(void *stream, gb_operand *ops, int nops, uint64_t seq);
 bool gb_any_tracked(const gb_operand *ops, int nops);
 bool gb_ptr_tracked(const void *p);
 

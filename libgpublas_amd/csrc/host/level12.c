@@ -7,8 +7,9 @@
  * blas_level2/gemv.cc:11-98, ger.cc:11-94, trsv.cc:11-88; meson.build:34-76).
  * Their behaviour is the netlib definition those wrappers forward to; the
  * argument checks are netlib's (INFO codes in the comments).  Scalar results
- * are written by the kernel into a pinned per-thread slot and returned by
- * value after the stream synchronises.
+ * are written by the kernel into a pinned per-thread slot together with a
+ * sequence number; the call polls for that number (a few microseconds less
+ * than a stream synchronisation) and returns the value.
  *
  * Host fall-through: when none of a call's operands is a tracked
  * (device-resident) object and its length is below the GEMM cutoff, the call
@@ -83,10 +84,11 @@ double ddot_(const int *n, const double *x, const int *incx, const double *y, co
     gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
     gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
     double *slot = (double *)gpublas_dev_result_slot();
+    const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_ddot(s, *n, (const double *)ops[0].dev, *incx,
                               (const double *)ops[1].dev, *incy, slot),
              "ddot launch");
-    gb_finish(s, ops, 2);
+    gb_finish_scalar(s, ops, 2, seq);
     double r = *(volatile double *)slot;
     GB_COUNT(device_calls, 1);
     gb_leave();
@@ -114,8 +116,9 @@ double dnrm2_(const int *n, const double *x, const int *incx)
     gb_operand op;
     gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
     double *slot = (double *)gpublas_dev_result_slot();
+    const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_dnrm2(s, *n, (const double *)op.dev, *incx, slot), "dnrm2 launch");
-    gb_finish(s, &op, 1);
+    gb_finish_scalar(s, &op, 1, seq);
     double r = *(volatile double *)slot;
     GB_COUNT(device_calls, 1);
     gb_leave();
@@ -147,8 +150,9 @@ int idamax_(const int *n, const double *x, const int *incx)
     gb_operand op;
     gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
     int *slot = (int *)gpublas_dev_result_slot();
+    const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_idamax(s, *n, (const double *)op.dev, *incx, slot), "idamax launch");
-    gb_finish(s, &op, 1);
+    gb_finish_scalar(s, &op, 1, seq);
     int r = *(volatile int *)slot;
     GB_COUNT(device_calls, 1);
     gb_leave();

@@ -169,9 +169,16 @@ void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero)
     return host;
 }
 
+void gb_obj_quiesce(struct gb_obj *o)
+{
+    if (o->use_stream)
+        gb_check(gpublas_dev_stream_wait(o->use_stream, o->use_seq), "wait for a tracked object");
+}
+
 static void obj_free(struct gb_obj *o)
 {
     gb_trace_alloc('U', o->nth, o->callsite, o->size, o->host);
+    gb_obj_quiesce(o); /* a kernel launched by a call that returned early may still use it */
     munmap(o->host, o->map_size);
     munmap(o->alias, o->map_size);
     if (o->dev) gpublas_dev_release(o->dev);
@@ -194,6 +201,7 @@ static void obj_pull_to_host(struct gb_obj *o)
 {
     /* caller holds o->mu */
     if (o->state == ST_DEVICE) {
+        gb_obj_quiesce(o);
         void *s = gpublas_dev_stream();
         gb_check(gpublas_dev_d2h(o->alias, o->dev, o->size, s), "download of a tracked object");
         gb_check(gpublas_dev_sync(s), "download of a tracked object");
@@ -223,14 +231,18 @@ void *gb_obj_realloc(void *p, size_t size)
 }
 
 /* ---- coherence ---- */
-void gb_obj_to_device(struct gb_obj *o, void *stream)
+int gb_obj_to_device(struct gb_obj *o, void *stream)
 {
+    int uploaded = 0;
     pthread_mutex_lock(&o->mu);
     if (!o->dev) {
         o->dev = gpublas_dev_alloc(o->map_size);
         if (!o->dev) gb_fatal("HBM exhausted allocating a %zu-byte tracked object", o->map_size);
     }
+    /* work queued on another stream may still read or write the mirror */
+    if (o->use_stream && o->use_stream != stream) gb_obj_quiesce(o);
     if (o->state == ST_HOST) {
+        uploaded = 1;
         /* protect first: a host write racing the upload faults and demotes */
         mprotect(o->host, o->map_size, PROT_READ);
         gb_check(gpublas_dev_h2d(o->dev, o->alias, o->size, stream), "upload of a tracked object");
@@ -240,6 +252,7 @@ void gb_obj_to_device(struct gb_obj *o, void *stream)
     }
     o->used = 1;
     pthread_mutex_unlock(&o->mu);
+    return uploaded;
 }
 
 void gb_obj_device_written(struct gb_obj *o)
@@ -334,7 +347,7 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
     const size_t span = ((cols - 1) * ld + rows) * es;
     struct gb_obj *o = gb_obj_find(host);
     if (o && (const char *)host + span <= o->host + o->size) {
-        gb_obj_to_device(o, stream);
+        op->uploaded = gb_obj_to_device(o, stream);
         op->obj = o;
         op->dev = o->dev + ((const char *)host - o->host);
         gb_trace_call("operand", host);
@@ -349,6 +362,7 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
                            : gpublas_dev_h2d_2d(op->dev, ld, host, ld, rows, cols, es, stream);
         gb_check(e, "operand upload");
         GB_COUNT(h2d_bytes, rows * cols * es);
+        op->uploaded = 1;
     }
     GB_COUNT(misses, 1);
 }
@@ -385,10 +399,41 @@ void gb_release(gb_operand *op, void *stream)
     }
 }
 
+static void mark_uses(void *stream, gb_operand *ops, int nops)
+{
+    const uint64_t seq = gpublas_dev_stream_mark(stream);
+    for (int i = 0; i < nops; ++i) {
+        struct gb_obj *o = ops[i].obj;
+        if (!o) continue;
+        pthread_mutex_lock(&o->mu);
+        o->use_stream = stream;
+        o->use_seq = seq;
+        pthread_mutex_unlock(&o->mu);
+    }
+}
+
 void gb_finish(void *stream, gb_operand *ops, int nops)
 {
-    for (int i = 0; i < nops; ++i) gb_release(&ops[i], stream);
+    int async_ok = 1;
+    for (int i = 0; i < nops; ++i) {
+        gb_release(&ops[i], stream);
+        if (ops[i].dev && (!ops[i].obj || ops[i].uploaded)) async_ok = 0;
+    }
+    mark_uses(stream, ops, nops);
+    if (async_ok) {
+        GB_COUNT(async_returns, 1);
+        return;
+    }
     gb_check(gpublas_dev_sync(stream), "BLAS kernel");
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].tmp) gpublas_dev_release(ops[i].tmp);
+}
+
+void gb_finish_scalar(void *stream, gb_operand *ops, int nops, uint64_t seq)
+{
+    mark_uses(stream, ops, nops);
+    gb_check(gpublas_dev_scalar_wait(stream, seq), "BLAS reduction kernel");
+    /* the kernel has consumed its inputs: staging blocks can be reused */
     for (int i = 0; i < nops; ++i)
         if (ops[i].tmp) gpublas_dev_release(ops[i].tmp);
 }

@@ -40,4 +40,5 @@ def test_dblat3_all_hot_path_routines_on_gpu(gpu):
     assert "FAIL" not in summ and "**" not in summ
     hdr = [h.strip() for h in stats[0].split(",")]
     vals = dict(zip(hdr, (int(v) for v in stats[1].split(","))))
-    assert vals["DeviceCalls"] > 10000 and vals["Fallthrough"] == 0
+    # every call that is not an error exit or a quick / alpha == 0 return ran on the GPU
+    assert vals["DeviceCalls"] > 5000 and vals["Fallthrough"] == 0
