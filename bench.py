@@ -122,7 +122,7 @@ def gemm_figures():
         bufs = [lib.gpublas_dev_alloc(n * n * 8) for _ in range(3)]
         for i, b in enumerate(bufs):
             lib.gpublas_dev_fill_uniform(None, 101 + i, n * n, b)
-        kern = "dgemm_mfma_128x128" if n >= 2048 else "dgemm_mfma_64x64"
+        kern = "dgemm_fast_128x128" if n >= 2048 else "dgemm_fast_64x64"
         for _ in range(2):
             lib.gpublas_dev_dgemm(None, b"N", b"N", n, n, n, 1.0, bufs[0], n, bufs[1], n, 0.5, bufs[2], n)
         lib.gpublas_dev_sync(None)

@@ -55,6 +55,7 @@ struct GemmArgs {
     long ldc;
     int tiles_m, tiles_n;
     int tri;  // 0 rectangular grid, SYM_LOWER / SYM_UPPER: triangular tile grid + mask
+    int full_m, full_n;  // generic kernel: tiles below both counts were done by the fast kernel
 };
 
 __device__ __forceinline__ bool stored(int sym, long r, long c)
@@ -117,6 +118,43 @@ struct TileLoader {
                 const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) st[r][e] = elem_load(op, o0 + o + e, k0 + kk, K);
+            }
+        }
+    }
+
+    // Fast path: full tile, aligned operand, pattern fixed at compile time.
+    template <int PAT>
+    __device__ __forceinline__ static void load_pat(const Operand<T> &op, int o0, int k0,
+                                                    Vec (&st)[NV])
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            const int v = t + r * 256;
+            if (PAT == PAT_OUTER) {
+                const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
+                st[r] = *(const Vec *)(op.p + (long)(o0 + o) + (long)(k0 + kk) * op.ld);
+            } else {
+                const int kk = (v % (BK / VEC)) * VEC, o = v / (BK / VEC);
+                st[r] = *(const Vec *)(op.p + (long)(k0 + kk) + (long)(o0 + o) * op.ld);
+            }
+        }
+    }
+
+    template <int PAT, int LDS_LD>
+    __device__ __forceinline__ static void store_pat(T (*s)[LDS_LD], const Vec (&st)[NV])
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            const int v = t + r * 256;
+            if (PAT == PAT_KCONT) {
+                const int kk = (v % (BK / VEC)) * VEC, o = v / (BK / VEC);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) s[kk + e][o] = st[r][e];
+            } else {
+                const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
+                *(Vec *)&s[kk][o] = st[r];
             }
         }
     }
@@ -191,6 +229,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> g)
         tri_decode(blockIdx.x, hi, lo);
         if (g.tri == SYM_LOWER) { tm = hi; tn = lo; } else { tm = lo; tn = hi; }
     }
+    if (tm < g.full_m && tn < g.full_n) return;  // interior tile: fast kernel's
     const int m0 = tm * BM, n0 = tn * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -268,6 +307,111 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> g)
     }
 }
 
+// Fast kernel: the grid holds only tiles that are full in M and N, both
+// operands are 16-B aligned with leading dimensions a multiple of the vector
+// width, and their access patterns are compile-time constants (PA / PB).
+// That leaves no per-element selects in the main loop and frees enough
+// registers for two workgroups per CU, so one workgroup's global loads and
+// barrier overlap the other's MFMAs.  Only a partial last k-tile takes the
+// guarded element loads.
+template <typename T, int BM, int BN, int BK, int PA, int PB, int TRI>
+__global__ __launch_bounds__(256, 2) void gemm_fast_kernel(GemmArgs<T> g)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    typedef TileLoader<T, BM, BK> LA;
+    typedef TileLoader<T, BN, BK> LB;
+    typedef typename Mfma<T>::Acc Acc;
+
+    __shared__ __attribute__((aligned(16))) T As[2][BK][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BK][BN + PAD];
+
+    int tm, tn;
+    if (TRI == SYM_NONE) {
+        tm = blockIdx.x % g.tiles_m;
+        tn = blockIdx.x / g.tiles_m;
+    } else {
+        int hi, lo;
+        tri_decode(blockIdx.x, hi, lo);
+        if (TRI == SYM_LOWER) { tm = hi; tn = lo; } else { tm = lo; tn = hi; }
+    }
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int K = g.k;
+    const int ktiles = (K + BK - 1) / BK;
+
+    Acc acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Acc{};
+
+    typename LA::Vec sa[LA::NV];
+    typename LB::Vec sb[LB::NV];
+    auto load = [&](int k0) -> bool {
+        if (k0 + BK <= K) {
+            LA::template load_pat<PA>(g.a, m0, k0, sa);
+            LB::template load_pat<PB>(g.b, n0, k0, sb);
+            return false;
+        }
+        LA::load(g.a, m0, k0, K, PAT_ELEM, sa);
+        LB::load(g.b, n0, k0, K, PAT_ELEM, sb);
+        return true;
+    };
+    auto store = [&](int buf, bool tail) {
+        if (!tail) {
+            LA::template store_pat<PA, BM + PAD>(As[buf], sa);
+            LB::template store_pat<PB, BN + PAD>(Bs[buf], sb);
+        } else {
+            LA::template store_pat<PAT_OUTER, BM + PAD>(As[buf], sa);
+            LB::template store_pat<PAT_OUTER, BN + PAD>(Bs[buf], sb);
+        }
+    };
+
+    store(0, load(0));
+    __syncthreads();
+    for (int kt = 0; kt < ktiles; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < ktiles;
+        bool tail = false;
+        if (more) tail = load((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T fa[FM], fb[FN];
+            const int kr = kk + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(fb[j], fa[i], acc[i][j]);
+        }
+        if (more) store(cur ^ 1, tail);
+        __syncthreads();
+    }
+
+    const T alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const long gi = m0 + wm * WM + i * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long gj = n0 + wn * WN + j * 16 + Mfma<T>::drow(lane, r);
+                if (TRI == SYM_LOWER && gi < gj) continue;
+                if (TRI == SYM_UPPER && gi > gj) continue;
+                T *c = g.C + gi + gj * g.ldc;
+                const T v = alpha * acc[i][j][r];
+                *c = beta == T(0) ? v : v + beta * *c;
+            }
+        }
+    }
+}
+
 // Scale (or zero) a triangle or rectangle of C: the alpha == 0 / k == 0
 // paths of the netlib routines (beta == 0 stores exact zeros).
 template <typename T>
@@ -315,12 +459,45 @@ Operand<T> make_op(const T *p, long ld, int olim, int form, int sym)
     return o;
 }
 
-template <typename T> const char *kname(int big);
-template <> const char *kname<double>(int big) { return big ? "dgemm_mfma_128x128" : "dgemm_mfma_64x64"; }
-template <> const char *kname<float>(int big) { return big ? "sgemm_mfma_128x128" : "sgemm_mfma_64x64"; }
+template <typename T> const char *kname(int big, int fast);
+template <> const char *kname<double>(int big, int fast)
+{
+    if (fast) return big ? "dgemm_fast_128x128" : "dgemm_fast_64x64";
+    return big ? "dgemm_edge_128x128" : "dgemm_edge_64x64";
+}
+template <> const char *kname<float>(int big, int fast)
+{
+    if (fast) return big ? "sgemm_fast_128x128" : "sgemm_fast_64x64";
+    return big ? "sgemm_edge_128x128" : "sgemm_edge_64x64";
+}
+
+template <typename T, int BM>
+void launch_fast(hipStream_t s, const GemmArgs<T> &g, unsigned blocks)
+{
+#define GB_FAST(PA_, PB_, TRI_)                                                                  \
+    hipLaunchKernelGGL((gemm_fast_kernel<T, BM, BM, 16, PA_, PB_, TRI_>), dim3(blocks),         \
+                       dim3(256), 0, s, g)
+    const int pa = g.a.form, pb = g.b.form;
+    if (g.tri == SYM_NONE) {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_FAST(PAT_OUTER, PAT_KCONT, SYM_NONE);      // NN
+        else if (pa == PAT_OUTER) GB_FAST(PAT_OUTER, PAT_OUTER, SYM_NONE);                    // NT
+        else if (pb == PAT_KCONT) GB_FAST(PAT_KCONT, PAT_KCONT, SYM_NONE);                    // TN
+        else GB_FAST(PAT_KCONT, PAT_OUTER, SYM_NONE);                                         // TT
+    } else if (g.tri == SYM_LOWER) {
+        if (pa == PAT_OUTER) GB_FAST(PAT_OUTER, PAT_OUTER, SYM_LOWER);  // A*A^T
+        else GB_FAST(PAT_KCONT, PAT_KCONT, SYM_LOWER);                  // A^T*A
+    } else {
+        if (pa == PAT_OUTER) GB_FAST(PAT_OUTER, PAT_OUTER, SYM_UPPER);
+        else GB_FAST(PAT_KCONT, PAT_KCONT, SYM_UPPER);
+    }
+#undef GB_FAST
+}
 
 // Core launcher: C = alpha*op(A)*op(B) + beta*C on an (optionally
-// triangular) tile grid.  k > 0, alpha != 0 assumed.
+// triangular) tile grid.  k > 0, alpha != 0 assumed.  Tiles that are full in
+// M and N go to the fast kernel when both operands allow it; the generic
+// kernel takes the ragged edge (and everything, for symmetric or unaligned
+// operands).
 template <typename T>
 int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a,
                 const Operand<T> &b, T beta, T *C, long ldc, int tri)
@@ -336,18 +513,34 @@ int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a
     const long t128 = (long)ceil_div(m, 128) * ceil_div(n, 128);
     const bool big = t128 >= 256;
     const int bm = big ? 128 : 64;
+    const bool fast_ok = a.sym == SYM_NONE && b.sym == SYM_NONE && a.vec_ok && b.vec_ok;
+    const int full_m = fast_ok ? m / bm : 0, full_n = fast_ok ? n / bm : 0;
+    const long fast_blocks = tri ? (long)full_m * (full_m + 1) / 2 : (long)full_m * full_n;
+    if (fast_blocks > 0) {
+        GemmArgs<T> f = g;
+        f.tiles_m = full_m;
+        f.tiles_n = full_n;
+        f.full_m = f.full_n = 0;
+        PROF_BEGIN(s, kname<T>(big, 1));
+        if (big) launch_fast<T, 128>(s, f, (unsigned)fast_blocks);
+        else launch_fast<T, 64>(s, f, (unsigned)fast_blocks);
+        PROF_END(s);
+    }
     g.tiles_m = ceil_div(m, bm);
     g.tiles_n = ceil_div(n, bm);
-    long blocks = tri ? (long)g.tiles_m * (g.tiles_m + 1) / 2 : (long)g.tiles_m * g.tiles_n;
-    if (blocks <= 0) return 0;
-    PROF_BEGIN(s, kname<T>(big));
-    if (big)
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, 128, 128, 16>), dim3((unsigned)blocks), dim3(256),
-                           0, s, g);
-    else
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16>), dim3((unsigned)blocks), dim3(256), 0,
-                           s, g);
-    PROF_END(s);
+    g.full_m = full_m;
+    g.full_n = full_n;
+    const long blocks = tri ? (long)g.tiles_m * (g.tiles_m + 1) / 2 : (long)g.tiles_m * g.tiles_n;
+    if (blocks > fast_blocks) {
+        PROF_BEGIN(s, kname<T>(big, 0));
+        if (big)
+            hipLaunchKernelGGL((gemm_mfma_kernel<T, 128, 128, 16>), dim3((unsigned)blocks),
+                               dim3(256), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm_mfma_kernel<T, 64, 64, 16>), dim3((unsigned)blocks),
+                               dim3(256), 0, s, g);
+        PROF_END(s);
+    }
     return (int)hipGetLastError();
 }
 
