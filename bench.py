@@ -42,7 +42,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_chain(steps, warmup, seed=20240601):
+def run_chain(steps, warmup, profile, seed=20240601):
     import libgpublas_amd as gb
     with tempfile.TemporaryDirectory() as td:
         objtrack = os.path.join(td, "chain.objtrack")
@@ -52,7 +52,8 @@ def run_chain(steps, warmup, seed=20240601):
         env = dict(os.environ)
         pre = env.get("LD_PRELOAD", "").strip()
         env["LD_PRELOAD"] = (pre + " " if pre else "") + gb.PRELOAD_PATH
-        env["GPUBLAS_OPTIONS"] = f"heuristic=objtrack:{objtrack};profile;cutoff=0"
+        opts = f"heuristic=objtrack:{objtrack};cutoff=0"
+        env["GPUBLAS_OPTIONS"] = opts + (";profile" if profile else "")
         cmd = [gb.CHAIN_BENCH, "--n", str(N), "--iters", str(steps), "--warmup", str(warmup),
                "--seed", str(seed)]
         log("chain: " + " ".join(cmd))
@@ -145,6 +146,45 @@ def gemm_figures():
     return out
 
 
+def level3_suite(reps):
+    """The Level-3 sweep: GEMM at the four trans combinations with M, N, K in
+    {4096, 8192} in FP64 and FP32; DSYRK lower n=8192 k=2048; DTRSM
+    left/lower/no-trans/non-unit m=8192 nrhs=4096 (diagonally dominant A).
+    One JSON line per shape, kernel TFLOP/s from HIP events."""
+    import itertools
+    import libgpublas_amd as gb
+    exe = os.path.join(gb.BIN_DIR, "blas_bench")
+    env = dict(os.environ)
+    env["GPUBLAS_OPTIONS"] = "profile;cutoff=0"
+    runs = []
+    for prec in ("d", "s"):
+        for ta, tb in itertools.product("NT", "NT"):
+            for m, n, k in itertools.product((4096, 8192), repeat=3):
+                runs.append(["--op", "gemm", "--prec", prec, "--ta", ta, "--tb", tb, "--m", str(m),
+                             "--n", str(n), "--k", str(k)])
+        runs.append(["--op", "syrk", "--prec", prec, "--uplo", "L", "--trans", "N", "--n", "8192",
+                     "--k", "2048"])
+        runs.append(["--op", "trsm", "--prec", prec, "--side", "L", "--uplo", "L", "--ta", "N",
+                     "--diag", "N", "--m", "8192", "--n", "4096"])
+    peak = {"d": FP64_PEAK_TFLOPS, "s": 157.3}
+    results = []
+    for r in runs:
+        cmd = [exe, "--lib", gb.LIB_PATH, "--reps", str(reps), "--warmup", "1"] + r
+        out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=300)
+        if out.returncode != 0:
+            raise RuntimeError(f"{' '.join(cmd)} failed: {out.stderr[-1500:]}")
+        res = json.loads(out.stdout.strip().splitlines()[-1])
+        kms = sum(v[1] for v in res["kernels"].values()) / reps
+        res["kernel_tflops"] = round(res["flops"] / (kms / 1e3) / 1e12, 3) if kms else None
+        res["frac_of_peak"] = round(res["kernel_tflops"] / peak[res["prec"]], 4) if kms else None
+        print(json.dumps(res), flush=True)
+        results.append(res)
+    gemm64 = [r["kernel_tflops"] for r in results if r["op"] == "gemm" and r["prec"] == "d"]
+    gemm32 = [r["kernel_tflops"] for r in results if r["op"] == "gemm" and r["prec"] == "s"]
+    print(json.dumps({"suite": "level3", "dgemm_tflops_min": min(gemm64), "dgemm_tflops_max": max(gemm64),
+                      "sgemm_tflops_min": min(gemm32), "sgemm_tflops_max": max(gemm32)}), flush=True)
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "dgemv_n_chain8192_pmc.txt")
     if not os.path.exists(path):
@@ -159,17 +199,25 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--suite", default="chain", choices=["chain", "level3"],
+                    help="chain: the headline line (default); level3: the GEMM/SYRK/TRSM sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: this build has no multi-GPU path "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    if args.suite == "level3":
+        level3_suite(reps=3)
+        return
 
-    chain = run_chain(args.steps, args.warmup)
+    # value: the K timed steps with no instrumentation; roofline: a second
+    # run with HIP events around every kernel on its launch stream
+    chain = run_chain(args.steps, args.warmup, profile=False)
+    prof = run_chain(args.steps, args.warmup, profile=True)
     secs = chain["seconds"]
     value = BYTES_PER_STEP * args.steps / secs / 1e9
-    kcount = chain.get("k_dgemv_n_count", 0)
-    kavg_s = chain["k_dgemv_n_ms"] / kcount / 1e3 if kcount else None
+    kcount = prof.get("k_dgemv_n_count", 0)
+    kavg_s = prof["k_dgemv_n_ms"] / kcount / 1e3 if kcount else None
     achieved = DGEMV_BYTES / kavg_s / 1e9 if kavg_s else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -177,8 +225,8 @@ def main():
                 "traffic": traffic_from_profile(), "kernel": "dgemv_n",
                 "kernel_avg_us": round(kavg_s * 1e6, 3) if kavg_s else None,
                 "algorithmic_bytes_per_launch": DGEMV_BYTES}
-    kernels = {k[2:-3]: round(chain[k] / max(chain[k[:-3] + "_count"], 1) * 1e3, 3)
-               for k in chain if k.startswith("k_") and k.endswith("_ms")}
+    kernels = {k[2:-3]: round(prof[k] / max(prof[k[:-3] + "_count"], 1) * 1e3, 3)
+               for k in prof if k.startswith("k_") and k.endswith("_ms")}
 
     gemm = gemm_figures()
     line = {

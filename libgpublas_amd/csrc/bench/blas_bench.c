@@ -1,0 +1,219 @@
+/*
+ * blas_bench.c -- times one Level-3 (or Level-1 calibration) call shape
+ * through the Fortran-ABI entry points of libgpublas on tracked operands.
+ *
+ *   blas_bench --lib libgpublas.so --op gemm --prec d --m M --n N --k K --ta N --tb N
+ *   blas_bench --lib ... --op syrk --n N --k K --uplo L --trans N
+ *   blas_bench --lib ... --op trsm --m M --n N --side L --uplo L --ta N --diag N
+ *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
+ *   common: --reps R --warmup W --seed S
+ *
+ * Operands come from gpublas_malloc and are filled on the device with the
+ * library's SplitMix64 generator (uniform[-1,1]); the TRSM triangle gets
+ * diag = n + uniform[0,1] on the host.  Prints one JSON object: wall time per
+ * call (calls on resident operands may return before their kernels finish,
+ * so the timed loop is closed by a stream synchronisation), the kernel times
+ * from the library's HIP-event profiler (GPUBLAS_OPTIONS=profile), and the
+ * rate.
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static double now(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC_RAW, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+static void *lib;
+static void *sym(const char *n)
+{
+    void *f = dlsym(lib, n);
+    if (!f) {
+        fprintf(stderr, "blas_bench: missing %s\n", n);
+        exit(2);
+    }
+    return f;
+}
+
+typedef void *(*malloc_t)(size_t);
+typedef void *(*devptr_t)(const void *);
+typedef int (*fill_t)(void *, uint64_t, int64_t, void *);
+typedef int (*fill32_t)(void *, uint64_t, int64_t, void *);
+
+static malloc_t g_malloc;
+static devptr_t g_devptr;
+static fill_t g_fill;
+static fill32_t g_fill32;
+static int (*g_sync)(void *);
+
+static void *operand(size_t elems, int prec, uint64_t seed)
+{
+    size_t es = prec == 'd' ? 8 : 4;
+    void *h = g_malloc(elems * es);
+    if (!h) {
+        fprintf(stderr, "blas_bench: gpublas_malloc(%zu) failed\n", elems * es);
+        exit(2);
+    }
+    void *d = g_devptr(h);
+    if (prec == 'd') g_fill(NULL, seed, (int64_t)elems, d);
+    else g_fill32(NULL, seed, (int64_t)elems, d);
+    g_sync(NULL);
+    return h;
+}
+
+int main(int argc, char **argv)
+{
+    const char *path = NULL, *op = "gemm";
+    int prec = 'd', m = 1024, n = 1024, k = 1024, reps = 5, warmup = 1;
+    char ta = 'N', tb = 'N', uplo = 'L', trans = 'N', side = 'L', diag = 'N';
+    uint64_t seed = 7;
+    for (int i = 1; i < argc; ++i) {
+        const char *a = argv[i], *v = i + 1 < argc ? argv[i + 1] : "";
+        if (!strcmp(a, "--lib")) path = v, ++i;
+        else if (!strcmp(a, "--op")) op = v, ++i;
+        else if (!strcmp(a, "--prec")) prec = v[0], ++i;
+        else if (!strcmp(a, "--m")) m = atoi(v), ++i;
+        else if (!strcmp(a, "--n")) n = atoi(v), ++i;
+        else if (!strcmp(a, "--k")) k = atoi(v), ++i;
+        else if (!strcmp(a, "--ta")) ta = v[0], ++i;
+        else if (!strcmp(a, "--tb")) tb = v[0], ++i;
+        else if (!strcmp(a, "--uplo")) uplo = v[0], ++i;
+        else if (!strcmp(a, "--trans")) trans = v[0], ++i;
+        else if (!strcmp(a, "--side")) side = v[0], ++i;
+        else if (!strcmp(a, "--diag")) diag = v[0], ++i;
+        else if (!strcmp(a, "--reps")) reps = atoi(v), ++i;
+        else if (!strcmp(a, "--warmup")) warmup = atoi(v), ++i;
+        else if (!strcmp(a, "--seed")) seed = strtoull(v, 0, 10), ++i;
+    }
+    if (!path) {
+        fprintf(stderr, "blas_bench: --lib required\n");
+        return 2;
+    }
+    lib = dlopen(path, RTLD_NOW | RTLD_GLOBAL);
+    if (!lib) {
+        fprintf(stderr, "blas_bench: %s\n", dlerror());
+        return 2;
+    }
+    g_malloc = (malloc_t)sym("gpublas_malloc");
+    g_devptr = (devptr_t)sym("gpublas_device_ptr");
+    g_fill = (fill_t)sym("gpublas_dev_fill_uniform");
+    g_fill32 = (fill32_t)sym("gpublas_dev_fill_uniform_f32");
+    g_sync = (int (*)(void *))sym("gpublas_dev_sync");
+    void (*set_cutoff)(int) = (void (*)(int))sym("gpublas_set_cutoff");
+    int (*ktime)(const char *, double *) = (int (*)(const char *, double *))sym("gpublas_kernel_time");
+    int (*dump)(char *, size_t) = (int (*)(char *, size_t))sym("gpublas_dev_prof_dump");
+    void (*prof_reset)(void) = (void (*)(void))sym("gpublas_dev_prof_reset");
+    set_cutoff(0);
+    (void)ktime;
+
+    double flops = 0, bytes = 0;
+    void *A = NULL, *B = NULL, *C = NULL;
+    double alpha_d = 1.0, beta_d = 0.5;
+    float alpha_s = 1.0f, beta_s = 0.5f;
+    void *alpha = prec == 'd' ? (void *)&alpha_d : (void *)&alpha_s;
+    void *beta = prec == 'd' ? (void *)&beta_d : (void *)&beta_s;
+    int lda = 0, ldb = 0, ldc = 0;
+    typedef void (*gemm_t)(const char *, const char *, const int *, const int *, const int *,
+                           const void *, const void *, const int *, const void *, const int *,
+                           const void *, void *, const int *);
+    typedef void (*syrk_t)(const char *, const char *, const int *, const int *, const void *,
+                           const void *, const int *, const void *, void *, const int *);
+    typedef void (*trsm_t)(const char *, const char *, const char *, const char *, const int *,
+                           const int *, const void *, const void *, const int *, void *,
+                           const int *);
+    typedef void (*axpy_t)(const int *, const double *, const double *, const int *, double *,
+                           const int *);
+    gemm_t gemm = NULL;
+    syrk_t syrk = NULL;
+    trsm_t trsm = NULL;
+    axpy_t axpy = NULL;
+    if (!strcmp(op, "gemm")) {
+        gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
+        lda = (ta == 'N' || ta == 'n') ? m : k;
+        ldb = (tb == 'N' || tb == 'n') ? k : n;
+        ldc = m;
+        A = operand((size_t)lda * ((ta == 'N' || ta == 'n') ? k : m), prec, seed);
+        B = operand((size_t)ldb * ((tb == 'N' || tb == 'n') ? n : k), prec, seed + 1);
+        C = operand((size_t)m * n, prec, seed + 2);
+        flops = 2.0 * m * n * k;
+    } else if (!strcmp(op, "syrk")) {
+        syrk = (syrk_t)sym(prec == 'd' ? "dsyrk_" : "ssyrk_");
+        lda = (trans == 'N' || trans == 'n') ? n : k;
+        ldc = n;
+        A = operand((size_t)lda * ((trans == 'N' || trans == 'n') ? k : n), prec, seed);
+        C = operand((size_t)n * n, prec, seed + 2);
+        flops = (double)n * (n + 1) * k;
+    } else if (!strcmp(op, "trsm")) {
+        trsm = (trsm_t)sym(prec == 'd' ? "dtrsm_" : "strsm_");
+        const int na = (side == 'L' || side == 'l') ? m : n;
+        lda = na;
+        ldb = m;
+        A = operand((size_t)na * na, prec, seed);
+        B = operand((size_t)m * n, prec, seed + 1);
+        /* diagonally dominant: diag = n + uniform[0,1] (host touch of the diagonal) */
+        uint64_t s = seed + 5;
+        for (int i = 0; i < na; ++i) {
+            s = s * 6364136223846793005ull + 1442695040888963407ull;
+            double u = (double)(s >> 11) * 0x1.0p-53;
+            if (prec == 'd') ((double *)A)[(size_t)i * na + i] = na + u;
+            else ((float *)A)[(size_t)i * na + i] = (float)(na + u);
+        }
+        flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
+    } else if (!strcmp(op, "axpy")) {
+        axpy = (axpy_t)sym("daxpy_");
+        A = operand((size_t)n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        bytes = 24.0 * n;
+    } else {
+        fprintf(stderr, "blas_bench: unknown op %s\n", op);
+        return 2;
+    }
+    double t0 = 0;
+    for (int r = 0; r < warmup + reps; ++r) {
+        if (r == warmup) {
+            g_sync(NULL); /* calls on resident operands may return before their kernels */
+            prof_reset();
+            t0 = now();
+        }
+        if (gemm) gemm(&ta, &tb, &m, &n, &k, alpha, A, &lda, B, &ldb, beta, C, &ldc);
+        else if (syrk) syrk(&uplo, &trans, &n, &k, alpha, A, &lda, beta, C, &ldc);
+        else if (trsm) trsm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
+        else {
+            const double a = 0.5;
+            const int one = 1;
+            axpy(&n, &a, (const double *)A, &one, (double *)B, &one);
+        }
+    }
+    g_sync(NULL);
+    const double wall = (now() - t0) / reps;
+    char buf[4096];
+    buf[0] = 0;
+    int nb = dump(buf, sizeof buf);
+    printf("{\"op\": \"%s\", \"prec\": \"%c\", \"m\": %d, \"n\": %d, \"k\": %d, \"ta\": \"%c\", "
+           "\"tb\": \"%c\", \"uplo\": \"%c\", \"trans\": \"%c\", \"side\": \"%c\", \"reps\": %d, "
+           "\"wall_ms\": %.6f",
+           op, prec, m, n, k, ta, tb, uplo, trans, side, reps, wall * 1e3);
+    if (flops > 0) printf(", \"flops\": %.6e, \"wall_tflops\": %.4f", flops, flops / wall / 1e12);
+    if (bytes > 0) printf(", \"bytes\": %.6e, \"wall_GBps\": %.2f", bytes, bytes / wall / 1e9);
+    /* kernels: "name count total_ms" lines */
+    printf(", \"kernels\": {");
+    int first = 1;
+    for (char *line = strtok(nb > 0 ? buf : NULL, "\n"); line; line = strtok(NULL, "\n")) {
+        char nm[128];
+        long cnt;
+        double ms;
+        if (sscanf(line, "%127s %ld %lf", nm, &cnt, &ms) == 3 && cnt > 0) {
+            printf("%s\"%s\": [%ld, %.6f]", first ? "" : ", ", nm, cnt, ms);
+            first = 0;
+        }
+    }
+    printf("}}\n");
+    return 0;
+}
