@@ -185,6 +185,77 @@ def level3_suite(reps):
                       "sgemm_tflops_min": min(gemm32), "sgemm_tflops_max": max(gemm32)}), flush=True)
 
 
+def split_suite(n=8192, slices=(3, 4, 5, 6)):
+    """Split-precision DGEMM M=N=K=n against plain FP64 DGEMM: (a) uniform[-1,1];
+    (b) uniform entries with every row of A and column of B scaled by
+    2^U[-20,20].  Errors are measured against the host BLAS (OpenBLAS, all
+    cores) on the same inputs; times are kernel times from HIP events (split:
+    exponent/splitting pass + slice products)."""
+    import numpy as np
+    import scipy.linalg.blas as sb
+    import libgpublas_amd as gb
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import uniform
+    lib = gb.library()
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.gpublas_dev_dgemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ci, ci, ci, cd, vp, ci, vp,
+                                      ci, cd, vp, ci]
+    lib.gpublas_dev_dgemm_split.argtypes = [vp, ci, ctypes.c_char, ctypes.c_char, ci, ci, ci, cd, vp,
+                                            ci, vp, ci, cd, vp, ci]
+    lib.gpublas_dev_prof_enable(1)
+    bytes_ = n * n * 8
+    dA, dB, dC = (lib.gpublas_dev_alloc(bytes_) for _ in range(3))
+    host = np.empty((n, n), order="F")
+
+    def ktime(names):
+        tot = 0.0
+        for nm in names:
+            t = ctypes.c_double()
+            lib.gpublas_dev_prof_query(nm.encode(), ctypes.byref(t), None, None)
+            tot += t.value
+        return tot
+
+    for dist in ("uniform", "rowcol_scaled"):
+        A = uniform(501, (n, n))
+        B = uniform(502, (n, n))
+        if dist == "rowcol_scaled":
+            A *= (2.0 ** np.floor(uniform(503, (n,)) * 20.0))[:, None]
+            B *= (2.0 ** np.floor(uniform(504, (n,)) * 20.0))[None, :]
+        lib.gpublas_dev_h2d(dA, A.ctypes.data, bytes_, None)
+        lib.gpublas_dev_h2d(dB, B.ctypes.data, bytes_, None)
+        lib.gpublas_dev_sync(None)
+        t0 = time.perf_counter()
+        R = sb.dgemm(1.0, A, B)
+        host_s = time.perf_counter() - t0
+        normR = np.linalg.norm(R)
+        maxR = np.max(np.abs(R))
+        variants = [("fp64", 0)] + [("split", s) for s in slices]
+        for kind, s in variants:
+            for rep in range(3):
+                if rep == 1:
+                    lib.gpublas_dev_prof_reset()
+                if kind == "fp64":
+                    lib.gpublas_dev_dgemm(None, b"N", b"N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+                else:
+                    lib.gpublas_dev_dgemm_split(None, s, b"N", b"N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+            lib.gpublas_dev_sync(None)
+            ms = ktime(["dgemm_fast_128x128"] if kind == "fp64" else ["ozaki_split", "ozaki_gemm"]) / 2
+            lib.gpublas_dev_d2h(host.ctypes.data, dC, bytes_, None)
+            lib.gpublas_dev_sync(None)
+            err_norm = float(np.linalg.norm(host - R) / normR)
+            err_max = float(np.max(np.abs(host - R)) / maxR)
+            npairs = s * (s + 1) // 2 if s else 0
+            rec = {"suite": "split", "dist": dist, "n": n, "kind": kind, "slices": s,
+                   "slice_products": npairs, "kernel_ms": round(ms, 3),
+                   "effective_tflops": round(2.0 * n ** 3 / (ms / 1e3) / 1e12, 2),
+                   "bf16_mfma_tflops": round(npairs * 2.0 * n ** 3 / (ms / 1e3) / 1e12, 1) if s else None,
+                   "normwise_rel_err": err_norm, "max_rel_err": err_max,
+                   "reference": f"OpenBLAS dgemm on the host ({host_s:.1f} s)"}
+            print(json.dumps(rec), flush=True)
+    for b in (dA, dB, dC):
+        lib.gpublas_dev_release(b)
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "dgemv_n_chain8192_pmc.txt")
     if not os.path.exists(path):
@@ -199,8 +270,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--suite", default="chain", choices=["chain", "level3"],
-                    help="chain: the headline line (default); level3: the GEMM/SYRK/TRSM sweep")
+    ap.add_argument("--suite", default="chain", choices=["chain", "level3", "split"],
+                    help="chain: the headline line (default); level3: the GEMM/SYRK/TRSM sweep; "
+                         "split: split-precision DGEMM vs FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: this build has no multi-GPU path "
@@ -208,6 +280,9 @@ def main():
         sys.exit(2)
     if args.suite == "level3":
         level3_suite(reps=3)
+        return
+    if args.suite == "split":
+        split_suite()
         return
 
     # value: the K timed steps with no instrumentation; roofline: a second

@@ -85,6 +85,18 @@ int gpublas_dev_dtrsm(void *stream, char side, char uplo, char transa, char diag
 int gpublas_dev_strsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
                       float alpha, const float *d_A, int lda, float *d_B, int ldb);
 
+/* Split-precision DGEMM / DSYRK (opt-in; no counterpart in the reference):
+ * Ozaki integer-slice scheme, `slices` (1..8) signed 8-bit slices per
+ * operand held in BF16, slice products on BF16 MFMA with exact FP32
+ * accumulation over 256-wide k-chunks, FP64 scaling and final sums.  Inputs
+ * with Inf/NaN fall back to the FP64 kernels.  Synchronous. */
+int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, int m, int n,
+                            int k, double alpha, const double *d_A, int lda, const double *d_B,
+                            int ldb, double beta, double *d_C, int ldc);
+int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int n, int k,
+                            double alpha, const double *d_A, int lda, double beta, double *d_C,
+                            int ldc);
+
 /* ---- Level 2 ---- */
 int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
                       int lda, const double *d_x, int incx, double beta, double *d_y, int incy);

@@ -76,6 +76,8 @@ static void print_help(void)
            "  cutoff=<n>           GEMM leading dimensions below n go to the host BLAS\n"
            "                       (512; 0 = always the GPU)\n"
            "  profile              time every kernel with HIP events\n"
+           "  split=<s>            DGEMM/DSYRK as split-precision BF16 slice products\n"
+           "                       (s = 1..8 slices per operand; 0 = plain FP64, default)\n"
            "  stats[=<file>]       write hits/misses at exit (statistics.csv)\n"
            "  trace=<file>         log allocations (T/U) and BLAS operand uses (C)\n"
            "  verbose              log decisions to stderr\n");
@@ -103,6 +105,14 @@ int gb_parse_options(const char *s)
         if (!*opt) continue;
         if (!strcmp(opt, "help")) print_help();
         else if (!strcmp(opt, "profile")) gb_opt.profile = 1;
+        else if (!strncmp(opt, "split=", 6)) {
+            gb_opt.split = atoi(opt + 6);
+            if (gb_opt.split < 0 || gb_opt.split > 8) {
+                gb_log(2, "libgpublas: split=%d out of range (0 = off, 1..8 slices)\n", gb_opt.split);
+                gb_opt.split = 0;
+                bad = 1;
+            }
+        }
         else if (!strcmp(opt, "verbose")) gb_opt.verbose = 1;
         else if (!strcmp(opt, "stats")) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "statistics.csv");
         else if (!strncmp(opt, "stats=", 6)) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "%s", opt + 6);

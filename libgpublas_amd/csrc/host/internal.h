@@ -33,6 +33,7 @@ struct gb_options {
     size_t min_track;        /* never track allocations below this */
     int cutoff;              /* gemm fall-through cutoff on leading dims */
     int profile;             /* kernel event timing */
+    int split;               /* >0: DGEMM/DSYRK as split-precision with this many slices */
     int verbose;
     char stats_file[256];    /* "" = no statistics file at exit */
     char trace_file[256];    /* "" = no allocation / call trace */

@@ -21,6 +21,27 @@
 #define MAXI(a, b) ((a) > (b) ? (a) : (b))
 #define IDX(i, j, ld) ((size_t)(j) * (size_t)(ld) + (size_t)(i))
 
+/* DGEMM / DSYRK device dispatch: plain FP64 MFMA, or the opt-in
+ * split-precision scheme (GPUBLAS_OPTIONS split=<slices>). */
+static int dgemm_dev(void *s, char ta, char tb, int m, int n, int k, double alpha,
+                     const double *A, int lda, const double *B, int ldb, double beta, double *C,
+                     int ldc)
+{
+    if (gb_opt.split > 0)
+        return gpublas_dev_dgemm_split(s, gb_opt.split, ta, tb, m, n, k, alpha, A, lda, B, ldb,
+                                       beta, C, ldc);
+    return gpublas_dev_dgemm(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha, const double *A,
+                     int lda, double beta, double *C, int ldc)
+{
+    if (gb_opt.split > 0)
+        return gpublas_dev_dsyrk_split(s, gb_opt.split, uplo, trans, n, k, alpha, A, lda, beta, C,
+                                       ldc);
+    return gpublas_dev_dsyrk(s, uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
 /* ------------------------------------------------------------------ */
 /* GEMM                                                                */
 /* ------------------------------------------------------------------ */
@@ -88,7 +109,7 @@
         gb_leave();                                                                              \
     }
 
-DEFINE_GEMM(d, double, "DGEMM ", gpublas_dev_dgemm)
+DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev)
 DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm)
 
 /* ------------------------------------------------------------------ */
@@ -145,7 +166,7 @@ DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm)
         gb_leave();                                                                              \
     }
 
-DEFINE_SYRK(d, double, "DSYRK ", gpublas_dev_dsyrk)
+DEFINE_SYRK(d, double, "DSYRK ", dsyrk_dev)
 DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
 
 /* ------------------------------------------------------------------ */
