@@ -256,6 +256,37 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
         lib.gpublas_dev_release(b)
 
 
+def serve_run(mode, rate, calls, threads=16, options="serve=64G"):
+    """One serving trace (csrc/bench/serve_bench.c) -> its JSON record."""
+    import libgpublas_amd as gb
+    cmd = [gb.SERVE_BENCH, "--lib", gb.LIB_PATH, "--calls", str(calls), "--threads", str(threads),
+           "--rate", str(rate)]
+    cmd += ["--plain"] if mode == "plain" else ["--options", options]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError(f"serve_bench failed ({out.returncode}): {out.stderr[-2000:]}")
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def serve_suite(calls=20000, rates=(0,)):
+    """Serving trace: 16 threads, Poisson arrivals (rate 0 = closed loop,
+    i.e. saturation), one-launch-per-call vs the batching loop (graph replay,
+    eager launches, and batches of one)."""
+    variants = [("plain", ""), ("serve", "serve=64G"), ("serve", "serve=64G;serve_graph=0"),
+                ("serve", "serve=64G;serve_batch=1")]
+    for rate in rates:
+        base = None
+        for mode, opts in variants:
+            rec = serve_run(mode, rate, calls, options=opts or "serve=64G")
+            rec["suite"] = "serve"
+            if mode == "plain":
+                base = rec
+            elif base:
+                rec["speedup_vs_plain"] = round(rec["calls_per_s"] / base["calls_per_s"], 3)
+                rec["p99_vs_plain"] = round(rec["p99_us"] / base["p99_us"], 3)
+            print(json.dumps(rec), flush=True)
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "dgemv_n_chain8192_pmc.txt")
     if not os.path.exists(path):
@@ -270,9 +301,12 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--suite", default="chain", choices=["chain", "level3", "split"],
+    ap.add_argument("--suite", default="chain", choices=["chain", "level3", "split", "serve"],
                     help="chain: the headline line (default); level3: the GEMM/SYRK/TRSM sweep; "
-                         "split: split-precision DGEMM vs FP64")
+                         "split: split-precision DGEMM vs FP64; serve: the serving trace")
+    ap.add_argument("--calls", type=int, default=20000, help="serve: calls per trace")
+    ap.add_argument("--rates", default="0", help="serve: comma-separated offered rates "
+                                                "(calls/s; 0 = closed loop)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: this build has no multi-GPU path "
@@ -283,6 +317,9 @@ def main():
         return
     if args.suite == "split":
         split_suite()
+        return
+    if args.suite == "serve":
+        serve_suite(args.calls, tuple(float(r) for r in args.rates.split(",")))
         return
 
     # value: the K timed steps with no instrumentation; roofline: a second

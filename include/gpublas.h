@@ -128,9 +128,19 @@ typedef struct {
     uint64_t allocs_seen;    /* malloc-family calls counted for ordinals */
     uint64_t async_returns;  /* void calls on resident objects that returned before
                                 their kernel finished (host access waits for it) */
+    uint64_t evictions;      /* device mirrors dropped to fit the serving budget */
+    uint64_t served;         /* calls run by the serving loop */
+    uint64_t batches;        /* grouped launches of the serving loop */
+    uint64_t max_batch;      /* most calls in one grouped launch */
+    uint64_t deferred;       /* admissions postponed until in-flight batches ended */
+    uint64_t over_budget;    /* admissions past the budget (nothing left to wait for) */
+    uint64_t resident_bytes; /* HBM held by device mirrors now (not reset) */
 } gpublas_stats_t;
 void gpublas_get_stats(gpublas_stats_t *out);
 void gpublas_reset_stats(void);
+/* Wait for every kernel the calling thread has launched (calls on resident
+ * operands may return before their kernel ends; see async_returns). */
+void gpublas_sync(void);
 
 /* Options string (same syntax as GPUBLAS_OPTIONS); returns 0 if all parsed. */
 int gpublas_set_options(const char *opts);

@@ -32,6 +32,7 @@ int gpublas_dev_sync(void *stream);
  * already observed it).  Streams live for the whole process. */
 uint64_t gpublas_dev_stream_mark(void *stream);
 int gpublas_dev_stream_wait(void *stream, uint64_t seq);
+int gpublas_dev_stream_done(void *stream, uint64_t seq);  /* non-blocking: 1 if complete */
 /* Scalar results: the next sequence number of the calling thread's result
  * slot, and a poll for the reduction kernel's publication of that number. */
 uint64_t gpublas_dev_scalar_next(void);
@@ -115,6 +116,46 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
 int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result);
 /* Pinned result slot of the calling thread (256 bytes). */
 void *gpublas_dev_result_slot(void);
+
+/* ---- Batch slots of the serving loop (no counterpart in the reference,
+ * which issues one cuBLAS call per intercepted call: blas_level3/gemm.cc:62-82,
+ * trsm.cc:52-72).  A slot collects up to 64 problems of each kind, of any
+ * sizes, and runs them with one grouped persistent launch per kind -- one
+ * replay of a captured hipGraph (use_graph != 0) or the same kernels
+ * launched eagerly.  Operands are device pointers.  add_*: 0 = added,
+ * 1 = slot full (launch it and use another), -1 = not supported in a batch
+ * (GEMM with alpha == 0 or k == 0, right-side TRSM, zero strides): the
+ * caller uses the per-call entry points.  Results do not depend on what
+ * else is in the batch.  A slot is used by one host thread at a time; clear
+ * it only after done() reports completion. ---- */
+typedef struct gpublas_batch gpublas_batch;
+gpublas_batch *gpublas_dev_batch_create(int use_graph);
+void gpublas_dev_batch_destroy(gpublas_batch *b);
+void gpublas_dev_batch_clear(gpublas_batch *b);
+int gpublas_dev_batch_count(const gpublas_batch *b);
+void *gpublas_dev_batch_stream(gpublas_batch *b);
+/* -1 = tile shape from the tuned table (default); 0/1/2 = 32/64/128 square tiles */
+void gpublas_dev_batch_force_cfg(gpublas_batch *b, int cfg);
+int gpublas_dev_batch_dgemm(gpublas_batch *b, char transa, char transb, int m, int n, int k,
+                            double alpha, const double *d_A, int lda, const double *d_B, int ldb,
+                            double beta, double *d_C, int ldc);
+int gpublas_dev_batch_sgemm(gpublas_batch *b, char transa, char transb, int m, int n, int k,
+                            float alpha, const float *d_A, int lda, const float *d_B, int ldb,
+                            float beta, float *d_C, int ldc);
+int gpublas_dev_batch_dgemv(gpublas_batch *b, char trans, int m, int n, double alpha,
+                            const double *d_A, int lda, const double *d_x, int incx, double beta,
+                            double *d_y, int incy);
+int gpublas_dev_batch_dtrsm(gpublas_batch *b, char side, char uplo, char transa, char diag, int m,
+                            int n, double alpha, const double *d_A, int lda, double *d_B, int ldb);
+int gpublas_dev_batch_launch(gpublas_batch *b);   /* asynchronous */
+int gpublas_dev_batch_done(gpublas_batch *b);     /* 1 once the launched batch has completed */
+int gpublas_dev_batch_wait(gpublas_batch *b);
+/* GEMM tile-shape table of the batch kernels: load from `path` (unless
+ * force), else measure every tile shape per (sqrt(mn), k) size bucket and
+ * write the table to `path` (NULL: keep in memory only). */
+int gpublas_dev_tune_gemm(const char *path, int force);
+int gpublas_dev_tune_choice(int is_single, int m, int n, int k);
+void gpublas_dev_tune_reset(void);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 LIB_PATH = os.path.join(LIB_DIR, "libgpublas.so")
 PRELOAD_PATH = os.path.join(LIB_DIR, "libgpublas_preload.so")
 CHAIN_BENCH = os.path.join(BIN_DIR, "chain_bench")
+SERVE_BENCH = os.path.join(BIN_DIR, "serve_bench")
 
 _lib = None
 
@@ -52,7 +53,8 @@ class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "calls", "device_calls", "fallthrough", "hits", "misses", "h2d_bytes", "d2h_bytes",
         "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen",
-        "async_returns")]
+        "async_returns", "evictions", "served", "batches", "max_batch", "deferred",
+        "over_budget", "resident_bytes")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -100,6 +102,18 @@ def _declare(lib):
     lib.gpublas_dev_stream.restype = vp
     lib.gpublas_dev_arch.restype = ctypes.c_char_p
     lib.gpublas_dev_fill_uniform.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, vp]
+    lib.gpublas_dev_batch_create.restype = vp
+    lib.gpublas_dev_batch_create.argtypes = [ci]
+    for f in ("destroy", "clear", "launch", "done", "wait", "count"):
+        getattr(lib, "gpublas_dev_batch_" + f).argtypes = [vp]
+    lib.gpublas_dev_batch_force_cfg.argtypes = [vp, ci]
+    ch = ctypes.c_char
+    lib.gpublas_dev_batch_dgemm.argtypes = [vp, ch, ch, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.gpublas_dev_batch_sgemm.argtypes = [vp, ch, ch, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci]
+    lib.gpublas_dev_batch_dgemv.argtypes = [vp, ch, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.gpublas_dev_batch_dtrsm.argtypes = [vp, ch, ch, ch, ch, ci, ci, cd, vp, ci, vp, ci]
+    lib.gpublas_dev_tune_gemm.argtypes = [ctypes.c_char_p, ci]
+    lib.gpublas_dev_tune_choice.argtypes = [ci, ci, ci, ci]
     lib.gpublas_dev_prof_enable.argtypes = [ci]
     lib.gpublas_dev_prof_query.argtypes = [ctypes.c_char_p, ctypes.POINTER(cd),
                                            ctypes.POINTER(cd), ctypes.POINTER(cd)]
@@ -240,6 +254,75 @@ def kernel_time(name: str):
     ms = ctypes.c_double(0)
     n = library().gpublas_kernel_time(name.encode(), ctypes.byref(ms))
     return n, ms.value
+
+
+def sync() -> None:
+    """Wait for every kernel the calling thread launched."""
+    library().gpublas_sync()
+
+
+def device_ptr(a) -> int:
+    """Device address of a tracked array's element 0.  The device copy is
+    treated as the newer one from then on (a host access downloads it)."""
+    addr = library().gpublas_device_ptr(_p(a))
+    if not addr:
+        raise GpublasError("not a tracked array")
+    return addr
+
+
+class Batch:
+    """One batch slot of the serving loop's device layer (device pointers)."""
+
+    def __init__(self, use_graph=True):
+        self._lib = library()
+        self._b = self._lib.gpublas_dev_batch_create(1 if use_graph else 0)
+        if not self._b:
+            raise GpublasError("gpublas_dev_batch_create failed")
+
+    def force_cfg(self, cfg):
+        self._lib.gpublas_dev_batch_force_cfg(self._b, int(cfg))
+
+    def dgemm(self, ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc):
+        return self._lib.gpublas_dev_batch_dgemm(self._b, ta.encode(), tb.encode(), m, n, k, alpha,
+                                                 dA, lda, dB, ldb, beta, dC, ldc)
+
+    def sgemm(self, ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc):
+        return self._lib.gpublas_dev_batch_sgemm(self._b, ta.encode(), tb.encode(), m, n, k, alpha,
+                                                 dA, lda, dB, ldb, beta, dC, ldc)
+
+    def dgemv(self, tr, m, n, alpha, dA, lda, dx, incx, beta, dy, incy):
+        return self._lib.gpublas_dev_batch_dgemv(self._b, tr.encode(), m, n, alpha, dA, lda, dx,
+                                                 incx, beta, dy, incy)
+
+    def dtrsm(self, side, uplo, ta, diag, m, n, alpha, dA, lda, dB, ldb):
+        return self._lib.gpublas_dev_batch_dtrsm(self._b, side.encode(), uplo.encode(),
+                                                 ta.encode(), diag.encode(), m, n, alpha, dA, lda,
+                                                 dB, ldb)
+
+    def count(self):
+        return self._lib.gpublas_dev_batch_count(self._b)
+
+    def run(self):
+        err = self._lib.gpublas_dev_batch_launch(self._b)
+        if err:
+            raise GpublasError(f"batch launch failed ({err})")
+        err = self._lib.gpublas_dev_batch_wait(self._b)
+        if err:
+            raise GpublasError(f"batch wait failed ({err})")
+        self._lib.gpublas_dev_batch_clear(self._b)
+
+    def close(self):
+        if self._b:
+            self._lib.gpublas_dev_batch_destroy(self._b)
+            self._b = None
+
+
+def tune_gemm(path=None, force=False) -> int:
+    return library().gpublas_dev_tune_gemm(path.encode() if path else None, 1 if force else 0)
+
+
+def tune_choice(single, m, n, k) -> int:
+    return library().gpublas_dev_tune_choice(1 if single else 0, m, n, k)
 
 
 def tracked_empty(shape, dtype=np.float64, order="F"):

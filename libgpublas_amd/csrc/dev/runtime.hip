@@ -191,10 +191,12 @@ struct CtxHolder {
     DevCtx *c = nullptr;
     ~CtxHolder()
     {
-        // Thread exit: finish the thread's work and park the context for the
-        // next thread (its stream may still be named by tracked objects).
+        // Thread exit: park the context for the next thread.  No HIP call
+        // here (the runtime may already be tearing down this thread's
+        // state); work still queued on the stream stays ordered before
+        // whatever the next owner enqueues, and tracked objects that name
+        // the stream wait on its sequence numbers as before.
         if (c && g_init_status == 0) {
-            (void)hipStreamSynchronize(c->stream);
             std::lock_guard<std::mutex> g(g_ctx_mu);
             g_ctx_free->push_back(c);
         }
@@ -314,6 +316,22 @@ uint64_t gpublas_dev_stream_mark(void *stream)
 {
     DevCtx *c = ctx_of(pick_stream(stream));
     return c ? __atomic_add_fetch(&c->submitted, 1, __ATOMIC_ACQ_REL) : 0;
+}
+
+int gpublas_dev_stream_done(void *stream, uint64_t seq)
+{
+    hipStream_t s = pick_stream(stream);
+    DevCtx *c = ctx_of(s);
+    if (c && __atomic_load_n(&c->completed, __ATOMIC_ACQUIRE) >= seq) return 1;
+    if (hipStreamQuery(s) != hipSuccess) return 0;
+    if (c) {
+        // the stream is idle: everything marked before this query is complete
+        uint64_t cur = __atomic_load_n(&c->completed, __ATOMIC_RELAXED);
+        while (cur < seq && !__atomic_compare_exchange_n(&c->completed, &cur, seq, false,
+                                                         __ATOMIC_RELEASE, __ATOMIC_RELAXED)) {
+        }
+    }
+    return 1;
 }
 
 int gpublas_dev_stream_wait(void *stream, uint64_t seq)

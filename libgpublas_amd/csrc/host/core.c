@@ -24,6 +24,10 @@ struct gb_options gb_opt = {
     .size_threshold = 1u << 20,
     .min_track = 4096,
     .cutoff = 512,
+    .serve_budget = (size_t)64 << 30,
+    .serve_slots = 4,
+    .serve_graph = 1,
+    .serve_batch = 256,
 };
 
 static int g_initialized = 0;
@@ -78,6 +82,14 @@ static void print_help(void)
            "  profile              time every kernel with HIP events\n"
            "  split=<s>            DGEMM/DSYRK as split-precision BF16 slice products\n"
            "                       (s = 1..8 slices per operand; 0 = plain FP64, default)\n"
+           "  serve[=<bytes>]      run DGEMM/SGEMM/DGEMV/left DTRSM calls whose operands are\n"
+           "                       all tracked through the batching serving loop, with an\n"
+           "                       HBM admission budget (64G); serve=0 turns it off\n"
+           "  serve_slots=<n>      batches in flight (4)\n"
+           "  serve_graph=<0|1>    replay captured hipGraphs (1) or launch eagerly (0)\n"
+           "  serve_batch=<n>      most calls per batch (256; 1 = one launch per call)\n"
+           "  tune=<file>          tile-shape table of the batch GEMM kernels: loaded\n"
+           "                       from <file>, measured and written there if absent\n"
            "  stats[=<file>]       write hits/misses at exit (statistics.csv)\n"
            "  trace=<file>         log allocations (T/U) and BLAS operand uses (C)\n"
            "  verbose              log decisions to stderr\n");
@@ -112,6 +124,31 @@ int gb_parse_options(const char *s)
                 gb_opt.split = 0;
                 bad = 1;
             }
+        }
+        else if (!strcmp(opt, "serve")) gb_opt.serve = 1;
+        else if (!strncmp(opt, "serve=", 6)) {
+            const size_t b = parse_size(opt + 6);
+            gb_opt.serve = b != 0;
+            if (b) gb_opt.serve_budget = b;
+        }
+        else if (!strncmp(opt, "serve_slots=", 12)) {
+            const int v = atoi(opt + 12);
+            if (v >= 1 && v <= 16) gb_opt.serve_slots = v;
+            else bad = 1;
+            gb_serve_stop();
+        }
+        else if (!strncmp(opt, "serve_graph=", 12)) {
+            gb_opt.serve_graph = atoi(opt + 12) != 0;
+            gb_serve_stop();
+        }
+        else if (!strncmp(opt, "serve_batch=", 12)) {
+            const int v = atoi(opt + 12);
+            if (v >= 1) gb_opt.serve_batch = v;
+            else bad = 1;
+        }
+        else if (!strncmp(opt, "tune=", 5)) {
+            snprintf(gb_opt.tune_file, sizeof gb_opt.tune_file, "%s", opt + 5);
+            gb_serve_stop();
         }
         else if (!strcmp(opt, "verbose")) gb_opt.verbose = 1;
         else if (!strcmp(opt, "stats")) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "statistics.csv");
@@ -167,6 +204,7 @@ __attribute__((destructor)) static void gb_fini(void)
 {
     gb_enter();
     gb_tracking = 0;
+    gb_serve_stop();
     if (gb_opt.stats_file[0]) {
         int fd = open(gb_opt.stats_file, O_WRONLY | O_CREAT | O_TRUNC, 0644);
         if (fd >= 0) {
@@ -308,6 +346,14 @@ void gpublas_get_stats(gpublas_stats_t *out)
     *out = gb_stats;
     out->tracked_objects = (uint64_t)gb_tracked_count();
     out->allocs_seen = gb_nallocs;
+    out->resident_bytes = __atomic_load_n(&gb_resident_bytes, __ATOMIC_RELAXED);
+}
+
+void gpublas_sync(void)
+{
+    gb_enter();
+    if (gpublas_dev_init() == 0) gb_check(gpublas_dev_sync(NULL), "thread synchronisation");
+    gb_leave();
 }
 
 void gpublas_reset_stats(void)

@@ -34,6 +34,12 @@ struct gb_options {
     int cutoff;              /* gemm fall-through cutoff on leading dims */
     int profile;             /* kernel event timing */
     int split;               /* >0: DGEMM/DSYRK as split-precision with this many slices */
+    int serve;               /* route calls on tracked operands through the serving loop */
+    size_t serve_budget;     /* HBM admission budget of the serving loop (bytes) */
+    int serve_slots;         /* batches in flight */
+    int serve_graph;         /* replay captured graphs (else launch eagerly) */
+    int serve_batch;         /* most requests per batch */
+    char tune_file[256];     /* GEMM tile-shape table of the batch kernels */
     int verbose;
     char stats_file[256];    /* "" = no statistics file at exit */
     char trace_file[256];    /* "" = no allocation / call trace */
@@ -57,6 +63,7 @@ void gb_require_device(void);
 
 /* ---- tracked objects (tracker.c) ---- */
 enum gb_state { ST_HOST = 0, ST_SHARED = 1, ST_DEVICE = 2 };
+#define GB_RD_SLOTS 32 /* reader streams remembered per object */
 struct gb_obj {
     char *host;          /* application view (page aligned) */
     char *alias;         /* library view of the same pages, always RW */
@@ -68,8 +75,13 @@ struct gb_obj {
     void *callsite;
     int state;
     int used;            /* seen as a BLAS operand */
-    void *use_stream;    /* stream of the last kernel that used the mirror */
-    uint64_t use_seq;    /* its sequence number on that stream */
+    void *wr_stream;     /* stream of the last kernel that wrote the mirror */
+    uint64_t wr_seq;     /* its sequence number on that stream */
+    void *rd_stream[GB_RD_SLOTS];  /* kernels reading the mirror since, one per stream */
+    uint64_t rd_seq[GB_RD_SLOTS];
+    int nrd;
+    int refs;            /* calls between acquire and finish, served requests in flight */
+    uint64_t last_use;   /* acquire tick, for least-recently-used eviction */
     pthread_mutex_t mu;
 };
 void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero);
@@ -80,10 +92,21 @@ size_t gb_obj_usable_size(const void *p);    /* 0 if untracked */
 int gb_tracked_count(void);
 void gb_install_fault_handler(void);
 /* Make the device copy valid (upload if the host copy is newer); returns 1 if
- * it uploaded.  Orders `stream` after the object's last use elsewhere. */
-int gb_obj_to_device(struct gb_obj *o, void *stream);
+ * it uploaded.  Takes a reference (released by gb_finish / gb_obj_unref).
+ * Orders `stream` after conflicting kernels on other streams: the last
+ * writer, and for a write (or an upload) every recorded reader too. */
+int gb_obj_to_device(struct gb_obj *o, void *stream, int write);
+void gb_obj_unref(struct gb_obj *o);
 /* Wait until no kernel uses the object's device mirror any more. */
 void gb_obj_quiesce(struct gb_obj *o);
+/* Wait for the kernels a use of the object must follow (host-side). */
+void gb_obj_wait_hazards(struct gb_obj *o, int write);
+/* HBM held by device mirrors of tracked objects. */
+extern size_t gb_resident_bytes;
+/* Drop the device mirrors of idle objects whose host copy is current (least
+ * recently used first) until `want` bytes are freed; objects in keep[] are
+ * spared.  Returns the bytes freed. */
+size_t gb_evict_idle(size_t want, struct gb_obj *const *keep, int nkeep);
 /* Mark the device copy as the only valid one (after a kernel wrote it). */
 void gb_obj_device_written(struct gb_obj *o);
 
@@ -125,6 +148,19 @@ void gb_finish(void *stream, gb_operand *ops, int nops);
  * in the result slot, then release staging buffers. */
 void gb_finish_scalar(void *stream, gb_operand *ops, int nops, uint64_t seq);
 bool gb_any_tracked(const gb_operand *ops, int nops);
+
+/* ---- serving loop (serve.c) ---- */
+/* Each returns 1 if the call was served (all operands tracked, shape
+ * supported), 0 if the caller must take the per-call path. */
+int gb_serve_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *a, int lda,
+                   const double *b, int ldb, double beta, double *c, int ldc);
+int gb_serve_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float *a, int lda,
+                   const float *b, int ldb, float beta, float *c, int ldc);
+int gb_serve_dgemv(char trans, int m, int n, double alpha, const double *a, int lda,
+                   const double *x, int incx, double beta, double *y, int incy);
+int gb_serve_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                   const double *a, int lda, double *b, int ldb);
+void gb_serve_stop(void);
 bool gb_ptr_tracked(const void *p);
 
 void gb_check(int err, const char *what);

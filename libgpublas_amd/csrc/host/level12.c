@@ -194,6 +194,12 @@ void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, 
         gb_leave();
         return;
     }
+    if (*alpha != 0.0 && gb_opt.serve &&
+        gb_serve_dgemv(*trans, *m, *n, *alpha, a, *lda, x, *incx, *beta, y, *incy)) {
+        GB_COUNT(device_calls, 1);
+        gb_leave();
+        return;
+    }
     gb_require_device();
     void *s = gpublas_dev_stream();
     gb_operand ops[3];

@@ -45,7 +45,7 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
 /* ------------------------------------------------------------------ */
 /* GEMM                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_GEMM(PRE, T, NAME6, DEVFN)                                                        \
+#define DEFINE_GEMM(PRE, T, NAME6, DEVFN, SERVEFN)                                                        \
     void PRE##gemm_(const char *transa, const char *transb, const int *m, const int *n,          \
                     const int *k, const T *alpha, const T *a, const int *lda, const T *b,        \
                     const int *ldb, const T *beta, T *c, const int *ldc)                          \
@@ -92,6 +92,12 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
             gb_leave();                                                                          \
             return;                                                                              \
         }                                                                                        \
+        if (!scale_only && gb_opt.serve &&                                                       \
+            SERVEFN(*transa, *transb, *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc)) {    \
+            GB_COUNT(device_calls, 1);                                                           \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
         gb_require_device();                                                                     \
         void *s = gpublas_dev_stream();                                                          \
         gb_operand ops[3] = {{0}};                                                               \
@@ -109,8 +115,8 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
         gb_leave();                                                                              \
     }
 
-DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev)
-DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm)
+DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev, gb_serve_dgemm)
+DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm)
 
 /* ------------------------------------------------------------------ */
 /* SYRK                                                                */
@@ -172,7 +178,7 @@ DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
 /* ------------------------------------------------------------------ */
 /* TRSM                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_TRSM(PRE, T, NAME6, DEVFN)                                                        \
+#define DEFINE_TRSM(PRE, T, NAME6, DEVFN, SERVEFN)                                                        \
     void PRE##trsm_(const char *side, const char *uplo, const char *transa, const char *diag,    \
                     const int *m, const int *n, const T *alpha, const T *a, const int *lda,      \
                     T *b, const int *ldb)                                                        \
@@ -207,6 +213,12 @@ DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
             gb_leave();                                                                          \
             return;                                                                              \
         }                                                                                        \
+        if (*alpha != 0 && gb_opt.serve &&                                                       \
+            SERVEFN(*side, *uplo, *transa, *diag, *m, *n, *alpha, a, *lda, b, *ldb)) {            \
+            GB_COUNT(device_calls, 1);                                                           \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
         gb_require_device();                                                                     \
         void *s = gpublas_dev_stream();                                                          \
         gb_operand ops[2] = {{0}};                                                               \
@@ -222,8 +234,15 @@ DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
         gb_leave();                                                                              \
     }
 
-DEFINE_TRSM(d, double, "DTRSM ", gpublas_dev_dtrsm)
-DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm)
+/* single-precision TRSM has no batch kernel: always the per-call path */
+static int no_serve_strsm(char side, char uplo, char transa, char diag, int m, int n, float alpha,
+                          const float *a, int lda, float *b, int ldb)
+{
+    return 0;
+}
+
+DEFINE_TRSM(d, double, "DTRSM ", gpublas_dev_dtrsm, gb_serve_dtrsm)
+DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm)
 
 /* ------------------------------------------------------------------ */
 /* SYMM                                                                */

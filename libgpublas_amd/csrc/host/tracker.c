@@ -169,10 +169,104 @@ void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero)
     return host;
 }
 
-void gb_obj_quiesce(struct gb_obj *o)
+size_t gb_resident_bytes = 0;
+static uint64_t g_tick = 0;
+
+static void wait_use(void *stream, uint64_t seq)
 {
-    if (o->use_stream)
-        gb_check(gpublas_dev_stream_wait(o->use_stream, o->use_seq), "wait for a tracked object");
+    gb_check(gpublas_dev_stream_wait(stream, seq), "wait for a tracked object");
+}
+
+/* Caller holds o->mu.  Wait (host-side) for the kernels on streams other
+ * than `stream` that a use must follow: the last writer, and for a write
+ * every recorded reader.  Entries found complete are dropped. */
+static void wait_hazards_locked(struct gb_obj *o, void *stream, int write)
+{
+    if (o->wr_stream && o->wr_stream != stream) {
+        wait_use(o->wr_stream, o->wr_seq);
+        o->wr_stream = NULL;
+    }
+    if (write) {
+        int k = 0;
+        for (int i = 0; i < o->nrd; ++i) {
+            if (o->rd_stream[i] != stream) {
+                wait_use(o->rd_stream[i], o->rd_seq[i]);
+                continue;
+            }
+            o->rd_stream[k] = o->rd_stream[i];
+            o->rd_seq[k++] = o->rd_seq[i];
+        }
+        o->nrd = k;
+    }
+}
+
+void gb_obj_quiesce(struct gb_obj *o) { wait_hazards_locked(o, NULL, 1); }
+
+void gb_obj_wait_hazards(struct gb_obj *o, int write)
+{
+    pthread_mutex_lock(&o->mu);
+    wait_hazards_locked(o, NULL, write);
+    pthread_mutex_unlock(&o->mu);
+}
+
+void gb_obj_unref(struct gb_obj *o)
+{
+    pthread_mutex_lock(&o->mu);
+    o->refs--;
+    pthread_mutex_unlock(&o->mu);
+}
+
+static int idle_locked(struct gb_obj *o)
+{
+    if (o->refs || !o->dev || o->state != ST_SHARED) return 0;
+    if (o->wr_stream && !gpublas_dev_stream_done(o->wr_stream, o->wr_seq)) return 0;
+    for (int i = 0; i < o->nrd; ++i)
+        if (!gpublas_dev_stream_done(o->rd_stream[i], o->rd_seq[i])) return 0;
+    return 1;
+}
+
+static void drop_mirror_locked(struct gb_obj *o)
+{
+    gpublas_dev_release(o->dev);
+    o->dev = NULL;
+    o->wr_stream = NULL;
+    o->nrd = 0;
+    __atomic_fetch_sub(&gb_resident_bytes, o->map_size, __ATOMIC_RELAXED);
+    /* the host copy is now the only one: let the application write it */
+    mprotect(o->host, o->map_size, PROT_READ | PROT_WRITE);
+    o->state = ST_HOST;
+}
+
+size_t gb_evict_idle(size_t want, struct gb_obj *const *keep, int nkeep)
+{
+    size_t freed = 0;
+    while (freed < want) {
+        struct gb_obj *best = NULL;
+        pthread_rwlock_rdlock(&g_tab_lock);
+        for (size_t i = 0; i < g_tab_n; ++i) {
+            struct gb_obj *o = g_tab[i];
+            int kept = 0;
+            for (int j = 0; j < nkeep; ++j) kept |= keep[j] == o;
+            if (kept || !o->dev || o->refs || o->state != ST_SHARED) continue;
+            if (!best || o->last_use < best->last_use) best = o;
+        }
+        int done = 0;
+        if (best) {
+            pthread_mutex_lock(&best->mu);
+            if (idle_locked(best)) {
+                freed += best->map_size;
+                drop_mirror_locked(best);
+                GB_COUNT(evictions, 1);
+            } else {
+                /* busy after all: stop rather than spin on it */
+                done = 1;
+            }
+            pthread_mutex_unlock(&best->mu);
+        }
+        pthread_rwlock_unlock(&g_tab_lock);
+        if (!best || done) break;
+    }
+    return freed;
 }
 
 static void obj_free(struct gb_obj *o)
@@ -181,7 +275,10 @@ static void obj_free(struct gb_obj *o)
     gb_obj_quiesce(o); /* a kernel launched by a call that returned early may still use it */
     munmap(o->host, o->map_size);
     munmap(o->alias, o->map_size);
-    if (o->dev) gpublas_dev_release(o->dev);
+    if (o->dev) {
+        gpublas_dev_release(o->dev);
+        __atomic_fetch_sub(&gb_resident_bytes, o->map_size, __ATOMIC_RELAXED);
+    }
     __atomic_fetch_sub(&gb_stats.tracked_bytes, (uint64_t)o->size, __ATOMIC_RELAXED);
     pthread_mutex_destroy(&o->mu);
     __libc_free(o);
@@ -231,16 +328,20 @@ void *gb_obj_realloc(void *p, size_t size)
 }
 
 /* ---- coherence ---- */
-int gb_obj_to_device(struct gb_obj *o, void *stream)
+int gb_obj_to_device(struct gb_obj *o, void *stream, int write)
 {
     int uploaded = 0;
     pthread_mutex_lock(&o->mu);
+    o->refs++;
+    o->last_use = __atomic_add_fetch(&g_tick, 1, __ATOMIC_RELAXED);
     if (!o->dev) {
         o->dev = gpublas_dev_alloc(o->map_size);
         if (!o->dev) gb_fatal("HBM exhausted allocating a %zu-byte tracked object", o->map_size);
+        __atomic_fetch_add(&gb_resident_bytes, o->map_size, __ATOMIC_RELAXED);
     }
-    /* work queued on another stream may still read or write the mirror */
-    if (o->use_stream && o->use_stream != stream) gb_obj_quiesce(o);
+    /* kernels queued on other streams may still use the mirror: an upload
+     * or a write must follow all of them, a read only the last writer */
+    wait_hazards_locked(o, stream, write || o->state == ST_HOST);
     if (o->state == ST_HOST) {
         uploaded = 1;
         /* protect first: a host write racing the upload faults and demotes */
@@ -347,7 +448,7 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
     const size_t span = ((cols - 1) * ld + rows) * es;
     struct gb_obj *o = gb_obj_find(host);
     if (o && (const char *)host + span <= o->host + o->size) {
-        op->uploaded = gb_obj_to_device(o, stream);
+        op->uploaded = gb_obj_to_device(o, stream, (mode & GB_OUT) != 0);
         op->obj = o;
         op->dev = o->dev + ((const char *)host - o->host);
         gb_trace_call("operand", host);
@@ -399,6 +500,8 @@ void gb_release(gb_operand *op, void *stream)
     }
 }
 
+/* Record the call's kernels as the objects' latest writer / readers and
+ * drop the references taken at acquire. */
 static void mark_uses(void *stream, gb_operand *ops, int nops)
 {
     const uint64_t seq = gpublas_dev_stream_mark(stream);
@@ -406,8 +509,30 @@ static void mark_uses(void *stream, gb_operand *ops, int nops)
         struct gb_obj *o = ops[i].obj;
         if (!o) continue;
         pthread_mutex_lock(&o->mu);
-        o->use_stream = stream;
-        o->use_seq = seq;
+        if (ops[i].mode & GB_OUT) {
+            /* readers on other streams were waited for at acquire */
+            o->wr_stream = stream;
+            o->wr_seq = seq;
+            o->nrd = 0;
+        } else {
+            int j = 0;
+            while (j < o->nrd && o->rd_stream[j] != stream) ++j;
+            if (j == o->nrd) {
+                if (o->nrd == GB_RD_SLOTS) {
+                    /* table full: retire the oldest reader */
+                    wait_use(o->rd_stream[0], o->rd_seq[0]);
+                    for (int q = 1; q < o->nrd; ++q) {
+                        o->rd_stream[q - 1] = o->rd_stream[q];
+                        o->rd_seq[q - 1] = o->rd_seq[q];
+                    }
+                    j = --o->nrd;
+                }
+                o->rd_stream[j] = stream;
+                o->nrd++;
+            }
+            o->rd_seq[j] = seq;
+        }
+        o->refs--;
         pthread_mutex_unlock(&o->mu);
     }
 }
@@ -468,8 +593,9 @@ void *gpublas_device_ptr(const void *p)
     gb_enter();
     gb_require_device();
     void *s = gpublas_dev_stream();
-    gb_obj_to_device(o, s);
+    gb_obj_to_device(o, s, 1);
     gb_check(gpublas_dev_sync(s), "upload");
+    gb_obj_unref(o);
     /* the caller may write through the device address: treat the device
      * copy as the newer one from now on */
     gb_obj_device_written(o);

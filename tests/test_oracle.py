@@ -74,9 +74,10 @@ def test_dgemm_vs_host_blas(L, ta, tb, m, n, k):
     C0 = uniform(3, (m, n))
     R = C0.copy(order="F")
     L.ora_dgemm(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B), B.shape[0], 1.3, p(R), m)
-    H = sb.dgemm(0.7, A, B, 1.3, C0.copy(order="F"), trans_a=ta == "T", trans_b=tb == "T")
+    H = np.asfortranarray(sb.dgemm(0.7, A, B, 1.3, C0.copy(order="F"), trans_a=ta == "T",
+                                   trans_b=tb == "T"))
     ratio = L.ora_gemm_gauge_ratio(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B),
-                                   B.shape[0], 1.3, p(C0), p(np.asfortranarray(H)), m, EPS)
+                                   B.shape[0], 1.3, p(C0), p(H), m, EPS)
     assert ratio < 16
     ratio = L.ora_gemm_gauge_ratio(c(ta), c(tb), m, n, k, 0.7, p(A), A.shape[0], p(B),
                                    B.shape[0], 1.3, p(C0), p(R), m, EPS)
