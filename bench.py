@@ -270,9 +270,9 @@ def serve_run(mode, rate, calls, threads=16, options="serve=64G"):
 
 def serve_suite(calls=20000, rates=(0,)):
     """Serving trace: 16 threads, Poisson arrivals (rate 0 = closed loop,
-    i.e. saturation), one-launch-per-call vs the batching loop (graph replay,
-    eager launches, and batches of one)."""
-    variants = [("plain", ""), ("serve", "serve=64G"), ("serve", "serve=64G;serve_graph=0"),
+    i.e. saturation), one-launch-per-call vs the batching loop (eager
+    launches, the default; graph replay; batches of one)."""
+    variants = [("plain", ""), ("serve", "serve=64G"), ("serve", "serve=64G;serve_graph=1"),
                 ("serve", "serve=64G;serve_batch=1")]
     for rate in rates:
         base = None

@@ -135,6 +135,9 @@ typedef struct {
     uint64_t deferred;       /* admissions postponed until in-flight batches ended */
     uint64_t over_budget;    /* admissions past the budget (nothing left to wait for) */
     uint64_t resident_bytes; /* HBM held by device mirrors now (not reset) */
+    uint64_t serve_wait_ns;  /* served calls: summed submission -> launch time */
+    uint64_t serve_run_ns;   /* served calls: summed launch -> completion seen */
+    uint64_t serve_launch_ns;/* engine time spent forming and launching batches */
 } gpublas_stats_t;
 void gpublas_get_stats(gpublas_stats_t *out);
 void gpublas_reset_stats(void);

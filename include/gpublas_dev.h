@@ -119,8 +119,9 @@ void *gpublas_dev_result_slot(void);
 
 /* ---- Batch slots of the serving loop (no counterpart in the reference,
  * which issues one cuBLAS call per intercepted call: blas_level3/gemm.cc:62-82,
- * trsm.cc:52-72).  A slot collects up to 64 problems of each kind, of any
- * sizes, and runs them with one grouped persistent launch per kind -- one
+ * trsm.cc:52-72).  A slot collects up to 24 GEMMs per tile shape and
+ * precision, 32 DGEMVs and 48 DTRSMs (m <= 1024), of any sizes, and runs
+ * them with one grouped persistent launch per kind -- one
  * replay of a captured hipGraph (use_graph != 0) or the same kernels
  * launched eagerly.  Operands are device pointers.  add_*: 0 = added,
  * 1 = slot full (launch it and use another), -1 = not supported in a batch
@@ -149,6 +150,10 @@ int gpublas_dev_batch_dtrsm(gpublas_batch *b, char side, char uplo, char transa,
                             int n, double alpha, const double *d_A, int lda, double *d_B, int ldb);
 int gpublas_dev_batch_launch(gpublas_batch *b);   /* asynchronous */
 int gpublas_dev_batch_done(gpublas_batch *b);     /* 1 once the launched batch has completed */
+/* 1 once the batch's problems of one kind (0 DGEMM, 1 SGEMM, 2 DGEMV,
+ * 3 DTRSM) have completed; eager launches run the kinds in increasing
+ * estimated cost, so cheap kinds finish first (graph replays: whole batch) */
+int gpublas_dev_batch_done_kind(gpublas_batch *b, int kind);
 int gpublas_dev_batch_wait(gpublas_batch *b);
 /* GEMM tile-shape table of the batch kernels: load from `path` (unless
  * force), else measure every tile shape per (sqrt(mn), k) size bucket and

@@ -49,6 +49,7 @@ typedef struct {
     uint64_t calls, device_calls, fallthrough, hits, misses, h2d_bytes, d2h_bytes;
     uint64_t tracked_objects, tracked_bytes, faults, uploads, allocs_seen, async_returns;
     uint64_t evictions, served, batches, max_batch, deferred, over_budget, resident_bytes;
+    uint64_t serve_wait_ns, serve_run_ns, serve_launch_ns;
 } stats_t;
 
 static gemm_t p_dgemm, p_sgemm;
@@ -321,6 +322,10 @@ int main(int argc, char **argv)
         printf(", \"%s\": {\"n\": %d, \"p50_us\": %.1f, \"p99_us\": %.1f}", op_name[o], nop[o],
                1e6 * pct(byop[o], nop[o], 0.5), 1e6 * pct(byop[o], nop[o], 0.99));
     }
+    const double ns = st.served ? (double)st.served : 1.0;
+    printf(", \"engine_wait_us\": %.1f, \"engine_run_us\": %.1f, \"engine_launch_us_per_batch\": %.1f",
+           st.serve_wait_ns / ns / 1e3, st.serve_run_ns / ns / 1e3,
+           st.batches ? st.serve_launch_ns / (double)st.batches / 1e3 : 0.0);
     printf(", \"served\": %llu, \"batches\": %llu, \"mean_batch\": %.2f, \"max_batch\": %llu, "
            "\"deferred\": %llu, \"evictions\": %llu, \"resident_bytes\": %llu, \"setup_s\": %.2f}\n",
            (unsigned long long)st.served, (unsigned long long)st.batches,

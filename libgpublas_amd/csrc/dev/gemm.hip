@@ -46,11 +46,6 @@ template <typename T, int BM, int BN, int BK, int PA, int PB, int TRI>
 __global__ __launch_bounds__(256, 2) void gemm_fast_kernel(GemmArgs<T> g)
 {
     constexpr int PAD = 128 / (int)sizeof(T);
-    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
-    typedef TileLoader<T, BM, BK> LA;
-    typedef TileLoader<T, BN, BK> LB;
-    typedef typename Mfma<T>::Acc Acc;
-
     __shared__ __attribute__((aligned(16))) T As[2][BK][BM + PAD];
     __shared__ __attribute__((aligned(16))) T Bs[2][BK][BN + PAD];
 
@@ -63,81 +58,7 @@ __global__ __launch_bounds__(256, 2) void gemm_fast_kernel(GemmArgs<T> g)
         tri_decode(blockIdx.x, hi, lo);
         if (TRI == SYM_LOWER) { tm = hi; tn = lo; } else { tm = lo; tn = hi; }
     }
-    const int m0 = tm * BM, n0 = tn * BN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int K = g.k;
-    const int ktiles = (K + BK - 1) / BK;
-
-    Acc acc[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = Acc{};
-
-    typename LA::Vec sa[LA::NV];
-    typename LB::Vec sb[LB::NV];
-    auto load = [&](int k0) -> bool {
-        if (k0 + BK <= K) {
-            LA::template load_pat<PA>(g.a, m0, k0, sa);
-            LB::template load_pat<PB>(g.b, n0, k0, sb);
-            return false;
-        }
-        LA::load(g.a, m0, k0, K, PAT_ELEM, sa);
-        LB::load(g.b, n0, k0, K, PAT_ELEM, sb);
-        return true;
-    };
-    auto store = [&](int buf, bool tail) {
-        if (!tail) {
-            LA::template store_pat<PA, BM + PAD>(As[buf], sa);
-            LB::template store_pat<PB, BN + PAD>(Bs[buf], sb);
-        } else {
-            LA::template store_pat<PAT_OUTER, BM + PAD>(As[buf], sa);
-            LB::template store_pat<PAT_OUTER, BN + PAD>(Bs[buf], sb);
-        }
-    };
-
-    store(0, load(0));
-    __syncthreads();
-    for (int kt = 0; kt < ktiles; ++kt) {
-        const int cur = kt & 1;
-        const bool more = kt + 1 < ktiles;
-        bool tail = false;
-        if (more) tail = load((kt + 1) * BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            T fa[FM], fb[FN];
-            const int kr = kk + (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm * WM + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn * WN + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(fb[j], fa[i], acc[i][j]);
-        }
-        if (more) store(cur ^ 1, tail);
-        __syncthreads();
-    }
-
-    const T alpha = g.alpha, beta = g.beta;
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-        const long gi = m0 + wm * WM + i * 16 + (lane & 15);
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const long gj = n0 + wn * WN + j * 16 + Mfma<T>::drow(lane, r);
-                if (TRI == SYM_LOWER && gi < gj) continue;
-                if (TRI == SYM_UPPER && gi > gj) continue;
-                T *c = g.C + gi + gj * g.ldc;
-                const T v = alpha * acc[i][j][r];
-                *c = beta == T(0) ? v : v + beta * *c;
-            }
-        }
-    }
+    gemm_fast_tile<T, BM, BN, BK, PA, PB, TRI>(g, tm, tn, As, Bs);
 }
 
 // Scale (or zero) a triangle or rectangle of C: the alpha == 0 / k == 0

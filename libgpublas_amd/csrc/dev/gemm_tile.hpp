@@ -296,6 +296,99 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T> &g, int tm, int tn,
     }
 }
 
+// Full C tile (tm, tn) of g with both operands 16-B aligned, leading
+// dimensions a multiple of the vector width and access patterns PA / PB
+// fixed at compile time: no per-element selects in the main loop (only a
+// partial last k-tile takes guarded element loads).  Same k order and
+// epilogue as gemm_tile, hence the same bits.
+template <typename T, int BM, int BN, int BK, int PA, int PB, int TRI>
+__device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int tn,
+                                               T (*As)[BK][BM + 128 / sizeof(T)],
+                                               T (*Bs)[BK][BN + 128 / sizeof(T)])
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    typedef TileLoader<T, BM, BK> LA;
+    typedef TileLoader<T, BN, BK> LB;
+    typedef typename Mfma<T>::Acc Acc;
+
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int K = g.k;
+    const int ktiles = (K + BK - 1) / BK;
+
+    Acc acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Acc{};
+
+    typename LA::Vec sa[LA::NV];
+    typename LB::Vec sb[LB::NV];
+    auto load = [&](int k0) -> bool {
+        if (k0 + BK <= K) {
+            LA::template load_pat<PA>(g.a, m0, k0, sa);
+            LB::template load_pat<PB>(g.b, n0, k0, sb);
+            return false;
+        }
+        LA::load(g.a, m0, k0, K, PAT_ELEM, sa);
+        LB::load(g.b, n0, k0, K, PAT_ELEM, sb);
+        return true;
+    };
+    auto store = [&](int buf, bool tail) {
+        if (!tail) {
+            LA::template store_pat<PA, BM + PAD>(As[buf], sa);
+            LB::template store_pat<PB, BN + PAD>(Bs[buf], sb);
+        } else {
+            LA::template store_pat<PAT_OUTER, BM + PAD>(As[buf], sa);
+            LB::template store_pat<PAT_OUTER, BN + PAD>(Bs[buf], sb);
+        }
+    };
+
+    store(0, load(0));
+    __syncthreads();
+    for (int kt = 0; kt < ktiles; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < ktiles;
+        bool tail = false;
+        if (more) tail = load((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T fa[FM], fb[FN];
+            const int kr = kk + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::mma(fb[j], fa[i], acc[i][j]);
+        }
+        if (more) store(cur ^ 1, tail);
+        __syncthreads();
+    }
+
+    const T alpha = g.alpha, beta = g.beta;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const long gi = m0 + wm * WM + i * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long gj = n0 + wn * WN + j * 16 + Mfma<T>::drow(lane, r);
+                if (TRI == SYM_LOWER && gi < gj) continue;
+                if (TRI == SYM_UPPER && gi > gj) continue;
+                T *c = g.C + gi + gj * g.ldc;
+                const T v = alpha * acc[i][j][r];
+                *c = beta == T(0) ? v : v + beta * *c;
+            }
+        }
+    }
+}
+
 inline bool is_n(char c) { return c == 'N' || c == 'n'; }
 inline bool is_l(char c) { return c == 'L' || c == 'l'; }
 inline bool is_u(char c) { return c == 'U' || c == 'u'; }

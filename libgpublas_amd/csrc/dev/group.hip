@@ -1,6 +1,6 @@
 // group.hip -- grouped persistent launches for the serving loop: one batch
-// slot holds up to kCap problems of each kind (DGEMM, SGEMM, DGEMV, DTRSM)
-// of arbitrary, uneven sizes and runs them all with one replay of a
+// slot holds up to 24 GEMMs per tile shape and precision, 32 DGEMVs and 48
+// DTRSMs of arbitrary, uneven sizes and runs them all with one replay of a
 // captured hipGraph (or the same kernels launched eagerly).
 //
 // No counterpart in the reference: blas2cuda issues one cuBLAS call per
@@ -8,10 +8,13 @@
 // global handle.  Here the host engine collects calls from many threads and
 // hands them over in batches.
 //
-// Layout of one batch in HBM: a BatchTable (problem descriptors and
-// per-group prefix sums of work units) copied from pinned host memory by the
-// first graph node, a GEMV partial-sum workspace, and the operands
-// themselves (device-resident tracked objects, addressed in place).
+// Problem descriptors travel per kind in a group struct (at most 4 KB):
+// eager launches pass each group by value as the kernel argument; a graph
+// replay reads the same structs from a device copy of the batch table,
+// refreshed by the graph's first node.  Besides the operands themselves
+// (device-resident tracked objects, addressed in place) a batch uses a
+// GEMV partial-sum workspace in HBM.  Completion is an event recorded after
+// the launch.
 //
 // Kernels (all persistent: a fixed grid loops over the batch's work units,
 // so one captured graph serves every batch of the slot):
@@ -26,8 +29,6 @@
 //    workgroup per 16-column panel of B held in LDS, right-looking over
 //    16-row blocks (in-register substitution of the diagonal block, MFMA
 //    updates of the blocks below);
-//  * publish_kernel: stores the batch sequence number into a pinned host
-//    word that the host polls.
 // Roofline: GEMM tiles are MFMA-bound (2mnk flop), GEMV is HBM-bound
 // (8(mn + m + n) bytes), TRSM panels are bound by their sequential
 // substitution chain for small n.
@@ -41,14 +42,27 @@
 
 namespace {
 
-constexpr int kCap = 64;  // problems of each kind per batch
 enum { CFG_32 = 0, CFG_64 = 1, CFG_128 = 2, kCfgs = 3 };
 enum { TY_D = 0, TY_S = 1 };
+// problems per kind per batch, sized so each group struct stays a legal
+// (<= 4 KB) kernel argument
+constexpr int kCapGemm = 22, kCapGemv = 32, kCapTrsm = 48;
 
+// One GEMM of a group.  Its tiles split into interior tiles (g.full_m x
+// g.full_n, only when both operands allow the vector loads), run by the
+// fast kernel with the trans pattern `pat`, and edge tiles (the rest), run
+// by the generic kernel.  A product with few tiles and a long k is also
+// split along k into `ksplit` chunks of `kchunk` (a multiple of 16): chunk
+// s computes its tiles into part + s*m*n (alpha 1, beta 0) and the fold
+// kernel sums the chunks in order and applies alpha / beta.  The split
+// depends only on (m, n, k).
 template <typename T>
 struct GGemm {
     GemmArgs<T> g;
-    int tile_begin;
+    T *part;
+    int fast_begin, edge_begin, fold_begin;
+    int pat;  // 0 NN, 1 NT, 2 TN, 3 TT
+    int ksplit, kchunk;
 };
 
 struct GGemv {
@@ -59,10 +73,12 @@ struct GGemv {
     long lda;
     int incx, incy;
     int m, n, trans;
-    int split;  // N: column blocks; T: row chunks
-    int span;   // N: columns per block; T: rows per chunk
+    int split;   // N: column blocks; T: row chunks
+    int span;    // N: columns per block; T: rows per chunk
     int vec_ok;  // A 16-B aligned, lda even
     int x_vec;   // x 16-B aligned, incx == 1
+    int direct;  // split == 1: the partial kernel writes y itself
+    long x0, y0; // offsets of logical element 0 (negative strides start at the far end)
     double alpha, beta;
     int unit_begin, fold_begin;
 };
@@ -76,63 +92,32 @@ struct GTrsm {
     int trans;  // op(A) = A^T
     int unit;
     double alpha;
-    int unit_begin;  // waves
+    int unit_begin;  // panels
 };
+
+template <typename T>
+struct GemmGroup {
+    int np, fast_tiles, edge_tiles, fold_units;
+    GGemm<T> p[kCapGemm];
+};
+struct GemvGroup {
+    int np, units, fold_units;
+    GGemv p[kCapGemv];
+};
+struct TrsmGroup {
+    int np, units;
+    GTrsm p[kCapTrsm];
+};
+static_assert(sizeof(GemmGroup<double>) <= 4000 && sizeof(GemvGroup) <= 4000 &&
+                  sizeof(TrsmGroup) <= 4000,
+              "group structs must fit a kernel argument");
 
 struct BatchTable {
-    uint64_t seq;
-    int gemm_n[2][kCfgs], gemm_first[2][kCfgs], gemm_tiles[2][kCfgs];
-    int ngemv, gemv_units, fold_units;
-    int ntrsm, trsm_units;
-    GGemm<double> dg[kCap];
-    GGemm<float> sg[kCap];
-    GGemv gv[kCap];
-    GTrsm tr[kCap];
+    GemmGroup<double> dg[kCfgs];
+    GemmGroup<float> sg[kCfgs];
+    GemvGroup gv;
+    TrsmGroup tr;
 };
-
-template <typename T> __device__ __forceinline__ const GGemm<T> *gemm_list(const BatchTable *t);
-template <> __device__ __forceinline__ const GGemm<double> *gemm_list<double>(const BatchTable *t)
-{
-    return t->dg;
-}
-template <> __device__ __forceinline__ const GGemm<float> *gemm_list<float>(const BatchTable *t)
-{
-    return t->sg;
-}
-
-// ---- grouped GEMM -----------------------------------------------------------
-template <typename T, int BM, int BK, int CFG>
-__global__ __launch_bounds__(256) void group_gemm_kernel(const BatchTable *tb)
-{
-    constexpr int PAD = 128 / (int)sizeof(T);
-    __shared__ T As[2][BK][BM + PAD];
-    __shared__ T Bs[2][BK][BM + PAD];
-    constexpr int ty = sizeof(T) == 8 ? TY_D : TY_S;
-    const int np = min(tb->gemm_n[ty][CFG], kCap);
-    const int total = tb->gemm_tiles[ty][CFG];
-    const GGemm<T> *P = gemm_list<T>(tb) + tb->gemm_first[ty][CFG];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        int lo = 0, hi = np - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (P[mid].tile_begin <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const GemmArgs<T> g = P[lo].g;
-        const int tl = t - P[lo].tile_begin;
-        gemm_tile<T, BM, BM, BK>(g, tl % g.tiles_m, tl / g.tiles_m, As, Bs);
-    }
-}
-
-// ---- grouped GEMV -----------------------------------------------------------
-constexpr int GV_ROWS = 512;    // N: rows per unit (2 per thread)
-constexpr int GV_TCOLS = 16;    // T: columns per unit (4 per wave)
-constexpr int GV_FOLD = 256;    // fold: outputs per unit
-
-__device__ __forceinline__ long vidx(int i, int inc, int len)
-{
-    return inc > 0 ? (long)i * inc : (long)(len - 1 - i) * (long)(-inc);
-}
 
 __device__ __forceinline__ int find_unit(const int *begin, size_t stride_bytes, int n, int u)
 {
@@ -146,55 +131,197 @@ __device__ __forceinline__ int find_unit(const int *begin, size_t stride_bytes, 
     return lo;
 }
 
-// y-partial for rows [rb*512, +512) over columns [cb*span, +span)
+// ---- grouped GEMM -----------------------------------------------------------
+// The product restricted to k-chunk s of a split GEMM (identity if unsplit).
+template <typename T>
+__device__ __forceinline__ GemmArgs<T> chunk_args(const GGemm<T> &e, int s)
+{
+    GemmArgs<T> g = e.g;
+    if (e.ksplit == 1) return g;
+    const long k0 = (long)s * e.kchunk;
+    g.k = min(e.kchunk, e.g.k - (int)k0);
+    g.a.p += g.a.form == PAT_OUTER ? k0 * g.a.ld : k0;
+    g.b.p += g.b.form == PAT_OUTER ? k0 * g.b.ld : k0;
+    g.C = e.part + (long)s * g.m * g.n;
+    g.ldc = g.m;
+    g.alpha = T(1);
+    g.beta = T(0);
+    return g;
+}
+
+template <typename T, int BM>
+__device__ __forceinline__ void gemm_fast_group_body(const GemmGroup<T> &G)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
+    const int np = min(G.np, kCapGemm);
+    const int total = G.fast_tiles;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int p = find_unit(&G.p[0].fast_begin, sizeof(GGemm<T>), np, t);
+        const int per = G.p[p].g.full_m * G.p[p].g.full_n;
+        const int tl0 = t - G.p[p].fast_begin, tl = tl0 % per;
+        const GemmArgs<T> g = chunk_args(G.p[p], tl0 / per);
+        const int tm = tl % g.full_m, tn = tl / g.full_m;
+        switch (G.p[p].pat) {
+        case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
+        case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+        case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
+        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+        }
+    }
+}
+
+// Edge tiles of a problem, enumerated as the column strip past the full
+// columns (all rows) followed by the row strip past the full rows.
+template <typename T, int BM>
+__device__ __forceinline__ void gemm_edge_group_body(const GemmGroup<T> &G)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    __shared__ T As[2][16][BM + PAD];
+    __shared__ T Bs[2][16][BM + PAD];
+    const int np = min(G.np, kCapGemm);
+    const int total = G.edge_tiles;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int p = find_unit(&G.p[0].edge_begin, sizeof(GGemm<T>), np, t);
+        const int per = G.p[p].g.tiles_m * G.p[p].g.tiles_n - G.p[p].g.full_m * G.p[p].g.full_n;
+        const int e0 = t - G.p[p].edge_begin, e = e0 % per;
+        const GemmArgs<T> g = chunk_args(G.p[p], e0 / per);
+        const int strip = g.tiles_m * (g.tiles_n - g.full_n);
+        int tm, tn;
+        if (e < strip) {
+            tm = e % g.tiles_m;
+            tn = g.full_n + e / g.tiles_m;
+        } else {
+            const int e2 = e - strip, rows = g.tiles_m - g.full_m;
+            tm = g.full_m + e2 % rows;
+            tn = e2 / rows;
+        }
+        gemm_tile<T, BM, BM, 16>(g, tm, tn, As, Bs);
+    }
+}
+
+// C = alpha * sum_s part_s + beta * C for the split products, 256 elements
+// per unit, chunks summed in order.
+template <typename T>
+__device__ __forceinline__ void gemm_fold_body(const GemmGroup<T> &G)
+{
+    const int np = min(G.np, kCapGemm);
+    const int total = G.fold_units;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
+        const int p = find_unit(&G.p[0].fold_begin, sizeof(GGemm<T>), np, u);
+        const GGemm<T> &e = G.p[p];
+        const long mn = (long)e.g.m * e.g.n;
+        const long idx = (long)(u - e.fold_begin) * 256 + threadIdx.x;
+        if (idx >= mn) continue;
+        T s = T(0);
+        for (int q = 0; q < e.ksplit; ++q) s += e.part[(long)q * mn + idx];
+        T *c = e.g.C + idx % e.g.m + (idx / e.g.m) * e.g.ldc;
+        const T v = e.g.alpha * s;
+        *c = e.g.beta == T(0) ? v : v + e.g.beta * *c;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void group_gemm_fold_arg(const GemmGroup<T> G)
+{
+    gemm_fold_body<T>(G);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void group_gemm_fold_tab(const GemmGroup<T> *G)
+{
+    gemm_fold_body<T>(*G);
+}
+
+template <typename T, int BM>
+__global__ __launch_bounds__(256, 2) void group_gemm_fast_arg(const GemmGroup<T> G)
+{
+    gemm_fast_group_body<T, BM>(G);
+}
+template <typename T, int BM>
+__global__ __launch_bounds__(256, 2) void group_gemm_fast_tab(const GemmGroup<T> *G)
+{
+    gemm_fast_group_body<T, BM>(*G);
+}
+template <typename T, int BM>
+__global__ __launch_bounds__(256) void group_gemm_edge_arg(const GemmGroup<T> G)
+{
+    gemm_edge_group_body<T, BM>(G);
+}
+template <typename T, int BM>
+__global__ __launch_bounds__(256) void group_gemm_edge_tab(const GemmGroup<T> *G)
+{
+    gemm_edge_group_body<T, BM>(*G);
+}
+
+// ---- grouped GEMV -----------------------------------------------------------
+constexpr int GV_ROWS = 512;    // N: rows per unit (2 per thread)
+constexpr int GV_TCOLS = 16;    // T: columns per unit (4 per wave)
+constexpr int GV_FOLD = 256;    // fold: outputs per unit
+
+__device__ __forceinline__ void gemv_out(const GGemv &d, long i, double acc)
+{
+    double *y = d.y + d.y0 + i * d.incy;
+    const double v = d.alpha * acc;
+    *y = d.beta == 0.0 ? v : v + d.beta * *y;
+}
+
+// N: rows [rb*512, +512) (2 per thread, wave w at rows 128w..) over the
+// columns [cb*span, +span), eight 16-B column loads in flight per lane
 __device__ void gemv_n_unit(const GGemv &d, int local)
 {
     typedef double d2 __attribute__((ext_vector_type(2)));
     const int nrb = (d.m + GV_ROWS - 1) / GV_ROWS;
     const int rb = local % nrb, cb = local / nrb;
-    const int r0 = rb * GV_ROWS + 2 * (int)threadIdx.x;
+    const long r0 = (long)rb * GV_ROWS + 2 * (int)threadIdx.x;
     const int c0 = cb * d.span, c1 = min(d.n, c0 + d.span);
+    const double *x = d.x + d.x0;
+    const long incx = d.incx;
     double acc0 = 0.0, acc1 = 0.0;
-    if (r0 < d.m) {
-        const bool pair = r0 + 1 < d.m;
-        if (pair && d.vec_ok) {
-            int j = c0;
-            for (; j + 8 <= c1; j += 8) {
-                d2 a[8];
-                double xv[8];
+    if (r0 >= d.m) return;
+    const bool pair = r0 + 1 < d.m;
+    if (pair && d.vec_ok) {
+        const double *a = d.A + r0;
+        int j = c0;
+        for (; j + 8 <= c1; j += 8) {
+            d2 v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    a[u] = __builtin_nontemporal_load(
-                        (const d2 *)(d.A + r0 + (long)(j + u) * d.lda));
-                    xv[u] = d.x[vidx(j + u, d.incx, d.n)];
-                }
+            for (int u = 0; u < 8; ++u)
+                v[u] = __builtin_nontemporal_load((const d2 *)(a + (long)(j + u) * d.lda));
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    acc0 = fma(a[u].x, xv[u], acc0);
-                    acc1 = fma(a[u].y, xv[u], acc1);
-                }
-            }
-            for (; j < c1; ++j) {
-                const d2 a = *(const d2 *)(d.A + r0 + (long)j * d.lda);
-                const double xv = d.x[vidx(j, d.incx, d.n)];
-                acc0 = fma(a.x, xv, acc0);
-                acc1 = fma(a.y, xv, acc1);
-            }
-        } else {
-            for (int j = c0; j < c1; ++j) {
-                const double xv = d.x[vidx(j, d.incx, d.n)];
-                const double *a = d.A + r0 + (long)j * d.lda;
-                acc0 = fma(a[0], xv, acc0);
-                if (pair) acc1 = fma(a[1], xv, acc1);
+            for (int u = 0; u < 8; ++u) {
+                const double xv = x[(long)(j + u) * incx];
+                acc0 = fma(v[u].x, xv, acc0);
+                acc1 = fma(v[u].y, xv, acc1);
             }
         }
-        double *p = d.part + (long)cb * d.m + r0;
-        p[0] = acc0;
-        if (pair) p[1] = acc1;
+        for (; j < c1; ++j) {
+            const d2 v = *(const d2 *)(a + (long)j * d.lda);
+            const double xv = x[(long)j * incx];
+            acc0 = fma(v.x, xv, acc0);
+            acc1 = fma(v.y, xv, acc1);
+        }
+    } else {
+        for (int j = c0; j < c1; ++j) {
+            const double xv = x[(long)j * incx];
+            const double *a = d.A + r0 + (long)j * d.lda;
+            acc0 = fma(a[0], xv, acc0);
+            if (pair) acc1 = fma(a[1], xv, acc1);
+        }
     }
+    if (d.direct) {
+        gemv_out(d, r0, acc0);
+        if (pair) gemv_out(d, r0 + 1, acc1);
+        return;
+    }
+    double *p = d.part + (long)cb * d.m + r0;
+    p[0] = acc0;
+    if (pair) p[1] = acc1;
 }
 
-// partial dot products of 16 columns over rows [rc*span, +span)
+// T: partial dot products of 16 columns over rows [rc*span, +span); wave w
+// owns columns 4w..4w+3, each lane takes row pairs 128 apart and keeps four
+// row steps (16 loads) in flight
 __device__ void gemv_t_unit(const GGemv &d, int local)
 {
     typedef double d2 __attribute__((ext_vector_type(2)));
@@ -203,35 +330,45 @@ __device__ void gemv_t_unit(const GGemv &d, int local)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j0 = cb * GV_TCOLS + wave * 4;
     const int r0 = rc * d.span, r1 = min(d.m, r0 + d.span);
+    const double *x = d.x + d.x0;
+    const long incx = d.incx;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const bool vec = d.x_vec;
-    for (int i = r0 + 2 * lane; i < r1; i += 128) {
-        const bool pair = i + 1 < r1;
-        double x0, x1 = 0.0;
-        if (vec && pair) {
-            const d2 xv = *(const d2 *)(d.x + i);
-            x0 = xv.x;
-            x1 = xv.y;
-        } else {
-            x0 = d.x[vidx(i, d.incx, d.m)];
-            if (pair) x1 = d.x[vidx(i + 1, d.incx, d.m)];
+    const double *a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = d.A + (long)min(j0 + c, d.n - 1) * d.lda;
+    const bool cok[4] = {j0 < d.n, j0 + 1 < d.n, j0 + 2 < d.n, j0 + 3 < d.n};
+    int i = r0 + 2 * lane;
+    if (d.vec_ok && d.x_vec) {
+        // full steps: 4 x 128 rows, every row pair inside [r0, r1)
+        for (; i + 3 * 128 + 1 < r1; i += 4 * 128) {
+            d2 av[4][4], xv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                xv[q] = *(const d2 *)(x + i + 128 * q);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    av[q][c] = cok[c] ? __builtin_nontemporal_load((const d2 *)(a[c] + i + 128 * q))
+                                      : d2{0.0, 0.0};
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    acc[c] = fma(av[q][c].x, xv[q].x, acc[c]);
+                    acc[c] = fma(av[q][c].y, xv[q].y, acc[c]);
+                }
         }
+    }
+    // remaining rows (and the general-stride / unaligned case)
+    for (; i < r1; i += 128) {
+        const bool pair = i + 1 < r1;
+        const double x0 = x[(long)i * incx];
+        const double x1 = pair ? x[(long)(i + 1) * incx] : 0.0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int j = j0 + c;
-            if (j >= d.n) continue;
-            const double *a = d.A + i + (long)j * d.lda;
-            double a0, a1 = 0.0;
-            if (d.vec_ok && pair) {
-                const d2 av = __builtin_nontemporal_load((const d2 *)a);
-                a0 = av.x;
-                a1 = av.y;
-            } else {
-                a0 = a[0];
-                if (pair) a1 = a[1];
-            }
-            acc[c] = fma(a0, x0, acc[c]);
-            if (pair) acc[c] = fma(a1, x1, acc[c]);
+            if (!cok[c]) continue;
+            acc[c] = fma(a[c][i], x0, acc[c]);
+            if (pair) acc[c] = fma(a[c][i + 1], x1, acc[c]);
         }
     }
 #pragma unroll
@@ -239,39 +376,55 @@ __device__ void gemv_t_unit(const GGemv &d, int local)
         double v = acc[c];
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0 && j0 + c < d.n) d.part[(long)rc * d.n + j0 + c] = v;
+        if (lane == 0 && cok[c]) {
+            if (d.direct) gemv_out(d, j0 + c, v);
+            else d.part[(long)rc * d.n + j0 + c] = v;
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void group_gemv_part_kernel(const BatchTable *tb)
+__device__ __forceinline__ void gemv_part_body(const GemvGroup &G)
 {
-    const int np = min(tb->ngemv, kCap);
-    const int total = tb->gemv_units;
+    const int np = min(G.np, kCapGemv);
+    const int total = G.units;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int p = find_unit(&tb->gv[0].unit_begin, sizeof(GGemv), np, u);
-        const GGemv d = tb->gv[p];
+        const int p = find_unit(&G.p[0].unit_begin, sizeof(GGemv), np, u);
+        const GGemv d = G.p[p];
         if (d.trans) gemv_t_unit(d, u - d.unit_begin);
         else gemv_n_unit(d, u - d.unit_begin);
     }
 }
 
-__global__ __launch_bounds__(256) void group_gemv_fold_kernel(const BatchTable *tb)
+__device__ __forceinline__ void gemv_fold_body(const GemvGroup &G)
 {
-    const int np = min(tb->ngemv, kCap);
-    const int total = tb->fold_units;
+    const int np = min(G.np, kCapGemv);
+    const int total = G.fold_units;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int p = find_unit(&tb->gv[0].fold_begin, sizeof(GGemv), np, u);
-        const GGemv d = tb->gv[p];
+        const int p = find_unit(&G.p[0].fold_begin, sizeof(GGemv), np, u);
+        const GGemv d = G.p[p];
         const int len = d.trans ? d.n : d.m;
         const int i = (u - d.fold_begin) * GV_FOLD + (int)threadIdx.x;
         if (i >= len) continue;
+        const double *pp = d.part + i;
         double s = 0.0;
-        for (int q = 0; q < d.split; ++q) s += d.part[(long)q * len + i];
-        double *y = d.y + vidx(i, d.incy, len);
-        const double v = d.alpha * s;
-        *y = d.beta == 0.0 ? v : v + d.beta * *y;
+        int q = 0;
+        for (; q + 4 <= d.split; q += 4) {
+            const double p0 = pp[(long)q * len], p1 = pp[(long)(q + 1) * len];
+            const double p2 = pp[(long)(q + 2) * len], p3 = pp[(long)(q + 3) * len];
+            s += p0;
+            s += p1;
+            s += p2;
+            s += p3;
+        }
+        for (; q < d.split; ++q) s += pp[(long)q * len];
+        gemv_out(d, i, s);
     }
 }
+
+__global__ __launch_bounds__(256) void group_gemv_part_arg(const GemvGroup G) { gemv_part_body(G); }
+__global__ __launch_bounds__(256) void group_gemv_part_tab(const GemvGroup *G) { gemv_part_body(*G); }
+__global__ __launch_bounds__(256) void group_gemv_fold_arg(const GemvGroup G) { gemv_fold_body(G); }
+__global__ __launch_bounds__(256) void group_gemv_fold_tab(const GemvGroup *G) { gemv_fold_body(*G); }
 
 // ---- grouped TRSM (left side) ----------------------------------------------
 // One 1024-thread workgroup solves op(A) X = alpha B for a 16-column panel
@@ -307,17 +460,17 @@ __device__ __forceinline__ void trsm_load_col(const GTrsm &d, int I, int nb, int
     }
 }
 
-__global__ __launch_bounds__(1024) void group_trsm_kernel(const BatchTable *tb)
+__device__ __forceinline__ void trsm_group_body(const TrsmGroup &G)
 {
     __shared__ double Bs[TR_MAXM][17];
     __shared__ double Ds[16][17];
-    const int np = min(tb->ntrsm, kCap);
-    const int total = tb->trsm_units;
+    const int np = min(G.np, kCapTrsm);
+    const int total = G.units;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q0 = lane >> 4, cl = lane & 15;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int p = find_unit(&tb->tr[0].unit_begin, sizeof(GTrsm), np, u);
-        const GTrsm d = tb->tr[p];
+        const int p = find_unit(&G.p[0].unit_begin, sizeof(GTrsm), np, u);
+        const GTrsm d = G.p[p];
         const int c0 = (u - d.unit_begin) * 16;
         const int nb = (d.m + 15) / 16;
         const int rows = nb * 16;
@@ -410,13 +563,8 @@ __global__ __launch_bounds__(1024) void group_trsm_kernel(const BatchTable *tb)
     }
 }
 
-__global__ void publish_kernel(const BatchTable *tb, uint64_t *flag)
-{
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        __hip_atomic_store(flag, tb->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
+__global__ __launch_bounds__(1024) void group_trsm_arg(const TrsmGroup G) { trsm_group_body(G); }
+__global__ __launch_bounds__(1024) void group_trsm_tab(const TrsmGroup *G) { trsm_group_body(*G); }
 
 // ---- tile-shape table (autotuned) ------------------------------------------
 // Bucket = (round(log2 sqrt(m n)), round(log2 k)), each clamped to 5..11.
@@ -469,9 +617,11 @@ int choose_cfg(int ty, int m, int n, int k)
 int cfg_bm(int cfg) { return cfg == CFG_32 ? 32 : cfg == CFG_64 ? 64 : 128; }
 
 // grid of each persistent kernel: enough workgroups to fill 256 CUs at the
-// kernel's occupancy, capped by the work when launched eagerly
-constexpr int kGridGemm[kCfgs] = {1024, 1024, 512};
+// kernel's occupancy (double / single), capped by the work when eager
+constexpr int kGridGemm[2][kCfgs] = {{1024, 768, 256}, {1024, 1024, 512}};
+constexpr int kGridGemmFast[2][kCfgs] = {{512, 512, 512}, {512, 512, 512}};  // 2 per CU
 constexpr int kGridGemv = 1024, kGridFold = 512, kGridTrsm = 256;  // TRSM: 1024-thread groups
+constexpr size_t kWsDoubles = (size_t)8 << 20;  // 64 MiB of GEMV partial sums
 
 }  // namespace
 
@@ -481,34 +631,36 @@ struct gpublas_batch {
     int force_cfg;
     hipStream_t s;
     hipStream_t br[3];
-    hipEvent_t ev_fork, ev_join[3];
-    BatchTable *h_tab;
-    BatchTable *d_tab;
+    hipEvent_t ev_fork, ev_join[3], ev_done;
+    hipEvent_t ev_kind[4];  // eager: completion of each kind (DGEMM, SGEMM, DGEMV, DTRSM)
+    int kind_used[4];
+    BatchTable *h_tab;  // pinned; eager launches pass its groups by value
+    BatchTable *d_tab;  // graph replays read this copy
     double *d_ws;
     size_t ws_cap, ws_used;  // doubles
-    uint64_t *h_flag;
-    uint64_t seq;
     int launched;
-    // staging, filled by add_*, laid out into h_tab at launch
-    GemmArgs<double> dq[kCfgs][kCap];
-    GemmArgs<float> sq[kCfgs][kCap];
-    int nd[kCfgs], ns[kCfgs], nd_all, ns_all;
     hipGraphExec_t exec[256];
     long graph_launches, eager_launches;
 };
 
 namespace {
 
-constexpr size_t kWsDoubles = (size_t)8 << 20;  // 64 MiB of GEMV partial sums
-
 void batch_clear(gpublas_batch *b)
 {
-    for (int c = 0; c < kCfgs; ++c) b->nd[c] = b->ns[c] = 0;
-    b->nd_all = b->ns_all = 0;
-    b->h_tab->ngemv = b->h_tab->gemv_units = b->h_tab->fold_units = 0;
-    b->h_tab->ntrsm = b->h_tab->trsm_units = 0;
+    BatchTable *t = b->h_tab;
+    for (int k = 0; k < 4; ++k) b->kind_used[k] = 0;
+    for (int c = 0; c < kCfgs; ++c) {
+        t->dg[c].np = t->dg[c].fast_tiles = t->dg[c].edge_tiles = t->dg[c].fold_units = 0;
+        t->sg[c].np = t->sg[c].fast_tiles = t->sg[c].edge_tiles = t->sg[c].fold_units = 0;
+    }
+    t->gv.np = t->gv.units = t->gv.fold_units = 0;
+    t->tr.np = t->tr.units = 0;
     b->ws_used = 0;
 }
+
+template <typename T> GemmGroup<T> &group_of(BatchTable *t, int cfg);
+template <> GemmGroup<double> &group_of<double>(BatchTable *t, int cfg) { return t->dg[cfg]; }
+template <> GemmGroup<float> &group_of<float>(BatchTable *t, int cfg) { return t->sg[cfg]; }
 
 template <typename T>
 int add_gemm(gpublas_batch *b, char ta, char tb, int m, int n, int k, T alpha, const T *A, int lda,
@@ -516,12 +668,13 @@ int add_gemm(gpublas_batch *b, char ta, char tb, int m, int n, int k, T alpha, c
 {
     constexpr int ty = sizeof(T) == 8 ? TY_D : TY_S;
     if (m <= 0 || n <= 0 || k <= 0 || alpha == T(0)) return -1;
-    int &count = ty == TY_D ? b->nd_all : b->ns_all;
-    if (count >= kCap) return 1;
-    int cfg = b->force_cfg >= 0 && cfg_valid(ty, b->force_cfg) ? b->force_cfg
-                                                               : choose_cfg(ty, m, n, k);
-    GemmArgs<T> g;
-    memset(&g, 0, sizeof g);
+    const int cfg = b->force_cfg >= 0 && cfg_valid(ty, b->force_cfg) ? b->force_cfg
+                                                                     : choose_cfg(ty, m, n, k);
+    GemmGroup<T> &G = group_of<T>(b->h_tab, cfg);
+    if (G.np >= kCapGemm) return 1;
+    GGemm<T> &e = G.p[G.np];
+    memset(&e, 0, sizeof e);
+    GemmArgs<T> &g = e.g;
     g.m = m;
     g.n = n;
     g.k = k;
@@ -535,95 +688,167 @@ int add_gemm(gpublas_batch *b, char ta, char tb, int m, int n, int k, T alpha, c
     g.tiles_m = ceil_div(m, bm);
     g.tiles_n = ceil_div(n, bm);
     g.tri = SYM_NONE;
-    g.full_m = g.full_n = 0;
-    if constexpr (ty == TY_D) b->dq[cfg][b->nd[cfg]++] = g;
-    else b->sq[cfg][b->ns[cfg]++] = g;
-    ++count;
+    // interior tiles take the fast kernel when both operands allow vector
+    // loads (the 32-wide shape has no fast variant)
+    const bool fast = cfg != CFG_32 && g.a.vec_ok && g.b.vec_ok;
+    g.full_m = fast ? m / bm : 0;
+    g.full_n = fast ? n / bm : 0;
+    if (g.full_m == 0 || g.full_n == 0) g.full_m = g.full_n = 0;
+    e.pat = (g.a.form == PAT_OUTER ? 0 : 2) + (g.b.form == PAT_KCONT ? 0 : 1);
+    // split along k when the tiles alone cannot spread over the CUs: about
+    // 256 units, chunks of at least 256, partials of at most 16 MiB.  Counted
+    // in 64 x 64 tiles whatever the tile shape, so every shape splits alike
+    // (and gives the same bits).
+    const long tiles = (long)ceil_div(m, 64) * ceil_div(n, 64);
+    long ks = (256 + tiles - 1) / tiles;
+    const long kmax = k / 256;
+    const long pmax = (16l << 20) / ((long)m * n * (long)sizeof(T));
+    if (ks > kmax) ks = kmax;
+    if (ks > pmax) ks = pmax;
+    if (ks < 1) ks = 1;
+    e.kchunk = (int)(((k + ks - 1) / ks + 15) / 16 * 16);
+    e.ksplit = (k + e.kchunk - 1) / e.kchunk;
+    e.part = nullptr;
+    e.fold_begin = G.fold_units;
+    if (e.ksplit > 1) {
+        const size_t bytes = (size_t)e.ksplit * m * n * sizeof(T);
+        const size_t words = (bytes + 255) / 256 * 32;  // doubles, 256-B aligned
+        if (b->ws_used + words > b->ws_cap) return 1;
+        e.part = (T *)(b->d_ws + b->ws_used);
+        b->ws_used += words;
+        G.fold_units += (int)(((long)m * n + 255) / 256);
+    }
+    e.fast_begin = G.fast_tiles;
+    e.edge_begin = G.edge_tiles;
+    G.fast_tiles += g.full_m * g.full_n * e.ksplit;
+    G.edge_tiles += (g.tiles_m * g.tiles_n - g.full_m * g.full_n) * e.ksplit;
+    G.np++;
     return 0;
 }
 
-void lay_out(gpublas_batch *b, unsigned *mask)
+unsigned batch_mask(const BatchTable *t)
 {
-    BatchTable *t = b->h_tab;
-    int off = 0;
+    unsigned mask = 0;
     for (int c = 0; c < kCfgs; ++c) {
-        t->gemm_first[TY_D][c] = off;
-        t->gemm_n[TY_D][c] = b->nd[c];
-        int tiles = 0;
-        for (int i = 0; i < b->nd[c]; ++i) {
-            t->dg[off + i].g = b->dq[c][i];
-            t->dg[off + i].tile_begin = tiles;
-            tiles += b->dq[c][i].tiles_m * b->dq[c][i].tiles_n;
-        }
-        t->gemm_tiles[TY_D][c] = tiles;
-        if (tiles) *mask |= 1u << c;
-        off += b->nd[c];
+        if (t->dg[c].np) mask |= 1u << c;
+        if (t->sg[c].np) mask |= 1u << (3 + c);
     }
-    off = 0;
-    for (int c = 0; c < kCfgs; ++c) {
-        t->gemm_first[TY_S][c] = off;
-        t->gemm_n[TY_S][c] = b->ns[c];
-        int tiles = 0;
-        for (int i = 0; i < b->ns[c]; ++i) {
-            t->sg[off + i].g = b->sq[c][i];
-            t->sg[off + i].tile_begin = tiles;
-            tiles += b->sq[c][i].tiles_m * b->sq[c][i].tiles_n;
-        }
-        t->gemm_tiles[TY_S][c] = tiles;
-        if (tiles) *mask |= 1u << (3 + c);
-        off += b->ns[c];
-    }
-    if (t->gemv_units) *mask |= 1u << 6;
-    if (t->trsm_units) *mask |= 1u << 7;
+    if (t->gv.units) mask |= 1u << 6;
+    if (t->tr.units) mask |= 1u << 7;
+    return mask;
 }
 
-template <typename T, int BM, int CFG>
-void launch_gemm_group(hipStream_t s, const BatchTable *d_tab, int grid)
+template <typename T, int BM>
+void launch_gemm_group(gpublas_batch *b, hipStream_t s, int cfg, bool capture)
 {
-    hipLaunchKernelGGL((group_gemm_kernel<T, BM, 16, CFG>), dim3(grid), dim3(256), 0, s, d_tab);
+    constexpr int ty = sizeof(T) == 8 ? TY_D : TY_S;
+    const GemmGroup<T> &G = group_of<T>(b->h_tab, cfg);
+    const int ffix = kGridGemmFast[ty][cfg], efix = kGridGemm[ty][cfg];
+    if (capture) {
+        const GemmGroup<T> *dG = &group_of<T>(b->d_tab, cfg);
+        if (BM != 32)
+            hipLaunchKernelGGL((group_gemm_fast_tab<T, BM>), dim3(ffix), dim3(256), 0, s, dG);
+        hipLaunchKernelGGL((group_gemm_edge_tab<T, BM>), dim3(efix), dim3(256), 0, s, dG);
+        hipLaunchKernelGGL((group_gemm_fold_tab<T>), dim3(512), dim3(256), 0, s, dG);
+        return;
+    }
+    if (BM != 32 && G.fast_tiles)
+        hipLaunchKernelGGL((group_gemm_fast_arg<T, BM>), dim3(G.fast_tiles < ffix ? G.fast_tiles : ffix),
+                           dim3(256), 0, s, G);
+    if (G.edge_tiles)
+        hipLaunchKernelGGL((group_gemm_edge_arg<T, BM>), dim3(G.edge_tiles < efix ? G.edge_tiles : efix),
+                           dim3(256), 0, s, G);
+    if (G.fold_units)
+        hipLaunchKernelGGL((group_gemm_fold_arg<T>), dim3(G.fold_units < 512 ? G.fold_units : 512),
+                           dim3(256), 0, s, G);
 }
 
-// Enqueue the batch's kernels.  In graph mode the grids are the fixed
-// persistent sizes (the table's counts are read on the device); eagerly
-// they are capped by the work.
+// Estimated device time of each kind's share of the batch (microseconds,
+// sustained rates of the batch kernels): eager launches run the cheaper
+// kinds first so their callers can leave before the expensive ones end.
+void kind_costs(const BatchTable *t, double (&cost)[4])
+{
+    cost[0] = cost[1] = cost[2] = cost[3] = 0.0;
+    for (int c = 0; c < kCfgs; ++c) {
+        for (int i = 0; i < t->dg[c].np; ++i) {
+            const GemmArgs<double> &g = t->dg[c].p[i].g;
+            cost[0] += 2.0 * g.m * (double)g.n * g.k / 30e6;
+        }
+        for (int i = 0; i < t->sg[c].np; ++i) {
+            const GemmArgs<float> &g = t->sg[c].p[i].g;
+            cost[1] += 2.0 * g.m * (double)g.n * g.k / 60e6;
+        }
+    }
+    for (int i = 0; i < t->gv.np; ++i) cost[2] += 8.0 * t->gv.p[i].m * (double)t->gv.p[i].n / 3e6;
+    for (int i = 0; i < t->tr.np; ++i) cost[3] += 0.04 * t->tr.p[i].m + 1.0;
+}
+
+// Enqueue the batch's kernels.  Captured: fixed persistent grids reading
+// the device table (copied by the first node), the three kinds on parallel
+// branches.  Eager: grids capped by the work, groups passed by value, kinds
+// in increasing estimated cost, each followed by its completion event.
 void enqueue(gpublas_batch *b, unsigned mask, bool capture)
 {
     const BatchTable *t = b->h_tab;
-    const BatchTable *d = b->d_tab;
-    auto grid = [&](int fixed, int work) { return capture ? fixed : (work < fixed ? work : fixed); };
     hipStream_t s0 = b->s, sg = b->s, sv = b->s, st = b->s;
-    (void)hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, s0);
-    if (capture) {
-        // three branches: GEMM groups, GEMV, TRSM
-        (void)hipEventRecord(b->ev_fork, s0);
-        sg = b->br[0];
-        sv = b->br[1];
-        st = b->br[2];
-        (void)hipStreamWaitEvent(sg, b->ev_fork, 0);
-        (void)hipStreamWaitEvent(sv, b->ev_fork, 0);
-        (void)hipStreamWaitEvent(st, b->ev_fork, 0);
+    if (!capture) {
+        double cost[4];
+        kind_costs(t, cost);
+        const unsigned kmask[4] = {1u | 2u | 4u, 16u | 32u, 64u, 128u};
+        int order[4] = {0, 1, 2, 3};
+        for (int i = 1; i < 4; ++i)
+            for (int j = i; j > 0 && cost[order[j]] < cost[order[j - 1]]; --j) {
+                const int tmp = order[j];
+                order[j] = order[j - 1];
+                order[j - 1] = tmp;
+            }
+        for (int q = 0; q < 4; ++q) {
+            const int k = order[q];
+            b->kind_used[k] = (mask & kmask[k]) != 0;
+            if (!b->kind_used[k]) continue;
+            if (k == 0) {
+                if (mask & 1u) launch_gemm_group<double, 32>(b, s0, CFG_32, false);
+                if (mask & 2u) launch_gemm_group<double, 64>(b, s0, CFG_64, false);
+                if (mask & 4u) launch_gemm_group<double, 128>(b, s0, CFG_128, false);
+            } else if (k == 1) {
+                if (mask & 16u) launch_gemm_group<float, 64>(b, s0, CFG_64, false);
+                if (mask & 32u) launch_gemm_group<float, 128>(b, s0, CFG_128, false);
+            } else if (k == 2) {
+                const int gp = t->gv.units < kGridGemv ? t->gv.units : kGridGemv;
+                const int gf = t->gv.fold_units < kGridFold ? t->gv.fold_units : kGridFold;
+                hipLaunchKernelGGL(group_gemv_part_arg, dim3(gp), dim3(256), 0, s0, t->gv);
+                if (gf) hipLaunchKernelGGL(group_gemv_fold_arg, dim3(gf), dim3(256), 0, s0, t->gv);
+            } else {
+                const int g = t->tr.units < kGridTrsm ? t->tr.units : kGridTrsm;
+                hipLaunchKernelGGL(group_trsm_arg, dim3(g), dim3(1024), 0, s0, t->tr);
+            }
+            (void)hipEventRecord(b->ev_kind[k], s0);
+        }
+        return;
     }
-    if (mask & 1u) launch_gemm_group<double, 32, CFG_32>(sg, d, grid(kGridGemm[0], t->gemm_tiles[0][0]));
-    if (mask & 2u) launch_gemm_group<double, 64, CFG_64>(sg, d, grid(kGridGemm[1], t->gemm_tiles[0][1]));
-    if (mask & 4u) launch_gemm_group<double, 128, CFG_128>(sg, d, grid(kGridGemm[2], t->gemm_tiles[0][2]));
-    if (mask & 16u) launch_gemm_group<float, 64, CFG_64>(sg, d, grid(kGridGemm[1], t->gemm_tiles[1][1]));
-    if (mask & 32u) launch_gemm_group<float, 128, CFG_128>(sg, d, grid(kGridGemm[2], t->gemm_tiles[1][2]));
+    (void)hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(BatchTable), hipMemcpyHostToDevice, s0);
+    (void)hipEventRecord(b->ev_fork, s0);
+    sg = b->br[0];
+    sv = b->br[1];
+    st = b->br[2];
+    (void)hipStreamWaitEvent(sg, b->ev_fork, 0);
+    (void)hipStreamWaitEvent(sv, b->ev_fork, 0);
+    (void)hipStreamWaitEvent(st, b->ev_fork, 0);
+    if (mask & 1u) launch_gemm_group<double, 32>(b, sg, CFG_32, true);
+    if (mask & 2u) launch_gemm_group<double, 64>(b, sg, CFG_64, true);
+    if (mask & 4u) launch_gemm_group<double, 128>(b, sg, CFG_128, true);
+    if (mask & 16u) launch_gemm_group<float, 64>(b, sg, CFG_64, true);
+    if (mask & 32u) launch_gemm_group<float, 128>(b, sg, CFG_128, true);
     if (mask & 64u) {
-        hipLaunchKernelGGL(group_gemv_part_kernel, dim3(grid(kGridGemv, t->gemv_units)), dim3(256), 0,
-                           sv, d);
-        hipLaunchKernelGGL(group_gemv_fold_kernel, dim3(grid(kGridFold, t->fold_units)), dim3(256), 0,
-                           sv, d);
+        hipLaunchKernelGGL(group_gemv_part_tab, dim3(kGridGemv), dim3(256), 0, sv, &b->d_tab->gv);
+        hipLaunchKernelGGL(group_gemv_fold_tab, dim3(kGridFold), dim3(256), 0, sv, &b->d_tab->gv);
     }
     if (mask & 128u)
-        hipLaunchKernelGGL(group_trsm_kernel, dim3(grid(kGridTrsm, t->trsm_units)), dim3(1024), 0,
-                           st, d);
-    if (capture) {
-        (void)hipEventRecord(b->ev_join[0], sg);
-        (void)hipEventRecord(b->ev_join[1], sv);
-        (void)hipEventRecord(b->ev_join[2], st);
-        for (int i = 0; i < 3; ++i) (void)hipStreamWaitEvent(s0, b->ev_join[i], 0);
-    }
-    hipLaunchKernelGGL(publish_kernel, dim3(1), dim3(64), 0, s0, d, b->h_flag);
+        hipLaunchKernelGGL(group_trsm_tab, dim3(kGridTrsm), dim3(1024), 0, st, &b->d_tab->tr);
+    (void)hipEventRecord(b->ev_join[0], sg);
+    (void)hipEventRecord(b->ev_join[1], sv);
+    (void)hipEventRecord(b->ev_join[2], st);
+    for (int i = 0; i < 3; ++i) (void)hipStreamWaitEvent(s0, b->ev_join[i], 0);
 }
 
 // Branch streams exist only for capture: created on the first capture, so
@@ -666,16 +891,17 @@ gpublas_batch *gpublas_dev_batch_create(int use_graph)
     b->use_graph = use_graph;
     b->force_cfg = -1;
     bool ok = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < 4 && ok; ++k)
+        ok = hipEventCreateWithFlags(&b->ev_kind[k], hipEventDisableTiming) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&b->h_tab, sizeof(BatchTable), hipHostMallocDefault) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&b->h_flag, 64, hipHostMallocDefault) == hipSuccess;
-    ok = ok && hipMalloc((void **)&b->d_tab, sizeof(BatchTable)) == hipSuccess;
+    ok = ok && (!use_graph || hipMalloc((void **)&b->d_tab, sizeof(BatchTable)) == hipSuccess);
     ok = ok && hipMalloc((void **)&b->d_ws, kWsDoubles * sizeof(double)) == hipSuccess;
     if (!ok) {
         gpublas_dev_batch_destroy(b);
         return nullptr;
     }
     memset(b->h_tab, 0, sizeof(BatchTable));
-    *b->h_flag = 0;
     b->ws_cap = kWsDoubles;
     batch_clear(b);
     return b;
@@ -692,9 +918,11 @@ void gpublas_dev_batch_destroy(gpublas_batch *b)
         if (b->ev_join[i]) (void)hipEventDestroy(b->ev_join[i]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    for (int k = 0; k < 4; ++k)
+        if (b->ev_kind[k]) (void)hipEventDestroy(b->ev_kind[k]);
     if (b->s) (void)hipStreamDestroy(b->s);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
-    if (b->h_flag) (void)hipHostFree(b->h_flag);
     if (b->d_tab) (void)hipFree(b->d_tab);
     if (b->d_ws) (void)hipFree(b->d_ws);
     free(b);
@@ -704,7 +932,10 @@ void gpublas_dev_batch_clear(gpublas_batch *b) { batch_clear(b); }
 
 int gpublas_dev_batch_count(const gpublas_batch *b)
 {
-    return b->nd_all + b->ns_all + b->h_tab->ngemv + b->h_tab->ntrsm;
+    const BatchTable *t = b->h_tab;
+    int n = t->gv.np + t->tr.np;
+    for (int c = 0; c < kCfgs; ++c) n += t->dg[c].np + t->sg[c].np;
+    return n;
 }
 
 void gpublas_dev_batch_force_cfg(gpublas_batch *b, int cfg) { b->force_cfg = cfg; }
@@ -730,8 +961,8 @@ int gpublas_dev_batch_dgemv(gpublas_batch *b, char trans, int m, int n, double a
                             double *d_y, int incy)
 {
     if (m <= 0 || n <= 0 || incx == 0 || incy == 0) return -1;
-    BatchTable *t = b->h_tab;
-    if (t->ngemv >= kCap) return 1;
+    GemvGroup &G = b->h_tab->gv;
+    if (G.np >= kCapGemv) return 1;
     GGemv d;
     memset(&d, 0, sizeof d);
     d.A = d_A;
@@ -749,9 +980,9 @@ int gpublas_dev_batch_dgemv(gpublas_batch *b, char trans, int m, int n, double a
     d.x_vec = (((uintptr_t)d_x & 15) == 0) && incx == 1;
     int units, len;
     if (!d.trans) {
-        // N: 512-row blocks; column blocks (>= 64 columns) for about 256 units
+        // N: 512-row blocks; column blocks (>= 64 columns) for about 512 units
         const int nrb = ceil_div(m, GV_ROWS);
-        int ncb = ceil_div(256, nrb);
+        int ncb = ceil_div(512, nrb);
         const int maxcb = ceil_div(n, 64);
         if (ncb > maxcb) ncb = maxcb;
         if (ncb < 1) ncb = 1;
@@ -760,26 +991,32 @@ int gpublas_dev_batch_dgemv(gpublas_batch *b, char trans, int m, int n, double a
         units = nrb * d.split;
         len = m;
     } else {
-        // T: 16-column units; row chunks (multiples of 128 rows) for about 256 units
+        // T: 16-column units; row chunks (multiples of 512 rows) for about 256 units
         const int ncb = ceil_div(n, GV_TCOLS);
         int nrc = ceil_div(256, ncb);
-        const int maxrc = ceil_div(m, 256);
+        const int maxrc = ceil_div(m, 512);
         if (nrc > maxrc) nrc = maxrc;
         if (nrc < 1) nrc = 1;
-        d.span = ceil_div(ceil_div(m, nrc), 128) * 128;
+        d.span = ceil_div(ceil_div(m, nrc), 512) * 512;
         d.split = ceil_div(m, d.span);
         units = ncb * d.split;
         len = n;
     }
-    const size_t need = (size_t)d.split * (size_t)len;
-    if (b->ws_used + need > b->ws_cap) return t->ngemv ? 1 : -1;
-    d.part = b->d_ws + b->ws_used;
-    b->ws_used += (need + 31) / 32 * 32;
-    d.unit_begin = t->gemv_units;
-    d.fold_begin = t->fold_units;
-    t->gemv_units += units;
-    t->fold_units += ceil_div(len, GV_FOLD);
-    t->gv[t->ngemv++] = d;
+    const int lenx = d.trans ? m : n;
+    d.x0 = incx > 0 ? 0 : (long)(lenx - 1) * (long)(-incx);
+    d.y0 = incy > 0 ? 0 : (long)(len - 1) * (long)(-incy);
+    d.direct = d.split == 1;
+    if (!d.direct) {
+        const size_t need = (size_t)d.split * (size_t)len;
+        if (b->ws_used + need > b->ws_cap) return G.np ? 1 : -1;
+        d.part = b->d_ws + b->ws_used;
+        b->ws_used += (need + 31) / 32 * 32;
+    }
+    d.unit_begin = G.units;
+    d.fold_begin = G.fold_units;
+    G.units += units;
+    if (!d.direct) G.fold_units += ceil_div(len, GV_FOLD);
+    G.p[G.np++] = d;
     return 0;
 }
 
@@ -787,8 +1024,8 @@ int gpublas_dev_batch_dtrsm(gpublas_batch *b, char side, char uplo, char transa,
                             int n, double alpha, const double *d_A, int lda, double *d_B, int ldb)
 {
     if (!is_l(side) || m <= 0 || m > TR_MAXM || n <= 0 || alpha == 0.0) return -1;
-    BatchTable *t = b->h_tab;
-    if (t->ntrsm >= kCap) return 1;
+    TrsmGroup &G = b->h_tab->tr;
+    if (G.np >= kCapTrsm) return 1;
     GTrsm d;
     memset(&d, 0, sizeof d);
     d.A = d_A;
@@ -801,44 +1038,51 @@ int gpublas_dev_batch_dtrsm(gpublas_batch *b, char side, char uplo, char transa,
     d.fwd = (!is_u(uplo)) == !d.trans;  // op(A) lower
     d.unit = diag == 'U' || diag == 'u';
     d.alpha = alpha;
-    d.unit_begin = t->trsm_units;
-    t->trsm_units += ceil_div(n, 16);
-    t->tr[t->ntrsm++] = d;
+    d.unit_begin = G.units;
+    G.units += ceil_div(n, 16);
+    G.p[G.np++] = d;
     return 0;
 }
 
 int gpublas_dev_batch_launch(gpublas_batch *b)
 {
-    unsigned mask = 0;
-    lay_out(b, &mask);
-    b->h_tab->seq = ++b->seq;
+    const unsigned mask = batch_mask(b->h_tab);
     b->launched = 1;
+    int err = 0;
     PROF_BEGIN(b->s, "serve_batch");
     if (b->use_graph) {
         hipGraphExec_t e = graph_for(b, mask);
         if (!e) return (int)hipErrorUnknown;
-        const int err = (int)hipGraphLaunch(e, b->s);
-        PROF_END(b->s);
+        err = (int)hipGraphLaunch(e, b->s);
         b->graph_launches++;
-        return err;
+    } else {
+        enqueue(b, mask, false);
+        err = (int)hipGetLastError();
+        b->eager_launches++;
     }
-    enqueue(b, mask, false);
     PROF_END(b->s);
-    b->eager_launches++;
-    return (int)hipGetLastError();
+    if (!err) err = (int)hipEventRecord(b->ev_done, b->s);
+    return err;
 }
 
 int gpublas_dev_batch_done(gpublas_batch *b)
 {
     if (!b->launched) return 1;
-    return __atomic_load_n(b->h_flag, __ATOMIC_ACQUIRE) >= b->seq;
+    return hipEventQuery(b->ev_done) == hipSuccess;
+}
+
+int gpublas_dev_batch_done_kind(gpublas_batch *b, int kind)
+{
+    if (!b->launched) return 1;
+    if (b->use_graph || kind < 0 || kind > 3) return gpublas_dev_batch_done(b);
+    if (!b->kind_used[kind]) return 1;
+    return hipEventQuery(b->ev_kind[kind]) == hipSuccess;
 }
 
 int gpublas_dev_batch_wait(gpublas_batch *b)
 {
     if (!b->launched) return 0;
-    const int err = (int)hipStreamSynchronize(b->s);
-    return err ? err : (gpublas_dev_batch_done(b) ? 0 : (int)hipErrorUnknown);
+    return (int)hipEventSynchronize(b->ev_done);
 }
 
 // Tile-shape table: measure each valid configuration on groups of 8
@@ -900,10 +1144,7 @@ int gpublas_dev_tune_gemm(const char *path, int force)
                                                 (const float *)(base + maxe), k, 0.0f,
                                                 (float *)(base + 2 * maxe), mn);
                         }
-                        unsigned mask = 0;
-                        lay_out(b, &mask);
-                        b->h_tab->seq = ++b->seq;
-                        b->launched = 1;
+                        const unsigned mask = batch_mask(b->h_tab);
                         (void)hipEventRecord(e0, b->s);
                         enqueue(b, mask, false);
                         (void)hipEventRecord(e1, b->s);

@@ -25,8 +25,8 @@ struct gb_options gb_opt = {
     .min_track = 4096,
     .cutoff = 512,
     .serve_budget = (size_t)64 << 30,
-    .serve_slots = 4,
-    .serve_graph = 1,
+    .serve_slots = 8,
+    .serve_graph = 0,
     .serve_batch = 256,
 };
 
@@ -85,8 +85,8 @@ static void print_help(void)
            "  serve[=<bytes>]      run DGEMM/SGEMM/DGEMV/left DTRSM calls whose operands are\n"
            "                       all tracked through the batching serving loop, with an\n"
            "                       HBM admission budget (64G); serve=0 turns it off\n"
-           "  serve_slots=<n>      batches in flight (4)\n"
-           "  serve_graph=<0|1>    replay captured hipGraphs (1) or launch eagerly (0)\n"
+           "  serve_slots=<n>      batches in flight (8)\n"
+           "  serve_graph=<0|1>    replay captured hipGraphs (1) or launch eagerly (0, default)\n"
            "  serve_batch=<n>      most calls per batch (256; 1 = one launch per call)\n"
            "  tune=<file>          tile-shape table of the batch GEMM kernels: loaded\n"
            "                       from <file>, measured and written there if absent\n"

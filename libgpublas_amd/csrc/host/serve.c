@@ -23,9 +23,12 @@
  *      stale and pins the objects.  The batch is launched at once -- it does
  *      not wait to fill: requests that arrive while batches run gather in
  *      the FIFO and form the next batch (continuous batching).
- *   3. When a batch's publication word reaches its sequence number the
- *      engine marks written objects device-resident (host access faults and
- *      downloads), unpins, and wakes each caller, which returns.
+ *   3. A batch runs its kinds (DGEMM, SGEMM, DGEMV, DTRSM groups) cheapest
+ *      first, each followed by an event; as soon as a kind's event has
+ *      fired the engine marks the objects its requests wrote
+ *      device-resident (host access faults and downloads), unpins them and
+ *      wakes those callers, which return.  The slot is reused when the whole
+ *      batch is done.
  * The slot's kernels give every problem the same work decomposition alone
  * and in any mix, so a served result does not depend on its batch.
  */
@@ -37,7 +40,8 @@
 #include <time.h>
 #include <unistd.h>
 
-enum { RQ_DGEMM, RQ_SGEMM, RQ_DGEMV, RQ_DTRSM };
+/* request kinds, numbered as the batch slot's kinds */
+enum { RQ_DGEMM = 0, RQ_SGEMM = 1, RQ_DGEMV = 2, RQ_DTRSM = 3 };
 #define MAX_SLOTS 16
 #define SPIN_POLLS 4000
 #define LONG_US 150.0   /* calls estimated above this run in batches of their own class */
@@ -61,6 +65,7 @@ struct req {
     int nobj;
     int admitted, deferred;
     int is_long;         /* estimated device time above LONG_US */
+    uint64_t t_submit, t_launch;  /* monotonic ns */
     struct waiter *w;
     uint32_t ticket;
     struct req *next;
@@ -89,6 +94,13 @@ static long futex(uint32_t *uaddr, int op, uint32_t val, const struct timespec *
 }
 
 static void cpu_relax(void) { __builtin_ia32_pause(); }
+
+static uint64_t now_ns(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (uint64_t)t.tv_sec * 1000000000ull + (uint64_t)t.tv_nsec;
+}
 
 /* ---- engine ---------------------------------------------------------- */
 static int any_busy(void)
@@ -174,22 +186,34 @@ static int add_to_batch(gpublas_batch *b, const struct req *r)
     }
 }
 
+/* Complete every request whose kind has finished (a batch's kinds end at
+ * different times); free a slot once all of its work is done. */
 static int reap(void)
 {
     int progress = 0;
     for (int i = 0; i < E.nslots; ++i) {
-        if (!E.busy[i] || !gpublas_dev_batch_done(E.slot[i])) continue;
-        struct req *r = E.slot_reqs[i];
-        while (r) {
-            struct req *next = r->next;
+        if (!E.busy[i]) continue;
+        gpublas_batch *b = E.slot[i];
+        int kdone[4] = {-1, -1, -1, -1};
+        struct req **link = &E.slot_reqs[i];
+        while (*link) {
+            struct req *r = *link;
+            if (kdone[r->op] < 0) kdone[r->op] = gpublas_dev_batch_done_kind(b, r->op);
+            if (!kdone[r->op]) {
+                link = &r->next;
+                continue;
+            }
+            *link = r->next;
+            GB_COUNT(serve_run_ns, now_ns() - r->t_launch);
             release_objs(r, 1);
-            complete(r, 0);
-            r = next;
+            complete(r, 0); /* r is gone after this */
+            progress = 1;
         }
-        E.slot_reqs[i] = NULL;
-        gpublas_dev_batch_clear(E.slot[i]);
-        E.busy[i] = 0;
-        progress = 1;
+        if (!E.slot_reqs[i] && gpublas_dev_batch_done(b)) {
+            gpublas_dev_batch_clear(b);
+            E.busy[i] = 0;
+            progress = 1;
+        }
     }
     return progress;
 }
@@ -235,6 +259,7 @@ static int launch_one(void)
             break;
         }
     if (si < 0 || !E.pend_head) return 0;
+    const uint64_t t0 = now_ns();
     gpublas_batch *b = E.slot[si];
     void *st = gpublas_dev_batch_stream(b);
     struct req *list = NULL, **tail = &list;
@@ -271,6 +296,12 @@ static int launch_one(void)
     gb_check(gpublas_dev_batch_launch(b), "serving batch launch");
     E.busy[si] = 1;
     E.slot_reqs[si] = list;
+    const uint64_t t1 = now_ns();
+    for (struct req *r = list; r; r = r->next) {
+        r->t_launch = t1;
+        GB_COUNT(serve_wait_ns, t1 - r->t_submit);
+    }
+    GB_COUNT(serve_launch_ns, t1 - t0);
     GB_COUNT(batches, 1);
     GB_COUNT(served, n);
     uint64_t mx = __atomic_load_n(&gb_stats.max_batch, __ATOMIC_RELAXED);
@@ -392,6 +423,7 @@ static int submit(struct req *r)
     struct waiter *w = t_waiter;
     r->w = w;
     r->ticket = ++w->issued;
+    r->t_submit = now_ns();
     /* kernels of the per-call path (any thread) that this call must follow */
     for (int i = 0; i < r->nobj; ++i) gb_obj_wait_hazards(r->obj[i], (r->mode[i] & GB_OUT) != 0);
     struct req *head = __atomic_load_n(&E.queue, __ATOMIC_RELAXED);

@@ -250,7 +250,7 @@ def test_graph_and_eager_launches_agree(gpu, ora):
     for graph in (1, 0):
         assert gpu.set_options(f"serve=64G;serve_graph={graph}") == 0
         outs.append([_run(gpu, k, cs) for k, cs in cases])
-    gpu.set_options("serve=0;serve_graph=1")
+    gpu.set_options("serve=0;serve_graph=0")
     for a, b in zip(*outs):
         assert np.array_equal(a, b, equal_nan=True)
 
@@ -276,6 +276,47 @@ def test_tile_shapes_give_identical_bits(gpu):
     assert np.max(np.abs(res[0] - ref)) < 1e-12
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+@pytest.mark.parametrize("single", [False, True])
+def test_split_k_products(gpu, ora, ta, tb, single):
+    """Few tiles, long k: the batch kernels split k into chunks and fold them
+    in order.  Same bits for every tile shape and alone or batched with
+    other products; DMMCH gauge against the oracle."""
+    gb = gpu
+    rng = np.random.default_rng(17)
+    m, n, k = 48, 40, 3000
+    dt = np.float32 if single else np.float64
+    A0 = uniform(91, (m, k) if ta == "N" else (k, m)).astype(dt)
+    B0 = uniform(92, (k, n) if tb == "N" else (n, k)).astype(dt)
+    C0 = uniform(93, (m, n)).astype(dt)
+    cs = dict(ta=ta, tb=tb, m=m, n=n, k=k, A=np.asfortranarray(A0), B=np.asfortranarray(B0),
+              C0=np.asfortranarray(C0), alpha=1.5, beta=-0.5, single=single)
+    outs = []
+    for cfg, others in ((0, 0), (1, 0), (2, 0), (1, 5)):
+        if single and cfg == 0:
+            continue
+        A, B, C = gb.tracked_copy(cs["A"]), gb.tracked_copy(cs["B"]), gb.tracked_copy(cs["C0"])
+        b = gb.Batch(use_graph=False)
+        b.force_cfg(cfg)
+        add = b.sgemm if single else b.dgemm
+        keep = []
+        for i in range(others):  # unrelated products in the same launch
+            oc = _gemm_case(rng, 900 + 10 * i, single=single, maxdim=200)
+            oA, oB, oC = (gb.tracked_copy(oc[x]) for x in ("A", "B", "C0"))
+            keep += [oA, oB, oC]
+            assert add(oc["ta"], oc["tb"], oc["m"], oc["n"], oc["k"], oc["alpha"],
+                       gb.device_ptr(oA), oA.shape[0], gb.device_ptr(oB), oB.shape[0],
+                       oc["beta"], gb.device_ptr(oC), oc["m"]) == 0
+        assert add(ta, tb, m, n, k, 1.5, gb.device_ptr(A), A.shape[0], gb.device_ptr(B),
+                   B.shape[0], -0.5, gb.device_ptr(C), m) == 0
+        b.run()
+        b.close()
+        outs.append(np.array(C))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+    assert _gemm_ratio(ora, cs, outs[0]) < 16
+
+
 def test_batch_rejects_what_it_cannot_run(gpu):
     gb = gpu
     b = gb.Batch(use_graph=False)
@@ -284,10 +325,11 @@ def test_batch_rejects_what_it_cannot_run(gpu):
     assert b.dgemm("N", "N", 8, 8, 8, 0.0, d, 8, d, 8, 1.0, d, 8) == -1  # alpha == 0
     assert b.dgemm("N", "N", 8, 8, 0, 1.0, d, 8, d, 8, 1.0, d, 8) == -1  # k == 0
     assert b.dtrsm("R", "L", "N", "N", 8, 8, 1.0, d, 8, d, 8) == -1      # right side
-    for _ in range(64):
+    assert b.dtrsm("L", "L", "N", "N", 1025, 8, 1.0, d, 1025, d, 1025) == -1  # m > 1024
+    for _ in range(32):
         assert b.dgemv("N", 8, 8, 1.0, d, 8, d, 1, 0.0, d, 1) == 0
     assert b.dgemv("N", 8, 8, 1.0, d, 8, d, 1, 0.0, d, 1) == 1          # slot full
-    assert b.count() == 64
+    assert b.count() == 32
     b.close()
 
 
