@@ -341,6 +341,22 @@ def main():
                for k in prof if k.startswith("k_") and k.endswith("_ms")}
 
     gemm = gemm_figures()
+    # serving trace (closed loop, 16 threads, 8000 calls): batching loop vs
+    # one launch per call
+    srv = {}
+    try:
+        plain = serve_run("plain", 0, 8000)
+        batched = serve_run("serve", 0, 8000)
+        srv = {"workload": "16 threads, closed loop, 8000 calls: 60% D/SGEMM logU[32,2048]^3, "
+                           "30% DGEMV logU[256,16384]^2, 10% DTRSM m logU[64,1024] x 256",
+               "calls_per_s": round(batched["calls_per_s"], 1),
+               "plain_calls_per_s": round(plain["calls_per_s"], 1),
+               "speedup_vs_plain": round(batched["calls_per_s"] / plain["calls_per_s"], 3),
+               "p50_us": batched["p50_us"], "p99_us": batched["p99_us"],
+               "plain_p50_us": plain["p50_us"], "plain_p99_us": plain["p99_us"],
+               "mean_batch": batched["mean_batch"]}
+    except Exception as e:  # reported, not fatal for the headline line
+        srv = {"error": str(e)[:300]}
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -354,6 +370,7 @@ def main():
         "kernel_avg_us": kernels,
         "chain_results": {"ddot": chain["ddot"], "dnrm2": chain["dnrm2"], "idamax": chain["idamax"]},
         "dgemm": gemm,
+        "serving": srv,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_oracle_chain()
