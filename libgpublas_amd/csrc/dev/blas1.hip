@@ -5,11 +5,13 @@
 // cublasDaxpy (blas_level1/axpy.cc:49), cublasDdot (dot.cc:46), cublasDnrm2
 // (nrm2.cc:39), cublasIdamax (amax.cc:40).
 //
-// The reductions are single launches: every workgroup reduces its
-// grid-stride share in a fixed order (per-lane sequential, then a wave64
-// shuffle tree, then the 4 waves in order), writes one partial, and the
-// workgroup that draws the last ticket folds the partials in index order and
-// writes the scalar straight into pinned host memory.  The grid depends only
+// The reductions are single launches.  Vectors of up to 65536 elements are
+// reduced by one 1024-thread workgroup (one memory round trip, no partials).
+// Longer ones: every workgroup reduces its grid-stride share in a fixed
+// order (per-lane sequential, then a wave64 shuffle tree, then the 4 waves
+// in order), writes one partial, and the workgroup that draws the last
+// ticket folds the partials in index order.  Either way the scalar goes
+// straight into pinned host memory.  The grid depends only
 // on n, so the result is bitwise reproducible run to run and independent of
 // what else is on the GPU.  Tickets reset themselves, so no memset is needed
 // between calls.
@@ -299,7 +301,153 @@ __global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long
     }
 }
 
+// ---- single-workgroup reductions for short vectors ---------------------------
+// Below kSmallN one 1024-thread workgroup reduces the whole vector: thread t
+// takes elements 2t, 2t+1, 2t+2048, ... (16-B loads, four per operand in
+// flight), then a wave64 shuffle tree and the 16 waves in order.  No
+// tickets, no partials: one memory round trip and the result publication.
+constexpr int kSmallN = 1 << 16;
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <typename F>
+__device__ __forceinline__ void for_pairs(int n, F &&f)
+{
+    // calls f(i) for the element indices of this thread in increasing order
+    int i = 2 * (int)threadIdx.x;
+    for (; i < n; i += 2048) {
+        f(i);
+        if (i + 1 < n) f(i + 1);
+    }
+}
+
+__device__ __forceinline__ double block_sum16(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void dot_small(int n, const double *x, long incx, long x0,
+                                                  const double *y, long incy, long y0, int vec,
+                                                  double *result, uint64_t *flag, uint64_t seq)
+{
+    __shared__ double red[16];
+    double s = 0;
+    if (vec) {
+        int i = 2 * (int)threadIdx.x;
+        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
+            d2v xv[4], yv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xv[u] = *(const d2v *)(x + i + 2048 * u);
+                yv[u] = *(const d2v *)(y + i + 2048 * u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                s = fma(xv[u].x, yv[u].x, s);
+                s = fma(xv[u].y, yv[u].y, s);
+            }
+        }
+        for (; i < n; i += 2048) {
+            s = fma(x[i], y[i], s);
+            if (i + 1 < n) s = fma(x[i + 1], y[i + 1], s);
+        }
+    } else {
+        for_pairs(n, [&](int i) { s = fma(x[x0 + i * incx], y[y0 + i * incy], s); });
+    }
+    s = block_sum16(s, red);
+    if (threadIdx.x == 0) {
+        *result = s;
+        publish(flag, seq);
+    }
+}
+
+__global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long incx, long x0,
+                                                   int vec, double *result, uint64_t *flag,
+                                                   uint64_t seq)
+{
+    __shared__ double red[16];
+    Acc3 a{0, 0, 0};
+    if (vec) {
+        int i = 2 * (int)threadIdx.x;
+        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
+            d2v xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc3_add(a, xv[u].x);
+                acc3_add(a, xv[u].y);
+            }
+        }
+        for (; i < n; i += 2048) {
+            acc3_add(a, x[i]);
+            if (i + 1 < n) acc3_add(a, x[i + 1]);
+        }
+    } else {
+        for_pairs(n, [&](int i) { acc3_add(a, x[x0 + i * incx]); });
+    }
+    a.sml = block_sum16(a.sml, red);
+    a.med = block_sum16(a.med, red);
+    a.big = block_sum16(a.big, red);
+    if (threadIdx.x == 0) {
+        *result = nrm2_finish(a.sml, a.med, a.big);
+        publish(flag, seq);
+    }
+}
+
+__global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long incx, int vec,
+                                                    int *result, uint64_t *flag, uint64_t seq)
+{
+    __shared__ Cand red[16];
+    Cand c{-1.0, 0x7fffffffffffffffL};
+    auto take = [&](long i, double v) {
+        double a = fabs(v);
+        if (isnan(a)) a = -1.0;
+        c = better(c, Cand{a, i});
+    };
+    if (vec) {
+        int i = 2 * (int)threadIdx.x;
+        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
+            d2v xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                take(i + 2048 * u, xv[u].x);
+                take(i + 2048 * u + 1, xv[u].y);
+            }
+        }
+        for (; i < n; i += 2048) {
+            take(i, x[i]);
+            if (i + 1 < n) take(i + 1, x[i + 1]);
+        }
+    } else {
+        for_pairs(n, [&](int i) { take(i, x[i * incx]); });
+    }
+    c = wave_best(c);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand t = red[0];
+#pragma unroll
+        for (int w = 1; w < 16; ++w) t = better(t, red[w]);
+        long r = t.i + 1;
+        if (isnan(x[0]) || t.v < 0) r = 1;
+        *result = (int)r;
+        publish(flag, seq);
+    }
+}
+
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
+inline int aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
 
@@ -337,8 +485,17 @@ int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const dou
         *h_result = 0.0;
         return 0;
     }
-    RedSlot r = next_slot();
     const Pub pb = pub_for(h_result);
+    if (n <= kSmallN) {
+        const int vec = incx == 1 && incy == 1 && aligned16(d_x) && aligned16(d_y);
+        PROF_BEGIN(s, "ddot");
+        hipLaunchKernelGGL(dot_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx,
+                           start_of(n, incx), d_y, (long)incy, start_of(n, incy), vec, h_result,
+                           pb.flag, pb.seq);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+    RedSlot r = next_slot();
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "ddot");
     hipLaunchKernelGGL(dot_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
@@ -355,8 +512,16 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
         *h_result = 0.0;
         return 0;
     }
-    RedSlot r = next_slot();
     const Pub pb = pub_for(h_result);
+    if (n <= kSmallN) {
+        const int vec = incx == 1 && aligned16(d_x);
+        PROF_BEGIN(s, "dnrm2");
+        hipLaunchKernelGGL(nrm2_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx,
+                           start_of(n, incx), vec, h_result, pb.flag, pb.seq);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+    RedSlot r = next_slot();
     int blocks = red_blocks(n);
     if (blocks > kMaxBlocks / 2) blocks = kMaxBlocks / 2;  // 3 partials per block
     PROF_BEGIN(s, "dnrm2");
@@ -374,8 +539,16 @@ int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_
         *h_result = 0;
         return 0;
     }
-    RedSlot r = next_slot();
     const Pub pb = pub_for(h_result);
+    if (n <= kSmallN) {
+        const int vec = incx == 1 && aligned16(d_x);
+        PROF_BEGIN(s, "idamax");
+        hipLaunchKernelGGL(iamax_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
+                           h_result, pb.flag, pb.seq);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+    RedSlot r = next_slot();
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "idamax");
     hipLaunchKernelGGL(iamax_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
