@@ -7,41 +7,47 @@
 //  1. every row i of op(A) is scaled by 2^-ea_i and every column j of op(B)
 //     by 2^-eb_j so its largest entry has magnitude below 1 (exact);
 //  2. each scaled entry x is cut into s signed 8-bit integers:
-//       t = 256 x;  q_p = trunc(t);  x = t - q_p      (p = 1..s, all exact)
-//     so a = 2^ea sum_p q_p 2^-8p + r with |r| < 2^(ea-8s).  |q_p| <= 255 is
-//     exactly representable in BF16 (8 significant bits);
-//  3. for every slice pair with p + q <= s + 1 the integer product
-//     Q_A^p Q_B^q is formed on v_mfma_f32_32x32x16_bf16.  Over a k-chunk of
-//     256 every partial sum is an integer of magnitude <= 255^2 * 256 < 2^24,
-//     so the FP32 accumulation is exact;
-//  4. after each chunk and pair the FP32 tile is added, scaled by
-//     2^-8(p+q), into an FP64 accumulator; the epilogue applies
-//     2^(ea_i + eb_j), alpha and beta in FP64.
-// The only approximations are the dropped pairs (p + q > s + 1), the slice
-// remainders and the FP64 additions.  FP64 is used exactly where the scheme
-// needs it: the scaling/splitting pass and the final summation.
+//       t = 256 x;  q_p = trunc(t);  x = t - q_p      (p = 0..s-1, all exact)
+//     so a = 2^(ea-8) sum_p q_p 2^-8p + r with |r| < 2^(ea-8s).  Slice p is
+//     stored as q_p 2^-8p, exact in BF16 (|q_p| <= 255: 8 significant bits);
+//  3. the slice products with p + q <= s - 1 run on v_mfma_f32_32x32x16_bf16
+//     and accumulate in FP32 by diagonal d = p + q.  Diagonals 0 to 3 are
+//     exact over k chunks of 256, 128, 64 and 64 (every partial sum is a
+//     multiple of 2^-8d below 2^(24-8d)); the diagonals from 4 on, whose
+//     weight is below 2^-32 of the leading term, share one rounding
+//     accumulator;
+//  4. each accumulator is added into FP64 at the end of its chunk; the
+//     epilogue applies 2^(ea_i + eb_j - 16), alpha and beta in FP64.
+// The approximations are the dropped pairs (p + q >= s), the slice
+// remainders, the tail accumulator's FP32 rounding and the FP64 additions.
+// FP64 is used exactly where the scheme needs it: the scaling/splitting pass
+// and the final summation.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 #include <math.h>
+#include <type_traits>
 
 namespace {
 
 typedef __bf16 bf16;
 typedef short v8s __attribute__((ext_vector_type(8)));  // 16 B of bf16 for loads
-constexpr int OZ_BM = 128, OZ_BN = 128, OZ_BKT = 32, OZ_CHUNK = 256, OZ_MAXS = 8;
-constexpr int OZ_LDSK = OZ_BKT + 8;  // bf16 per LDS row (16-B pad)
+constexpr int OZ_MAXS = 8;
+#ifndef OZ_WNT
+#define OZ_WNT 1  // 32-column MFMA tiles per wave (tile N = 64 * OZ_WNT)
+#endif
+#ifndef OZ_OCC
+#define OZ_OCC 1  // workgroups per CU the register budget is sized for
+#endif
 
 struct OzArgs {
-    int M, N, Kp;           // Kp: K rounded up to a multiple of OZ_BKT (zero slices)
-    int s, npairs;
-    const bf16 *SA, *SB;    // [s][M][Kp], [s][N][Kp], k contiguous
+    int M, N, Mp, Np, KT;   // Mp, Np: rows padded to 128; KT: 16-deep k slabs
+    const bf16 *SA, *SB;    // [s][KT][Mp][16], [s][KT][Np][16]
     const int *ea, *eb;     // row exponents of op(A), column exponents of op(B)
     double alpha, beta;
     double *C;
     long ldc;
-    int tiles_m;
+    int tiles_m, tiles_n;
     int tri;                // 0 rectangle, 1 lower, 2 upper (SYRK)
-    unsigned char pp[64], pq[64];  // slice pairs
 };
 
 // ---- exponents -----------------------------------------------------------
@@ -88,10 +94,17 @@ __global__ void exp_rows_contig(int R, int K, const double *X, long ld, int *e, 
 }
 
 // ---- splitting -----------------------------------------------------------
+// Slices are stored pre-scaled, slice p (0-based) holding q_p * 2^-8p, so a
+// product of slices p and q carries its true relative weight 2^-8(p+q) and
+// the products of one "diagonal" d = p + q can share an accumulator.  Layout
+// S[p][k/16][row][16] (bf16, rows padded to a multiple of 128, k to 16): a
+// 16-deep k slab of 32 consecutive rows is one contiguous kilobyte, the unit
+// the product kernel moves straight from HBM into LDS.
 // 32 (r) x 64 (k) tile through LDS so both the FP64 read (either layout) and
-// the BF16 slice writes (k contiguous) are coalesced.
-__global__ __launch_bounds__(256) void split_kernel(int R, int K, int Kp, int s, const double *X,
-                                                    long ld, int rcontig, const int *e, bf16 *S)
+// the slice writes are coalesced.
+__global__ __launch_bounds__(256) void split_kernel(int R, int Rp, int K, int KT, int s,
+                                                    const double *X, long ld, int rcontig,
+                                                    const int *e, bf16 *S)
 {
     __shared__ double t[32][65];
     const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 64;
@@ -105,162 +118,255 @@ __global__ __launch_bounds__(256) void split_kernel(int R, int K, int Kp, int s,
         t[rr][kk] = v;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 32 * 32; idx += 256) {
-        const int kk = (idx % 32) * 2, rr = idx / 32;
-        const long r = r0 + rr, k = k0 + kk;
-        if (r >= R || k >= Kp) continue;
-        const int er = e[r];
+    // thread -> (k slab kb, row, k pair): 8 threads write one row's 32 bytes
+    for (int idx = threadIdx.x; idx < 4 * 32 * 8; idx += 256) {
+        const int kb = idx / 256, rr = (idx % 256) / 8, kp = idx % 8;
+        const int kt = blockIdx.y * 4 + kb;
+        const long r = r0 + rr;
+        if (kt >= KT || r >= Rp) continue;
+        const int kk = kb * 16 + 2 * kp;
+        const int er = r < R ? e[r] : 0;
         double x0 = ldexp(t[rr][kk], -er), x1 = ldexp(t[rr][kk + 1], -er);
         for (int p = 0; p < s; ++p) {
             const double u0 = x0 * 256.0, u1 = x1 * 256.0;
             const double q0 = trunc(u0), q1 = trunc(u1);
             x0 = u0 - q0;
             x1 = u1 - q1;
-            bf16 *o = S + ((long)p * R + r) * Kp + k;
-            o[0] = (bf16)(float)q0;
-            o[1] = (bf16)(float)q1;
+            bf16 *o = S + (((long)p * KT + kt) * Rp + r) * 16 + 2 * kp;
+            o[0] = (bf16)(float)ldexp(q0, -8 * p);
+            o[1] = (bf16)(float)ldexp(q1, -8 * p);
         }
     }
 }
 
 // ---- slice products ----------------------------------------------------------
-__device__ __forceinline__ void tri_decode(int b, int &hi, int &lo)
+typedef const __attribute__((address_space(1))) void *gptr_t;
+typedef __attribute__((address_space(3))) void *lptr_t;
+
+// Accumulator set of diagonal d: d = 0, 1, 2, 3 are exact in FP32 over chunks
+// of 256, 128, 64 and 64 k (1, 2, 3 and 4 pairs of products below 2^16 each,
+// every partial sum a multiple of 2^-8d below 2^(24-8d)); every d >= 4 shares
+// one set, whose terms weigh below 2^-32 of the leading ones, so its FP32
+// rounding stays near 2^-56 of the result.
+// The rounding part is kept in two sets (d = 4 and d >= 5) so no
+// accumulator receives a long chain of dependent MFMAs within a slab.
+__host__ __device__ constexpr int set_of(int d) { return d < 5 ? d : 5; }
+__host__ __device__ constexpr int nsets(int s) { return s < 6 ? s : 6; }
+// fold period of each set, in 16-deep k slabs
+__host__ __device__ constexpr int fold_every(int set)
 {
-    int h = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-    while (h * (h + 1) / 2 > b) --h;
-    while ((h + 1) * (h + 2) / 2 <= b) ++h;
-    hi = h;
-    lo = b - h * (h + 1) / 2;
+    return set == 0 ? 16 : set == 1 ? 8 : set >= 4 ? 16 : 4;
 }
 
-__global__ __launch_bounds__(256) void ozaki_gemm_kernel(OzArgs g)
+// One 256-thread workgroup per 128 x 64 tile of C, 64 x 32 per wave.  Per
+// 16-deep k slab all S slices of the A and B tiles arrive in LDS by direct
+// HBM->LDS loads through a ring of up to four stages (two or three slabs in
+// flight while one is multiplied); each wave keeps the fragments of every B slice in registers
+// and runs the pairs p + q <= S - 1 with v_mfma_f32_32x32x16_bf16, each into
+// the accumulator set of its diagonal.  Sets are folded into FP64 on their
+// own period; the epilogue applies 2^(ea_i + eb_j - 16), alpha and beta.
+// LDS stages of the slab pipeline: as many 16-deep slabs of all S slices as
+// fit in 144 KB, at most four.
+__host__ __device__ constexpr int oz_stages(int s, int tn)
 {
-    __shared__ __attribute__((aligned(16))) bf16 As[2][OZ_BM][OZ_LDSK];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[2][OZ_BN][OZ_LDSK];
+    return (147456 / (s * (128 + tn) * 32)) < 4 ? (147456 / (s * (128 + tn) * 32)) : 4;
+}
+
+// Tile of workgroup b.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2: XCD x gets the contiguous range of tile ids
+// x*T/8 .. (x+1)*T/8 (when T divides), walked in groups of 8 tile rows, so
+// the workgroups one XCD runs together share 8 A row-tiles and a few B
+// column-tiles of every k slab instead of streaming them from HBM each.
+__device__ __forceinline__ void tile_of(int b, int TM, int TN, int &tm, int &tn)
+{
+    const int T = TM * TN;
+    const int t = (T & 7) == 0 ? (b & 7) * (T >> 3) + (b >> 3) : b;
+    constexpr int GM = 8;
+    const int per_group = GM * TN;
+    const int first = (t / per_group) * GM;
+    const int gs = min(TM - first, GM);
+    const int r = t % per_group;
+    tm = first + r % gs;
+    tn = r / gs;
+}
+
+// s_waitcnt immediate: wait until at most n vector-memory operations of this
+// wave are outstanding (LDS and export counters left at their maxima)
+__host__ __device__ constexpr int vm_wait(int n)
+{
+    return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
+}
+
+// one pipeline stage of the slice product: slab kt of every A and B slice
+template <int NSL, int TN> struct OzStage {
+    __attribute__((aligned(16))) bf16 a[NSL][128][16];
+    __attribute__((aligned(16))) bf16 b[NSL][TN][16];
+};
+
+template <int S, int WNT, int OCC>
+__global__ __launch_bounds__(256, OCC) void ozaki_gemm_kernel(OzArgs g)
+{
+    constexpr int NS = nsets(S);
+    constexpr int TN = 64 * WNT;  // tile columns
+    constexpr int D = oz_stages(S, TN);
+    static_assert(TN == 64 && D >= 2 && D <= 4 && 2 * S * (D - 1) < 64, "slab pipeline shape");
+    // one variable per stage, so the compiler can tell the slab being read
+    // from the slabs still arriving (a shared array would make every LDS read
+    // wait for all loads in flight; so would more than a few variables, which
+    // is why A and B of a stage share one)
+    __shared__ OzStage<S, TN> L0, L1;
+    __shared__ OzStage<(D > 2 ? S : 1), TN> L2;
+    __shared__ OzStage<(D > 3 ? S : 1), TN> L3;
+    auto abuf = [&](auto jc) -> bf16(*)[128][16] {
+        constexpr int J = decltype(jc)::value;
+        if constexpr (J == 0) return L0.a;
+        else if constexpr (J == 1) return L1.a;
+        else if constexpr (J == 2) return L2.a;
+        else return L3.a;
+    };
+    auto bbuf = [&](auto jc) -> bf16(*)[TN][16] {
+        constexpr int J = decltype(jc)::value;
+        if constexpr (J == 0) return L0.b;
+        else if constexpr (J == 1) return L1.b;
+        else if constexpr (J == 2) return L2.b;
+        else return L3.b;
+    };
+
     int tm, tn;
-    if (g.tri == 0) {
-        tm = blockIdx.x % g.tiles_m;
-        tn = blockIdx.x / g.tiles_m;
-    } else {
-        int hi, lo;
-        tri_decode(blockIdx.x, hi, lo);
-        if (g.tri == 1) { tm = hi; tn = lo; } else { tm = lo; tn = hi; }
-    }
-    const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
+    tile_of(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
+    const int m0 = tm * 128, n0 = tn * TN;
+    // SYRK: tiles wholly outside the stored triangle have nothing to do
+    if (g.tri == 1 && n0 > m0 + 127) return;
+    if (g.tri == 2 && n0 + TN - 1 < m0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
 
-    v16f acc32[2][2];
-    double acc64[2][2][16];
+    v16f acc[NS][2][WNT];
+    double acc64[2][WNT][16];
+#pragma unroll
+    for (int c = 0; c < NS; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < WNT; ++j) acc[c][i][j] = v16f{};
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            acc32[i][j] = v16f{};
+        for (int j = 0; j < WNT; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
-        }
 
-    const int nkt_total = g.Kp / OZ_BKT;               // k-tiles over all of K
-    const int kt_per_chunk = OZ_CHUNK / OZ_BKT;
-    const int nchunks = (nkt_total + kt_per_chunk - 1) / kt_per_chunk;
-    // flattened step list: chunk-major, then pair, then k-tile within the chunk
-    auto chunk_tiles = [&](int c) { return min(kt_per_chunk, nkt_total - c * kt_per_chunk); };
-
-    v8s ra[2], rb[2];
-    auto load = [&](int c, int pr, int kt) {
-        const int k = (c * kt_per_chunk + kt) * OZ_BKT;
-        const bf16 *sa = g.SA + (long)g.pp[pr] * g.M * g.Kp;
-        const bf16 *sb = g.SB + (long)g.pq[pr] * g.N * g.Kp;
+    const int KT = g.KT;
+    // wave w moves rows 32w..32w+31 of every A slice; waves 0 and 1 also
+    // rows 32w..32w+31 of every B slice (16 B per lane, 1 KB per load)
+    const bool loads_b = 32 * wave < TN;
+    const long sa0 = (long)(m0 + 32 * wave) * 16 + lane * 8;
+    const long sb0 = (long)(n0 + (loads_b ? 32 * wave : 0)) * 16 + lane * 8;
+    auto issue = [&](int kt, auto jc) {
+        bf16(*la)[128][16] = abuf(jc);
+        bf16(*lb)[TN][16] = bbuf(jc);
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int v = threadIdx.x + r * 256;
-            const int row = v >> 2, part = v & 3;
-            const long ga = m0 + row, gb = n0 + row;
-            ra[r] = ga < g.M ? *(const v8s *)(sa + ga * g.Kp + k + part * 8) : v8s{};
-            rb[r] = gb < g.N ? *(const v8s *)(sb + gb * g.Kp + k + part * 8) : v8s{};
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int v = threadIdx.x + r * 256;
-            const int row = v >> 2, part = v & 3;
-            *(v8s *)&As[buf][row][part * 8] = ra[r];
-            *(v8s *)&Bs[buf][row][part * 8] = rb[r];
-        }
-    };
-
-    int c = 0, pr = 0, kt = 0, buf = 0;
-    load(0, 0, 0);
-    store(0);
-    __syncthreads();
-    while (true) {
-        // next step
-        int c1 = c, pr1 = pr, kt1 = kt + 1;
-        if (kt1 == chunk_tiles(c1)) {
-            kt1 = 0;
-            if (++pr1 == g.npairs) {
-                pr1 = 0;
-                ++c1;
+        for (int p = 0; p < S; ++p) {
+            const bf16 *sa = g.SA + ((long)p * KT + kt) * g.Mp * 16 + sa0;
+            __builtin_amdgcn_global_load_lds((gptr_t)sa, (lptr_t)&la[p][32 * wave][0], 16, 0, 0);
+            if (loads_b) {
+                const bf16 *sb = g.SB + ((long)p * KT + kt) * g.Np * 16 + sb0;
+                __builtin_amdgcn_global_load_lds((gptr_t)sb, (lptr_t)&lb[p][32 * wave][0], 16, 0,
+                                                 0);
             }
         }
-        const bool more = c1 < nchunks;
-        if (more) load(c1, pr1, kt1);
+    };
+    auto fold = [&](int c) {
 #pragma unroll
-        for (int kk = 0; kk < OZ_BKT; kk += 16) {
-            v8bf fa[2], fb[2];
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < WNT; ++j) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc64[i][j][r] += (double)acc[c][i][j][r];
+                acc[c][i][j] = v16f{};
+            }
+    };
+    // one slab: wait for it, barrier, refill the stage freed by the previous
+    // slab, multiply all pairs, fold the sets whose chunk ends here
+    auto step = [&](int kt, auto jc) {
+        constexpr int J = decltype(jc)::value;
+        constexpr int JN = (J + D - 1) % D;  // stage of slab kt + D - 1
+        // this wave's loads of slab kt are done when at most the loads of the
+        // D-2 later slabs are outstanding (2 per slice for B loaders, else 1)
+        if (kt + D - 2 >= KT) __builtin_amdgcn_s_waitcnt(vm_wait(0));
+        else if (loads_b) __builtin_amdgcn_s_waitcnt(vm_wait(2 * S * (D - 2)));
+        else __builtin_amdgcn_s_waitcnt(vm_wait(S * (D - 2)));
+        // every wave's part of slab kt is in LDS and every wave is done with
+        // slab kt-1.  A bare barrier: the fence of __syncthreads would also
+        // drain the slabs still in flight.
+        __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));  // lgkmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (kt + D - 1 < KT) issue(kt + D - 1, std::integral_constant<int, JN>{});
+        bf16(*as)[128][16] = abuf(jc);
+        bf16(*bs)[TN][16] = bbuf(jc);
+        v8bf fb[S][WNT];
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+            for (int j = 0; j < WNT; ++j)
+                fb[q][j] = *(const v8bf *)&bs[q][wn * 32 * WNT + j * 32 + (lane & 31)][8 * (lane >> 5)];
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            v8bf fa[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                fa[i] = *(const v8bf *)&As[buf][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+                fa[i] = *(const v8bf *)&as[p][wm * 64 + i * 32 + (lane & 31)][8 * (lane >> 5)];
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                fb[j] = *(const v8bf *)&Bs[buf][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+            for (int q = 0; q + p < S; ++q) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc32[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[j], fa[i], acc32[i][j],
-                                                                          0, 0, 0);
+                    for (int j = 0; j < WNT; ++j)
+                        acc[set_of(p + q)][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            fb[q][j], fa[i], acc[set_of(p + q)][i][j], 0, 0, 0);
+            }
         }
-        if (kt + 1 == chunk_tiles(c)) {
-            // end of this (chunk, pair): fold the exact FP32 integers into FP64
-            const double sc = ldexp(1.0, -8 * (g.pp[pr] + g.pq[pr] + 2));
+        const bool last = kt + 1 == KT;
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) acc64[i][j][r] = fma((double)acc32[i][j][r], sc, acc64[i][j][r]);
-                    acc32[i][j] = v16f{};
-                }
-        }
-        if (!more) break;
-        store(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
-        c = c1;
-        pr = pr1;
-        kt = kt1;
+        for (int c = 0; c < NS; ++c)
+            if (last || (kt + 1) % fold_every(c) == 0) fold(c);
+    };
+
+    // slabs kt+1 .. kt+D-2 stay in flight while slab kt is multiplied
+    issue(0, std::integral_constant<int, 0>{});
+    if (1 < KT) issue(1, std::integral_constant<int, 1>{});
+    if constexpr (D > 2) if (2 < KT) issue(2, std::integral_constant<int, 2>{});
+    // the first slab is peeled so that the loop head is not the stage-0 step:
+    // there the compiler cannot tell stage 0 from the stages still arriving
+    // and drains every load in flight
+    step(0, std::integral_constant<int, 0>{});
+    for (int kt = 1; kt < KT; kt += D) {
+        step(kt, std::integral_constant<int, 1>{});
+        if constexpr (D > 2) if (kt + 1 < KT) step(kt + 1, std::integral_constant<int, 2>{});
+        if constexpr (D > 3) if (kt + 2 < KT) step(kt + 2, std::integral_constant<int, 3>{});
+        if (kt + D - 1 < KT) step(kt + D - 1, std::integral_constant<int, 0>{});
     }
 
     // epilogue: element r of tile (i, j) is C[row][col] with
-    //   row = m0 + wm*64 + i*32 + lane%32, col = n0 + wn*64 + j*32 + 8*(r/4) + 4*(lane/32) + r%4
+    //   row = m0 + wm*64 + i*32 + lane%32,
+    //   col = n0 + wn*32*WNT + j*32 + 8*(r/4) + 4*(lane/32) + r%4
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const long gi = m0 + wm * 64 + i * 32 + (lane & 31);
         if (gi >= g.M) continue;
         const int ei = g.ea[gi];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < WNT; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const long gj = n0 + wn * 64 + j * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const long gj = n0 + wn * 32 * WNT + j * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (gj >= g.N) continue;
                 if (g.tri == 1 && gi < gj) continue;
                 if (g.tri == 2 && gi > gj) continue;
                 double *cp = g.C + gi + gj * g.ldc;
-                const double v = g.alpha * ldexp(acc64[i][j][r], ei + g.eb[gj]);
+                const double v = g.alpha * ldexp(acc64[i][j][r], ei + g.eb[gj] - 16);
                 *cp = g.beta == 0.0 ? v : fma(g.beta, *cp, v);
             }
     }
@@ -271,14 +377,16 @@ inline bool isN(char c) { return c == 'N' || c == 'n'; }
 struct Split {
     bf16 *S;
     int *e;
+    int Rp;
 };
 
 // Exponents and slices of an R x K operand given by (X, ld, rcontig).
-int make_split(hipStream_t st, int R, int K, int Kp, int s, const double *X, long ld, int rcontig,
+int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, long ld, int rcontig,
                Split &out, int *bad)
 {
+    out.Rp = ceil_div(R, 128) * 128;
     out.e = (int *)gpublas_dev_alloc((size_t)R * sizeof(int));
-    out.S = (bf16 *)gpublas_dev_alloc((size_t)s * R * Kp * sizeof(bf16));
+    out.S = (bf16 *)gpublas_dev_alloc((size_t)s * KT * out.Rp * 16 * sizeof(bf16));
     if (!out.e || !out.S) return (int)hipErrorOutOfMemory;
     PROF_BEGIN(st, "ozaki_split");
     if (rcontig)
@@ -287,20 +395,27 @@ int make_split(hipStream_t st, int R, int K, int Kp, int s, const double *X, lon
     else
         hipLaunchKernelGGL(exp_rows_contig, dim3((unsigned)ceil_div(R, 4)), dim3(256), 0, st, R, K,
                            X, ld, out.e, bad);
-    hipLaunchKernelGGL(split_kernel, dim3((unsigned)ceil_div(R, 32), (unsigned)ceil_div(Kp, 64)),
-                       dim3(256), 0, st, R, K, Kp, s, X, ld, rcontig, out.e, out.S);
+    hipLaunchKernelGGL(split_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 4)),
+                       dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, out.S);
     PROF_END(st);
     return (int)hipGetLastError();
 }
 
-int run_products(hipStream_t st, int M, int N, int Kp, int s, const Split &a, const Split &b,
+template <int S>
+void launch_products(hipStream_t st, unsigned blocks, const OzArgs &g)
+{
+    hipLaunchKernelGGL((ozaki_gemm_kernel<S, OZ_WNT, OZ_OCC>), dim3(blocks), dim3(256), 0, st, g);
+}
+
+int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, const Split &b,
                  double alpha, double beta, double *C, int ldc, int tri)
 {
     OzArgs g;
     g.M = M;
     g.N = N;
-    g.Kp = Kp;
-    g.s = s;
+    g.Mp = a.Rp;
+    g.Np = b.Rp;
+    g.KT = KT;
     g.SA = a.S;
     g.SB = b.S;
     g.ea = a.e;
@@ -310,19 +425,20 @@ int run_products(hipStream_t st, int M, int N, int Kp, int s, const Split &a, co
     g.C = C;
     g.ldc = ldc;
     g.tri = tri;
-    g.npairs = 0;
-    for (int p = 0; p < s; ++p)
-        for (int q = 0; q < s; ++q)
-            if (p + q <= s - 1) {  // 1-based p' + q' <= s + 1
-                g.pp[g.npairs] = (unsigned char)p;
-                g.pq[g.npairs] = (unsigned char)q;
-                ++g.npairs;
-            }
-    g.tiles_m = ceil_div(M, OZ_BM);
-    const long tn = ceil_div(N, OZ_BN);
-    const long blocks = tri ? (long)g.tiles_m * (g.tiles_m + 1) / 2 : (long)g.tiles_m * tn;
+    g.tiles_m = ceil_div(M, 128);
+    g.tiles_n = ceil_div(N, 64 * OZ_WNT);
+    const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
     PROF_BEGIN(st, "ozaki_gemm");
-    hipLaunchKernelGGL(ozaki_gemm_kernel, dim3((unsigned)blocks), dim3(256), 0, st, g);
+    switch (s) {
+    case 1: launch_products<1>(st, blocks, g); break;
+    case 2: launch_products<2>(st, blocks, g); break;
+    case 3: launch_products<3>(st, blocks, g); break;
+    case 4: launch_products<4>(st, blocks, g); break;
+    case 5: launch_products<5>(st, blocks, g); break;
+    case 6: launch_products<6>(st, blocks, g); break;
+    case 7: launch_products<7>(st, blocks, g); break;
+    default: launch_products<8>(st, blocks, g); break;
+    }
     PROF_END(st);
     return (int)hipGetLastError();
 }
@@ -346,14 +462,14 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
     if (slices < 1 || slices > OZ_MAXS) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
-    const int Kp = ceil_div(k, OZ_BKT) * OZ_BKT;
+    const int KT = ceil_div(k, 16);
     int *bad = (int *)dev_workspace(sizeof(int));
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{}, b{};
     // op(A) rows: A 'N' -> X[r + k*lda] (rows contiguous); 'T' -> X[k + r*lda]
-    int err = make_split(st, m, k, Kp, slices, d_A, lda, isN(transa), a, bad);
+    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), a, bad);
     // op(B) columns: B 'N' -> element (k, j) at B[k + j*ldb] (k contiguous); 'T' -> B[j + k*ldb]
-    if (!err) err = make_split(st, n, k, Kp, slices, d_B, ldb, !isN(transb), b, bad);
+    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), b, bad);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
@@ -361,7 +477,7 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
         if (hbad)  // Inf / NaN input: the splitting is undefined, use the FP64 kernel
             err = gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
         else
-            err = run_products(st, m, n, Kp, slices, a, b, alpha, beta, d_C, ldc, 0);
+            err = run_products(st, m, n, KT, slices, a, b, alpha, beta, d_C, ldc, 0);
     }
     (void)hipStreamSynchronize(st);
     gpublas_dev_release(a.S);
@@ -380,18 +496,18 @@ int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int
     if (slices < 1 || slices > OZ_MAXS) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
-    const int Kp = ceil_div(k, OZ_BKT) * OZ_BKT;
+    const int KT = ceil_div(k, 16);
     int *bad = (int *)dev_workspace(sizeof(int));
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{};
     // op(A) = A (trans N: rows contiguous) or A^T; op(B) = op(A)^T has the same lines
-    int err = make_split(st, n, k, Kp, slices, d_A, lda, isN(trans), a, bad);
+    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), a, bad);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
     if (!err) {
         if (hbad) err = gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
-        else err = run_products(st, n, n, Kp, slices, a, a, alpha, beta, d_C, ldc,
+        else err = run_products(st, n, n, KT, slices, a, a, alpha, beta, d_C, ldc,
                                 (uplo == 'U' || uplo == 'u') ? 2 : 1);
     }
     (void)hipStreamSynchronize(st);
