@@ -13,18 +13,20 @@
 //  3. the slice products with p + q <= s - 1 run on v_mfma_f32_32x32x16_bf16
 //     and accumulate in FP32 by diagonal d = p + q.  Diagonals 0 to 3 are
 //     exact over k chunks of 256, 128, 64 and 64 (every partial sum is a
-//     multiple of 2^-8d below 2^(24-8d)); the diagonals from 4 on, whose
-//     weight is below 2^-32 of the leading term, share one rounding
-//     accumulator;
+//     multiple of 2^-8d below 2^(24-8d)); diagonal 4 and the diagonals
+//     from 5 on, whose weight is below 2^-32 of the leading term, have one
+//     rounding accumulator each;
 //  4. each accumulator is added into FP64 at the end of its chunk; the
 //     epilogue applies 2^(ea_i + eb_j - 16), alpha and beta in FP64.
 // The approximations are the dropped pairs (p + q >= s), the slice
-// remainders, the tail accumulator's FP32 rounding and the FP64 additions.
+// remainders, the FP32 rounding of the tail accumulators and the FP64
+// additions.
 // FP64 is used exactly where the scheme needs it: the scaling/splitting pass
 // and the final summation.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 #include <math.h>
+#include <algorithm>
 #include <type_traits>
 
 namespace {
@@ -32,12 +34,17 @@ namespace {
 typedef __bf16 bf16;
 typedef short v8s __attribute__((ext_vector_type(8)));  // 16 B of bf16 for loads
 constexpr int OZ_MAXS = 8;
-#ifndef OZ_WNT
-#define OZ_WNT 1  // 32-column MFMA tiles per wave (tile N = 64 * OZ_WNT)
+#ifndef OZ_TM
+#define OZ_TM 64   // rows of a product tile (64 or 128; tiles are 64 columns wide)
 #endif
-#ifndef OZ_OCC
-#define OZ_OCC 1  // workgroups per CU the register budget is sized for
+#ifndef OZ_NW
+#define OZ_NW 4    // waves of a product workgroup (4 or 8)
 #endif
+// A wave owning one 32 x 32 MFMA tile keeps its accumulators near 200
+// registers, so two waves share a SIMD and hide each other's waits: 8 waves
+// on a 128-row tile, or two 4-wave workgroups of 64-row tiles per CU (which
+// move a third more bytes from L2 per flop).
+constexpr int kOzOcc = OZ_TM == 16 * OZ_NW ? 8 / OZ_NW : 1;  // two waves per SIMD
 
 struct OzArgs {
     int M, N, Mp, Np, KT;   // Mp, Np: rows padded to 128; KT: 16-deep k slabs
@@ -62,17 +69,30 @@ __device__ __forceinline__ int exp_of(double mx)
 }
 
 // line r of an operand stored so that consecutive r are contiguous
-// (element (r, k) at X[r + k*ld]): one thread per line, coalesced over r.
-__global__ void exp_rows_strided(int R, int K, const double *X, long ld, int *e, int *bad)
+// (element (r, k) at X[r + k*ld]): thread per line, coalesced over r, with K
+// cut into gridDim.y chunks so the pass fills the chip; chunk c writes the
+// partial maximum of line r to part[c*R + r].
+__global__ void exp_rows_strided(int R, int K, const double *X, long ld, double *part, int *bad)
 {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
+    const int chunk = (K + gridDim.y - 1) / gridDim.y;
+    const int k0 = blockIdx.y * chunk, k1 = min(K, k0 + chunk);
     double mx = 0;
-    for (int k = 0; k < K; ++k) {
+    for (int k = k0; k < k1; ++k) {
         const double v = fabs(X[r + (long)k * ld]);
         if (!(v <= 1.7976931348623157e308)) *bad = 1;
         mx = fmax(mx, v);
     }
+    part[(long)blockIdx.y * R + r] = mx;
+}
+
+__global__ void exp_rows_finish(int R, int chunks, const double *part, int *e)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    double mx = 0;
+    for (int c = 0; c < chunks; ++c) mx = fmax(mx, part[(long)c * R + r]);
     e[r] = exp_of(mx);
 }
 
@@ -145,11 +165,10 @@ typedef __attribute__((address_space(3))) void *lptr_t;
 
 // Accumulator set of diagonal d: d = 0, 1, 2, 3 are exact in FP32 over chunks
 // of 256, 128, 64 and 64 k (1, 2, 3 and 4 pairs of products below 2^16 each,
-// every partial sum a multiple of 2^-8d below 2^(24-8d)); every d >= 4 shares
-// one set, whose terms weigh below 2^-32 of the leading ones, so its FP32
-// rounding stays near 2^-56 of the result.
-// The rounding part is kept in two sets (d = 4 and d >= 5) so no
-// accumulator receives a long chain of dependent MFMAs within a slab.
+// every partial sum a multiple of 2^-8d below 2^(24-8d)); d = 4 and d >= 5
+// round, but their terms weigh below 2^-32 of the leading ones, so their FP32
+// rounding stays near 2^-56 of the result.  Two rounding sets rather than one
+// keep the dependent MFMA chains of a slab short.
 __host__ __device__ constexpr int set_of(int d) { return d < 5 ? d : 5; }
 __host__ __device__ constexpr int nsets(int s) { return s < 6 ? s : 6; }
 // fold period of each set, in 16-deep k slabs
@@ -158,18 +177,22 @@ __host__ __device__ constexpr int fold_every(int set)
     return set == 0 ? 16 : set == 1 ? 8 : set >= 4 ? 16 : 4;
 }
 
-// One 256-thread workgroup per 128 x 64 tile of C, 64 x 32 per wave.  Per
-// 16-deep k slab all S slices of the A and B tiles arrive in LDS by direct
-// HBM->LDS loads through a ring of up to four stages (two or three slabs in
-// flight while one is multiplied); each wave keeps the fragments of every B slice in registers
-// and runs the pairs p + q <= S - 1 with v_mfma_f32_32x32x16_bf16, each into
-// the accumulator set of its diagonal.  Sets are folded into FP64 on their
-// own period; the epilogue applies 2^(ea_i + eb_j - 16), alpha and beta.
+// One workgroup per TM x 64 tile of C, one 32 x 32 MFMA tile per wave (8
+// waves on 128 rows, or 4 on 64 rows with two workgroups per CU), so two
+// waves share each SIMD.  Per 16-deep k slab all S slices of the A and B
+// tiles arrive in LDS by direct HBM->LDS loads through a ring of up to four
+// stages (one to three slabs in flight while one is multiplied); each wave
+// reads the fragments of every B slice once and runs the pairs p + q <= S - 1
+// with v_mfma_f32_32x32x16_bf16, each into the accumulator set of its
+// diagonal.  Sets are folded into FP64 on their own period; the epilogue
+// applies 2^(ea_i + eb_j - 16), alpha and beta.
 // LDS stages of the slab pipeline: as many 16-deep slabs of all S slices as
-// fit in 144 KB, at most four.
-__host__ __device__ constexpr int oz_stages(int s, int tn)
+// fit in the LDS share of one of OCC workgroups, at most four.
+__host__ __device__ constexpr int oz_stages(int s, int tm, int occ)
 {
-    return (147456 / (s * (128 + tn) * 32)) < 4 ? (147456 / (s * (128 + tn) * 32)) : 4;
+    return ((occ == 1 ? 147456 : 163840 / occ) / (s * (tm + 64) * 32)) < 4
+               ? ((occ == 1 ? 147456 : 163840 / occ) / (s * (tm + 64) * 32))
+               : 4;
 }
 
 // Tile of workgroup b.  Workgroups are dispatched round-robin over the 8
@@ -197,178 +220,214 @@ __host__ __device__ constexpr int vm_wait(int n)
     return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
 }
 
-// one pipeline stage of the slice product: slab kt of every A and B slice
-template <int NSL, int TN> struct OzStage {
-    __attribute__((aligned(16))) bf16 a[NSL][128][16];
-    __attribute__((aligned(16))) bf16 b[NSL][TN][16];
+// one pipeline stage of the slice product: slab kt of every A slice (rows
+// 0..TM-1) and B slice (rows TM..TM+TN-1)
+template <int NSL, int TM, int TN> struct OzStage {
+    __attribute__((aligned(16))) bf16 r[NSL][TM + TN][16];
 };
 
-template <int S, int WNT, int OCC>
-__global__ __launch_bounds__(256, OCC) void ozaki_gemm_kernel(OzArgs g)
+template <int S, int TM, int NW, int OCC>
+__global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
 {
     constexpr int NS = nsets(S);
-    constexpr int TN = 64 * WNT;  // tile columns
-    constexpr int D = oz_stages(S, TN);
-    static_assert(TN == 64 && D >= 2 && D <= 4 && 2 * S * (D - 1) < 64, "slab pipeline shape");
+    constexpr int TN = 64;             // tile columns: two columns of waves, 32 each
+    constexpr int MW = NW / 2;         // waves down the tile
+    constexpr int MI = TM / (32 * MW); // 32-row MFMA tiles per wave
+    constexpr int D = oz_stages(S, TM, OCC);
+    constexpr int P = (TM + TN) / 32;  // 1 KB pieces of one slice slab
+    constexpr int PW = (P + NW - 1) / NW;  // pieces per wave (waves w, w+NW, ...)
+    static_assert((TM == 64 || TM == 128) && MI >= 1 && D >= 2 && D <= 4 &&
+                      PW * S * D < 64,
+                  "slab pipeline shape");
     // one variable per stage, so the compiler can tell the slab being read
     // from the slabs still arriving (a shared array would make every LDS read
     // wait for all loads in flight; so would more than a few variables, which
     // is why A and B of a stage share one)
-    __shared__ OzStage<S, TN> L0, L1;
-    __shared__ OzStage<(D > 2 ? S : 1), TN> L2;
-    __shared__ OzStage<(D > 3 ? S : 1), TN> L3;
-    auto abuf = [&](auto jc) -> bf16(*)[128][16] {
+    __shared__ OzStage<S, TM, TN> L0, L1;
+    __shared__ OzStage<(D > 2 ? S : 1), TM, TN> L2;
+    __shared__ OzStage<(D > 3 ? S : 1), TM, TN> L3;
+    auto stage = [&](auto jc) -> bf16(*)[TM + TN][16] {
         constexpr int J = decltype(jc)::value;
-        if constexpr (J == 0) return L0.a;
-        else if constexpr (J == 1) return L1.a;
-        else if constexpr (J == 2) return L2.a;
-        else return L3.a;
-    };
-    auto bbuf = [&](auto jc) -> bf16(*)[TN][16] {
-        constexpr int J = decltype(jc)::value;
-        if constexpr (J == 0) return L0.b;
-        else if constexpr (J == 1) return L1.b;
-        else if constexpr (J == 2) return L2.b;
-        else return L3.b;
+        if constexpr (J == 0) return L0.r;
+        else if constexpr (J == 1) return L1.r;
+        else if constexpr (J == 2) return L2.r;
+        else return L3.r;
     };
 
     int tm, tn;
     tile_of(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
-    const int m0 = tm * 128, n0 = tn * TN;
+    const int m0 = tm * TM, n0 = tn * TN;
     // SYRK: tiles wholly outside the stored triangle have nothing to do
-    if (g.tri == 1 && n0 > m0 + 127) return;
+    if (g.tri == 1 && n0 > m0 + TM - 1) return;
     if (g.tri == 2 && n0 + TN - 1 < m0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % MW, wn = wave / MW;
 
-    v16f acc[NS][2][WNT];
-    double acc64[2][WNT][16];
+    v16f acc[NS][MI];
+    double acc64[MI][16];
 #pragma unroll
     for (int c = 0; c < NS; ++c)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < MI; ++i) acc[c][i] = v16f{};
 #pragma unroll
-            for (int j = 0; j < WNT; ++j) acc[c][i][j] = v16f{};
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < WNT; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc64[i][j][r] = 0.0;
+        for (int r = 0; r < 16; ++r) acc64[i][r] = 0.0;
 
     const int KT = g.KT;
-    // wave w moves rows 32w..32w+31 of every A slice; waves 0 and 1 also
-    // rows 32w..32w+31 of every B slice (16 B per lane, 1 KB per load)
-    const bool loads_b = 32 * wave < TN;
-    const long sa0 = (long)(m0 + 32 * wave) * 16 + lane * 8;
-    const long sb0 = (long)(n0 + (loads_b ? 32 * wave : 0)) * 16 + lane * 8;
-    auto issue = [&](int kt, auto jc) {
-        bf16(*la)[128][16] = abuf(jc);
-        bf16(*lb)[TN][16] = bbuf(jc);
+    // stage rows 0..TM-1 hold the A tile, TM..TM+TN-1 the B tile; piece
+    // x (32 rows, 1 KB per slice, 16 B per lane) is moved by wave x % NW.
+    // The two 16-byte halves of stage row r are swapped when bit 3 of r is
+    // set, so the 16 lanes of a fragment read hit 64 distinct banks (rows r
+    // and r + 8 would share them): a DMA lane writes a fixed LDS slot, so the
+    // swap is made on the global side.
+    const int half_src = (lane & 1) ^ ((lane >> 4) & 1);
+    long src0[PW];
+    bool mine[PW], isa[PW];
 #pragma unroll
-        for (int p = 0; p < S; ++p) {
-            const bf16 *sa = g.SA + ((long)p * KT + kt) * g.Mp * 16 + sa0;
-            __builtin_amdgcn_global_load_lds((gptr_t)sa, (lptr_t)&la[p][32 * wave][0], 16, 0, 0);
-            if (loads_b) {
-                const bf16 *sb = g.SB + ((long)p * KT + kt) * g.Np * 16 + sb0;
-                __builtin_amdgcn_global_load_lds((gptr_t)sb, (lptr_t)&lb[p][32 * wave][0], 16, 0,
-                                                 0);
+    for (int u = 0; u < PW; ++u) {
+        const int x = wave + NW * u;
+        mine[u] = x < P;
+        isa[u] = 32 * x < TM;
+        src0[u] = (isa[u] ? (long)(m0 + 32 * x) : (long)(n0 + 32 * x - TM)) * 16 +
+                  (lane >> 1) * 16 + half_src * 8;
+    }
+    int nmine = 0;
+#pragma unroll
+    for (int u = 0; u < PW; ++u) nmine += mine[u] ? 1 : 0;
+    auto issue = [&](int kt, auto jc) {
+        bf16(*ls)[TM + TN][16] = stage(jc);
+#pragma unroll
+        for (int p = 0; p < S; ++p)
+#pragma unroll
+            for (int u = 0; u < PW; ++u) {
+                if (!mine[u]) continue;
+                const bf16 *src = isa[u] ? g.SA + ((long)p * KT + kt) * g.Mp * 16 + src0[u]
+                                         : g.SB + ((long)p * KT + kt) * g.Np * 16 + src0[u];
+                __builtin_amdgcn_global_load_lds((gptr_t)src, (lptr_t)&ls[p][32 * (wave + NW * u)][0],
+                                                 16, 0, 0);
             }
-        }
     };
     auto fold = [&](int c) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < MI; ++i) {
 #pragma unroll
-            for (int j = 0; j < WNT; ++j) {
-#pragma unroll
-                for (int r = 0; r < 16; ++r) acc64[i][j][r] += (double)acc[c][i][j][r];
-                acc[c][i][j] = v16f{};
-            }
+            for (int r = 0; r < 16; ++r) acc64[i][r] += (double)acc[c][i][r];
+            acc[c][i] = v16f{};
+        }
     };
-    // one slab: wait for it, barrier, refill the stage freed by the previous
-    // slab, multiply all pairs, fold the sets whose chunk ends here
-    auto step = [&](int kt, auto jc) {
-        constexpr int J = decltype(jc)::value;
-        constexpr int JN = (J + D - 1) % D;  // stage of slab kt + D - 1
-        // this wave's loads of slab kt are done when at most the loads of the
-        // D-2 later slabs are outstanding (2 per slice for B loaders, else 1)
-        if (kt + D - 2 >= KT) __builtin_amdgcn_s_waitcnt(vm_wait(0));
-        else if (loads_b) __builtin_amdgcn_s_waitcnt(vm_wait(2 * S * (D - 2)));
-        else __builtin_amdgcn_s_waitcnt(vm_wait(S * (D - 2)));
-        // every wave's part of slab kt is in LDS and every wave is done with
-        // slab kt-1.  A bare barrier: the fence of __syncthreads would also
-        // drain the slabs still in flight.
+    // fragments of one slab: every B slice (column block wn) and every A
+    // slice (MI row blocks of wm), with the swapped half of each row
+    struct Frag {
+        v8bf b[S];
+        v8bf a[S][MI];
+    };
+    const int hr = 8 * ((lane >> 5) ^ ((lane >> 3) & 1));
+    auto read_b = [&](Frag &f, int q, auto jc) {
+        bf16(*ls)[TM + TN][16] = stage(jc);
+        f.b[q] = *(const v8bf *)&ls[q][TM + wn * 32 + (lane & 31)][hr];
+    };
+    auto read_a = [&](Frag &f, int p, auto jc) {
+        bf16(*ls)[TM + TN][16] = stage(jc);
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+            f.a[p][i] = *(const v8bf *)&ls[p][wm * 32 * MI + i * 32 + (lane & 31)][hr];
+    };
+    // wait until this wave's loads of the slab after the current one are in
+    // LDS: at most `later` younger slabs (nmine loads per slice) outstanding
+    auto wait_slab = [&](bool drain, auto later_c) {
+        constexpr int LATER = decltype(later_c)::value;
+        if (drain) __builtin_amdgcn_s_waitcnt(vm_wait(0));
+        else if (nmine == PW) __builtin_amdgcn_s_waitcnt(vm_wait(PW * S * LATER));
+        else __builtin_amdgcn_s_waitcnt(vm_wait((PW - 1) * S * LATER));
+    };
+    // A bare barrier after this wave's LDS reads are back: the fence of
+    // __syncthreads would also drain the slabs still in flight.
+    auto barrier = [&]() {
         __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));  // lgkmcnt(0)
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_barrier();
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (kt + D - 1 < KT) issue(kt + D - 1, std::integral_constant<int, JN>{});
-        bf16(*as)[128][16] = abuf(jc);
-        bf16(*bs)[TN][16] = bbuf(jc);
-        v8bf fb[S][WNT];
-#pragma unroll
-        for (int q = 0; q < S; ++q)
-#pragma unroll
-            for (int j = 0; j < WNT; ++j)
-                fb[q][j] = *(const v8bf *)&bs[q][wn * 32 * WNT + j * 32 + (lane & 31)][8 * (lane >> 5)];
+    };
+    Frag cur, nxt;
+    // one slab kt, its fragments already in `cur`: wait for slab kt+1,
+    // barrier (every wave has its fragments of slab kt, so that stage is
+    // free), refill it with slab kt+D, read the fragments of slab kt+1 while
+    // multiplying all pairs of slab kt, fold the sets whose chunk ends here
+    auto step = [&](int kt, auto jc) {
+        constexpr int J = decltype(jc)::value;
+        constexpr int J1 = (J + 1) % D;  // stage of slab kt + 1
+        const bool more = kt + 1 < KT;
+        if (more) {
+            wait_slab(kt + D - 1 >= KT, std::integral_constant<int, D - 2>{});
+            barrier();
+            if (kt + D < KT) issue(kt + D, jc);
+        }
+        // row p of pairs is the last use of A slice p and B slice S-1-p:
+        // their registers take the next slab's fragments right after it
 #pragma unroll
         for (int p = 0; p < S; ++p) {
-            v8bf fa[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                fa[i] = *(const v8bf *)&as[p][wm * 64 + i * 32 + (lane & 31)][8 * (lane >> 5)];
+            for (int q = 0; q + p < S; ++q)
 #pragma unroll
-            for (int q = 0; q + p < S; ++q) {
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < WNT; ++j)
-                        acc[set_of(p + q)][i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            fb[q][j], fa[i], acc[set_of(p + q)][i][j], 0, 0, 0);
+                for (int i = 0; i < MI; ++i)
+                    acc[set_of(p + q)][i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        cur.b[q], cur.a[p][i], acc[set_of(p + q)][i], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (more) {
+                read_a(nxt, p, std::integral_constant<int, J1>{});
+                read_b(nxt, S - 1 - p, std::integral_constant<int, J1>{});
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        const bool last = kt + 1 == KT;
+        const bool last = !more;
 #pragma unroll
         for (int c = 0; c < NS; ++c)
             if (last || (kt + 1) % fold_every(c) == 0) fold(c);
+        cur = nxt;
     };
 
-    // slabs kt+1 .. kt+D-2 stay in flight while slab kt is multiplied
+    // all D stages start filled; slab 0's fragments go to registers first
     issue(0, std::integral_constant<int, 0>{});
-    if (1 < KT) issue(1, std::integral_constant<int, 1>{});
-    if constexpr (D > 2) if (2 < KT) issue(2, std::integral_constant<int, 2>{});
+    if (1 < KT) issue(1, std::integral_constant<int, 1 % D>{});
+    if constexpr (D > 2) if (2 < KT) issue(2, std::integral_constant<int, 2 % D>{});
+    if constexpr (D > 3) if (3 < KT) issue(3, std::integral_constant<int, 3 % D>{});
+    wait_slab(D - 1 >= KT, std::integral_constant<int, D - 1>{});
+    barrier();
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        read_a(cur, p, std::integral_constant<int, 0>{});
+        read_b(cur, p, std::integral_constant<int, 0>{});
+    }
     // the first slab is peeled so that the loop head is not the stage-0 step:
     // there the compiler cannot tell stage 0 from the stages still arriving
     // and drains every load in flight
     step(0, std::integral_constant<int, 0>{});
     for (int kt = 1; kt < KT; kt += D) {
-        step(kt, std::integral_constant<int, 1>{});
-        if constexpr (D > 2) if (kt + 1 < KT) step(kt + 1, std::integral_constant<int, 2>{});
-        if constexpr (D > 3) if (kt + 2 < KT) step(kt + 2, std::integral_constant<int, 3>{});
+        step(kt, std::integral_constant<int, 1 % D>{});
+        if constexpr (D > 2) if (kt + 1 < KT) step(kt + 1, std::integral_constant<int, 2 % D>{});
+        if constexpr (D > 3) if (kt + 2 < KT) step(kt + 2, std::integral_constant<int, 3 % D>{});
         if (kt + D - 1 < KT) step(kt + D - 1, std::integral_constant<int, 0>{});
     }
 
-    // epilogue: element r of tile (i, j) is C[row][col] with
-    //   row = m0 + wm*64 + i*32 + lane%32,
-    //   col = n0 + wn*32*WNT + j*32 + 8*(r/4) + 4*(lane/32) + r%4
+    // epilogue: element r of MFMA tile i is C[row][col] with
+    //   row = m0 + wm*32*MI + i*32 + lane%32,
+    //   col = n0 + wn*32 + 8*(r/4) + 4*(lane/32) + r%4
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const long gi = m0 + wm * 64 + i * 32 + (lane & 31);
+    for (int i = 0; i < MI; ++i) {
+        const long gi = m0 + wm * 32 * MI + i * 32 + (lane & 31);
         if (gi >= g.M) continue;
         const int ei = g.ea[gi];
 #pragma unroll
-        for (int j = 0; j < WNT; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const long gj = n0 + wn * 32 * WNT + j * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gj >= g.N) continue;
-                if (g.tri == 1 && gi < gj) continue;
-                if (g.tri == 2 && gi > gj) continue;
-                double *cp = g.C + gi + gj * g.ldc;
-                const double v = g.alpha * ldexp(acc64[i][j][r], ei + g.eb[gj] - 16);
-                *cp = g.beta == 0.0 ? v : fma(g.beta, *cp, v);
-            }
+        for (int r = 0; r < 16; ++r) {
+            const long gj = n0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (gj >= g.N) continue;
+            if (g.tri == 1 && gi < gj) continue;
+            if (g.tri == 2 && gi > gj) continue;
+            double *cp = g.C + gi + gj * g.ldc;
+            const double v = g.alpha * ldexp(acc64[i][r], ei + g.eb[gj] - 16);
+            *cp = g.beta == 0.0 ? v : fma(g.beta, *cp, v);
+        }
     }
 }
 
@@ -380,19 +439,43 @@ struct Split {
     int Rp;
 };
 
+// K chunks of the strided exponent pass: enough for about 2048 workgroups,
+// each at least 64 deep
+int exp_chunks(int R, int K)
+{
+    return std::max(1, std::min(ceil_div(2048, ceil_div(R, 256)), K / 64));
+}
+
+// Workspace of one split call: the non-finite flag, then room for the
+// partial line maxima of the strided exponent pass.
+struct SplitWs {
+    int *bad;
+    double *part;
+};
+SplitWs split_workspace(int R1, int R2, int K)
+{
+    const size_t part = (size_t)std::max((long)exp_chunks(R1, K) * R1, (long)exp_chunks(R2, K) * R2);
+    char *w = (char *)dev_workspace(256 + part * sizeof(double));
+    if (!w) return {nullptr, nullptr};
+    return {(int *)w, (double *)(w + 256)};
+}
+
 // Exponents and slices of an R x K operand given by (X, ld, rcontig).
 int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, long ld, int rcontig,
-               Split &out, int *bad)
+               Split &out, int *bad, double *part)
 {
     out.Rp = ceil_div(R, 128) * 128;
     out.e = (int *)gpublas_dev_alloc((size_t)R * sizeof(int));
     out.S = (bf16 *)gpublas_dev_alloc((size_t)s * KT * out.Rp * 16 * sizeof(bf16));
     if (!out.e || !out.S) return (int)hipErrorOutOfMemory;
     PROF_BEGIN(st, "ozaki_split");
-    if (rcontig)
-        hipLaunchKernelGGL(exp_rows_strided, dim3((unsigned)ceil_div(R, 256)), dim3(256), 0, st, R,
-                           K, X, ld, out.e, bad);
-    else
+    if (rcontig) {
+        const int chunks = exp_chunks(R, K);
+        hipLaunchKernelGGL(exp_rows_strided, dim3((unsigned)ceil_div(R, 256), (unsigned)chunks),
+                           dim3(256), 0, st, R, K, X, ld, part, bad);
+        hipLaunchKernelGGL(exp_rows_finish, dim3((unsigned)ceil_div(R, 256)), dim3(256), 0, st, R,
+                           chunks, part, out.e);
+    } else
         hipLaunchKernelGGL(exp_rows_contig, dim3((unsigned)ceil_div(R, 4)), dim3(256), 0, st, R, K,
                            X, ld, out.e, bad);
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 4)),
@@ -404,7 +487,8 @@ int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, lon
 template <int S>
 void launch_products(hipStream_t st, unsigned blocks, const OzArgs &g)
 {
-    hipLaunchKernelGGL((ozaki_gemm_kernel<S, OZ_WNT, OZ_OCC>), dim3(blocks), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((ozaki_gemm_kernel<S, OZ_TM, OZ_NW, kOzOcc>), dim3(blocks), dim3(64 * OZ_NW), 0,
+                       st, g);
 }
 
 int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, const Split &b,
@@ -425,8 +509,8 @@ int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, co
     g.C = C;
     g.ldc = ldc;
     g.tri = tri;
-    g.tiles_m = ceil_div(M, 128);
-    g.tiles_n = ceil_div(N, 64 * OZ_WNT);
+    g.tiles_m = ceil_div(M, OZ_TM);
+    g.tiles_n = ceil_div(N, 64);
     const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
     PROF_BEGIN(st, "ozaki_gemm");
     switch (s) {
@@ -463,13 +547,15 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
     if (alpha == 0.0 || k <= 0)
         return gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
     const int KT = ceil_div(k, 16);
-    int *bad = (int *)dev_workspace(sizeof(int));
+    const SplitWs ws = split_workspace(m, n, k);
+    if (!ws.bad) return (int)hipErrorOutOfMemory;
+    int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{}, b{};
     // op(A) rows: A 'N' -> X[r + k*lda] (rows contiguous); 'T' -> X[k + r*lda]
-    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), a, bad);
+    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), a, bad, ws.part);
     // op(B) columns: B 'N' -> element (k, j) at B[k + j*ldb] (k contiguous); 'T' -> B[j + k*ldb]
-    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), b, bad);
+    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), b, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
@@ -497,11 +583,13 @@ int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int
     if (alpha == 0.0 || k <= 0)
         return gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
     const int KT = ceil_div(k, 16);
-    int *bad = (int *)dev_workspace(sizeof(int));
+    const SplitWs ws = split_workspace(n, n, k);
+    if (!ws.bad) return (int)hipErrorOutOfMemory;
+    int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{};
     // op(A) = A (trans N: rows contiguous) or A^T; op(B) = op(A)^T has the same lines
-    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), a, bad);
+    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), a, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
