@@ -34,17 +34,12 @@ namespace {
 typedef __bf16 bf16;
 typedef short v8s __attribute__((ext_vector_type(8)));  // 16 B of bf16 for loads
 constexpr int OZ_MAXS = 8;
-#ifndef OZ_TM
-#define OZ_TM 64   // rows of a product tile (64 or 128; tiles are 64 columns wide)
-#endif
-#ifndef OZ_NW
-#define OZ_NW 4    // waves of a product workgroup (4 or 8)
-#endif
-// A wave owning one 32 x 32 MFMA tile keeps its accumulators near 200
-// registers, so two waves share a SIMD and hide each other's waits: 8 waves
-// on a 128-row tile, or two 4-wave workgroups of 64-row tiles per CU (which
-// move a third more bytes from L2 per flop).
-constexpr int kOzOcc = OZ_TM == 16 * OZ_NW ? 8 / OZ_NW : 1;  // two waves per SIMD
+// Product tiles are 64 columns wide with one 32 x 32 MFMA tile per wave,
+// which keeps a wave's accumulators under 256 registers so two waves share
+// each SIMD.  Up to four slices: 64-row tiles, 4 waves, two workgroups per CU
+// (independent barriers); five or more: 128-row tiles, 8 waves, one
+// workgroup per CU, which moves a quarter fewer bytes from L2 per flop.
+constexpr int oz_tile_rows(int s) { return s <= 4 ? 64 : 128; }
 
 struct OzArgs {
     int M, N, Mp, Np, KT;   // Mp, Np: rows padded to 128; KT: 16-deep k slabs
@@ -295,18 +290,21 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
     int nmine = 0;
 #pragma unroll
     for (int u = 0; u < PW; ++u) nmine += mine[u] ? 1 : 0;
-    auto issue = [&](int kt, auto jc) {
+    // this wave's pieces of slice p of slab kt, into stage jc
+    auto issue_slice = [&](int kt, int p, auto jc) {
         bf16(*ls)[TM + TN][16] = stage(jc);
 #pragma unroll
-        for (int p = 0; p < S; ++p)
+        for (int u = 0; u < PW; ++u) {
+            if (!mine[u]) continue;
+            const bf16 *src = isa[u] ? g.SA + ((long)p * KT + kt) * g.Mp * 16 + src0[u]
+                                     : g.SB + ((long)p * KT + kt) * g.Np * 16 + src0[u];
+            __builtin_amdgcn_global_load_lds((gptr_t)src, (lptr_t)&ls[p][32 * (wave + NW * u)][0],
+                                             16, 0, 0);
+        }
+    };
+    auto issue = [&](int kt, auto jc) {
 #pragma unroll
-            for (int u = 0; u < PW; ++u) {
-                if (!mine[u]) continue;
-                const bf16 *src = isa[u] ? g.SA + ((long)p * KT + kt) * g.Mp * 16 + src0[u]
-                                         : g.SB + ((long)p * KT + kt) * g.Np * 16 + src0[u];
-                __builtin_amdgcn_global_load_lds((gptr_t)src, (lptr_t)&ls[p][32 * (wave + NW * u)][0],
-                                                 16, 0, 0);
-            }
+        for (int p = 0; p < S; ++p) issue_slice(kt, p, jc);
     };
     auto fold = [&](int c) {
 #pragma unroll
@@ -361,10 +359,12 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
         if (more) {
             wait_slab(kt + D - 1 >= KT, std::integral_constant<int, D - 2>{});
             barrier();
-            if (kt + D < KT) issue(kt + D, jc);
         }
+        const bool refill = kt + D < KT;
         // row p of pairs is the last use of A slice p and B slice S-1-p:
-        // their registers take the next slab's fragments right after it
+        // their registers take the next slab's fragments right after it.
+        // Slice p of slab kt+D goes to the freed stage between the rows too,
+        // so no wave stalls on a burst of loads before its MFMAs.
 #pragma unroll
         for (int p = 0; p < S; ++p) {
 #pragma unroll
@@ -378,6 +378,7 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
                 read_a(nxt, p, std::integral_constant<int, J1>{});
                 read_b(nxt, S - 1 - p, std::integral_constant<int, J1>{});
             }
+            if (refill) issue_slice(kt + D, p, jc);
             __builtin_amdgcn_sched_barrier(0);
         }
         const bool last = !more;
@@ -487,8 +488,8 @@ int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, lon
 template <int S>
 void launch_products(hipStream_t st, unsigned blocks, const OzArgs &g)
 {
-    hipLaunchKernelGGL((ozaki_gemm_kernel<S, OZ_TM, OZ_NW, kOzOcc>), dim3(blocks), dim3(64 * OZ_NW), 0,
-                       st, g);
+    constexpr int TM = oz_tile_rows(S), NW = TM / 16, OCC = 8 / NW;
+    hipLaunchKernelGGL((ozaki_gemm_kernel<S, TM, NW, OCC>), dim3(blocks), dim3(64 * NW), 0, st, g);
 }
 
 int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, const Split &b,
@@ -509,7 +510,7 @@ int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, co
     g.C = C;
     g.ldc = ldc;
     g.tri = tri;
-    g.tiles_m = ceil_div(M, OZ_TM);
+    g.tiles_m = ceil_div(M, oz_tile_rows(s));
     g.tiles_n = ceil_div(N, 64);
     const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
     PROF_BEGIN(st, "ozaki_gemm");
