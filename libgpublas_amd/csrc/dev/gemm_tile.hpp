@@ -60,6 +60,18 @@ __device__ __forceinline__ bool stored(int sym, long r, long c)
 }
 
 // Pattern for the tile o in [o0, o0+R), k in [k0, k0+BK).
+// LDS tile layout: element (k, o) of a staged tile lives at column
+// o ^ lds_swz<T>(k).  Rows are BM + 128 bytes apart (0 modulo the 64 banks
+// for every other row), so the transposed stores of a k-contiguous operand
+// would put the 8 (double) or 4 (float) k rows one wave writes for a
+// column on the same banks; the XOR moves each row pair (quad) to its own
+// bank group.  It keeps 16-byte vectors and 32-column blocks intact, so
+// vector stores and the MFMA fragment reads stay conflict-free.
+template <typename T> __device__ __forceinline__ int lds_swz(int k)
+{
+    return sizeof(T) == 8 ? 4 * (k >> 1) : 16 * (k >> 2);
+}
+
 template <typename T, int R, int BK>
 __device__ __forceinline__ int tile_pattern(const Operand<T> &op, int o0, int k0, int K)
 {
@@ -95,6 +107,9 @@ struct TileLoader {
     static constexpr int VEC = 16 / (int)sizeof(T);
     static constexpr int NV = R * BK / VEC / 256;
     static_assert(NV >= 1, "tile too small for 256 threads");
+    static_assert((R & (R - 1)) == 0 &&
+                      (sizeof(T) == 8 ? 4 * ((BK - 1) >> 1) : 16 * ((BK - 1) >> 2)) < R,
+                  "lds_swz must stay inside the tile rows");
     typedef T Vec __attribute__((ext_vector_type(VEC)));
 
     __device__ __forceinline__ static void load(const Operand<T> &op, int o0, int k0, int K,
@@ -147,10 +162,10 @@ struct TileLoader {
             if (PAT == PAT_KCONT) {
                 const int kk = (v % (BK / VEC)) * VEC, o = v / (BK / VEC);
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) s[kk + e][o] = st[r][e];
+                for (int e = 0; e < VEC; ++e) s[kk + e][o ^ lds_swz<T>(kk + e)] = st[r][e];
             } else {
                 const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
-                *(Vec *)&s[kk][o] = st[r];
+                *(Vec *)&s[kk][o ^ lds_swz<T>(kk)] = st[r];
             }
         }
     }
@@ -165,10 +180,10 @@ struct TileLoader {
             if (pat == PAT_KCONT) {
                 const int kk = (v % (BK / VEC)) * VEC, o = v / (BK / VEC);
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) s[kk + e][o] = st[r][e];
+                for (int e = 0; e < VEC; ++e) s[kk + e][o ^ lds_swz<T>(kk + e)] = st[r][e];
             } else {
                 const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
-                *(Vec *)&s[kk][o] = st[r];
+                *(Vec *)&s[kk][o ^ lds_swz<T>(kk)] = st[r];
             }
         }
     }
@@ -257,9 +272,11 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T> &g, int tm, int tn,
             T fa[FM], fb[FN];
             const int kr = kk + (lane >> 4);
 #pragma unroll
-            for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm * WM + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i)
+                fa[i] = As[cur][kr][(wm * WM + i * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn * WN + j * 16 + (lane & 15)];
+            for (int j = 0; j < FN; ++j)
+                fb[j] = Bs[cur][kr][(wn * WN + j * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -358,9 +375,11 @@ __device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int
             T fa[FM], fb[FN];
             const int kr = kk + (lane >> 4);
 #pragma unroll
-            for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm * WM + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i)
+                fa[i] = As[cur][kr][(wm * WM + i * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn * WN + j * 16 + (lane & 15)];
+            for (int j = 0; j < FN; ++j)
+                fb[j] = Bs[cur][kr][(wn * WN + j * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
