@@ -271,12 +271,18 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T> &g, int tm, int tn,
         for (int kk = 0; kk < BK; kk += 4) {
             T fa[FM], fb[FN];
             const int kr = kk + (lane >> 4);
+            // (c0 + l) ^ swz = (c0 ^ shi) + (l ^ slo) for c0 a multiple of 16 and
+            // l < 16: the high part of the swizzle depends on kk alone (a
+            // constant of the unrolled loop), so the fragment offsets stay
+            // immediate; the low part is one lane-dependent term per step
+            const int shi = lds_swz<T>(kk) & ~15;
+            const int lo = (lane & 15) ^ (lds_swz<T>(kr) & 15);
 #pragma unroll
             for (int i = 0; i < FM; ++i)
-                fa[i] = As[cur][kr][(wm * WM + i * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
+                fa[i] = As[cur][kr][((wm * WM + i * 16) ^ shi) + lo];
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                fb[j] = Bs[cur][kr][(wn * WN + j * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
+                fb[j] = Bs[cur][kr][((wn * WN + j * 16) ^ shi) + lo];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -374,12 +380,18 @@ __device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int
         for (int kk = 0; kk < BK; kk += 4) {
             T fa[FM], fb[FN];
             const int kr = kk + (lane >> 4);
+            // (c0 + l) ^ swz = (c0 ^ shi) + (l ^ slo) for c0 a multiple of 16 and
+            // l < 16: the high part of the swizzle depends on kk alone (a
+            // constant of the unrolled loop), so the fragment offsets stay
+            // immediate; the low part is one lane-dependent term per step
+            const int shi = lds_swz<T>(kk) & ~15;
+            const int lo = (lane & 15) ^ (lds_swz<T>(kr) & 15);
 #pragma unroll
             for (int i = 0; i < FM; ++i)
-                fa[i] = As[cur][kr][(wm * WM + i * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
+                fa[i] = As[cur][kr][((wm * WM + i * 16) ^ shi) + lo];
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                fb[j] = Bs[cur][kr][(wn * WN + j * 16 + (lane & 15)) ^ lds_swz<T>(kr)];
+                fb[j] = Bs[cur][kr][((wn * WN + j * 16) ^ shi) + lo];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
