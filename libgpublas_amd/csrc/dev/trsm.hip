@@ -10,10 +10,9 @@
 // solved.  So all but O(m * 64 * n) of the m^2 n flops run in large GEMMs.
 // The 64 x 64 diagonal blocks are solved in FP64 (FP32 for STRSM) by leaf
 // kernels that keep the triangle in LDS:
-//  * left side:  one wave per right-hand-side column, lane i owns row i,
-//                substitution broadcasts x_k with a wave shuffle;
-//  * right side: one lane per row of B, the 64 unknowns of the row in
-//                registers, triangle entries broadcast from LDS.
+//  * one leaf kernel for both sides (the right side solved as the left
+//    side of the transposed system): lane i owns unknown i of two
+//    right-hand sides per wave, x_k broadcast from lane k with v_readlane.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 
@@ -46,86 +45,91 @@ __device__ __forceinline__ T opT(const T *A, long lda, int transN, int i, int k)
     return transN ? A[i + (long)k * lda] : A[k + (long)i * lda];
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_leaf_left(int nb, int n, T alpha, const T *A, long lda,
-                                                      int transN, int effLower, int nounit, T *B,
-                                                      long ldb)
+// x from lane l of the wave (l uniform): two v_readlane for a double
+__device__ __forceinline__ double lane_value(double v, int l)
 {
-    __shared__ T Ts[NB][NB + 1];
-    for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
-        const int i = idx % NB, k = idx / NB;
-        const bool in = i < nb && k < nb && (effLower ? i >= k : i <= k);
-        Ts[i][k] = in ? opT(A, lda, transN, i, k) : T(0);
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = blockIdx.x * 4 + wave; j < n; j += gridDim.x * 4) {
-        T *col = B + (long)j * ldb;
-        T b = lane < nb ? alpha * col[lane] : T(0);
-        if (effLower) {
-            for (int k = 0; k < nb; ++k) {
-                T xk = __shfl(b, k, 64);
-                if (nounit) xk = xk / Ts[k][k];
-                if (lane == k) b = xk;
-                else if (lane > k) b -= Ts[lane][k] * xk;
-            }
-        } else {
-            for (int k = nb - 1; k >= 0; --k) {
-                T xk = __shfl(b, k, 64);
-                if (nounit) xk = xk / Ts[k][k];
-                if (lane == k) b = xk;
-                else if (lane < k) b -= Ts[lane][k] * xk;
-            }
-        }
-        if (lane < nb) col[lane] = b;
-    }
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_leaf_right(int m, int nb, T alpha, const T *A,
-                                                       long lda, int transN, int effLower,
-                                                       int nounit, T *B, long ldb)
+// Solve of a 64 x 64 triangle, op(T) X = alpha B, for n right-hand sides:
+// element s of right-hand side r is B[s*sstride + r*rstride] (left side:
+// sstride 1, rstride ldb; right side, solved as op(T)^T X^T = alpha B^T:
+// sstride ldb, rstride 1, with transN and the triangle flipped).  Lane i owns
+// unknown i of LPW right-hand sides per wave (four waves per workgroup share
+// the triangle in LDS).  Step k takes b_k of each right-hand side from lane k
+// with v_readlane, forms x_k = b_k / T_kk as b_k * (1/T_kk) plus one Newton
+// correction (three dependent operations instead of a full division on the
+// step-to-step chain), and every later unknown subtracts x_k * T(i, k).  The
+// LPW right-hand sides are independent chains, interleaved.  On the left side
+// the update order per column is the reference's (netlib DTRSM, left side:
+// B(k) /= A(k,k); B(i) -= B(k)*A(i,k)); on the right side the reference
+// accumulates each unknown as a dot product instead, so sums are ordered
+// differently.
+constexpr int LPW = 2;
+template <typename T, bool LOWER, bool NOUNIT>
+__global__ __launch_bounds__(256) void trsm_leaf(int nb, int n, T alpha, const T *__restrict__ A,
+                                                 long lda, int transN, T *__restrict__ B,
+                                                 long sstride, long rstride)
 {
-    __shared__ T Ts[NB][NB + 1];
-    for (int idx = threadIdx.x; idx < NB * NB; idx += 256) {
-        const int k = idx % NB, j = idx / NB;
-        const bool in = k < nb && j < nb && (effLower ? k >= j : k <= j);
-        Ts[k][j] = in ? opT(A, lda, transN, k, j) : T(0);
+    __shared__ T Ts[NB][NB + 1];  // Ts[i][k] = op(T)(i, k) (identity outside nb)
+    __shared__ T rd[NB];          // 1 / op(T)(k, k)
+    // op(T)(i, k) = A[i + k*lda] (transN) or A[k + i*lda]: consecutive
+    // threads run along the contiguous index.  All 16 loads of a thread are
+    // issued before the first LDS store, so their latency is paid once.
+    constexpr int PER = NB * NB / 256;
+    T tv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int idx = threadIdx.x + 256 * u;
+        const int i = transN ? idx % NB : idx / NB, k = transN ? idx / NB : idx % NB;
+        const bool in = i < nb && k < nb && (LOWER ? i >= k : i <= k);
+        tv[u] = in ? opT(A, lda, transN, i, k) : (i == k ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int idx = threadIdx.x + 256 * u;
+        const int i = transN ? idx % NB : idx / NB, k = transN ? idx / NB : idx % NB;
+        Ts[i][k] = tv[u];
     }
     __syncthreads();
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    T x[NB];
+    if (threadIdx.x < NB) rd[threadIdx.x] = T(1) / Ts[threadIdx.x][threadIdx.x];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jb = (blockIdx.x * 4 + wave) * LPW;
+    T b[LPW];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) x[j] = j < nb ? alpha * B[i + (long)j * ldb] : T(0);
-    if (effLower) {
-        // X * L = B: x_j = (b_j - sum_{k>j} x_k L(k,j)) / L(j,j), j descending
+    for (int c = 0; c < LPW; ++c) {
+        const int j = jb + c;
+        b[c] = (j < n && lane < nb) ? alpha * B[lane * sstride + j * rstride] : T(0);
+    }
+    for (int st = 0; st < NB; ++st) {
+        const int k = LOWER ? st : NB - 1 - st;
+        const T t = Ts[lane][k];
+        const T r = rd[k], d = Ts[k][k];
+        const bool below = LOWER ? lane > k : lane < k;
 #pragma unroll
-        for (int j = NB - 1; j >= 0; --j) {
-            if (j < nb) {
-                T v = x[j];
-#pragma unroll
-                for (int k = j + 1; k < NB; ++k) v -= x[k] * Ts[k][j];
-                if (nounit) v = v / Ts[j][j];
-                x[j] = v;
+        for (int c = 0; c < LPW; ++c) {
+            const T bk = lane_value(b[c], k);
+            T x = bk;
+            if (NOUNIT) {
+                const T x0 = bk * r;
+                x = fma(fma(-x0, d, bk), r, x0);
             }
-        }
-    } else {
-        // X * U = B: x_j = (b_j - sum_{k<j} x_k U(k,j)) / U(j,j), j ascending
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < nb) {
-                T v = x[j];
-#pragma unroll
-                for (int k = 0; k < j; ++k) v -= x[k] * Ts[k][j];
-                if (nounit) v = v / Ts[j][j];
-                x[j] = v;
-            }
+            b[c] = lane == k ? x : below ? fma(-x, t, b[c]) : b[c];
         }
     }
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-        if (j < nb) B[i + (long)j * ldb] = x[j];
+    for (int c = 0; c < LPW; ++c) {
+        const int j = jb + c;
+        if (j < n && lane < nb) B[lane * sstride + j * rstride] = b[c];
+    }
 }
 
 template <typename T>
@@ -154,20 +158,20 @@ struct Solve {
 
     int leaf(int m, int n, T alpha, const T *a, T *b)
     {
-        if (left) {
-            int blocks = (n + 3) / 4;
-            if (blocks > 4096) blocks = 4096;
-            PROF_BEGIN(s, sizeof(T) == 8 ? "dtrsm_leaf_left" : "strsm_leaf_left");
-            hipLaunchKernelGGL(trsm_leaf_left<T>, dim3(blocks), dim3(256), 0, s, m, n, alpha, a,
-                               (long)lda, transN, effLower, nounit, b, (long)ldb);
-            PROF_END(s);
-        } else {
-            int blocks = (m + 255) / 256;
-            PROF_BEGIN(s, sizeof(T) == 8 ? "dtrsm_leaf_right" : "strsm_leaf_right");
-            hipLaunchKernelGGL(trsm_leaf_right<T>, dim3(blocks), dim3(256), 0, s, m, n, alpha, a,
-                               (long)lda, transN, effLower, nounit, b, (long)ldb);
-            PROF_END(s);
-        }
+        // right side: op(T)^T X^T = alpha B^T, the triangle of op(T)^T is
+        // op(T) read the other way and of the other shape
+        const bool lower = left ? effLower : !effLower;
+        const int tn = left ? transN : !transN;
+        const int nrhs = left ? n : m, order = left ? m : n;
+        const long ss = left ? 1 : ldb, rs = left ? ldb : 1;
+        const unsigned blocks = (unsigned)ceil_div(nrhs, 4 * LPW);
+        auto k = lower ? (nounit ? trsm_leaf<T, true, true> : trsm_leaf<T, true, false>)
+                       : (nounit ? trsm_leaf<T, false, true> : trsm_leaf<T, false, false>);
+        PROF_BEGIN(s, sizeof(T) == 8 ? (left ? "dtrsm_leaf_left" : "dtrsm_leaf_right")
+                                     : (left ? "strsm_leaf_left" : "strsm_leaf_right"));
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, order, nrhs, alpha, a, (long)lda, tn, b,
+                           ss, rs);
+        PROF_END(s);
         return (int)hipGetLastError();
     }
 
