@@ -61,6 +61,34 @@ __global__ __launch_bounds__(256, 2) void gemm_fast_kernel(GemmArgs<T> g)
     gemm_fast_tile<T, BM, BN, BK, PA, PB, TRI>(g, tm, tn, As, Bs);
 }
 
+// The last, partly filled round of a 128 x 128 fast grid, run as 64 x 64
+// quarter tiles: block b is quarter b % 4 of big tile base + b / 4 (same
+// decoding as gemm_fast_kernel), so the round is four times shorter.
+// Quarters wholly outside the stored triangle have nothing to do.  Same k
+// order per element as the big tile: the bits do not change.
+template <typename T, int PA, int PB, int TRI>
+__global__ __launch_bounds__(256, 2) void gemm_fast_sub_kernel(GemmArgs<T> g, int base)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) T As[2][16][64 + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][16][64 + PAD];
+    const int big = base + (int)(blockIdx.x >> 2), q = blockIdx.x & 3;
+    int tm, tn;
+    if (TRI == SYM_NONE) {
+        tm = big % g.tiles_m;
+        tn = big / g.tiles_m;
+    } else {
+        int hi, lo;
+        tri_decode(big, hi, lo);
+        if (TRI == SYM_LOWER) { tm = hi; tn = lo; } else { tm = lo; tn = hi; }
+    }
+    tm = 2 * tm + (q & 1);
+    tn = 2 * tn + (q >> 1);
+    if (TRI == SYM_LOWER && tn > tm) return;
+    if (TRI == SYM_UPPER && tm > tn) return;
+    gemm_fast_tile<T, 64, 64, 16, PA, PB, TRI>(g, tm, tn, As, Bs);
+}
+
 // Scale (or zero) a triangle or rectangle of C: the alpha == 0 / k == 0
 // paths of the netlib routines (beta == 0 stores exact zeros).
 template <typename T>
@@ -124,6 +152,44 @@ void launch_fast(hipStream_t s, const GemmArgs<T> &g, unsigned blocks)
 #undef GB_FAST
 }
 
+template <typename T>
+void launch_fast_sub(hipStream_t s, const GemmArgs<T> &g, int base, unsigned blocks)
+{
+#define GB_SUB(PA_, PB_, TRI_)                                                                   \
+    hipLaunchKernelGGL((gemm_fast_sub_kernel<T, PA_, PB_, TRI_>), dim3(blocks), dim3(256), 0, s, \
+                       g, base)
+    const int pa = g.a.form, pb = g.b.form;
+    if (g.tri == SYM_NONE) {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SUB(PAT_OUTER, PAT_KCONT, SYM_NONE);
+        else if (pa == PAT_OUTER) GB_SUB(PAT_OUTER, PAT_OUTER, SYM_NONE);
+        else if (pb == PAT_KCONT) GB_SUB(PAT_KCONT, PAT_KCONT, SYM_NONE);
+        else GB_SUB(PAT_KCONT, PAT_OUTER, SYM_NONE);
+    } else if (g.tri == SYM_LOWER) {
+        if (pa == PAT_OUTER) GB_SUB(PAT_OUTER, PAT_OUTER, SYM_LOWER);
+        else GB_SUB(PAT_KCONT, PAT_KCONT, SYM_LOWER);
+    } else {
+        if (pa == PAT_OUTER) GB_SUB(PAT_OUTER, PAT_OUTER, SYM_UPPER);
+        else GB_SUB(PAT_KCONT, PAT_KCONT, SYM_UPPER);
+    }
+#undef GB_SUB
+}
+
+// Concurrent workgroup slots of the fast kernels: two per CU.
+long fast_slots()
+{
+    static const long slots = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0) {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+        return 2L * cus;
+    }();
+    return slots;
+}
+
 // Core launcher: C = alpha*op(A)*op(B) + beta*C on an (optionally
 // triangular) tile grid.  k > 0, alpha != 0 assumed.  Tiles that are full in
 // M and N go to the fast kernel when both operands allow it; the generic
@@ -152,9 +218,16 @@ int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a
         f.tiles_m = full_m;
         f.tiles_n = full_n;
         f.full_m = f.full_n = 0;
+        // A last round of 128 x 128 tiles that would leave most of the chip
+        // idle runs as 64 x 64 quarters instead (a quarter of the time).
+        const long slots = fast_slots(), tail = fast_blocks % slots;
+        const long main_blocks =
+            (big && fast_blocks > slots && 2 * tail < slots) ? fast_blocks - tail : fast_blocks;
         PROF_BEGIN(s, kname<T>(big, 1));
-        if (big) launch_fast<T, 128>(s, f, (unsigned)fast_blocks);
+        if (big) launch_fast<T, 128>(s, f, (unsigned)main_blocks);
         else launch_fast<T, 64>(s, f, (unsigned)fast_blocks);
+        if (main_blocks < fast_blocks)
+            launch_fast_sub<T>(s, f, (int)main_blocks, (unsigned)(4 * (fast_blocks - main_blocks)));
         PROF_END(s);
     }
     g.tiles_m = ceil_div(m, bm);

@@ -157,6 +157,40 @@ def test_dsyrk(gpu, ora, uplo, trans, n, k):
     assert ratio < 16.0, ratio
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N")])
+def test_dgemm_short_last_round_exact(gpu, ora, ta, tb):
+    """m = n = 2944: 23 x 23 = 529 full 128 x 128 tiles, one more than a round
+    of 512 workgroups, so the last 17 tiles run as 64 x 64 quarters.  Integer
+    data: bit-exact against the oracle, quarters and big tiles alike."""
+    rng = np.random.default_rng(21)
+    m = n = 2944
+    k = 48
+    A = np.asfortranarray(rng.integers(-4, 5, size=(m, k) if ta == "N" else (k, m)).astype(float))
+    B = np.asfortranarray(rng.integers(-4, 5, size=(k, n) if tb == "N" else (n, k)).astype(float))
+    C = np.asfortranarray(rng.integers(-4, 5, size=(m, n)).astype(float))
+    R = C.copy(order="F")
+    gpu.dgemm(ta, tb, m, n, k, 1.0, A, A.shape[0], B, B.shape[0], 2.0, C, m)
+    ora.ora_dgemm(c(ta), c(tb), m, n, k, 1.0, p(A), A.shape[0], p(B), B.shape[0], 2.0, p(R), m)
+    assert np.array_equal(C, R)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_dsyrk_short_last_round_exact(gpu, ora, uplo):
+    """n = 4096: 32 * 33 / 2 = 528 triangle tiles, so the last 16 (the tail of
+    the last tile row, diagonal tile included) run as 64 x 64 quarters, the
+    quarter above (below) the diagonal skipped.  Integer data: bit-exact, and
+    the other triangle untouched."""
+    rng = np.random.default_rng(22)
+    n, k = 4096, 40
+    A = np.asfortranarray(rng.integers(-4, 5, size=(n, k)).astype(float))
+    C0 = np.asfortranarray(rng.integers(-4, 5, size=(n, n)).astype(float))
+    C = C0.copy(order="F")
+    R = C0.copy(order="F")
+    gpu.dsyrk(uplo, "N", n, k, 1.0, A, n, 3.0, C, n)
+    ora.ora_dsyrk(c(uplo), c("N"), n, k, 1.0, p(A), n, 3.0, p(R), n)
+    assert np.array_equal(C, R)
+
+
 @pytest.mark.parametrize("side", ["L", "R"])
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("m,n", [(5, 9), (70, 130), (200, 64)])
