@@ -149,8 +149,9 @@ def gemm_figures():
 def level3_suite(reps):
     """The Level-3 sweep: GEMM at the four trans combinations with M, N, K in
     {4096, 8192} in FP64 and FP32; DSYRK lower n=8192 k=2048; DTRSM
-    left/lower/no-trans/non-unit m=8192 nrhs=4096 (diagonally dominant A).
-    One JSON line per shape, kernel TFLOP/s from HIP events."""
+    left/lower/no-trans/non-unit m=8192 nrhs=4096 (diagonally dominant A);
+    DSYMM left/lower m=n=8192.  One JSON line per shape, kernel TFLOP/s from
+    HIP events.
     import itertools
     import libgpublas_amd as gb
     exe = os.path.join(gb.BIN_DIR, "blas_bench")
@@ -166,6 +167,8 @@ def level3_suite(reps):
                      "--k", "2048"])
         runs.append(["--op", "trsm", "--prec", prec, "--side", "L", "--uplo", "L", "--ta", "N",
                      "--diag", "N", "--m", "8192", "--n", "4096"])
+        runs.append(["--op", "symm", "--prec", prec, "--side", "L", "--uplo", "L", "--m", "8192",
+                     "--n", "8192"])
     peak = {"d": FP64_PEAK_TFLOPS, "s": 157.3}
     results = []
     for r in runs:

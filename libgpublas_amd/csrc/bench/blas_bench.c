@@ -5,6 +5,7 @@
  *   blas_bench --lib libgpublas.so --op gemm --prec d --m M --n N --k K --ta N --tb N
  *   blas_bench --lib ... --op syrk --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trsm --m M --n N --side L --uplo L --ta N --diag N
+ *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   common: --reps R --warmup W --seed S
  *
@@ -130,10 +131,14 @@ int main(int argc, char **argv)
                            const int *);
     typedef void (*axpy_t)(const int *, const double *, const double *, const int *, double *,
                            const int *);
+    typedef void (*symm_t)(const char *, const char *, const int *, const int *, const void *,
+                           const void *, const int *, const void *, const int *, const void *,
+                           void *, const int *);
     gemm_t gemm = NULL;
     syrk_t syrk = NULL;
     trsm_t trsm = NULL;
     axpy_t axpy = NULL;
+    symm_t symm = NULL;
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -166,6 +171,15 @@ int main(int argc, char **argv)
             else ((float *)A)[(size_t)i * na + i] = (float)(na + u);
         }
         flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
+    } else if (!strcmp(op, "symm")) {
+        symm = (symm_t)sym(prec == 'd' ? "dsymm_" : "ssymm_");
+        const int na = (side == 'L' || side == 'l') ? m : n;
+        lda = na;
+        ldb = ldc = m;
+        A = operand((size_t)na * na, prec, seed);
+        B = operand((size_t)m * n, prec, seed + 1);
+        C = operand((size_t)m * n, prec, seed + 2);
+        flops = 2.0 * m * n * na;
     } else if (!strcmp(op, "axpy")) {
         axpy = (axpy_t)sym("daxpy_");
         A = operand((size_t)n, 'd', seed);
@@ -185,6 +199,7 @@ int main(int argc, char **argv)
         if (gemm) gemm(&ta, &tb, &m, &n, &k, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else if (syrk) syrk(&uplo, &trans, &n, &k, alpha, A, &lda, beta, C, &ldc);
         else if (trsm) trsm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
+        else if (symm) symm(&side, &uplo, &m, &n, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else {
             const double a = 0.5;
             const int one = 1;

@@ -41,6 +41,7 @@ DevCtx *ctx_of(hipStream_t stream);
 DevCtx *dev_ctx();
 // Workspace of at least `bytes` on the calling thread's context (grows).
 void *dev_workspace(size_t bytes);
+void dev_release_after(void *d_ptr, hipStream_t s);  // pool release ordered behind stream s
 
 static constexpr int kTicketSlots = 4096;  // ticket counters per context
 

@@ -282,6 +282,53 @@ int syrk_entry(void *stream, char uplo, char trans, int n, int k, T alpha, const
     return launch_gemm<T>(s, n, n, k, alpha, a, b, beta, C, ldc, tri);
 }
 
+// Full symmetric matrix S (n x n, leading dimension lds) from the stored
+// triangle of A: one workgroup per 64 x 64 tile of S.  Tiles inside the
+// stored triangle are copied, tiles in the other one are the transposed
+// mirror tile (through LDS, so both sides stay coalesced), diagonal tiles
+// pick per element.  Every thread issues its 16 loads before its stores.
+template <typename T>
+__global__ __launch_bounds__(256) void symmetrize_kernel(int n, const T *__restrict__ A, long lda,
+                                                         int lower, T *__restrict__ S, long lds)
+{
+    __shared__ T t[64][65];
+    const int I = blockIdx.x, J = blockIdx.y;
+    const int i0 = I * 64, j0 = J * 64;
+    const bool diag = I == J, direct = lower ? I > J : I < J;
+    // source tile: (I, J) itself, or its mirror (J, I); element (a, b) of it
+    // is A[(r0 + a) + (c0 + b) * lda]
+    const int r0 = direct || diag ? i0 : j0, c0 = direct || diag ? j0 : i0;
+    T v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, a = idx & 63, b = idx >> 6;
+        v[u] = (r0 + a < n && c0 + b < n) ? A[(r0 + a) + (long)(c0 + b) * lda] : T(0);
+    }
+    if (direct) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = threadIdx.x + 256 * u, a = idx & 63, b = idx >> 6;
+            if (i0 + a < n && j0 + b < n) S[(i0 + a) + (long)(j0 + b) * lds] = v[u];
+        }
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u;
+        t[idx & 63][idx >> 6] = v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, a = idx & 63, b = idx >> 6;
+        if (i0 + a >= n || j0 + b >= n) continue;
+        // S(i0+a, j0+b): mirror tile -> A(j0+b, i0+a) = t[b][a]; diagonal
+        // tile -> the stored one of (a, b) and (b, a)
+        const bool stored = lower ? a >= b : a <= b;
+        S[(i0 + a) + (long)(j0 + b) * lds] = (diag && stored) ? t[a][b] : t[b][a];
+    }
+}
+
 template <typename T>
 int symm_entry(void *stream, char side, char uplo, int m, int n, T alpha, const T *A, int lda,
                const T *B, int ldb, T beta, T *C, int ldc)
@@ -293,6 +340,36 @@ int symm_entry(void *stream, char side, char uplo, int m, int n, T alpha, const 
         return launch_scale<T>(s, m, n, beta, C, ldc, SYM_NONE);
     }
     const int sym = is_u(uplo) ? SYM_UPPER : SYM_LOWER;
+    // From order 256 up, the stored triangle is first expanded into a full
+    // symmetric matrix (one pass over na^2 elements) so the product runs on
+    // the fast GEMM kernels; same products in the same k order, same bits.
+    const int na = is_l(side) ? m : n;
+    if (na >= 256) {
+        const long lds = (na + 7) / 8 * 8;
+        T *S = (T *)gpublas_dev_alloc((size_t)lds * na * sizeof(T));
+        if (S) {
+            const unsigned tiles = (unsigned)ceil_div(na, 64);
+            PROF_BEGIN(s, "symmetrize");
+            hipLaunchKernelGGL(symmetrize_kernel<T>, dim3(tiles, tiles), dim3(256), 0, s, na, A,
+                               (long)lda, sym == SYM_LOWER ? 1 : 0, S, lds);
+            PROF_END(s);
+            int err = (int)hipGetLastError();
+            if (!err) {
+                if (is_l(side))
+                    err = launch_gemm<T>(s, m, n, m, alpha,
+                                         make_op<T>(S, lds, m, PAT_OUTER, SYM_NONE),
+                                         make_op<T>(B, ldb, n, PAT_KCONT, SYM_NONE), beta, C, ldc,
+                                         SYM_NONE);
+                else
+                    err = launch_gemm<T>(s, m, n, n, alpha,
+                                         make_op<T>(B, ldb, m, PAT_OUTER, SYM_NONE),
+                                         make_op<T>(S, lds, n, PAT_KCONT, SYM_NONE), beta, C, ldc,
+                                         SYM_NONE);
+            }
+            dev_release_after(S, s);
+            return err;
+        }
+    }
     if (is_l(side)) {
         // C = alpha*A*B + beta*C, A symmetric m x m: k = m
         Operand<T> a = make_op<T>(A, lda, m, PAT_OUTER, sym);

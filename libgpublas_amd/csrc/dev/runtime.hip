@@ -30,6 +30,13 @@ std::mutex g_pool_mu;
 std::unordered_map<size_t, std::vector<void *>> *g_free;
 std::unordered_map<void *, size_t> *g_live;
 size_t g_pool_bytes = 0;
+// blocks released behind a stream event: reusable once the event completes
+struct Deferred {
+    void *p;
+    size_t c;
+    hipEvent_t ev;
+};
+std::vector<Deferred> *g_deferred;
 
 size_t size_class(size_t b)
 {
@@ -43,8 +50,28 @@ size_t size_class(size_t b)
     return (b + two_mib - 1) / two_mib * two_mib;
 }
 
+// Move deferred blocks whose event has completed to the free lists
+// (all of them when `wait`).
+void pool_reap_locked(bool wait)
+{
+    auto &d = *g_deferred;
+    size_t keep = 0;
+    for (size_t i = 0; i < d.size(); ++i) {
+        if (wait) (void)hipEventSynchronize(d[i].ev);
+        if (hipEventQuery(d[i].ev) == hipSuccess) {
+            (void)hipEventDestroy(d[i].ev);
+            (*g_free)[d[i].c].push_back(d[i].p);
+        } else {
+            d[keep++] = d[i];
+        }
+    }
+    d.resize(keep);
+    (void)hipGetLastError();  // a not-ready query is not an error
+}
+
 void pool_trim_locked()
 {
+    pool_reap_locked(true);
     for (auto &kv : *g_free) {
         for (void *p : kv.second) {
             (void)hipFree(p);
@@ -147,6 +174,7 @@ void do_init()
 {
     g_free = new std::unordered_map<size_t, std::vector<void *>>();
     g_live = new std::unordered_map<void *, size_t>();
+    g_deferred = new std::vector<Deferred>();
     g_pending = new std::vector<Pending>();
     g_event_cache = new std::vector<hipEvent_t>();
     int n = 0;
@@ -375,6 +403,7 @@ void *gpublas_dev_alloc(size_t bytes)
     if (gpublas_dev_init() != 0) return nullptr;
     size_t c = size_class(bytes ? bytes : 1);
     std::lock_guard<std::mutex> g(g_pool_mu);
+    if (!g_deferred->empty()) pool_reap_locked(false);
     auto it = g_free->find(c);
     void *p = nullptr;
     if (it != g_free->end() && !it->second.empty()) {
@@ -403,6 +432,30 @@ void gpublas_dev_release(void *d_ptr)
     auto it = g_live->find(d_ptr);
     if (it == g_live->end()) return;
     (*g_free)[it->second].push_back(d_ptr);
+    g_live->erase(it);
+}
+
+// Release a block whose last users are queued on stream s: it returns to
+// the free lists once the work queued so far on s has completed, so the
+// caller need not wait for it.
+extern "C++" void dev_release_after(void *d_ptr, hipStream_t s)
+{
+    if (!d_ptr) return;
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_live->find(d_ptr);
+    if (it == g_live->end()) return;
+    hipEvent_t ev = nullptr;
+    bool ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventRecord(ev, s) == hipSuccess;
+    if (ok) {
+        g_deferred->push_back(Deferred{d_ptr, it->second, ev});
+    } else {
+        // cannot order the release behind the stream: wait for it instead
+        (void)hipGetLastError();
+        if (ev) (void)hipEventDestroy(ev);
+        (void)hipStreamSynchronize(s);
+        (*g_free)[it->second].push_back(d_ptr);
+    }
     g_live->erase(it);
 }
 

@@ -213,6 +213,33 @@ def test_dsymm(gpu, ora, side, uplo, m, n):
     np.testing.assert_allclose(C, 1.1 * full + 0.3 * C0, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("m,n", [(300, 257), (257, 320)])
+def test_dsymm_expanded_exact(gpu, ora, side, uplo, m, n):
+    """From order 256 the stored triangle is expanded into a full symmetric
+    matrix before the product (ragged 64-tiles on the edge, padded lda, the
+    other triangle ROGUE so any read of it shows).  Integer data: bit-exact."""
+    rng = np.random.default_rng(31)
+    na = m if side == "L" else n
+    lda = na + 5
+    S = rng.integers(-4, 5, size=(na, na)).astype(float)
+    S = np.tril(S) + np.tril(S, -1).T
+    A = np.full((lda, na), ROGUE, order="F")
+    A[:na, :] = S
+    junk = np.triu(np.ones((na, na), bool), 1) if uplo == "L" else np.tril(np.ones((na, na), bool), -1)
+    A[:na, :][junk] = ROGUE
+    B = np.asfortranarray(rng.integers(-4, 5, size=(m, n)).astype(float))
+    C0 = np.asfortranarray(rng.integers(-4, 5, size=(m, n)).astype(float))
+    C = C0.copy(order="F")
+    R = C0.copy(order="F")
+    gpu.dsymm(side, uplo, m, n, 2.0, A, lda, B, m, -1.0, C, m)
+    ora.ora_dsymm(c(side), c(uplo), m, n, 2.0, p(A), lda, p(B), m, -1.0, p(R), m)
+    assert np.array_equal(C, R)
+    full = S @ B if side == "L" else B @ S
+    assert np.array_equal(C, 2.0 * full - C0)
+
+
 def _tri_matrix(seed, n, uplo, diag_boost=True):
     A = uniform(seed, (n, n))
     if diag_boost:
