@@ -76,6 +76,25 @@ def test_dgemm_integer_exact(gpu, ora, ta, tb):
     assert np.array_equal(C, R)
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(512, 512, 2048), (1024, 1024, 1024), (576, 448, 1024)])
+def test_dgemm_splitk_exact(gpu, ora, ta, tb, m, n, k):
+    """Few 64 x 64 tiles and a long k take the split-K path (8, 2 and 4 k
+    chunks here; partial sums folded in chunk order); ldc padded, beta != 0.
+    Integer data: bit-exact, rows beyond m untouched."""
+    rng = np.random.default_rng(41)
+    A = np.asfortranarray(rng.integers(-4, 5, size=(m, k) if ta == "N" else (k, m)).astype(float))
+    B = np.asfortranarray(rng.integers(-4, 5, size=(k, n) if tb == "N" else (n, k)).astype(float))
+    ldc = m + 3
+    C0 = _padded(rng.integers(-4, 5, size=(m, n)).astype(float), ldc)
+    C = C0.copy(order="F")
+    R = C0.copy(order="F")
+    gpu.dgemm(ta, tb, m, n, k, 2.0, A, A.shape[0], B, B.shape[0], -3.0, C, ldc)
+    ora.ora_dgemm(c(ta), c(tb), m, n, k, 2.0, p(A), A.shape[0], p(B), B.shape[0], -3.0, p(R), ldc)
+    assert np.array_equal(C, R)
+    assert np.array_equal(C[m:, :], C0[m:, :])
+
+
 def test_dgemm_beta_zero_ignores_nan(gpu):
     m = n = k = 70
     A = uniform(1, (m, k))
