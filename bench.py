@@ -151,7 +151,7 @@ def level3_suite(reps):
     {4096, 8192} in FP64 and FP32; DSYRK lower n=8192 k=2048; DTRSM
     left/lower/no-trans/non-unit m=8192 nrhs=4096 (diagonally dominant A);
     DSYMM left/lower m=n=8192.  One JSON line per shape, kernel TFLOP/s from
-    HIP events.
+    HIP events."""
     import itertools
     import libgpublas_amd as gb
     exe = os.path.join(gb.BIN_DIR, "blas_bench")
@@ -203,8 +203,8 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
     vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.gpublas_dev_dgemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ci, ci, ci, cd, vp, ci, vp,
                                       ci, cd, vp, ci]
-    lib.gpublas_dev_dgemm_split.argtypes = [vp, ci, ctypes.c_char, ctypes.c_char, ci, ci, ci, cd, vp,
-                                            ci, vp, ci, cd, vp, ci]
+    lib.gpublas_dev_dgemm_split_fmt.argtypes = [vp, ci, ci, ctypes.c_char, ctypes.c_char, ci, ci,
+                                                ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.gpublas_dev_prof_enable(1)
     bytes_ = n * n * 8
     dA, dB, dC = (lib.gpublas_dev_alloc(bytes_) for _ in range(3))
@@ -232,15 +232,16 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
         host_s = time.perf_counter() - t0
         normR = np.linalg.norm(R)
         maxR = np.max(np.abs(R))
-        variants = [("fp64", 0)] + [("split", s) for s in slices]
-        for kind, s in variants:
+        variants = [("fp64", 0, None)] + [("split", s, f) for f in ("bf16", "fp16") for s in slices]
+        for kind, s, fmt in variants:
             for rep in range(3):
                 if rep == 1:
                     lib.gpublas_dev_prof_reset()
                 if kind == "fp64":
                     lib.gpublas_dev_dgemm(None, b"N", b"N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
                 else:
-                    lib.gpublas_dev_dgemm_split(None, s, b"N", b"N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+                    lib.gpublas_dev_dgemm_split_fmt(None, s, 1 if fmt == "fp16" else 0, b"N", b"N", n,
+                                                    n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
             lib.gpublas_dev_sync(None)
             ms = ktime(["dgemm_fast_128x128"] if kind == "fp64" else ["ozaki_split", "ozaki_gemm"]) / 2
             lib.gpublas_dev_d2h(host.ctypes.data, dC, bytes_, None)
@@ -248,7 +249,7 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
             err_norm = float(np.linalg.norm(host - R) / normR)
             err_max = float(np.max(np.abs(host - R)) / maxR)
             npairs = s * (s + 1) // 2 if s else 0
-            rec = {"suite": "split", "dist": dist, "n": n, "kind": kind, "slices": s,
+            rec = {"suite": "split", "dist": dist, "n": n, "kind": kind, "slices": s, "slice_format": fmt,
                    "slice_products": npairs, "kernel_ms": round(ms, 3),
                    "effective_tflops": round(2.0 * n ** 3 / (ms / 1e3) / 1e12, 2),
                    "bf16_mfma_tflops": round(npairs * 2.0 * n ** 3 / (ms / 1e3) / 1e12, 1) if s else None,

@@ -87,16 +87,28 @@ int gpublas_dev_strsm(void *stream, char side, char uplo, char transa, char diag
                       float alpha, const float *d_A, int lda, float *d_B, int ldb);
 
 /* Split-precision DGEMM / DSYRK (opt-in; no counterpart in the reference):
- * Ozaki integer-slice scheme, `slices` (1..8) signed 8-bit slices per
- * operand held in BF16, slice products on BF16 MFMA with exact FP32
- * accumulation over 256-wide k-chunks, FP64 scaling and final sums.  Inputs
- * with Inf/NaN fall back to the FP64 kernels.  Synchronous. */
+ * Ozaki integer-slice scheme, `slices` signed 8-bit slices per operand, slice
+ * products p + q < slices on MFMA with FP32 accumulation per diagonal p + q
+ * (exact for the leading diagonals), FP64 scaling and final sums.  Inputs
+ * with Inf/NaN fall back to the FP64 kernels.  Synchronous.
+ * Slice formats: BF16 (slices pre-scaled by 2^-8p; 1..8 slices) and FP16
+ * (the integers themselves, each diagonal's weight applied in FP64; 1..6
+ * slices); both give the same results at the same MFMA rate.  The entry
+ * points without _fmt use BF16. */
+#define GPUBLAS_SLICE_BF16 0
+#define GPUBLAS_SLICE_FP16 1
 int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, int m, int n,
                             int k, double alpha, const double *d_A, int lda, const double *d_B,
                             int ldb, double beta, double *d_C, int ldc);
 int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int n, int k,
                             double alpha, const double *d_A, int lda, double beta, double *d_C,
                             int ldc);
+int gpublas_dev_dgemm_split_fmt(void *stream, int slices, int fmt, char transa, char transb, int m,
+                                int n, int k, double alpha, const double *d_A, int lda,
+                                const double *d_B, int ldb, double beta, double *d_C, int ldc);
+int gpublas_dev_dsyrk_split_fmt(void *stream, int slices, int fmt, char uplo, char trans, int n,
+                                int k, double alpha, const double *d_A, int lda, double beta,
+                                double *d_C, int ldc);
 
 /* ---- Level 2 ---- */
 int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,

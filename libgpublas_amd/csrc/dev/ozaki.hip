@@ -119,7 +119,7 @@ __global__ void exp_rows_contig(int R, int K, const double *X, long ld, int *e, 
 // the slice writes are coalesced.
 __global__ __launch_bounds__(256) void split_kernel(int R, int Rp, int K, int KT, int s,
                                                     const double *X, long ld, int rcontig,
-                                                    const int *e, bf16 *S)
+                                                    const int *e, int f16, bf16 *S)
 {
     __shared__ double t[32][65];
     const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 64;
@@ -148,8 +148,13 @@ __global__ __launch_bounds__(256) void split_kernel(int R, int Rp, int K, int KT
             x0 = u0 - q0;
             x1 = u1 - q1;
             bf16 *o = S + (((long)p * KT + kt) * Rp + r) * 16 + 2 * kp;
-            o[0] = (bf16)(float)ldexp(q0, -8 * p);
-            o[1] = (bf16)(float)ldexp(q1, -8 * p);
+            if (f16) {  // the integer itself: 2^-8p is below FP16's range for p >= 3
+                o[0] = __builtin_bit_cast(bf16, (_Float16)q0);
+                o[1] = __builtin_bit_cast(bf16, (_Float16)q1);
+            } else {
+                o[0] = (bf16)(float)ldexp(q0, -8 * p);
+                o[1] = (bf16)(float)ldexp(q1, -8 * p);
+            }
         }
     }
 }
@@ -215,13 +220,26 @@ __host__ __device__ constexpr int vm_wait(int n)
     return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
 }
 
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+// one 32x32x16 slice product: BF16 slices are pre-scaled, FP16 slices hold
+// the integers (same bits in the LDS stages, read as the MFMA's type)
+template <bool F16>
+__device__ __forceinline__ v16f slice_mfma(v8bf b, v8bf a, v16f c)
+{
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, b),
+                                                      __builtin_bit_cast(v8h, a), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, c, 0, 0, 0);
+}
+
 // one pipeline stage of the slice product: slab kt of every A slice (rows
 // 0..TM-1) and B slice (rows TM..TM+TN-1)
 template <int NSL, int TM, int TN> struct OzStage {
     __attribute__((aligned(16))) bf16 r[NSL][TM + TN][16];
 };
 
-template <int S, int TM, int NW, int OCC>
+template <int S, int TM, int NW, int OCC, bool F16>
 __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
 {
     constexpr int NS = nsets(S);
@@ -231,6 +249,7 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
     constexpr int D = oz_stages(S, TM, OCC);
     constexpr int P = (TM + TN) / 32;  // 1 KB pieces of one slice slab
     constexpr int PW = (P + NW - 1) / NW;  // pieces per wave (waves w, w+NW, ...)
+    static_assert(!F16 || S <= 6, "FP16 slices: one diagonal per accumulator set");
     static_assert((TM == 64 || TM == 128) && MI >= 1 && D >= 2 && D <= 4 &&
                       PW * S * D < 64,
                   "slab pipeline shape");
@@ -310,7 +329,8 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
 #pragma unroll
         for (int i = 0; i < MI; ++i) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc64[i][r] += (double)acc[c][i][r];
+            for (int r = 0; r < 16; ++r)
+                acc64[i][r] += F16 ? ldexp((double)acc[c][i][r], -8 * c) : (double)acc[c][i][r];
             acc[c][i] = v16f{};
         }
     };
@@ -371,8 +391,7 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
             for (int q = 0; q + p < S; ++q)
 #pragma unroll
                 for (int i = 0; i < MI; ++i)
-                    acc[set_of(p + q)][i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                        cur.b[q], cur.a[p][i], acc[set_of(p + q)][i], 0, 0, 0);
+                    acc[set_of(p + q)][i] = slice_mfma<F16>(cur.b[q], cur.a[p][i], acc[set_of(p + q)][i]);
             __builtin_amdgcn_sched_barrier(0);
             if (more) {
                 read_a(nxt, p, std::integral_constant<int, J1>{});
@@ -463,7 +482,7 @@ SplitWs split_workspace(int R1, int R2, int K)
 
 // Exponents and slices of an R x K operand given by (X, ld, rcontig).
 int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, long ld, int rcontig,
-               Split &out, int *bad, double *part)
+               int f16, Split &out, int *bad, double *part)
 {
     out.Rp = ceil_div(R, 128) * 128;
     out.e = (int *)gpublas_dev_alloc((size_t)R * sizeof(int));
@@ -480,20 +499,21 @@ int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, lon
         hipLaunchKernelGGL(exp_rows_contig, dim3((unsigned)ceil_div(R, 4)), dim3(256), 0, st, R, K,
                            X, ld, out.e, bad);
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 4)),
-                       dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, out.S);
+                       dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, f16, out.S);
     PROF_END(st);
     return (int)hipGetLastError();
 }
 
-template <int S>
+template <int S, bool F16>
 void launch_products(hipStream_t st, unsigned blocks, const OzArgs &g)
 {
     constexpr int TM = oz_tile_rows(S), NW = TM / 16, OCC = 8 / NW;
-    hipLaunchKernelGGL((ozaki_gemm_kernel<S, TM, NW, OCC>), dim3(blocks), dim3(64 * NW), 0, st, g);
+    hipLaunchKernelGGL((ozaki_gemm_kernel<S, TM, NW, OCC, F16>), dim3(blocks), dim3(64 * NW), 0, st,
+                       g);
 }
 
-int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, const Split &b,
-                 double alpha, double beta, double *C, int ldc, int tri)
+int run_products(hipStream_t st, int M, int N, int KT, int s, int f16, const Split &a,
+                 const Split &b, double alpha, double beta, double *C, int ldc, int tri)
 {
     OzArgs g;
     g.M = M;
@@ -514,15 +534,26 @@ int run_products(hipStream_t st, int M, int N, int KT, int s, const Split &a, co
     g.tiles_n = ceil_div(N, 64);
     const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
     PROF_BEGIN(st, "ozaki_gemm");
-    switch (s) {
-    case 1: launch_products<1>(st, blocks, g); break;
-    case 2: launch_products<2>(st, blocks, g); break;
-    case 3: launch_products<3>(st, blocks, g); break;
-    case 4: launch_products<4>(st, blocks, g); break;
-    case 5: launch_products<5>(st, blocks, g); break;
-    case 6: launch_products<6>(st, blocks, g); break;
-    case 7: launch_products<7>(st, blocks, g); break;
-    default: launch_products<8>(st, blocks, g); break;
+    if (f16) {
+        switch (s) {
+        case 1: launch_products<1, true>(st, blocks, g); break;
+        case 2: launch_products<2, true>(st, blocks, g); break;
+        case 3: launch_products<3, true>(st, blocks, g); break;
+        case 4: launch_products<4, true>(st, blocks, g); break;
+        case 5: launch_products<5, true>(st, blocks, g); break;
+        default: launch_products<6, true>(st, blocks, g); break;
+        }
+    } else {
+        switch (s) {
+        case 1: launch_products<1, false>(st, blocks, g); break;
+        case 2: launch_products<2, false>(st, blocks, g); break;
+        case 3: launch_products<3, false>(st, blocks, g); break;
+        case 4: launch_products<4, false>(st, blocks, g); break;
+        case 5: launch_products<5, false>(st, blocks, g); break;
+        case 6: launch_products<6, false>(st, blocks, g); break;
+        case 7: launch_products<7, false>(st, blocks, g); break;
+        default: launch_products<8, false>(st, blocks, g); break;
+        }
     }
     PROF_END(st);
     return (int)hipGetLastError();
@@ -538,15 +569,18 @@ int gpublas_dev_dsyrk(void *stream, char uplo, char trans, int n, int k, double 
 
 extern "C" {
 
-int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, int m, int n,
-                            int k, double alpha, const double *d_A, int lda, const double *d_B,
-                            int ldb, double beta, double *d_C, int ldc)
+int gpublas_dev_dgemm_split_fmt(void *stream, int slices, int fmt, char transa, char transb, int m,
+                                int n, int k, double alpha, const double *d_A, int lda,
+                                const double *d_B, int ldb, double beta, double *d_C, int ldc)
 {
     hipStream_t st = pick_stream(stream);
     if (m <= 0 || n <= 0) return 0;
-    if (slices < 1 || slices > OZ_MAXS) return (int)hipErrorInvalidValue;
+    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16) return (int)hipErrorInvalidValue;
+    const int maxs = fmt == GPUBLAS_SLICE_FP16 ? 6 : OZ_MAXS;
+    if (slices < 1 || slices > maxs) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+    const int f16 = fmt == GPUBLAS_SLICE_FP16;
     const int KT = ceil_div(k, 16);
     const SplitWs ws = split_workspace(m, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
@@ -554,9 +588,9 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{}, b{};
     // op(A) rows: A 'N' -> X[r + k*lda] (rows contiguous); 'T' -> X[k + r*lda]
-    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), a, bad, ws.part);
+    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), f16, a, bad, ws.part);
     // op(B) columns: B 'N' -> element (k, j) at B[k + j*ldb] (k contiguous); 'T' -> B[j + k*ldb]
-    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), b, bad, ws.part);
+    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), f16, b, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
@@ -564,7 +598,7 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
         if (hbad)  // Inf / NaN input: the splitting is undefined, use the FP64 kernel
             err = gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
         else
-            err = run_products(st, m, n, KT, slices, a, b, alpha, beta, d_C, ldc, 0);
+            err = run_products(st, m, n, KT, slices, f16, a, b, alpha, beta, d_C, ldc, 0);
     }
     (void)hipStreamSynchronize(st);
     gpublas_dev_release(a.S);
@@ -574,15 +608,26 @@ int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, 
     return err;
 }
 
-int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int n, int k,
-                            double alpha, const double *d_A, int lda, double beta, double *d_C,
-                            int ldc)
+int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, int m, int n,
+                            int k, double alpha, const double *d_A, int lda, const double *d_B,
+                            int ldb, double beta, double *d_C, int ldc)
+{
+    return gpublas_dev_dgemm_split_fmt(stream, slices, GPUBLAS_SLICE_BF16, transa, transb, m, n, k,
+                                       alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+}
+
+int gpublas_dev_dsyrk_split_fmt(void *stream, int slices, int fmt, char uplo, char trans, int n,
+                                int k, double alpha, const double *d_A, int lda, double beta,
+                                double *d_C, int ldc)
 {
     hipStream_t st = pick_stream(stream);
     if (n <= 0) return 0;
-    if (slices < 1 || slices > OZ_MAXS) return (int)hipErrorInvalidValue;
+    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16) return (int)hipErrorInvalidValue;
+    const int maxs = fmt == GPUBLAS_SLICE_FP16 ? 6 : OZ_MAXS;
+    if (slices < 1 || slices > maxs) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
+    const int f16 = fmt == GPUBLAS_SLICE_FP16;
     const int KT = ceil_div(k, 16);
     const SplitWs ws = split_workspace(n, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
@@ -590,19 +635,27 @@ int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{};
     // op(A) = A (trans N: rows contiguous) or A^T; op(B) = op(A)^T has the same lines
-    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), a, bad, ws.part);
+    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), f16, a, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
     if (!err) {
         if (hbad) err = gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
-        else err = run_products(st, n, n, KT, slices, a, a, alpha, beta, d_C, ldc,
+        else err = run_products(st, n, n, KT, slices, f16, a, a, alpha, beta, d_C, ldc,
                                 (uplo == 'U' || uplo == 'u') ? 2 : 1);
     }
     (void)hipStreamSynchronize(st);
     gpublas_dev_release(a.S);
     gpublas_dev_release(a.e);
     return err;
+}
+
+int gpublas_dev_dsyrk_split(void *stream, int slices, char uplo, char trans, int n, int k,
+                            double alpha, const double *d_A, int lda, double beta, double *d_C,
+                            int ldc)
+{
+    return gpublas_dev_dsyrk_split_fmt(stream, slices, GPUBLAS_SLICE_BF16, uplo, trans, n, k, alpha,
+                                       d_A, lda, beta, d_C, ldc);
 }
 
 }  // extern "C"

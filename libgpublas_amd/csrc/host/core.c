@@ -80,8 +80,9 @@ static void print_help(void)
            "  cutoff=<n>           GEMM leading dimensions below n go to the host BLAS\n"
            "                       (512; 0 = always the GPU)\n"
            "  profile              time every kernel with HIP events\n"
-           "  split=<s>            DGEMM/DSYRK as split-precision BF16 slice products\n"
+           "  split=<s>            DGEMM/DSYRK as split-precision slice products\n"
            "                       (s = 1..8 slices per operand; 0 = plain FP64, default)\n"
+           "  slice=<bf16|fp16>    slice format of split (bf16; fp16 takes s <= 6)\n"
            "  serve[=<bytes>]      run DGEMM/SGEMM/DGEMV/left DTRSM calls whose operands are\n"
            "                       all tracked through the batching serving loop, with an\n"
            "                       HBM admission budget (64G); serve=0 turns it off\n"
@@ -125,6 +126,8 @@ int gb_parse_options(const char *s)
                 bad = 1;
             }
         }
+        else if (!strcmp(opt, "slice=bf16")) gb_opt.split_fmt = GPUBLAS_SLICE_BF16;
+        else if (!strcmp(opt, "slice=fp16")) gb_opt.split_fmt = GPUBLAS_SLICE_FP16;
         else if (!strcmp(opt, "serve")) gb_opt.serve = 1;
         else if (!strncmp(opt, "serve=", 6)) {
             const size_t b = parse_size(opt + 6);
@@ -180,6 +183,11 @@ int gb_parse_options(const char *s)
             gb_log(2, "libgpublas: unknown option '%s' (GPUBLAS_OPTIONS=help)\n", opt);
             bad = 1;
         }
+    }
+    if (gb_opt.split_fmt == GPUBLAS_SLICE_FP16 && gb_opt.split > 6) {
+        gb_log(2, "libgpublas: slice=fp16 takes at most 6 slices; using split=6\n");
+        gb_opt.split = 6;
+        bad = 1;
     }
     return bad;
 }

@@ -34,6 +34,7 @@ struct gb_options {
     int cutoff;              /* gemm fall-through cutoff on leading dims */
     int profile;             /* kernel event timing */
     int split;               /* >0: DGEMM/DSYRK as split-precision with this many slices */
+    int split_fmt;           /* GPUBLAS_SLICE_BF16 (default) or GPUBLAS_SLICE_FP16 */
     int serve;               /* route calls on tracked operands through the serving loop */
     size_t serve_budget;     /* HBM admission budget of the serving loop (bytes) */
     int serve_slots;         /* batches in flight */

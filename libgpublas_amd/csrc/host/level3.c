@@ -28,8 +28,8 @@ static int dgemm_dev(void *s, char ta, char tb, int m, int n, int k, double alph
                      int ldc)
 {
     if (gb_opt.split > 0)
-        return gpublas_dev_dgemm_split(s, gb_opt.split, ta, tb, m, n, k, alpha, A, lda, B, ldb,
-                                       beta, C, ldc);
+        return gpublas_dev_dgemm_split_fmt(s, gb_opt.split, gb_opt.split_fmt, ta, tb, m, n, k,
+                                           alpha, A, lda, B, ldb, beta, C, ldc);
     return gpublas_dev_dgemm(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
@@ -37,8 +37,8 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
                      int lda, double beta, double *C, int ldc)
 {
     if (gb_opt.split > 0)
-        return gpublas_dev_dsyrk_split(s, gb_opt.split, uplo, trans, n, k, alpha, A, lda, beta, C,
-                                       ldc);
+        return gpublas_dev_dsyrk_split_fmt(s, gb_opt.split, gb_opt.split_fmt, uplo, trans, n, k,
+                                           alpha, A, lda, beta, C, ldc);
     return gpublas_dev_dsyrk(s, uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
 }
 

@@ -13,7 +13,11 @@ Bars:
     per-row/column exponents exist for): the same 1e-13 normwise bound;
   * fewer slices give a larger error (monotone in s), so the bound is not met
     by accident;
-  * a NaN input falls back to the FP64 kernel (NaN propagates like DGEMM).
+  * a NaN input falls back to the FP64 kernel (NaN propagates like DGEMM);
+  * BF16 slices (pre-scaled by 2^-8p) and FP16 slices (the integers, each
+    diagonal's power-of-two weight applied in FP64) give bit-identical
+    results: the weights are powers of two and no partial sum is subnormal.
+Every test runs with both slice formats.
 """
 import numpy as np
 import pytest
@@ -23,12 +27,13 @@ from _oracle import c, p, uniform
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture()
-def split(gpu):
+@pytest.fixture(params=["bf16", "fp16"])
+def split(gpu, request):
+    """Split mode on, with either slice format (every test runs for both)."""
     def use(s):
-        assert gpu.set_options(f"split={s}") == 0
+        assert gpu.set_options(f"split={s};slice={request.param}") == 0
     yield use
-    gpu.set_options("split=0")
+    gpu.set_options("split=0;slice=bf16")
 
 
 def _ref(ora, ta, tb, m, n, k, alpha, A, B, beta, C0):
@@ -117,3 +122,20 @@ def test_split_nan_falls_back_to_fp64(gpu, split):
     split(6)
     gpu.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
     assert np.all(np.isnan(C[3, :])) and np.all(np.isfinite(np.delete(C, 3, axis=0)))
+
+
+def test_split_fp16_matches_bf16_bitwise(gpu):
+    m, n, k = 256, 192, 900
+    A = uniform(71, (m, k))
+    B = uniform(72, (k, n))
+    C0 = uniform(73, (m, n))
+    out = {}
+    try:
+        for fmt in ("bf16", "fp16"):
+            assert gpu.set_options(f"split=6;slice={fmt}") == 0
+            C = C0.copy(order="F")
+            gpu.dgemm("N", "N", m, n, k, 1.3, A, m, B, k, -0.7, C, m)
+            out[fmt] = C
+    finally:
+        gpu.set_options("split=0;slice=bf16")
+    assert np.array_equal(out["bf16"], out["fp16"])
