@@ -11,6 +11,9 @@
  *       dsyrk_/ssyrk_  blas_level3/syrk.cc:148-180 (blas.h:268-274)
  *       dtrsm_/strsm_  blas_level3/trsm.cc:134-172 (blas.h:303-314)
  *       dsymm_/ssymm_  blas_level3/symm.cc:124-156 (blas.h:254-266)
+ *       dsyr2k_/ssyr2k_ blas_level3/syr2k.cc:146-180 (blas.h:276-286)
+ *       dtrmm_/strmm_  blas_level3/trmm.cc:139-175 (blas.h:290-299; not
+ *                      built by the reference, meson.build:87: netlib DTRMM)
  *       dgemv_         blas_level2/gemv.cc:90-98 (blas.h:85-92)
  *       dger_          blas_level2/ger.cc:86-94
  *       dtrsv_         blas_level2/trsv.cc:81-88
@@ -65,6 +68,18 @@ void dsymm_(const char *side, const char *uplo, const int *m, const int *n, cons
 void ssymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
             const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
             float *c, const int *ldc);
+void dsyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+             const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+             double *c, const int *ldc);
+void ssyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+             const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+             float *c, const int *ldc);
+void dtrmm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const double *alpha, const double *a, const int *lda, double *b,
+            const int *ldb);
+void strmm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const float *alpha, const float *a, const int *lda, float *b,
+            const int *ldb);
 void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, const double *a,
             const int *lda, const double *x, const int *incx, const double *beta, double *y,
             const int *incy);
@@ -100,6 +115,12 @@ void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TR
 void cblas_dsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n, double alpha,
                  const double *a, int lda, const double *b, int ldb, double beta, double *c,
                  int ldc);
+void cblas_dsyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  double alpha, const double *a, int lda, const double *b, int ldb, double beta,
+                  double *c, int ldc);
+void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
+                 int ldb);
 void cblas_dgemv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, double alpha,
                  const double *a, int lda, const double *x, int incx, double beta, double *y,
                  int incy);

@@ -85,6 +85,20 @@ int gpublas_dev_dtrsm(void *stream, char side, char uplo, char transa, char diag
                       double alpha, const double *d_A, int lda, double *d_B, int ldb);
 int gpublas_dev_strsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
                       float alpha, const float *d_A, int lda, float *d_B, int ldb);
+/* Rank-2k update and triangular multiply on the same tiles (netlib DSYR2K /
+ * DTRMM semantics; the reference builds syr2k.cc:164-180 and leaves trmm.cc
+ * disabled, meson.build:84-87).  TRMM uses two device workspaces of the
+ * pool: the expanded triangle and the product before it is copied into B. */
+int gpublas_dev_dsyr2k(void *stream, char uplo, char trans, int n, int k, double alpha,
+                       const double *d_A, int lda, const double *d_B, int ldb, double beta,
+                       double *d_C, int ldc);
+int gpublas_dev_ssyr2k(void *stream, char uplo, char trans, int n, int k, float alpha,
+                       const float *d_A, int lda, const float *d_B, int ldb, float beta,
+                       float *d_C, int ldc);
+int gpublas_dev_dtrmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      double alpha, const double *d_A, int lda, double *d_B, int ldb);
+int gpublas_dev_strmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      float alpha, const float *d_A, int lda, float *d_B, int ldb);
 
 /* Split-precision DGEMM / DSYRK (opt-in; no counterpart in the reference):
  * Ozaki integer-slice scheme, `slices` signed 8-bit slices per operand, slice

@@ -176,6 +176,26 @@ def strsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
                      _i(lda), _p(b), _i(ldb))
 
 
+def dsyr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().dsyr2k_(_c(uplo), _c(trans), _i(n), _i(k), _d(alpha), _p(a), _i(lda), _p(b),
+                      _i(ldb), _d(beta), _p(c), _i(ldc))
+
+
+def ssyr2k(uplo, trans, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    library().ssyr2k_(_c(uplo), _c(trans), _i(n), _i(k), _f(alpha), _p(a), _i(lda), _p(b),
+                      _i(ldb), _f(beta), _p(c), _i(ldc))
+
+
+def dtrmm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    library().dtrmm_(_c(side), _c(uplo), _c(transa), _c(diag), _i(m), _i(n), _d(alpha), _p(a),
+                     _i(lda), _p(b), _i(ldb))
+
+
+def strmm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    library().strmm_(_c(side), _c(uplo), _c(transa), _c(diag), _i(m), _i(n), _f(alpha), _p(a),
+                     _i(lda), _p(b), _i(ldb))
+
+
 def dsymm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
     library().dsymm_(_c(side), _c(uplo), _i(m), _i(n), _d(alpha), _p(a), _i(lda), _p(b),
                      _i(ldb), _d(beta), _p(c), _i(ldc))

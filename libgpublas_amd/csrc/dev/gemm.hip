@@ -418,6 +418,29 @@ __global__ __launch_bounds__(256) void symmetrize_kernel(int n, const T *__restr
     }
 }
 
+// Full n x n matrix of a triangle of A: the stored triangle copied, zeros
+// elsewhere, ones on the diagonal when `unit` (the diagonal of A is then not
+// read).  One workgroup per 64 x 64 tile; coalesced on both sides.
+template <typename T>
+__global__ __launch_bounds__(256) void triangle_kernel(int n, const T *__restrict__ A, long lda,
+                                                       int lower, int unit, T *__restrict__ Tm,
+                                                       long ldt)
+{
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    T v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, i = i0 + (idx & 63), j = j0 + (idx >> 6);
+        const bool in = i < n && j < n && (lower ? i >= j : i <= j) && !(unit && i == j);
+        v[u] = in ? A[i + (long)j * lda] : (unit && i == j ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, i = i0 + (idx & 63), j = j0 + (idx >> 6);
+        if (i < n && j < n) Tm[i + (long)j * ldt] = v[u];
+    }
+}
+
 template <typename T>
 int symm_entry(void *stream, char side, char uplo, int m, int n, T alpha, const T *A, int lda,
                const T *B, int ldb, T beta, T *C, int ldc)
@@ -471,6 +494,74 @@ int symm_entry(void *stream, char side, char uplo, int m, int n, T alpha, const 
     return launch_gemm<T>(s, m, n, n, alpha, a, b, beta, C, ldc, SYM_NONE);
 }
 
+// C = alpha*op(A)*op(B)^T + alpha*op(B)*op(A)^T + beta*C on the uplo triangle
+// (op(X) = X, n x k, for trans N; X^T for trans T): two triangular-grid
+// products, the second accumulating onto the first.
+template <typename T>
+int syr2k_entry(void *stream, char uplo, char trans, int n, int k, T alpha, const T *A, int lda,
+                const T *B, int ldb, T beta, T *C, int ldc)
+{
+    hipStream_t s = pick_stream(stream);
+    const int tri = is_u(uplo) ? SYM_UPPER : SYM_LOWER;
+    if (n <= 0) return 0;
+    if (alpha == T(0) || k <= 0) {
+        if (beta == T(1)) return 0;
+        return launch_scale<T>(s, n, n, beta, C, ldc, tri);
+    }
+    const bool nt = is_n(trans);
+    Operand<T> a = make_op<T>(A, lda, n, nt ? PAT_OUTER : PAT_KCONT, SYM_NONE);
+    Operand<T> b = make_op<T>(B, ldb, n, nt ? PAT_OUTER : PAT_KCONT, SYM_NONE);
+    int err = launch_gemm<T>(s, n, n, k, alpha, a, b, beta, C, ldc, tri);
+    if (!err) err = launch_gemm<T>(s, n, n, k, alpha, b, a, T(1), C, ldc, tri);
+    return err;
+}
+
+// B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R), A triangular: the
+// triangle is expanded into a full matrix (zeros outside it, ones on a unit
+// diagonal), the product runs on the GEMM kernels into a workspace, which is
+// then copied over B.
+template <typename T>
+int trmm_entry(void *stream, char side, char uplo, char transa, char diag, int m, int n, T alpha,
+               const T *A, int lda, T *B, int ldb)
+{
+    hipStream_t s = pick_stream(stream);
+    if (m <= 0 || n <= 0) return 0;
+    if (alpha == T(0)) return launch_scale<T>(s, m, n, T(0), B, ldb, SYM_NONE);
+    const bool left = is_l(side);
+    const int na = left ? m : n;
+    const long ldt = (na + 7) / 8 * 8, ldw = (m + 7) / 8 * 8;
+    T *Tm = (T *)gpublas_dev_alloc((size_t)ldt * na * sizeof(T));
+    T *W = (T *)gpublas_dev_alloc((size_t)ldw * n * sizeof(T));
+    if (!Tm || !W) {
+        gpublas_dev_release(Tm);
+        gpublas_dev_release(W);
+        return (int)hipErrorOutOfMemory;
+    }
+    const unsigned tiles = (unsigned)ceil_div(na, 64);
+    PROF_BEGIN(s, "triangle");
+    hipLaunchKernelGGL(triangle_kernel<T>, dim3(tiles, tiles), dim3(256), 0, s, na, A, (long)lda,
+                       is_u(uplo) ? 0 : 1, (diag == 'U' || diag == 'u') ? 1 : 0, Tm, ldt);
+    PROF_END(s);
+    int err = (int)hipGetLastError();
+    const bool tn = is_n(transa);
+    if (!err) {
+        if (left)
+            err = launch_gemm<T>(s, m, n, m, alpha, make_op<T>(Tm, ldt, m, tn ? PAT_OUTER : PAT_KCONT,
+                                                                SYM_NONE),
+                                 make_op<T>(B, ldb, n, PAT_KCONT, SYM_NONE), T(0), W, ldw, SYM_NONE);
+        else
+            err = launch_gemm<T>(s, m, n, n, alpha, make_op<T>(B, ldb, m, PAT_OUTER, SYM_NONE),
+                                 make_op<T>(Tm, ldt, n, tn ? PAT_KCONT : PAT_OUTER, SYM_NONE), T(0),
+                                 W, ldw, SYM_NONE);
+    }
+    if (!err)
+        err = (int)hipMemcpy2DAsync(B, (size_t)ldb * sizeof(T), W, (size_t)ldw * sizeof(T),
+                                    (size_t)m * sizeof(T), (size_t)n, hipMemcpyDeviceToDevice, s);
+    dev_release_after(Tm, s);
+    dev_release_after(W, s);
+    return err;
+}
+
 }  // namespace
 
 // Shared with trsm.hip: plain GEMM on device pointers.
@@ -514,6 +605,32 @@ int gpublas_dev_ssyrk(void *stream, char uplo, char trans, int n, int k, float a
                       const float *d_A, int lda, float beta, float *d_C, int ldc)
 {
     return syrk_entry<float>(stream, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
+}
+
+int gpublas_dev_dsyr2k(void *stream, char uplo, char trans, int n, int k, double alpha,
+                       const double *d_A, int lda, const double *d_B, int ldb, double beta,
+                       double *d_C, int ldc)
+{
+    return syr2k_entry<double>(stream, uplo, trans, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+}
+
+int gpublas_dev_ssyr2k(void *stream, char uplo, char trans, int n, int k, float alpha,
+                       const float *d_A, int lda, const float *d_B, int ldb, float beta,
+                       float *d_C, int ldc)
+{
+    return syr2k_entry<float>(stream, uplo, trans, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+}
+
+int gpublas_dev_dtrmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      double alpha, const double *d_A, int lda, double *d_B, int ldb)
+{
+    return trmm_entry<double>(stream, side, uplo, transa, diag, m, n, alpha, d_A, lda, d_B, ldb);
+}
+
+int gpublas_dev_strmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      float alpha, const float *d_A, int lda, float *d_B, int ldb)
+{
+    return trmm_entry<float>(stream, side, uplo, transa, diag, m, n, alpha, d_A, lda, d_B, ldb);
 }
 
 int gpublas_dev_dsymm(void *stream, char side, char uplo, int m, int n, double alpha,

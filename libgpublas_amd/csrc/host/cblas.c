@@ -135,6 +135,42 @@ void cblas_dsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, i
     }
 }
 
+/* ---- SYR2K: 1 layout 2 uplo 3 trans 4 n 5 k 6 alpha 7 A 8 lda 9 B 10 ldb 11 beta 12 C
+ *      13 ldc (row major: the same update of C^T, uplo and trans flipped) ---- */
+static const int syr2k_map[13] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 10, 0, 0, 13};
+
+void cblas_dsyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  double alpha, const double *a, int lda, const double *b, int ldb, double beta,
+                  double *c, int ldc)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYR2K", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYR2K", 2); return; }
+    if (!tr_ok(trans)) { cbl_err("DSYR2K", 3); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    WITH_MAP(syr2k_map, dsyr2k_(&u, &t, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
+}
+
+/* ---- TRMM: 1 layout 2 side 3 uplo 4 ta 5 diag 6 m 7 n 8 alpha 9 A 10 lda 11 B 12 ldb ---- */
+void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
+                 int ldb)
+{
+    if (!lay_ok(layout)) { cbl_err("DTRMM ", 1); return; }
+    if (!side_ok(side)) { cbl_err("DTRMM ", 2); return; }
+    if (!up_ok(uplo)) { cbl_err("DTRMM ", 3); return; }
+    if (!tr_ok(ta)) { cbl_err("DTRMM ", 4); return; }
+    if (!diag_ok(diag)) { cbl_err("DTRMM ", 5); return; }
+    char t = tr(ta), d = dg(diag);
+    if (layout == CblasColMajor) {
+        char s = sd(side), u = up(uplo);
+        WITH_MAP(trsm_col, dtrmm_(&s, &u, &t, &d, &m, &n, &alpha, a, &lda, b, &ldb));
+    } else {
+        char s = sdflip(side), u = upflip(uplo);
+        WITH_MAP(trsm_row, dtrmm_(&s, &u, &t, &d, &n, &m, &alpha, a, &lda, b, &ldb));
+    }
+}
+
 /* ---- GEMV: 1 layout 2 trans 3 m 4 n 5 alpha 6 A 7 lda 8 x 9 incx 10 beta 11 y 12 incy ---- */
 static const int gemv_col[12] = {0, 2, 3, 4, 0, 0, 7, 0, 9, 0, 0, 12};
 static const int gemv_row[12] = {0, 2, 4, 3, 0, 0, 7, 0, 9, 0, 0, 12};

@@ -1,5 +1,5 @@
 /*
- * level3.c -- Fortran-ABI Level-3 entry points: ?gemm_, ?syrk_, ?trsm_,
+ * level3.c -- Fortran-ABI Level-3 entry points: ?gemm_, ?syrk_, ?syr2k_, ?trmm_, ?trsm_,
  * ?symm_ for real single and double precision.
  *
  * Each entry follows the reference's three steps (blas_level3/gemm.cc:87-179
@@ -174,6 +174,123 @@ DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm)
 
 DEFINE_SYRK(d, double, "DSYRK ", dsyrk_dev)
 DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
+
+/* ------------------------------------------------------------------ */
+/* SYR2K (netlib DSYR2K; the reference's syr2k.cc:78-146 check, info   */
+/* 1 uplo, 2 trans, 3 n, 4 k, 7 lda, 9 ldb, 12 ldc)                    */
+/* ------------------------------------------------------------------ */
+#define DEFINE_SYR2K(PRE, T, NAME6, DEVFN)                                                       \
+    void PRE##syr2k_(const char *uplo, const char *trans, const int *n, const int *k,            \
+                     const T *alpha, const T *a, const int *lda, const T *b, const int *ldb,     \
+                     const T *beta, T *c, const int *ldc)                                        \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int notr = gb_lsame(*trans, 'N');                                                  \
+        const int nrowa = notr ? *n : *k;                                                        \
+        const int upper = gb_lsame(*uplo, 'U');                                                  \
+        int info = 0;                                                                            \
+        if (!upper && !gb_lsame(*uplo, 'L')) info = 1;                                           \
+        else if (!notr && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 2;            \
+        else if (*n < 0) info = 3;                                                               \
+        else if (*k < 0) info = 4;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 7;                                                \
+        else if (*ldb < MAXI(1, nrowa)) info = 9;                                                \
+        else if (*ldc < MAXI(1, *n)) info = 12;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*n == 0 || ((*alpha == 0 || *k == 0) && *beta == 1)) {                               \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int scale_only = (*alpha == 0 || *k == 0);                                         \
+        if (scale_only && !gb_ptr_tracked(c)) {                                                  \
+            for (int j = 0; j < *n; ++j) {                                                       \
+                const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : *n;                           \
+                for (int i = i0; i < i1; ++i)                                                    \
+                    c[IDX(i, j, *ldc)] = *beta == 0 ? (T)0 : *beta * c[IDX(i, j, *ldc)];         \
+            }                                                                                    \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (!scale_only) {                                                                       \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, notr ? *k : *n, sizeof(T), GB_IN, s);     \
+            gb_acquire_matrix(&ops[1], b, *ldb, nrowa, notr ? *k : *n, sizeof(T), GB_IN, s);     \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *n, *n, sizeof(T), GB_INOUT, s);                     \
+        gb_check(DEVFN(s, *uplo, *trans, *n, *k, *alpha, (const T *)ops[0].dev, *lda,            \
+                       (const T *)ops[1].dev, *ldb, *beta, (T *)ops[2].dev, *ldc),               \
+                 #PRE "syr2k launch");                                                           \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_SYR2K(d, double, "DSYR2K", gpublas_dev_dsyr2k)
+DEFINE_SYR2K(s, float, "SSYR2K", gpublas_dev_ssyr2k)
+
+/* ------------------------------------------------------------------ */
+/* TRMM (netlib DTRMM: info 1 side, 2 uplo, 3 transa, 4 diag, 5 m,     */
+/* 6 n, 9 lda, 11 ldb; trmm.cc is not built by the reference)          */
+/* ------------------------------------------------------------------ */
+#define DEFINE_TRMM(PRE, T, NAME6, DEVFN)                                                        \
+    void PRE##trmm_(const char *side, const char *uplo, const char *transa, const char *diag,    \
+                    const int *m, const int *n, const T *alpha, const T *a, const int *lda,      \
+                    T *b, const int *ldb)                                                        \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int lside = gb_lsame(*side, 'L');                                                  \
+        const int nrowa = lside ? *m : *n;                                                       \
+        const int upper = gb_lsame(*uplo, 'U');                                                  \
+        int info = 0;                                                                            \
+        if (!lside && !gb_lsame(*side, 'R')) info = 1;                                           \
+        else if (!upper && !gb_lsame(*uplo, 'L')) info = 2;                                      \
+        else if (!gb_lsame(*transa, 'N') && !gb_lsame(*transa, 'T') && !gb_lsame(*transa, 'C'))  \
+            info = 3;                                                                            \
+        else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 4;                       \
+        else if (*m < 0) info = 5;                                                               \
+        else if (*n < 0) info = 6;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 9;                                                \
+        else if (*ldb < MAXI(1, *m)) info = 11;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0) {                                                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*alpha == 0 && !gb_ptr_tracked(b)) {                                                 \
+            for (int j = 0; j < *n; ++j)                                                         \
+                for (int i = 0; i < *m; ++i) b[IDX(i, j, *ldb)] = 0;                             \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[2] = {{0}};                                                               \
+        if (*alpha != 0)                                                                         \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nrowa, sizeof(T), GB_IN, s);              \
+        gb_acquire_matrix(&ops[1], b, *ldb, *m, *n, sizeof(T), *alpha == 0 ? GB_OUT : GB_INOUT,  \
+                          s);                                                                    \
+        gb_check(DEVFN(s, *side, *uplo, *transa, *diag, *m, *n, *alpha, (const T *)ops[0].dev,   \
+                       *lda, (T *)ops[1].dev, *ldb),                                             \
+                 #PRE "trmm launch");                                                            \
+        gb_finish(s, ops, 2);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_TRMM(d, double, "DTRMM ", gpublas_dev_dtrmm)
+DEFINE_TRMM(s, float, "STRMM ", gpublas_dev_strmm)
 
 /* ------------------------------------------------------------------ */
 /* TRSM                                                                */

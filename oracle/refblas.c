@@ -7,6 +7,9 @@
  *   syrk  syrk.cc:78-146   (info 1,2,3,4,7,10)
  *   trsm  trsm.cc:76-132   (info 1,2,3,4,5,6,9,11)
  *   symm  symm.cc:81-122   (info 1,2,3,4,7,9,12)
+ *   syr2k syr2k.cc:79-114  (info 1,2,3,4,7,9,12)
+ *   trmm  netlib DTRMM     (info 1,2,3,4,5,6,9,11; trmm.cc is not built,
+ *                           meson.build:87)
  * and, for routines the reference only specifies through its unbuilt CBLAS
  * wrappers (blas_level1/axpy.cc, dot.cc, nrm2.cc, amax.cc,
  * blas_level2/gemv.cc, ger.cc, trsv.cc), the netlib reference BLAS
@@ -397,6 +400,190 @@ void NAME(char side, char uplo, int m, int n, T alpha, const T *A, int lda, cons
 }
 DEFINE_SYMM(ora_dsymm, double)
 DEFINE_SYMM(ora_ssymm, float)
+
+/* ------------------------------------------------------------------ */
+/* DSYR2K: info per syr2k.cc:79-114 (1 uplo, 2 trans, 3 n, 4 k, 7 lda,  */
+/* 9 ldb, 12 ldc); arithmetic in the netlib DSYR2K loop order          */
+/* ------------------------------------------------------------------ */
+int ora_syr2k_info(char uplo, char trans, int n, int k, int lda, int ldb, int ldc)
+{
+    int nrowa = ora_lsame(trans, 'N') ? n : k;
+    int upper = ora_lsame(uplo, 'U');
+    if (!upper && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (n < 0) return 3;
+    if (k < 0) return 4;
+    if (lda < imax(1, nrowa)) return 7;
+    if (ldb < imax(1, nrowa)) return 9;
+    if (ldc < imax(1, n)) return 12;
+    return 0;
+}
+
+void ora_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc)
+{
+    int upper = ora_lsame(uplo, 'U');
+    if (ora_syr2k_info(uplo, trans, n, k, lda, ldb, ldc)) return;
+    if (n == 0 || ((alpha == 0 || k == 0) && beta == 1)) return;
+    if (alpha == 0) {
+        for (int j = 0; j < n; ++j) {
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+            for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] = beta == 0 ? 0 : beta * C[IX(i, j, ldc)];
+        }
+        return;
+    }
+    if (ora_lsame(trans, 'N')) {
+        /* C := alpha*A*B**T + alpha*B*A**T + beta*C, one column at a time */
+        for (int j = 0; j < n; ++j) {
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+            if (beta == 0)
+                for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] = 0;
+            else if (beta != 1)
+                for (int i = i0; i < i1; ++i) C[IX(i, j, ldc)] *= beta;
+            for (int l = 0; l < k; ++l) {
+                if (A[IX(j, l, lda)] == 0 && B[IX(j, l, ldb)] == 0) continue;
+                double t1 = alpha * B[IX(j, l, ldb)], t2 = alpha * A[IX(j, l, lda)];
+                for (int i = i0; i < i1; ++i)
+                    C[IX(i, j, ldc)] += A[IX(i, l, lda)] * t1 + B[IX(i, l, ldb)] * t2;
+            }
+        }
+    } else {
+        /* C := alpha*A**T*B + alpha*B**T*A + beta*C, dot products */
+        for (int j = 0; j < n; ++j) {
+            int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+            for (int i = i0; i < i1; ++i) {
+                double t1 = 0, t2 = 0;
+                for (int l = 0; l < k; ++l) {
+                    t1 += A[IX(l, i, lda)] * B[IX(l, j, ldb)];
+                    t2 += B[IX(l, i, ldb)] * A[IX(l, j, lda)];
+                }
+                C[IX(i, j, ldc)] = beta == 0 ? alpha * t1 + alpha * t2
+                                             : beta * C[IX(i, j, ldc)] + alpha * t1 + alpha * t2;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* DTRMM: netlib info (1 side, 2 uplo, 3 transa, 4 diag, 5 m, 6 n,      */
+/* 9 lda, 11 ldb); the reference does not build trmm.cc                */
+/* (meson.build:87), so the netlib DTRMM definition is the oracle      */
+/* ------------------------------------------------------------------ */
+int ora_trmm_info(char side, char uplo, char ta, char diag, int m, int n, int lda, int ldb)
+{
+    return ora_trsm_info(side, uplo, ta, diag, m, n, lda, ldb);
+}
+
+void ora_dtrmm(char side, char uplo, char ta, char diag, int m, int n, double alpha,
+               const double *A, int lda, double *B, int ldb)
+{
+    if (ora_trmm_info(side, uplo, ta, diag, m, n, lda, ldb)) return;
+    if (m == 0 || n == 0) return;
+    int lside = ora_lsame(side, 'L'), upper = ora_lsame(uplo, 'U');
+    int notr = ora_lsame(ta, 'N'), nounit = ora_lsame(diag, 'N');
+    if (alpha == 0) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] = 0;
+        return;
+    }
+    if (lside) {
+        if (notr) {
+            /* B := alpha*A*B: each column of B, A applied column by column */
+            for (int j = 0; j < n; ++j) {
+                if (upper) {
+                    for (int kk = 0; kk < m; ++kk) {
+                        if (B[IX(kk, j, ldb)] == 0) continue;
+                        double t = alpha * B[IX(kk, j, ldb)];
+                        for (int i = 0; i < kk; ++i) B[IX(i, j, ldb)] += t * A[IX(i, kk, lda)];
+                        if (nounit) t *= A[IX(kk, kk, lda)];
+                        B[IX(kk, j, ldb)] = t;
+                    }
+                } else {
+                    for (int kk = m - 1; kk >= 0; --kk) {
+                        if (B[IX(kk, j, ldb)] == 0) continue;
+                        double t = alpha * B[IX(kk, j, ldb)];
+                        B[IX(kk, j, ldb)] = t;
+                        if (nounit) B[IX(kk, j, ldb)] *= A[IX(kk, kk, lda)];
+                        for (int i = kk + 1; i < m; ++i) B[IX(i, j, ldb)] += t * A[IX(i, kk, lda)];
+                    }
+                }
+            }
+        } else {
+            /* B := alpha*A**T*B: dot products down the columns of A */
+            for (int j = 0; j < n; ++j) {
+                if (upper) {
+                    for (int i = m - 1; i >= 0; --i) {
+                        double t = B[IX(i, j, ldb)];
+                        if (nounit) t *= A[IX(i, i, lda)];
+                        for (int kk = 0; kk < i; ++kk) t += A[IX(kk, i, lda)] * B[IX(kk, j, ldb)];
+                        B[IX(i, j, ldb)] = alpha * t;
+                    }
+                } else {
+                    for (int i = 0; i < m; ++i) {
+                        double t = B[IX(i, j, ldb)];
+                        if (nounit) t *= A[IX(i, i, lda)];
+                        for (int kk = i + 1; kk < m; ++kk) t += A[IX(kk, i, lda)] * B[IX(kk, j, ldb)];
+                        B[IX(i, j, ldb)] = alpha * t;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (notr) {
+        /* B := alpha*B*A: column j of the result from columns of B */
+        if (upper) {
+            for (int j = n - 1; j >= 0; --j) {
+                double t = alpha;
+                if (nounit) t *= A[IX(j, j, lda)];
+                for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= t;
+                for (int kk = 0; kk < j; ++kk) {
+                    if (A[IX(kk, j, lda)] == 0) continue;
+                    double u = alpha * A[IX(kk, j, lda)];
+                    for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] += u * B[IX(i, kk, ldb)];
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                double t = alpha;
+                if (nounit) t *= A[IX(j, j, lda)];
+                for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] *= t;
+                for (int kk = j + 1; kk < n; ++kk) {
+                    if (A[IX(kk, j, lda)] == 0) continue;
+                    double u = alpha * A[IX(kk, j, lda)];
+                    for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] += u * B[IX(i, kk, ldb)];
+                }
+            }
+        }
+    } else {
+        /* B := alpha*B*A**T */
+        if (upper) {
+            for (int kk = 0; kk < n; ++kk) {
+                for (int j = 0; j < kk; ++j) {
+                    if (A[IX(j, kk, lda)] == 0) continue;
+                    double u = alpha * A[IX(j, kk, lda)];
+                    for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] += u * B[IX(i, kk, ldb)];
+                }
+                double t = alpha;
+                if (nounit) t *= A[IX(kk, kk, lda)];
+                if (t != 1)
+                    for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= t;
+            }
+        } else {
+            for (int kk = n - 1; kk >= 0; --kk) {
+                for (int j = kk + 1; j < n; ++j) {
+                    if (A[IX(j, kk, lda)] == 0) continue;
+                    double u = alpha * A[IX(j, kk, lda)];
+                    for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] += u * B[IX(i, kk, ldb)];
+                }
+                double t = alpha;
+                if (nounit) t *= A[IX(kk, kk, lda)];
+                if (t != 1)
+                    for (int i = 0; i < m; ++i) B[IX(i, kk, ldb)] *= t;
+            }
+        }
+    }
+}
 
 /* ------------------------------------------------------------------ */
 /* Level 2                                                             */

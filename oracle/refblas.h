@@ -30,6 +30,12 @@ int ora_gemm_info(char ta, char tb, int m, int n, int k, int lda, int ldb, int l
 int ora_syrk_info(char uplo, char trans, int n, int k, int lda, int ldc);
 int ora_trsm_info(char side, char uplo, char ta, char diag, int m, int n, int lda, int ldb);
 int ora_symm_info(char side, char uplo, int m, int n, int lda, int ldb, int ldc);
+int ora_syr2k_info(char uplo, char trans, int n, int k, int lda, int ldb, int ldc);
+int ora_trmm_info(char side, char uplo, char ta, char diag, int m, int n, int lda, int ldb);
+void ora_dsyr2k(char uplo, char trans, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc);
+void ora_dtrmm(char side, char uplo, char ta, char diag, int m, int n, double alpha,
+               const double *A, int lda, double *B, int ldb);
 int ora_gemv_info(char trans, int m, int n, int lda, int incx, int incy);
 int ora_ger_info(int m, int n, int incx, int incy, int lda);
 int ora_trsv_info(char uplo, char trans, char diag, int n, int lda, int incx);

@@ -33,6 +33,10 @@ def load():
     L.ora_dtrsm.argtypes = [C, C, C, C, I, I, D, VP, I, VP, I]
     L.ora_strsm.argtypes = [C, C, C, C, I, I, F, VP, I, VP, I]
     L.ora_dsymm.argtypes = [C, C, I, I, D, VP, I, VP, I, D, VP, I]
+    L.ora_syr2k_info.argtypes = [C, C, I, I, I, I, I]
+    L.ora_trmm_info.argtypes = [C, C, C, C, I, I, I, I]
+    L.ora_dsyr2k.argtypes = [C, C, I, I, D, VP, I, VP, I, D, VP, I]
+    L.ora_dtrmm.argtypes = [C, C, C, C, I, I, D, VP, I, VP, I]
     L.ora_ssymm.argtypes = [C, C, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_dgemv.argtypes = [C, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_dger.argtypes = [I, I, D, VP, I, VP, I, VP, I]

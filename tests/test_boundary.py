@@ -82,6 +82,18 @@ def test_level3_xerbla_positions(ora):
     assert gb.last_xerbla() == ("DSYMM ", 12)
     gb.ssyrk("Q", "N", 3, 2, 1.0, d, 3, 0.0, d, 3)
     assert gb.last_xerbla() == ("SSYRK ", 1)
+    gb.dsyr2k("L", "T", 3, 4, 1.0, d, 4, d, 3, 0.0, d, 3)
+    assert gb.last_xerbla() == ("DSYR2K", ora.ora_syr2k_info(b"L", b"T", 3, 4, 4, 3, 3))
+    gb.dsyr2k("L", "N", 3, 2, 1.0, d, 3, d, 3, 0.0, d, 2)
+    assert gb.last_xerbla() == ("DSYR2K", 12)
+    gb.ssyr2k("L", "Q", 3, 2, 1.0, d, 3, d, 3, 0.0, d, 3)
+    assert gb.last_xerbla() == ("SSYR2K", 2)
+    gb.dtrmm("L", "L", "N", "X", 2, 2, 1.0, d, 2, d, 2)
+    assert gb.last_xerbla() == ("DTRMM ", 4)
+    gb.dtrmm("R", "L", "N", "N", 2, 3, 1.0, d, 2, d, 2)
+    assert gb.last_xerbla() == ("DTRMM ", ora.ora_trmm_info(b"R", b"L", b"N", b"N", 2, 3, 2, 2))
+    gb.strmm("L", "U", "T", "U", 3, 2, 1.0, d, 3, d, 2)
+    assert gb.last_xerbla() == ("STRMM ", 11)
 
 
 def test_level12_xerbla_positions():

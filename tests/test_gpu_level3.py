@@ -259,6 +259,82 @@ def test_dsymm_expanded_exact(gpu, ora, side, uplo, m, n):
     assert np.array_equal(C, 2.0 * full - C0)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k", [(7, 5), (130, 70), (300, 257)])
+def test_dsyr2k(gpu, ora, uplo, trans, n, k):
+    rows, cols = (n, k) if trans == "N" else (k, n)
+    A = uniform(35, (rows, cols))
+    B = uniform(36, (rows, cols))
+    C0 = uniform(37, (n, n))
+    C = C0.copy(order="F")
+    R = C0.copy(order="F")
+    before = _device_calls(gpu)
+    gpu.dsyr2k(uplo, trans, n, k, 0.9, A, rows, B, rows, 0.4, C, n)
+    assert _device_calls(gpu) == before + 1
+    ora.ora_dsyr2k(c(uplo), c(trans), n, k, 0.9, p(A), rows, p(B), rows, 0.4, p(R), n)
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    assert np.array_equal(C[~tri], C0[~tri])  # other triangle untouched
+    aA, aB = (np.abs(A), np.abs(B)) if trans == "N" else (np.abs(A).T, np.abs(B).T)
+    G = 0.9 * (aA @ aB.T + aB @ aA.T) + 0.4 * np.abs(C0)
+    ratio = np.max(np.abs(C - R)[tri] / (EPS64 * G[tri]))
+    assert ratio < 16.0, ratio
+
+
+def test_dsyr2k_integer_exact(gpu, ora):
+    rng = np.random.default_rng(51)
+    n, k = 290, 150
+    A = np.asfortranarray(rng.integers(-5, 6, size=(n, k)).astype(float))
+    B = np.asfortranarray(rng.integers(-5, 6, size=(n, k)).astype(float))
+    C = np.asfortranarray(rng.integers(-5, 6, size=(n, n)).astype(float))
+    R = C.copy(order="F")
+    gpu.dsyr2k("U", "N", n, k, 2.0, A, n, B, n, -1.0, C, n)
+    ora.ora_dsyr2k(c("U"), c("N"), n, k, 2.0, p(A), n, p(B), n, -1.0, p(R), n)
+    assert np.array_equal(C, R)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("m,n", [(7, 5), (100, 70), (300, 257)])
+def test_dtrmm(gpu, ora, side, uplo, ta, diag, m, n):
+    na = m if side == "L" else n
+    A = uniform(55, (na, na))
+    junk = np.triu(np.ones((na, na), bool), 1) if uplo == "L" else np.tril(np.ones((na, na), bool), -1)
+    A[junk] = ROGUE  # the other triangle must not be read
+    if diag == "U":
+        A[np.diag_indices(na)] = ROGUE  # nor the diagonal of a unit triangle
+    A = np.asfortranarray(A)
+    ldb = m + 2
+    B0 = _padded(uniform(56, (m, n)), ldb)
+    B = B0.copy(order="F")
+    R = B0.copy(order="F")
+    before = _device_calls(gpu)
+    gpu.dtrmm(side, uplo, ta, diag, m, n, 0.8, A, na, B, ldb)
+    assert _device_calls(gpu) == before + 1
+    ora.ora_dtrmm(c(side), c(uplo), c(ta), c(diag), m, n, 0.8, p(A), na, p(R), ldb)
+    assert np.array_equal(B[m:, :], B0[m:, :])  # padding rows untouched
+    T = np.where(junk, 0.0, A)
+    if diag == "U":
+        T[np.diag_indices(na)] = 1.0
+    opT = np.abs(T if ta == "N" else T.T)
+    G = 0.8 * (opT @ np.abs(B0[:m]) if side == "L" else np.abs(B0[:m]) @ opT)
+    ratio = np.max(np.abs(B[:m] - R[:m]) / (EPS64 * np.maximum(G, np.finfo(float).tiny)))
+    assert ratio < 16.0, ratio
+
+
+def test_dtrmm_integer_exact(gpu, ora):
+    rng = np.random.default_rng(57)
+    m, n = 260, 130
+    A = np.asfortranarray(rng.integers(-5, 6, size=(m, m)).astype(float))
+    B = np.asfortranarray(rng.integers(-5, 6, size=(m, n)).astype(float))
+    R = B.copy(order="F")
+    gpu.dtrmm("L", "L", "T", "N", m, n, 3.0, A, m, B, m)
+    ora.ora_dtrmm(c("L"), c("L"), c("T"), c("N"), m, n, 3.0, p(A), m, p(R), m)
+    assert np.array_equal(B, R)
+
+
 def _tri_matrix(seed, n, uplo, diag_boost=True):
     A = uniform(seed, (n, n))
     if diag_boost:

@@ -124,6 +124,69 @@ def test_dsyrk_vs_host_blas(L, uplo, trans):
     assert np.array_equal(R[~tri], C0[~tri])
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("beta", [0.0, 2.0])
+def test_dsyr2k_vs_host_blas_and_closed_form(L, uplo, trans, beta):
+    n, k = 13, 7
+    A = uniform(19, (n, k) if trans == "N" else (k, n))
+    B = uniform(20, (n, k) if trans == "N" else (k, n))
+    C0 = uniform(21, (n, n))
+    R = C0.copy(order="F")
+    L.ora_dsyr2k(c(uplo), c(trans), n, k, 0.5, p(A), A.shape[0], p(B), B.shape[0], beta, p(R), n)
+    H = sb.dsyr2k(0.5, A, B, beta, C0.copy(order="F"), trans=int(trans == "T"),
+                  lower=int(uplo == "L"))
+    opA, opB = (A, B) if trans == "N" else (A.T, B.T)
+    exact = 0.5 * (opA @ opB.T + opB @ opA.T) + beta * C0
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    np.testing.assert_allclose(R[tri], H[tri], rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(R[tri], exact[tri], rtol=1e-13, atol=1e-13)
+    assert np.array_equal(R[~tri], C0[~tri])
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_dtrmm_vs_host_blas_and_closed_form(L, side, uplo, ta, diag):
+    m, n = 11, 9
+    na = m if side == "L" else n
+    A = uniform(23, (na, na))
+    B0 = uniform(24, (m, n))
+    R = B0.copy(order="F")
+    L.ora_dtrmm(c(side), c(uplo), c(ta), c(diag), m, n, 0.7, p(A), na, p(R), m)
+    H = sb.dtrmm(0.7, A, B0.copy(order="F"), side=int(side == "R"), lower=int(uplo == "L"),
+                 trans_a=int(ta == "T"), diag=int(diag == "U"))
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    if diag == "U":
+        T[np.diag_indices(na)] = 1.0
+    opT = T if ta == "N" else T.T
+    exact = 0.7 * (opT @ B0 if side == "L" else B0 @ opT)
+    np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(R, exact, rtol=1e-13, atol=1e-13)
+
+
+def test_syr2k_trmm_info_codes(L):
+    s = L.ora_syr2k_info
+    assert s(b"X", b"N", 2, 2, 2, 2, 2) == 1
+    assert s(b"L", b"X", 2, 2, 2, 2, 2) == 2
+    assert s(b"L", b"N", -1, 2, 2, 2, 2) == 3
+    assert s(b"L", b"N", 2, -1, 2, 2, 2) == 4
+    assert s(b"L", b"N", 3, 2, 2, 3, 3) == 7
+    assert s(b"L", b"T", 3, 4, 4, 3, 3) == 9
+    assert s(b"L", b"N", 3, 2, 3, 3, 2) == 12
+    assert s(b"U", b"T", 3, 4, 4, 4, 3) == 0
+    t = L.ora_trmm_info
+    assert t(b"X", b"L", b"N", b"N", 2, 2, 2, 2) == 1
+    assert t(b"L", b"X", b"N", b"N", 2, 2, 2, 2) == 2
+    assert t(b"L", b"L", b"X", b"N", 2, 2, 2, 2) == 3
+    assert t(b"L", b"L", b"N", b"X", 2, 2, 2, 2) == 4
+    assert t(b"L", b"L", b"N", b"N", -1, 2, 2, 2) == 5
+    assert t(b"L", b"L", b"N", b"N", 2, -1, 2, 2) == 6
+    assert t(b"R", b"L", b"N", b"N", 2, 3, 2, 2) == 9
+    assert t(b"L", b"L", b"N", b"N", 3, 2, 3, 2) == 11
+
+
 @pytest.mark.parametrize("side", ["L", "R"])
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("ta", ["N", "T"])
