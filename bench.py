@@ -150,8 +150,8 @@ def level3_suite(reps):
     """The Level-3 sweep: GEMM at the four trans combinations with M, N, K in
     {4096, 8192} in FP64 and FP32; DSYRK lower n=8192 k=2048; DTRSM
     left/lower/no-trans/non-unit m=8192 nrhs=4096 (diagonally dominant A);
-    DSYMM left/lower m=n=8192.  One JSON line per shape, kernel TFLOP/s from
-    HIP events."""
+    DSYMM left/lower m=n=8192; DSYR2K lower n=8192 k=2048; DTRMM left/lower
+    m=n=8192.  One JSON line per shape, kernel TFLOP/s from HIP events."""
     import itertools
     import libgpublas_amd as gb
     exe = os.path.join(gb.BIN_DIR, "blas_bench")
@@ -169,6 +169,10 @@ def level3_suite(reps):
                      "--diag", "N", "--m", "8192", "--n", "4096"])
         runs.append(["--op", "symm", "--prec", prec, "--side", "L", "--uplo", "L", "--m", "8192",
                      "--n", "8192"])
+        runs.append(["--op", "syr2k", "--prec", prec, "--uplo", "L", "--trans", "N", "--n", "8192",
+                     "--k", "2048"])
+        runs.append(["--op", "trmm", "--prec", prec, "--side", "L", "--uplo", "L", "--ta", "N",
+                     "--diag", "N", "--m", "8192", "--n", "8192"])
     peak = {"d": FP64_PEAK_TFLOPS, "s": 157.3}
     results = []
     for r in runs:

@@ -6,6 +6,8 @@
  *   blas_bench --lib ... --op syrk --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trsm --m M --n N --side L --uplo L --ta N --diag N
  *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
+ *   blas_bench --lib ... --op syr2k --n N --k K --uplo L --trans N
+ *   blas_bench --lib ... --op trmm --m M --n N --side L --uplo L --ta N --diag N
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   common: --reps R --warmup W --seed S
  *
@@ -139,6 +141,11 @@ int main(int argc, char **argv)
     trsm_t trsm = NULL;
     axpy_t axpy = NULL;
     symm_t symm = NULL;
+    typedef void (*syr2k_t)(const char *, const char *, const int *, const int *, const void *,
+                            const void *, const int *, const void *, const int *, const void *,
+                            void *, const int *);
+    syr2k_t syr2k = NULL;
+    trsm_t trmm = NULL; /* same argument list as TRSM */
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -171,6 +178,22 @@ int main(int argc, char **argv)
             else ((float *)A)[(size_t)i * na + i] = (float)(na + u);
         }
         flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
+    } else if (!strcmp(op, "syr2k")) {
+        syr2k = (syr2k_t)sym(prec == 'd' ? "dsyr2k_" : "ssyr2k_");
+        lda = ldb = (trans == 'N' || trans == 'n') ? n : k;
+        ldc = n;
+        A = operand((size_t)lda * ((trans == 'N' || trans == 'n') ? k : n), prec, seed);
+        B = operand((size_t)ldb * ((trans == 'N' || trans == 'n') ? k : n), prec, seed + 1);
+        C = operand((size_t)n * n, prec, seed + 2);
+        flops = 2.0 * n * (n + 1) * k;
+    } else if (!strcmp(op, "trmm")) {
+        trmm = (trsm_t)sym(prec == 'd' ? "dtrmm_" : "strmm_");
+        const int na = (side == 'L' || side == 'l') ? m : n;
+        lda = na;
+        ldb = m;
+        A = operand((size_t)na * na, prec, seed);
+        B = operand((size_t)m * n, prec, seed + 1);
+        flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
     } else if (!strcmp(op, "symm")) {
         symm = (symm_t)sym(prec == 'd' ? "dsymm_" : "ssymm_");
         const int na = (side == 'L' || side == 'l') ? m : n;
@@ -200,6 +223,8 @@ int main(int argc, char **argv)
         else if (syrk) syrk(&uplo, &trans, &n, &k, alpha, A, &lda, beta, C, &ldc);
         else if (trsm) trsm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
         else if (symm) symm(&side, &uplo, &m, &n, alpha, A, &lda, B, &ldb, beta, C, &ldc);
+        else if (syr2k) syr2k(&uplo, &trans, &n, &k, alpha, A, &lda, B, &ldb, beta, C, &ldc);
+        else if (trmm) trmm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
         else {
             const double a = 0.5;
             const int one = 1;

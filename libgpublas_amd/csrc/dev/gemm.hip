@@ -418,6 +418,43 @@ __global__ __launch_bounds__(256) void symmetrize_kernel(int n, const T *__restr
     }
 }
 
+// C = alpha*op(A)*op(B) where one factor is a full-tile triangle: tile
+// (tm, tn) runs only over the k range where that factor is nonzero (the
+// other products are exact zeros), so a triangular multiply costs half a
+// GEMM.  KR: 1 k < (tm+1)*BM, 2 k >= tm*BM (left factor lower / upper),
+// 3 k >= tn*BM, 4 k < (tn+1)*BM (right factor lower / upper).  Operand
+// pointers are shifted to the range start as in the split-K kernel; tiles
+// are dispatched longest range first.
+template <typename T, int BM, int PA, int PB, int KR>
+__global__ __launch_bounds__(256, 2) void gemm_fast_krange_kernel(GemmArgs<T> g)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
+    // longest ranges first, so the last round holds the shortest tiles
+    const int b = (int)blockIdx.x;
+    int tm, tn;
+    if (KR == 1 || KR == 2) {
+        tm = b / g.tiles_n;
+        if (KR == 1) tm = g.tiles_m - 1 - tm;
+        tn = b % g.tiles_n;
+    } else {
+        tn = b / g.tiles_m;
+        if (KR == 4) tn = g.tiles_n - 1 - tn;
+        tm = b % g.tiles_m;
+    }
+    int kb = 0, ke = g.k;
+    if (KR == 1) ke = min(g.k, (tm + 1) * BM);
+    if (KR == 2) kb = tm * BM;
+    if (KR == 3) kb = tn * BM;
+    if (KR == 4) ke = min(g.k, (tn + 1) * BM);
+    GemmArgs<T> h = g;
+    h.a.p += PA == PAT_OUTER ? (long)kb * g.a.ld : kb;
+    h.b.p += PB == PAT_OUTER ? (long)kb * g.b.ld : kb;
+    h.k = ke - kb;
+    gemm_fast_tile<T, BM, BM, 16, PA, PB, SYM_NONE>(h, tm, tn, As, Bs);
+}
+
 // Full n x n matrix of a triangle of A: the stored triangle copied, zeros
 // elsewhere, ones on the diagonal when `unit` (the diagonal of A is then not
 // read).  One workgroup per 64 x 64 tile; coalesced on both sides.
@@ -494,6 +531,42 @@ int symm_entry(void *stream, char side, char uplo, int m, int n, T alpha, const 
     return launch_gemm<T>(s, m, n, n, alpha, a, b, beta, C, ldc, SYM_NONE);
 }
 
+// Triangular-k product (see gemm_fast_krange_kernel); -1 when the shape or
+// the operands do not allow the fast tiles (the caller then runs the
+// ordinary full-k GEMM).  beta = 0.
+template <typename T, int BM>
+int launch_krange(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a,
+                  const Operand<T> &b, T *C, long ldc, int kr)
+{
+    if (m % BM || n % BM || !a.vec_ok || !b.vec_ok) return -1;
+    GemmArgs<T> g;
+    g.m = m; g.n = n; g.k = k;
+    g.alpha = alpha; g.beta = T(0);
+    g.a = a; g.b = b;
+    g.C = C; g.ldc = ldc;
+    g.tri = SYM_NONE;
+    g.tiles_m = m / BM;
+    g.tiles_n = n / BM;
+    g.full_m = g.full_n = 0;
+    const unsigned blocks = (unsigned)(g.tiles_m * g.tiles_n);
+#define GB_KR(PA_, PB_)                                                                          \
+    do {                                                                                         \
+        if (kr == 1) hipLaunchKernelGGL((gemm_fast_krange_kernel<T, BM, PA_, PB_, 1>), dim3(blocks), dim3(256), 0, s, g); \
+        else if (kr == 2) hipLaunchKernelGGL((gemm_fast_krange_kernel<T, BM, PA_, PB_, 2>), dim3(blocks), dim3(256), 0, s, g); \
+        else if (kr == 3) hipLaunchKernelGGL((gemm_fast_krange_kernel<T, BM, PA_, PB_, 3>), dim3(blocks), dim3(256), 0, s, g); \
+        else hipLaunchKernelGGL((gemm_fast_krange_kernel<T, BM, PA_, PB_, 4>), dim3(blocks), dim3(256), 0, s, g); \
+    } while (0)
+    const int pa = a.form, pb = b.form;
+    PROF_BEGIN(s, sizeof(T) == 8 ? "dgemm_krange" : "sgemm_krange");
+    if (pa == PAT_OUTER && pb == PAT_KCONT) GB_KR(PAT_OUTER, PAT_KCONT);
+    else if (pa == PAT_OUTER) GB_KR(PAT_OUTER, PAT_OUTER);
+    else if (pb == PAT_KCONT) GB_KR(PAT_KCONT, PAT_KCONT);
+    else GB_KR(PAT_KCONT, PAT_OUTER);
+#undef GB_KR
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
 // C = alpha*op(A)*op(B)^T + alpha*op(B)*op(A)^T + beta*C on the uplo triangle
 // (op(X) = X, n x k, for trans N; X^T for trans T): two triangular-grid
 // products, the second accumulating onto the first.
@@ -518,8 +591,9 @@ int syr2k_entry(void *stream, char uplo, char trans, int n, int k, T alpha, cons
 
 // B := alpha*op(A)*B (side L) or alpha*B*op(A) (side R), A triangular: the
 // triangle is expanded into a full matrix (zeros outside it, ones on a unit
-// diagonal), the product runs on the GEMM kernels into a workspace, which is
-// then copied over B.
+// diagonal), the product runs on the GEMM tiles into a workspace -- each tile
+// only over the k range where the triangle is nonzero when the tiles are
+// full -- and the workspace is copied over B.
 template <typename T>
 int trmm_entry(void *stream, char side, char uplo, char transa, char diag, int m, int n, T alpha,
                const T *A, int lda, T *B, int ldb)
@@ -544,15 +618,19 @@ int trmm_entry(void *stream, char side, char uplo, char transa, char diag, int m
     PROF_END(s);
     int err = (int)hipGetLastError();
     const bool tn = is_n(transa);
+    // op(A) lower: the transposed upper triangle, or the lower one as stored
+    const bool oplower = is_u(uplo) ? !tn : tn;
     if (!err) {
-        if (left)
-            err = launch_gemm<T>(s, m, n, m, alpha, make_op<T>(Tm, ldt, m, tn ? PAT_OUTER : PAT_KCONT,
-                                                                SYM_NONE),
-                                 make_op<T>(B, ldb, n, PAT_KCONT, SYM_NONE), T(0), W, ldw, SYM_NONE);
-        else
-            err = launch_gemm<T>(s, m, n, n, alpha, make_op<T>(B, ldb, m, PAT_OUTER, SYM_NONE),
-                                 make_op<T>(Tm, ldt, n, tn ? PAT_KCONT : PAT_OUTER, SYM_NONE), T(0),
-                                 W, ldw, SYM_NONE);
+        const Operand<T> a = left ? make_op<T>(Tm, ldt, m, tn ? PAT_OUTER : PAT_KCONT, SYM_NONE)
+                                  : make_op<T>(B, ldb, m, PAT_OUTER, SYM_NONE);
+        const Operand<T> b = left ? make_op<T>(B, ldb, n, PAT_KCONT, SYM_NONE)
+                                  : make_op<T>(Tm, ldt, n, tn ? PAT_KCONT : PAT_OUTER, SYM_NONE);
+        const int k = left ? m : n;
+        const int kr = left ? (oplower ? 1 : 2) : (oplower ? 3 : 4);
+        const long t128 = (long)(m / 128) * (n / 128);
+        err = t128 >= 256 ? launch_krange<T, 128>(s, m, n, k, alpha, a, b, W, ldw, kr)
+                          : launch_krange<T, 64>(s, m, n, k, alpha, a, b, W, ldw, kr);
+        if (err < 0) err = launch_gemm<T>(s, m, n, k, alpha, a, b, T(0), W, ldw, SYM_NONE);
     }
     if (!err)
         err = (int)hipMemcpy2DAsync(B, (size_t)ldb * sizeof(T), W, (size_t)ldw * sizeof(T),

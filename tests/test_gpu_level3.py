@@ -335,6 +335,40 @@ def test_dtrmm_integer_exact(gpu, ora):
     assert np.array_equal(B, R)
 
 
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_dtrmm_triangular_k_exact(gpu, ora, side, uplo, ta, diag):
+    """Full 64 x 64 tiles: every tile runs only over the k range where op(A)
+    is nonzero (all four range rules across side x op(A) lower/upper).
+    Integer data, the other triangle ROGUE: bit-exact."""
+    rng = np.random.default_rng(61)
+    m, n = 256, 192
+    na = m if side == "L" else n
+    A = rng.integers(-5, 6, size=(na, na)).astype(float)
+    junk = np.triu(np.ones((na, na), bool), 1) if uplo == "L" else np.tril(np.ones((na, na), bool), -1)
+    A[junk] = ROGUE
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(rng.integers(-5, 6, size=(m, n)).astype(float))
+    R = B.copy(order="F")
+    gpu.dtrmm(side, uplo, ta, diag, m, n, 2.0, A, na, B, m)
+    ora.ora_dtrmm(c(side), c(uplo), c(ta), c(diag), m, n, 2.0, p(A), na, p(R), m)
+    assert np.array_equal(B, R)
+
+
+def test_dtrmm_triangular_k_big_tiles_exact(gpu, ora):
+    """2048 x 2048: 256 tiles of 128 x 128 with the triangular k range."""
+    rng = np.random.default_rng(62)
+    m = n = 2048
+    A = np.asfortranarray(np.tril(rng.integers(-3, 4, size=(m, m)).astype(float)))
+    B = np.asfortranarray(rng.integers(-3, 4, size=(m, n)).astype(float))
+    R = B.copy(order="F")
+    gpu.dtrmm("L", "L", "N", "N", m, n, 1.0, A, m, B, m)
+    ora.ora_dtrmm(c("L"), c("L"), c("N"), c("N"), m, n, 1.0, p(A), m, p(R), m)
+    assert np.array_equal(B, R)
+
+
 def _tri_matrix(seed, n, uplo, diag_boost=True):
     A = uniform(seed, (n, n))
     if diag_boost:
