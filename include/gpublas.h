@@ -22,6 +22,10 @@
  *                      not the reference's void macro blas.h:45-51)
  *       dnrm2_         blas_level1/nrm2.cc:37-41 (returns double)
  *       idamax_        blas_level1/amax.cc:38-43 (returns the 1-based index)
+ *       dscal_ dcopy_ dswap_ drot_ dasum_  netlib definitions behind the
+ *                      reference's CBLAS specs blas_level1/scal.cc:42,
+ *                      copy.cc:46, swap.cc:12-29, rot.cc:50, asum.cc:48 (dasum_
+ *                      returns double)
  *     Illegal arguments call xerbla_("DGEMM ", &info, 6) of the process
  *     (runtime-blas.c:34-52) with netlib INFO codes and return.
  *
@@ -92,6 +96,12 @@ void daxpy_(const int *n, const double *alpha, const double *x, const int *incx,
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
 double dnrm2_(const int *n, const double *x, const int *incx);
 int idamax_(const int *n, const double *x, const int *incx);
+void dscal_(const int *n, const double *alpha, double *x, const int *incx);
+void dcopy_(const int *n, const double *x, const int *incx, double *y, const int *incy);
+void dswap_(const int *n, double *x, const int *incx, double *y, const int *incy);
+void drot_(const int *n, double *x, const int *incx, double *y, const int *incy, const double *c,
+           const double *s);
+double dasum_(const int *n, const double *x, const int *incx);
 int lsame_(const char *ca, const char *cb);
 
 /* ---------------- CBLAS (cblas.h:21-25 enum values) ---------------- */
@@ -132,6 +142,11 @@ void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int 
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);
 int cblas_idamax(int n, const double *x, int incx);
+void cblas_dscal(int n, double alpha, double *x, int incx);
+void cblas_dcopy(int n, const double *x, int incx, double *y, int incy);
+void cblas_dswap(int n, double *x, int incx, double *y, int incy);
+void cblas_drot(int n, double *x, int incx, double *y, int incy, double c, double s);
+double cblas_dasum(int n, const double *x, int incx);
 
 /* ---------------- extension API ---------------- */
 /* Statistics (the reference keeps hits/misses, runtime-mem.hpp:83,112). */

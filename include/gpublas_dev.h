@@ -124,6 +124,15 @@ int gpublas_dev_dsyrk_split_fmt(void *stream, int slices, int fmt, char uplo, ch
                                 int k, double alpha, const double *d_A, int lda, double beta,
                                 double *d_C, int ldc);
 
+/* ---- more Level 1 (netlib semantics: DSCAL and DASUM do nothing / return 0
+ * for incx <= 0; negative increments walk the vectors backwards) ---- */
+int gpublas_dev_dscal(void *stream, int n, double alpha, double *d_x, int incx);
+int gpublas_dev_dcopy(void *stream, int n, const double *d_x, int incx, double *d_y, int incy);
+int gpublas_dev_dswap(void *stream, int n, double *d_x, int incx, double *d_y, int incy);
+int gpublas_dev_drot(void *stream, int n, double *d_x, int incx, double *d_y, int incy, double c,
+                     double s);
+int gpublas_dev_dasum(void *stream, int n, const double *d_x, int incx, double *h_result);
+
 /* ---- Level 2 ---- */
 int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
                       int lda, const double *d_x, int incx, double beta, double *d_y, int incy);

@@ -78,6 +78,9 @@ def _declare(lib):
     lib.ddot_.restype = cd
     lib.dnrm2_.restype = cd
     lib.idamax_.restype = ci
+    lib.dasum_.restype = cd
+    lib.cblas_dasum.restype = cd
+    lib.cblas_dasum.argtypes = [ci, vp, ci]
     lib.cblas_ddot.restype = cd
     lib.cblas_ddot.argtypes = [ci, vp, ci, vp, ci]
     lib.cblas_dnrm2.restype = cd
@@ -225,6 +228,26 @@ def daxpy(n, alpha, x, incx, y, incy):
 
 def ddot(n, x, incx, y, incy):
     return library().ddot_(_i(n), _p(x), _i(incx), _p(y), _i(incy))
+
+
+def dscal(n, alpha, x, incx):
+    library().dscal_(_i(n), _d(alpha), _p(x), _i(incx))
+
+
+def dcopy(n, x, incx, y, incy):
+    library().dcopy_(_i(n), _p(x), _i(incx), _p(y), _i(incy))
+
+
+def dswap(n, x, incx, y, incy):
+    library().dswap_(_i(n), _p(x), _i(incx), _p(y), _i(incy))
+
+
+def drot(n, x, incx, y, incy, c, s):
+    library().drot_(_i(n), _p(x), _i(incx), _p(y), _i(incy), _d(c), _d(s))
+
+
+def dasum(n, x, incx):
+    return library().dasum_(_i(n), _p(x), _i(incx))
 
 
 def dnrm2(n, x, incx):

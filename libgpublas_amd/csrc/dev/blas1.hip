@@ -1,5 +1,5 @@
 // blas1.hip -- Level-1 streaming kernels on gfx950: DAXPY, DDOT, DNRM2,
-// IDAMAX.
+// IDAMAX, and DSCAL, DCOPY, DSWAP, DROT, DASUM.
 //
 // Replaces the vendor calls the reference's (unbuilt) CBLAS wrappers make:
 // cublasDaxpy (blas_level1/axpy.cc:49), cublasDdot (dot.cc:46), cublasDnrm2
@@ -449,6 +449,102 @@ __global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
 inline int aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// ---- DSCAL / DCOPY / DSWAP / DROT: one strided element pair per thread -------
+// (netlib orders: element i of a negative-increment vector sits at
+// (n-1-i)*|inc| from its start, which x0/y0 and the signed increments encode)
+enum PairOp { OP_SCAL, OP_COPY, OP_SWAP, OP_ROT };
+
+template <int OP>
+__global__ __launch_bounds__(256) void pair_kernel(int n, double *x, long incx, long x0, double *y,
+                                                   long incy, long y0, double c, double sn)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double *px = x + x0 + i * incx;
+        if (OP == OP_SCAL) {
+            *px = c * *px;
+            continue;
+        }
+        double *py = y + y0 + i * incy;
+        if (OP == OP_COPY) {
+            *py = *px;
+        } else if (OP == OP_SWAP) {
+            const double t = *px;
+            *px = *py;
+            *py = t;
+        } else {
+            const double xv = *px, yv = *py;
+            *px = c * xv + sn * yv;
+            *py = c * yv - sn * xv;
+        }
+    }
+}
+
+template <int OP>
+int launch_pair(hipStream_t s, const char *name, int n, double *x, int incx, double *y, int incy,
+                double c, double sn)
+{
+    int blocks = (int)(((long)n + 255) / 256);
+    if (blocks > 16384) blocks = 16384;
+    PROF_BEGIN(s, name);
+    hipLaunchKernelGGL(pair_kernel<OP>, dim3(blocks), dim3(256), 0, s, n, x, (long)incx,
+                       start_of(n, incx), y, (long)incy, start_of(n, incy), c, sn);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+// ---- DASUM: the DDOT reductions with |x_i| in place of x_i*y_i -----------------
+__global__ __launch_bounds__(1024) void asum_small(int n, const double *x, long incx, int vec,
+                                                   double *result, uint64_t *flag, uint64_t seq)
+{
+    __shared__ double red[16];
+    double s = 0;
+    if (vec) {
+        int i = 2 * (int)threadIdx.x;
+        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
+            d2v xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                s += fabs(xv[u].x);
+                s += fabs(xv[u].y);
+            }
+        }
+        for (; i < n; i += 2048) {
+            s += fabs(x[i]);
+            if (i + 1 < n) s += fabs(x[i + 1]);
+        }
+    } else {
+        for_pairs(n, [&](int i) { s += fabs(x[i * incx]); });
+    }
+    s = block_sum16(s, red);
+    if (threadIdx.x == 0) {
+        *result = s;
+        publish(flag, seq);
+    }
+}
+
+__global__ __launch_bounds__(256) void asum_kernel(int n, const double *x, long incx,
+                                                   double *partials, unsigned *ticket,
+                                                   double *result, uint64_t *flag, uint64_t seq)
+{
+    __shared__ double red[4];
+    double s = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        s += fabs(x[i * incx]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (!last_block(ticket)) return;
+    double t = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) t += partials[b];
+    t = block_sum(t, red);
+    if (threadIdx.x == 0) {
+        *result = t;
+        publish(flag, seq);
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -552,6 +648,57 @@ int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "idamax");
     hipLaunchKernelGGL(iamax_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
+                       (double *)r.partials, r.ticket, h_result, pb.flag, pb.seq);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_dscal(void *stream, int n, double alpha, double *d_x, int incx)
+{
+    if (n <= 0 || incx <= 0) return 0;  // netlib DSCAL: no-op for incx <= 0
+    return launch_pair<OP_SCAL>(pick_stream(stream), "dscal", n, d_x, incx, nullptr, 1, alpha, 0.0);
+}
+
+int gpublas_dev_dcopy(void *stream, int n, const double *d_x, int incx, double *d_y, int incy)
+{
+    if (n <= 0) return 0;
+    return launch_pair<OP_COPY>(pick_stream(stream), "dcopy", n, (double *)d_x, incx, d_y, incy,
+                                0.0, 0.0);
+}
+
+int gpublas_dev_dswap(void *stream, int n, double *d_x, int incx, double *d_y, int incy)
+{
+    if (n <= 0) return 0;
+    return launch_pair<OP_SWAP>(pick_stream(stream), "dswap", n, d_x, incx, d_y, incy, 0.0, 0.0);
+}
+
+int gpublas_dev_drot(void *stream, int n, double *d_x, int incx, double *d_y, int incy, double c,
+                     double s)
+{
+    if (n <= 0) return 0;
+    return launch_pair<OP_ROT>(pick_stream(stream), "drot", n, d_x, incx, d_y, incy, c, s);
+}
+
+int gpublas_dev_dasum(void *stream, int n, const double *d_x, int incx, double *h_result)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0 || incx <= 0) {  // netlib DASUM returns 0
+        *h_result = 0.0;
+        return 0;
+    }
+    const Pub pb = pub_for(h_result);
+    if (n <= kSmallN) {
+        const int vec = incx == 1 && aligned16(d_x);
+        PROF_BEGIN(s, "dasum");
+        hipLaunchKernelGGL(asum_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
+                           h_result, pb.flag, pb.seq);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+    RedSlot r = next_slot();
+    int blocks = red_blocks(n);
+    PROF_BEGIN(s, "dasum");
+    hipLaunchKernelGGL(asum_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
                        (double *)r.partials, r.ticket, h_result, pb.flag, pb.seq);
     PROF_END(s);
     return (int)hipGetLastError();

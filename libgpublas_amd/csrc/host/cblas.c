@@ -238,3 +238,19 @@ int cblas_idamax(int n, const double *x, int incx)
     int r = idamax_(&n, x, &incx);
     return r ? r - 1 : 0; /* CBLAS indices are 0-based */
 }
+
+void cblas_dscal(int n, double alpha, double *x, int incx) { dscal_(&n, &alpha, x, &incx); }
+
+void cblas_dcopy(int n, const double *x, int incx, double *y, int incy)
+{
+    dcopy_(&n, x, &incx, y, &incy);
+}
+
+void cblas_dswap(int n, double *x, int incx, double *y, int incy) { dswap_(&n, x, &incx, y, &incy); }
+
+void cblas_drot(int n, double *x, int incx, double *y, int incy, double c, double s)
+{
+    drot_(&n, x, &incx, y, &incy, &c, &s);
+}
+
+double cblas_dasum(int n, const double *x, int incx) { return dasum_(&n, x, &incx); }

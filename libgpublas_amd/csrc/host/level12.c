@@ -1,6 +1,7 @@
 /*
  * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
- * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_.
+ * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_, and dscal_,
+ * dcopy_, dswap_, drot_, dasum_.
  *
  * The reference specifies these only through CBLAS wrappers it never builds
  * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
@@ -62,6 +63,150 @@ void daxpy_(const int *n, const double *alpha, const double *x, const int *incx,
 }
 
 /* ---- DDOT ---- */
+/* DSCAL / DCOPY / DSWAP / DROT / DASUM: netlib definitions, which the
+ * reference's CBLAS specs forward to (blas_level1/scal.cc:42, copy.cc:46,
+ * swap.cc:12-29, rot.cc:50, asum.cc:48); same fall-through rule as DAXPY for small
+ * untracked vectors. */
+void dscal_(const int *n, const double *alpha, double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0 || *incx <= 0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, x, NULL, NULL)) {
+        void (*f)(const int *, const double *, double *, const int *) = gb_host_fn("dscal_");
+        GB_COUNT(fallthrough, 1);
+        f(n, alpha, x, incx);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[1];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dscal(s, *n, *alpha, (double *)ops[0].dev, *incx), "dscal launch");
+    gb_finish(s, ops, 1);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void dcopy_(const int *n, const double *x, const int *incx, double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, x, y, NULL)) {
+        void (*f)(const int *, const double *, const int *, double *, const int *) =
+            gb_host_fn("dcopy_");
+        GB_COUNT(fallthrough, 1);
+        f(n, x, incx, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_OUT, s);
+    gb_check(gpublas_dev_dcopy(s, *n, (const double *)ops[0].dev, *incx, (double *)ops[1].dev,
+                               *incy),
+             "dcopy launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void dswap_(const int *n, double *x, const int *incx, double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, x, y, NULL)) {
+        void (*f)(const int *, double *, const int *, double *, const int *) = gb_host_fn("dswap_");
+        GB_COUNT(fallthrough, 1);
+        f(n, x, incx, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_INOUT, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dswap(s, *n, (double *)ops[0].dev, *incx, (double *)ops[1].dev, *incy),
+             "dswap launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void drot_(const int *n, double *x, const int *incx, double *y, const int *incy, const double *c,
+           const double *sn)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0) {
+        gb_leave();
+        return;
+    }
+    if (small_untracked(*n, x, y, NULL)) {
+        void (*f)(const int *, double *, const int *, double *, const int *, const double *,
+                  const double *) = gb_host_fn("drot_");
+        GB_COUNT(fallthrough, 1);
+        f(n, x, incx, y, incy, c, sn);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_INOUT, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_drot(s, *n, (double *)ops[0].dev, *incx, (double *)ops[1].dev, *incy, *c,
+                              *sn),
+             "drot launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+double dasum_(const int *n, const double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0 || *incx <= 0) {
+        gb_leave();
+        return 0.0;
+    }
+    if (small_untracked(*n, x, NULL, NULL)) {
+        double (*f)(const int *, const double *, const int *) = gb_host_fn("dasum_");
+        GB_COUNT(fallthrough, 1);
+        double r = f(n, x, incx);
+        gb_leave();
+        return r;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[1];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    double *slot = (double *)gpublas_dev_result_slot();
+    const uint64_t seq = gpublas_dev_scalar_next();
+    gb_check(gpublas_dev_dasum(s, *n, (const double *)ops[0].dev, *incx, slot), "dasum launch");
+    gb_finish_scalar(s, ops, 1, seq);
+    double r = *(volatile double *)slot;
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+    return r;
+}
+
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy)
 {
     gb_enter();

@@ -691,6 +691,55 @@ void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy)
     for (int i = 0; i < n; ++i, ix += incx, iy += incy) y[iy] += a * x[ix];
 }
 
+/* netlib DSCAL/DCOPY/DSWAP/DROT/DASUM (the definitions behind the
+ * reference's CBLAS specs blas_level1/scal.cc:42, copy.cc:46, swap.cc:12-29,
+ * rot.cc:50, asum.cc:48); element i of a negative-increment vector is at
+ * (n-1-i)*|inc| from its start */
+static long ora_start(int n, int inc) { return inc < 0 ? (long)(1 - n) * inc : 0; }
+
+void ora_dscal(int n, double a, double *x, int incx)
+{
+    if (n <= 0 || incx <= 0) return;
+    for (int i = 0; i < n; ++i) x[(long)i * incx] = a * x[(long)i * incx];
+}
+
+void ora_dcopy(int n, const double *x, int incx, double *y, int incy)
+{
+    if (n <= 0) return;
+    long ix = ora_start(n, incx), iy = ora_start(n, incy);
+    for (int i = 0; i < n; ++i, ix += incx, iy += incy) y[iy] = x[ix];
+}
+
+void ora_dswap(int n, double *x, int incx, double *y, int incy)
+{
+    if (n <= 0) return;
+    long ix = ora_start(n, incx), iy = ora_start(n, incy);
+    for (int i = 0; i < n; ++i, ix += incx, iy += incy) {
+        double t = x[ix];
+        x[ix] = y[iy];
+        y[iy] = t;
+    }
+}
+
+void ora_drot(int n, double *x, int incx, double *y, int incy, double c, double s)
+{
+    if (n <= 0) return;
+    long ix = ora_start(n, incx), iy = ora_start(n, incy);
+    for (int i = 0; i < n; ++i, ix += incx, iy += incy) {
+        double t = c * x[ix] + s * y[iy];
+        y[iy] = c * y[iy] - s * x[ix];
+        x[ix] = t;
+    }
+}
+
+double ora_dasum(int n, const double *x, int incx)
+{
+    if (n <= 0 || incx <= 0) return 0.0;
+    double t = 0;
+    for (int i = 0; i < n; ++i) t += fabs(x[(long)i * incx]);
+    return t;
+}
+
 double ora_ddot(int n, const double *x, int incx, const double *y, int incy)
 {
     double s = 0;

@@ -67,6 +67,11 @@ void ora_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda
                int incx);
 
 /* ---- Level 1 ---- */
+void ora_dscal(int n, double a, double *x, int incx);
+void ora_dcopy(int n, const double *x, int incx, double *y, int incy);
+void ora_dswap(int n, double *x, int incx, double *y, int incy);
+void ora_drot(int n, double *x, int incx, double *y, int incy, double c, double s);
+double ora_dasum(int n, const double *x, int incx);
 void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);
 double ora_ddot(int n, const double *x, int incx, const double *y, int incy);
 double ora_dnrm2(int n, const double *x, int incx);

@@ -42,6 +42,12 @@ def load():
     L.ora_dger.argtypes = [I, I, D, VP, I, VP, I, VP, I]
     L.ora_dtrsv.argtypes = [C, C, C, I, VP, I, VP, I]
     L.ora_daxpy.argtypes = [I, D, VP, I, VP, I]
+    L.ora_dscal.argtypes = [I, D, VP, I]
+    L.ora_dcopy.argtypes = [I, VP, I, VP, I]
+    L.ora_dswap.argtypes = [I, VP, I, VP, I]
+    L.ora_drot.argtypes = [I, VP, I, VP, I, D, D]
+    L.ora_dasum.argtypes = [I, VP, I]
+    L.ora_dasum.restype = D
     L.ora_ddot.argtypes = [I, VP, I, VP, I]
     L.ora_ddot.restype = D
     L.ora_dnrm2.argtypes = [I, VP, I]

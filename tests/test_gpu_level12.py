@@ -63,6 +63,52 @@ def test_ddot(gpu, ora, n, incx, incy):
     assert abs(got - ref) <= 2 * n * EPS * np.sum(np.abs(xv * yv))
 
 
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (1000, 1, 1), (70001, 1, 1), (777, 2, -3),
+                                         (500, -1, 2), (300, -3, -2)])
+def test_dcopy_dswap_drot(gpu, ora, n, incx, incy):
+    """Exact element moves (copy, swap) and one rotation per pair, against the
+    netlib restatements; the fill between strided elements stays put."""
+    x0 = _strided(uniform(5, (n,)), incx)
+    y0 = _strided(uniform(6, (n,)), incy)
+    x, y, rx, ry = x0.copy(), y0.copy(), x0.copy(), y0.copy()
+    gpu.dcopy(n, x, incx, y, incy)
+    ora.ora_dcopy(n, p(rx), incx, p(ry), incy)
+    assert np.array_equal(y, ry) and np.array_equal(x, x0)
+    x, y, rx, ry = x0.copy(), y0.copy(), x0.copy(), y0.copy()
+    gpu.dswap(n, x, incx, y, incy)
+    ora.ora_dswap(n, p(rx), incx, p(ry), incy)
+    assert np.array_equal(x, rx) and np.array_equal(y, ry)
+    x, y, rx, ry = x0.copy(), y0.copy(), x0.copy(), y0.copy()
+    gpu.drot(n, x, incx, y, incy, 0.6, 0.8)
+    ora.ora_drot(n, p(rx), incx, p(ry), incy, 0.6, 0.8)
+    bound = 2 * EPS * (np.abs(x0).max() + np.abs(y0).max())
+    assert np.max(np.abs(x - rx)) <= bound and np.max(np.abs(y - ry)) <= bound
+
+
+@pytest.mark.parametrize("n,incx", [(1, 1), (1000, 1), (8192, 1), (70001, 1), (777, 3), (50, -1)])
+def test_dscal_dasum(gpu, ora, n, incx):
+    xv = uniform(7, (n,))
+    x0 = _strided(xv, incx)
+    x, r = x0.copy(), x0.copy()
+    gpu.dscal(n, -2.5, x, incx)
+    ora.ora_dscal(n, -2.5, p(r), incx)
+    assert np.array_equal(x, r)  # one multiplication per element; incx < 0: no-op
+    got = gpu.dasum(n, x0, incx)
+    ref = ora.ora_dasum(n, p(x0), incx)
+    if incx <= 0:
+        assert got == 0.0 and ref == 0.0
+    else:
+        assert abs(got - ref) <= 2 * n * EPS * np.sum(np.abs(xv))
+
+
+def test_dasum_integer_exact_and_repeatable(gpu):
+    rng = np.random.default_rng(9)
+    x = rng.integers(-1000, 1000, 200001).astype(float)
+    a = gpu.dasum(len(x), x, 1)
+    assert a == float(np.sum(np.abs(x)))
+    assert all(gpu.dasum(len(x), x, 1) == a for _ in range(3))
+
+
 def test_ddot_integer_exact_and_repeatable(gpu):
     rng = np.random.default_rng(2)
     x = rng.integers(-1000, 1000, 300000).astype(float)

@@ -349,3 +349,38 @@ def test_info_codes(L):
     assert [tv(b"X", b"N", b"N", 2, 2, 1), tv(b"L", b"X", b"N", 2, 2, 1), tv(b"L", b"N", b"X", 2, 2, 1),
             tv(b"L", b"N", b"N", -1, 2, 1), tv(b"L", b"N", b"N", 3, 2, 1),
             tv(b"L", b"N", b"N", 2, 2, 0)] == [1, 2, 3, 4, 6, 8]
+
+
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -3), (-1, 2), (-2, -1)])
+def test_level1_more_vs_host_blas(L, incx, incy):
+    """DSCAL/DCOPY/DSWAP/DROT/DASUM restatements against scipy's OpenBLAS,
+    negative increments included (netlib walks those vectors backwards)."""
+    n = 37
+    x0 = uniform(81, (n * abs(incx),))
+    y0 = uniform(82, (n * abs(incy),))
+    # dcopy / dswap / drot: the host routine gets the same strided views
+    X, Y = x0.copy(), y0.copy()
+    L.ora_dcopy(n, p(X), incx, p(Y), incy)
+    Hy = sb.dcopy(x0, y0.copy(), n=n, incx=incx, incy=incy)
+    assert np.array_equal(Y, Hy)
+    X, Y = x0.copy(), y0.copy()
+    L.ora_dswap(n, p(X), incx, p(Y), incy)
+    Hx, Hy = sb.dswap(x0.copy(), y0.copy(), n=n, incx=incx, incy=incy)
+    assert np.array_equal(X, Hx) and np.array_equal(Y, Hy)
+    X, Y = x0.copy(), y0.copy()
+    L.ora_drot(n, p(X), incx, p(Y), incy, 0.6, 0.8)
+    Hx, Hy = sb.drot(x0.copy(), y0.copy(), 0.6, 0.8, n=n, incx=incx, incy=incy)
+    np.testing.assert_allclose(X, Hx, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(Y, Hy, rtol=1e-15, atol=1e-15)
+    if incx > 0:
+        X = x0.copy()
+        L.ora_dscal(n, -1.5, p(X), incx)
+        assert np.array_equal(X, sb.dscal(-1.5, x0.copy(), n=n, incx=incx))
+        a = L.ora_dasum(n, p(x0), incx)
+        np.testing.assert_allclose(a, sb.dasum(x0, n=n, incx=incx), rtol=1e-14)
+        assert a == pytest.approx(np.sum(np.abs(x0[::incx][:n])), rel=1e-14)
+    else:
+        X = x0.copy()
+        L.ora_dscal(n, -1.5, p(X), incx)  # netlib: no-op for incx <= 0
+        assert np.array_equal(X, x0)
+        assert L.ora_dasum(n, p(x0), incx) == 0.0
