@@ -184,9 +184,33 @@ void gpublas_sync(void);
 /* Options string (same syntax as GPUBLAS_OPTIONS); returns 0 if all parsed. */
 int gpublas_set_options(const char *opts);
 /* GEMM host fall-through cutoff: leading dimensions below it go to the host
- * BLAS (reference rule gemm.cc:129-141, default 512); 0 = always the GPU. */
+ * BLAS (reference rule gemm.cc:129-141, default 512); 0 = always the GPU;
+ * a negative value (GPUBLAS_CUTOFF_AUTO) = the measured crossover table. */
 void gpublas_set_cutoff(int cutoff);
 int gpublas_get_cutoff(void);
+
+/* Measured host/GPU crossover (cutoff=auto, GPUBLAS_CUTOFF_AUTO as the
+ * cutoff).  The reference has no counterpart: it replaces the fixed rule of
+ * gemm.cc:129-141 where its TODO.md:7 asks for the host BLAS to be picked
+ * dynamically.  A call with no resident operand runs on the GPU when its
+ * flops reach the routine's min_flops and its flops per operand byte reach
+ * min_intensity.  Routine names: dgemm sgemm dsyrk ssyrk dsyr2k ssyr2k dsymm
+ * ssymm dtrsm strsm dtrmm strmm dgemv dger dtrsv daxpy (all streaming Level-1
+ * calls) ddot (all Level-1 reductions).  Return 0, or -1 for an unknown
+ * routine or a negative / NaN threshold. */
+#define GPUBLAS_CUTOFF_AUTO (-1)
+int gpublas_set_crossover(const char *routine, double min_flops, double min_intensity);
+int gpublas_get_crossover(const char *routine, double *min_flops, double *min_intensity);
+/* 1 if the table sends a call of this routine with these flops and operand
+ * bytes to the GPU, 0 if to the host, -1 for an unknown routine. */
+int gpublas_crossover_gpu(const char *routine, double flops, double bytes);
+/* Time square problems (n up to max_n for Level 3, 2*max_n for Level 2,
+ * 4096*max_n elements for Level 1) on the host BLAS and on the GPU with
+ * operands that are not resident, install the resulting table and, if path is
+ * not NULL, write it there (the measurements as comments).  Needs the GPU;
+ * takes seconds.  Returns 0, -1 (max_n < 16), -2 (out of memory) or -3 (cannot
+ * write path). */
+int gpublas_tune_crossover(const char *path, int max_n);
 
 /* Explicit tracked allocation: host-visible memory whose contents stay
  * resident in HBM between BLAS calls.  Host reads/writes are coherent (a

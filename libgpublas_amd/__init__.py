@@ -72,6 +72,10 @@ def _declare(lib):
     lib.gpublas_device_ptr.argtypes = [vp]
     lib.gpublas_set_options.argtypes = [ctypes.c_char_p]
     lib.gpublas_set_cutoff.argtypes = [ci]
+    lib.gpublas_set_crossover.argtypes = [ctypes.c_char_p, cd, cd]
+    lib.gpublas_get_crossover.argtypes = [ctypes.c_char_p, ctypes.POINTER(cd), ctypes.POINTER(cd)]
+    lib.gpublas_crossover_gpu.argtypes = [ctypes.c_char_p, cd, cd]
+    lib.gpublas_tune_crossover.argtypes = [ctypes.c_char_p, ci]
     lib.gpublas_get_stats.argtypes = [ctypes.POINTER(Stats)]
     lib.gpublas_kernel_time.argtypes = [ctypes.c_char_p, ctypes.POINTER(cd)]
     lib.gpublas_last_xerbla.argtypes = [ctypes.c_char_p]
@@ -270,7 +274,45 @@ def set_cutoff(cutoff: int) -> None:
 
 
 def get_cutoff() -> int:
+    """Current cutoff; CUTOFF_AUTO (-1) under cutoff=auto."""
     return library().gpublas_get_cutoff()
+
+
+CUTOFF_AUTO = -1
+CROSSOVER_ROUTINES = ("dgemm", "sgemm", "dsyrk", "ssyrk", "dsyr2k", "ssyr2k", "dsymm", "ssymm",
+                      "dtrsm", "strsm", "dtrmm", "strmm", "dgemv", "dger", "dtrsv", "daxpy",
+                      "ddot")
+
+
+def set_crossover(routine: str, min_flops: float, min_intensity: float) -> None:
+    """Set one row of the host/GPU crossover table used under cutoff=auto."""
+    if library().gpublas_set_crossover(routine.encode(), float(min_flops),
+                                       float(min_intensity)) != 0:
+        raise ValueError(f"bad crossover row {routine!r} {min_flops} {min_intensity}")
+
+
+def get_crossover(routine: str) -> tuple:
+    """(min_flops, min_intensity) of one routine's crossover row."""
+    f, i = ctypes.c_double(), ctypes.c_double()
+    if library().gpublas_get_crossover(routine.encode(), ctypes.byref(f), ctypes.byref(i)) != 0:
+        raise ValueError(f"unknown routine {routine!r}")
+    return f.value, i.value
+
+
+def crossover_gpu(routine: str, flops: float, nbytes: float) -> bool:
+    """True if the crossover table sends such a call (no resident operand) to the GPU."""
+    r = library().gpublas_crossover_gpu(routine.encode(), float(flops), float(nbytes))
+    if r < 0:
+        raise ValueError(f"unknown routine {routine!r}")
+    return bool(r)
+
+
+def tune_crossover(path=None, max_n: int = 2048) -> None:
+    """Measure the crossover table on this machine (needs the GPU), install it
+    and write it to `path` if given (load it later with xover=<path>)."""
+    r = library().gpublas_tune_crossover(None if path is None else str(path).encode(), int(max_n))
+    if r != 0:
+        raise RuntimeError(f"gpublas_tune_crossover failed ({r})")
 
 
 def stats() -> dict:

@@ -77,8 +77,11 @@ static void print_help(void)
            "                       false (default) | true | random | size:<bytes> |\n"
            "                       objtrack:<file> (alias oracle:<file>)\n"
            "  min_track=<bytes>    never track allocations smaller than this (4096)\n"
-           "  cutoff=<n>           GEMM leading dimensions below n go to the host BLAS\n"
-           "                       (512; 0 = always the GPU)\n"
+           "  cutoff=<n>           GEMM leading dimensions (Level-1/2 lengths) below n go\n"
+           "                       to the host BLAS when no operand is resident (512;\n"
+           "                       0 = always the GPU)\n"
+           "  cutoff=auto          host or GPU by the measured crossover table\n"
+           "  xover=<file>         crossover table written by gpublas_tune_crossover\n"
            "  profile              time every kernel with HIP events\n"
            "  split=<s>            DGEMM/DSYRK as split-precision slice products\n"
            "                       (s = 1..8 slices per operand; 0 = plain FP64, default)\n"
@@ -157,7 +160,19 @@ int gb_parse_options(const char *s)
         else if (!strcmp(opt, "stats")) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "statistics.csv");
         else if (!strncmp(opt, "stats=", 6)) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "%s", opt + 6);
         else if (!strncmp(opt, "trace=", 6)) snprintf(gb_opt.trace_file, sizeof gb_opt.trace_file, "%s", opt + 6);
-        else if (!strncmp(opt, "cutoff=", 7)) gb_opt.cutoff = atoi(opt + 7);
+        else if (!strcmp(opt, "cutoff=auto")) gb_opt.cutoff = GB_CUTOFF_AUTO;
+        else if (!strncmp(opt, "cutoff=", 7)) {
+            char *end;
+            long v = strtol(opt + 7, &end, 10);
+            if (*end || v < 0 || v > 1 << 30) bad = 1;
+            else gb_opt.cutoff = (int)v;
+        }
+        else if (!strncmp(opt, "xover=", 6)) {
+            if (gb_xover_load(opt + 6) != 0) {
+                gb_log(2, "libgpublas: cannot load crossover table '%s'\n", opt + 6);
+                bad = 1;
+            }
+        }
         else if (!strncmp(opt, "min_track=", 10)) gb_opt.min_track = parse_size(opt + 10);
         else if (!strncmp(opt, "heuristic=", 10)) {
             const char *h = opt + 10;
@@ -381,7 +396,7 @@ int gpublas_set_options(const char *opts)
     return r;
 }
 
-void gpublas_set_cutoff(int cutoff) { gb_opt.cutoff = cutoff; }
+void gpublas_set_cutoff(int cutoff) { gb_opt.cutoff = cutoff < 0 ? GB_CUTOFF_AUTO : cutoff; }
 int gpublas_get_cutoff(void) { return gb_opt.cutoff; }
 
 int gpublas_kernel_time(const char *kernel, double *total_ms)

@@ -59,6 +59,24 @@ int gb_lsame(char a, char b);
 /* ---- host BLAS fall-through (runtime-blas.c:60-69) ---- */
 void *gb_host_fn(const char *f77name);
 
+/* ---- host or GPU for calls with no resident operand (route.c) ---- */
+#define GB_CUTOFF_AUTO (-1) /* cutoff=auto: the measured crossover table */
+enum gb_route_kind {
+    GB_RK_DGEMM, GB_RK_SGEMM, GB_RK_DSYRK, GB_RK_SSYRK, GB_RK_DSYR2K, GB_RK_SSYR2K,
+    GB_RK_DSYMM, GB_RK_SSYMM, GB_RK_DTRSM, GB_RK_STRSM, GB_RK_DTRMM, GB_RK_STRMM,
+    GB_RK_DGEMV, GB_RK_DGER, GB_RK_DTRSV,
+    GB_RK_STREAM, /* daxpy, dscal, dcopy, dswap, drot */
+    GB_RK_REDUCE, /* ddot, dnrm2, dasum, idamax */
+    GB_RK_COUNT
+};
+/* 1 = run the call on the host BLAS.  below_fixed is the fixed rule's verdict
+ * (cutoff=<n>); flops and bytes feed the crossover table (cutoff=auto); a
+ * tracked p, q or r (NULL allowed) keeps the call on the GPU. */
+int gb_host_route(int kind, int below_fixed, double flops, double bytes, const void *p,
+                  const void *q, const void *r);
+int gb_xover_load(const char *path);
+extern GB_TLS int gb_route_gpu; /* set by the crossover measurement: force the GPU */
+
 /* ---- device availability: aborts with a message if no gfx950 device ---- */
 void gb_require_device(void);
 

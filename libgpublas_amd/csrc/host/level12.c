@@ -16,7 +16,8 @@
  * (device-resident) object and its length is below the GEMM cutoff, the call
  * goes to the host BLAS -- the reference's small-problem rule (gemm.cc:129-
  * 141) applied to vectors, where staging 2 x 4 KB through the host link
- * costs more than the arithmetic.  cutoff=0 sends everything to the GPU.
+ * costs more than the arithmetic.  cutoff=0 sends everything to the GPU;
+ * cutoff=auto decides by the measured crossover table of route.c.
  */
 #include "internal.h"
 #include <math.h>
@@ -24,11 +25,14 @@
 
 #define MAXI(a, b) ((a) > (b) ? (a) : (b))
 
-static int small_untracked(int len, const void *p, const void *q, const void *r)
+/* Level-1: the fixed rule compares the length with the cutoff; the crossover
+ * table sees 2n flops over 24n (streaming: x, y in and out) or 16n (reduction:
+ * x, y) bytes. */
+static int l1_host(int kind, int n, const void *p, const void *q)
 {
-    const int cut = gb_opt.cutoff;
-    if (cut <= 0 || len >= cut) return 0;
-    return !gb_ptr_tracked(p) && !gb_ptr_tracked(q) && !gb_ptr_tracked(r);
+    const double nn = n;
+    return gb_host_route(kind, gb_opt.cutoff > 0 && n < gb_opt.cutoff, 2 * nn,
+                         (kind == GB_RK_STREAM ? 24 : 16) * nn, p, q, NULL);
 }
 
 /* ---- DAXPY: y := alpha*x + y ---- */
@@ -41,7 +45,7 @@ void daxpy_(const int *n, const double *alpha, const double *x, const int *incx,
         gb_leave();
         return;
     }
-    if (small_untracked(*n, x, y, NULL)) {
+    if (l1_host(GB_RK_STREAM, *n, x, y)) {
         void (*f)(const int *, const double *, const double *, const int *, double *,
                   const int *) = gb_host_fn("daxpy_");
         GB_COUNT(fallthrough, 1);
@@ -62,7 +66,6 @@ void daxpy_(const int *n, const double *alpha, const double *x, const int *incx,
     gb_leave();
 }
 
-/* ---- DDOT ---- */
 /* DSCAL / DCOPY / DSWAP / DROT / DASUM: netlib definitions, which the
  * reference's CBLAS specs forward to (blas_level1/scal.cc:42, copy.cc:46,
  * swap.cc:12-29, rot.cc:50, asum.cc:48); same fall-through rule as DAXPY for small
@@ -75,7 +78,7 @@ void dscal_(const int *n, const double *alpha, double *x, const int *incx)
         gb_leave();
         return;
     }
-    if (small_untracked(*n, x, NULL, NULL)) {
+    if (l1_host(GB_RK_STREAM, *n, x, NULL)) {
         void (*f)(const int *, const double *, double *, const int *) = gb_host_fn("dscal_");
         GB_COUNT(fallthrough, 1);
         f(n, alpha, x, incx);
@@ -100,7 +103,7 @@ void dcopy_(const int *n, const double *x, const int *incx, double *y, const int
         gb_leave();
         return;
     }
-    if (small_untracked(*n, x, y, NULL)) {
+    if (l1_host(GB_RK_STREAM, *n, x, y)) {
         void (*f)(const int *, const double *, const int *, double *, const int *) =
             gb_host_fn("dcopy_");
         GB_COUNT(fallthrough, 1);
@@ -129,7 +132,7 @@ void dswap_(const int *n, double *x, const int *incx, double *y, const int *incy
         gb_leave();
         return;
     }
-    if (small_untracked(*n, x, y, NULL)) {
+    if (l1_host(GB_RK_STREAM, *n, x, y)) {
         void (*f)(const int *, double *, const int *, double *, const int *) = gb_host_fn("dswap_");
         GB_COUNT(fallthrough, 1);
         f(n, x, incx, y, incy);
@@ -157,7 +160,7 @@ void drot_(const int *n, double *x, const int *incx, double *y, const int *incy,
         gb_leave();
         return;
     }
-    if (small_untracked(*n, x, y, NULL)) {
+    if (l1_host(GB_RK_STREAM, *n, x, y)) {
         void (*f)(const int *, double *, const int *, double *, const int *, const double *,
                   const double *) = gb_host_fn("drot_");
         GB_COUNT(fallthrough, 1);
@@ -186,7 +189,7 @@ double dasum_(const int *n, const double *x, const int *incx)
         gb_leave();
         return 0.0;
     }
-    if (small_untracked(*n, x, NULL, NULL)) {
+    if (l1_host(GB_RK_REDUCE, *n, x, NULL)) {
         double (*f)(const int *, const double *, const int *) = gb_host_fn("dasum_");
         GB_COUNT(fallthrough, 1);
         double r = f(n, x, incx);
@@ -215,7 +218,7 @@ double ddot_(const int *n, const double *x, const int *incx, const double *y, co
         gb_leave();
         return 0.0;
     }
-    if (small_untracked(*n, x, y, NULL)) {
+    if (l1_host(GB_RK_REDUCE, *n, x, y)) {
         double (*f)(const int *, const double *, const int *, const double *, const int *) =
             gb_host_fn("ddot_");
         GB_COUNT(fallthrough, 1);
@@ -249,7 +252,7 @@ double dnrm2_(const int *n, const double *x, const int *incx)
         gb_leave();
         return 0.0;
     }
-    if (small_untracked(*n, x, NULL, NULL)) {
+    if (l1_host(GB_RK_REDUCE, *n, x, NULL)) {
         double (*f)(const int *, const double *, const int *) = gb_host_fn("dnrm2_");
         GB_COUNT(fallthrough, 1);
         double r = f(n, x, incx);
@@ -283,7 +286,7 @@ int idamax_(const int *n, const double *x, const int *incx)
         gb_leave();
         return 1;
     }
-    if (small_untracked(*n, x, NULL, NULL)) {
+    if (l1_host(GB_RK_REDUCE, *n, x, NULL)) {
         int (*f)(const int *, const double *, const int *) = gb_host_fn("idamax_");
         GB_COUNT(fallthrough, 1);
         int r = f(n, x, incx);
@@ -330,7 +333,8 @@ void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, 
         return;
     }
     const int lenx = notr ? *n : *m, leny = notr ? *m : *n;
-    if (small_untracked(MAXI(*m, *n), a, x, y)) {
+    if (gb_host_route(GB_RK_DGEMV, gb_opt.cutoff > 0 && MAXI(*m, *n) < gb_opt.cutoff,
+                      2.0 * *m * *n, 8.0 * ((double)*m * *n + *m + *n), a, x, y)) {
         void (*f)(const char *, const int *, const int *, const double *, const double *,
                   const int *, const double *, const int *, const double *, double *,
                   const int *) = gb_host_fn("dgemv_");
@@ -385,7 +389,8 @@ void dger_(const int *m, const int *n, const double *alpha, const double *x, con
         gb_leave();
         return;
     }
-    if (small_untracked(MAXI(*m, *n), a, x, y)) {
+    if (gb_host_route(GB_RK_DGER, gb_opt.cutoff > 0 && MAXI(*m, *n) < gb_opt.cutoff,
+                      2.0 * *m * *n, 8.0 * (2.0 * *m * *n + *m + *n), a, x, y)) {
         void (*f)(const int *, const int *, const double *, const double *, const int *,
                   const double *, const int *, double *, const int *) = gb_host_fn("dger_");
         GB_COUNT(fallthrough, 1);
@@ -430,7 +435,8 @@ void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n,
         gb_leave();
         return;
     }
-    if (small_untracked(*n, a, x, NULL)) {
+    if (gb_host_route(GB_RK_DTRSV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, (double)*n * *n,
+                      8.0 * ((double)*n * *n + 2.0 * *n), a, x, NULL)) {
         void (*f)(const char *, const char *, const char *, const int *, const double *,
                   const int *, double *, const int *) = gb_host_fn("dtrsv_");
         GB_COUNT(fallthrough, 1);
@@ -450,3 +456,11 @@ void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n,
     GB_COUNT(device_calls, 1);
     gb_leave();
 }
+
+/* library-internal names for the crossover measurement (see level3.c) */
+#define GB_SELF(f) extern __typeof__(f) gb_self_##f __attribute__((alias(#f), visibility("hidden")));
+GB_SELF(dgemv_)
+GB_SELF(dger_)
+GB_SELF(dtrsv_)
+GB_SELF(daxpy_)
+GB_SELF(ddot_)

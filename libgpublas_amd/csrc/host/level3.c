@@ -4,8 +4,9 @@
  *
  * Each entry follows the reference's three steps (blas_level3/gemm.cc:87-179
  * and the syrk/trsm/symm twins): netlib argument check with xerbla, quick
- * return, then either the host fall-through (GEMM cutoff, gemm.cc:129-141)
- * or the device path: stage the operands (tracked objects in place, others
+ * return, then either the host fall-through (the GEMM cutoff of
+ * gemm.cc:129-141; under cutoff=auto the measured crossover of route.c, for
+ * every routine) or the device path: stage the operands (tracked objects in place, others
  * through pool buffers), launch the gfx950 kernel, copy results back, and
  * synchronise before returning so results are visible to the caller (the
  * reference skips that sync on some devices, runtime.h:33-34).
@@ -45,7 +46,7 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
 /* ------------------------------------------------------------------ */
 /* GEMM                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_GEMM(PRE, T, NAME6, DEVFN, SERVEFN)                                                        \
+#define DEFINE_GEMM(PRE, T, NAME6, DEVFN, SERVEFN, RK)                                                    \
     void PRE##gemm_(const char *transa, const char *transb, const int *m, const int *n,          \
                     const int *k, const T *alpha, const T *a, const int *lda, const T *b,        \
                     const int *ldb, const T *beta, T *c, const int *ldc)                          \
@@ -73,8 +74,10 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
             return;                                                                              \
         }                                                                                        \
         const int cut = gb_opt.cutoff;                                                           \
-        if (cut > 0 && (*lda < cut || *ldb < cut || *ldc < cut) && !gb_ptr_tracked(a) &&         \
-            !gb_ptr_tracked(b) && !gb_ptr_tracked(c)) {                                          \
+        const double es_ = sizeof(T), mm_ = *m, nn_ = *n, kk_ = *k;                              \
+        if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldb < cut || *ldc < cut),             \
+                          2 * mm_ * nn_ * kk_, es_ * (mm_ * kk_ + kk_ * nn_ + 2 * mm_ * nn_), a,  \
+                          b, c)) {                                                               \
             typedef void (*fn_t)(const char *, const char *, const int *, const int *,           \
                                  const int *, const T *, const T *, const int *, const T *,      \
                                  const int *, const T *, T *, const int *);                      \
@@ -115,13 +118,13 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
         gb_leave();                                                                              \
     }
 
-DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev, gb_serve_dgemm)
-DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm)
+DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev, gb_serve_dgemm, GB_RK_DGEMM)
+DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm, GB_RK_SGEMM)
 
 /* ------------------------------------------------------------------ */
 /* SYRK                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_SYRK(PRE, T, NAME6, DEVFN)                                                        \
+#define DEFINE_SYRK(PRE, T, NAME6, DEVFN, RK)                                                    \
     void PRE##syrk_(const char *uplo, const char *trans, const int *n, const int *k,             \
                     const T *alpha, const T *a, const int *lda, const T *beta, T *c,             \
                     const int *ldc)                                                              \
@@ -146,6 +149,19 @@ DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm)
         if (*n == 0 || ((*alpha == 0 || *k == 0) && *beta == 1)) {                               \
             gb_leave();                                                                          \
             return;                                                                              \
+        }                                                                                        \
+        {                                                                                        \
+            const double es_ = sizeof(T), nn_ = *n, kk_ = *k;                                    \
+            if (gb_host_route(RK, 0, nn_ * nn_ * kk_, es_ * (nn_ * kk_ + 2 * nn_ * nn_), a, c,   \
+                              NULL)) {                                                           \
+                typedef void (*fn_t)(const char *, const char *, const int *, const int *,       \
+                                     const T *, const T *, const int *, const T *, T *,          \
+                                     const int *);                                               \
+                GB_COUNT(fallthrough, 1);                                                        \
+                ((fn_t)gb_host_fn(#PRE "syrk_"))(uplo, trans, n, k, alpha, a, lda, beta, c, ldc); \
+                gb_leave();                                                                      \
+                return;                                                                          \
+            }                                                                                    \
         }                                                                                        \
         const int scale_only = (*alpha == 0 || *k == 0);                                         \
         if (scale_only && !gb_ptr_tracked(c)) {                                                  \
@@ -172,14 +188,14 @@ DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm)
         gb_leave();                                                                              \
     }
 
-DEFINE_SYRK(d, double, "DSYRK ", dsyrk_dev)
-DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
+DEFINE_SYRK(d, double, "DSYRK ", dsyrk_dev, GB_RK_DSYRK)
+DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk, GB_RK_SSYRK)
 
 /* ------------------------------------------------------------------ */
 /* SYR2K (netlib DSYR2K; the reference's syr2k.cc:78-146 check, info   */
 /* 1 uplo, 2 trans, 3 n, 4 k, 7 lda, 9 ldb, 12 ldc)                    */
 /* ------------------------------------------------------------------ */
-#define DEFINE_SYR2K(PRE, T, NAME6, DEVFN)                                                       \
+#define DEFINE_SYR2K(PRE, T, NAME6, DEVFN, RK)                                                   \
     void PRE##syr2k_(const char *uplo, const char *trans, const int *n, const int *k,            \
                      const T *alpha, const T *a, const int *lda, const T *b, const int *ldb,     \
                      const T *beta, T *c, const int *ldc)                                        \
@@ -205,6 +221,20 @@ DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
         if (*n == 0 || ((*alpha == 0 || *k == 0) && *beta == 1)) {                               \
             gb_leave();                                                                          \
             return;                                                                              \
+        }                                                                                        \
+        {                                                                                        \
+            const double es_ = sizeof(T), nn_ = *n, kk_ = *k;                                    \
+            if (gb_host_route(RK, 0, 2 * nn_ * nn_ * kk_,                                        \
+                              es_ * (2 * nn_ * kk_ + 2 * nn_ * nn_), a, b, c)) {                 \
+                typedef void (*fn_t)(const char *, const char *, const int *, const int *,       \
+                                     const T *, const T *, const int *, const T *, const int *,  \
+                                     const T *, T *, const int *);                               \
+                GB_COUNT(fallthrough, 1);                                                        \
+                ((fn_t)gb_host_fn(#PRE "syr2k_"))(uplo, trans, n, k, alpha, a, lda, b, ldb,      \
+                                                  beta, c, ldc);                                 \
+                gb_leave();                                                                      \
+                return;                                                                          \
+            }                                                                                    \
         }                                                                                        \
         const int scale_only = (*alpha == 0 || *k == 0);                                         \
         if (scale_only && !gb_ptr_tracked(c)) {                                                  \
@@ -232,14 +262,14 @@ DEFINE_SYRK(s, float, "SSYRK ", gpublas_dev_ssyrk)
         gb_leave();                                                                              \
     }
 
-DEFINE_SYR2K(d, double, "DSYR2K", gpublas_dev_dsyr2k)
-DEFINE_SYR2K(s, float, "SSYR2K", gpublas_dev_ssyr2k)
+DEFINE_SYR2K(d, double, "DSYR2K", gpublas_dev_dsyr2k, GB_RK_DSYR2K)
+DEFINE_SYR2K(s, float, "SSYR2K", gpublas_dev_ssyr2k, GB_RK_SSYR2K)
 
 /* ------------------------------------------------------------------ */
 /* TRMM (netlib DTRMM: info 1 side, 2 uplo, 3 transa, 4 diag, 5 m,     */
 /* 6 n, 9 lda, 11 ldb; trmm.cc is not built by the reference)          */
 /* ------------------------------------------------------------------ */
-#define DEFINE_TRMM(PRE, T, NAME6, DEVFN)                                                        \
+#define DEFINE_TRMM(PRE, T, NAME6, DEVFN, RK, HOSTNAME)                                          \
     void PRE##trmm_(const char *side, const char *uplo, const char *transa, const char *diag,    \
                     const int *m, const int *n, const T *alpha, const T *a, const int *lda,      \
                     T *b, const int *ldb)                                                        \
@@ -268,6 +298,20 @@ DEFINE_SYR2K(s, float, "SSYR2K", gpublas_dev_ssyr2k)
             gb_leave();                                                                          \
             return;                                                                              \
         }                                                                                        \
+        {                                                                                        \
+            const double es_ = sizeof(T), mm_ = *m, nn_ = *n, na_ = nrowa;                      \
+            if (gb_host_route(RK, 0, mm_ * nn_ * na_, es_ * (na_ * na_ + 2 * mm_ * nn_), a, b,   \
+                              NULL)) {                                                           \
+                typedef void (*fn_t)(const char *, const char *, const char *, const char *,     \
+                                     const int *, const int *, const T *, const T *,             \
+                                     const int *, T *, const int *);                             \
+                GB_COUNT(fallthrough, 1);                                                        \
+                ((fn_t)gb_host_fn(HOSTNAME))(side, uplo, transa, diag, m, n, alpha, a, lda, b,   \
+                                             ldb);                                               \
+                gb_leave();                                                                      \
+                return;                                                                          \
+            }                                                                                    \
+        }                                                                                        \
         if (*alpha == 0 && !gb_ptr_tracked(b)) {                                                 \
             for (int j = 0; j < *n; ++j)                                                         \
                 for (int i = 0; i < *m; ++i) b[IDX(i, j, *ldb)] = 0;                             \
@@ -289,13 +333,13 @@ DEFINE_SYR2K(s, float, "SSYR2K", gpublas_dev_ssyr2k)
         gb_leave();                                                                              \
     }
 
-DEFINE_TRMM(d, double, "DTRMM ", gpublas_dev_dtrmm)
-DEFINE_TRMM(s, float, "STRMM ", gpublas_dev_strmm)
+DEFINE_TRMM(d, double, "DTRMM ", gpublas_dev_dtrmm, GB_RK_DTRMM, "dtrmm_")
+DEFINE_TRMM(s, float, "STRMM ", gpublas_dev_strmm, GB_RK_STRMM, "strmm_")
 
 /* ------------------------------------------------------------------ */
 /* TRSM                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_TRSM(PRE, T, NAME6, DEVFN, SERVEFN)                                                        \
+#define DEFINE_TRSM(PRE, T, NAME6, DEVFN, SERVEFN, RK, HOSTNAME)                                      \
     void PRE##trsm_(const char *side, const char *uplo, const char *transa, const char *diag,    \
                     const int *m, const int *n, const T *alpha, const T *a, const int *lda,      \
                     T *b, const int *ldb)                                                        \
@@ -323,6 +367,20 @@ DEFINE_TRMM(s, float, "STRMM ", gpublas_dev_strmm)
         if (*m == 0 || *n == 0) {                                                                \
             gb_leave();                                                                          \
             return;                                                                              \
+        }                                                                                        \
+        {                                                                                        \
+            const double es_ = sizeof(T), mm_ = *m, nn_ = *n, na_ = nrowa;                      \
+            if (gb_host_route(RK, 0, mm_ * nn_ * na_, es_ * (na_ * na_ + 2 * mm_ * nn_), a, b,   \
+                              NULL)) {                                                           \
+                typedef void (*fn_t)(const char *, const char *, const char *, const char *,     \
+                                     const int *, const int *, const T *, const T *,             \
+                                     const int *, T *, const int *);                             \
+                GB_COUNT(fallthrough, 1);                                                        \
+                ((fn_t)gb_host_fn(HOSTNAME))(side, uplo, transa, diag, m, n, alpha, a, lda, b,   \
+                                             ldb);                                               \
+                gb_leave();                                                                      \
+                return;                                                                          \
+            }                                                                                    \
         }                                                                                        \
         if (*alpha == 0 && !gb_ptr_tracked(b)) {                                                 \
             for (int j = 0; j < *n; ++j)                                                         \
@@ -358,13 +416,13 @@ static int no_serve_strsm(char side, char uplo, char transa, char diag, int m, i
     return 0;
 }
 
-DEFINE_TRSM(d, double, "DTRSM ", gpublas_dev_dtrsm, gb_serve_dtrsm)
-DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm)
+DEFINE_TRSM(d, double, "DTRSM ", gpublas_dev_dtrsm, gb_serve_dtrsm, GB_RK_DTRSM, "dtrsm_")
+DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm, GB_RK_STRSM, "strsm_")
 
 /* ------------------------------------------------------------------ */
 /* SYMM                                                                */
 /* ------------------------------------------------------------------ */
-#define DEFINE_SYMM(PRE, T, NAME6, DEVFN)                                                        \
+#define DEFINE_SYMM(PRE, T, NAME6, DEVFN, RK)                                                    \
     void PRE##symm_(const char *side, const char *uplo, const int *m, const int *n,              \
                     const T *alpha, const T *a, const int *lda, const T *b, const int *ldb,      \
                     const T *beta, T *c, const int *ldc)                                         \
@@ -391,6 +449,20 @@ DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm)
             gb_leave();                                                                          \
             return;                                                                              \
         }                                                                                        \
+        {                                                                                        \
+            const double es_ = sizeof(T), mm_ = *m, nn_ = *n, na_ = nrowa;                      \
+            if (gb_host_route(RK, 0, 2 * mm_ * nn_ * na_, es_ * (na_ * na_ + 3 * mm_ * nn_), a,  \
+                              b, c)) {                                                           \
+                typedef void (*fn_t)(const char *, const char *, const int *, const int *,       \
+                                     const T *, const T *, const int *, const T *, const int *,  \
+                                     const T *, T *, const int *);                               \
+                GB_COUNT(fallthrough, 1);                                                        \
+                ((fn_t)gb_host_fn(#PRE "symm_"))(side, uplo, m, n, alpha, a, lda, b, ldb, beta,  \
+                                                 c, ldc);                                        \
+                gb_leave();                                                                      \
+                return;                                                                          \
+            }                                                                                    \
+        }                                                                                        \
         if (*alpha == 0 && !gb_ptr_tracked(c)) {                                                 \
             for (int j = 0; j < *n; ++j)                                                         \
                 for (int i = 0; i < *m; ++i)                                                     \
@@ -415,5 +487,22 @@ DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm)
         gb_leave();                                                                              \
     }
 
-DEFINE_SYMM(d, double, "DSYMM ", gpublas_dev_dsymm)
-DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm)
+DEFINE_SYMM(d, double, "DSYMM ", gpublas_dev_dsymm, GB_RK_DSYMM)
+DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm, GB_RK_SSYMM)
+
+/* Library-internal names of the entry points above, bound inside the library
+ * (the crossover measurement calls them; a process may hold another BLAS
+ * whose exported dgemm_ would win an ordinary symbol lookup). */
+#define GB_SELF(f) extern __typeof__(f) gb_self_##f __attribute__((alias(#f), visibility("hidden")));
+GB_SELF(dgemm_)
+GB_SELF(sgemm_)
+GB_SELF(dsyrk_)
+GB_SELF(ssyrk_)
+GB_SELF(dsyr2k_)
+GB_SELF(ssyr2k_)
+GB_SELF(dsymm_)
+GB_SELF(ssymm_)
+GB_SELF(dtrsm_)
+GB_SELF(strsm_)
+GB_SELF(dtrmm_)
+GB_SELF(strmm_)
