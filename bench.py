@@ -123,7 +123,9 @@ def gemm_figures():
         bufs = [lib.gpublas_dev_alloc(n * n * 8) for _ in range(3)]
         for i, b in enumerate(bufs):
             lib.gpublas_dev_fill_uniform(None, 101 + i, n * n, b)
-        kern = "dgemm_fast_128x128" if n >= 2048 else "dgemm_fast_64x64"
+        # 1024^3 runs as split-K over 64 x 64 tiles (its 256 tiles fill half
+        # the workgroup slots); 8192^3 on 128 x 128 tiles
+        kerns = ["dgemm_fast_128x128"] if n >= 2048 else ["dgemm_splitk", "dgemm_fast_64x64"]
         for _ in range(2):
             lib.gpublas_dev_dgemm(None, b"N", b"N", n, n, n, 1.0, bufs[0], n, bufs[1], n, 0.5, bufs[2], n)
         lib.gpublas_dev_sync(None)
@@ -134,7 +136,11 @@ def gemm_figures():
         lib.gpublas_dev_sync(None)
         wall = time.perf_counter() - t0
         tot, mn, mx = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        cnt = lib.gpublas_dev_prof_query(kern.encode(), ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(mx))
+        for kern in kerns:
+            cnt = lib.gpublas_dev_prof_query(kern.encode(), ctypes.byref(tot), ctypes.byref(mn),
+                                             ctypes.byref(mx))
+            if cnt:
+                break
         flops = 2.0 * n ** 3
         kavg = tot.value / max(cnt, 1) / 1e3
         out[f"dgemm_nn_{n}"] = {

@@ -9,6 +9,8 @@
  *   blas_bench --lib ... --op syr2k --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trmm --m M --n N --side L --uplo L --ta N --diag N
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
+ *   blas_bench --lib ... --op link --n BYTES        (host-link copies of pageable
+ *                                                    memory: plain vs staged)
  *   common: --reps R --warmup W --seed S
  *
  * Operands come from gpublas_malloc and are filled on the device with the
@@ -115,6 +117,50 @@ int main(int argc, char **argv)
     void (*prof_reset)(void) = (void (*)(void))sym("gpublas_dev_prof_reset");
     set_cutoff(0);
     (void)ktime;
+
+    if (!strcmp(op, "link")) {
+        /* best-of-reps time of one contiguous copy each way, plain pageable
+         * hipMemcpyAsync against the library's staged copy; each timed copy
+         * ends with a stream synchronisation */
+        typedef int (*cp_t)(void *, const void *, size_t, void *);
+        typedef int (*cps_t)(void *, size_t, const void *, size_t, size_t, size_t, size_t, void *);
+        cp_t h2d = (cp_t)sym("gpublas_dev_h2d"), d2h = (cp_t)sym("gpublas_dev_d2h");
+        cps_t h2ds = (cps_t)sym("gpublas_dev_h2d_staged"), d2hs = (cps_t)sym("gpublas_dev_d2h_staged");
+        void *(*dalloc)(size_t) = (void *(*)(size_t))sym("gpublas_dev_alloc");
+        const size_t nb = (size_t)n;
+        char *h = malloc(nb), *h2 = malloc(nb);
+        void *d = dalloc(nb);
+        if (!h || !h2 || !d) {
+            fprintf(stderr, "blas_bench: cannot allocate %zu bytes\n", nb);
+            return 2;
+        }
+        memset(h, 1, nb);
+        memset(h2, 2, nb);
+        double t[4] = {1e30, 1e30, 1e30, 1e30};
+        for (int r = 0; r < warmup + reps; ++r) {
+            double t0 = now(), dt;
+            h2d(d, h, nb, NULL), g_sync(NULL), dt = now() - t0;
+            if (r >= warmup && dt < t[0]) t[0] = dt;
+            t0 = now(), h2ds(d, nb, h, nb, nb, 1, 1, NULL), g_sync(NULL), dt = now() - t0;
+            if (r >= warmup && dt < t[1]) t[1] = dt;
+            t0 = now(), d2h(h2, d, nb, NULL), g_sync(NULL), dt = now() - t0;
+            if (r >= warmup && dt < t[2]) t[2] = dt;
+            t0 = now(), d2hs(h2, nb, d, nb, nb, 1, 1, NULL), g_sync(NULL), dt = now() - t0;
+            if (r >= warmup && dt < t[3]) t[3] = dt;
+        }
+        if (memcmp(h, h2, nb) != 0) {
+            fprintf(stderr, "blas_bench: link copy round trip differs\n");
+            return 1;
+        }
+        printf("{\"op\": \"link\", \"bytes\": %zu, \"upload_pageable_us\": %.1f, "
+               "\"upload_staged_us\": %.1f, \"download_pageable_us\": %.1f, "
+               "\"download_staged_us\": %.1f, \"upload_pageable_gbs\": %.1f, "
+               "\"upload_staged_gbs\": %.1f, \"download_pageable_gbs\": %.1f, "
+               "\"download_staged_gbs\": %.1f}\n",
+               nb, t[0] * 1e6, t[1] * 1e6, t[2] * 1e6, t[3] * 1e6, nb / t[0] / 1e9,
+               nb / t[1] / 1e9, nb / t[2] / 1e9, nb / t[3] / 1e9);
+        return 0;
+    }
 
     double flops = 0, bytes = 0;
     void *A = NULL, *B = NULL, *C = NULL;

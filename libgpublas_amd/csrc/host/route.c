@@ -23,7 +23,8 @@
  * times square problems of growing size on the host BLAS and on the device
  * path with operands that are not resident (so the device time includes the
  * host-link copies and the synchronisation) and takes the smallest size from
- * which the GPU is faster at every larger size measured.  min_flops is that
+ * which the GPU is faster at every larger size measured (each side timed in
+ * a phase of its own).  min_flops is that
  * size's flop count (the fixed cost of a device call) and min_intensity its
  * flops per byte copied (the link cost: a long thin product with the same
  * flops but far more bytes stays on the host).  A routine on which the GPU
@@ -55,9 +56,10 @@ struct xover {
 };
 
 /* Built-in table, measured with gpublas_tune_crossover(path, 2048) on an
- * MI355X host (16 host threads, OpenBLAS 0.3 from scipy): the square size
- * from which the GPU, copying its operands over the host link on every call,
- * beat the host BLAS at every larger size measured.  Level-1/2 calls never
+ * MI355X host (16 host threads, scipy's OpenBLAS): the square size from
+ * which the GPU, copying its operands over the host link on every call
+ * (staged through pinned memory), beat the host BLAS at every larger size
+ * measured.  Level-1/2 calls never
  * won (copying the operands costs more than the host's one pass over them).
  * Other machines measure their own table and load it with xover=<file>. */
 #define SQ3(a, n, es) {(a) * (double)(n) * (n) * (n), (a) * (double)(n) / (es)}
@@ -65,16 +67,16 @@ struct xover {
 static struct xover g_xover[GB_RK_COUNT] = {
     [GB_RK_DGEMM] = SQ3(2, 1024, 32),  /* 2n^3 flops over 4 n^2 doubles */
     [GB_RK_SGEMM] = SQ3(2, 1024, 16),
-    [GB_RK_DSYRK] = SQ3(1, 1024, 24),  /* n^3 flops over 3 n^2 doubles */
-    [GB_RK_SSYRK] = SQ3(1, 1024, 12),
-    [GB_RK_DSYR2K] = SQ3(2, 768, 32),
+    [GB_RK_DSYRK] = SQ3(1, 512, 24),   /* n^3 flops over 3 n^2 doubles */
+    [GB_RK_SSYRK] = SQ3(1, 768, 12),
+    [GB_RK_DSYR2K] = SQ3(2, 384, 32),
     [GB_RK_SSYR2K] = SQ3(2, 384, 16),
     [GB_RK_DSYMM] = SQ3(2, 1024, 32),
     [GB_RK_SSYMM] = SQ3(2, 1024, 16),
-    [GB_RK_DTRSM] = SQ3(1, 2048, 24),
-    [GB_RK_STRSM] = SQ3(1, 1536, 12),
-    [GB_RK_DTRMM] = SQ3(1, 2048, 24),
-    [GB_RK_STRMM] = SQ3(1, 2048, 12),
+    [GB_RK_DTRSM] = SQ3(1, 1280, 24),
+    [GB_RK_STRSM] = SQ3(1, 1280, 12),
+    [GB_RK_DTRMM] = SQ3(1, 1280, 24),
+    [GB_RK_STRMM] = SQ3(1, 1280, 12),
     [GB_RK_DGEMV] = {INF_, INF_},
     [GB_RK_DGER] = {INF_, INF_},
     [GB_RK_DTRSV] = {INF_, INF_},
@@ -442,35 +444,45 @@ int gpublas_tune_crossover(const char *path, int max_n)
         const size_t cap = lvl == 3 ? (size_t)max3 : lvl == 2 ? (size_t)max2 : max1;
         const void *host = gb_host_fn(k_f77[k]);
         const void *dev = self_fn(k);
-        int run_start = -1, run = 0;
-        res[k].flops = res[k].intensity = HUGE_VAL;
-        for (int i = 0; i < nl && (size_t)lad[i] <= cap; ++i) {
-            const int n = lad[i];
-            const size_t len = lvl == 1 ? (size_t)n : (size_t)n * n;
-            const int tri = k == GB_RK_DTRSM || k == GB_RK_STRSM || k == GB_RK_DTRMM ||
-                            k == GB_RK_STRMM || k == GB_RK_DTRSV;
-            /* TRSM divides by a diagonal of 1 + n/2: the solve stays near the
-             * right-hand side's scale; TRMM keeps the same matrix */
-            fill(u.a, len, single, 1, tri ? 1.0 + n / 2.0 : 0.0, lvl == 1 ? 0 : n);
-            fill(u.b0, len, single, 2, 0, 0);
-            fill(u.c0, len, single, 3, 0, 0);
-            memcpy(u.b, u.b0, len * (single ? 4 : 8));
-            const double th = best_us(k, n, host, &u, 0);
-            const double tg = best_us(k, n, dev, &u, 1);
+        int ns[32], cnt = 0;
+        for (int i = 0; i < nl && (size_t)lad[i] <= cap && cnt < 32; ++i) ns[cnt++] = lad[i];
+        double th[32], tg[32];
+        /* Each side is timed in a phase of its own, the device first after a
+         * pause: a host BLAS's threads keep spinning for a while after a call
+         * and would otherwise take the cores the device path's copies use. */
+        const struct timespec pause = {0, 200 * 1000 * 1000};
+        nanosleep(&pause, NULL);
+        for (int side = 1; side >= 0; --side)
+            for (int i = 0; i < cnt; ++i) {
+                const int n = ns[i];
+                const size_t len = lvl == 1 ? (size_t)n : (size_t)n * n;
+                const int tri = k == GB_RK_DTRSM || k == GB_RK_STRSM || k == GB_RK_DTRMM ||
+                                k == GB_RK_STRMM || k == GB_RK_DTRSV;
+                /* TRSM divides by a diagonal of 1 + n/2: the solve stays near
+                 * the right-hand side's scale; TRMM keeps the same matrix */
+                fill(u.a, len, single, 1, tri ? 1.0 + n / 2.0 : 0.0, lvl == 1 ? 0 : n);
+                fill(u.b0, len, single, 2, 0, 0);
+                fill(u.c0, len, single, 3, 0, 0);
+                memcpy(u.b, u.b0, len * (single ? 4 : 8));
+                if (side) tg[i] = best_us(k, n, dev, &u, 1);
+                else th[i] = best_us(k, n, host, &u, 0);
+            }
+        int run_start = -1;
+        for (int i = 0; i < cnt; ++i) {
             double fl, by;
-            gb_route_model(k, n, &fl, &by);
+            gb_route_model(k, ns[i], &fl, &by);
             if (f) {
-                fprintf(f, "# %s %d %.6g %.6g %.2f %.2f\n", k_names[k], n, fl, by, th, tg);
+                fprintf(f, "# %s %d %.6g %.6g %.2f %.2f\n", k_names[k], ns[i], fl, by, th[i],
+                        tg[i]);
                 fflush(f); /* a long measurement shows its progress in the file */
             }
-            if (tg < th) {
-                if (run_start < 0) run_start = n;
-                if (++run >= 3) break;
+            if (tg[i] < th[i]) {
+                if (run_start < 0) run_start = ns[i];
             } else {
                 run_start = -1;
-                run = 0;
             }
         }
+        res[k].flops = res[k].intensity = HUGE_VAL;
         if (run_start > 0) {
             double fl, by;
             gb_route_model(k, run_start, &fl, &by);

@@ -459,8 +459,7 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
     if (!op->tmp) gb_fatal("HBM exhausted staging a %zu-byte operand", span);
     op->dev = op->tmp;
     if (mode & GB_IN) {
-        int e = ld == rows ? gpublas_dev_h2d(op->dev, host, span, stream)
-                           : gpublas_dev_h2d_2d(op->dev, ld, host, ld, rows, cols, es, stream);
+        int e = gpublas_dev_h2d_staged(op->dev, ld, host, ld, rows, cols, es, stream);
         gb_check(e, "operand upload");
         GB_COUNT(h2d_bytes, rows * cols * es);
         op->uploaded = 1;
@@ -490,11 +489,8 @@ void gb_release(gb_operand *op, void *stream)
         return;
     }
     if (op->mode & GB_OUT) {
-        int e = op->ld == op->rows
-                    ? gpublas_dev_d2h((void *)op->host, op->dev,
-                                      ((op->cols - 1) * op->ld + op->rows) * op->es, stream)
-                    : gpublas_dev_d2h_2d((void *)op->host, op->ld, op->dev, op->ld, op->rows,
-                                         op->cols, op->es, stream);
+        int e = gpublas_dev_d2h_staged((void *)op->host, op->ld, op->dev, op->ld, op->rows,
+                                       op->cols, op->es, stream);
         gb_check(e, "operand download");
         GB_COUNT(d2h_bytes, op->rows * op->cols * op->es);
     }
