@@ -17,6 +17,8 @@
  *       dgemv_         blas_level2/gemv.cc:90-98 (blas.h:85-92)
  *       dger_          blas_level2/ger.cc:86-94
  *       dtrsv_         blas_level2/trsv.cc:81-88
+ *       dsymv_         blas_level2/symv.cc:11-75 (CBLAS spec; netlib DSYMV)
+ *       dsyr_          blas_level2/syr.cc:11-68 (CBLAS spec; netlib DSYR)
  *       daxpy_         blas_level1/axpy.cc:48-50 (blas.h:20-31)
  *       ddot_          blas_level1/dot.cc:44-48 (returns double by value,
  *                      not the reference's void macro blas.h:45-51)
@@ -91,6 +93,10 @@ void dger_(const int *m, const int *n, const double *alpha, const double *x, con
            const double *y, const int *incy, double *a, const int *lda);
 void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *a,
             const int *lda, double *x, const int *incx);
+void dsymv_(const char *uplo, const int *n, const double *alpha, const double *a, const int *lda,
+            const double *x, const int *incx, const double *beta, double *y, const int *incy);
+void dsyr_(const char *uplo, const int *n, const double *alpha, const double *x, const int *incx,
+           double *a, const int *lda);
 void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
             const int *incy);
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
@@ -138,6 +144,10 @@ void cblas_dger(CBLAS_LAYOUT layout, int m, int n, double alpha, const double *x
                 const double *y, int incy, double *a, int lda);
 void cblas_dtrsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
                  int n, const double *a, int lda, double *x, int incx);
+void cblas_dsymv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *a,
+                 int lda, const double *x, int incx, double beta, double *y, int incy);
+void cblas_dsyr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                int incx, double *a, int lda);
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy);
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);

@@ -153,6 +153,12 @@ int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x
                      const double *d_y, int incy, double *d_A, int lda);
 int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,
                       int lda, double *d_x, int incx);
+/* DSYMV / DSYR on the uplo triangle (replace cublasDsymv, blas_level2/symv.cc:57-64,
+ * and cublasDsyr, syr.cc:50-55). */
+int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double *d_A, int lda,
+                      const double *d_x, int incx, double beta, double *d_y, int incy);
+int gpublas_dev_dsyr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                     double *d_A, int lda);
 
 /* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
  * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */

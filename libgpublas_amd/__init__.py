@@ -96,6 +96,8 @@ def _declare(lib):
     lib.cblas_sgemm.argtypes = [ci, ci, ci, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci]
     lib.cblas_dgemv.argtypes = [ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.cblas_dger.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, vp, ci]
+    lib.cblas_dsymv.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
+    lib.cblas_dsyr.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci]
     lib.cblas_dsyrk.argtypes = [ci, ci, ci, ci, ci, cd, vp, ci, cd, vp, ci]
     lib.cblas_dtrsm.argtypes = [ci, ci, ci, ci, ci, ci, ci, cd, vp, ci, vp, ci]
     lib.cblas_dsymm.argtypes = [ci, ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
@@ -226,6 +228,15 @@ def dtrsv(uplo, trans, diag, n, a, lda, x, incx):
     library().dtrsv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(a), _i(lda), _p(x), _i(incx))
 
 
+def dsymv(uplo, n, alpha, a, lda, x, incx, beta, y, incy):
+    library().dsymv_(_c(uplo), _i(n), _d(alpha), _p(a), _i(lda), _p(x), _i(incx), _d(beta),
+                     _p(y), _i(incy))
+
+
+def dsyr(uplo, n, alpha, x, incx, a, lda):
+    library().dsyr_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(a), _i(lda))
+
+
 def daxpy(n, alpha, x, incx, y, incy):
     library().daxpy_(_i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy))
 
@@ -280,8 +291,8 @@ def get_cutoff() -> int:
 
 CUTOFF_AUTO = -1
 CROSSOVER_ROUTINES = ("dgemm", "sgemm", "dsyrk", "ssyrk", "dsyr2k", "ssyr2k", "dsymm", "ssymm",
-                      "dtrsm", "strsm", "dtrmm", "strmm", "dgemv", "dger", "dtrsv", "daxpy",
-                      "ddot")
+                      "dtrsm", "strsm", "dtrmm", "strmm", "dgemv", "dger", "dtrsv", "dsymv",
+                      "dsyr", "daxpy", "ddot")
 
 
 def set_crossover(routine: str, min_flops: float, min_intensity: float) -> None:

@@ -8,6 +8,8 @@
  *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
  *   blas_bench --lib ... --op syr2k --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trmm --m M --n N --side L --uplo L --ta N --diag N
+ *   blas_bench --lib ... --op symv --n N --uplo L   (DSYMV, bytes = the triangle once)
+ *   blas_bench --lib ... --op syr --n N --uplo L    (DSYR, the triangle read and written)
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   blas_bench --lib ... --op link --n BYTES        (host-link copies of pageable
  *                                                    memory: plain vs staged)
@@ -192,6 +194,12 @@ int main(int argc, char **argv)
                             void *, const int *);
     syr2k_t syr2k = NULL;
     trsm_t trmm = NULL; /* same argument list as TRSM */
+    typedef void (*symv_t)(const char *, const int *, const double *, const double *, const int *,
+                           const double *, const int *, const double *, double *, const int *);
+    typedef void (*syr_t)(const char *, const int *, const double *, const double *, const int *,
+                          double *, const int *);
+    symv_t symv = NULL;
+    syr_t syr = NULL;
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -249,6 +257,19 @@ int main(int argc, char **argv)
         B = operand((size_t)m * n, prec, seed + 1);
         C = operand((size_t)m * n, prec, seed + 2);
         flops = 2.0 * m * n * na;
+    } else if (!strcmp(op, "symv")) {
+        symv = (symv_t)sym("dsymv_");
+        A = operand((size_t)n * n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        C = operand((size_t)n, 'd', seed + 2);
+        bytes = 8.0 * ((double)n * (n + 1) / 2 + 3.0 * n); /* triangle once, x, y in/out */
+        flops = 2.0 * n * n;
+    } else if (!strcmp(op, "syr")) {
+        syr = (syr_t)sym("dsyr_");
+        A = operand((size_t)n * n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        bytes = 8.0 * ((double)n * (n + 1) + n); /* triangle read and written, x */
+        flops = (double)n * n;
     } else if (!strcmp(op, "axpy")) {
         axpy = (axpy_t)sym("daxpy_");
         A = operand((size_t)n, 'd', seed);
@@ -271,7 +292,16 @@ int main(int argc, char **argv)
         else if (symm) symm(&side, &uplo, &m, &n, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else if (syr2k) syr2k(&uplo, &trans, &n, &k, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else if (trmm) trmm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
-        else {
+        else if (symv) {
+            const double a = 1.0, b = 0.5;
+            const int one = 1;
+            symv(&uplo, &n, &a, (const double *)A, &n, (const double *)B, &one, &b, (double *)C,
+                 &one);
+        } else if (syr) {
+            const double a = 1e-3;
+            const int one = 1;
+            syr(&uplo, &n, &a, (const double *)B, &one, (double *)A, &n);
+        } else {
             const double a = 0.5;
             const int one = 1;
             axpy(&n, &a, (const double *)A, &one, (double *)B, &one);

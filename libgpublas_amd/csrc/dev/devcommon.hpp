@@ -64,3 +64,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+
+// s_waitcnt immediate: wait until at most n vector-memory operations of this
+// wave are outstanding (LDS and export counters left at their maxima)
+__host__ __device__ constexpr int vm_wait(int n)
+{
+    return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
+}
+
+// Workgroup barrier that waits only for this wave's LDS operations
+// (lgkmcnt 0): global loads stay in flight across it, where the fence of
+// __syncthreads would drain them.  For barriers that order LDS traffic alone.
+// The scheduling barriers keep all code on its side: arithmetic hoisted above
+// it onto registers whose loads were issued just before would make the
+// compiler wait for those loads here.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_sched_barrier(0);
+}

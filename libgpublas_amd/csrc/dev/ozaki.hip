@@ -213,13 +213,6 @@ __device__ __forceinline__ void tile_of(int b, int TM, int TN, int &tm, int &tn)
     tn = r / gs;
 }
 
-// s_waitcnt immediate: wait until at most n vector-memory operations of this
-// wave are outstanding (LDS and export counters left at their maxima)
-__host__ __device__ constexpr int vm_wait(int n)
-{
-    return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8);
-}
-
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 // one 32x32x16 slice product: BF16 slices are pre-scaled, FP16 slices hold
 // the integers (same bits in the LDS stages, read as the MFMA's type)

@@ -220,6 +220,31 @@ void cblas_dtrsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CB
     WITH_MAP(trsv_map, dtrsv_(&u, &t, &d, &n, a, &lda, x, &incx));
 }
 
+/* ---- SYMV: 1 layout 2 uplo 3 n 4 alpha 5 A 6 lda 7 x 8 incx 9 beta 10 y 11 incy.
+ * A row-major symmetric matrix is the column-major one of the other triangle. */
+static const int symv_map[11] = {0, 2, 3, 0, 0, 6, 0, 8, 0, 0, 11};
+
+void cblas_dsymv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *a,
+                 int lda, const double *x, int incx, double beta, double *y, int incy)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYMV ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYMV ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(symv_map, dsymv_(&u, &n, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+}
+
+/* ---- SYR: 1 layout 2 uplo 3 n 4 alpha 5 x 6 incx 7 A 8 lda ---- */
+static const int syr_map[8] = {0, 2, 3, 0, 0, 6, 0, 8};
+
+void cblas_dsyr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                int incx, double *a, int lda)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYR  ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYR  ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(syr_map, dsyr_(&u, &n, &alpha, x, &incx, a, &lda));
+}
+
 /* ---- Level 1 (no layout argument) ---- */
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy)
 {

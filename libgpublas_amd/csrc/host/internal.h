@@ -64,7 +64,7 @@ void *gb_host_fn(const char *f77name);
 enum gb_route_kind {
     GB_RK_DGEMM, GB_RK_SGEMM, GB_RK_DSYRK, GB_RK_SSYRK, GB_RK_DSYR2K, GB_RK_SSYR2K,
     GB_RK_DSYMM, GB_RK_SSYMM, GB_RK_DTRSM, GB_RK_STRSM, GB_RK_DTRMM, GB_RK_STRMM,
-    GB_RK_DGEMV, GB_RK_DGER, GB_RK_DTRSV,
+    GB_RK_DGEMV, GB_RK_DGER, GB_RK_DTRSV, GB_RK_DSYMV, GB_RK_DSYR,
     GB_RK_STREAM, /* daxpy, dscal, dcopy, dswap, drot */
     GB_RK_REDUCE, /* ddot, dnrm2, dasum, idamax */
     GB_RK_COUNT

@@ -1,7 +1,7 @@
 /*
  * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
  * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_, and dscal_,
- * dcopy_, dswap_, drot_, dasum_.
+ * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_.
  *
  * The reference specifies these only through CBLAS wrappers it never builds
  * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
@@ -457,10 +457,111 @@ void dtrsv_(const char *uplo, const char *trans, const char *diag, const int *n,
     gb_leave();
 }
 
+/* ---- DSYMV: y := alpha*A*x + beta*y, A symmetric (uplo triangle)
+ * netlib DSYMV, the definition blas_level2/symv.cc:11-75 forwards to.
+ * INFO: 1 uplo, 2 n, 5 lda, 7 incx, 10 incy ---- */
+void dsymv_(const char *uplo, const int *n, const double *alpha, const double *a, const int *lda,
+            const double *x, const int *incx, const double *beta, double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*lda < MAXI(1, *n)) info = 5;
+    else if (*incx == 0) info = 7;
+    else if (*incy == 0) info = 10;
+    if (info) {
+        gb_xerbla("DSYMV ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || (*alpha == 0.0 && *beta == 1.0)) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n;
+    if (gb_host_route(GB_RK_DSYMV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, 2 * nn * nn,
+                      8.0 * (nn * nn + 3 * nn), a, x, y)) {
+        void (*f)(const char *, const int *, const double *, const double *, const int *,
+                  const double *, const int *, const double *, double *, const int *) =
+            gb_host_fn("dsymv_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, n, alpha, a, lda, x, incx, beta, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    if (*alpha != 0.0) {
+        gb_acquire_matrix(&ops[0], a, *lda, *n, *n, 8, GB_IN, s);
+        gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_IN, s);
+    }
+    gb_acquire_vector(&ops[2], y, *n, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);
+    gb_check(gpublas_dev_dsymv(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *lda,
+                               (const double *)ops[1].dev, *incx, *beta, (double *)ops[2].dev,
+                               *incy),
+             "dsymv launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DSYR: A := alpha*x*x**T + A on the uplo triangle
+ * netlib DSYR, the definition blas_level2/syr.cc:11-68 forwards to.
+ * INFO: 1 uplo, 2 n, 5 incx, 7 lda ---- */
+void dsyr_(const char *uplo, const int *n, const double *alpha, const double *x, const int *incx,
+           double *a, const int *lda)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*incx == 0) info = 5;
+    else if (*lda < MAXI(1, *n)) info = 7;
+    if (info) {
+        gb_xerbla("DSYR  ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || *alpha == 0.0) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n;
+    if (gb_host_route(GB_RK_DSYR, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, nn * nn,
+                      8.0 * (2 * nn * nn + nn), a, x, NULL)) {
+        void (*f)(const char *, const int *, const double *, const double *, const int *,
+                  double *, const int *) = gb_host_fn("dsyr_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, n, alpha, x, incx, a, lda);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    memset(ops, 0, sizeof ops);
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    /* the other triangle round-trips unchanged */
+    gb_acquire_matrix(&ops[1], a, *lda, *n, *n, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dsyr(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
+                              (double *)ops[1].dev, *lda),
+             "dsyr launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
 /* library-internal names for the crossover measurement (see level3.c) */
 #define GB_SELF(f) extern __typeof__(f) gb_self_##f __attribute__((alias(#f), visibility("hidden")));
 GB_SELF(dgemv_)
 GB_SELF(dger_)
 GB_SELF(dtrsv_)
+GB_SELF(dsymv_)
+GB_SELF(dsyr_)
 GB_SELF(daxpy_)
 GB_SELF(ddot_)

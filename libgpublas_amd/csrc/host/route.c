@@ -47,7 +47,8 @@ static const char *const k_names[GB_RK_COUNT] = {
     [GB_RK_DSYMM] = "dsymm",   [GB_RK_SSYMM] = "ssymm",   [GB_RK_DTRSM] = "dtrsm",
     [GB_RK_STRSM] = "strsm",   [GB_RK_DTRMM] = "dtrmm",   [GB_RK_STRMM] = "strmm",
     [GB_RK_DGEMV] = "dgemv",   [GB_RK_DGER] = "dger",     [GB_RK_DTRSV] = "dtrsv",
-    [GB_RK_STREAM] = "daxpy",  [GB_RK_REDUCE] = "ddot",
+    [GB_RK_DSYMV] = "dsymv",   [GB_RK_DSYR] = "dsyr",     [GB_RK_STREAM] = "daxpy",
+    [GB_RK_REDUCE] = "ddot",
 };
 
 struct xover {
@@ -80,6 +81,8 @@ static struct xover g_xover[GB_RK_COUNT] = {
     [GB_RK_DGEMV] = {INF_, INF_},
     [GB_RK_DGER] = {INF_, INF_},
     [GB_RK_DTRSV] = {INF_, INF_},
+    [GB_RK_DSYMV] = {INF_, INF_},
+    [GB_RK_DSYR] = {INF_, INF_},
     [GB_RK_STREAM] = {INF_, INF_},
     [GB_RK_REDUCE] = {INF_, INF_},
 };
@@ -191,6 +194,8 @@ extern __typeof__(strmm_) gb_self_strmm_;
 extern __typeof__(dgemv_) gb_self_dgemv_;
 extern __typeof__(dger_) gb_self_dger_;
 extern __typeof__(dtrsv_) gb_self_dtrsv_;
+extern __typeof__(dsymv_) gb_self_dsymv_;
+extern __typeof__(dsyr_) gb_self_dsyr_;
 extern __typeof__(daxpy_) gb_self_daxpy_;
 extern __typeof__(ddot_) gb_self_ddot_;
 
@@ -200,7 +205,8 @@ static const char *const k_f77[GB_RK_COUNT] = {
     [GB_RK_DSYMM] = "dsymm_",   [GB_RK_SSYMM] = "ssymm_",   [GB_RK_DTRSM] = "dtrsm_",
     [GB_RK_STRSM] = "strsm_",   [GB_RK_DTRMM] = "dtrmm_",   [GB_RK_STRMM] = "strmm_",
     [GB_RK_DGEMV] = "dgemv_",   [GB_RK_DGER] = "dger_",     [GB_RK_DTRSV] = "dtrsv_",
-    [GB_RK_STREAM] = "daxpy_",  [GB_RK_REDUCE] = "ddot_",
+    [GB_RK_DSYMV] = "dsymv_",   [GB_RK_DSYR] = "dsyr_",     [GB_RK_STREAM] = "daxpy_",
+    [GB_RK_REDUCE] = "ddot_",
 };
 
 static const void *self_fn(int k)
@@ -221,6 +227,8 @@ static const void *self_fn(int k)
     case GB_RK_DGEMV: return (const void *)gb_self_dgemv_;
     case GB_RK_DGER: return (const void *)gb_self_dger_;
     case GB_RK_DTRSV: return (const void *)gb_self_dtrsv_;
+    case GB_RK_DSYMV: return (const void *)gb_self_dsymv_;
+    case GB_RK_DSYR: return (const void *)gb_self_dsyr_;
     case GB_RK_STREAM: return (const void *)gb_self_daxpy_;
     default: return (const void *)gb_self_ddot_;
     }
@@ -232,7 +240,7 @@ static int is_single(int k)
            k == GB_RK_STRSM || k == GB_RK_STRMM;
 }
 
-static int level_of(int k) { return k <= GB_RK_STRMM ? 3 : k <= GB_RK_DTRSV ? 2 : 1; }
+static int level_of(int k) { return k <= GB_RK_STRMM ? 3 : k <= GB_RK_DSYR ? 2 : 1; }
 
 /* work model of a square problem of size n: the same formulas the entry
  * points apply to general shapes */
@@ -252,6 +260,8 @@ void gb_route_model(int k, double n, double *flops, double *bytes)
     case GB_RK_DGEMV: f = 2 * n * n, b = es * (n * n + 2 * n); break; /* A, x, y */
     case GB_RK_DGER: f = 2 * n * n, b = es * (2 * n * n + 2 * n); break; /* A in and out */
     case GB_RK_DTRSV: f = n * n, b = es * (n * n + 2 * n); break;
+    case GB_RK_DSYMV: f = 2 * n * n, b = es * (n * n + 3 * n); break; /* A, x, y in and out */
+    case GB_RK_DSYR: f = n * n, b = es * (2 * n * n + n); break;      /* A in and out, x */
     case GB_RK_STREAM: f = 2 * n, b = es * 3 * n; break; /* x, y in and out */
     default: f = 2 * n, b = es * 2 * n; break;           /* x, y */
     }
@@ -311,6 +321,11 @@ static void call_once(int k, int n, const void *fn, struct bufs *u)
                            const int *);
     typedef double (*dot_t)(const int *, const double *, const int *, const double *,
                             const int *);
+    typedef void (*symv_t)(const char *, const int *, const double *, const double *,
+                           const int *, const double *, const int *, const double *, double *,
+                           const int *);
+    typedef void (*syr_t)(const char *, const int *, const double *, const double *, const int *,
+                          double *, const int *);
     switch (k) {
     case GB_RK_DGEMM:
         ((gemm_d)fn)(&N, &N, &n, &n, &n, &d1, u->a, &n, u->b, &n, &d1, u->c, &n); break;
@@ -336,6 +351,8 @@ static void call_once(int k, int n, const void *fn, struct bufs *u)
         ((gemv_t)fn)(&N, &n, &n, &d1, u->a, &n, u->b, &one, &d1, u->c, &one); break;
     case GB_RK_DGER: ((ger_t)fn)(&n, &n, &dh, u->b, &one, u->c, &one, u->a, &n); break;
     case GB_RK_DTRSV: ((trsv_t)fn)(&L, &N, &N, &n, u->a, &n, u->b, &one); break;
+    case GB_RK_DSYMV: ((symv_t)fn)(&L, &n, &d1, u->a, &n, u->b, &one, &d1, u->c, &one); break;
+    case GB_RK_DSYR: ((syr_t)fn)(&L, &n, &dh, u->b, &one, u->a, &n); break;
     case GB_RK_STREAM: ((axpy_t)fn)(&n, &dh, u->a, &one, u->b, &one); break;
     default: {
         volatile double r = ((dot_t)fn)(&n, u->a, &one, u->b, &one);
@@ -355,7 +372,8 @@ static void reset(int k, int n, struct bufs *u)
         memcpy(u->b, u->b0, len * es);
     else if (k == GB_RK_DTRSV) memcpy(u->b, u->b0, (size_t)n * es);
     else if (k == GB_RK_STREAM) memcpy(u->b, u->b0, len * es);
-    else if (k == GB_RK_DGEMV) memcpy(u->c, u->c0, (size_t)n * es);
+    else if (k == GB_RK_DGEMV || k == GB_RK_DSYMV) memcpy(u->c, u->c0, (size_t)n * es);
+    else if (k == GB_RK_DSYR) return; /* grows by alpha*x*x' per call: bounded */
     else if (k != GB_RK_REDUCE) memcpy(u->c, u->c0, len * es);
 }
 

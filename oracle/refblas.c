@@ -133,6 +133,27 @@ int ora_trsv_info(char uplo, char trans, char diag, int n, int lda, int incx)
     return 0;
 }
 
+int ora_symv_info(char uplo, int n, int lda, int incx, int incy)
+{
+    /* netlib DSYMV, which blas_level2/symv.cc:11-75 forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (lda < imax(1, n)) return 5;
+    if (incx == 0) return 7;
+    if (incy == 0) return 10;
+    return 0;
+}
+
+int ora_syr_info(char uplo, int n, int incx, int lda)
+{
+    /* netlib DSYR, which blas_level2/syr.cc:11-68 forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 5;
+    if (lda < imax(1, n)) return 7;
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Level 3, generated for double and float                            */
 /* ------------------------------------------------------------------ */
@@ -845,5 +866,66 @@ void ora_fill_uniform(uint64_t seed, int64_t n, double *out)
         uint64_t z = ora_splitmix64_at(seed, (uint64_t)i);
         double u = (double)(z >> 11) * 0x1.0p-53; /* [0,1), exact */
         out[i] = 2.0 * u - 1.0;                   /* [-1,1), exact */
+    }
+}
+
+/* netlib DSYMV: y := alpha*A*x + beta*y with A symmetric, only the uplo
+ * triangle referenced (blas_level2/symv.cc:11-75 forwards here).  Column j
+ * adds temp1*A(i,j) to y(i) for the stored off-diagonal i and gathers
+ * A(i,j)*x(i) into temp2 for y(j), in netlib's loop order. */
+void ora_dsymv(char uplo, int n, double alpha, const double *A, int lda, const double *x,
+               int incx, double beta, double *y, int incy)
+{
+    if (ora_symv_info(uplo, n, lda, incx, incy)) return;
+    if (n == 0 || (alpha == 0 && beta == 1)) return;
+    int kx = incx > 0 ? 0 : -(n - 1) * incx;
+    int ky = incy > 0 ? 0 : -(n - 1) * incy;
+    if (beta != 1) {
+        int iy = ky;
+        for (int i = 0; i < n; ++i, iy += incy) y[iy] = beta == 0 ? 0.0 : beta * y[iy];
+    }
+    if (alpha == 0) return;
+    int jx = kx, jy = ky;
+    if (ora_lsame(uplo, 'U')) {
+        for (int j = 0; j < n; ++j, jx += incx, jy += incy) {
+            double temp1 = alpha * x[jx], temp2 = 0;
+            int ix = kx, iy = ky;
+            for (int i = 0; i < j; ++i, ix += incx, iy += incy) {
+                y[iy] += temp1 * A[IX(i, j, lda)];
+                temp2 += A[IX(i, j, lda)] * x[ix];
+            }
+            y[jy] += temp1 * A[IX(j, j, lda)] + alpha * temp2;
+        }
+    } else {
+        for (int j = 0; j < n; ++j, jx += incx, jy += incy) {
+            double temp1 = alpha * x[jx], temp2 = 0;
+            y[jy] += temp1 * A[IX(j, j, lda)];
+            int ix = jx, iy = jy;
+            for (int i = j + 1; i < n; ++i) {
+                ix += incx;
+                iy += incy;
+                y[iy] += temp1 * A[IX(i, j, lda)];
+                temp2 += A[IX(i, j, lda)] * x[ix];
+            }
+            y[jy] += alpha * temp2;
+        }
+    }
+}
+
+/* netlib DSYR: A := alpha*x*x' + A on the uplo triangle (blas_level2/
+ * syr.cc:11-68 forwards here); columns with x(j) == 0 are skipped. */
+void ora_dsyr(char uplo, int n, double alpha, const double *x, int incx, double *A, int lda)
+{
+    if (ora_syr_info(uplo, n, incx, lda)) return;
+    if (n == 0 || alpha == 0) return;
+    int kx = incx > 0 ? 0 : -(n - 1) * incx;
+    int upper = ora_lsame(uplo, 'U');
+    int jx = kx;
+    for (int j = 0; j < n; ++j, jx += incx) {
+        if (x[jx] == 0) continue;
+        double temp = alpha * x[jx];
+        int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+        int ix = kx + i0 * incx;
+        for (int i = i0; i < i1; ++i, ix += incx) A[IX(i, j, lda)] += x[ix] * temp;
     }
 }

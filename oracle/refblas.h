@@ -39,6 +39,8 @@ void ora_dtrmm(char side, char uplo, char ta, char diag, int m, int n, double al
 int ora_gemv_info(char trans, int m, int n, int lda, int incx, int incy);
 int ora_ger_info(int m, int n, int incx, int incy, int lda);
 int ora_trsv_info(char uplo, char trans, char diag, int n, int lda, int incx);
+int ora_symv_info(char uplo, int n, int lda, int incx, int incy);
+int ora_syr_info(char uplo, int n, int incx, int lda);
 
 /* ---- Level 3 ---- */
 void ora_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
@@ -72,6 +74,9 @@ void ora_dcopy(int n, const double *x, int incx, double *y, int incy);
 void ora_dswap(int n, double *x, int incx, double *y, int incy);
 void ora_drot(int n, double *x, int incx, double *y, int incy, double c, double s);
 double ora_dasum(int n, const double *x, int incx);
+void ora_dsymv(char uplo, int n, double alpha, const double *A, int lda, const double *x,
+               int incx, double beta, double *y, int incy);
+void ora_dsyr(char uplo, int n, double alpha, const double *x, int incx, double *A, int lda);
 void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);
 double ora_ddot(int n, const double *x, int incx, const double *y, int incy);
 double ora_dnrm2(int n, const double *x, int incx);

@@ -26,6 +26,10 @@ def load():
     L.ora_gemv_info.argtypes = [C, I, I, I, I, I]
     L.ora_ger_info.argtypes = [I, I, I, I, I]
     L.ora_trsv_info.argtypes = [C, C, C, I, I, I]
+    L.ora_symv_info.argtypes = [C, I, I, I, I]
+    L.ora_syr_info.argtypes = [C, I, I, I]
+    L.ora_dsymv.argtypes = [C, I, D, VP, I, VP, I, D, VP, I]
+    L.ora_dsyr.argtypes = [C, I, D, VP, I, VP, I]
     L.ora_dgemm.argtypes = [C, C, I, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]

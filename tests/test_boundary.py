@@ -284,3 +284,44 @@ def test_netlib_dblat3_error_exits_and_fallthrough_gemm():
     assert "DGEMM  PASSED THE TESTS OF ERROR-EXITS" in summ
     assert "DGEMM  PASSED THE COMPUTATIONAL TESTS" in summ
     assert "FAIL" not in summ and "**" not in summ
+
+
+def test_symv_syr_xerbla_and_fall_through(ora):
+    """DSYMV / DSYR: netlib INFO codes, the CBLAS positions they map to, and
+    the small-problem fall-through (n < cutoff) bit-identical to the host
+    BLAS -- row-major CBLAS calls take the other triangle."""
+    lib = gb.library()
+    d = np.zeros(64)
+    gb.dsymv("X", 2, 1.0, d, 2, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DSYMV ", ora.ora_symv_info(b"X", 2, 2, 1, 1))
+    gb.dsymv("L", 3, 1.0, d, 2, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DSYMV ", 5)
+    gb.dsymv("U", 2, 1.0, d, 2, d, 1, 0.0, d, 0)
+    assert gb.last_xerbla() == ("DSYMV ", 10)
+    gb.dsyr("L", 2, 1.0, d, 0, d, 2)
+    assert gb.last_xerbla() == ("DSYR  ", ora.ora_syr_info(b"L", 2, 0, 2))
+    gb.dsyr("U", 3, 1.0, d, 1, d, 2)
+    assert gb.last_xerbla() == ("DSYR  ", 7)
+    lib.cblas_dsymv(102, 121, 3, 1.0, p(d), 2, p(d), 1, 0.0, p(d), 1)
+    assert gb.last_xerbla() == ("DSYMV ", 6)  # lda is CBLAS parameter 6
+    lib.cblas_dsyr(102, 999, 2, 1.0, p(d), 1, p(d), 2)
+    assert gb.last_xerbla() == ("DSYR  ", 2)
+
+    gb.set_cutoff(512)
+    n = 50
+    A = uniform(61, (n, n))
+    x, y0 = uniform(62, (n,)), uniform(63, (n,))
+    gb.reset_stats()
+    y = y0.copy()
+    gb.dsymv("U", n, 0.5, A, n, x, 1, 2.0, y, 1)
+    assert np.array_equal(y, sb.dsymv(0.5, A, x, beta=2.0, y=y0.copy(), lower=0))
+    R = A.copy(order="F")
+    gb.dsyr("L", n, -0.75, x, 1, R, n)
+    assert np.array_equal(R, sb.dsyr(-0.75, x, lower=1, a=A.copy(order="F")))
+    assert gb.stats()["fallthrough"] == 2
+    # row-major: the memory of A read as row-major is A.T column-major; upper
+    # row-major = lower column-major of the same bytes
+    y = np.zeros(n)
+    lib.cblas_dsymv(101, 121, n, 1.0, p(A), n, p(x), 1, 0.0, p(y), 1)  # 121 = CblasUpper
+    S = np.triu(A.T) + np.triu(A.T, 1).T  # the row-major upper triangle, symmetrised
+    np.testing.assert_allclose(y, S @ x, rtol=1e-13, atol=1e-14)

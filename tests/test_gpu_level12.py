@@ -249,3 +249,91 @@ def test_dtrsv(gpu, ora, uplo, trans, diag, n, incx):
     gpu.dtrsv(uplo, trans, diag, n, A, n, x, incx)
     ora.ora_dtrsv(c(uplo), c(trans), c(diag), n, p(A), n, p(r), incx)
     np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+
+
+def _other_triangle(n, uplo):
+    return np.triu_indices(n, 1) if uplo == "L" else np.tril_indices(n, -1)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (2, 1, 1), (127, 1, 1), (128, 1, 1),
+                                         (129, 2, -3), (1000, -1, 2), (4096, 1, 1), (4099, 1, 1)])
+def test_dsymv(gpu, ora, uplo, n, incx, incy):
+    """Only the uplo triangle is read (the other holds NaN); bound: 2n eps
+    times the sum of |terms| per element.  Even leading dimensions take the
+    16-B load path on interior tiles, odd ones (n = 127, 129, 4099) the
+    per-element path everywhere."""
+    A = uniform(31, (n, n))
+    A[_other_triangle(n, uplo)] = np.nan
+    x = _strided(uniform(32, (n,)), incx)
+    y0 = _strided(uniform(33, (n,)), incy)
+    y, r = y0.copy(), y0.copy()
+    gpu.dsymv(uplo, n, 0.75, A, n, x, incx, -0.5, y, incy)
+    ora.ora_dsymv(c(uplo), n, 0.75, p(A), n, p(x), incx, -0.5, p(r), incy)
+    S = np.where(np.isnan(A), 0.0, A)
+    S = S + S.T - np.diag(np.diag(S))
+    xs = x[:: abs(incx)][:n] if incx > 0 else x[:: abs(incx)][:n][::-1]
+    ys = y0[:: abs(incy)][:n] if incy > 0 else y0[:: abs(incy)][:n][::-1]
+    mag = 0.75 * (np.abs(S) @ np.abs(xs)) + 0.5 * np.abs(ys)
+    got = y[:: abs(incy)][:n] if incy > 0 else y[:: abs(incy)][:n][::-1]
+    ref = r[:: abs(incy)][:n] if incy > 0 else r[:: abs(incy)][:n][::-1]
+    assert np.all(np.abs(got - ref) <= 2 * n * EPS * mag + 1e-300)
+    gaps = np.ones(len(y), bool)
+    gaps[:: abs(incy)][:n] = False
+    assert np.array_equal(y[gaps], y0[gaps])  # stride gaps untouched
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_dsymv_integer_exact(gpu, ora, uplo):
+    """Integer data with exactly representable sums: bit-exact, any order."""
+    n = 1537
+    rng = np.random.default_rng(34)
+    A = np.asfortranarray(rng.integers(-8, 9, (n, n)).astype(float))
+    x = rng.integers(-8, 9, n).astype(float)
+    y0 = rng.integers(-8, 9, n).astype(float)
+    y, r = y0.copy(), y0.copy()
+    gpu.dsymv(uplo, n, 2.0, A, n, x, 1, 3.0, y, 1)
+    ora.ora_dsymv(c(uplo), n, 2.0, p(A), n, p(x), 1, 3.0, p(r), 1)
+    assert np.array_equal(y, r)
+
+
+def test_dsymv_alpha_beta_special(gpu, ora):
+    n = 300
+    A = uniform(35, (n, n))
+    x = uniform(36, (n,))
+    y = np.full(n, np.nan)
+    gpu.dsymv("L", n, 1.0, A, n, x, 1, 0.0, y, 1)  # beta = 0: y overwritten
+    r = np.full(n, np.nan)
+    ora.ora_dsymv(b"L", n, 1.0, p(A), n, p(x), 1, 0.0, p(r), 1)
+    assert np.all(np.isfinite(y))
+    np.testing.assert_allclose(y, r, rtol=1e-12, atol=1e-12)
+    y0 = uniform(37, (n,))
+    y = y0.copy()
+    gpu.dsymv("U", n, 0.0, A, n, x, 1, -2.0, y, 1)  # alpha = 0: scale only
+    assert np.array_equal(y, -2.0 * y0)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,incx", [(1, 1), (255, 1), (256, 2), (1000, -3), (3001, 1)])
+def test_dsyr(gpu, ora, uplo, n, incx):
+    A0 = uniform(38, (n, n))
+    x = _strided(uniform(39, (n,)), incx)
+    x[0] = 0.0  # one zero entry: its column is left as it is
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dsyr(uplo, n, -0.625, x, incx, A, n)
+    ora.ora_dsyr(c(uplo), n, -0.625, p(x), incx, p(R), n)
+    other = _other_triangle(n, uplo)
+    assert np.array_equal(A[other], A0[other])  # never touched
+    # one fused rounding on the GPU against netlib's two
+    np.testing.assert_allclose(A, R, rtol=2 * EPS, atol=2 * EPS)
+
+
+def test_dsyr_integer_exact(gpu, ora):
+    n = 777
+    rng = np.random.default_rng(40)
+    A0 = np.asfortranarray(rng.integers(-50, 51, (n, n)).astype(float))
+    x = rng.integers(-9, 10, n).astype(float)
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dsyr("U", n, 3.0, x, 1, A, n)
+    ora.ora_dsyr(b"U", n, 3.0, p(x), 1, p(R), n)
+    assert np.array_equal(A, R)

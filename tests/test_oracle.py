@@ -384,3 +384,73 @@ def test_level1_more_vs_host_blas(L, incx, incy):
         L.ora_dscal(n, -1.5, p(X), incx)  # netlib: no-op for incx <= 0
         assert np.array_equal(X, x0)
         assert L.ora_dasum(n, p(x0), incx) == 0.0
+
+
+def _sym_from(A, uplo):
+    """Full symmetric matrix from the uplo triangle of A (the other ignored)."""
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    return T + T.T - np.diag(np.diag(A))
+
+
+def _strided_get(buf, n, inc):
+    v = buf[:: abs(inc)][:n]
+    return v if inc > 0 else v[::-1]
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1), (-3, 2)])
+def test_dsymv_vs_host_blas_and_closed_form(L, uplo, incx, incy):
+    """netlib DSYMV restatement: scipy's OpenBLAS on unit strides, and the
+    closed form alpha*S@x + beta*y with S built from the uplo triangle while
+    the other triangle holds NaN (it must never be read)."""
+    n = 13
+    A = uniform(91, (n, n))
+    other = np.triu_indices(n, 1) if uplo == "L" else np.tril_indices(n, -1)
+    An = A.copy(order="F")
+    An[other] = np.nan
+    x = uniform(92, (1 + (n - 1) * abs(incx),))
+    y0 = uniform(93, (1 + (n - 1) * abs(incy),))
+    R = y0.copy()
+    L.ora_dsymv(c(uplo), n, 0.7, p(An), n, p(x), incx, -0.3, p(R), incy)
+    want = 0.7 * (_sym_from(A, uplo) @ _strided_get(x, n, incx)) - 0.3 * _strided_get(y0, n, incy)
+    np.testing.assert_allclose(_strided_get(R, n, incy), want, rtol=1e-13, atol=1e-14)
+    if incx > 0 and incy > 0:
+        H = sb.dsymv(0.7, A, x, beta=-0.3, y=y0.copy(), incx=incx, incy=incy,
+                     lower=int(uplo == "L"))
+        np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-15)
+    # beta = 0 overwrites y (NaN in y must not survive); alpha = 0 only scales
+    Y = np.full_like(y0, np.nan)
+    L.ora_dsymv(c(uplo), n, 1.0, p(An), n, p(x), incx, 0.0, p(Y), incy)
+    assert np.all(np.isfinite(_strided_get(Y, n, incy)))
+    Y = y0.copy()
+    L.ora_dsymv(c(uplo), n, 0.0, p(An), n, p(x), incx, 2.0, p(Y), incy)
+    assert np.array_equal(_strided_get(Y, n, incy), 2.0 * _strided_get(y0, n, incy))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("incx", [1, 2, -2])
+def test_dsyr_vs_host_blas_and_closed_form(L, uplo, incx):
+    n = 11
+    A0 = uniform(94, (n, n))
+    x = uniform(95, (1 + (n - 1) * abs(incx),))
+    x[0] = 0.0  # a zero entry: netlib skips its column
+    R = A0.copy(order="F")
+    L.ora_dsyr(c(uplo), n, -1.25, p(x), incx, p(R), n)
+    xs = _strided_get(x, n, incx)
+    full = A0 - 1.25 * np.outer(xs, xs)
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    np.testing.assert_allclose(R[tri], full[tri], rtol=1e-14, atol=1e-15)
+    assert np.array_equal(R[~tri], A0[~tri])  # the other triangle is untouched
+    if incx == 1:
+        H = sb.dsyr(-1.25, x, lower=int(uplo == "L"), a=A0.copy(order="F"))
+        # OpenBLAS fuses the update (one rounding), netlib rounds twice
+        np.testing.assert_allclose(R[tri], H[tri], rtol=1e-15, atol=1e-16)
+
+
+def test_symv_syr_info_codes(L):
+    assert [L.ora_symv_info(b"X", 2, 2, 1, 1), L.ora_symv_info(b"L", -1, 2, 1, 1),
+            L.ora_symv_info(b"U", 3, 2, 1, 1), L.ora_symv_info(b"L", 2, 2, 0, 1),
+            L.ora_symv_info(b"L", 2, 2, 1, 0), L.ora_symv_info(b"l", 2, 2, -1, -1)] == [1, 2, 5, 7, 10, 0]
+    assert [L.ora_syr_info(b"X", 2, 1, 2), L.ora_syr_info(b"U", -1, 1, 2),
+            L.ora_syr_info(b"U", 2, 0, 2), L.ora_syr_info(b"L", 3, 1, 2),
+            L.ora_syr_info(b"u", 3, -2, 3)] == [1, 2, 5, 7, 0]
