@@ -96,6 +96,9 @@ static void print_help(void)
            "                       from <file>, measured and written there if absent\n"
            "  stats[=<file>]       write hits/misses at exit (statistics.csv)\n"
            "  trace=<file>         log allocations (T/U) and BLAS operand uses (C)\n"
+           "  objtrack_out=<file>  at exit, write an object-tracking file naming the\n"
+           "                       call sites and ordinals of allocations that were BLAS\n"
+           "                       operands (capture with heuristic=true)\n"
            "  verbose              log decisions to stderr\n");
 }
 
@@ -160,6 +163,8 @@ int gb_parse_options(const char *s)
         else if (!strcmp(opt, "stats")) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "statistics.csv");
         else if (!strncmp(opt, "stats=", 6)) snprintf(gb_opt.stats_file, sizeof gb_opt.stats_file, "%s", opt + 6);
         else if (!strncmp(opt, "trace=", 6)) snprintf(gb_opt.trace_file, sizeof gb_opt.trace_file, "%s", opt + 6);
+        else if (!strncmp(opt, "objtrack_out=", 13))
+            snprintf(gb_opt.objtrack_out, sizeof gb_opt.objtrack_out, "%s", opt + 13);
         else if (!strcmp(opt, "cutoff=auto")) gb_opt.cutoff = GB_CUTOFF_AUTO;
         else if (!strncmp(opt, "cutoff=", 7)) {
             char *end;
@@ -228,6 +233,7 @@ __attribute__((destructor)) static void gb_fini(void)
     gb_enter();
     gb_tracking = 0;
     gb_serve_stop();
+    if (gb_opt.objtrack_out[0]) gb_objtrack_write();
     if (gb_opt.stats_file[0]) {
         int fd = open(gb_opt.stats_file, O_WRONLY | O_CREAT | O_TRUNC, 0644);
         if (fd >= 0) {

@@ -44,6 +44,7 @@ struct gb_options {
     int verbose;
     char stats_file[256];    /* "" = no statistics file at exit */
     char trace_file[256];    /* "" = no allocation / call trace */
+    char objtrack_out[256];  /* "" = no object-tracking file written at exit */
 };
 extern struct gb_options gb_opt;
 int gb_parse_options(const char *s);
@@ -81,6 +82,12 @@ extern GB_TLS int gb_route_gpu; /* set by the crossover measurement: force the G
 void gb_require_device(void);
 
 /* ---- tracked objects (tracker.c) ---- */
+/* ---- object-tracking capture (objtrack.c): allocations used as BLAS
+ * operands, written at exit as an object-tracking file ---- */
+void gb_objtrack_note(uint64_t nth, void *callsite, size_t size);
+void gb_objtrack_write(void);
+void gb_obj_foreach_used(void (*fn)(uint64_t nth, void *callsite, size_t size));
+
 enum gb_state { ST_HOST = 0, ST_SHARED = 1, ST_DEVICE = 2 };
 #define GB_RD_SLOTS 32 /* reader streams remembered per object */
 struct gb_obj {

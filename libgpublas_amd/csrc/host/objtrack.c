@@ -4,7 +4,9 @@
  *
  * Reference counterparts: lib/oracle.c:14-73 (file loader and per-ordinal
  * bitmap), lib/obj_tracker.c:226-242 (heuristic switch) and the trace records
- * of the standalone tracer (lib/obj_tracker.c:426-483).
+ * of the standalone tracer (lib/obj_tracker.c:426-483).  The capture at the
+ * end (objtrack_out=<file>) writes the file directly from a run, which the
+ * reference leaves to scripts/objtrace.sh:22-31 and scripts/analyze_trace.py.
  *
  * File format, one decision per line (the reference's "%1[HD] #%lu" format,
  * oracle.c:41, plus call-site keys):
@@ -213,4 +215,89 @@ void gb_trace_call(const char *routine, const void *p)
     struct gb_obj *o = gb_obj_find(p);
     if (!o) return;
     gb_log(fd, "C #%" PRIu64 " [%p] fun=[%s]\n", o->nth, p, routine);
+}
+
+/* ---- capture: allocations that were BLAS operands -> object-tracking file
+ * Records come from freed objects (obj_free) and, at exit, from the live
+ * ones.  The file holds one call-site rule per site (stable across runs and
+ * library versions) and the ordinals (valid for an identical run), each with
+ * the counts behind it as a comment.  Storage is mmap-backed: this runs
+ * inside the allocator hooks. ---- */
+struct use_rec {
+    uint64_t nth;
+    void *site;
+    size_t size;
+};
+static struct use_rec *g_use = NULL;
+static size_t g_use_n = 0, g_use_cap = 0;
+static pthread_mutex_t g_use_mu = PTHREAD_MUTEX_INITIALIZER;
+
+void gb_objtrack_note(uint64_t nth, void *callsite, size_t size)
+{
+    if (nth == UINT64_MAX) return; /* explicit gpublas_malloc: no ordinal, always tracked */
+    pthread_mutex_lock(&g_use_mu);
+    if (g_use_n == g_use_cap) {
+        size_t nc = g_use_cap ? 2 * g_use_cap : 4096;
+        struct use_rec *nt = map_zeroed(nc * sizeof *nt);
+        if (!nt) {
+            pthread_mutex_unlock(&g_use_mu);
+            return;
+        }
+        if (g_use) {
+            memcpy(nt, g_use, g_use_n * sizeof *nt);
+            munmap(g_use, g_use_cap * sizeof *nt);
+        }
+        g_use = nt;
+        g_use_cap = nc;
+    }
+    g_use[g_use_n++] = (struct use_rec){nth, callsite, size};
+    pthread_mutex_unlock(&g_use_mu);
+}
+
+static int by_site_then_nth(const void *a, const void *b)
+{
+    const struct use_rec *x = a, *y = b;
+    if (x->site != y->site) return (uintptr_t)x->site < (uintptr_t)y->site ? -1 : 1;
+    return x->nth < y->nth ? -1 : x->nth > y->nth;
+}
+
+static int by_nth(const void *a, const void *b)
+{
+    const struct use_rec *x = a, *y = b;
+    return x->nth < y->nth ? -1 : x->nth > y->nth;
+}
+
+void gb_objtrack_write(void)
+{
+    gb_obj_foreach_used(gb_objtrack_note);
+    int fd = open(gb_opt.objtrack_out, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) {
+        gb_log(2, "libgpublas: cannot write object-tracking file '%s'\n", gb_opt.objtrack_out);
+        return;
+    }
+    pthread_mutex_lock(&g_use_mu);
+    gb_log(fd, "# object-tracking file written by libgpublas (objtrack_out) for %s:\n"
+               "# allocations that were BLAS operands.  Load it with heuristic=objtrack:<file>.\n"
+               "# Call-site rules (D @module+0xoffset) hold across runs; the ordinals (D #n)\n"
+               "# only for an identical run.\n",
+           program_invocation_short_name);
+    qsort(g_use, g_use_n, sizeof *g_use, by_site_then_nth);
+    for (size_t k = 0; k < g_use_n;) {
+        size_t e = k;
+        uint64_t bytes = 0;
+        while (e < g_use_n && g_use[e].site == g_use[k].site) bytes += g_use[e++].size;
+        char cs[192];
+        gb_callsite_name(g_use[k].site, cs, sizeof cs);
+        gb_log(fd, "D @%s # %zu allocation(s), %" PRIu64 " bytes\n", cs, e - k, bytes);
+        k = e;
+    }
+    qsort(g_use, g_use_n, sizeof *g_use, by_nth);
+    for (size_t k = 0; k < g_use_n; ++k) {
+        if (k && g_use[k].nth == g_use[k - 1].nth) continue;
+        char cs[192];
+        gb_callsite_name(g_use[k].site, cs, sizeof cs);
+        gb_log(fd, "D #%" PRIu64 " site=[%s] reqsize=[%zu]\n", g_use[k].nth, cs, g_use[k].size);
+    }
+    pthread_mutex_unlock(&g_use_mu);
+    close(fd);
 }

@@ -116,6 +116,14 @@ struct gb_obj *gb_obj_find(const void *p)
 }
 
 int gb_tracked_count(void) { return (int)g_tab_n; }
+
+void gb_obj_foreach_used(void (*fn)(uint64_t nth, void *callsite, size_t size))
+{
+    pthread_rwlock_rdlock(&g_tab_lock);
+    for (size_t k = 0; k < g_tab_n; ++k)
+        if (g_tab[k]->used) fn(g_tab[k]->nth, g_tab[k]->callsite, g_tab[k]->size);
+    pthread_rwlock_unlock(&g_tab_lock);
+}
 bool gb_ptr_tracked(const void *p) { return gb_obj_find(p) != NULL; }
 
 size_t gb_obj_usable_size(const void *p)
@@ -272,6 +280,7 @@ size_t gb_evict_idle(size_t want, struct gb_obj *const *keep, int nkeep)
 static void obj_free(struct gb_obj *o)
 {
     gb_trace_alloc('U', o->nth, o->callsite, o->size, o->host);
+    if (o->used && gb_opt.objtrack_out[0]) gb_objtrack_note(o->nth, o->callsite, o->size);
     gb_obj_quiesce(o); /* a kernel launched by a call that returned early may still use it */
     munmap(o->host, o->map_size);
     munmap(o->alias, o->map_size);

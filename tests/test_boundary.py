@@ -325,3 +325,20 @@ def test_symv_syr_xerbla_and_fall_through(ora):
     lib.cblas_dsymv(101, 121, n, 1.0, p(A), n, p(x), 1, 0.0, p(y), 1)  # 121 = CblasUpper
     S = np.triu(A.T) + np.triu(A.T, 1).T  # the row-major upper triangle, symmetrised
     np.testing.assert_allclose(y, S @ x, rtol=1e-13, atol=1e-14)
+
+
+def test_objtrack_capture_without_uses_writes_an_empty_file():
+    """objtrack_out writes the object-tracking file at exit; a run that makes
+    no BLAS call leaves it with its header and no rule."""
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "cap.txt")
+        env = dict(os.environ)
+        env["LD_PRELOAD"] = gb.PRELOAD_PATH
+        env["GPUBLAS_OPTIONS"] = f"heuristic=true;min_track=0;objtrack_out={out}"
+        r = subprocess.run([gb.CHAIN_BENCH, "--n", "64", "--iters", "0", "--warmup", "0"],
+                           env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        text = open(out).read()
+    assert text.startswith("# object-tracking file written by libgpublas")
+    assert "chain_bench" in text.splitlines()[0]
+    assert [ln for ln in text.splitlines() if ln and not ln.startswith("#")] == []
