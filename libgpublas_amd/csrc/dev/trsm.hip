@@ -132,6 +132,112 @@ __global__ __launch_bounds__(256) void trsm_leaf(int nb, int n, T alpha, const T
     }
 }
 
+// Solve of an nb x nb triangle (nb <= 64 * R) for n right-hand sides in one
+// launch: the base of the recursion, which otherwise solves four 64 x 64
+// leaves and three small GEMM updates per 256 rows, each a launch of its own
+// with too few tiles to fill the chip.  Lane i of a wave owns rows i, i + 64,
+// ... (R groups) of LPW right-hand sides.  The triangle goes through LDS one
+// 64-column panel at a time (the panel's rows from its diagonal block down
+// for a lower triangle, up for an upper one); within a panel, step k takes x_k
+// from lane k of the panel's row group (v_readlane, reciprocal plus one Newton
+// correction as in trsm_leaf) and updates the panel's later rows and every
+// row group beyond the panel.  Same element order per right-hand side as the
+// leaves and GEMM updates of the recursion it replaces, up to the order of
+// the sums over k within a 64-column panel.
+constexpr int BIG_R = 4;                  // row groups: orders up to 256
+// Right-hand sides per wave in the base kernel.  Every workgroup streams the
+// whole triangle through LDS (133 KB: one workgroup per CU), so fewer, wider
+// workgroups pay for it fewer times: at 8192 x 4096 the base launches took
+// 98 us (2 per wave, 512 workgroups), 68 us (4, 256) and 104 us (8, 128).
+constexpr int TRSM_LPWB = 4;
+template <typename T, bool LOWER, bool NOUNIT, int LPWB>
+__global__ __launch_bounds__(256) void trsm_leaf_big(int nb, int n, T alpha,
+                                                     const T *__restrict__ A, long lda,
+                                                     int transN, T *__restrict__ B, long sstride,
+                                                     long rstride)
+{
+    __shared__ T Ts[NB * BIG_R][NB + 1];  // the current panel: Ts[row][k]
+    __shared__ T rd[NB];                  // 1 / op(T)(kg, kg) of the panel
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jb = (blockIdx.x * 4 + wave) * LPWB;
+    T b[LPWB][BIG_R];
+#pragma unroll
+    for (int c = 0; c < LPWB; ++c)
+#pragma unroll
+        for (int g = 0; g < BIG_R; ++g) {
+            const int j = jb + c, r = NB * g + lane;
+            b[c][g] = (j < n && r < nb) ? alpha * B[r * sstride + j * rstride] : T(0);
+        }
+#pragma unroll
+    for (int pp = 0; pp < BIG_R; ++pp) {
+        const int p = LOWER ? pp : BIG_R - 1 - pp;
+        if (NB * p >= nb) continue;  // uniform: panels past the order are empty
+        // rows the panel feeds: lower from NB*p down, upper from NB*p+NB-1 up
+        const int r_lo = LOWER ? NB * p : 0, r_hi = LOWER ? NB * BIG_R : NB * p + NB;
+        __syncthreads();  // the previous panel's readers are done
+        for (int e0 = r_lo * NB; e0 < r_hi * NB; e0 += 256 * 16) {
+            T tv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + threadIdx.x + 256 * u;
+                // consecutive threads along the contiguous index of op(T)
+                const int r = transN ? r_lo + (e - r_lo * NB) % (r_hi - r_lo)
+                                     : r_lo + (e - r_lo * NB) / NB;
+                const int kk = transN ? (e - r_lo * NB) / (r_hi - r_lo) : (e - r_lo * NB) % NB;
+                const int kg = NB * p + kk;
+                const bool in = e < r_hi * NB && r < nb && kg < nb && (LOWER ? r >= kg : r <= kg);
+                const long off = transN ? r + (long)kg * lda : kg + (long)r * lda;
+                const T v = A[in ? off : 0];  // always a valid address: no branch per load
+                tv[u] = in ? v : (r == kg ? T(1) : T(0));
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + threadIdx.x + 256 * u;
+                if (e < r_hi * NB) {
+                    const int r = transN ? r_lo + (e - r_lo * NB) % (r_hi - r_lo)
+                                         : r_lo + (e - r_lo * NB) / NB;
+                    const int kk =
+                        transN ? (e - r_lo * NB) / (r_hi - r_lo) : (e - r_lo * NB) % NB;
+                    Ts[r][kk] = tv[u];
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < NB) rd[threadIdx.x] = T(1) / Ts[NB * p + threadIdx.x][threadIdx.x];
+        __syncthreads();
+        for (int st = 0; st < NB; ++st) {
+            const int k = LOWER ? st : NB - 1 - st;
+            const T t = Ts[NB * p + lane][k];
+            const T r = rd[k], d = Ts[NB * p + k][k];
+            const bool below = LOWER ? lane > k : lane < k;
+            T tg[BIG_R];
+#pragma unroll
+            for (int g = 0; g < BIG_R; ++g)
+                if (LOWER ? g > p : g < p) tg[g] = Ts[NB * g + lane][k];
+#pragma unroll
+            for (int c = 0; c < LPWB; ++c) {
+                const T bk = lane_value(b[c][p], k);
+                T x = bk;
+                if (NOUNIT) {
+                    const T x0 = bk * r;
+                    x = fma(fma(-x0, d, bk), r, x0);
+                }
+                b[c][p] = lane == k ? x : below ? fma(-x, t, b[c][p]) : b[c][p];
+#pragma unroll
+                for (int g = 0; g < BIG_R; ++g)
+                    if (LOWER ? g > p : g < p) b[c][g] = fma(-x, tg[g], b[c][g]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < LPWB; ++c)
+#pragma unroll
+        for (int g = 0; g < BIG_R; ++g) {
+            const int j = jb + c, r = NB * g + lane;
+            if (j < n && r < nb) B[r * sstride + j * rstride] = b[c][g];
+        }
+}
+
 template <typename T>
 __global__ void zero_kernel(int m, int n, T *B, long ldb)
 {
@@ -175,6 +281,26 @@ struct Solve {
         return (int)hipGetLastError();
     }
 
+    // orders NB+1 .. NB*BIG_R in one launch (trsm_leaf_big)
+    template <int LPWB>
+    int leaf_big(int m, int n, T alpha, const T *a, T *b)
+    {
+        const bool lower = left ? effLower : !effLower;
+        const int tn = left ? transN : !transN;
+        const int nrhs = left ? n : m, order = left ? m : n;
+        const long ss = left ? 1 : ldb, rs = left ? ldb : 1;
+        const unsigned blocks = (unsigned)ceil_div(nrhs, 4 * LPWB);
+        auto k = lower ? (nounit ? trsm_leaf_big<T, true, true, LPWB>
+                                 : trsm_leaf_big<T, true, false, LPWB>)
+                       : (nounit ? trsm_leaf_big<T, false, true, LPWB>
+                                 : trsm_leaf_big<T, false, false, LPWB>);
+        PROF_BEGIN(s, sizeof(T) == 8 ? "dtrsm_base" : "strsm_base");
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, order, nrhs, alpha, a, (long)lda, tn,
+                           b, ss, rs);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+
     // Solve on the sub-problem whose triangle starts at a (order `t`), with
     // right-hand sides at b (m x n, m = t for left, n = t for right).
     int rec(int m, int n, T alpha, const T *a, T *b)
@@ -182,6 +308,7 @@ struct Solve {
         int err;
         const int t = left ? m : n;
         if (t <= NB) return leaf(m, n, alpha, a, b);
+        if (t <= NB * BIG_R) return leaf_big<TRSM_LPWB>(m, n, alpha, a, b);
         const int t1 = split(t), t2 = t - t1;
         const T *a11 = a, *a22 = a + t1 + (long)t1 * lda;
         const T *a21 = a + t1, *a12 = a + (long)t1 * lda;  // stored blocks

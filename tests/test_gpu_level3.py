@@ -382,8 +382,10 @@ def _tri_matrix(seed, n, uplo, diag_boost=True):
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("diag", ["N", "U"])
-@pytest.mark.parametrize("m,n", [(7, 5), (100, 70), (200, 150)])
+@pytest.mark.parametrize("m,n", [(7, 5), (100, 70), (200, 150), (300, 260)])
 def test_dtrsm(gpu, ora, side, uplo, ta, diag, m, n):
+    """Orders up to 64 go to the 64 x 64 leaf, up to 256 to the one-launch
+    base solve, larger ones recurse into those with GEMM updates."""
     na = m if side == "L" else n
     A = _tri_matrix(51, na, uplo, diag_boost=(diag == "N"))
     if diag == "U":
