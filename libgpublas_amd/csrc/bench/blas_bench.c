@@ -8,6 +8,7 @@
  *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
  *   blas_bench --lib ... --op syr2k --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trmm --m M --n N --side L --uplo L --ta N --diag N
+ *   blas_bench --lib ... --op gemv --m M --n N --ta N (DGEMV, N or T)
  *   blas_bench --lib ... --op symv --n N --uplo L   (DSYMV, bytes = the triangle once)
  *   blas_bench --lib ... --op syr --n N --uplo L    (DSYR, the triangle read and written)
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
@@ -199,6 +200,10 @@ int main(int argc, char **argv)
     typedef void (*syr_t)(const char *, const int *, const double *, const double *, const int *,
                           double *, const int *);
     symv_t symv = NULL;
+    typedef void (*gemv_t)(const char *, const int *, const int *, const double *, const double *,
+                           const int *, const double *, const int *, const double *, double *,
+                           const int *);
+    gemv_t gemv = NULL;
     syr_t syr = NULL;
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
@@ -257,6 +262,13 @@ int main(int argc, char **argv)
         B = operand((size_t)m * n, prec, seed + 1);
         C = operand((size_t)m * n, prec, seed + 2);
         flops = 2.0 * m * n * na;
+    } else if (!strcmp(op, "gemv")) {
+        gemv = (gemv_t)sym("dgemv_");
+        A = operand((size_t)m * n, 'd', seed);
+        B = operand((size_t)(ta == 'N' ? n : m), 'd', seed + 1);
+        C = operand((size_t)(ta == 'N' ? m : n), 'd', seed + 2);
+        bytes = 8.0 * ((double)m * n + m + n);
+        flops = 2.0 * m * n;
     } else if (!strcmp(op, "symv")) {
         symv = (symv_t)sym("dsymv_");
         A = operand((size_t)n * n, 'd', seed);
@@ -292,7 +304,12 @@ int main(int argc, char **argv)
         else if (symm) symm(&side, &uplo, &m, &n, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else if (syr2k) syr2k(&uplo, &trans, &n, &k, alpha, A, &lda, B, &ldb, beta, C, &ldc);
         else if (trmm) trmm(&side, &uplo, &ta, &diag, &m, &n, alpha, A, &lda, B, &ldb);
-        else if (symv) {
+        else if (gemv) {
+            const double a = 1.0, b = 0.5;
+            const int one = 1;
+            gemv(&ta, &m, &n, &a, (const double *)A, &m, (const double *)B, &one, &b, (double *)C,
+                 &one);
+        } else if (symv) {
             const double a = 1.0, b = 0.5;
             const int one = 1;
             symv(&uplo, &n, &a, (const double *)A, &n, (const double *)B, &one, &b, (double *)C,
