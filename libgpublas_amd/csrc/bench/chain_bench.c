@@ -20,6 +20,12 @@
 #include <string.h>
 #include <time.h>
 
+static int cmp_double(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : x > y;
+}
+
 typedef void (*gemv_t)(const char *, const int *, const int *, const double *, const double *,
                        const int *, const double *, const int *, const double *, double *,
                        const int *);
@@ -128,17 +134,25 @@ int main(int argc, char **argv)
     const int inc = 1;
     double d = 0, r = 0;
     int im = 0;
-    double t0 = 0;
+    double t0 = 0, tp = 0;
+    /* per-iteration times: every iteration ends in idamax, whose result the
+     * caller holds, so the iteration's work is complete at its end */
+    double *it_s = iters > 0 ? malloc((size_t)iters * sizeof(double)) : NULL;
     for (int it = 0; it < warmup + iters; ++it) {
         if (it == warmup) {
             if (prof_reset) prof_reset();
-            t0 = now();
+            t0 = tp = now();
         }
         gemv("N", &n, &n, &one, A, &n, x, &inc, &zero, y, &inc);
         axpy(&n, &a3, y, &inc, z, &inc);
         d = dot(&n, y, &inc, z, &inc);
         r = nrm2(&n, z, &inc);
         im = iamax(&n, z, &inc);
+        if (it >= warmup && it_s) {
+            const double tn = now();
+            it_s[it - warmup] = tn - tp;
+            tp = tn;
+        }
     }
     const double secs = now() - t0;
     const double bytes = ((double)nn + 9.0 * n) * 8.0 * iters;
@@ -147,6 +161,14 @@ int main(int argc, char **argv)
     printf("{\"n\": %d, \"iters\": %d, \"warmup\": %d, \"seconds\": %.6f, \"ms_per_iter\": %.6f, "
            "\"GBps\": %.3f, \"ddot\": %.17g, \"dnrm2\": %.17g, \"idamax\": %d, \"tracked\": %d",
            n, iters, warmup, secs, 1e3 * secs / iters, bytes / secs / 1e9, d, r, im, tracked);
+    if (it_s) {
+        qsort(it_s, (size_t)iters, sizeof(double), cmp_double);
+        const double p10 = it_s[(iters - 1) / 10], p50 = it_s[(iters - 1) / 2],
+                     p90 = it_s[(iters - 1) * 9 / 10];
+        printf(", \"iter_ms_p10\": %.6f, \"iter_ms_p50\": %.6f, \"iter_ms_p90\": %.6f", 1e3 * p10,
+               1e3 * p50, 1e3 * p90);
+        free(it_s);
+    }
     if (ktime) {
         const char *ks[] = {"dgemv_n", "daxpy", "ddot", "dnrm2", "idamax"};
         for (int i = 0; i < 5; ++i) {
