@@ -19,6 +19,8 @@
  *       dtrsv_         blas_level2/trsv.cc:81-88
  *       dsymv_         blas_level2/symv.cc:11-75 (CBLAS spec; netlib DSYMV)
  *       dsyr_          blas_level2/syr.cc:11-68 (CBLAS spec; netlib DSYR)
+ *       dsyr2_ dspr_ dspr2_  blas_level2/syr2.cc, spr.cc, spr2.cc (CBLAS specs;
+ *                      netlib DSYR2, DSPR, DSPR2; packed = netlib column packing)
  *       daxpy_         blas_level1/axpy.cc:48-50 (blas.h:20-31)
  *       ddot_          blas_level1/dot.cc:44-48 (returns double by value,
  *                      not the reference's void macro blas.h:45-51)
@@ -97,6 +99,12 @@ void dsymv_(const char *uplo, const int *n, const double *alpha, const double *a
             const double *x, const int *incx, const double *beta, double *y, const int *incy);
 void dsyr_(const char *uplo, const int *n, const double *alpha, const double *x, const int *incx,
            double *a, const int *lda);
+void dsyr2_(const char *uplo, const int *n, const double *alpha, const double *x,
+            const int *incx, const double *y, const int *incy, double *a, const int *lda);
+void dspr_(const char *uplo, const int *n, const double *alpha, const double *x, const int *incx,
+           double *ap);
+void dspr2_(const char *uplo, const int *n, const double *alpha, const double *x,
+            const int *incx, const double *y, const int *incy, double *ap);
 void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
             const int *incy);
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
@@ -148,6 +156,12 @@ void cblas_dsymv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, cons
                  int lda, const double *x, int incx, double beta, double *y, int incy);
 void cblas_dsyr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
                 int incx, double *a, int lda);
+void cblas_dsyr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                 int incx, const double *y, int incy, double *a, int lda);
+void cblas_dspr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                int incx, double *ap);
+void cblas_dspr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                 int incx, const double *y, int incy, double *ap);
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy);
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);

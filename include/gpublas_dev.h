@@ -159,6 +159,15 @@ int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double
                       const double *d_x, int incx, double beta, double *d_y, int incy);
 int gpublas_dev_dsyr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
                      double *d_A, int lda);
+/* DSYR2 (replaces cublasDsyr2, blas_level2/syr2.cc) and the packed rank
+ * updates DSPR / DSPR2 (cublasDspr / cublasDspr2, spr.cc, spr2.cc) on the
+ * netlib packed layout. */
+int gpublas_dev_dsyr2(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                      const double *d_y, int incy, double *d_A, int lda);
+int gpublas_dev_dspr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                     double *d_AP);
+int gpublas_dev_dspr2(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                      const double *d_y, int incy, double *d_AP);
 
 /* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
  * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */

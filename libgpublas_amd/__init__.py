@@ -237,6 +237,19 @@ def dsyr(uplo, n, alpha, x, incx, a, lda):
     library().dsyr_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(a), _i(lda))
 
 
+def dsyr2(uplo, n, alpha, x, incx, y, incy, a, lda):
+    library().dsyr2_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy), _p(a),
+                     _i(lda))
+
+
+def dspr(uplo, n, alpha, x, incx, ap):
+    library().dspr_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(ap))
+
+
+def dspr2(uplo, n, alpha, x, incx, y, incy, ap):
+    library().dspr2_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy), _p(ap))
+
+
 def daxpy(n, alpha, x, incx, y, incy):
     library().daxpy_(_i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy))
 

@@ -1,5 +1,5 @@
 // blas2.hip -- Level-2 kernels on gfx950: DGEMV (N and T), DGER, DTRSV,
-// DSYMV, DSYR.
+// DSYMV, DSYR, DSYR2, and the packed rank updates DSPR, DSPR2.
 //
 // Replaces the vendor calls of the reference's unbuilt CBLAS wrappers:
 // cublasDgemv (blas_level2/gemv.cc:97), cublasDger (ger.cc:92),
@@ -396,30 +396,44 @@ __global__ __launch_bounds__(256) void symv_fold(int n, int lower, int nb, const
 constexpr int SR_ROWS = 512;  // rows per workgroup (256 threads x 2)
 constexpr int SR_COLS = 16;   // columns per workgroup
 
-template <bool LOWER>
+// TWO: the rank-2 update of DSYR2, A += x*(alpha y(j)) + y*(alpha x(j)) per
+// column (netlib's order), a column skipped when x(j) and y(j) are both 0.
+template <bool LOWER, bool TWO>
 __global__ __launch_bounds__(256) void syr_kernel(int n, double alpha, const double *x, long incx,
-                                                  long x0, double *A, long lda, int vec)
+                                                  long x0, const double *y, long incy, long y0,
+                                                  double *A, long lda, int vec)
 {
     const long r0 = (long)blockIdx.x * SR_ROWS, c0 = (long)blockIdx.y * SR_COLS;
     const long rl = r0 + SR_ROWS - 1, cl = c0 + SR_COLS - 1;
     if (LOWER ? rl < c0 : r0 > cl) return;  // no stored element in the block
     const long i = r0 + 2 * threadIdx.x;
     const bool full = vec && rl < n && cl < n && (LOWER ? r0 >= cl : rl <= c0);
-    double xt[SR_COLS];
+    double xt[SR_COLS], yt[SR_COLS];
 #pragma unroll
-    for (int u = 0; u < SR_COLS; ++u) xt[u] = c0 + u < n ? x[x0 + (c0 + u) * incx] : 0.0;
+    for (int u = 0; u < SR_COLS; ++u) {
+        xt[u] = c0 + u < n ? x[x0 + (c0 + u) * incx] : 0.0;
+        yt[u] = TWO && c0 + u < n ? y[y0 + (c0 + u) * incy] : 0.0;
+    }
     if (full) {
         const double xi0 = x[x0 + i * incx], xi1 = x[x0 + (i + 1) * incx];
+        const double yi0 = TWO ? y[y0 + i * incy] : 0.0, yi1 = TWO ? y[y0 + (i + 1) * incy] : 0.0;
         d2 v[SR_COLS];
 #pragma unroll
         for (int u = 0; u < SR_COLS; ++u)
             v[u] = __builtin_nontemporal_load((const d2 *)(A + i + (c0 + u) * lda));
 #pragma unroll
         for (int u = 0; u < SR_COLS; ++u) {
-            const double t = alpha * xt[u];
             d2 w;
-            w.x = xt[u] != 0.0 ? fma(xi0, t, v[u].x) : v[u].x;
-            w.y = xt[u] != 0.0 ? fma(xi1, t, v[u].y) : v[u].y;
+            if (TWO) {
+                const double t1 = alpha * yt[u], t2 = alpha * xt[u];
+                const bool on = xt[u] != 0.0 || yt[u] != 0.0;
+                w.x = on ? fma(yi0, t2, fma(xi0, t1, v[u].x)) : v[u].x;
+                w.y = on ? fma(yi1, t2, fma(xi1, t1, v[u].y)) : v[u].y;
+            } else {
+                const double t = alpha * xt[u];
+                w.x = xt[u] != 0.0 ? fma(xi0, t, v[u].x) : v[u].x;
+                w.y = xt[u] != 0.0 ? fma(xi1, t, v[u].y) : v[u].y;
+            }
             __builtin_nontemporal_store(w, (d2 *)(A + i + (c0 + u) * lda));
         }
         return;
@@ -428,13 +442,38 @@ __global__ __launch_bounds__(256) void syr_kernel(int n, double alpha, const dou
     for (int h = 0; h < 2; ++h) {
         const long r = i + h;
         if (r >= n) break;
-        const double xr = x[x0 + r * incx];
+        const double xr = x[x0 + r * incx], yr = TWO ? y[y0 + r * incy] : 0.0;
         for (int u = 0; u < SR_COLS; ++u) {
             const long j = c0 + u;
-            if (j >= n || (LOWER ? r < j : r > j) || xt[u] == 0.0) continue;
+            if (j >= n || (LOWER ? r < j : r > j) || (xt[u] == 0.0 && yt[u] == 0.0)) continue;
             double *p = A + r + j * lda;
-            *p = fma(xr, alpha * xt[u], *p);
+            *p = TWO ? fma(yr, alpha * xt[u], fma(xr, alpha * yt[u], *p))
+                     : fma(xr, alpha * xt[u], *p);
         }
+    }
+}
+
+// DSPR / DSPR2 on packed storage: column j of an upper triangle holds rows
+// 0..j at AP[j(j+1)/2 ..], of a lower one rows j..n-1 at AP[j(2n-j+1)/2 ..];
+// within a column consecutive rows are consecutive elements, so the 256
+// threads of a workgroup (one row each, 16 columns) access memory coalesced.
+template <bool LOWER, bool TWO>
+__global__ __launch_bounds__(256) void spr_kernel(int n, double alpha, const double *x, long incx,
+                                                  long x0, const double *y, long incy, long y0,
+                                                  double *AP)
+{
+    const long r0 = (long)blockIdx.x * 256, c0 = (long)blockIdx.y * SR_COLS;
+    if (LOWER ? r0 + 255 < c0 : r0 > c0 + SR_COLS - 1) return;
+    const long r = r0 + threadIdx.x;
+    if (r >= n) return;
+    const double xr = x[x0 + r * incx], yr = TWO ? y[y0 + r * incy] : 0.0;
+    for (int u = 0; u < SR_COLS; ++u) {
+        const long j = c0 + u;
+        if (j >= n || (LOWER ? r < j : r > j)) continue;
+        const double xj = x[x0 + j * incx], yj = TWO ? y[y0 + j * incy] : 0.0;
+        if (xj == 0.0 && yj == 0.0) continue;
+        const long k = LOWER ? j * (2L * n - j + 1) / 2 + (r - j) : j * (j + 1) / 2 + r;
+        AP[k] = TWO ? fma(yr, alpha * xj, fma(xr, alpha * yj, AP[k])) : fma(xr, alpha * xj, AP[k]);
     }
 }
 
@@ -560,13 +599,72 @@ int gpublas_dev_dsyr(void *stream, char uplo, int n, double alpha, const double 
     dim3 grid((unsigned)ceil_div(n, SR_ROWS), (unsigned)ceil_div(n, SR_COLS));
     PROF_BEGIN(s, "dsyr");
     if (lower)
-        hipLaunchKernelGGL(syr_kernel<true>, grid, dim3(256), 0, s, n, alpha, d_x, (long)incx,
-                           start_of(n, incx), d_A, (long)lda, vec);
+        hipLaunchKernelGGL((syr_kernel<true, false>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, start_of(n, incx), d_x, 1L, 0L, d_A, (long)lda, vec);
     else
-        hipLaunchKernelGGL(syr_kernel<false>, grid, dim3(256), 0, s, n, alpha, d_x, (long)incx,
-                           start_of(n, incx), d_A, (long)lda, vec);
+        hipLaunchKernelGGL((syr_kernel<false, false>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, start_of(n, incx), d_x, 1L, 0L, d_A, (long)lda, vec);
     PROF_END(s);
     return (int)hipGetLastError();
+}
+
+int gpublas_dev_dsyr2(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                      const double *d_y, int incy, double *d_A, int lda)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0 || alpha == 0.0) return 0;
+    const bool lower = uplo == 'L' || uplo == 'l';
+    const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
+    dim3 grid((unsigned)ceil_div(n, SR_ROWS), (unsigned)ceil_div(n, SR_COLS));
+    PROF_BEGIN(s, "dsyr2");
+    if (lower)
+        hipLaunchKernelGGL((syr_kernel<true, true>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, start_of(n, incx), d_y, (long)incy, start_of(n, incy), d_A,
+                           (long)lda, vec);
+    else
+        hipLaunchKernelGGL((syr_kernel<false, true>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, start_of(n, incx), d_y, (long)incy, start_of(n, incy), d_A,
+                           (long)lda, vec);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+static int spr_launch(hipStream_t s, bool two, char uplo, int n, double alpha, const double *d_x,
+                      int incx, const double *d_y, int incy, double *d_AP)
+{
+    if (n <= 0 || alpha == 0.0) return 0;
+    const bool lower = uplo == 'L' || uplo == 'l';
+    dim3 grid((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, SR_COLS));
+    const long x0 = start_of(n, incx), y0 = two ? start_of(n, incy) : 0;
+    const long iy = two ? incy : 1;
+    const double *y = two ? d_y : d_x;
+    PROF_BEGIN(s, two ? "dspr2" : "dspr");
+    if (lower && two)
+        hipLaunchKernelGGL((spr_kernel<true, true>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, x0, y, iy, y0, d_AP);
+    else if (lower)
+        hipLaunchKernelGGL((spr_kernel<true, false>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, x0, y, iy, y0, d_AP);
+    else if (two)
+        hipLaunchKernelGGL((spr_kernel<false, true>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, x0, y, iy, y0, d_AP);
+    else
+        hipLaunchKernelGGL((spr_kernel<false, false>), grid, dim3(256), 0, s, n, alpha, d_x,
+                           (long)incx, x0, y, iy, y0, d_AP);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+int gpublas_dev_dspr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                     double *d_AP)
+{
+    return spr_launch(pick_stream(stream), false, uplo, n, alpha, d_x, incx, nullptr, 1, d_AP);
+}
+
+int gpublas_dev_dspr2(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
+                      const double *d_y, int incy, double *d_AP)
+{
+    return spr_launch(pick_stream(stream), true, uplo, n, alpha, d_x, incx, d_y, incy, d_AP);
 }
 
 int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,

@@ -245,6 +245,42 @@ void cblas_dsyr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const
     WITH_MAP(syr_map, dsyr_(&u, &n, &alpha, x, &incx, a, &lda));
 }
 
+/* ---- SYR2: 1 layout 2 uplo 3 n 4 alpha 5 x 6 incx 7 y 8 incy 9 A 10 lda ---- */
+static const int syr2_map[10] = {0, 2, 3, 0, 0, 6, 0, 8, 0, 10};
+
+void cblas_dsyr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                 int incx, const double *y, int incy, double *a, int lda)
+{
+    if (!lay_ok(layout)) { cbl_err("DSYR2 ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSYR2 ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(syr2_map, dsyr2_(&u, &n, &alpha, x, &incx, y, &incy, a, &lda));
+}
+
+/* ---- SPR: 1 layout 2 uplo 3 n 4 alpha 5 x 6 incx 7 Ap;
+ *      SPR2: ... 7 y 8 incy 9 Ap.  A row-major packed triangle is the
+ *      column-major packed other triangle. ---- */
+static const int spr_map[6] = {0, 2, 3, 0, 0, 6};
+static const int spr2_map[8] = {0, 2, 3, 0, 0, 6, 0, 8};
+
+void cblas_dspr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                int incx, double *ap)
+{
+    if (!lay_ok(layout)) { cbl_err("DSPR  ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSPR  ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(spr_map, dspr_(&u, &n, &alpha, x, &incx, ap));
+}
+
+void cblas_dspr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
+                 int incx, const double *y, int incy, double *ap)
+{
+    if (!lay_ok(layout)) { cbl_err("DSPR2 ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSPR2 ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(spr2_map, dspr2_(&u, &n, &alpha, x, &incx, y, &incy, ap));
+}
+
 /* ---- Level 1 (no layout argument) ---- */
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy)
 {

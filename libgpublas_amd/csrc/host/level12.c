@@ -1,7 +1,7 @@
 /*
  * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
  * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_, and dscal_,
- * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_.
+ * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_, dsyr2_, dspr_, dspr2_.
  *
  * The reference specifies these only through CBLAS wrappers it never builds
  * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
@@ -554,6 +554,124 @@ void dsyr_(const char *uplo, const int *n, const double *alpha, const double *x,
     gb_finish(s, ops, 2);
     GB_COUNT(device_calls, 1);
     gb_leave();
+}
+
+/* ---- DSYR2: A := alpha*x*y**T + alpha*y*x**T + A on the uplo triangle
+ * netlib DSYR2 (the definition blas_level2/syr2.cc forwards to).
+ * INFO: 1 uplo, 2 n, 5 incx, 7 incy, 9 lda ---- */
+void dsyr2_(const char *uplo, const int *n, const double *alpha, const double *x,
+            const int *incx, const double *y, const int *incy, double *a, const int *lda)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*incx == 0) info = 5;
+    else if (*incy == 0) info = 7;
+    else if (*lda < MAXI(1, *n)) info = 9;
+    if (info) {
+        gb_xerbla("DSYR2 ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || *alpha == 0.0) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n;
+    if (gb_host_route(GB_RK_DSYR, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, 2 * nn * nn,
+                      8.0 * (2 * nn * nn + 2 * nn), a, x, y)) {
+        void (*f)(const char *, const int *, const double *, const double *, const int *,
+                  const double *, const int *, double *, const int *) = gb_host_fn("dsyr2_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, n, alpha, x, incx, y, incy, a, lda);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
+    gb_acquire_matrix(&ops[2], a, *lda, *n, *n, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_dsyr2(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
+                               (const double *)ops[1].dev, *incy, (double *)ops[2].dev, *lda),
+             "dsyr2 launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DSPR / DSPR2: packed rank-1 / rank-2 updates (netlib DSPR, DSPR2; the
+ * definitions blas_level2/spr.cc and spr2.cc forward to).  The packed
+ * triangle is n(n+1)/2 contiguous elements.
+ * INFO: 1 uplo, 2 n, 5 incx, 7 incy (DSPR2) ---- */
+static void spr_common(int two, const char *uplo, const int *n, const double *alpha,
+                       const double *x, const int *incx, const double *y, const int *incy,
+                       double *ap)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*incx == 0) info = 5;
+    else if (two && *incy == 0) info = 7;
+    if (info) {
+        gb_xerbla(two ? "DSPR2 " : "DSPR  ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || *alpha == 0.0) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n, len = nn * (nn + 1) / 2;
+    if (gb_host_route(GB_RK_DSYR, gb_opt.cutoff > 0 && *n < gb_opt.cutoff,
+                      (two ? 2 : 1) * nn * nn, 8.0 * (2 * len + (two ? 2 : 1) * nn), ap, x,
+                      two ? y : NULL)) {
+        GB_COUNT(fallthrough, 1);
+        if (two) {
+            void (*f)(const char *, const int *, const double *, const double *, const int *,
+                      const double *, const int *, double *) = gb_host_fn("dspr2_");
+            f(uplo, n, alpha, x, incx, y, incy, ap);
+        } else {
+            void (*f)(const char *, const int *, const double *, const double *, const int *,
+                      double *) = gb_host_fn("dspr_");
+            f(uplo, n, alpha, x, incx, ap);
+        }
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    if (two) gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
+    gb_acquire_vector(&ops[2], ap, (int)len, 1, 8, GB_INOUT, s);
+    int e = two ? gpublas_dev_dspr2(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
+                                    (const double *)ops[1].dev, *incy, (double *)ops[2].dev)
+                : gpublas_dev_dspr(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
+                                   (double *)ops[2].dev);
+    gb_check(e, two ? "dspr2 launch" : "dspr launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void dspr_(const char *uplo, const int *n, const double *alpha, const double *x, const int *incx,
+           double *ap)
+{
+    spr_common(0, uplo, n, alpha, x, incx, NULL, NULL, ap);
+}
+
+void dspr2_(const char *uplo, const int *n, const double *alpha, const double *x,
+            const int *incx, const double *y, const int *incy, double *ap)
+{
+    spr_common(1, uplo, n, alpha, x, incx, y, incy, ap);
 }
 
 /* library-internal names for the crossover measurement (see level3.c) */

@@ -144,6 +144,36 @@ int ora_symv_info(char uplo, int n, int lda, int incx, int incy)
     return 0;
 }
 
+int ora_syr2_info(char uplo, int n, int incx, int incy, int lda)
+{
+    /* netlib DSYR2, which blas_level2/syr2.cc forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 5;
+    if (incy == 0) return 7;
+    if (lda < imax(1, n)) return 9;
+    return 0;
+}
+
+int ora_spr_info(char uplo, int n, int incx)
+{
+    /* netlib DSPR, which blas_level2/spr.cc forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 5;
+    return 0;
+}
+
+int ora_spr2_info(char uplo, int n, int incx, int incy)
+{
+    /* netlib DSPR2, which blas_level2/spr2.cc forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 5;
+    if (incy == 0) return 7;
+    return 0;
+}
+
 int ora_syr_info(char uplo, int n, int incx, int lda)
 {
     /* netlib DSYR, which blas_level2/syr.cc:11-68 forwards to */
@@ -927,5 +957,68 @@ void ora_dsyr(char uplo, int n, double alpha, const double *x, int incx, double 
         int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
         int ix = kx + i0 * incx;
         for (int i = i0; i < i1; ++i, ix += incx) A[IX(i, j, lda)] += x[ix] * temp;
+    }
+}
+
+/* netlib DSYR2: A := alpha*x*y' + alpha*y*x' + A on the uplo triangle
+ * (blas_level2/syr2.cc forwards here); a column with x(j) == y(j) == 0 is
+ * skipped, and each element takes A + x(i)*temp1 + y(i)*temp2 in that order. */
+void ora_dsyr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
+               int incy, double *A, int lda)
+{
+    if (ora_syr2_info(uplo, n, incx, incy, lda)) return;
+    if (n == 0 || alpha == 0) return;
+    const int kx = incx > 0 ? 0 : -(n - 1) * incx, ky = incy > 0 ? 0 : -(n - 1) * incy;
+    const int upper = ora_lsame(uplo, 'U');
+    for (int j = 0; j < n; ++j) {
+        const double xj = x[kx + j * incx], yj = y[ky + j * incy];
+        if (xj == 0 && yj == 0) continue;
+        const double temp1 = alpha * yj, temp2 = alpha * xj;
+        const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+        for (int i = i0; i < i1; ++i)
+            A[IX(i, j, lda)] = A[IX(i, j, lda)] + x[kx + i * incx] * temp1 + y[ky + i * incy] * temp2;
+    }
+}
+
+/* Packed storage (netlib): column j of an upper triangle holds rows 0..j at
+ * AP[j(j+1)/2 ..], of a lower one rows j..n-1 at AP[j(2n-j+1)/2 ..]. */
+static long pk(int upper, int n, int i, int j)
+{
+    return upper ? (long)j * (j + 1) / 2 + i : (long)j * (2 * n - j + 1) / 2 + (i - j);
+}
+
+/* netlib DSPR: AP := alpha*x*x' + AP (blas_level2/spr.cc forwards here) */
+void ora_dspr(char uplo, int n, double alpha, const double *x, int incx, double *AP)
+{
+    if (ora_spr_info(uplo, n, incx)) return;
+    if (n == 0 || alpha == 0) return;
+    const int kx = incx > 0 ? 0 : -(n - 1) * incx;
+    const int upper = ora_lsame(uplo, 'U');
+    for (int j = 0; j < n; ++j) {
+        const double xj = x[kx + j * incx];
+        if (xj == 0) continue;
+        const double temp = alpha * xj;
+        const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+        for (int i = i0; i < i1; ++i) AP[pk(upper, n, i, j)] += x[kx + i * incx] * temp;
+    }
+}
+
+/* netlib DSPR2: AP := alpha*x*y' + alpha*y*x' + AP (blas_level2/spr2.cc) */
+void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
+               int incy, double *AP)
+{
+    if (ora_spr2_info(uplo, n, incx, incy)) return;
+    if (n == 0 || alpha == 0) return;
+    const int kx = incx > 0 ? 0 : -(n - 1) * incx, ky = incy > 0 ? 0 : -(n - 1) * incy;
+    const int upper = ora_lsame(uplo, 'U');
+    for (int j = 0; j < n; ++j) {
+        const double xj = x[kx + j * incx], yj = y[ky + j * incy];
+        if (xj == 0 && yj == 0) continue;
+        const double temp1 = alpha * yj, temp2 = alpha * xj;
+        const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+        for (int i = i0; i < i1; ++i) {
+            const long k = pk(upper, n, i, j);
+            AP[k] = AP[k] + x[kx + i * incx] * temp1 + y[ky + i * incy] * temp2;
+        }
     }
 }

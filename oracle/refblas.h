@@ -41,6 +41,9 @@ int ora_ger_info(int m, int n, int incx, int incy, int lda);
 int ora_trsv_info(char uplo, char trans, char diag, int n, int lda, int incx);
 int ora_symv_info(char uplo, int n, int lda, int incx, int incy);
 int ora_syr_info(char uplo, int n, int incx, int lda);
+int ora_syr2_info(char uplo, int n, int incx, int incy, int lda);
+int ora_spr_info(char uplo, int n, int incx);
+int ora_spr2_info(char uplo, int n, int incx, int incy);
 
 /* ---- Level 3 ---- */
 void ora_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
@@ -77,6 +80,11 @@ double ora_dasum(int n, const double *x, int incx);
 void ora_dsymv(char uplo, int n, double alpha, const double *A, int lda, const double *x,
                int incx, double beta, double *y, int incy);
 void ora_dsyr(char uplo, int n, double alpha, const double *x, int incx, double *A, int lda);
+void ora_dsyr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
+               int incy, double *A, int lda);
+void ora_dspr(char uplo, int n, double alpha, const double *x, int incx, double *AP);
+void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
+               int incy, double *AP);
 void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);
 double ora_ddot(int n, const double *x, int incx, const double *y, int incy);
 double ora_dnrm2(int n, const double *x, int incx);

@@ -30,6 +30,12 @@ def load():
     L.ora_syr_info.argtypes = [C, I, I, I]
     L.ora_dsymv.argtypes = [C, I, D, VP, I, VP, I, D, VP, I]
     L.ora_dsyr.argtypes = [C, I, D, VP, I, VP, I]
+    L.ora_syr2_info.argtypes = [C, I, I, I, I]
+    L.ora_spr_info.argtypes = [C, I, I]
+    L.ora_spr2_info.argtypes = [C, I, I, I]
+    L.ora_dsyr2.argtypes = [C, I, D, VP, I, VP, I, VP, I]
+    L.ora_dspr.argtypes = [C, I, D, VP, I, VP]
+    L.ora_dspr2.argtypes = [C, I, D, VP, I, VP, I, VP]
     L.ora_dgemm.argtypes = [C, C, I, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]

@@ -342,3 +342,33 @@ def test_objtrack_capture_without_uses_writes_an_empty_file():
     assert text.startswith("# object-tracking file written by libgpublas")
     assert "chain_bench" in text.splitlines()[0]
     assert [ln for ln in text.splitlines() if ln and not ln.startswith("#")] == []
+
+
+def test_rank2_and_packed_updates_xerbla_and_fall_through(ora):
+    """DSYR2 / DSPR / DSPR2: netlib INFO codes and the host fall-through for
+    small problems, bit-identical to the host BLAS (scipy's OpenBLAS)."""
+    d = np.zeros(64)
+    gb.dsyr2("L", 3, 1.0, d, 1, d, 1, d, 2)
+    assert gb.last_xerbla() == ("DSYR2 ", ora.ora_syr2_info(b"L", 3, 1, 1, 2))
+    gb.dsyr2("U", 2, 1.0, d, 1, d, 0, d, 2)
+    assert gb.last_xerbla() == ("DSYR2 ", 7)
+    gb.dspr("X", 2, 1.0, d, 1, d)
+    assert gb.last_xerbla() == ("DSPR  ", 1)
+    gb.dspr2("L", 2, 1.0, d, 0, d, 1, d)
+    assert gb.last_xerbla() == ("DSPR2 ", 5)
+    gb.set_cutoff(512)
+    n = 40
+    A = uniform(71, (n, n))
+    x, y = uniform(72, (n,)), uniform(73, (n,))
+    AP = np.concatenate([A[j:, j] for j in range(n)])
+    gb.reset_stats()
+    R = A.copy(order="F")
+    gb.dsyr2("U", n, 0.5, x, 1, y, 1, R, n)
+    assert np.array_equal(R, sb.dsyr2(0.5, x, y, lower=0, a=A.copy(order="F")))
+    P = AP.copy()
+    gb.dspr("L", n, -1.5, x, 1, P)
+    assert np.array_equal(P, sb.dspr(n, -1.5, x, AP.copy(), lower=1))
+    P = AP.copy()
+    gb.dspr2("L", n, 0.25, x, 1, y, 1, P)
+    assert np.array_equal(P, sb.dspr2(n, 0.25, x, y, AP.copy(), lower=1))
+    assert gb.stats()["fallthrough"] == 3

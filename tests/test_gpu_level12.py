@@ -337,3 +337,60 @@ def test_dsyr_integer_exact(gpu, ora):
     gpu.dsyr("U", n, 3.0, x, 1, A, n)
     ora.ora_dsyr(b"U", n, 3.0, p(x), 1, p(R), n)
     assert np.array_equal(A, R)
+
+
+def _pack(A, uplo):
+    n = A.shape[0]
+    if uplo == "U":
+        return np.concatenate([A[: j + 1, j] for j in range(n)])
+    return np.concatenate([A[j:, j] for j in range(n)])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (255, 1, 1), (513, 2, -3), (2048, 1, 1),
+                                         (3001, -1, 2)])
+def test_dsyr2(gpu, ora, uplo, n, incx, incy):
+    A0 = uniform(41, (n, n))
+    x = _strided(uniform(42, (n,)), incx)
+    y = _strided(uniform(43, (n,)), incy)
+    x[0] = y[0] = 0.0  # a column with x(j) = y(j) = 0 is left as it is
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dsyr2(uplo, n, 0.375, x, incx, y, incy, A, n)
+    ora.ora_dsyr2(c(uplo), n, 0.375, p(x), incx, p(y), incy, p(R), n)
+    other = _other_triangle(n, uplo)
+    assert np.array_equal(A[other], A0[other])
+    np.testing.assert_allclose(A, R, rtol=4 * EPS, atol=4 * EPS)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (300, 1, 1), (1000, -2, 3)])
+def test_dspr_dspr2(gpu, ora, uplo, n, incx, incy):
+    A0 = uniform(44, (n, n))
+    AP0 = _pack(A0, uplo)
+    x = _strided(uniform(45, (n,)), incx)
+    y = _strided(uniform(46, (n,)), incy)
+    P, R = AP0.copy(), AP0.copy()
+    gpu.dspr(uplo, n, -0.5, x, incx, P)
+    ora.ora_dspr(c(uplo), n, -0.5, p(x), incx, p(R))
+    np.testing.assert_allclose(P, R, rtol=2 * EPS, atol=2 * EPS)
+    P, R = AP0.copy(), AP0.copy()
+    gpu.dspr2(uplo, n, 0.75, x, incx, y, incy, P)
+    ora.ora_dspr2(c(uplo), n, 0.75, p(x), incx, p(y), incy, p(R))
+    np.testing.assert_allclose(P, R, rtol=4 * EPS, atol=4 * EPS)
+
+
+def test_rank2_updates_integer_exact(gpu, ora):
+    n = 700
+    rng = np.random.default_rng(47)
+    A0 = np.asfortranarray(rng.integers(-50, 51, (n, n)).astype(float))
+    x = rng.integers(-9, 10, n).astype(float)
+    y = rng.integers(-9, 10, n).astype(float)
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dsyr2("L", n, 2.0, x, 1, y, 1, A, n)
+    ora.ora_dsyr2(b"L", n, 2.0, p(x), 1, p(y), 1, p(R), n)
+    assert np.array_equal(A, R)
+    AP0 = _pack(A0, "U")
+    P, Q = AP0.copy(), AP0.copy()
+    gpu.dspr2("U", n, 3.0, x, 1, y, 1, P)
+    ora.ora_dspr2(b"U", n, 3.0, p(x), 1, p(y), 1, p(Q))
+    assert np.array_equal(P, Q)

@@ -454,3 +454,56 @@ def test_symv_syr_info_codes(L):
     assert [L.ora_syr_info(b"X", 2, 1, 2), L.ora_syr_info(b"U", -1, 1, 2),
             L.ora_syr_info(b"U", 2, 0, 2), L.ora_syr_info(b"L", 3, 1, 2),
             L.ora_syr_info(b"u", 3, -2, 3)] == [1, 2, 5, 7, 0]
+
+
+def _pack(A, uplo):
+    """netlib packed storage of the uplo triangle of A (column by column)."""
+    n = A.shape[0]
+    if uplo == "U":
+        return np.concatenate([A[: j + 1, j] for j in range(n)])
+    return np.concatenate([A[j:, j] for j in range(n)])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1)])
+def test_dsyr2_dspr_dspr2_vs_host_blas_and_closed_form(L, uplo, incx, incy):
+    """netlib DSYR2 / DSPR / DSPR2: closed forms A + a(xy' + yx') and A + a xx'
+    on the stored triangle (full or packed), the other triangle untouched, and
+    scipy's OpenBLAS on unit strides; a zero column of x (and y) is skipped."""
+    n = 12
+    A0 = uniform(101, (n, n))
+    x = uniform(102, (1 + (n - 1) * abs(incx),))
+    y = uniform(103, (1 + (n - 1) * abs(incy),))
+    x[0] = 0.0
+    y[0] = 0.0
+    xs, ys = _strided_get(x, n, incx), _strided_get(y, n, incy)
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    R = A0.copy(order="F")
+    L.ora_dsyr2(c(uplo), n, 0.5, p(x), incx, p(y), incy, p(R), n)
+    want = A0 + 0.5 * (np.outer(xs, ys) + np.outer(ys, xs))
+    np.testing.assert_allclose(R[tri], want[tri], rtol=1e-14, atol=1e-15)
+    assert np.array_equal(R[~tri], A0[~tri])
+    if incx == 1 and incy == 1:
+        H = sb.dsyr2(0.5, x, y, lower=int(uplo == "L"), a=A0.copy(order="F"))
+        np.testing.assert_allclose(R[tri], H[tri], rtol=1e-15, atol=1e-16)
+    AP0 = _pack(A0, uplo)
+    P = AP0.copy()
+    L.ora_dspr(c(uplo), n, -0.75, p(x), incx, p(P))
+    np.testing.assert_allclose(P, _pack(A0 - 0.75 * np.outer(xs, xs), uplo), rtol=1e-14, atol=1e-15)
+    P2 = AP0.copy()
+    L.ora_dspr2(c(uplo), n, 0.5, p(x), incx, p(y), incy, p(P2))
+    np.testing.assert_allclose(P2, _pack(want, uplo), rtol=1e-14, atol=1e-15)
+    if incx == 1 and incy == 1:
+        np.testing.assert_allclose(P, sb.dspr(n, -0.75, x, AP0.copy(), lower=int(uplo == "L")),
+                                   rtol=1e-15, atol=1e-16)
+        np.testing.assert_allclose(P2, sb.dspr2(n, 0.5, x, y, AP0.copy(), lower=int(uplo == "L")),
+                                   rtol=1e-15, atol=1e-16)
+
+
+def test_rank_update_info_codes(L):
+    assert [L.ora_syr2_info(b"X", 2, 1, 1, 2), L.ora_syr2_info(b"L", -1, 1, 1, 2),
+            L.ora_syr2_info(b"L", 2, 0, 1, 2), L.ora_syr2_info(b"L", 2, 1, 0, 2),
+            L.ora_syr2_info(b"U", 3, 1, 1, 2)] == [1, 2, 5, 7, 9]
+    assert [L.ora_spr_info(b"X", 2, 1), L.ora_spr_info(b"U", -2, 1), L.ora_spr_info(b"U", 2, 0),
+            L.ora_spr_info(b"l", 2, -1)] == [1, 2, 5, 0]
+    assert [L.ora_spr2_info(b"U", 2, 1, 0), L.ora_spr2_info(b"U", 2, 0, 1)] == [7, 5]
