@@ -21,6 +21,8 @@
  *       dsyr_          blas_level2/syr.cc:11-68 (CBLAS spec; netlib DSYR)
  *       dsyr2_ dspr_ dspr2_  blas_level2/syr2.cc, spr.cc, spr2.cc (CBLAS specs;
  *                      netlib DSYR2, DSPR, DSPR2; packed = netlib column packing)
+ *       dtrmv_ dtpmv_ dtpsv_ dspmv_  blas_level2/trmv.cc, tpmv.cc, tpsv.cc, pmv.cc
+ *                      (CBLAS specs; netlib DTRMV, DTPMV, DTPSV, DSPMV)
  *       daxpy_         blas_level1/axpy.cc:48-50 (blas.h:20-31)
  *       ddot_          blas_level1/dot.cc:44-48 (returns double by value,
  *                      not the reference's void macro blas.h:45-51)
@@ -105,6 +107,14 @@ void dspr_(const char *uplo, const int *n, const double *alpha, const double *x,
            double *ap);
 void dspr2_(const char *uplo, const int *n, const double *alpha, const double *x,
             const int *incx, const double *y, const int *incy, double *ap);
+void dtrmv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *a,
+            const int *lda, double *x, const int *incx);
+void dtpmv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *ap,
+            double *x, const int *incx);
+void dtpsv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *ap,
+            double *x, const int *incx);
+void dspmv_(const char *uplo, const int *n, const double *alpha, const double *ap, const double *x,
+            const int *incx, const double *beta, double *y, const int *incy);
 void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
             const int *incy);
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
@@ -162,6 +172,14 @@ void cblas_dspr(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const
                 int incx, double *ap);
 void cblas_dspr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *x,
                  int incx, const double *y, int incy, double *ap);
+void cblas_dtrmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *a, int lda, double *x, int incx);
+void cblas_dtpmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *ap, double *x, int incx);
+void cblas_dtpsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *ap, double *x, int incx);
+void cblas_dspmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *ap,
+                 const double *x, int incx, double beta, double *y, int incy);
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy);
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);

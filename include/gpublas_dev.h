@@ -168,6 +168,16 @@ int gpublas_dev_dspr(void *stream, char uplo, int n, double alpha, const double 
                      double *d_AP);
 int gpublas_dev_dspr2(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
                       const double *d_y, int incy, double *d_AP);
+/* DTRMV, DTPMV, DTPSV, DSPMV (replace cublasDtrmv / Dtpmv / Dtpsv / Dspmv,
+ * blas_level2/trmv.cc, tpmv.cc, tpsv.cc, pmv.cc); packed = netlib packing. */
+int gpublas_dev_dtrmv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,
+                      int lda, double *d_x, int incx);
+int gpublas_dev_dtpmv(void *stream, char uplo, char trans, char diag, int n, const double *d_AP,
+                      double *d_x, int incx);
+int gpublas_dev_dtpsv(void *stream, char uplo, char trans, char diag, int n, const double *d_AP,
+                      double *d_x, int incx);
+int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double *d_AP,
+                      const double *d_x, int incx, double beta, double *d_y, int incy);
 
 /* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
  * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */

@@ -250,6 +250,23 @@ def dspr2(uplo, n, alpha, x, incx, y, incy, ap):
     library().dspr2_(_c(uplo), _i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy), _p(ap))
 
 
+def dtrmv(uplo, trans, diag, n, a, lda, x, incx):
+    library().dtrmv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(a), _i(lda), _p(x), _i(incx))
+
+
+def dtpmv(uplo, trans, diag, n, ap, x, incx):
+    library().dtpmv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(ap), _p(x), _i(incx))
+
+
+def dtpsv(uplo, trans, diag, n, ap, x, incx):
+    library().dtpsv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(ap), _p(x), _i(incx))
+
+
+def dspmv(uplo, n, alpha, ap, x, incx, beta, y, incy):
+    library().dspmv_(_c(uplo), _i(n), _d(alpha), _p(ap), _p(x), _i(incx), _d(beta), _p(y),
+                     _i(incy))
+
+
 def daxpy(n, alpha, x, incx, y, incy):
     library().daxpy_(_i(n), _d(alpha), _p(x), _i(incx), _p(y), _i(incy))
 

@@ -11,6 +11,8 @@
  *   blas_bench --lib ... --op gemv --m M --n N --ta N (DGEMV, N or T)
  *   blas_bench --lib ... --op symv --n N --uplo L   (DSYMV, bytes = the triangle once)
  *   blas_bench --lib ... --op syr --n N --uplo L    (DSYR, the triangle read and written)
+ *   blas_bench --lib ... --op trmv --n N --uplo L --ta N --diag N  (DTRMV, the triangle once)
+ *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   blas_bench --lib ... --op link --n BYTES        (host-link copies of pageable
  *                                                    memory: plain vs staged)
@@ -205,6 +207,12 @@ int main(int argc, char **argv)
                            const int *);
     gemv_t gemv = NULL;
     syr_t syr = NULL;
+    typedef void (*trmv_t)(const char *, const char *, const char *, const int *, const double *,
+                           const int *, double *, const int *);
+    trmv_t trmv = NULL;
+    typedef void (*spmv_t)(const char *, const int *, const double *, const double *,
+                           const double *, const int *, const double *, double *, const int *);
+    spmv_t spmv = NULL;
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -282,6 +290,19 @@ int main(int argc, char **argv)
         B = operand((size_t)n, 'd', seed + 1);
         bytes = 8.0 * ((double)n * (n + 1) + n); /* triangle read and written, x */
         flops = (double)n * n;
+    } else if (!strcmp(op, "trmv")) {
+        trmv = (trmv_t)sym("dtrmv_");
+        A = operand((size_t)n * n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        bytes = 8.0 * ((double)n * (n + 1) / 2 + 2.0 * n); /* triangle once, x in/out */
+        flops = (double)n * n;
+    } else if (!strcmp(op, "spmv")) {
+        spmv = (spmv_t)sym("dspmv_");
+        A = operand((size_t)n * (n + 1) / 2, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        C = operand((size_t)n, 'd', seed + 2);
+        bytes = 8.0 * ((double)n * (n + 1) / 2 + 3.0 * n); /* packed triangle once, x, y */
+        flops = 2.0 * n * n;
     } else if (!strcmp(op, "axpy")) {
         axpy = (axpy_t)sym("daxpy_");
         A = operand((size_t)n, 'd', seed);
@@ -314,6 +335,13 @@ int main(int argc, char **argv)
             const int one = 1;
             symv(&uplo, &n, &a, (const double *)A, &n, (const double *)B, &one, &b, (double *)C,
                  &one);
+        } else if (trmv) {
+            const int one = 1;
+            trmv(&uplo, &ta, &diag, &n, (const double *)A, &n, (double *)B, &one);
+        } else if (spmv) {
+            const double a = 1.0, b = 0.5;
+            const int one = 1;
+            spmv(&uplo, &n, &a, (const double *)A, (const double *)B, &one, &b, (double *)C, &one);
         } else if (syr) {
             const double a = 1e-3;
             const int one = 1;

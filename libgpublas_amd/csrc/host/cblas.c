@@ -281,6 +281,62 @@ void cblas_dspr2(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, cons
     WITH_MAP(spr2_map, dspr2_(&u, &n, &alpha, x, &incx, y, &incy, ap));
 }
 
+/* ---- TRMV (positions as TRSV); TPMV / TPSV: 1 layout 2 uplo 3 trans 4 diag
+ *      5 n 6 Ap 7 x 8 incx.  Row-major: the transposed system, other triangle. ---- */
+static const int tpmv_map[8] = {0, 2, 3, 4, 5, 0, 0, 8};
+
+void cblas_dtrmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *a, int lda, double *x, int incx)
+{
+    if (!lay_ok(layout)) { cbl_err("DTRMV ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DTRMV ", 2); return; }
+    if (!tr_ok(trans)) { cbl_err("DTRMV ", 3); return; }
+    if (!diag_ok(diag)) { cbl_err("DTRMV ", 4); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    char d = dg(diag);
+    WITH_MAP(trsv_map, dtrmv_(&u, &t, &d, &n, a, &lda, x, &incx));
+}
+
+static void cblas_tp(int solve, CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans,
+                     CBLAS_DIAG diag, int n, const double *ap, double *x, int incx)
+{
+    const char *nm = solve ? "DTPSV " : "DTPMV ";
+    if (!lay_ok(layout)) { cbl_err(nm, 1); return; }
+    if (!up_ok(uplo)) { cbl_err(nm, 2); return; }
+    if (!tr_ok(trans)) { cbl_err(nm, 3); return; }
+    if (!diag_ok(diag)) { cbl_err(nm, 4); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    char d = dg(diag);
+    if (solve) WITH_MAP(tpmv_map, dtpsv_(&u, &t, &d, &n, ap, x, &incx));
+    else WITH_MAP(tpmv_map, dtpmv_(&u, &t, &d, &n, ap, x, &incx));
+}
+
+void cblas_dtpmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *ap, double *x, int incx)
+{
+    cblas_tp(0, layout, uplo, trans, diag, n, ap, x, incx);
+}
+
+void cblas_dtpsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, const double *ap, double *x, int incx)
+{
+    cblas_tp(1, layout, uplo, trans, diag, n, ap, x, incx);
+}
+
+/* ---- SPMV: 1 layout 2 uplo 3 n 4 alpha 5 Ap 6 x 7 incx 8 beta 9 y 10 incy ---- */
+static const int spmv_map[10] = {0, 2, 3, 0, 0, 0, 7, 0, 0, 10};
+
+void cblas_dspmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *ap,
+                 const double *x, int incx, double beta, double *y, int incy)
+{
+    if (!lay_ok(layout)) { cbl_err("DSPMV ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSPMV ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(spmv_map, dspmv_(&u, &n, &alpha, ap, x, &incx, &beta, y, &incy));
+}
+
 /* ---- Level 1 (no layout argument) ---- */
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy)
 {

@@ -1,7 +1,8 @@
 /*
  * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
  * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_, and dscal_,
- * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_, dsyr2_, dspr_, dspr2_.
+ * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_, dsyr2_, dspr_, dspr2_,
+ * dtrmv_, dtpmv_, dtpsv_, dspmv_.
  *
  * The reference specifies these only through CBLAS wrappers it never builds
  * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
@@ -672,6 +673,140 @@ void dspr2_(const char *uplo, const int *n, const double *alpha, const double *x
             const int *incx, const double *y, const int *incy, double *ap)
 {
     spr_common(1, uplo, n, alpha, x, incx, y, incy, ap);
+}
+
+/* ---- DTRMV / DTPMV / DTPSV: x := op(A)*x and the packed solve (netlib
+ * DTRMV, DTPMV, DTPSV; the definitions blas_level2/trmv.cc, tpmv.cc, tpsv.cc
+ * forward to).  INFO: 1 uplo, 2 trans, 3 diag, 4 n, 6 lda (DTRMV), 8 incx
+ * (DTRMV) / 7 incx (packed) ---- */
+enum { TRI_MV = 0, TRI_PMV = 1, TRI_PSV = 2 };
+static void tri_common(int kind, const char *uplo, const char *trans, const char *diag,
+                       const int *n, const double *a, const int *lda, double *x, const int *incx)
+{
+    static const char *names[3] = {"DTRMV ", "DTPMV ", "DTPSV "};
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (!gb_lsame(*trans, 'N') && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 2;
+    else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 3;
+    else if (*n < 0) info = 4;
+    else if (kind == TRI_MV && *lda < MAXI(1, *n)) info = 6;
+    else if (*incx == 0) info = kind == TRI_MV ? 8 : 7;
+    if (info) {
+        gb_xerbla(names[kind], info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n, tri = nn * (nn + 1) / 2;
+    if (gb_host_route(GB_RK_DTRSV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, nn * nn,
+                      8.0 * (tri + 2 * nn), a, x, NULL)) {
+        GB_COUNT(fallthrough, 1);
+        if (kind == TRI_MV) {
+            void (*f)(const char *, const char *, const char *, const int *, const double *,
+                      const int *, double *, const int *) = gb_host_fn("dtrmv_");
+            f(uplo, trans, diag, n, a, lda, x, incx);
+        } else {
+            void (*f)(const char *, const char *, const char *, const int *, const double *,
+                      double *, const int *) = gb_host_fn(kind == TRI_PMV ? "dtpmv_" : "dtpsv_");
+            f(uplo, trans, diag, n, a, x, incx);
+        }
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    memset(ops, 0, sizeof ops);
+    if (kind == TRI_MV) gb_acquire_matrix(&ops[0], a, *lda, *n, *n, 8, GB_IN, s);
+    else gb_acquire_vector(&ops[0], a, (int)tri, 1, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_INOUT, s);
+    int e;
+    if (kind == TRI_MV)
+        e = gpublas_dev_dtrmv(s, *uplo, *trans, *diag, *n, (const double *)ops[0].dev, *lda,
+                              (double *)ops[1].dev, *incx);
+    else if (kind == TRI_PMV)
+        e = gpublas_dev_dtpmv(s, *uplo, *trans, *diag, *n, (const double *)ops[0].dev,
+                              (double *)ops[1].dev, *incx);
+    else
+        e = gpublas_dev_dtpsv(s, *uplo, *trans, *diag, *n, (const double *)ops[0].dev,
+                              (double *)ops[1].dev, *incx);
+    gb_check(e, "triangular level-2 launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void dtrmv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *a,
+            const int *lda, double *x, const int *incx)
+{
+    tri_common(TRI_MV, uplo, trans, diag, n, a, lda, x, incx);
+}
+
+void dtpmv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *ap,
+            double *x, const int *incx)
+{
+    tri_common(TRI_PMV, uplo, trans, diag, n, ap, NULL, x, incx);
+}
+
+void dtpsv_(const char *uplo, const char *trans, const char *diag, const int *n, const double *ap,
+            double *x, const int *incx)
+{
+    tri_common(TRI_PSV, uplo, trans, diag, n, ap, NULL, x, incx);
+}
+
+/* ---- DSPMV: y := alpha*A*x + beta*y, packed symmetric A (netlib DSPMV, the
+ * definition blas_level2/pmv.cc forwards to).  INFO: 1 uplo, 2 n, 6 incx,
+ * 9 incy ---- */
+void dspmv_(const char *uplo, const int *n, const double *alpha, const double *ap, const double *x,
+            const int *incx, const double *beta, double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*incx == 0) info = 6;
+    else if (*incy == 0) info = 9;
+    if (info) {
+        gb_xerbla("DSPMV ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || (*alpha == 0.0 && *beta == 1.0)) {
+        gb_leave();
+        return;
+    }
+    const double nn = *n, tri = nn * (nn + 1) / 2;
+    if (gb_host_route(GB_RK_DSYMV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, 2 * nn * nn,
+                      8.0 * (tri + 3 * nn), ap, x, y)) {
+        void (*f)(const char *, const int *, const double *, const double *, const double *,
+                  const int *, const double *, double *, const int *) = gb_host_fn("dspmv_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, n, alpha, ap, x, incx, beta, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    if (*alpha != 0.0) {
+        gb_acquire_vector(&ops[0], ap, (int)tri, 1, 8, GB_IN, s);
+        gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_IN, s);
+    }
+    gb_acquire_vector(&ops[2], y, *n, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);
+    gb_check(gpublas_dev_dspmv(s, *uplo, *n, *alpha, (const double *)ops[0].dev,
+                               (const double *)ops[1].dev, *incx, *beta, (double *)ops[2].dev,
+                               *incy),
+             "dspmv launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
 }
 
 /* library-internal names for the crossover measurement (see level3.c) */

@@ -174,6 +174,39 @@ int ora_spr2_info(char uplo, int n, int incx, int incy)
     return 0;
 }
 
+int ora_trmv_info(char uplo, char trans, char diag, int n, int lda, int incx)
+{
+    /* netlib DTRMV, which blas_level2/trmv.cc forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (!ora_lsame(diag, 'U') && !ora_lsame(diag, 'N')) return 3;
+    if (n < 0) return 4;
+    if (lda < imax(1, n)) return 6;
+    if (incx == 0) return 8;
+    return 0;
+}
+
+int ora_tpmv_info(char uplo, char trans, char diag, int n, int incx)
+{
+    /* netlib DTPMV / DTPSV, which blas_level2/tpmv.cc and tpsv.cc forward to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (!ora_lsame(diag, 'U') && !ora_lsame(diag, 'N')) return 3;
+    if (n < 0) return 4;
+    if (incx == 0) return 7;
+    return 0;
+}
+
+int ora_spmv_info(char uplo, int n, int incx, int incy)
+{
+    /* netlib DSPMV, which blas_level2/pmv.cc forwards to */
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (incx == 0) return 6;
+    if (incy == 0) return 9;
+    return 0;
+}
+
 int ora_syr_info(char uplo, int n, int incx, int lda)
 {
     /* netlib DSYR, which blas_level2/syr.cc:11-68 forwards to */
@@ -1019,6 +1052,154 @@ void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const 
         for (int i = i0; i < i1; ++i) {
             const long k = pk(upper, n, i, j);
             AP[k] = AP[k] + x[kx + i * incx] * temp1 + y[ky + i * incy] * temp2;
+        }
+    }
+}
+
+/* Element (i, j) of a triangle in full (lda >= 1) or netlib packed (lda = 0)
+ * storage; callers only ask for elements of the stored triangle. */
+struct tri_acc {
+    const double *a;
+    int lda, upper, n;
+};
+static double tget(const struct tri_acc *t, int i, int j)
+{
+    return t->lda ? t->a[IX(i, j, t->lda)] : t->a[pk(t->upper, t->n, i, j)];
+}
+
+/* netlib DTRMV / DTPMV: x := op(A)*x, in netlib's loop order (notrans:
+ * column sweeps adding x(j)*A(:,j); trans: dot products per element). */
+static void trmv_core(const struct tri_acc *t, int trans, int nounit, double *x, int incx)
+{
+    const int n = t->n, kx = incx > 0 ? 0 : -(n - 1) * incx;
+#define XE(j) x[kx + (j) * incx]
+    if (!trans) {
+        if (t->upper) {
+            for (int j = 0; j < n; ++j)
+                if (XE(j) != 0) {
+                    const double temp = XE(j);
+                    for (int i = 0; i < j; ++i) XE(i) += temp * tget(t, i, j);
+                    if (nounit) XE(j) *= tget(t, j, j);
+                }
+        } else {
+            for (int j = n - 1; j >= 0; --j)
+                if (XE(j) != 0) {
+                    const double temp = XE(j);
+                    for (int i = n - 1; i > j; --i) XE(i) += temp * tget(t, i, j);
+                    if (nounit) XE(j) *= tget(t, j, j);
+                }
+        }
+    } else {
+        if (t->upper) {
+            for (int j = n - 1; j >= 0; --j) {
+                double temp = XE(j);
+                if (nounit) temp *= tget(t, j, j);
+                for (int i = j - 1; i >= 0; --i) temp += tget(t, i, j) * XE(i);
+                XE(j) = temp;
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                double temp = XE(j);
+                if (nounit) temp *= tget(t, j, j);
+                for (int i = j + 1; i < n; ++i) temp += tget(t, i, j) * XE(i);
+                XE(j) = temp;
+            }
+        }
+    }
+#undef XE
+}
+
+void ora_dtrmv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x,
+               int incx)
+{
+    if (ora_trmv_info(uplo, trans, diag, n, lda, incx)) return;
+    if (n == 0) return;
+    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n};
+    trmv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+void ora_dtpmv(char uplo, char trans, char diag, int n, const double *AP, double *x, int incx)
+{
+    if (ora_tpmv_info(uplo, trans, diag, n, incx)) return;
+    if (n == 0) return;
+    const struct tri_acc t = {AP, 0, ora_lsame(uplo, 'U'), n};
+    trmv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+/* netlib DTPSV: op(A)*x = b in place, packed A (blas_level2/tpsv.cc) */
+void ora_dtpsv(char uplo, char trans, char diag, int n, const double *AP, double *x, int incx)
+{
+    if (ora_tpmv_info(uplo, trans, diag, n, incx)) return;
+    if (n == 0) return;
+    const struct tri_acc tt = {AP, 0, ora_lsame(uplo, 'U'), n};
+    const struct tri_acc *t = &tt;
+    const int nounit = ora_lsame(diag, 'N'), kx = incx > 0 ? 0 : -(n - 1) * incx;
+#define XE(j) x[kx + (j) * incx]
+    if (ora_lsame(trans, 'N')) {
+        if (t->upper) {
+            for (int j = n - 1; j >= 0; --j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= tget(t, j, j);
+                    const double temp = XE(j);
+                    for (int i = j - 1; i >= 0; --i) XE(i) -= temp * tget(t, i, j);
+                }
+        } else {
+            for (int j = 0; j < n; ++j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= tget(t, j, j);
+                    const double temp = XE(j);
+                    for (int i = j + 1; i < n; ++i) XE(i) -= temp * tget(t, i, j);
+                }
+        }
+    } else {
+        if (t->upper) {
+            for (int j = 0; j < n; ++j) {
+                double temp = XE(j);
+                for (int i = 0; i < j; ++i) temp -= tget(t, i, j) * XE(i);
+                if (nounit) temp /= tget(t, j, j);
+                XE(j) = temp;
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                double temp = XE(j);
+                for (int i = n - 1; i > j; --i) temp -= tget(t, i, j) * XE(i);
+                if (nounit) temp /= tget(t, j, j);
+                XE(j) = temp;
+            }
+        }
+    }
+#undef XE
+}
+
+/* netlib DSPMV: y := alpha*A*x + beta*y, packed symmetric A (blas_level2/pmv.cc) */
+void ora_dspmv(char uplo, int n, double alpha, const double *AP, const double *x, int incx,
+               double beta, double *y, int incy)
+{
+    if (ora_spmv_info(uplo, n, incx, incy)) return;
+    if (n == 0 || (alpha == 0 && beta == 1)) return;
+    const int kx = incx > 0 ? 0 : -(n - 1) * incx, ky = incy > 0 ? 0 : -(n - 1) * incy;
+    if (beta != 1)
+        for (int i = 0; i < n; ++i) y[ky + i * incy] = beta == 0 ? 0.0 : beta * y[ky + i * incy];
+    if (alpha == 0) return;
+    const int upper = ora_lsame(uplo, 'U');
+    for (int j = 0; j < n; ++j) {
+        const double temp1 = alpha * x[kx + j * incx];
+        double temp2 = 0;
+        if (upper) {
+            for (int i = 0; i < j; ++i) {
+                const double a = AP[pk(1, n, i, j)];
+                y[ky + i * incy] += temp1 * a;
+                temp2 += a * x[kx + i * incx];
+            }
+            y[ky + j * incy] += temp1 * AP[pk(1, n, j, j)] + alpha * temp2;
+        } else {
+            y[ky + j * incy] += temp1 * AP[pk(0, n, j, j)];
+            for (int i = j + 1; i < n; ++i) {
+                const double a = AP[pk(0, n, i, j)];
+                y[ky + i * incy] += temp1 * a;
+                temp2 += a * x[kx + i * incx];
+            }
+            y[ky + j * incy] += alpha * temp2;
         }
     }
 }

@@ -44,6 +44,9 @@ int ora_syr_info(char uplo, int n, int incx, int lda);
 int ora_syr2_info(char uplo, int n, int incx, int incy, int lda);
 int ora_spr_info(char uplo, int n, int incx);
 int ora_spr2_info(char uplo, int n, int incx, int incy);
+int ora_trmv_info(char uplo, char trans, char diag, int n, int lda, int incx);
+int ora_tpmv_info(char uplo, char trans, char diag, int n, int incx); /* also DTPSV */
+int ora_spmv_info(char uplo, int n, int incx, int incy);
 
 /* ---- Level 3 ---- */
 void ora_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
@@ -83,6 +86,12 @@ void ora_dsyr(char uplo, int n, double alpha, const double *x, int incx, double 
 void ora_dsyr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
                int incy, double *A, int lda);
 void ora_dspr(char uplo, int n, double alpha, const double *x, int incx, double *AP);
+void ora_dtrmv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x,
+               int incx);
+void ora_dtpmv(char uplo, char trans, char diag, int n, const double *AP, double *x, int incx);
+void ora_dtpsv(char uplo, char trans, char diag, int n, const double *AP, double *x, int incx);
+void ora_dspmv(char uplo, int n, double alpha, const double *AP, const double *x, int incx,
+               double beta, double *y, int incy);
 void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
                int incy, double *AP);
 void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);

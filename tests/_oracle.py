@@ -36,6 +36,13 @@ def load():
     L.ora_dsyr2.argtypes = [C, I, D, VP, I, VP, I, VP, I]
     L.ora_dspr.argtypes = [C, I, D, VP, I, VP]
     L.ora_dspr2.argtypes = [C, I, D, VP, I, VP, I, VP]
+    L.ora_trmv_info.argtypes = [C, C, C, I, I, I]
+    L.ora_tpmv_info.argtypes = [C, C, C, I, I]
+    L.ora_spmv_info.argtypes = [C, I, I, I]
+    L.ora_dtrmv.argtypes = [C, C, C, I, VP, I, VP, I]
+    L.ora_dtpmv.argtypes = [C, C, C, I, VP, VP, I]
+    L.ora_dtpsv.argtypes = [C, C, C, I, VP, VP, I]
+    L.ora_dspmv.argtypes = [C, I, D, VP, VP, I, D, VP, I]
     L.ora_dgemm.argtypes = [C, C, I, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]

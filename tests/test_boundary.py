@@ -372,3 +372,53 @@ def test_rank2_and_packed_updates_xerbla_and_fall_through(ora):
     gb.dspr2("L", n, 0.25, x, 1, y, 1, P)
     assert np.array_equal(P, sb.dspr2(n, 0.25, x, y, AP.copy(), lower=1))
     assert gb.stats()["fallthrough"] == 3
+
+
+def test_triangular_and_packed_mv_xerbla_and_fall_through(ora):
+    """DTRMV / DTPMV / DTPSV / DSPMV: netlib INFO codes (and their CBLAS
+    positions), and the small-problem fall-through bit-identical to the host
+    BLAS (scipy's OpenBLAS)."""
+    lib = gb.library()
+    d = np.zeros(64)
+    gb.dtrmv("L", "X", "N", 2, d, 2, d, 1)
+    assert gb.last_xerbla() == ("DTRMV ", ora.ora_trmv_info(b"L", b"X", b"N", 2, 2, 1))
+    gb.dtrmv("L", "N", "N", 3, d, 2, d, 1)
+    assert gb.last_xerbla() == ("DTRMV ", 6)
+    gb.dtrmv("U", "T", "U", 2, d, 2, d, 0)
+    assert gb.last_xerbla() == ("DTRMV ", 8)
+    gb.dtpmv("U", "N", "Q", 2, d, d, 1)
+    assert gb.last_xerbla() == ("DTPMV ", ora.ora_tpmv_info(b"U", b"N", b"Q", 2, 1))
+    gb.dtpsv("L", "N", "N", 2, d, d, 0)
+    assert gb.last_xerbla() == ("DTPSV ", 7)
+    gb.dspmv("L", -1, 1.0, d, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DSPMV ", ora.ora_spmv_info(b"L", -1, 1, 1))
+    gb.dspmv("U", 2, 1.0, d, d, 1, 0.0, d, 0)
+    assert gb.last_xerbla() == ("DSPMV ", 9)
+    lib.cblas_dtrmv(102, 122, 111, 131, 3, p(d), 2, p(d), 1)
+    assert gb.last_xerbla() == ("DTRMV ", 7)  # lda is CBLAS parameter 7
+    lib.cblas_dtpsv(102, 122, 111, 131, 2, p(d), p(d), 0)
+    assert gb.last_xerbla() == ("DTPSV ", 8)
+    lib.cblas_dspmv(101, 122, 2, ctypes.c_double(1.0), p(d), p(d), 0, ctypes.c_double(0.0), p(d), 1)
+    assert gb.last_xerbla() == ("DSPMV ", 7)
+
+    gb.set_cutoff(512)
+    n = 40
+    A = uniform(81, (n, n)) + np.diag(np.full(n, 4.0))
+    A = np.asfortranarray(A)
+    x0, y0 = uniform(82, (n,)), uniform(83, (n,))
+    APl = np.concatenate([A[j:, j] for j in range(n)])
+    APu = np.concatenate([A[: j + 1, j] for j in range(n)])
+    gb.reset_stats()
+    x = x0.copy()
+    gb.dtrmv("U", "T", "N", n, A, n, x, 1)
+    assert np.array_equal(x, sb.dtrmv(A, x0.copy(), lower=0, trans=1))
+    x = x0.copy()
+    gb.dtpmv("L", "N", "U", n, APl, x, 1)
+    assert np.array_equal(x, sb.dtpmv(n, APl, x0.copy(), lower=1, diag=1))
+    x = x0.copy()
+    gb.dtpsv("U", "N", "N", n, APu, x, 1)
+    assert np.array_equal(x, sb.dtpsv(n, APu, x0.copy(), lower=0))
+    y = y0.copy()
+    gb.dspmv("L", n, 0.5, APl, x0, 1, 2.0, y, 1)
+    assert np.array_equal(y, sb.dspmv(n, 0.5, APl, x0, beta=2.0, y=y0.copy(), lower=1))
+    assert gb.stats()["fallthrough"] == 4

@@ -243,6 +243,8 @@ def test_dtrsv(gpu, ora, uplo, trans, diag, n, incx):
         A[np.diag_indices(n)] = n + 1.0
     else:
         A *= 1.0 / n
+        A[np.diag_indices(n)] = np.nan  # a unit diagonal is not read
+    A[_other_triangle(n, uplo)] = np.nan  # nor is the other triangle
     A = np.asfortranarray(A)
     x0 = _strided(uniform(19, (n,)), incx)
     x, r = x0.copy(), x0.copy()
@@ -394,3 +396,120 @@ def test_rank2_updates_integer_exact(gpu, ora):
     gpu.dspr2("U", n, 3.0, x, 1, y, 1, P)
     ora.ora_dspr2(b"U", n, 3.0, p(x), 1, p(y), 1, p(Q))
     assert np.array_equal(P, Q)
+
+
+def _tri_mag(A, uplo, trans, diag, xs):
+    """sum |terms| per element of op(T) x (T the uplo triangle, unit diagonal
+    if diag == 'U'): the scale of the inner-product error bound."""
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    T = np.where(np.isnan(T), 0.0, T)
+    if diag == "U":
+        T[np.diag_indices(len(xs))] = 1.0
+    if trans != "N":
+        T = T.T
+    return np.abs(T) @ np.abs(xs)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("n,incx,pad", [(1, 1, 0), (50, 1, 3), (257, -2, 0), (1000, 1, 1),
+                                        (3000, 3, 0)])
+def test_dtrmv(gpu, ora, uplo, trans, diag, n, incx, pad):
+    """Only the uplo triangle is read (the other holds NaN, and so does a unit
+    diagonal); bound 2 n eps sum|terms|; stride gaps and padding rows kept."""
+    lda = n + pad
+    A = np.full((lda, n), -5.0, order="F")
+    A[:n] = uniform(91, (n, n))
+    A[:n][_other_triangle(n, uplo)] = np.nan
+    if diag == "U":
+        A[np.arange(n), np.arange(n)] = np.nan
+    x0 = _strided(uniform(92, (n,)), incx)
+    x, r = x0.copy(), x0.copy()
+    gpu.dtrmv(uplo, trans, diag, n, A, lda, x, incx)
+    ora.ora_dtrmv(c(uplo), c(trans), c(diag), n, p(A), lda, p(r), incx)
+    mag = _tri_mag(A[:n], uplo, trans, diag, _strided_get(x0, n, incx))
+    got, ref = _strided_get(x, n, incx), _strided_get(r, n, incx)
+    assert np.all(np.abs(got - ref) <= 2 * n * EPS * mag + 1e-300)
+    gaps = np.ones(len(x), bool)
+    gaps[:: abs(incx)][:n] = False
+    assert np.array_equal(x[gaps], x0[gaps])
+
+
+def _strided_get(buf, n, inc):
+    v = buf[:: abs(inc)][:n]
+    return v if inc > 0 else v[::-1]
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dtrmv_integer_exact(gpu, ora, uplo, trans):
+    n = 1300
+    rng = np.random.default_rng(93)
+    A = np.asfortranarray(rng.integers(-9, 10, (n, n)).astype(float))
+    x0 = rng.integers(-9, 10, n).astype(float)
+    for diag in "NU":
+        x, r = x0.copy(), x0.copy()
+        gpu.dtrmv(uplo, trans, diag, n, A, n, x, 1)
+        ora.ora_dtrmv(c(uplo), c(trans), c(diag), n, p(A), n, p(r), 1)
+        assert np.array_equal(x, r)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("n,incx", [(1, 1), (64, 1), (300, -2), (2049, 1)])
+def test_dtpmv_dtpsv(gpu, ora, uplo, trans, diag, n, incx):
+    """Packed triangular product (bound 2 n eps sum|terms|) and solve
+    (diagonally dominant, relative 1e-12); a unit diagonal's packed slots hold
+    NaN and must not be read."""
+    A = uniform(94, (n, n)) * (1.0 / n)
+    A[np.diag_indices(n)] = np.nan if diag == "U" else 2.0
+    AP = _pack(A, uplo)
+    x0 = _strided(uniform(95, (n,)), incx)
+    x, r = x0.copy(), x0.copy()
+    gpu.dtpmv(uplo, trans, diag, n, AP, x, incx)
+    ora.ora_dtpmv(c(uplo), c(trans), c(diag), n, p(AP), p(r), incx)
+    mag = _tri_mag(A, uplo, trans, diag, _strided_get(x0, n, incx))
+    assert np.all(np.abs(_strided_get(x, n, incx) - _strided_get(r, n, incx))
+                  <= 2 * n * EPS * mag + 1e-300)
+    x, r = x0.copy(), x0.copy()
+    gpu.dtpsv(uplo, trans, diag, n, AP, x, incx)
+    ora.ora_dtpsv(c(uplo), c(trans), c(diag), n, p(AP), p(r), incx)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+    assert np.all(np.isfinite(x))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (129, 2, -3), (1000, -1, 2), (4099, 1, 1)])
+def test_dspmv(gpu, ora, uplo, n, incx, incy):
+    A = uniform(96, (n, n))
+    AP = _pack(A, uplo)
+    x = _strided(uniform(97, (n,)), incx)
+    y0 = _strided(uniform(98, (n,)), incy)
+    y, r = y0.copy(), y0.copy()
+    gpu.dspmv(uplo, n, 0.75, AP, x, incx, -0.5, y, incy)
+    ora.ora_dspmv(c(uplo), n, 0.75, p(AP), p(x), incx, -0.5, p(r), incy)
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    S = T + T.T - np.diag(np.diag(A))
+    mag = 0.75 * (np.abs(S) @ np.abs(_strided_get(x, n, incx))) + 0.5 * np.abs(_strided_get(y0, n, incy))
+    assert np.all(np.abs(_strided_get(y, n, incy) - _strided_get(r, n, incy)) <= 2 * n * EPS * mag + 1e-300)
+    gaps = np.ones(len(y), bool)
+    gaps[:: abs(incy)][:n] = False
+    assert np.array_equal(y[gaps], y0[gaps])
+
+
+def test_dspmv_alpha_zero_and_integer_exact(gpu, ora):
+    n = 900
+    rng = np.random.default_rng(99)
+    A = rng.integers(-9, 10, (n, n)).astype(float)
+    AP = _pack(A, "L")
+    x = rng.integers(-9, 10, n).astype(float)
+    y0 = rng.integers(-9, 10, n).astype(float)
+    y, r = y0.copy(), y0.copy()
+    gpu.dspmv("L", n, 2.0, AP, x, 1, 3.0, y, 1)
+    ora.ora_dspmv(b"L", n, 2.0, p(AP), p(x), 1, 3.0, p(r), 1)
+    assert np.array_equal(y, r)
+    y = y0.copy()
+    gpu.dspmv("U", n, 0.0, AP, x, 1, 0.5, y, 1)  # alpha = 0: y := beta y, AP unread
+    assert np.array_equal(y, 0.5 * y0)

@@ -507,3 +507,68 @@ def test_rank_update_info_codes(L):
     assert [L.ora_spr_info(b"X", 2, 1), L.ora_spr_info(b"U", -2, 1), L.ora_spr_info(b"U", 2, 0),
             L.ora_spr_info(b"l", 2, -1)] == [1, 2, 5, 0]
     assert [L.ora_spr2_info(b"U", 2, 1, 0), L.ora_spr2_info(b"U", 2, 0, 1)] == [7, 5]
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("incx", [1, -2])
+def test_trmv_tpmv_tpsv_vs_host_blas_and_closed_form(L, uplo, trans, diag, incx):
+    """netlib DTRMV / DTPMV / DTPSV: closed forms op(T) x and op(T)^-1 x with
+    T the uplo triangle (unit diagonal for diag U; the other triangle NaN in
+    the full-storage call, never read), packed storage giving the same
+    result, and scipy's OpenBLAS on unit strides."""
+    n = 11
+    A = uniform(111, (n, n)) + np.diag(np.full(n, 3.0))
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    if diag == "U":
+        T = T - np.diag(np.diag(T)) + np.eye(n)
+    opT = T if trans == "N" else T.T
+    An = A.copy(order="F")
+    An[np.triu_indices(n, 1) if uplo == "L" else np.tril_indices(n, -1)] = np.nan
+    x0 = uniform(112, (1 + (n - 1) * abs(incx),))
+    xs = _strided_get(x0, n, incx)
+    X = x0.copy()
+    L.ora_dtrmv(c(uplo), c(trans), c(diag), n, p(An), n, p(X), incx)
+    np.testing.assert_allclose(_strided_get(X, n, incx), opT @ xs, rtol=1e-13, atol=1e-14)
+    AP = _pack(A, uplo)
+    Xp = x0.copy()
+    L.ora_dtpmv(c(uplo), c(trans), c(diag), n, p(AP), p(Xp), incx)
+    np.testing.assert_allclose(Xp, X, rtol=1e-15, atol=1e-15)
+    Xs = x0.copy()
+    L.ora_dtpsv(c(uplo), c(trans), c(diag), n, p(AP), p(Xs), incx)
+    np.testing.assert_allclose(_strided_get(Xs, n, incx), np.linalg.solve(opT, xs), rtol=1e-12,
+                               atol=1e-13)
+    if incx == 1:
+        kw = dict(lower=int(uplo == "L"), trans=int(trans == "T"), diag=int(diag == "U"))
+        np.testing.assert_allclose(X, sb.dtrmv(A.copy(order="F"), x0.copy(), **kw), rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(Xp, sb.dtpmv(n, AP, x0.copy(), **kw), rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(Xs, sb.dtpsv(n, AP, x0.copy(), **kw), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1)])
+def test_dspmv_vs_dsymv_and_host_blas(L, uplo, incx, incy):
+    """netlib DSPMV equals DSYMV on the same triangle (packed vs full), and
+    scipy's OpenBLAS on unit strides."""
+    n = 13
+    A = uniform(113, (n, n))
+    x = uniform(114, (1 + (n - 1) * abs(incx),))
+    y0 = uniform(115, (1 + (n - 1) * abs(incy),))
+    R, S = y0.copy(), y0.copy()
+    L.ora_dspmv(c(uplo), n, 0.7, p(_pack(A, uplo)), p(x), incx, -0.3, p(R), incy)
+    L.ora_dsymv(c(uplo), n, 0.7, p(A), n, p(x), incx, -0.3, p(S), incy)
+    np.testing.assert_allclose(R, S, rtol=1e-15, atol=1e-15)
+    if incx == 1 and incy == 1:
+        H = sb.dspmv(n, 0.7, _pack(A, uplo), x, beta=-0.3, y=y0.copy(), lower=int(uplo == "L"))
+        np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-15)
+
+
+def test_trmv_tpmv_spmv_info_codes(L):
+    assert [L.ora_trmv_info(b"X", b"N", b"N", 2, 2, 1), L.ora_trmv_info(b"L", b"X", b"N", 2, 2, 1),
+            L.ora_trmv_info(b"L", b"N", b"X", 2, 2, 1), L.ora_trmv_info(b"L", b"N", b"N", -1, 2, 1),
+            L.ora_trmv_info(b"L", b"N", b"N", 3, 2, 1), L.ora_trmv_info(b"L", b"N", b"N", 2, 2, 0)] \
+        == [1, 2, 3, 4, 6, 8]
+    assert [L.ora_tpmv_info(b"U", b"T", b"U", -1, 1), L.ora_tpmv_info(b"U", b"T", b"U", 2, 0)] == [4, 7]
+    assert [L.ora_spmv_info(b"X", 2, 1, 1), L.ora_spmv_info(b"U", -1, 1, 1),
+            L.ora_spmv_info(b"U", 2, 0, 1), L.ora_spmv_info(b"U", 2, 1, 0)] == [1, 2, 6, 9]
