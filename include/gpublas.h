@@ -23,6 +23,8 @@
  *                      netlib DSYR2, DSPR, DSPR2; packed = netlib column packing)
  *       dtrmv_ dtpmv_ dtpsv_ dspmv_  blas_level2/trmv.cc, tpmv.cc, tpsv.cc, pmv.cc
  *                      (CBLAS specs; netlib DTRMV, DTPMV, DTPSV, DSPMV)
+ *       dgbmv_ dsbmv_ dtbmv_ dtbsv_  blas_level2/gbmv.cc, sbmv.cc, tbmv.cc, tbsv.cc
+ *                      (CBLAS specs; netlib DGBMV, DSBMV, DTBMV, DTBSV; band storage)
  *       daxpy_         blas_level1/axpy.cc:48-50 (blas.h:20-31)
  *       ddot_          blas_level1/dot.cc:44-48 (returns double by value,
  *                      not the reference's void macro blas.h:45-51)
@@ -115,6 +117,16 @@ void dtpsv_(const char *uplo, const char *trans, const char *diag, const int *n,
             double *x, const int *incx);
 void dspmv_(const char *uplo, const int *n, const double *alpha, const double *ap, const double *x,
             const int *incx, const double *beta, double *y, const int *incy);
+void dgbmv_(const char *trans, const int *m, const int *n, const int *kl, const int *ku,
+            const double *alpha, const double *a, const int *lda, const double *x, const int *incx,
+            const double *beta, double *y, const int *incy);
+void dsbmv_(const char *uplo, const int *n, const int *k, const double *alpha, const double *a,
+            const int *lda, const double *x, const int *incx, const double *beta, double *y,
+            const int *incy);
+void dtbmv_(const char *uplo, const char *trans, const char *diag, const int *n, const int *k,
+            const double *a, const int *lda, double *x, const int *incx);
+void dtbsv_(const char *uplo, const char *trans, const char *diag, const int *n, const int *k,
+            const double *a, const int *lda, double *x, const int *incx);
 void daxpy_(const int *n, const double *alpha, const double *x, const int *incx, double *y,
             const int *incy);
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
@@ -180,6 +192,15 @@ void cblas_dtpsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CB
                  int n, const double *ap, double *x, int incx);
 void cblas_dspmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, const double *ap,
                  const double *x, int incx, double beta, double *y, int incy);
+void cblas_dgbmv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, int kl, int ku,
+                 double alpha, const double *a, int lda, const double *x, int incx, double beta,
+                 double *y, int incy);
+void cblas_dsbmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, int k, double alpha, const double *a,
+                 int lda, const double *x, int incx, double beta, double *y, int incy);
+void cblas_dtbmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, int k, const double *a, int lda, double *x, int incx);
+void cblas_dtbsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, int k, const double *a, int lda, double *x, int incx);
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy);
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);

@@ -178,6 +178,18 @@ int gpublas_dev_dtpsv(void *stream, char uplo, char trans, char diag, int n, con
                       double *d_x, int incx);
 int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double *d_AP,
                       const double *d_x, int incx, double beta, double *d_y, int incy);
+/* DGBMV, DSBMV, DTBMV, DTBSV (replace cublasDgbmv / Dsbmv / Dtbmv / Dtbsv,
+ * blas_level2/gbmv.cc, sbmv.cc, tbmv.cc, tbsv.cc); netlib band storage.
+ * DTBSV supports k <= 8191 (hipErrorInvalidValue beyond). */
+int gpublas_dev_dgbmv(void *stream, char trans, int m, int n, int kl, int ku, double alpha,
+                      const double *d_A, int lda, const double *d_x, int incx, double beta,
+                      double *d_y, int incy);
+int gpublas_dev_dsbmv(void *stream, char uplo, int n, int k, double alpha, const double *d_A,
+                      int lda, const double *d_x, int incx, double beta, double *d_y, int incy);
+int gpublas_dev_dtbmv(void *stream, char uplo, char trans, char diag, int n, int k,
+                      const double *d_A, int lda, double *d_x, int incx);
+int gpublas_dev_dtbsv(void *stream, char uplo, char trans, char diag, int n, int k,
+                      const double *d_A, int lda, double *d_x, int incx);
 
 /* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
  * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */

@@ -262,6 +262,24 @@ def dtpsv(uplo, trans, diag, n, ap, x, incx):
     library().dtpsv_(_c(uplo), _c(trans), _c(diag), _i(n), _p(ap), _p(x), _i(incx))
 
 
+def dgbmv(trans, m, n, kl, ku, alpha, a, lda, x, incx, beta, y, incy):
+    library().dgbmv_(_c(trans), _i(m), _i(n), _i(kl), _i(ku), _d(alpha), _p(a), _i(lda), _p(x),
+                     _i(incx), _d(beta), _p(y), _i(incy))
+
+
+def dsbmv(uplo, n, k, alpha, a, lda, x, incx, beta, y, incy):
+    library().dsbmv_(_c(uplo), _i(n), _i(k), _d(alpha), _p(a), _i(lda), _p(x), _i(incx), _d(beta),
+                     _p(y), _i(incy))
+
+
+def dtbmv(uplo, trans, diag, n, k, a, lda, x, incx):
+    library().dtbmv_(_c(uplo), _c(trans), _c(diag), _i(n), _i(k), _p(a), _i(lda), _p(x), _i(incx))
+
+
+def dtbsv(uplo, trans, diag, n, k, a, lda, x, incx):
+    library().dtbsv_(_c(uplo), _c(trans), _c(diag), _i(n), _i(k), _p(a), _i(lda), _p(x), _i(incx))
+
+
 def dspmv(uplo, n, alpha, ap, x, incx, beta, y, incy):
     library().dspmv_(_c(uplo), _i(n), _d(alpha), _p(ap), _p(x), _i(incx), _d(beta), _p(y),
                      _i(incy))

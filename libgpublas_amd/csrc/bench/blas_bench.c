@@ -13,6 +13,8 @@
  *   blas_bench --lib ... --op syr --n N --uplo L    (DSYR, the triangle read and written)
  *   blas_bench --lib ... --op trmv --n N --uplo L --ta N --diag N  (DTRMV, the triangle once)
  *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
+ *   blas_bench --lib ... --op gbmv --m M --n N --k KL=KU --ta N   (DGBMV, the band once)
+ *   blas_bench --lib ... --op sbmv|tbmv|tbsv --n N --k K --uplo L [--ta N --diag N]
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   blas_bench --lib ... --op link --n BYTES        (host-link copies of pageable
  *                                                    memory: plain vs staged)
@@ -213,6 +215,18 @@ int main(int argc, char **argv)
     typedef void (*spmv_t)(const char *, const int *, const double *, const double *,
                            const double *, const int *, const double *, double *, const int *);
     spmv_t spmv = NULL;
+    typedef void (*gbmv_t)(const char *, const int *, const int *, const int *, const int *,
+                           const double *, const double *, const int *, const double *,
+                           const int *, const double *, double *, const int *);
+    gbmv_t gbmv = NULL;
+    typedef void (*sbmv_t)(const char *, const int *, const int *, const double *, const double *,
+                           const int *, const double *, const int *, const double *, double *,
+                           const int *);
+    sbmv_t sbmv = NULL;
+    typedef void (*tbmv_t)(const char *, const char *, const char *, const int *, const int *,
+                           const double *, const int *, double *, const int *);
+    tbmv_t tbmv = NULL;
+    int bw = 0; /* band array rows */
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -303,6 +317,28 @@ int main(int argc, char **argv)
         C = operand((size_t)n, 'd', seed + 2);
         bytes = 8.0 * ((double)n * (n + 1) / 2 + 3.0 * n); /* packed triangle once, x, y */
         flops = 2.0 * n * n;
+    } else if (!strcmp(op, "gbmv")) {
+        gbmv = (gbmv_t)sym("dgbmv_");
+        bw = 2 * k + 1;
+        A = operand((size_t)bw * n, 'd', seed);
+        B = operand((size_t)(ta == 'N' ? n : m), 'd', seed + 1);
+        C = operand((size_t)(ta == 'N' ? m : n), 'd', seed + 2);
+        bytes = 8.0 * ((double)bw * n + m + 2.0 * (ta == 'N' ? m : n)); /* band, x, y in/out */
+        flops = 2.0 * bw * n;
+    } else if (!strcmp(op, "sbmv") || !strcmp(op, "tbmv") || !strcmp(op, "tbsv")) {
+        bw = k + 1;
+        A = operand((size_t)bw * n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        if (op[0] == 's') {
+            sbmv = (sbmv_t)sym("dsbmv_");
+            C = operand((size_t)n, 'd', seed + 2);
+            bytes = 8.0 * ((double)bw * n + 3.0 * n); /* the stored band once, x, y in/out */
+            flops = 4.0 * bw * n;
+        } else {
+            tbmv = (tbmv_t)sym(op[2] == 'm' ? "dtbmv_" : "dtbsv_");
+            bytes = 8.0 * ((double)bw * n + 2.0 * n);
+            flops = 2.0 * bw * n;
+        }
     } else if (!strcmp(op, "axpy")) {
         axpy = (axpy_t)sym("daxpy_");
         A = operand((size_t)n, 'd', seed);
@@ -342,6 +378,19 @@ int main(int argc, char **argv)
             const double a = 1.0, b = 0.5;
             const int one = 1;
             spmv(&uplo, &n, &a, (const double *)A, (const double *)B, &one, &b, (double *)C, &one);
+        } else if (gbmv) {
+            const double a = 1.0, b = 0.5;
+            const int one = 1;
+            gbmv(&ta, &m, &n, &k, &k, &a, (const double *)A, &bw, (const double *)B, &one, &b,
+                 (double *)C, &one);
+        } else if (sbmv) {
+            const double a = 1.0, b = 0.5;
+            const int one = 1;
+            sbmv(&uplo, &n, &k, &a, (const double *)A, &bw, (const double *)B, &one, &b,
+                 (double *)C, &one);
+        } else if (tbmv) {
+            const int one = 1;
+            tbmv(&uplo, &ta, &diag, &n, &k, (const double *)A, &bw, (double *)B, &one);
         } else if (syr) {
             const double a = 1e-3;
             const int one = 1;

@@ -1,0 +1,506 @@
+// blas2b.hip -- band Level-2 routines on gfx950: DGBMV, DSBMV, DTBMV, DTBSV.
+//
+// Replace the vendor calls of the reference's unbuilt CBLAS wrappers
+// (cublasDgbmv, blas_level2/gbmv.cc; cublasDsbmv, sbmv.cc; cublasDtbmv /
+// cublasDtbsv, tbmv.cc, tbsv.cc).
+//
+// Band storage (netlib): element (i, j) of the band sits at A(ku + i - j, j)
+// of an lda x n array, lda >= kl + ku + 1; every band row r = 0 .. kl + ku of
+// a column is contiguous.  A symmetric or triangular band in the upper
+// triangle is the general band with (kl, ku) = (0, k), in the lower one with
+// (k, 0); its diagonal is band row ku.
+//
+// Products (band_pass).  Output element o sums A(r, col) * v(in) over the
+// band rows r:
+//   notrans  o = row i,    col = i + ku - r, in = col;
+//   trans    o = column j, col = j,          in = j - ku + r.
+// A workgroup owns BR = 256 consecutive outputs and walks the band rows in
+// chunks of BD = 16; per chunk it stages the chunk's rows of the columns it
+// needs (BR + BD - 1 of them for notrans, BR for trans: each column's 16
+// rows are one contiguous 128-byte run) and the matching slice of v in LDS,
+// then every thread adds BD products from LDS.  HBM reads are the band plus
+// (BD - 1) / BR of it for notrans.  When the outputs alone give too few
+// workgroups, the chunks are split over a second grid dimension and per-split
+// partial sums are added in a fixed order by band_fold (deterministic).
+// Elements outside the matrix (the corners of the band array) are replaced by
+// zero, never read; a unit diagonal is replaced by one.
+//   DGBMV: one pass.  DSBMV: y = A x with A = B + B' - diag(B), B the stored
+//   band: a notrans pass over B and a trans pass with band row ku skipped,
+//   summed in the fold (the band is read twice).  DTBMV: one pass from a copy
+//   of x, written back to x.
+//
+// DTBSV (tbsv_reg / tbsv_lds).  The solve is a recurrence along the diagonal:
+// one wave walks the columns in solve order and, after finishing x(j), takes
+// x(j) * coefficient off each of the next k unknowns (the column sweep of the
+// netlib notrans solve; for trans the coefficients of row j are read along
+// the band's diagonals instead).  Unknowns still being updated live in a
+// window: for k <= 63 one register per lane (the finished value is read with
+// v_readlane, 64 coefficients per lane are prefetched a block ahead); for
+// 64 <= k <= 8191 an LDS ring of a power-of-two size > k (at most 64 KiB).  The divide is
+// q = b * (1/d) followed by one residual correction q += (b - q d) / d with
+// the reciprocal, which gives the correctly rounded quotient except in rare
+// double-rounding cases.  One wave runs the whole solve: its instruction
+// issue per column (about 25 instructions) bounds it, not HBM.
+#include "devcommon.hpp"
+#include "gpublas_dev.h"
+
+namespace {
+
+constexpr int BR = 256, BD = 16, BTC = BR + BD - 1;
+
+inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
+inline bool up(char c) { return c == 'U' || c == 'u'; }
+inline bool isN(char c) { return c == 'N' || c == 'n'; }
+
+enum { DG_LOAD = 0, DG_ONE = 1, DG_SKIP = 2 };
+
+struct BandArgs {
+    int m, n, kl, ku;
+    const double *A;
+    long lda;
+    const double *v;  // input vector (length n for notrans, m for trans)
+    long incv, v0;
+    int nchunks, cpw;  // band-row chunks, chunks per split
+    int lenout;
+    // direct write (one pass, one split): out = alpha * s + beta * out
+    double *y;
+    long incy, y0;
+    double alpha, beta;
+    double *P;  // else partial sums P[split][lenout]
+};
+
+template <bool TRANS, int DG, bool DIRECT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void band_pass(BandArgs a)
+{
+    __shared__ double Ts[BTC * (BD + 1)];
+    __shared__ double xs[BR + BD];
+    const int tid = threadIdx.x;
+    const int o0 = blockIdx.x * BR;
+    const int w = a.kl + a.ku + 1;
+    const int cA = blockIdx.y * a.cpw, cB = min(a.nchunks, cA + a.cpw);
+    // this thread loads band row rr of tile columns c0 + 16 u
+    const int rr = tid % BD, c0 = tid / BD;
+    constexpr int TC = TRANS ? BR : BTC;
+    constexpr int NL = (TC + 15) / 16;
+    double s = 0;
+#pragma clang loop unroll(disable)
+    for (int ch = cA; ch < cB; ++ch) {
+        const int r0 = ch * BD, r = r0 + rr;
+        // first tile column and first v index of the tile
+        const int jb = TRANS ? o0 : o0 + a.ku - r0 - (BD - 1);
+        const int vb = TRANS ? o0 - a.ku + r0 : jb;
+        // column j holds matrix row j - ku + r: valid j in [jlo, jhi)
+        const int jlo = max(0, a.ku - r), jhi = min(a.n, a.m + a.ku - r);
+        const bool rok = r < w;
+        const double val_diag = DG == DG_ONE ? 1.0 : 0.0;
+        const bool isdiag = DG != DG_LOAD && r == a.ku;
+        const double *col = a.A + r + (long)(jb + c0) * a.lda;
+        double val[NL];
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int c = c0 + 16 * u, j = jb + c;
+            val[u] = 0;
+            if (c < TC && rok && j >= jlo && j < jhi)
+                val[u] = isdiag ? val_diag : col[(long)(16 * u) * a.lda];
+        }
+        const int lenv = TRANS ? a.m : a.n;
+        const int t0 = vb + tid, t1 = vb + tid + 256;
+        const double xv0 = t0 >= 0 && t0 < lenv ? a.v[a.v0 + t0 * a.incv] : 0.0;
+        const double xv1 = tid + 256 < BTC && t1 >= 0 && t1 < lenv ? a.v[a.v0 + t1 * a.incv] : 0.0;
+        if (ch != cA) __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int c = c0 + 16 * u;
+            if (c < TC) Ts[c * (BD + 1) + rr] = val[u];
+        }
+        xs[tid] = xv0;
+        if (tid + 256 < BTC) xs[tid + 256] = xv1;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BD; ++q) {
+            const int c = TRANS ? tid : tid + (BD - 1) - q;
+            s = fma(Ts[c * (BD + 1) + q], xs[TRANS ? tid + q : c], s);
+        }
+    }
+    const int o = o0 + tid;
+    if (o >= a.lenout) return;
+    if (DIRECT) {
+        double *yo = a.y + a.y0 + o * a.incy;
+        *yo = a.beta == 0.0 ? a.alpha * s : fma(a.alpha, s, a.beta * *yo);
+    } else {
+        a.P[(long)blockIdx.y * a.lenout + o] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void band_fold(int len, int np, const double *P, double alpha,
+                                                 double beta, double *y, long incy, long y0)
+{
+    const long o = (long)blockIdx.x * 256 + threadIdx.x;
+    if (o >= len) return;
+    double s = 0;
+    for (int p = 0; p < np; ++p) s += P[(long)p * len + o];
+    double *yo = y + y0 + o * incy;
+    *yo = beta == 0.0 ? alpha * s : fma(alpha, s, beta * *yo);
+}
+
+__global__ void scale_vec(int n, double beta, double *y, long incy, long y0)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        double *p = y + y0 + i * incy;
+        *p = beta == 0.0 ? 0.0 : beta * *p;
+    }
+}
+
+__global__ void copy_vec(int n, const double *x, long incx, long x0, double *t)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        t[i] = x[x0 + i * incx];
+}
+
+struct Pass {
+    bool trans;
+    int dg;
+};
+
+// out := alpha * sum over passes of op(B) v + beta * out (all passes share
+// the output length).
+int band_apply(hipStream_t s, int m, int n, int kl, int ku, const double *A, long lda,
+               const double *v, long incv, long v0, double alpha, double beta, double *y,
+               long incy, long y0, const Pass *passes, int npass)
+{
+    const int lenout = passes[0].trans ? n : m;
+    if (lenout <= 0) return 0;
+    const int rb = ceil_div(lenout, BR);
+    const int nchunks = ceil_div(kl + ku + 1, BD);
+    int nsplit = ceil_div(1024, (long)rb * npass);
+    if (nsplit > nchunks) nsplit = nchunks;
+    if (nsplit < 1) nsplit = 1;
+    const int cpw = ceil_div(nchunks, nsplit);
+    nsplit = ceil_div(nchunks, cpw);
+    const bool direct = npass == 1 && nsplit == 1;
+    double *P = nullptr;
+    if (!direct) {
+        P = (double *)gpublas_dev_alloc((size_t)npass * nsplit * lenout * sizeof(double));
+        if (!P) return (int)hipErrorOutOfMemory;
+    }
+    PROF_BEGIN(s, "band_mv");
+    for (int p = 0; p < npass; ++p) {
+        BandArgs a{m, n, kl, ku, A, lda, v, incv, v0, nchunks, cpw, lenout,
+                   y, incy, y0, alpha, beta, P ? P + (size_t)p * nsplit * lenout : nullptr};
+        const dim3 grid((unsigned)rb, (unsigned)nsplit);
+#define GB_BP(T, D, DIR) hipLaunchKernelGGL((band_pass<T, D, DIR>), grid, dim3(256), 0, s, a)
+        const bool t = passes[p].trans;
+        const int dg = passes[p].dg;
+        if (direct) {
+            if (t) { if (dg == DG_ONE) GB_BP(true, DG_ONE, true); else GB_BP(true, DG_LOAD, true); }
+            else { if (dg == DG_ONE) GB_BP(false, DG_ONE, true); else GB_BP(false, DG_LOAD, true); }
+        } else if (t) {
+            if (dg == DG_ONE) GB_BP(true, DG_ONE, false);
+            else if (dg == DG_SKIP) GB_BP(true, DG_SKIP, false);
+            else GB_BP(true, DG_LOAD, false);
+        } else {
+            if (dg == DG_ONE) GB_BP(false, DG_ONE, false);
+            else GB_BP(false, DG_LOAD, false);
+        }
+#undef GB_BP
+    }
+    if (!direct) {
+        hipLaunchKernelGGL(band_fold, dim3((unsigned)rb), dim3(256), 0, s, lenout, npass * nsplit, P,
+                           alpha, beta, y, incy, y0);
+        dev_release_after(P, s);
+    }
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------- DTBSV
+
+// Solve order: step s handles column row(s) = FWD ? s : n-1-s; the unknown
+// reached from it with offset d is row(s + d).  Coefficient of offset d at
+// step s: notrans A(band row of (row(s+d), row(s)), row(s)), trans the same
+// with the roles of row and column swapped (row j of op(A) = column j of A
+// read along the band's diagonals).
+struct TbArgs {
+    int n, k;
+    const double *A;
+    long lda;
+    double *x;
+    long incx, x0;
+};
+
+template <bool TR, bool UPPER>
+__device__ __forceinline__ long tb_coef(const TbArgs &a, long j, long d)
+{
+    // j = row(s) (the finished unknown), target = j +- d
+    constexpr bool FWD = TR ? UPPER : !UPPER;
+    const long t = FWD ? j + d : j - d;
+    const long col = TR ? t : j;
+    const long brow = UPPER ? a.k - d : d;
+    return brow + col * a.lda;
+}
+
+template <bool UPPER>
+__device__ __forceinline__ long tb_diag(const TbArgs &a, long j)
+{
+    return (UPPER ? a.k : 0) + j * a.lda;
+}
+
+__device__ __forceinline__ double rl(double v, int lane)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), lane);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double qdiv(double b, double d, double rcp)
+{
+    const double q = b * rcp;
+    return fma(fma(-q, d, b), rcp, q);
+}
+
+// Lanes without a coefficient at a step load this zero instead (branch-free
+// loads in the unrolled step sequence).
+__device__ const double kNoCoef = 0.0;
+
+// k <= 63: window of 64 unknowns, one per lane (lane l holds the unknown of
+// order index p = l mod 64); the coefficients of step st + 32 are loaded at
+// step st into a ring of 32 registers.  Every lane forms its own quotient
+// w / d; the finishing lane's one is broadcast with v_readlane.
+template <bool TR, bool UPPER, bool NOUNIT, bool FULL>
+__device__ __forceinline__ void tbsv_block(const TbArgs &a, int n, int s0, int lw, double (&ring)[32],
+                                           double &w, double nb, double dg, double rcp, double &out)
+{
+    constexpr bool FWD = TR ? UPPER : !UPPER;
+#pragma unroll
+    for (int o = 0; o < 64; ++o) {
+        const double c = ring[o % 32];
+        {
+            // coefficient of step st = s0 + o + 32 for this lane: offset d
+            const int st = s0 + o + 32, d = (lw - o - 32) & 63;
+            const bool ok = (unsigned)(d - 1) < (unsigned)a.k && st + d < n;
+            const int j = FWD ? st : n - 1 - st;
+            const int col = TR ? (FWD ? j + d : j - d) : j;
+            const double *ptr = a.A + (UPPER ? a.k - d : d) + (long)col * a.lda;
+            ring[o % 32] = *(ok ? ptr : &kNoCoef);
+        }
+        if (FULL || s0 + o < n) {
+            const double q = NOUNIT ? qdiv(w, dg, rcp) : w;
+            const double xj = rl(q, o);
+            w = fma(-xj, c, w);
+            if (lw == o) {
+                out = xj;
+                w = nb;
+            }
+        }
+        // keep each step's loads and arithmetic in place: hoisting later
+        // steps' address arithmetic only lengthens live ranges
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <bool TR, bool UPPER, bool NOUNIT>
+__global__ __launch_bounds__(64) void tbsv_reg(TbArgs a)
+{
+    constexpr bool FWD = TR ? UPPER : !UPPER;
+    const int l = threadIdx.x;
+    const int n = a.n;
+    auto row = [&](int p) { return FWD ? p : n - 1 - p; };
+    auto xat = [&](int p) -> double & { return a.x[a.x0 + (long)row(p) * a.incx]; };
+    double ring[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int d = (l - q) & 63;
+        const bool ok = (unsigned)(d - 1) < (unsigned)a.k && q + d < n;
+        const int j = row(q);
+        const int col = TR ? (FWD ? j + d : j - d) : j;
+        ring[q] = ok ? a.A[(UPPER ? a.k - d : d) + (long)col * a.lda] : 0.0;
+    }
+    double w = l < n ? xat(l) : 0.0;
+    double nb = 64 + l < n ? xat(64 + l) : 0.0;
+    double dg = NOUNIT && l < n ? a.A[tb_diag<UPPER>(a, row(l))] : 1.0;
+    for (int s0 = 0; s0 < n; s0 += 64) {
+        // next block: its diagonal and the unknowns entering the window
+        const double dgn = NOUNIT && s0 + 64 + l < n ? a.A[tb_diag<UPPER>(a, row(s0 + 64 + l))] : 1.0;
+        const double nnb = s0 + 128 + l < n ? xat(s0 + 128 + l) : 0.0;
+        const double rcp = NOUNIT ? 1.0 / dg : 1.0;
+        double out = 0;
+        // the lane id through an empty asm: otherwise the 64 per-step offsets
+        // and lane masks, invariant across blocks, are hoisted out of the
+        // block loop and held in registers
+        int lw = l;
+        asm volatile("" : "+v"(lw));
+        if (s0 + 64 <= n) tbsv_block<TR, UPPER, NOUNIT, true>(a, n, s0, lw, ring, w, nb, dg, rcp, out);
+        else tbsv_block<TR, UPPER, NOUNIT, false>(a, n, s0, lw, ring, w, nb, dg, rcp, out);
+        if (s0 + l < n) xat(s0 + l) = out;
+        dg = dgn;
+        nb = nnb;
+    }
+}
+
+// k >= 64: window of W (power of two > k) unknowns in LDS; lane l handles the
+// offsets d = 1 + l + 64 g.  G > 0 (k < 64 G): the next step's coefficients
+// are loaded one step ahead into registers; G = 0: any k, loaded in the step.
+// One wave: its LDS accesses complete in program order, so no barrier.
+template <bool TR, bool UPPER, bool NOUNIT, int G>
+__global__ __launch_bounds__(64) void tbsv_lds(TbArgs a, int W)
+{
+    extern __shared__ double win[];
+    const int l = threadIdx.x;
+    const long n = a.n;
+    const int msk = W - 1;
+    constexpr bool FWD = TR ? UPPER : !UPPER;
+    auto row = [&](long p) { return FWD ? p : n - 1 - p; };
+    auto xat = [&](long p) -> double & { return a.x[a.x0 + row(p) * a.incx]; };
+    auto coef = [&](long st, long d) {
+        return (st < n && d <= a.k && st + d < n) ? a.A[tb_coef<TR, UPPER>(a, row(st), d)] : 0.0;
+    };
+    for (int q = l; q < W; q += 64) win[q] = q < n ? xat(q) : 0.0;
+    constexpr int GR = G > 0 ? G : 1;
+    double cf[GR];
+    if constexpr (G > 0) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) cf[g] = coef(0, 1 + l + 64 * g);
+    }
+    double dgv = NOUNIT ? a.A[tb_diag<UPPER>(a, row(0))] : 1.0;
+    for (long st = 0; st < n; ++st) {
+        double cn[GR];
+        if constexpr (G > 0) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) cn[g] = coef(st + 1, 1 + l + 64 * g);
+        }
+        const double dgnext = NOUNIT && st + 1 < n ? a.A[tb_diag<UPPER>(a, row(st + 1))] : 1.0;
+        const double enter = st + W < n ? xat(st + W) : 0.0;
+        double xj = win[st & msk];
+        if (NOUNIT) xj = xj / dgv;
+        if constexpr (G > 0) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const long d = 1 + l + 64 * g;
+                if (d <= a.k) {
+                    const int slot = (int)((st + d) & msk);
+                    win[slot] = fma(-xj, cf[g], win[slot]);
+                }
+            }
+        } else {
+            for (long d = 1 + l; d <= a.k; d += 64) {
+                const int slot = (int)((st + d) & msk);
+                win[slot] = fma(-xj, coef(st, d), win[slot]);
+            }
+        }
+        if (l == 0) {
+            xat(st) = xj;
+            win[st & msk] = enter;
+        }
+        if constexpr (G > 0) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) cf[g] = cn[g];
+        }
+        dgv = dgnext;
+    }
+}
+
+int tbsv(hipStream_t s, char uplo, char trans, char diag, int n, int k, const double *A, long lda,
+         double *x, int incx)
+{
+    if (n <= 0) return 0;
+    const bool upper = up(uplo), tr = !isN(trans), nounit = isN(diag);
+    TbArgs a{n, k, A, lda, x, (long)incx, start_of(n, incx)};
+    PROF_BEGIN(s, "dtbsv");
+    if (k <= 63) {
+#define GB_TBR(T, U, D) hipLaunchKernelGGL((tbsv_reg<T, U, D>), dim3(1), dim3(64), 0, s, a)
+#define GB_TBR2(T, U) { if (nounit) GB_TBR(T, U, true); else GB_TBR(T, U, false); }
+        if (tr) { if (upper) GB_TBR2(true, true) else GB_TBR2(true, false) }
+        else { if (upper) GB_TBR2(false, true) else GB_TBR2(false, false) }
+#undef GB_TBR2
+#undef GB_TBR
+    } else {
+        int W = 128;
+        while (W < k + 1) W *= 2;
+        if (W > 8192) {  // 64 KiB of LDS
+            PROF_END(s);
+            return (int)hipErrorInvalidValue;
+        }
+        const size_t lds = (size_t)W * sizeof(double);
+#define GB_TBL(T, U, D, G) hipLaunchKernelGGL((tbsv_lds<T, U, D, G>), dim3(1), dim3(64), lds, s, a, W)
+#define GB_TBL_G(T, U, D)                                                                         \
+    { if (k < 128) GB_TBL(T, U, D, 2); else if (k < 1024) GB_TBL(T, U, D, 16); else GB_TBL(T, U, D, 0); }
+#define GB_TBL2(T, U) { if (nounit) GB_TBL_G(T, U, true) else GB_TBL_G(T, U, false) }
+        if (tr) { if (upper) GB_TBL2(true, true) else GB_TBL2(true, false) }
+        else { if (upper) GB_TBL2(false, true) else GB_TBL2(false, false) }
+#undef GB_TBL2
+#undef GB_TBL_G
+#undef GB_TBL
+    }
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpublas_dev_dgbmv(void *stream, char trans, int m, int n, int kl, int ku, double alpha,
+                      const double *d_A, int lda, const double *d_x, int incx, double beta,
+                      double *d_y, int incy)
+{
+    hipStream_t s = pick_stream(stream);
+    if (m <= 0 || n <= 0) return 0;
+    const bool t = !isN(trans);
+    const int lenx = t ? m : n, leny = t ? n : m;
+    const long y0 = start_of(leny, incy);
+    if (alpha == 0.0) {
+        if (beta == 1.0) return 0;
+        hipLaunchKernelGGL(scale_vec, dim3((unsigned)std::min(ceil_div(leny, 256), 4096)), dim3(256),
+                           0, s, leny, beta, d_y, (long)incy, y0);
+        return (int)hipGetLastError();
+    }
+    const Pass p{t, DG_LOAD};
+    return band_apply(s, m, n, kl, ku, d_A, lda, d_x, incx, start_of(lenx, incx), alpha, beta, d_y,
+                      incy, y0, &p, 1);
+}
+
+int gpublas_dev_dsbmv(void *stream, char uplo, int n, int k, double alpha, const double *d_A,
+                      int lda, const double *d_x, int incx, double beta, double *d_y, int incy)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    const long y0 = start_of(n, incy);
+    if (alpha == 0.0) {
+        if (beta == 1.0) return 0;
+        hipLaunchKernelGGL(scale_vec, dim3((unsigned)std::min(ceil_div(n, 256), 4096)), dim3(256), 0,
+                           s, n, beta, d_y, (long)incy, y0);
+        return (int)hipGetLastError();
+    }
+    const int kl = up(uplo) ? 0 : k, ku = up(uplo) ? k : 0;
+    const Pass p[2] = {{false, DG_LOAD}, {true, DG_SKIP}};
+    return band_apply(s, n, n, kl, ku, d_A, lda, d_x, incx, start_of(n, incx), alpha, beta, d_y,
+                      incy, y0, p, 2);
+}
+
+int gpublas_dev_dtbmv(void *stream, char uplo, char trans, char diag, int n, int k,
+                      const double *d_A, int lda, double *d_x, int incx)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
+    if (!t) return (int)hipErrorOutOfMemory;
+    const long x0 = start_of(n, incx);
+    hipLaunchKernelGGL(copy_vec, dim3((unsigned)std::min(ceil_div(n, 256), 4096)), dim3(256), 0, s, n,
+                       d_x, (long)incx, x0, t);
+    const int kl = up(uplo) ? 0 : k, ku = up(uplo) ? k : 0;
+    const Pass p{!isN(trans), isN(diag) ? DG_LOAD : DG_ONE};
+    const int e = band_apply(s, n, n, kl, ku, d_A, lda, t, 1, 0, 1.0, 0.0, d_x, incx, x0, &p, 1);
+    dev_release_after(t, s);
+    return e;
+}
+
+int gpublas_dev_dtbsv(void *stream, char uplo, char trans, char diag, int n, int k,
+                      const double *d_A, int lda, double *d_x, int incx)
+{
+    return tbsv(pick_stream(stream), uplo, trans, diag, n, k, d_A, lda, d_x, incx);
+}
+
+}  // extern "C"

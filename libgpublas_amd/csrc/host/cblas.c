@@ -337,6 +337,73 @@ void cblas_dspmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, double alpha, cons
     WITH_MAP(spmv_map, dspmv_(&u, &n, &alpha, ap, x, &incx, &beta, y, &incy));
 }
 
+/* ---- GBMV: 1 layout 2 trans 3 M 4 N 5 KL 6 KU 7 alpha 8 A 9 lda 10 x 11 incx
+ *      12 beta 13 y 14 incy.  Row-major band storage of A is the column-major
+ *      band storage of A' (n x m, bands ku / kl). ---- */
+static const int gbmv_col[14] = {0, 2, 3, 4, 5, 6, 0, 0, 9, 0, 11, 0, 0, 14};
+static const int gbmv_row[14] = {0, 2, 4, 3, 6, 5, 0, 0, 9, 0, 11, 0, 0, 14};
+
+void cblas_dgbmv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, int kl, int ku,
+                 double alpha, const double *a, int lda, const double *x, int incx, double beta,
+                 double *y, int incy)
+{
+    if (!lay_ok(layout)) { cbl_err("DGBMV ", 1); return; }
+    if (!tr_ok(trans)) { cbl_err("DGBMV ", 2); return; }
+    if (layout == CblasColMajor) {
+        char t = tr(trans);
+        WITH_MAP(gbmv_col,
+                 dgbmv_(&t, &m, &n, &kl, &ku, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+    } else {
+        char t = trflip(trans);
+        WITH_MAP(gbmv_row,
+                 dgbmv_(&t, &n, &m, &ku, &kl, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+    }
+}
+
+/* ---- SBMV: 1 layout 2 uplo 3 n 4 k 5 alpha 6 A 7 lda 8 x 9 incx 10 beta 11 y
+ *      12 incy.  Row-major: the other triangle of the same band bytes. ---- */
+static const int sbmv_map[12] = {0, 2, 3, 4, 0, 0, 7, 0, 9, 0, 0, 12};
+
+void cblas_dsbmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, int n, int k, double alpha, const double *a,
+                 int lda, const double *x, int incx, double beta, double *y, int incy)
+{
+    if (!lay_ok(layout)) { cbl_err("DSBMV ", 1); return; }
+    if (!up_ok(uplo)) { cbl_err("DSBMV ", 2); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    WITH_MAP(sbmv_map, dsbmv_(&u, &n, &k, &alpha, a, &lda, x, &incx, &beta, y, &incy));
+}
+
+/* ---- TBMV / TBSV: 1 layout 2 uplo 3 trans 4 diag 5 n 6 k 7 A 8 lda 9 x
+ *      10 incx.  Row-major: the transposed system, other triangle. ---- */
+static const int tbmv_map[10] = {0, 2, 3, 4, 5, 6, 0, 8, 0, 10};
+
+static void cblas_tb(int solve, CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans,
+                     CBLAS_DIAG diag, int n, int k, const double *a, int lda, double *x, int incx)
+{
+    const char *nm = solve ? "DTBSV " : "DTBMV ";
+    if (!lay_ok(layout)) { cbl_err(nm, 1); return; }
+    if (!up_ok(uplo)) { cbl_err(nm, 2); return; }
+    if (!tr_ok(trans)) { cbl_err(nm, 3); return; }
+    if (!diag_ok(diag)) { cbl_err(nm, 4); return; }
+    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
+    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
+    char d = dg(diag);
+    if (solve) WITH_MAP(tbmv_map, dtbsv_(&u, &t, &d, &n, &k, a, &lda, x, &incx));
+    else WITH_MAP(tbmv_map, dtbmv_(&u, &t, &d, &n, &k, a, &lda, x, &incx));
+}
+
+void cblas_dtbmv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, int k, const double *a, int lda, double *x, int incx)
+{
+    cblas_tb(0, layout, uplo, trans, diag, n, k, a, lda, x, incx);
+}
+
+void cblas_dtbsv(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, CBLAS_DIAG diag,
+                 int n, int k, const double *a, int lda, double *x, int incx)
+{
+    cblas_tb(1, layout, uplo, trans, diag, n, k, a, lda, x, incx);
+}
+
 /* ---- Level 1 (no layout argument) ---- */
 void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int incy)
 {

@@ -2,7 +2,7 @@
  * level12.c -- Fortran-ABI Level-1 and Level-2 entry points on the hot
  * path: daxpy_, ddot_, dnrm2_, idamax_, dgemv_, dger_, dtrsv_, and dscal_,
  * dcopy_, dswap_, drot_, dasum_, dsymv_, dsyr_, dsyr2_, dspr_, dspr2_,
- * dtrmv_, dtpmv_, dtpsv_, dspmv_.
+ * dtrmv_, dtpmv_, dtpsv_, dspmv_, dgbmv_, dsbmv_, dtbmv_, dtbsv_.
  *
  * The reference specifies these only through CBLAS wrappers it never builds
  * (blas_level1/axpy.cc:10-50, dot.cc:9-48, nrm2.cc:9-41, amax.cc:9-43,
@@ -807,6 +807,189 @@ void dspmv_(const char *uplo, const int *n, const double *alpha, const double *a
     gb_finish(s, ops, 3);
     GB_COUNT(device_calls, 1);
     gb_leave();
+}
+
+/* ---- DGBMV: y := alpha*op(A)*x + beta*y, general band A (netlib DGBMV, the
+ * definition blas_level2/gbmv.cc forwards to).  INFO: 1 trans, 2 m, 3 n,
+ * 4 kl, 5 ku, 8 lda, 10 incx, 13 incy ---- */
+void dgbmv_(const char *trans, const int *m, const int *n, const int *kl, const int *ku,
+            const double *alpha, const double *a, const int *lda, const double *x, const int *incx,
+            const double *beta, double *y, const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*trans, 'N') && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 1;
+    else if (*m < 0) info = 2;
+    else if (*n < 0) info = 3;
+    else if (*kl < 0) info = 4;
+    else if (*ku < 0) info = 5;
+    else if (*lda < *kl + *ku + 1) info = 8;
+    else if (*incx == 0) info = 10;
+    else if (*incy == 0) info = 13;
+    if (info) {
+        gb_xerbla("DGBMV ", info);
+        gb_leave();
+        return;
+    }
+    if (*m == 0 || *n == 0 || (*alpha == 0.0 && *beta == 1.0)) {
+        gb_leave();
+        return;
+    }
+    const int nt = gb_lsame(*trans, 'N');
+    const int lenx = nt ? *n : *m, leny = nt ? *m : *n, w = *kl + *ku + 1;
+    const double band = (double)w * *n;
+    if (gb_host_route(GB_RK_DGEMV,
+                      gb_opt.cutoff > 0 && (*m < gb_opt.cutoff || *n < gb_opt.cutoff), 2 * band,
+                      8.0 * (band + lenx + leny), a, x, y)) {
+        void (*f)(const char *, const int *, const int *, const int *, const int *, const double *,
+                  const double *, const int *, const double *, const int *, const double *,
+                  double *, const int *) = gb_host_fn("dgbmv_");
+        GB_COUNT(fallthrough, 1);
+        f(trans, m, n, kl, ku, alpha, a, lda, x, incx, beta, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    if (*alpha != 0.0) {
+        gb_acquire_matrix(&ops[0], a, *lda, w, *n, 8, GB_IN, s);
+        gb_acquire_vector(&ops[1], x, lenx, *incx, 8, GB_IN, s);
+    }
+    gb_acquire_vector(&ops[2], y, leny, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);
+    gb_check(gpublas_dev_dgbmv(s, *trans, *m, *n, *kl, *ku, *alpha, (const double *)ops[0].dev,
+                               *lda, (const double *)ops[1].dev, *incx, *beta,
+                               (double *)ops[2].dev, *incy),
+             "dgbmv launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DSBMV: y := alpha*A*x + beta*y, symmetric band A (netlib DSBMV, the
+ * definition blas_level2/sbmv.cc forwards to).  INFO: 1 uplo, 2 n, 3 k,
+ * 6 lda, 8 incx, 11 incy ---- */
+void dsbmv_(const char *uplo, const int *n, const int *k, const double *alpha, const double *a,
+            const int *lda, const double *x, const int *incx, const double *beta, double *y,
+            const int *incy)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (*n < 0) info = 2;
+    else if (*k < 0) info = 3;
+    else if (*lda < *k + 1) info = 6;
+    else if (*incx == 0) info = 8;
+    else if (*incy == 0) info = 11;
+    if (info) {
+        gb_xerbla("DSBMV ", info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0 || (*alpha == 0.0 && *beta == 1.0)) {
+        gb_leave();
+        return;
+    }
+    const double band = (double)(*k + 1) * *n;
+    if (gb_host_route(GB_RK_DSYMV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, 4 * band,
+                      8.0 * (band + 3.0 * *n), a, x, y)) {
+        void (*f)(const char *, const int *, const int *, const double *, const double *,
+                  const int *, const double *, const int *, const double *, double *,
+                  const int *) = gb_host_fn("dsbmv_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, n, k, alpha, a, lda, x, incx, beta, y, incy);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[3];
+    memset(ops, 0, sizeof ops);
+    if (*alpha != 0.0) {
+        gb_acquire_matrix(&ops[0], a, *lda, *k + 1, *n, 8, GB_IN, s);
+        gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_IN, s);
+    }
+    gb_acquire_vector(&ops[2], y, *n, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);
+    gb_check(gpublas_dev_dsbmv(s, *uplo, *n, *k, *alpha, (const double *)ops[0].dev, *lda,
+                               (const double *)ops[1].dev, *incx, *beta, (double *)ops[2].dev,
+                               *incy),
+             "dsbmv launch");
+    gb_finish(s, ops, 3);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+/* ---- DTBMV / DTBSV: x := op(A)*x and the solve, triangular band A (netlib
+ * DTBMV, DTBSV; the definitions blas_level2/tbmv.cc, tbsv.cc forward to).
+ * INFO: 1 uplo, 2 trans, 3 diag, 4 n, 5 k, 7 lda, 9 incx.  The device solve
+ * keeps the unknowns in flight in LDS, at most 8192 of them: a DTBSV with
+ * k > 8191 runs on the host BLAS. ---- */
+static void tb_common(int solve, const char *uplo, const char *trans, const char *diag,
+                      const int *n, const int *k, const double *a, const int *lda, double *x,
+                      const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    const char *name = solve ? "DTBSV " : "DTBMV ";
+    int info = 0;
+    if (!gb_lsame(*uplo, 'U') && !gb_lsame(*uplo, 'L')) info = 1;
+    else if (!gb_lsame(*trans, 'N') && !gb_lsame(*trans, 'T') && !gb_lsame(*trans, 'C')) info = 2;
+    else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 3;
+    else if (*n < 0) info = 4;
+    else if (*k < 0) info = 5;
+    else if (*lda < *k + 1) info = 7;
+    else if (*incx == 0) info = 9;
+    if (info) {
+        gb_xerbla(name, info);
+        gb_leave();
+        return;
+    }
+    if (*n == 0) {
+        gb_leave();
+        return;
+    }
+    const double band = (double)(*k + 1) * *n;
+    const int host = (solve && *k > 8191) ||
+                     gb_host_route(GB_RK_DTRSV, gb_opt.cutoff > 0 && *n < gb_opt.cutoff, 2 * band,
+                                   8.0 * (band + 2.0 * *n), a, x, NULL);
+    if (host) {
+        void (*f)(const char *, const char *, const char *, const int *, const int *,
+                  const double *, const int *, double *, const int *) =
+            gb_host_fn(solve ? "dtbsv_" : "dtbmv_");
+        GB_COUNT(fallthrough, 1);
+        f(uplo, trans, diag, n, k, a, lda, x, incx);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    memset(ops, 0, sizeof ops);
+    gb_acquire_matrix(&ops[0], a, *lda, *k + 1, *n, 8, GB_IN, s);
+    gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_INOUT, s);
+    const double *da = (const double *)ops[0].dev;
+    double *dx = (double *)ops[1].dev;
+    gb_check(solve ? gpublas_dev_dtbsv(s, *uplo, *trans, *diag, *n, *k, da, *lda, dx, *incx)
+                   : gpublas_dev_dtbmv(s, *uplo, *trans, *diag, *n, *k, da, *lda, dx, *incx),
+             "band triangular launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
+void dtbmv_(const char *uplo, const char *trans, const char *diag, const int *n, const int *k,
+            const double *a, const int *lda, double *x, const int *incx)
+{
+    tb_common(0, uplo, trans, diag, n, k, a, lda, x, incx);
+}
+
+void dtbsv_(const char *uplo, const char *trans, const char *diag, const int *n, const int *k,
+            const double *a, const int *lda, double *x, const int *incx)
+{
+    tb_common(1, uplo, trans, diag, n, k, a, lda, x, incx);
 }
 
 /* library-internal names for the crossover measurement (see level3.c) */

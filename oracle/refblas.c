@@ -1056,15 +1056,23 @@ void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const 
     }
 }
 
-/* Element (i, j) of a triangle in full (lda >= 1) or netlib packed (lda = 0)
- * storage; callers only ask for elements of the stored triangle. */
+/* Element (i, j) of a triangle in full (lda >= 1, k < 0), netlib band
+ * (lda >= k + 1, k >= 0: upper A(k+i-j, j), lower A(i-j, j)) or netlib packed
+ * (lda = 0) storage; callers only ask for elements of the stored triangle
+ * (within the band).  blo / bhi clip a column's loop to the band. */
 struct tri_acc {
     const double *a;
-    int lda, upper, n;
+    int lda, upper, n, k;
 };
 static double tget(const struct tri_acc *t, int i, int j)
 {
+    if (t->k >= 0) return t->a[IX(t->upper ? t->k + i - j : i - j, j, t->lda)];
     return t->lda ? t->a[IX(i, j, t->lda)] : t->a[pk(t->upper, t->n, i, j)];
+}
+static int blo(const struct tri_acc *t, int j) { return t->k >= 0 && j - t->k > 0 ? j - t->k : 0; }
+static int bhi(const struct tri_acc *t, int j)
+{
+    return t->k >= 0 && j + t->k < t->n - 1 ? j + t->k : t->n - 1;
 }
 
 /* netlib DTRMV / DTPMV: x := op(A)*x, in netlib's loop order (notrans:
@@ -1078,14 +1086,14 @@ static void trmv_core(const struct tri_acc *t, int trans, int nounit, double *x,
             for (int j = 0; j < n; ++j)
                 if (XE(j) != 0) {
                     const double temp = XE(j);
-                    for (int i = 0; i < j; ++i) XE(i) += temp * tget(t, i, j);
+                    for (int i = blo(t, j); i < j; ++i) XE(i) += temp * tget(t, i, j);
                     if (nounit) XE(j) *= tget(t, j, j);
                 }
         } else {
             for (int j = n - 1; j >= 0; --j)
                 if (XE(j) != 0) {
                     const double temp = XE(j);
-                    for (int i = n - 1; i > j; --i) XE(i) += temp * tget(t, i, j);
+                    for (int i = bhi(t, j); i > j; --i) XE(i) += temp * tget(t, i, j);
                     if (nounit) XE(j) *= tget(t, j, j);
                 }
         }
@@ -1094,14 +1102,14 @@ static void trmv_core(const struct tri_acc *t, int trans, int nounit, double *x,
             for (int j = n - 1; j >= 0; --j) {
                 double temp = XE(j);
                 if (nounit) temp *= tget(t, j, j);
-                for (int i = j - 1; i >= 0; --i) temp += tget(t, i, j) * XE(i);
+                for (int i = j - 1; i >= blo(t, j); --i) temp += tget(t, i, j) * XE(i);
                 XE(j) = temp;
             }
         } else {
             for (int j = 0; j < n; ++j) {
                 double temp = XE(j);
                 if (nounit) temp *= tget(t, j, j);
-                for (int i = j + 1; i < n; ++i) temp += tget(t, i, j) * XE(i);
+                for (int i = j + 1; i <= bhi(t, j); ++i) temp += tget(t, i, j) * XE(i);
                 XE(j) = temp;
             }
         }
@@ -1114,7 +1122,7 @@ void ora_dtrmv(char uplo, char trans, char diag, int n, const double *A, int lda
 {
     if (ora_trmv_info(uplo, trans, diag, n, lda, incx)) return;
     if (n == 0) return;
-    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n};
+    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n, -1};
     trmv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
 }
 
@@ -1122,8 +1130,50 @@ void ora_dtpmv(char uplo, char trans, char diag, int n, const double *AP, double
 {
     if (ora_tpmv_info(uplo, trans, diag, n, incx)) return;
     if (n == 0) return;
-    const struct tri_acc t = {AP, 0, ora_lsame(uplo, 'U'), n};
+    const struct tri_acc t = {AP, 0, ora_lsame(uplo, 'U'), n, -1};
     trmv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+/* netlib DTPSV / DTBSV: op(A)*x = b in place (column sweeps for notrans, dot
+ * products for trans), loops clipped to the band for band storage. */
+static void trsv_core(const struct tri_acc *t, int trans, int nounit, double *x, int incx)
+{
+    const int n = t->n, kx = incx > 0 ? 0 : -(n - 1) * incx;
+#define XE(j) x[kx + (j) * incx]
+    if (!trans) {
+        if (t->upper) {
+            for (int j = n - 1; j >= 0; --j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= tget(t, j, j);
+                    const double temp = XE(j);
+                    for (int i = j - 1; i >= blo(t, j); --i) XE(i) -= temp * tget(t, i, j);
+                }
+        } else {
+            for (int j = 0; j < n; ++j)
+                if (XE(j) != 0) {
+                    if (nounit) XE(j) /= tget(t, j, j);
+                    const double temp = XE(j);
+                    for (int i = j + 1; i <= bhi(t, j); ++i) XE(i) -= temp * tget(t, i, j);
+                }
+        }
+    } else {
+        if (t->upper) {
+            for (int j = 0; j < n; ++j) {
+                double temp = XE(j);
+                for (int i = blo(t, j); i < j; ++i) temp -= tget(t, i, j) * XE(i);
+                if (nounit) temp /= tget(t, j, j);
+                XE(j) = temp;
+            }
+        } else {
+            for (int j = n - 1; j >= 0; --j) {
+                double temp = XE(j);
+                for (int i = bhi(t, j); i > j; --i) temp -= tget(t, i, j) * XE(i);
+                if (nounit) temp /= tget(t, j, j);
+                XE(j) = temp;
+            }
+        }
+    }
+#undef XE
 }
 
 /* netlib DTPSV: op(A)*x = b in place, packed A (blas_level2/tpsv.cc) */
@@ -1131,44 +1181,126 @@ void ora_dtpsv(char uplo, char trans, char diag, int n, const double *AP, double
 {
     if (ora_tpmv_info(uplo, trans, diag, n, incx)) return;
     if (n == 0) return;
-    const struct tri_acc tt = {AP, 0, ora_lsame(uplo, 'U'), n};
-    const struct tri_acc *t = &tt;
-    const int nounit = ora_lsame(diag, 'N'), kx = incx > 0 ? 0 : -(n - 1) * incx;
-#define XE(j) x[kx + (j) * incx]
-    if (ora_lsame(trans, 'N')) {
-        if (t->upper) {
-            for (int j = n - 1; j >= 0; --j)
-                if (XE(j) != 0) {
-                    if (nounit) XE(j) /= tget(t, j, j);
-                    const double temp = XE(j);
-                    for (int i = j - 1; i >= 0; --i) XE(i) -= temp * tget(t, i, j);
-                }
+    const struct tri_acc t = {AP, 0, ora_lsame(uplo, 'U'), n, -1};
+    trsv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+/* INFO of netlib DTBMV / DTBSV: 1 uplo, 2 trans, 3 diag, 4 n, 5 k, 7 lda, 9 incx */
+int ora_tbmv_info(char uplo, char trans, char diag, int n, int k, int lda, int incx)
+{
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 2;
+    if (!ora_lsame(diag, 'U') && !ora_lsame(diag, 'N')) return 3;
+    if (n < 0) return 4;
+    if (k < 0) return 5;
+    if (lda < k + 1) return 7;
+    if (incx == 0) return 9;
+    return 0;
+}
+
+/* netlib DTBMV: x := op(A)*x, triangular band A (blas_level2/tbmv.cc) */
+void ora_dtbmv(char uplo, char trans, char diag, int n, int k, const double *A, int lda,
+               double *x, int incx)
+{
+    if (ora_tbmv_info(uplo, trans, diag, n, k, lda, incx)) return;
+    if (n == 0) return;
+    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n, k};
+    trmv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+/* netlib DTBSV: op(A)*x = b in place, triangular band A (blas_level2/tbsv.cc) */
+void ora_dtbsv(char uplo, char trans, char diag, int n, int k, const double *A, int lda,
+               double *x, int incx)
+{
+    if (ora_tbmv_info(uplo, trans, diag, n, k, lda, incx)) return;
+    if (n == 0) return;
+    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n, k};
+    trsv_core(&t, !ora_lsame(trans, 'N'), ora_lsame(diag, 'N'), x, incx);
+}
+
+/* INFO of netlib DGBMV: 1 trans, 2 m, 3 n, 4 kl, 5 ku, 8 lda, 10 incx, 13 incy */
+int ora_gbmv_info(char trans, int m, int n, int kl, int ku, int lda, int incx, int incy)
+{
+    if (!ora_lsame(trans, 'N') && !ora_lsame(trans, 'T') && !ora_lsame(trans, 'C')) return 1;
+    if (m < 0) return 2;
+    if (n < 0) return 3;
+    if (kl < 0) return 4;
+    if (ku < 0) return 5;
+    if (lda < kl + ku + 1) return 8;
+    if (incx == 0) return 10;
+    if (incy == 0) return 13;
+    return 0;
+}
+
+/* netlib DGBMV: y := alpha*op(A)*x + beta*y, general band A stored as
+ * A(ku+i-j, j) (blas_level2/gbmv.cc) */
+void ora_dgbmv(char trans, int m, int n, int kl, int ku, double alpha, const double *A, int lda,
+               const double *x, int incx, double beta, double *y, int incy)
+{
+    if (ora_gbmv_info(trans, m, n, kl, ku, lda, incx, incy)) return;
+    if (m == 0 || n == 0 || (alpha == 0 && beta == 1)) return;
+    const int notr = ora_lsame(trans, 'N');
+    const int lenx = notr ? n : m, leny = notr ? m : n;
+    const int kx = incx > 0 ? 0 : -(lenx - 1) * incx, ky = incy > 0 ? 0 : -(leny - 1) * incy;
+    if (beta != 1)
+        for (int i = 0; i < leny; ++i) y[ky + i * incy] = beta == 0 ? 0.0 : beta * y[ky + i * incy];
+    if (alpha == 0) return;
+    for (int j = 0; j < n; ++j) {
+        const int i0 = j - ku > 0 ? j - ku : 0, i1 = j + kl < m - 1 ? j + kl : m - 1;
+        if (notr) {
+            const double temp = alpha * x[kx + j * incx];
+            for (int i = i0; i <= i1; ++i) y[ky + i * incy] += temp * A[IX(ku + i - j, j, lda)];
         } else {
-            for (int j = 0; j < n; ++j)
-                if (XE(j) != 0) {
-                    if (nounit) XE(j) /= tget(t, j, j);
-                    const double temp = XE(j);
-                    for (int i = j + 1; i < n; ++i) XE(i) -= temp * tget(t, i, j);
-                }
-        }
-    } else {
-        if (t->upper) {
-            for (int j = 0; j < n; ++j) {
-                double temp = XE(j);
-                for (int i = 0; i < j; ++i) temp -= tget(t, i, j) * XE(i);
-                if (nounit) temp /= tget(t, j, j);
-                XE(j) = temp;
-            }
-        } else {
-            for (int j = n - 1; j >= 0; --j) {
-                double temp = XE(j);
-                for (int i = n - 1; i > j; --i) temp -= tget(t, i, j) * XE(i);
-                if (nounit) temp /= tget(t, j, j);
-                XE(j) = temp;
-            }
+            double temp = 0;
+            for (int i = i0; i <= i1; ++i) temp += A[IX(ku + i - j, j, lda)] * x[kx + i * incx];
+            y[ky + j * incy] += alpha * temp;
         }
     }
-#undef XE
+}
+
+/* INFO of netlib DSBMV: 1 uplo, 2 n, 3 k, 6 lda, 8 incx, 11 incy */
+int ora_sbmv_info(char uplo, int n, int k, int lda, int incx, int incy)
+{
+    if (!ora_lsame(uplo, 'U') && !ora_lsame(uplo, 'L')) return 1;
+    if (n < 0) return 2;
+    if (k < 0) return 3;
+    if (lda < k + 1) return 6;
+    if (incx == 0) return 8;
+    if (incy == 0) return 11;
+    return 0;
+}
+
+/* netlib DSBMV: y := alpha*A*x + beta*y, symmetric band A (blas_level2/sbmv.cc) */
+void ora_dsbmv(char uplo, int n, int k, double alpha, const double *A, int lda, const double *x,
+               int incx, double beta, double *y, int incy)
+{
+    if (ora_sbmv_info(uplo, n, k, lda, incx, incy)) return;
+    if (n == 0 || (alpha == 0 && beta == 1)) return;
+    const int kx = incx > 0 ? 0 : -(n - 1) * incx, ky = incy > 0 ? 0 : -(n - 1) * incy;
+    if (beta != 1)
+        for (int i = 0; i < n; ++i) y[ky + i * incy] = beta == 0 ? 0.0 : beta * y[ky + i * incy];
+    if (alpha == 0) return;
+    const struct tri_acc t = {A, lda, ora_lsame(uplo, 'U'), n, k};
+    for (int j = 0; j < n; ++j) {
+        const double temp1 = alpha * x[kx + j * incx];
+        double temp2 = 0;
+        if (t.upper) {
+            for (int i = blo(&t, j); i < j; ++i) {
+                const double a = tget(&t, i, j);
+                y[ky + i * incy] += temp1 * a;
+                temp2 += a * x[kx + i * incx];
+            }
+            y[ky + j * incy] += temp1 * tget(&t, j, j) + alpha * temp2;
+        } else {
+            y[ky + j * incy] += temp1 * tget(&t, j, j);
+            for (int i = j + 1; i <= bhi(&t, j); ++i) {
+                const double a = tget(&t, i, j);
+                y[ky + i * incy] += temp1 * a;
+                temp2 += a * x[kx + i * incx];
+            }
+            y[ky + j * incy] += alpha * temp2;
+        }
+    }
 }
 
 /* netlib DSPMV: y := alpha*A*x + beta*y, packed symmetric A (blas_level2/pmv.cc) */

@@ -47,6 +47,9 @@ int ora_spr2_info(char uplo, int n, int incx, int incy);
 int ora_trmv_info(char uplo, char trans, char diag, int n, int lda, int incx);
 int ora_tpmv_info(char uplo, char trans, char diag, int n, int incx); /* also DTPSV */
 int ora_spmv_info(char uplo, int n, int incx, int incy);
+int ora_tbmv_info(char uplo, char trans, char diag, int n, int k, int lda, int incx); /* + DTBSV */
+int ora_gbmv_info(char trans, int m, int n, int kl, int ku, int lda, int incx, int incy);
+int ora_sbmv_info(char uplo, int n, int k, int lda, int incx, int incy);
 
 /* ---- Level 3 ---- */
 void ora_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
@@ -92,6 +95,14 @@ void ora_dtpmv(char uplo, char trans, char diag, int n, const double *AP, double
 void ora_dtpsv(char uplo, char trans, char diag, int n, const double *AP, double *x, int incx);
 void ora_dspmv(char uplo, int n, double alpha, const double *AP, const double *x, int incx,
                double beta, double *y, int incy);
+void ora_dtbmv(char uplo, char trans, char diag, int n, int k, const double *A, int lda,
+               double *x, int incx);
+void ora_dtbsv(char uplo, char trans, char diag, int n, int k, const double *A, int lda,
+               double *x, int incx);
+void ora_dgbmv(char trans, int m, int n, int kl, int ku, double alpha, const double *A, int lda,
+               const double *x, int incx, double beta, double *y, int incy);
+void ora_dsbmv(char uplo, int n, int k, double alpha, const double *A, int lda, const double *x,
+               int incx, double beta, double *y, int incy);
 void ora_dspr2(char uplo, int n, double alpha, const double *x, int incx, const double *y,
                int incy, double *AP);
 void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);

@@ -43,6 +43,13 @@ def load():
     L.ora_dtpmv.argtypes = [C, C, C, I, VP, VP, I]
     L.ora_dtpsv.argtypes = [C, C, C, I, VP, VP, I]
     L.ora_dspmv.argtypes = [C, I, D, VP, VP, I, D, VP, I]
+    L.ora_tbmv_info.argtypes = [C, C, C, I, I, I, I]
+    L.ora_gbmv_info.argtypes = [C, I, I, I, I, I, I, I]
+    L.ora_sbmv_info.argtypes = [C, I, I, I, I, I]
+    L.ora_dtbmv.argtypes = [C, C, C, I, I, VP, I, VP, I]
+    L.ora_dtbsv.argtypes = [C, C, C, I, I, VP, I, VP, I]
+    L.ora_dgbmv.argtypes = [C, I, I, I, I, D, VP, I, VP, I, D, VP, I]
+    L.ora_dsbmv.argtypes = [C, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_dgemm.argtypes = [C, C, I, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]

@@ -422,3 +422,85 @@ def test_triangular_and_packed_mv_xerbla_and_fall_through(ora):
     gb.dspmv("L", n, 0.5, APl, x0, 1, 2.0, y, 1)
     assert np.array_equal(y, sb.dspmv(n, 0.5, APl, x0, beta=2.0, y=y0.copy(), lower=1))
     assert gb.stats()["fallthrough"] == 4
+
+
+def _band_of(D, kl, ku, lda):
+    """netlib column-major band storage of D (entries outside the band: 0)."""
+    m, n = D.shape
+    AB = np.zeros((lda, n), order="F")
+    for j in range(n):
+        for i in range(max(0, j - ku), min(m, j + kl + 1)):
+            AB[ku + i - j, j] = D[i, j]
+    return AB
+
+
+def test_band_routines_xerbla_and_fall_through(ora):
+    """DGBMV / DSBMV / DTBMV / DTBSV: netlib INFO codes and their CBLAS
+    positions (row-major swaps m/n and kl/ku), the host fall-through
+    bit-identical to scipy's OpenBLAS, and the row-major CBLAS mappings
+    checked against dense closed forms."""
+    lib = gb.library()
+    d = np.zeros(64)
+    gb.dgbmv("N", 2, 2, 1, 1, 1.0, d, 2, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DGBMV ", ora.ora_gbmv_info(b"N", 2, 2, 1, 1, 2, 1, 1))
+    gb.dgbmv("N", 2, 2, 0, -1, 1.0, d, 2, d, 1, 0.0, d, 1)
+    assert gb.last_xerbla() == ("DGBMV ", 5)
+    gb.dsbmv("U", 2, 1, 1.0, d, 2, d, 1, 0.0, d, 0)
+    assert gb.last_xerbla() == ("DSBMV ", ora.ora_sbmv_info(b"U", 2, 1, 2, 1, 0))
+    gb.dtbmv("L", "N", "N", 2, 2, d, 2, d, 1)
+    assert gb.last_xerbla() == ("DTBMV ", ora.ora_tbmv_info(b"L", b"N", b"N", 2, 2, 2, 1))
+    gb.dtbsv("L", "N", "N", 2, 1, d, 2, d, 0)
+    assert gb.last_xerbla() == ("DTBSV ", 9)
+    dd = ctypes.c_double
+    lib.cblas_dgbmv(101, 111, 2, 3, 1, 0, dd(1.0), p(d), 1, p(d), 1, dd(0.0), p(d), 1)
+    assert gb.last_xerbla() == ("DGBMV ", 9)  # lda < kl + ku + 1: CBLAS parameter 9
+    lib.cblas_dgbmv(101, 111, 2, 3, -1, 0, dd(1.0), p(d), 2, p(d), 1, dd(0.0), p(d), 1)
+    assert gb.last_xerbla() == ("DGBMV ", 5)  # row-major: kl stays CBLAS parameter 5
+    lib.cblas_dgbmv(101, 111, -1, 3, 0, 0, dd(1.0), p(d), 2, p(d), 1, dd(0.0), p(d), 1)
+    assert gb.last_xerbla() == ("DGBMV ", 3)
+    lib.cblas_dsbmv(102, 121, 2, 3, dd(1.0), p(d), 3, p(d), 1, dd(0.0), p(d), 1)
+    assert gb.last_xerbla() == ("DSBMV ", 7)
+    lib.cblas_dtbsv(102, 121, 111, 131, 2, 0, p(d), 1, p(d), 0)
+    assert gb.last_xerbla() == ("DTBSV ", 10)
+
+    gb.set_cutoff(512)
+    rng = np.random.default_rng(85)
+    m, n, kl, ku = 30, 25, 3, 2
+    D = rng.standard_normal((m, n))
+    D[np.subtract.outer(np.arange(m), np.arange(n)) > kl] = 0
+    D[np.subtract.outer(np.arange(n), np.arange(m)).T > ku] = 0
+    AB = _band_of(D, kl, ku, kl + ku + 1)
+    x, y0 = rng.standard_normal(n), rng.standard_normal(m)
+    gb.reset_stats()
+    y = y0.copy()
+    gb.dgbmv("N", m, n, kl, ku, 0.5, AB, kl + ku + 1, x, 1, 2.0, y, 1)
+    assert np.array_equal(y, sb.dgbmv(m, n, kl, ku, 0.5, AB, x, beta=2.0, y=y0.copy()))
+    # row-major: the band array read row by row holds D' band-stored column-major,
+    # i.e. the row-major band storage of D' (n x m, kl' = ku, ku' = kl)
+    y = np.zeros(n)
+    lib.cblas_dgbmv(101, 111, n, m, ku, kl, dd(1.0), p(AB), kl + ku + 1, p(y0), 1, dd(0.0),
+                    p(y), 1)
+    np.testing.assert_allclose(y, D.T @ y0, rtol=1e-13, atol=1e-13)
+    k = 3
+    S = rng.standard_normal((n, n))
+    S = np.triu(S) - np.triu(S, k + 1)
+    AS = _band_of(S, 0, k, k + 1)  # upper band
+    y = y0[:n].copy()
+    gb.dsbmv("U", n, k, 0.75, AS, k + 1, x, 1, -1.0, y, 1)
+    assert np.array_equal(y, sb.dsbmv(k, 0.75, AS, x, beta=-1.0, y=y0[:n].copy()))
+    y = np.zeros(n)
+    lib.cblas_dsbmv(101, 122, n, k, dd(1.0), p(AS), k + 1, p(x), 1, dd(0.0), p(y), 1)
+    Sym = S + S.T - np.diag(np.diag(S))
+    np.testing.assert_allclose(y, Sym @ x, rtol=1e-13, atol=1e-13)  # row-major lower = same bytes
+    T = S + np.diag(np.full(n, 3.0))
+    AT = _band_of(T, 0, k, k + 1)
+    xt = x.copy()
+    gb.dtbmv("U", "T", "N", n, k, AT, k + 1, xt, 1)
+    assert np.array_equal(xt, sb.dtbmv(k, AT, x.copy(), trans=1))
+    xt = x.copy()
+    gb.dtbsv("U", "N", "N", n, k, AT, k + 1, xt, 1)
+    assert np.array_equal(xt, sb.dtbsv(k, AT, x.copy()))
+    assert gb.stats()["fallthrough"] == 6  # the two row-major CBLAS calls included
+    xt = x.copy()  # row-major lower band triangle = T' column-major upper: solve T' z = x
+    lib.cblas_dtbsv(101, 122, 111, 131, n, k, p(AT), k + 1, p(xt), 1)
+    np.testing.assert_allclose(T.T @ xt, x, rtol=1e-12, atol=1e-12)

@@ -513,3 +513,169 @@ def test_dspmv_alpha_zero_and_integer_exact(gpu, ora):
     y = y0.copy()
     gpu.dspmv("U", n, 0.0, AP, x, 1, 0.5, y, 1)  # alpha = 0: y := beta y, AP unread
     assert np.array_equal(y, 0.5 * y0)
+
+
+def _band_array(m, n, kl, ku, lda, seed, fill=np.nan):
+    """Band storage (lda x n) with random band entries; the corners of the
+    band array outside the matrix and the rows beyond kl + ku hold `fill`."""
+    AB = np.full((lda, n), fill, order="F")
+    vals = uniform(seed, (kl + ku + 1, n))
+    for j in range(n):
+        i0, i1 = max(0, j - ku), min(m, j + kl + 1)
+        if i1 > i0:
+            AB[ku + i0 - j: ku + i1 - j, j] = vals[ku + i0 - j: ku + i1 - j, j]
+    return AB
+
+
+def _gb_abs_product(AB, m, n, kl, ku, v, trans):
+    """sum |A(i,j)| |v| per output element (the error-bound scale)."""
+    A = np.where(np.isnan(AB), 0.0, np.abs(AB))
+    out = np.zeros(n if trans else m)
+    for r in range(kl + ku + 1):
+        j = np.arange(n)
+        i = j - ku + r
+        ok = (i >= 0) & (i < m)
+        if trans:
+            np.add.at(out, j[ok], A[r, j[ok]] * np.abs(v[i[ok]]))
+        else:
+            np.add.at(out, i[ok], A[r, j[ok]] * np.abs(v[j[ok]]))
+    return out
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m,n,kl,ku,incx,incy", [
+    (1, 1, 0, 0, 1, 1), (2000, 2000, 5, 5, 1, 1),       # one chunk: direct write
+    (5000, 3000, 0, 40, 2, -1), (3000, 5000, 70, 0, -3, 2),
+    (257, 300, 300, 2, 1, 1),                            # kl >= m
+    (300000, 300000, 8, 8, 1, 1),                        # two chunks per workgroup
+    (100000, 90000, 16, 16, 1, 1)])                      # chunks split, partial fold
+def test_dgbmv(gpu, ora, trans, m, n, kl, ku, incx, incy):
+    """Band corners and padding rows hold NaN (never read); bound
+    2 w eps sum|terms|, w = kl + ku + 1; stride gaps of y untouched."""
+    lda = kl + ku + 2
+    AB = _band_array(m, n, kl, ku, lda, 101)
+    lx, ly = (m, n) if trans == "T" else (n, m)
+    x = _strided(uniform(102, (lx,)), incx)
+    y0 = _strided(uniform(103, (ly,)), incy)
+    y, r = y0.copy(), y0.copy()
+    gpu.dgbmv(trans, m, n, kl, ku, 0.5, AB, lda, x, incx, -1.5, y, incy)
+    ora.ora_dgbmv(c(trans), m, n, kl, ku, 0.5, p(AB), lda, p(x), incx, -1.5, p(r), incy)
+    mag = 0.5 * _gb_abs_product(AB, m, n, kl, ku, _strided_get(x, lx, incx), trans == "T") \
+        + 1.5 * np.abs(_strided_get(y0, ly, incy))
+    w = kl + ku + 1
+    err = np.abs(_strided_get(y, ly, incy) - _strided_get(r, ly, incy))
+    assert np.all(err <= 2 * w * EPS * mag + 1e-300)
+    gaps = np.ones(len(y), bool)
+    gaps[:: abs(incy)][:ly] = False
+    assert np.array_equal(y[gaps], y0[gaps])
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dgbmv_integer_exact_and_alpha_zero(gpu, ora, trans):
+    m, n, kl, ku = 70000, 65000, 20, 11
+    rng = np.random.default_rng(104)
+    AB = np.asfortranarray(rng.integers(-9, 10, (kl + ku + 1, n)).astype(float))
+    lx, ly = (m, n) if trans == "T" else (n, m)
+    x = rng.integers(-9, 10, lx).astype(float)
+    y0 = rng.integers(-9, 10, ly).astype(float)
+    y, r = y0.copy(), y0.copy()
+    gpu.dgbmv(trans, m, n, kl, ku, 2.0, AB, kl + ku + 1, x, 1, 3.0, y, 1)
+    ora.ora_dgbmv(c(trans), m, n, kl, ku, 2.0, p(AB), kl + ku + 1, p(x), 1, 3.0, p(r), 1)
+    assert np.array_equal(y, r)
+    y = y0.copy()
+    gpu.dgbmv(trans, m, n, kl, ku, 0.0, AB, kl + ku + 1, x, 1, 0.5, y, 1)
+    assert np.array_equal(y, 0.5 * y0)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,k,incx,incy", [(1, 0, 1, 1), (1000, 7, 1, 1), (5000, 100, -2, 3),
+                                           (70000, 20, 1, 1), (300, 400, 1, -1)])
+def test_dsbmv(gpu, ora, uplo, n, k, incx, incy):
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = _band_array(n, n, kl, ku, k + 1, 105)
+    x = _strided(uniform(106, (n,)), incx)
+    y0 = _strided(uniform(107, (n,)), incy)
+    y, r = y0.copy(), y0.copy()
+    gpu.dsbmv(uplo, n, k, 0.75, AB, k + 1, x, incx, 0.25, y, incy)
+    ora.ora_dsbmv(c(uplo), n, k, 0.75, p(AB), k + 1, p(x), incx, 0.25, p(r), incy)
+    xs = _strided_get(x, n, incx)
+    mag = 0.75 * (_gb_abs_product(AB, n, n, kl, ku, xs, False)
+                  + _gb_abs_product(AB, n, n, kl, ku, xs, True)) \
+        + 0.25 * np.abs(_strided_get(y0, n, incy))
+    err = np.abs(_strided_get(y, n, incy) - _strided_get(r, n, incy))
+    assert np.all(err <= 4 * (k + 1) * EPS * mag + 1e-300)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("n,k,incx", [(1, 0, 1), (1000, 5, 1), (3000, 64, -2), (20000, 200, 1)])
+def test_dtbmv(gpu, ora, uplo, trans, diag, n, k, incx):
+    """A unit diagonal's band row holds NaN (not read); bound 2 (k+1) eps
+    sum|terms|."""
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = _band_array(n, n, kl, ku, k + 1, 108)
+    if diag == "U":
+        AB[ku, :] = np.nan
+    x0 = _strided(uniform(109, (n,)), incx)
+    x, r = x0.copy(), x0.copy()
+    gpu.dtbmv(uplo, trans, diag, n, k, AB, k + 1, x, incx)
+    ora.ora_dtbmv(c(uplo), c(trans), c(diag), n, k, p(AB), k + 1, p(r), incx)
+    A1 = AB.copy(order="F")
+    if diag == "U":
+        A1[ku, :] = 1.0
+    mag = _gb_abs_product(A1, n, n, kl, ku, _strided_get(x0, n, incx), trans == "T")
+    err = np.abs(_strided_get(x, n, incx) - _strided_get(r, n, incx))
+    assert np.all(err <= 2 * (k + 1) * EPS * mag + 1e-300)
+    gaps = np.ones(len(x), bool)
+    gaps[:: abs(incx)][:n] = False
+    assert np.array_equal(x[gaps], x0[gaps])
+
+
+TBSV_CASES = [(1, 0, 1), (100, 1, 1), (5000, 5, 1), (3001, 63, -2),    # register window
+              (3000, 64, 1), (4000, 100, 3), (2500, 700, 1),           # LDS window, prefetched
+              (3000, 1500, 1)]                                          # LDS window, k >= 1024
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k,incx", TBSV_CASES)
+def test_dtbsv_integer_exact(gpu, ora, uplo, trans, n, k, incx):
+    """b = T x with small integers and a diagonal of ones (unit or stored):
+    every partial sum of the solve is an integer, so the solution must come
+    back exactly; the oracle's solve too.  Band corners hold NaN."""
+    rng = np.random.default_rng(110 + k)
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = np.full((k + 1, n), np.nan, order="F")
+    for j in range(n):
+        i0, i1 = max(0, j - ku), min(n, j + kl + 1)
+        AB[ku + i0 - j: ku + i1 - j, j] = rng.integers(-1, 2, i1 - i0)
+    for diag in "NU":
+        A = AB.copy(order="F")
+        A[ku, :] = 1.0 if diag == "N" else np.nan
+        xt = rng.integers(-3, 4, n).astype(float)
+        # b = op(T) xt through the oracle's DTBMV (exact on these integers)
+        b0 = _strided(xt, incx)
+        ora.ora_dtbmv(c(uplo), c(trans), c(diag), n, k, p(A), k + 1, p(b0), incx)
+        assert np.abs(b0).max() < 2 ** 40
+        x, r = b0.copy(), b0.copy()
+        gpu.dtbsv(uplo, trans, diag, n, k, A, k + 1, x, incx)
+        ora.ora_dtbsv(c(uplo), c(trans), c(diag), n, k, p(A), k + 1, p(r), incx)
+        assert np.array_equal(_strided_get(r, n, incx), xt)
+        assert np.array_equal(_strided_get(x, n, incx), xt), (diag, np.abs(_strided_get(x, n, incx) - xt).max())
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k", [(4000, 10), (3000, 90), (2000, 1100)])
+def test_dtbsv_random(gpu, ora, uplo, trans, n, k):
+    """Diagonally dominant random band (diagonal in [2, 3], off-diagonal
+    scaled by 1 / (k + 1)): relative agreement with the oracle to 1e-12."""
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = _band_array(n, n, kl, ku, k + 1, 111) * (1.0 / (k + 1))
+    AB[ku, :] = 2.5 + 0.5 * uniform(112, (n,))
+    x0 = uniform(113, (n,))
+    x, r = x0.copy(), x0.copy()
+    gpu.dtbsv(uplo, trans, "N", n, k, AB, k + 1, x, 1)
+    ora.ora_dtbsv(c(uplo), c(trans), b"N", n, k, p(AB), k + 1, p(r), 1)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)

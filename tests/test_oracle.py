@@ -572,3 +572,137 @@ def test_trmv_tpmv_spmv_info_codes(L):
     assert [L.ora_tpmv_info(b"U", b"T", b"U", -1, 1), L.ora_tpmv_info(b"U", b"T", b"U", 2, 0)] == [4, 7]
     assert [L.ora_spmv_info(b"X", 2, 1, 1), L.ora_spmv_info(b"U", -1, 1, 1),
             L.ora_spmv_info(b"U", 2, 0, 1), L.ora_spmv_info(b"U", 2, 1, 0)] == [1, 2, 6, 9]
+
+
+def _gb_dense(AB, m, n, kl, ku):
+    """Dense m x n matrix of netlib general band storage A(ku+i-j, j)."""
+    D = np.zeros((m, n))
+    for j in range(n):
+        for i in range(max(0, j - ku), min(m, j + kl + 1)):
+            D[i, j] = AB[ku + i - j, j]
+    return D
+
+
+def _band_storage(D, kl, ku, lda, fill=np.nan):
+    """netlib band storage of the band of D; entries outside the matrix (the
+    corners of the band array) and rows beyond kl + ku hold `fill`."""
+    m, n = D.shape
+    AB = np.full((lda, n), fill, order="F")
+    for j in range(n):
+        for i in range(max(0, j - ku), min(m, j + kl + 1)):
+            AB[ku + i - j, j] = D[i, j]
+    return AB
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m,n,kl,ku", [(9, 9, 2, 1), (12, 7, 0, 3), (6, 11, 4, 0), (5, 5, 7, 6),
+                                       (1, 4, 1, 2)])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (-2, 3)])
+def test_dgbmv_vs_dense_and_host_blas(L, trans, m, n, kl, ku, incx, incy):
+    """netlib DGBMV equals alpha*op(D)*x + beta*y for the dense D of the band
+    (the band array's unreferenced corners hold NaN and are never read), and
+    scipy's OpenBLAS on the same band with finite corners."""
+    D = np.zeros((m, n))
+    for j in range(n):
+        for i in range(max(0, j - ku), min(m, j + kl + 1)):
+            D[i, j] = uniform(120 + i, (n,))[j]
+    lda = kl + ku + 2
+    AB = _band_storage(D, kl, ku, lda)
+    assert np.array_equal(_gb_dense(AB, m, n, kl, ku), D)
+    lx, ly = (n, m) if trans == "N" else (m, n)
+    xs, ys = uniform(121, (lx,)), uniform(122, (ly,))
+    x = np.full(1 + (lx - 1) * abs(incx), 9.0)
+    x[:: abs(incx)] = xs if incx > 0 else xs[::-1]
+    y0 = np.full(1 + (ly - 1) * abs(incy), 9.0)
+    y0[:: abs(incy)] = ys if incy > 0 else ys[::-1]
+    R = y0.copy()
+    L.ora_dgbmv(c(trans), m, n, kl, ku, 0.6, p(AB), lda, p(x), incx, -1.25, p(R), incy)
+    Op = D if trans == "N" else D.T
+    want = 0.6 * Op @ xs - 1.25 * ys
+    got = R[:: abs(incy)][:ly]
+    got = got if incy > 0 else got[::-1]
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14)
+    gaps = np.ones(len(R), bool)
+    gaps[:: abs(incy)][:ly] = False
+    assert np.all(R[gaps] == 9.0)
+    if incx == 1 and incy == 1 and m >= kl + ku + 1:  # scipy's wrapper requires the latter
+        ABf = np.where(np.isnan(AB), 0.0, AB)
+        H = sb.dgbmv(m, n, kl, ku, 0.6, ABf, x, beta=-1.25, y=y0.copy(), trans=int(trans == "T"))
+        np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("n,k", [(10, 3), (6, 0), (5, 8), (1, 2)])
+def test_dsbmv_vs_dense_and_host_blas(L, uplo, n, k):
+    """netlib DSBMV: the symmetric matrix of the stored band triangle times x;
+    scipy's OpenBLAS agrees."""
+    A = uniform(123, (n, n))
+    S = np.triu(A) if uplo == "U" else np.tril(A)
+    S = S + S.T - np.diag(np.diag(A))
+    S[np.abs(np.subtract.outer(np.arange(n), np.arange(n))) > k] = 0.0
+    T = np.triu(S) if uplo == "U" else np.tril(S)
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = _band_storage(T, kl, ku, k + 1)
+    x, y0 = uniform(124, (n,)), uniform(125, (n,))
+    R = y0.copy()
+    L.ora_dsbmv(c(uplo), n, k, 0.8, p(AB), k + 1, p(x), 1, 0.5, p(R), 1)
+    np.testing.assert_allclose(R, 0.8 * S @ x + 0.5 * y0, rtol=1e-14, atol=1e-14)
+    H = sb.dsbmv(k, 0.8, np.where(np.isnan(AB), 0.0, AB), x, beta=0.5, y=y0.copy(),
+                 lower=int(uplo == "L"))
+    np.testing.assert_allclose(R, H, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+@pytest.mark.parametrize("n,k,incx", [(11, 3, 1), (7, 0, 1), (6, 9, -2), (1, 1, 1)])
+def test_dtbmv_dtbsv_vs_dense_and_host_blas(L, uplo, trans, diag, n, k, incx):
+    """netlib DTBMV / DTBSV: op(T) x and the solve of op(T) x = b with the
+    dense band triangle T (a unit diagonal's band slots hold NaN and are not
+    read); the solve is checked on integer data whose solution is exact, and
+    both against scipy's OpenBLAS."""
+    rng = np.random.default_rng(126)
+    T = rng.integers(-3, 4, (n, n)).astype(float)
+    T = np.triu(T) if uplo == "U" else np.tril(T)
+    T[np.abs(np.subtract.outer(np.arange(n), np.arange(n))) > k] = 0.0
+    T[np.diag_indices(n)] = 1.0 if diag == "U" else 2.0
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = _band_storage(T, kl, ku, k + 1)
+    if diag == "U":
+        AB[ku, :] = np.nan
+    Op = T if trans == "N" else T.T
+    xs = rng.integers(-5, 6, n).astype(float)
+    x = np.full(1 + (n - 1) * abs(incx), 9.0)
+    x[:: abs(incx)] = xs if incx > 0 else xs[::-1]
+
+    def get(v):
+        g = v[:: abs(incx)][:n]
+        return g if incx > 0 else g[::-1]
+
+    R = x.copy()
+    L.ora_dtbmv(c(uplo), c(trans), c(diag), n, k, p(AB), k + 1, p(R), incx)
+    assert np.array_equal(get(R), Op @ xs)  # small integers: exact
+    b = x.copy()
+    b[:: abs(incx)] = (Op @ xs) if incx > 0 else (Op @ xs)[::-1]
+    L.ora_dtbsv(c(uplo), c(trans), c(diag), n, k, p(AB), k + 1, p(b), incx)
+    np.testing.assert_allclose(get(b), xs, rtol=1e-13, atol=1e-13)
+    if incx == 1:
+        ABf = np.where(np.isnan(AB), 1.0, AB)
+        kw = dict(lower=int(uplo == "L"), trans=int(trans == "T"), diag=int(diag == "U"))
+        assert np.array_equal(sb.dtbmv(k, ABf, xs.copy(), **kw), get(R))
+        np.testing.assert_allclose(sb.dtbsv(k, ABf, Op @ xs, **kw), get(b), rtol=1e-14, atol=1e-14)
+
+
+def test_band_info_codes(L):
+    assert [L.ora_gbmv_info(b"X", 1, 1, 0, 0, 1, 1, 1), L.ora_gbmv_info(b"N", -1, 1, 0, 0, 1, 1, 1),
+            L.ora_gbmv_info(b"N", 1, -1, 0, 0, 1, 1, 1), L.ora_gbmv_info(b"N", 1, 1, -1, 0, 1, 1, 1),
+            L.ora_gbmv_info(b"N", 1, 1, 0, -1, 1, 1, 1), L.ora_gbmv_info(b"N", 1, 1, 1, 1, 2, 1, 1),
+            L.ora_gbmv_info(b"N", 1, 1, 0, 0, 1, 0, 1), L.ora_gbmv_info(b"N", 1, 1, 0, 0, 1, 1, 0)] \
+        == [1, 2, 3, 4, 5, 8, 10, 13]
+    assert [L.ora_sbmv_info(b"X", 1, 0, 1, 1, 1), L.ora_sbmv_info(b"U", -1, 0, 1, 1, 1),
+            L.ora_sbmv_info(b"U", 1, -1, 1, 1, 1), L.ora_sbmv_info(b"U", 1, 1, 1, 1, 1),
+            L.ora_sbmv_info(b"U", 1, 0, 1, 0, 1), L.ora_sbmv_info(b"U", 1, 0, 1, 1, 0)] \
+        == [1, 2, 3, 6, 8, 11]
+    assert [L.ora_tbmv_info(b"U", b"N", b"N", -1, 0, 1, 1), L.ora_tbmv_info(b"U", b"N", b"N", 1, -1, 1, 1),
+            L.ora_tbmv_info(b"U", b"N", b"N", 1, 2, 2, 1), L.ora_tbmv_info(b"U", b"N", b"N", 1, 0, 1, 0)] \
+        == [4, 5, 7, 9]
