@@ -30,6 +30,8 @@
  *                      not the reference's void macro blas.h:45-51)
  *       dnrm2_         blas_level1/nrm2.cc:37-41 (returns double)
  *       idamax_        blas_level1/amax.cc:38-43 (returns the 1-based index)
+ *       idamin_ drotm_ blas_level1/amin.cc:37-42, rotm.cc:51-56 (IDAMIN: the extension
+ *                      cublasIdamin implements; DROTM: netlib)
  *       dscal_ dcopy_ dswap_ drot_ dasum_  netlib definitions behind the
  *                      reference's CBLAS specs blas_level1/scal.cc:42,
  *                      copy.cc:46, swap.cc:12-29, rot.cc:50, asum.cc:48 (dasum_
@@ -132,6 +134,9 @@ void daxpy_(const int *n, const double *alpha, const double *x, const int *incx,
 double ddot_(const int *n, const double *x, const int *incx, const double *y, const int *incy);
 double dnrm2_(const int *n, const double *x, const int *incx);
 int idamax_(const int *n, const double *x, const int *incx);
+int idamin_(const int *n, const double *x, const int *incx);
+void drotm_(const int *n, double *x, const int *incx, double *y, const int *incy,
+            const double *param);
 void dscal_(const int *n, const double *alpha, double *x, const int *incx);
 void dcopy_(const int *n, const double *x, const int *incx, double *y, const int *incy);
 void dswap_(const int *n, double *x, const int *incx, double *y, const int *incy);
@@ -205,6 +210,8 @@ void cblas_daxpy(int n, double alpha, const double *x, int incx, double *y, int 
 double cblas_ddot(int n, const double *x, int incx, const double *y, int incy);
 double cblas_dnrm2(int n, const double *x, int incx);
 int cblas_idamax(int n, const double *x, int incx);
+int cblas_idamin(int n, const double *x, int incx);
+void cblas_drotm(int n, double *x, int incx, double *y, int incy, const double *param);
 void cblas_dscal(int n, double alpha, double *x, int incx);
 void cblas_dcopy(int n, const double *x, int incx, double *y, int incy);
 void cblas_dswap(int n, double *x, int incx, double *y, int incy);

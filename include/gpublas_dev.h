@@ -199,6 +199,11 @@ int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const dou
                      int incy, double *h_result);
 int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *h_result);
 int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result);
+/* IDAMIN (replaces cublasIdamin, blas_level1/amin.cc:40) and DROTM (replaces
+ * cublasDrotm, rotm.cc:51; h_param = flag, h11, h21, h12, h22 on the host). */
+int gpublas_dev_idamin(void *stream, int n, const double *d_x, int incx, int *h_result);
+int gpublas_dev_drotm(void *stream, int n, double *d_x, int incx, double *d_y, int incy,
+                      const double *h_param);
 /* Pinned result slot of the calling thread (256 bytes). */
 void *gpublas_dev_result_slot(void);
 

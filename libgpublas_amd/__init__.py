@@ -82,6 +82,9 @@ def _declare(lib):
     lib.ddot_.restype = cd
     lib.dnrm2_.restype = cd
     lib.idamax_.restype = ci
+    lib.idamin_.restype = ci
+    lib.cblas_idamin.restype = ci
+    lib.cblas_idamin.argtypes = [ci, vp, ci]
     lib.dasum_.restype = cd
     lib.cblas_dasum.restype = cd
     lib.cblas_dasum.argtypes = [ci, vp, ci]
@@ -319,6 +322,14 @@ def dnrm2(n, x, incx):
 
 def idamax(n, x, incx):
     return library().idamax_(_i(n), _p(x), _i(incx))
+
+
+def idamin(n, x, incx):
+    return library().idamin_(_i(n), _p(x), _i(incx))
+
+
+def drotm(n, x, incx, y, incy, param):
+    library().drotm_(_i(n), _p(x), _i(incx), _p(y), _i(incy), _p(param))
 
 
 # ---------------------------------------------------------------------------

@@ -237,12 +237,40 @@ struct Cand {
     long i;
 };
 
+// IDAMAX (MIN = false) and IDAMIN (MIN = true): the candidate with the
+// larger (smaller) |x_i|, ties to the smaller index.  A NaN element enters as
+// -1 for the maximum and +inf for the minimum, so it is never preferred; the
+// netlib loop's NaN-at-position-1 rule is applied at the end.
+template <bool MIN>
 __device__ __forceinline__ Cand better(Cand a, Cand b)
 {
-    if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+    if ((MIN ? b.v < a.v : b.v > a.v) || (b.v == a.v && b.i < a.i)) return b;
     return a;
 }
 
+template <bool MIN>
+__device__ __forceinline__ Cand cand_of(long i, double v)
+{
+    double a = fabs(v);
+    if (isnan(a)) a = MIN ? __builtin_inf() : -1.0;
+    return Cand{a, i};
+}
+
+template <bool MIN>
+__device__ __forceinline__ Cand cand_none()
+{
+    return Cand{MIN ? __builtin_inf() : -1.0, 0x7fffffffffffffffL};
+}
+
+template <bool MIN>
+__device__ __forceinline__ long cand_result(Cand t, const double *x)
+{
+    long r = t.i + 1;
+    if (isnan(x[0]) || (!MIN && t.v < 0)) r = 1;
+    return r;
+}
+
+template <bool MIN>
 __device__ __forceinline__ Cand wave_best(Cand c)
 {
 #pragma unroll
@@ -250,52 +278,48 @@ __device__ __forceinline__ Cand wave_best(Cand c)
         Cand d;
         d.v = __shfl_xor(c.v, o, 64);
         d.i = __shfl_xor(c.i, o, 64);
-        c = better(c, d);
+        c = better<MIN>(c, d);
     }
     return c;
 }
 
+template <bool MIN>
 __device__ Cand block_best(Cand c, Cand *red)
 {
-    c = wave_best(c);
+    c = wave_best<MIN>(c);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = c;
     __syncthreads();
     Cand r = red[0];
-    r = better(r, red[1]);
-    r = better(r, red[2]);
-    r = better(r, red[3]);
+    r = better<MIN>(r, red[1]);
+    r = better<MIN>(r, red[2]);
+    r = better<MIN>(r, red[3]);
     __syncthreads();
     return r;
 }
 
+template <bool MIN>
 __global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long incx,
                                                     double *partials, unsigned *ticket,
                                                     int *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ Cand red[4];
-    Cand c{-1.0, 0x7fffffffffffffffL};
+    Cand c = cand_none<MIN>();
     long i = (long)blockIdx.x * 256 + threadIdx.x;
     const long stride = (long)gridDim.x * 256;
-    for (; i < n; i += stride) {
-        double a = fabs(x[i * incx]);
-        if (isnan(a)) a = -1.0;
-        c = better(c, Cand{a, i});
-    }
-    c = block_best(c, red);
+    for (; i < n; i += stride) c = better<MIN>(c, cand_of<MIN>(i, x[i * incx]));
+    c = block_best<MIN>(c, red);
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = c.v;
         ((long *)partials)[2 * blockIdx.x + 1] = c.i;
     }
     if (!last_block(ticket)) return;
-    Cand t{-1.0, 0x7fffffffffffffffL};
+    Cand t = cand_none<MIN>();
     for (int b = threadIdx.x; b < (int)gridDim.x; b += 256)
-        t = better(t, Cand{partials[2 * b], ((long *)partials)[2 * b + 1]});
-    t = block_best(t, red);
+        t = better<MIN>(t, Cand{partials[2 * b], ((long *)partials)[2 * b + 1]});
+    t = block_best<MIN>(t, red);
     if (threadIdx.x == 0) {
-        long r = t.i + 1;
-        if (isnan(x[0]) || t.v < 0) r = 1;
-        *result = (int)r;
+        *result = (int)cand_result<MIN>(t, x);
         publish(flag, seq);
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -402,16 +426,13 @@ __global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long 
     }
 }
 
+template <bool MIN>
 __global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long incx, int vec,
                                                     int *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ Cand red[16];
-    Cand c{-1.0, 0x7fffffffffffffffL};
-    auto take = [&](long i, double v) {
-        double a = fabs(v);
-        if (isnan(a)) a = -1.0;
-        c = better(c, Cand{a, i});
-    };
+    Cand c = cand_none<MIN>();
+    auto take = [&](long i, double v) { c = better<MIN>(c, cand_of<MIN>(i, v)); };
     if (vec) {
         int i = 2 * (int)threadIdx.x;
         for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
@@ -431,17 +452,15 @@ __global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long
     } else {
         for_pairs(n, [&](int i) { take(i, x[i * incx]); });
     }
-    c = wave_best(c);
+    c = wave_best<MIN>(c);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) red[wave] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
         Cand t = red[0];
 #pragma unroll
-        for (int w = 1; w < 16; ++w) t = better(t, red[w]);
-        long r = t.i + 1;
-        if (isnan(x[0]) || t.v < 0) r = 1;
-        *result = (int)r;
+        for (int w = 1; w < 16; ++w) t = better<MIN>(t, red[w]);
+        *result = (int)cand_result<MIN>(t, x);
         publish(flag, seq);
     }
 }
@@ -490,6 +509,30 @@ int launch_pair(hipStream_t s, const char *name, int n, double *x, int incx, dou
                        start_of(n, incx), y, (long)incy, start_of(n, incy), c, sn);
     PROF_END(s);
     return (int)hipGetLastError();
+}
+
+// ---- DROTM: (x_i, y_i) := H (x_i, y_i) with the modified-Givens H of the
+// flag, in the netlib DROTM forms (FLAG < 0: full H; 0: unit diagonal;
+// 1: h12 = 1, h21 = -1) ----------------------------------------------------------
+template <int FLAG>
+__global__ __launch_bounds__(256) void rotm_kernel(int n, double *x, long incx, long x0, double *y,
+                                                   long incy, long y0, double h11, double h21,
+                                                   double h12, double h22)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double *px = x + x0 + i * incx, *py = y + y0 + i * incy;
+        const double w = *px, z = *py;
+        if (FLAG < 0) {
+            *px = w * h11 + z * h12;
+            *py = w * h21 + z * h22;
+        } else if (FLAG == 0) {
+            *px = w + z * h12;
+            *py = w * h21 + z;
+        } else {
+            *px = w * h11 + z;
+            *py = -w + h22 * z;
+        }
+    }
 }
 
 // ---- DASUM: the DDOT reductions with |x_i| in place of x_i*y_i -----------------
@@ -628,27 +671,68 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
     return (int)hipGetLastError();
 }
 
-int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result)
+}  // extern "C"
+
+namespace {
+
+template <bool MIN>
+int launch_iamax(void *stream, int n, const double *d_x, int incx, int *h_result)
 {
     hipStream_t s = pick_stream(stream);
     if (n < 1 || incx <= 0) {
         *h_result = 0;
         return 0;
     }
+    const char *name = MIN ? "idamin" : "idamax";
     const Pub pb = pub_for(h_result);
     if (n <= kSmallN) {
         const int vec = incx == 1 && aligned16(d_x);
-        PROF_BEGIN(s, "idamax");
-        hipLaunchKernelGGL(iamax_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
+        PROF_BEGIN(s, name);
+        hipLaunchKernelGGL(iamax_small<MIN>, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
                            h_result, pb.flag, pb.seq);
         PROF_END(s);
         return (int)hipGetLastError();
     }
     RedSlot r = next_slot();
     int blocks = red_blocks(n);
-    PROF_BEGIN(s, "idamax");
-    hipLaunchKernelGGL(iamax_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
+    PROF_BEGIN(s, name);
+    hipLaunchKernelGGL(iamax_kernel<MIN>, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
                        (double *)r.partials, r.ticket, h_result, pb.flag, pb.seq);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpublas_dev_idamax(void *stream, int n, const double *d_x, int incx, int *h_result)
+{
+    return launch_iamax<false>(stream, n, d_x, incx, h_result);
+}
+
+int gpublas_dev_idamin(void *stream, int n, const double *d_x, int incx, int *h_result)
+{
+    return launch_iamax<true>(stream, n, d_x, incx, h_result);
+}
+
+int gpublas_dev_drotm(void *stream, int n, double *d_x, int incx, double *d_y, int incy,
+                      const double *h_param)
+{
+    const double flag = h_param[0];
+    if (n <= 0 || flag == -2.0) return 0;
+    hipStream_t s = pick_stream(stream);
+    int blocks = (int)(((long)n + 255) / 256);
+    if (blocks > 16384) blocks = 16384;
+    const long x0 = start_of(n, incx), y0 = start_of(n, incy);
+    PROF_BEGIN(s, "drotm");
+#define GB_RM(F)                                                                                   \
+    hipLaunchKernelGGL(rotm_kernel<F>, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx, x0, d_y, \
+                       (long)incy, y0, h_param[1], h_param[2], h_param[3], h_param[4])
+    if (flag < 0) GB_RM(-1);
+    else if (flag == 0) GB_RM(0);
+    else GB_RM(1);
+#undef GB_RM
     PROF_END(s);
     return (int)hipGetLastError();
 }

@@ -423,6 +423,17 @@ int cblas_idamax(int n, const double *x, int incx)
     return r ? r - 1 : 0; /* CBLAS indices are 0-based */
 }
 
+int cblas_idamin(int n, const double *x, int incx)
+{
+    int r = idamin_(&n, x, &incx);
+    return r ? r - 1 : 0; /* CBLAS indices are 0-based */
+}
+
+void cblas_drotm(int n, double *x, int incx, double *y, int incy, const double *param)
+{
+    drotm_(&n, x, &incx, y, &incy, param);
+}
+
 void cblas_dscal(int n, double alpha, double *x, int incx) { dscal_(&n, &alpha, x, &incx); }
 
 void cblas_dcopy(int n, const double *x, int incx, double *y, int incy)

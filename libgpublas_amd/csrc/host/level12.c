@@ -308,6 +308,73 @@ int idamax_(const int *n, const double *x, const int *incx)
     return r;
 }
 
+/* IDAMIN: the extension the reference's cblas_idamin forwards to
+ * (blas_level1/amin.cc:40, cublasIdamin); the IDAMAX rules with the minimum. */
+int idamin_(const int *n, const double *x, const int *incx)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n < 1 || *incx <= 0) {
+        gb_leave();
+        return 0;
+    }
+    if (*n == 1) {
+        gb_leave();
+        return 1;
+    }
+    if (l1_host(GB_RK_REDUCE, *n, x, NULL)) {
+        int (*f)(const int *, const double *, const int *) = gb_host_fn("idamin_");
+        GB_COUNT(fallthrough, 1);
+        int r = f(n, x, incx);
+        gb_leave();
+        return r;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand op;
+    gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    int *slot = (int *)gpublas_dev_result_slot();
+    const uint64_t seq = gpublas_dev_scalar_next();
+    gb_check(gpublas_dev_idamin(s, *n, (const double *)op.dev, *incx, slot), "idamin launch");
+    gb_finish_scalar(s, &op, 1, seq);
+    int r = *(volatile int *)slot;
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+    return r;
+}
+
+/* DROTM: netlib DROTM, the definition blas_level1/rotm.cc:51 forwards to;
+ * param = (flag, h11, h21, h12, h22) stays on the host. */
+void drotm_(const int *n, double *x, const int *incx, double *y, const int *incy,
+            const double *param)
+{
+    gb_enter();
+    GB_COUNT(calls, 1);
+    if (*n <= 0 || param[0] == -2.0) {
+        gb_leave();
+        return;
+    }
+    if (l1_host(GB_RK_STREAM, *n, x, y)) {
+        void (*f)(const int *, double *, const int *, double *, const int *, const double *) =
+            gb_host_fn("drotm_");
+        GB_COUNT(fallthrough, 1);
+        f(n, x, incx, y, incy, param);
+        gb_leave();
+        return;
+    }
+    gb_require_device();
+    void *s = gpublas_dev_stream();
+    gb_operand ops[2];
+    gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_INOUT, s);
+    gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_INOUT, s);
+    gb_check(gpublas_dev_drotm(s, *n, (double *)ops[0].dev, *incx, (double *)ops[1].dev, *incy,
+                               param),
+             "drotm launch");
+    gb_finish(s, ops, 2);
+    GB_COUNT(device_calls, 1);
+    gb_leave();
+}
+
 /* ---- DGEMV: y := alpha*op(A)*x + beta*y
  * INFO: 1 trans, 2 m, 3 n, 6 lda, 8 incx, 11 incy ---- */
 void dgemv_(const char *trans, const int *m, const int *n, const double *alpha, const double *a,

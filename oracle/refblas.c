@@ -880,6 +880,50 @@ int ora_idamax(int n, const double *x, int incx)
     return best;
 }
 
+int ora_idamin(int n, const double *x, int incx)
+{
+    /* IDAMIN (the extension the reference's cblas_idamin forwards to,
+     * blas_level1/amin.cc:40): 1-based index of the first minimal |x_i|, the
+     * IDAMAX loop with '<': ties keep the smaller index and a NaN is never
+     * selected after position 1.  n < 1 or incx <= 0 -> 0. */
+    if (n < 1 || incx <= 0) return 0;
+    if (n == 1) return 1;
+    int best = 1;
+    double dmin = fabs(x[0]);
+    for (int i = 1; i < n; ++i) {
+        double a = fabs(x[(size_t)i * incx]);
+        if (a < dmin) {
+            best = i + 1;
+            dmin = a;
+        }
+    }
+    return best;
+}
+
+void ora_drotm(int n, double *x, int incx, double *y, int incy, const double *param)
+{
+    /* netlib DROTM (blas_level1/rotm.cc:51 forwards to it): apply the
+     * modified Givens H, param = (flag, h11, h21, h12, h22), flag -1: full
+     * H, 0: unit diagonal, 1: h12 = 1, h21 = -1, -2: identity. */
+    const double flag = param[0];
+    if (n <= 0 || flag == -2.0) return;
+    const int kx = incx < 0 ? (1 - n) * incx : 0, ky = incy < 0 ? (1 - n) * incy : 0;
+    for (int i = 0; i < n; ++i) {
+        double *px = x + kx + (long)i * incx, *py = y + ky + (long)i * incy;
+        const double w = *px, z = *py;
+        if (flag < 0) {
+            *px = w * param[1] + z * param[3];
+            *py = w * param[2] + z * param[4];
+        } else if (flag == 0) {
+            *px = w + z * param[3];
+            *py = w * param[2] + z;
+        } else {
+            *px = w * param[1] + z;
+            *py = -w + param[4] * z;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Checkers                                                            */
 /* ------------------------------------------------------------------ */

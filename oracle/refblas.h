@@ -109,6 +109,8 @@ void ora_daxpy(int n, double a, const double *x, int incx, double *y, int incy);
 double ora_ddot(int n, const double *x, int incx, const double *y, int incy);
 double ora_dnrm2(int n, const double *x, int incx);
 int ora_idamax(int n, const double *x, int incx);
+int ora_idamin(int n, const double *x, int incx);
+void ora_drotm(int n, double *x, int incx, double *y, int incy, const double *param);
 
 /* ---- checkers ---- */
 /* netlib DMMCH gauge ratio for C = alpha*op(A)*op(B) + beta*C0:

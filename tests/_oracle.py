@@ -43,6 +43,8 @@ def load():
     L.ora_dtpmv.argtypes = [C, C, C, I, VP, VP, I]
     L.ora_dtpsv.argtypes = [C, C, C, I, VP, VP, I]
     L.ora_dspmv.argtypes = [C, I, D, VP, VP, I, D, VP, I]
+    L.ora_idamin.argtypes = [I, VP, I]
+    L.ora_drotm.argtypes = [I, VP, I, VP, I, VP]
     L.ora_tbmv_info.argtypes = [C, C, C, I, I, I, I]
     L.ora_gbmv_info.argtypes = [C, I, I, I, I, I, I, I]
     L.ora_sbmv_info.argtypes = [C, I, I, I, I, I]

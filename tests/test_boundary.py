@@ -504,3 +504,28 @@ def test_band_routines_xerbla_and_fall_through(ora):
     xt = x.copy()  # row-major lower band triangle = T' column-major upper: solve T' z = x
     lib.cblas_dtbsv(101, 122, 111, 131, n, k, p(AT), k + 1, p(xt), 1)
     np.testing.assert_allclose(T.T @ xt, x, rtol=1e-12, atol=1e-12)
+
+
+def test_idamin_drotm_fall_through(ora):
+    """IDAMIN / DROTM below the cutoff run on the host BLAS: bit-identical to
+    scipy's OpenBLAS (drotm) and to the oracle on finite data (idamin);
+    cblas_idamin is 0-based."""
+    lib = gb.library()
+    gb.set_cutoff(512)
+    rng = np.random.default_rng(86)
+    x = rng.standard_normal(200)
+    gb.reset_stats()
+    assert gb.idamin(200, x, 1) == ora.ora_idamin(200, p(x), 1) == np.argmin(np.abs(x)) + 1
+    assert lib.cblas_idamin(200, p(x), 1) == np.argmin(np.abs(x))
+    assert gb.idamin(0, x, 1) == 0 and gb.idamin(5, x, -1) == 0
+    xs, ys = rng.standard_normal(100), rng.standard_normal(100)
+    for flag in (-1.0, 0.0, 1.0):
+        param = np.array([flag, 0.5, -0.25, 1.5, 0.75])
+        X, Y = xs.copy(), ys.copy()
+        gb.drotm(100, X, 1, Y, 1, param)
+        hx, hy = sb.drotm(xs.copy(), ys.copy(), param)
+        assert np.array_equal(X, hx) and np.array_equal(Y, hy)
+    assert gb.stats()["fallthrough"] == 5  # two idamin calls + three drotm calls
+    X = xs.copy()
+    gb.drotm(100, X, 1, Y, 1, np.array([-2.0, 9, 9, 9, 9]))  # identity flag: nothing happens
+    assert np.array_equal(X, xs)

@@ -679,3 +679,54 @@ def test_dtbsv_random(gpu, ora, uplo, trans, n, k):
     gpu.dtbsv(uplo, trans, "N", n, k, AB, k + 1, x, 1)
     ora.ora_dtbsv(c(uplo), c(trans), b"N", n, k, p(AB), k + 1, p(r), 1)
     np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,incx", [(2, 1), (1000, 1), (1001, 3), (65536, 1), (65537, 1),
+                                    (3_000_000, 1), (200_000, 2)])
+def test_idamin(gpu, ora, n, incx):
+    """Bit-exact index: first minimal |x_i|, with a planted tie (the earlier
+    of two equal minima wins) on both reduction paths (one workgroup up to
+    2^16 elements, multi-block beyond)."""
+    x = _strided(uniform(140, (n,)) + 2.0, incx)  # |x| in [1, 3]
+    a, b = n // 3, (2 * n) // 3
+    if n > 2:
+        x[a * incx] = -0.5
+        x[b * incx] = 0.5  # tie with element a: a must win
+    assert gpu.idamin(n, x, incx) == ora.ora_idamin(n, p(x), incx)
+    if n > 2:
+        assert gpu.idamin(n, x, incx) == a + 1
+
+
+@pytest.mark.parametrize("n", [1000, 200_000])
+def test_idamin_nan_rules(gpu, ora, n):
+    x = uniform(141, (n,)) + 2.0
+    x[n // 2] = 0.25
+    x[n // 4] = np.nan  # never selected after position 1
+    assert gpu.idamin(n, x, 1) == ora.ora_idamin(n, p(x), 1) == n // 2 + 1
+    x[0] = np.nan  # a NaN at position 1 wins
+    assert gpu.idamin(n, x, 1) == ora.ora_idamin(n, p(x), 1) == 1
+    x = np.full(n, np.nan)
+    assert gpu.idamin(n, x, 1) == 1
+
+
+@pytest.mark.parametrize("flag", [-2.0, -1.0, 0.0, 1.0])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (1000, 1, 1), (777, 2, -3), (1_000_001, 1, 1)])
+def test_drotm(gpu, ora, flag, n, incx, incy):
+    """Every flag of the modified-Givens H; the device may fuse a product
+    into the sum: bound 2 eps (|h_a w| + |h_b z|) per element."""
+    param = np.array([flag, 0.7, -0.4, 1.3, 0.2])
+    x0 = _strided(uniform(142, (n,)), incx)
+    y0 = _strided(uniform(143, (n,)), incy)
+    X, Y, RX, RY = x0.copy(), y0.copy(), x0.copy(), y0.copy()
+    gpu.drotm(n, X, incx, Y, incy, param)
+    ora.ora_drotm(n, p(RX), incx, p(RY), incy, p(param))
+    w, z = np.abs(_strided_get(x0, n, incx)), np.abs(_strided_get(y0, n, incy))
+    H = {-2.0: [[1, 0], [0, 1]], -1.0: [[0.7, 1.3], [0.4, 0.2]], 0.0: [[1, 1.3], [0.4, 1]],
+         1.0: [[0.7, 1], [1, 0.2]]}[flag]
+    bx, by = 2 * EPS * (H[0][0] * w + H[0][1] * z), 2 * EPS * (H[1][0] * w + H[1][1] * z)
+    assert np.all(np.abs(_strided_get(X, n, incx) - _strided_get(RX, n, incx)) <= bx)
+    assert np.all(np.abs(_strided_get(Y, n, incy) - _strided_get(RY, n, incy)) <= by)
+    for got, orig, inc in ((X, x0, incx), (Y, y0, incy)):
+        gaps = np.ones(len(got), bool)
+        gaps[:: abs(inc)][:n] = False
+        assert np.array_equal(got[gaps], orig[gaps])

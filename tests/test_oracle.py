@@ -706,3 +706,76 @@ def test_band_info_codes(L):
     assert [L.ora_tbmv_info(b"U", b"N", b"N", -1, 0, 1, 1), L.ora_tbmv_info(b"U", b"N", b"N", 1, -1, 1, 1),
             L.ora_tbmv_info(b"U", b"N", b"N", 1, 2, 2, 1), L.ora_tbmv_info(b"U", b"N", b"N", 1, 0, 1, 0)] \
         == [4, 5, 7, 9]
+
+
+def _host_openblas():
+    """scipy's bundled OpenBLAS (the host BLAS of this image), for routines
+    scipy.linalg.blas does not wrap."""
+    import glob
+    import os
+    import scipy
+    d = os.path.dirname(scipy.__file__)
+    libs = glob.glob(d + ".libs/libscipy_openblas*.so") + glob.glob(d + ".libs/libopenblas*.so")
+    if not libs:
+        pytest.skip("no bundled OpenBLAS next to scipy")
+    return ctypes.CDLL(libs[0])
+
+
+def test_idamin_closed_form_and_host_blas(L):
+    """IDAMIN: first index of the minimal |x_i| (numpy argmin of |x| on finite
+    data); ties keep the smaller index; NaN is never selected after position
+    1 and a NaN at position 1 wins (the netlib IDAMAX loop with '<').  The
+    host OpenBLAS idamin_ agrees on finite data; on NaN it follows no single
+    rule (0.3.29 returns 1 for [2, NaN, 1, NaN] and 9, the NaN, for
+    [5, 4, 3, 2, 1, .5, .25, .1, NaN, .01]), so NaN cases pin the loop only."""
+    ob = _host_openblas()
+    f = getattr(ob, "scipy_idamin_", None) or getattr(ob, "idamin_")
+    f.restype = ctypes.c_int
+    I = ctypes.c_int
+
+    def host(x, inc=1):
+        return f(ctypes.byref(I(len(x[::inc]))), p(x), ctypes.byref(I(inc)))
+
+    rng = np.random.default_rng(130)
+    for n in (1, 2, 7, 1000):
+        x = rng.standard_normal(n)
+        assert L.ora_idamin(n, p(x), 1) == np.argmin(np.abs(x)) + 1 == host(x)
+    x = np.array([3.0, -1.0, 2.0, 1.0, -1.0])  # tie: first of the two -1 / 1 minima
+    assert L.ora_idamin(5, p(x), 1) == 2 == host(x)
+    x = np.array([2.0, np.nan, 1.0, np.nan])
+    assert L.ora_idamin(4, p(x), 1) == 3
+    x = np.array([np.nan, 0.5, 0.25])
+    assert L.ora_idamin(3, p(x), 1) == 1
+    x = np.array([5.0, 9.0, 0.0, 7.0, -0.0, 1.0])
+    assert L.ora_idamin(3, p(x), 2) == 2  # elements 5, 0, -0: first zero, the 2nd element
+    assert L.ora_idamin(0, p(x), 1) == 0 and L.ora_idamin(3, p(x), 0) == 0
+
+
+@pytest.mark.parametrize("flag", [-2.0, -1.0, 0.0, 1.0])
+@pytest.mark.parametrize("incx,incy", [(1, 1), (2, -1), (-3, 2)])
+def test_drotm_vs_host_blas_and_closed_form(L, flag, incx, incy):
+    """netlib DROTM for every flag: the 2x2 H of the flag applied to each
+    (x_i, y_i) pair (closed form), and scipy's OpenBLAS drotm."""
+    n = 11
+    rng = np.random.default_rng(131)
+    xs, ys = rng.standard_normal(n), rng.standard_normal(n)
+    param = np.array([flag, 0.7, -0.4, 1.3, 0.2])
+    H = {-2.0: [[1, 0], [0, 1]], -1.0: [[0.7, 1.3], [-0.4, 0.2]], 0.0: [[1, 1.3], [-0.4, 1]],
+         1.0: [[0.7, 1], [-1, 0.2]]}[flag]
+    x = np.full(1 + (n - 1) * abs(incx), 9.0)
+    x[:: abs(incx)] = xs if incx > 0 else xs[::-1]
+    y = np.full(1 + (n - 1) * abs(incy), 9.0)
+    y[:: abs(incy)] = ys if incy > 0 else ys[::-1]
+    X, Y = x.copy(), y.copy()
+    L.ora_drotm(n, p(X), incx, p(Y), incy, p(param))
+
+    def get(v, inc):
+        g = v[:: abs(inc)][:n]
+        return g if inc > 0 else g[::-1]
+
+    np.testing.assert_allclose(get(X, incx), H[0][0] * xs + H[0][1] * ys, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(get(Y, incy), H[1][0] * xs + H[1][1] * ys, rtol=1e-15, atol=1e-15)
+    assert np.all(np.delete(X, np.s_[:: abs(incx)]) == 9.0)
+    hx, hy = sb.drotm(xs.copy(), ys.copy(), param)
+    np.testing.assert_allclose(get(X, incx), hx, rtol=1e-15, atol=1e-16)
+    np.testing.assert_allclose(get(Y, incy), hy, rtol=1e-15, atol=1e-16)
