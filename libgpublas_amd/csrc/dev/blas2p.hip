@@ -5,15 +5,16 @@
 // (cublasDtrmv, blas_level2/trmv.cc; cublasDtpmv / cublasDtpsv, tpmv.cc,
 // tpsv.cc; cublasDspmv, pmv.cc).
 //
-// DTRMV x := op(T) x keeps a copy t of x (gathered through incx) and writes
-// x from it:
+// DTRMV x := op(T) x:
 //  * op = N: the streaming DGEMV layout (512-row x cbw-column panels, 16-B
 //    non-temporal loads, eight columns in flight per wave) with each panel
-//    clipped to the stored triangle; partial sums per panel, folded in panel
-//    order by trmv_fold (deterministic).  (A first version with 256 x 256
+//    clipped to the stored triangle, reading x in place; partial sums per
+//    panel, folded in panel order by trmv_fold (deterministic), which then
+//    writes x.  (A first version with 256 x 256
 //    chunks and 8-B loads ran at 3.05 TB/s at n = 8192.)
 //  * op = T: one wave per column j, a dot product over the column's stored
-//    segment (contiguous) with a fixed shuffle tree.
+//    segment (contiguous) with a fixed shuffle tree, from a copy t of x
+//    (gathered through incx) since x(j) is written while others read.
 // The packed routines unpack the triangle into a full-storage block of a
 // pool buffer (netlib column packing: column j of an upper triangle holds
 // rows 0..j at AP[j(j+1)/2 ..], of a lower one rows j..n-1 at
@@ -47,9 +48,11 @@ constexpr int TS_ROWS = 512, TS_U = 8;
 // triangle are selected away (a unit diagonal is taken as one).
 template <bool LOWER, bool UNIT>
 __global__ __launch_bounds__(256) void trmv_n_stream(int n, const double *A, long lda,
-                                                     const double *t, int nrb, int cbw, int vec,
-                                                     double *P)
+                                                     const double *x, long incx, long x0, int nrb,
+                                                     int cbw, int vec, double *P)
 {
+    // x is only read here; trmv_fold, the next launch on the stream, writes it
+    auto t = [&](long c) { return x[x0 + c * incx]; };
     const int rb = blockIdx.x % nrb, cb = blockIdx.x / nrb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long r0 = (long)rb * TS_ROWS, row = r0 + wave * 128 + 2 * lane;
@@ -73,20 +76,20 @@ __global__ __launch_bounds__(256) void trmv_n_stream(int n, const double *A, lon
                     v[u] = __builtin_nontemporal_load((const d2v *)(a + (c + u) * lda));
 #pragma unroll
                 for (int u = 0; u < TS_U; ++u) {
-                    const double xv = t[c + u];
+                    const double xv = t(c + u);
                     s0 = fma(v[u].x, xv, s0);
                     s1 = fma(v[u].y, xv, s1);
                 }
             }
             for (; c < f1; ++c) {
                 const d2v v = *(const d2v *)(a + c * lda);
-                s0 = fma(v.x, t[c], s0);
-                s1 = fma(v.y, t[c], s1);
+                s0 = fma(v.x, t(c), s0);
+                s1 = fma(v.y, t(c), s1);
             }
         } else {
             for (long c = f0; c < f1; ++c) {
-                if (row < n) s0 = fma(A[row + c * lda], t[c], s0);
-                if (row + 1 < n) s1 = fma(A[row + 1 + c * lda], t[c], s1);
+                if (row < n) s0 = fma(A[row + c * lda], t(c), s0);
+                if (row + 1 < n) s1 = fma(A[row + 1 + c * lda], t(c), s1);
             }
         }
     }
@@ -107,20 +110,20 @@ __global__ __launch_bounds__(256) void trmv_n_stream(int n, const double *A, lon
                 for (int u = 0; u < TS_U; ++u) v[u] = *(const d2v *)(a + (c + u) * lda);
 #pragma unroll
                 for (int u = 0; u < TS_U; ++u) {
-                    const double xv = t[c + u];
+                    const double xv = t(c + u);
                     s0 = fma(keep(row, c + u, v[u].x), xv, s0);
                     s1 = fma(keep(row + 1, c + u, v[u].y), xv, s1);
                 }
             }
             for (; c < d1; ++c) {
                 const d2v v = *(const d2v *)(a + c * lda);
-                s0 = fma(keep(row, c, v.x), t[c], s0);
-                s1 = fma(keep(row + 1, c, v.y), t[c], s1);
+                s0 = fma(keep(row, c, v.x), t(c), s0);
+                s1 = fma(keep(row + 1, c, v.y), t(c), s1);
             }
         } else {
             for (long c = d0; c < d1; ++c) {
-                if (row < n) s0 = fma(keep(row, c, A[row + c * lda]), t[c], s0);
-                if (row + 1 < n) s1 = fma(keep(row + 1, c, A[row + 1 + c * lda]), t[c], s1);
+                if (row < n) s0 = fma(keep(row, c, A[row + c * lda]), t(c), s0);
+                if (row + 1 < n) s1 = fma(keep(row + 1, c, A[row + 1 + c * lda]), t(c), s1);
             }
         }
     }
@@ -207,13 +210,9 @@ int trmv(hipStream_t s, char uplo, char trans, char diag, int n, const double *A
 {
     if (n <= 0) return 0;
     const bool lower = !up(uplo), unit = !isN(diag);
-    double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
-    if (!t) return (int)hipErrorOutOfMemory;
     const long x0 = start_of(n, incx);
-    hipLaunchKernelGGL(gather_vec, dim3((unsigned)std::min(ceil_div(n, 256), 4096)), dim3(256), 0,
-                       s, n, x, (long)incx, x0, t);
-    PROF_BEGIN(s, "dtrmv");
     if (isN(trans)) {
+        PROF_BEGIN(s, "dtrmv");
         // ~1024 panels of at least 16 columns, about half in the triangle
         // (2048 measured 3 % slower at n = 8192 and 4 % faster at 30000)
         const int nrb = ceil_div(n, TS_ROWS);
@@ -224,13 +223,14 @@ int trmv(hipStream_t s, char uplo, char trans, char diag, int n, const double *A
         ncb = ceil_div(n, cbw);
         double *P = (double *)gpublas_dev_alloc((size_t)ncb * n * sizeof(double));
         if (!P) {
-            dev_release_after(t, s);
+            PROF_END(s);
             return (int)hipErrorOutOfMemory;
         }
         const int vec = (((uintptr_t)A & 15) == 0) && (lda % 2 == 0);
         const dim3 grid((unsigned)nrb * ncb);
-#define GB_TRN(L, U) \
-    hipLaunchKernelGGL((trmv_n_stream<L, U>), grid, dim3(256), 0, s, n, A, lda, t, nrb, cbw, vec, P)
+#define GB_TRN(L, U)                                                                              \
+    hipLaunchKernelGGL((trmv_n_stream<L, U>), grid, dim3(256), 0, s, n, A, lda, x, (long)incx, x0, \
+                       nrb, cbw, vec, P)
         if (lower) { if (unit) GB_TRN(true, true); else GB_TRN(true, false); }
         else { if (unit) GB_TRN(false, true); else GB_TRN(false, false); }
 #undef GB_TRN
@@ -241,7 +241,14 @@ int trmv(hipStream_t s, char uplo, char trans, char diag, int n, const double *A
             hipLaunchKernelGGL(trmv_fold<false>, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, s, n,
                                ncb, cbw, P, x, (long)incx, x0);
         dev_release_after(P, s);
+        PROF_END(s);
     } else {
+        // op = T writes x(j) while other waves still read x(i): work from a copy
+        double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
+        if (!t) return (int)hipErrorOutOfMemory;
+        hipLaunchKernelGGL(gather_vec, dim3((unsigned)std::min(ceil_div(n, 256), 4096)), dim3(256),
+                           0, s, n, x, (long)incx, x0, t);
+        PROF_BEGIN(s, "dtrmv");
         const unsigned blocks = (unsigned)std::min(ceil_div(n, 4), 8192);
 #define GB_TRT(L, U)                                                                              \
     hipLaunchKernelGGL((trmv_t_kernel<L, U>), dim3(blocks), dim3(256), 0, s, n, A, lda, t, x,      \
@@ -249,9 +256,9 @@ int trmv(hipStream_t s, char uplo, char trans, char diag, int n, const double *A
         if (lower) { if (unit) GB_TRT(true, true); else GB_TRT(true, false); }
         else { if (unit) GB_TRT(false, true); else GB_TRT(false, false); }
 #undef GB_TRT
+        PROF_END(s);
+        dev_release_after(t, s);
     }
-    PROF_END(s);
-    dev_release_after(t, s);
     return (int)hipGetLastError();
 }
 
