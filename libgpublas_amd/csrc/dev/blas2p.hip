@@ -15,12 +15,12 @@
 //  * op = T: one wave per column j, a dot product over the column's stored
 //    segment (contiguous) with a fixed shuffle tree, from a copy t of x
 //    (gathered through incx) since x(j) is written while others read.
-// The packed routines unpack the triangle into a full-storage block of a
-// pool buffer (netlib column packing: column j of an upper triangle holds
+// Packed storage (netlib column packing: column j of an upper triangle holds
 // rows 0..j at AP[j(j+1)/2 ..], of a lower one rows j..n-1 at
-// AP[j(2n-j+1)/2 ..]) and run DTRMV, DTRSV or DSYMV on it: two more passes
-// over the triangle than a packed kernel, in exchange for the tuned
-// full-storage kernels.
+// AP[j(2n-j+1)/2 ..]): DTPMV and DSPMV read it in place (pk_n_stream,
+// pk_t_kernel; DSPMV = the 'N' sweep with the diagonal + the 'T' sweep without
+// it, so the triangle is read twice); DTPSV unpacks it into a full-storage
+// block of a pool buffer and runs the DTRSV solve.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 
@@ -189,6 +189,195 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(int n, const double *A, lon
     }
 }
 
+// ---- packed storage read in place (DSPMV, DTPMV) ----------------------------
+// Column j of the packed triangle is contiguous: lower rows j .. n-1 from
+// AP[j (2n - j + 1) / 2], upper rows 0 .. j from AP[j (j + 1) / 2].  pcol(j)
+// is the address of element (0, j) of that column, so pcol(j)[i] is (i, j)
+// for every stored i.  Inside the diagonal block the 'N' kernel also reads
+// the few entries of the neighbouring column next to the stored run (rows on
+// the other side of the diagonal); those addresses stay inside AP and the
+// values are selected away.
+template <bool LOWER>
+__device__ __forceinline__ const double *pcol(const double *AP, long n, long j)
+{
+    return LOWER ? AP + j * (2 * n - j + 1) / 2 - j : AP + j * (j + 1) / 2;
+}
+
+// op = N on the packed triangle: the panel layout of trmv_n_stream with 8-B
+// loads (a packed column's 16-B alignment alternates), two rows per lane,
+// eight columns in flight.
+template <bool LOWER, bool UNIT>
+__global__ __launch_bounds__(256) void pk_n_stream(int n, const double *AP, const double *x,
+                                                   long incx, long x0, int nrb, int cbw, double *P)
+{
+    const int rb = blockIdx.x % nrb, cb = blockIdx.x / nrb;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long r0 = (long)rb * TS_ROWS, row = r0 + wave * 128 + 2 * lane;
+    long c0 = (long)cb * cbw, c1 = min((long)n, c0 + cbw);
+    if (LOWER) c1 = min(c1, r0 + TS_ROWS);
+    else c0 = max(c0, r0);
+    if (c0 >= c1) return;
+    auto keep = [&](long i, long c, double v) {
+        return (LOWER ? i >= c : i <= c) ? (UNIT && i == c ? 1.0 : v) : 0.0;
+    };
+    // loads are unconditional: for 0 <= c, row < n, pcol(c) + row is
+    // AP + c (2n - c - 1) / 2 + row (lower) or AP + c (c + 1) / 2 + row
+    // (upper), both within AP[0 .. n (n + 1) / 2 - 1]; the select in keep()
+    // drops what lies outside the triangle.  row1 repeats row on the
+    // matrix's last row (its value is not stored then).
+    const bool r1ok = row + 1 < n;
+    const long row1 = r1ok ? row + 1 : row;
+    double s0 = 0, s1 = 0;
+    if (row < n) {
+        long c = c0;
+        for (; c + TS_U <= c1; c += TS_U) {
+            double v0[TS_U], v1[TS_U], xv[TS_U];
+#pragma unroll
+            for (int u = 0; u < TS_U; ++u) {
+                const double *col = pcol<LOWER>(AP, n, c + u);
+                v0[u] = col[row];
+                v1[u] = col[row1];
+                xv[u] = x[x0 + (c + u) * incx];
+            }
+#pragma unroll
+            for (int u = 0; u < TS_U; ++u) {
+                s0 = fma(keep(row, c + u, v0[u]), xv[u], s0);
+                s1 = fma(keep(row1, c + u, v1[u]), xv[u], s1);
+            }
+        }
+        for (; c < c1; ++c) {
+            const double *col = pcol<LOWER>(AP, n, c);
+            const double xv = x[x0 + c * incx];
+            s0 = fma(keep(row, c, col[row]), xv, s0);
+            s1 = fma(keep(row1, c, col[row1]), xv, s1);
+        }
+    }
+    double *pp = P + (long)cb * n;
+    if (row < n) pp[row] = s0;
+    if (r1ok) pp[row + 1] = s1;
+}
+
+// op = T on the packed triangle: one wave per column, a dot product over the
+// column's contiguous stored run (without the diagonal when DIAG == 0, with a
+// unit one when DIAG == 2), four loads in flight per lane.
+template <bool LOWER, int DIAG>
+__global__ __launch_bounds__(256) void pk_t_kernel(int n, const double *AP, const double *t,
+                                                   long inct, long t0, double *out, long inco,
+                                                   long o0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (long j = (long)blockIdx.x * 4 + wave; j < n; j += (long)gridDim.x * 4) {
+        const double *a = pcol<LOWER>(AP, n, j);
+        const long lo = LOWER ? j + (DIAG != 1) : 0, hi = LOWER ? n : j + (DIAG == 1);
+        double s = 0, s2 = 0;
+        long i = lo + lane;
+        for (; i + 192 < hi; i += 256) {
+            double av[4], tv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                av[u] = a[i + 64 * u];
+                tv[u] = t[t0 + (i + 64 * u) * inct];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (u & 1) s2 = fma(av[u], tv[u], s2);
+                else s = fma(av[u], tv[u], s);
+            }
+        }
+        for (; i < hi; i += 64) s = fma(a[i], t[t0 + i * inct], s);
+        s = wave_sum(s + s2);
+        if (lane == 0) out[o0 + j * inco] = DIAG == 2 ? s + t[t0 + j * inct] : s;
+    }
+}
+
+// out(i) := alpha * (panel partials of row i in panel order [+ the extra
+// partial P[ncb][i]]) + beta * out(i)
+template <bool LOWER, bool EXTRA>
+__global__ __launch_bounds__(64) void pk_fold(int n, int ncb, int cbw, const double *P,
+                                              double alpha, double beta, double *y, long incy,
+                                              long y0)
+{
+    const long i = (long)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const long r0 = i / TS_ROWS * TS_ROWS;
+    const int c_lo = LOWER ? 0 : (int)(r0 / cbw);
+    const int c_hi = LOWER ? (int)min((long)ncb - 1, (r0 + TS_ROWS - 1) / cbw) : ncb - 1;
+    double s = 0;
+    int c = c_lo;
+    for (; c + 8 <= c_hi + 1; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = P[(long)(c + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c <= c_hi; ++c) s += P[(long)c * n + i];
+    if (EXTRA) s += P[(long)ncb * n + i];
+    double *yo = y + y0 + i * incy;
+    *yo = beta == 0.0 ? alpha * s : fma(alpha, s, beta * *yo);
+}
+
+// op(T) x on packed storage (DTPMV) or alpha A x + beta y with A the packed
+// symmetric matrix (DSPMV: the 'N' sweep with the diagonal plus the 'T' sweep
+// without it, added in one fold).
+int packed_mv(hipStream_t s, bool lower, bool sym, bool trans, bool unit, int n, const double *AP,
+              const double *x, int incx, double alpha, double beta, double *y, int incy)
+{
+    const long x0 = start_of(n, incx), y0 = start_of(n, incy);
+    const unsigned tblocks = (unsigned)std::min(ceil_div(n, 4), 8192);
+    if (!sym && trans) {
+        // x(j) is written while other waves read x(i): work from a copy
+        double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
+        if (!t) return (int)hipErrorOutOfMemory;
+        hipLaunchKernelGGL(gather_vec, dim3((unsigned)std::min(ceil_div(n, 256), 4096)), dim3(256),
+                           0, s, n, x, (long)incx, x0, t);
+        PROF_BEGIN(s, "dtpmv");
+#define GB_PT(L, D) \
+    hipLaunchKernelGGL((pk_t_kernel<L, D>), dim3(tblocks), dim3(256), 0, s, n, AP, t, 1L, 0L, y, \
+                       (long)incy, y0)
+        if (lower) { if (unit) GB_PT(true, 2); else GB_PT(true, 1); }
+        else { if (unit) GB_PT(false, 2); else GB_PT(false, 1); }
+#undef GB_PT
+        PROF_END(s);
+        dev_release_after(t, s);
+        return (int)hipGetLastError();
+    }
+    const int nrb = ceil_div(n, TS_ROWS);
+    int ncb = ceil_div(1024, nrb);
+    const int maxcb = ceil_div(n, 16);
+    if (ncb > maxcb) ncb = maxcb;
+    const int cbw = ceil_div(n, ncb);
+    ncb = ceil_div(n, cbw);
+    double *P = (double *)gpublas_dev_alloc((size_t)(ncb + (sym ? 1 : 0)) * n * sizeof(double));
+    if (!P) return (int)hipErrorOutOfMemory;
+    PROF_BEGIN(s, sym ? "dspmv" : "dtpmv");
+    const dim3 grid((unsigned)nrb * ncb);
+#define GB_PN(L, U) \
+    hipLaunchKernelGGL((pk_n_stream<L, U>), grid, dim3(256), 0, s, n, AP, x, (long)incx, x0, nrb, cbw, P)
+    if (lower) { if (unit) GB_PN(true, true); else GB_PN(true, false); }
+    else { if (unit) GB_PN(false, true); else GB_PN(false, false); }
+#undef GB_PN
+    if (sym) {
+        double *Pt = P + (size_t)ncb * n;
+        if (lower)
+            hipLaunchKernelGGL((pk_t_kernel<true, 0>), dim3(tblocks), dim3(256), 0, s, n, AP, x,
+                               (long)incx, x0, Pt, 1L, 0L);
+        else
+            hipLaunchKernelGGL((pk_t_kernel<false, 0>), dim3(tblocks), dim3(256), 0, s, n, AP, x,
+                               (long)incx, x0, Pt, 1L, 0L);
+    }
+    const dim3 fg((unsigned)ceil_div(n, 64));
+#define GB_PF(L, E) \
+    hipLaunchKernelGGL((pk_fold<L, E>), fg, dim3(64), 0, s, n, ncb, cbw, P, alpha, beta, y, \
+                       (long)incy, y0)
+    if (lower) { if (sym) GB_PF(true, true); else GB_PF(true, false); }
+    else { if (sym) GB_PF(false, true); else GB_PF(false, false); }
+#undef GB_PF
+    PROF_END(s);
+    dev_release_after(P, s);
+    return (int)hipGetLastError();
+}
+
 // AP (packed) -> W (full storage, leading dimension n), stored triangle only.
 template <bool LOWER>
 __global__ __launch_bounds__(256) void unpack_kernel(int n, const double *AP, double *W)
@@ -288,11 +477,8 @@ int gpublas_dev_dtpmv(void *stream, char uplo, char trans, char diag, int n, con
 {
     hipStream_t s = pick_stream(stream);
     if (n <= 0) return 0;
-    double *W = unpack(s, uplo, n, d_AP);
-    if (!W) return (int)hipErrorOutOfMemory;
-    const int e = trmv(s, uplo, trans, diag, n, W, n, d_x, incx);
-    dev_release_after(W, s);
-    return e;
+    return packed_mv(s, !up(uplo), false, !isN(trans), !isN(diag), n, d_AP, d_x, incx, 1.0, 0.0,
+                     d_x, incx);
 }
 
 int gpublas_dev_dtpsv(void *stream, char uplo, char trans, char diag, int n, const double *d_AP,
@@ -313,11 +499,7 @@ int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double
     hipStream_t s = pick_stream(stream);
     if (n <= 0) return 0;
     if (alpha == 0.0) return gpublas_dev_dsymv(s, uplo, n, alpha, nullptr, n, d_x, incx, beta, d_y, incy);
-    double *W = unpack(s, uplo, n, d_AP);
-    if (!W) return (int)hipErrorOutOfMemory;
-    const int e = gpublas_dev_dsymv(s, uplo, n, alpha, W, n, d_x, incx, beta, d_y, incy);
-    dev_release_after(W, s);
-    return e;
+    return packed_mv(s, !up(uplo), true, false, false, n, d_AP, d_x, incx, alpha, beta, d_y, incy);
 }
 
 }  // extern "C"
