@@ -173,24 +173,21 @@ __global__ void copy_vec(int n, const double *x, long incx, long x0, double *t)
         t[i] = x[x0 + i * incx];
 }
 
-// ---- column-owned band product (kl + ku + 1 <= 128; used for DSBMV) ------
-// A workgroup owns BC consecutive columns and stages their whole band (w rows
-// each: one contiguous run per column) in LDS, so the band is read once with
-// full-line loads.  Row stride in LDS W = w | 1 (odd: no bank conflicts for
-// the per-column walks).
-//   trans: y(j) for the owned columns, written directly;
-//   notrans: column j adds A(r, j) x(j) to row i = j - ku + r; the owned
-//   columns reach rows j0 - ku .. j0 + BC - 1 + kl, shared with neighbouring
-//   blocks, so each block writes its BC + w - 1 partial row sums and
-//   band_cols_fold adds the (at most ceil((w - 1) / BC) + 1) blocks of a row in
-//   block order;
-//   symmetric: both in one pass over the stored band (the trans sums without
-//   the diagonal added into the block's partial rows).
+// ---- DSBMV in one read of the band (kl + ku + 1 <= 128) -----------------------
+// A workgroup owns BC consecutive columns and stages their whole stored band
+// (w rows each: one contiguous run per column) in LDS, so the band is read
+// once with full-line loads.  Row stride in LDS W = w | 1 (odd: no bank
+// conflicts for the per-column walks).  With A = B + B' - diag(B), B the
+// stored band:
+//   B x: column j adds B(r, j) x(j) to row i = j - ku + r; the owned columns
+//   reach rows j0 - ku .. j0 + BC - 1 + kl, shared with neighbouring blocks,
+//   so each block writes its BC + w - 1 partial row sums and band_cols_fold
+//   adds the (at most ceil((w - 1) / BC) + 1) blocks of a row in block order;
+//   B' x without the diagonal: per owned column a dot product with x, added
+//   into the block's partial of row j.
 // BC = min(512, BC_LDS / W) columns: about 4000 band elements per workgroup
 // whatever the width, and 40 KB of LDS in all (four workgroups per CU).
 constexpr int BC_MAXW = 128, BC_LDS = 4000, BC_MAX = 512, BC_XW = BC_MAX + 16;
-
-enum { BM_N = 0, BM_T = 1, BM_SYM = 2 };
 
 inline int band_cols_bc(int w)
 {
@@ -198,13 +195,10 @@ inline int band_cols_bc(int w)
     return bc < BC_MAX ? bc : BC_MAX;
 }
 
-template <int MODE, int DG>
-__global__ __launch_bounds__(256) void band_cols(BandArgs a, int BC)
+__global__ __launch_bounds__(256) void band_cols_sym(BandArgs a, int BC)
 {
     __shared__ double As[BC_LDS];
-    // v slice: notrans x(j0 + q); trans and symmetric x(j0 - ku + q), whose
-    // entries from q = ku on are the owned columns' x
-    __shared__ double xw[BC_XW];
+    __shared__ double xw[BC_XW];  // x(j0 - ku + q); from q = ku on, the owned columns' x
     __shared__ double tsum[BC_MAX];
     const int tid = threadIdx.x;
     const int w = a.kl + a.ku + 1, W = w | 1;
@@ -218,52 +212,34 @@ __global__ __launch_bounds__(256) void band_cols(BandArgs a, int BC)
         const int e = tid + 256 * u, jl = e / w, r = e - jl * w;
         const int j = j0 + jl, i = j - a.ku + r;
         v[u] = 0;
-        if (e < BC * w && j < a.n && i >= 0 && i < a.m)
-            v[u] = DG == DG_ONE && r == a.ku ? 1.0 : a.A[r + (long)j * a.lda];
+        if (e < BC * w && j < a.n && i >= 0 && i < a.m) v[u] = a.A[r + (long)j * a.lda];
     }
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
         const int e = tid + 256 * u, jl = e / w, r = e - jl * w;
         if (e < BC * w) As[jl * W + r] = v[u];
     }
-    if (MODE == BM_N)
-        for (int q = tid; q < BC; q += 256) {
-            const int j = j0 + q;
-            xw[q] = j < a.n ? a.v[a.v0 + (long)j * a.incv] : 0.0;
-        }
-    else
-        for (int q = tid; q < BC + w - 1; q += 256) {
-            const int i = j0 - a.ku + q;
-            xw[q] = i >= 0 && i < a.m ? a.v[a.v0 + (long)i * a.incv] : 0.0;
-        }
-    __syncthreads();
-    if (MODE != BM_N) {
-        for (int c = tid; c < BC; c += 256) {
-            // column sums; the symmetric pass leaves the diagonal to the row sums
-            double s = 0;
-            const double *col = As + c * W;
-            for (int r = 0; r < w; ++r)
-                if (MODE != BM_SYM || r != a.ku) s = fma(col[r], xw[c + r], s);
-            if (MODE == BM_T) {
-                const int j = j0 + c;
-                if (j < a.n) {
-                    double *yo = a.y + a.y0 + (long)j * a.incy;
-                    *yo = a.beta == 0.0 ? a.alpha * s : fma(a.alpha, s, a.beta * *yo);
-                }
-            } else {
-                tsum[c] = s;
-            }
-        }
+    for (int q = tid; q < BC + w - 1; q += 256) {
+        const int i = j0 - a.ku + q;
+        xw[q] = i >= 0 && i < a.m ? a.v[a.v0 + (long)i * a.incv] : 0.0;
     }
-    if (MODE == BM_T) return;
-    if (MODE == BM_SYM) __syncthreads();
-    const int nt = BC + w - 1, xoff = MODE == BM_SYM ? a.ku : 0;
+    __syncthreads();
+    for (int c = tid; c < BC; c += 256) {
+        // B' x for the owned column, without the diagonal
+        double s = 0;
+        const double *col = As + c * W;
+        for (int r = 0; r < w; ++r)
+            if (r != a.ku) s = fma(col[r], xw[c + r], s);
+        tsum[c] = s;
+    }
+    __syncthreads();
+    const int nt = BC + w - 1;
     for (int t = tid; t < nt; t += 256) {
         // row i = j0 - ku + t: owned columns jl in [t - w + 1, t]
         double s = 0;
         const int lo = max(0, t - w + 1), hi = min(BC - 1, t);
-        for (int jl = lo; jl <= hi; ++jl) s = fma(As[jl * (W - 1) + t], xw[jl + xoff], s);
-        if (MODE == BM_SYM && t - a.ku >= 0 && t - a.ku < BC) s += tsum[t - a.ku];
+        for (int jl = lo; jl <= hi; ++jl) s = fma(As[jl * (W - 1) + t], xw[jl + a.ku], s);
+        if (t - a.ku >= 0 && t - a.ku < BC) s += tsum[t - a.ku];
         a.P[(long)blockIdx.x * nt + t] = s;
     }
 }
@@ -291,8 +267,8 @@ struct Pass {
     int dg;
 };
 
-// DSBMV for bands up to 128 wide: the column-owned kernel in its symmetric
-// mode (one read of the band) and the ordered fold; -1 when the band is wider.
+// DSBMV for bands up to 128 wide: band_cols_sym (one read of the band) and
+// the ordered fold; -1 when the band is wider.
 int band_sym_apply(hipStream_t s, int n, int kl, int ku, const double *A, long lda,
                    const double *v, long incv, long v0, double alpha, double beta, double *y,
                    long incy, long y0)
@@ -305,7 +281,7 @@ int band_sym_apply(hipStream_t s, int n, int kl, int ku, const double *A, long l
     if (!P) return (int)hipErrorOutOfMemory;
     BandArgs a{n, n, kl, ku, A, lda, v, incv, v0, 0, 0, n, y, incy, y0, alpha, beta, P};
     PROF_BEGIN(s, "band_mv");
-    hipLaunchKernelGGL((band_cols<BM_SYM, DG_LOAD>), dim3((unsigned)nb), dim3(256), 0, s, a, BC);
+    hipLaunchKernelGGL(band_cols_sym, dim3((unsigned)nb), dim3(256), 0, s, a, BC);
     hipLaunchKernelGGL(band_cols_fold, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, ku, w,
                        BC, nb, P, alpha, beta, y, incy, y0);
     dev_release_after(P, s);

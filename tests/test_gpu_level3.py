@@ -117,7 +117,7 @@ def test_dgemm_alpha_zero_scales(gpu):
 
 
 def test_dgemm_config1_normwise(gpu, ora):
-    """M=N=K=1024 NN, alpha=1, beta=0.5, U[-1,1]: the north-star normwise bound
+    """M=N=K=1024 NN, alpha=1, beta=0.5, U[-1,1]: the normwise bound of the 1024^3 DGEMM workload
     ||C - Cref||_F / (||A||_F ||B||_F k eps) <= 4 against the oracle."""
     n = 1024
     A, B, C0 = uniform(101, (n, n)), uniform(102, (n, n)), uniform(103, (n, n))
