@@ -15,11 +15,11 @@
 //   notrans  o = row i,    col = i + ku - r, in = col;
 //   trans    o = column j, col = j,          in = j - ku + r.
 // A workgroup owns BR = 256 consecutive outputs and walks the band rows in
-// chunks of BD = 16; per chunk it stages the chunk's rows of the columns it
-// needs (BR + BD - 1 of them for notrans, BR for trans: each column's 16
-// rows are one contiguous 128-byte run) and the matching slice of v in LDS,
-// then every thread adds BD products from LDS.  HBM reads are the band plus
-// (BD - 1) / BR of it for notrans.  When the outputs alone give too few
+// chunks of BD (16 for bands up to 16 wide, else 32); per chunk it stages the
+// chunk's rows of the columns it needs (BR + BD - 1 of them for notrans, BR
+// for trans: each column's BD rows are one contiguous run) and the matching
+// slice of v in LDS, then every thread adds BD products from LDS.  HBM reads
+// are the band plus (BD - 1) / BR of it for notrans.  When the outputs alone give too few
 // workgroups, the chunks are split over a second grid dimension and per-split
 // partial sums are added in a fixed order by band_fold (deterministic).
 // Elements outside the matrix (the corners of the band array) are replaced by
@@ -29,13 +29,15 @@
 //   kl + ku + 1 <= 128 the column-owned kernel below (one read of the band);
 //   wider, a notrans pass over B and a trans pass with band row ku skipped,
 //   summed in the fold.
-// For band widths above 16 the 16-row runs of a column start at arbitrary
-// 8-byte offsets, so a run straddles two 128-byte lines, and the PMC counters
-// show twice the band's bytes fetched (DGBMV n = 2^20, kl = ku = 32).  The
-// column-owned kernel reads whole columns instead but pays a partial-sum fold
-// and, for narrow bands, small workgroups: it measured slower for DGBMV and
-// DTBMV (2.0-3.2 TB/s against 3.2-4.0) and faster for DSBMV (one read of the
-// band instead of two), which is where it is used.
+// The runs of a column start at arbitrary 8-byte offsets, so a 16-row run
+// straddles two 128-byte lines: with 16-row chunks the PMC counters show twice
+// the band's bytes fetched (DGBMV n = 2^20, kl = ku = 32).  32-row chunks
+// (three lines for two lines of data) measured 12-18 % faster for bands wider
+// than 16 (DGBMV 'N' 3.57, 'T' 4.09, DTBMV 3.12 TB/s), and half as fast for a
+// 9-wide band, hence the split at 16.  The column-owned kernel below reads
+// whole columns but pays a partial-sum fold and, for narrow bands, small
+// workgroups: it measured slower for DGBMV and DTBMV and faster for DSBMV
+// (one read of the band instead of two), which is where it is used.
 //
 // DTBSV (tbsv_reg / tbsv_lds).  The solve is a recurrence along the diagonal:
 // one wave walks the columns in solve order and, after finishing x(j), takes
@@ -54,7 +56,10 @@
 
 namespace {
 
-constexpr int BR = 256, BD = 16, BTC = BR + BD - 1;
+// band_pass tiles: BR outputs per workgroup, BD band rows per chunk (16 for
+// bands up to 16 wide: 37 KB of LDS, four workgroups per CU; 32 beyond: the
+// 256-byte runs waste less of their lines, 78 KB, two per CU)
+constexpr int BR = 256;
 
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
 inline bool up(char c) { return c == 'U' || c == 'u'; }
@@ -77,9 +82,11 @@ struct BandArgs {
     double *P;  // else partial sums P[split][lenout]
 };
 
-template <bool TRANS, int DG, bool DIRECT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void band_pass(BandArgs a)
+template <int BD, bool TRANS, int DG, bool DIRECT>
+__global__ __launch_bounds__(256, BD == 16 ? 4 : 2) void band_pass(BandArgs a)
 {
+    constexpr int BTC = BR + BD - 1;
+    constexpr int BCS = 256 / BD;  // tile columns between one thread's loads
     __shared__ double Ts[BTC * (BD + 1)];
     __shared__ double xs[BR + BD];
     const int tid = threadIdx.x;
@@ -89,7 +96,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // this thread loads band row rr of tile columns c0 + 16 u
     const int rr = tid % BD, c0 = tid / BD;
     constexpr int TC = TRANS ? BR : BTC;
-    constexpr int NL = (TC + 15) / 16;
+    constexpr int NL = (TC + BCS - 1) / BCS;
     const int lenv = TRANS ? a.m : a.n;
     double val[NL], xv0, xv1;
     // the tile of chunk ch into registers: band row r0 + rr of tile columns
@@ -106,10 +113,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         const double *col = a.A + r + (long)(jb + c0) * a.lda;
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
-            const int c = c0 + 16 * u, j = jb + c;
+            const int c = c0 + BCS * u, j = jb + c;
             val[u] = 0;
             if (c < TC && rok && j >= jlo && j < jhi)
-                val[u] = isdiag ? (DG == DG_ONE ? 1.0 : 0.0) : col[(long)(16 * u) * a.lda];
+                val[u] = isdiag ? (DG == DG_ONE ? 1.0 : 0.0) : col[(long)(BCS * u) * a.lda];
         }
         const int t0 = vb + tid, t1 = vb + tid + 256;
         xv0 = t0 >= 0 && t0 < lenv ? a.v[a.v0 + t0 * a.incv] : 0.0;
@@ -122,7 +129,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         if (ch != cA) __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
-            const int c = c0 + 16 * u;
+            const int c = c0 + BCS * u;
             if (c < TC) Ts[c * (BD + 1) + rr] = val[u];
         }
         xs[tid] = xv0;
@@ -298,7 +305,8 @@ int band_apply(hipStream_t s, int m, int n, int kl, int ku, const double *A, lon
     const int lenout = passes[0].trans ? n : m;
     if (lenout <= 0) return 0;
     const int rb = ceil_div(lenout, BR);
-    const int nchunks = ceil_div(kl + ku + 1, BD);
+    const int bd = kl + ku + 1 <= 16 ? 16 : 32;
+    const int nchunks = ceil_div(kl + ku + 1, bd);
     int nsplit = ceil_div(1024, (long)rb * npass);
     if (nsplit > nchunks) nsplit = nchunks;
     if (nsplit < 1) nsplit = 1;
@@ -315,7 +323,11 @@ int band_apply(hipStream_t s, int m, int n, int kl, int ku, const double *A, lon
         BandArgs a{m, n, kl, ku, A, lda, v, incv, v0, nchunks, cpw, lenout,
                    y, incy, y0, alpha, beta, P ? P + (size_t)p * nsplit * lenout : nullptr};
         const dim3 grid((unsigned)rb, (unsigned)nsplit);
-#define GB_BP(T, D, DIR) hipLaunchKernelGGL((band_pass<T, D, DIR>), grid, dim3(256), 0, s, a)
+#define GB_BP(T, D, DIR)                                                                      \
+    do {                                                                                      \
+        if (bd == 16) hipLaunchKernelGGL((band_pass<16, T, D, DIR>), grid, dim3(256), 0, s, a); \
+        else hipLaunchKernelGGL((band_pass<32, T, D, DIR>), grid, dim3(256), 0, s, a);          \
+    } while (0)
         const bool t = passes[p].trans;
         const int dg = passes[p].dg;
         if (direct) {
