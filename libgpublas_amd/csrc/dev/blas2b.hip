@@ -478,13 +478,18 @@ __global__ __launch_bounds__(64) void tbsv_reg(TbArgs a)
 }
 
 // k >= 64: window of W (power of two > k) unknowns in LDS; lane l handles the
-// offsets d = 1 + l + 64 g.  G > 0 (k < 64 G): the next step's coefficients
-// are loaded one step ahead into registers; G = 0: any k, loaded in the step.
-// One wave: its LDS accesses complete in program order, so no barrier.
+// offsets d = 1 + l + 64 g.  Everything a step needs from memory -- its
+// coefficients (G > 0, k < 64 G), its diagonal and reciprocal, the unknown
+// entering the window -- is loaded PD = 8 steps ahead into a ring of
+// registers whose slots the 8-way unrolled step loop names statically (one
+// step ahead left each step waiting a full memory latency: ~1 us per column).
+// G = 0: any k, coefficients loaded in the step.  One wave: its LDS accesses
+// complete in program order, so no barrier.
 template <bool TR, bool UPPER, bool NOUNIT, int G>
 __global__ __launch_bounds__(64) void tbsv_lds(TbArgs a, int W)
 {
     extern __shared__ double win[];
+    constexpr int PD = 8;
     const int l = threadIdx.x;
     const long n = a.n;
     const int msk = W - 1;
@@ -496,46 +501,55 @@ __global__ __launch_bounds__(64) void tbsv_lds(TbArgs a, int W)
     };
     for (int q = l; q < W; q += 64) win[q] = q < n ? xat(q) : 0.0;
     constexpr int GR = G > 0 ? G : 1;
-    double cf[GR];
-    if constexpr (G > 0) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) cf[g] = coef(0, 1 + l + 64 * g);
-    }
-    double dgv = NOUNIT ? a.A[tb_diag<UPPER>(a, row(0))] : 1.0;
-    for (long st = 0; st < n; ++st) {
-        double cn[GR];
+    double cf[PD][GR], dgr[PD], rcr[PD], ent[PD];
+    // ring slot o holds what step s0 + o needs
+    auto fill = [&](int o, long st) {
         if constexpr (G > 0) {
 #pragma unroll
-            for (int g = 0; g < G; ++g) cn[g] = coef(st + 1, 1 + l + 64 * g);
+            for (int g = 0; g < G; ++g) cf[o][g] = coef(st, 1 + l + 64 * g);
         }
-        const double dgnext = NOUNIT && st + 1 < n ? a.A[tb_diag<UPPER>(a, row(st + 1))] : 1.0;
-        const double enter = st + W < n ? xat(st + W) : 0.0;
-        double xj = win[st & msk];
-        if (NOUNIT) xj = xj / dgv;
-        if constexpr (G > 0) {
+        dgr[o] = NOUNIT && st < n ? a.A[tb_diag<UPPER>(a, row(st))] : 1.0;
+        ent[o] = st + W < n ? xat(st + W) : 0.0;
+    };
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const long d = 1 + l + 64 * g;
-                if (d <= a.k) {
-                    const int slot = (int)((st + d) & msk);
-                    win[slot] = fma(-xj, cf[g], win[slot]);
+    for (int o = 0; o < PD; ++o) fill(o, o);
+    for (long s0 = 0; s0 < n; s0 += PD) {
+#pragma unroll
+        for (int o = 0; o < PD; ++o) rcr[o] = NOUNIT ? 1.0 / dgr[o] : 1.0;
+#pragma unroll
+        for (int o = 0; o < PD; ++o) {
+            const long st = s0 + o;
+            if (st < n) {
+                double xj = win[st & msk];
+                if (NOUNIT) xj = qdiv(xj, dgr[o], rcr[o]);
+                if constexpr (G > 0) {
+                    // this lane's window slots (distinct: d < W): all reads,
+                    // then the updates, then all writes -- one LDS round trip
+                    // per step instead of one per slot
+                    double wv[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const long d = 1 + l + 64 * g;
+                        wv[g] = d <= a.k ? win[(st + d) & msk] : 0.0;
+                    }
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const long d = 1 + l + 64 * g;
+                        if (d <= a.k) win[(st + d) & msk] = fma(-xj, cf[o][g], wv[g]);
+                    }
+                } else {
+                    for (long d = 1 + l; d <= a.k; d += 64) {
+                        const int slot = (int)((st + d) & msk);
+                        win[slot] = fma(-xj, coef(st, d), win[slot]);
+                    }
+                }
+                if (l == 0) {
+                    xat(st) = xj;
+                    win[st & msk] = ent[o];
                 }
             }
-        } else {
-            for (long d = 1 + l; d <= a.k; d += 64) {
-                const int slot = (int)((st + d) & msk);
-                win[slot] = fma(-xj, coef(st, d), win[slot]);
-            }
+            fill(o, st + PD);
         }
-        if (l == 0) {
-            xat(st) = xj;
-            win[st & msk] = enter;
-        }
-        if constexpr (G > 0) {
-#pragma unroll
-            for (int g = 0; g < G; ++g) cf[g] = cn[g];
-        }
-        dgv = dgnext;
     }
 }
 
@@ -563,7 +577,13 @@ int tbsv(hipStream_t s, char uplo, char trans, char diag, int n, int k, const do
         const size_t lds = (size_t)W * sizeof(double);
 #define GB_TBL(T, U, D, G) hipLaunchKernelGGL((tbsv_lds<T, U, D, G>), dim3(1), dim3(64), lds, s, a, W)
 #define GB_TBL_G(T, U, D)                                                                         \
-    { if (k < 128) GB_TBL(T, U, D, 2); else if (k < 1024) GB_TBL(T, U, D, 16); else GB_TBL(T, U, D, 0); }
+    {                                                                                             \
+        if (k < 128) GB_TBL(T, U, D, 2);                                                          \
+        else if (k < 256) GB_TBL(T, U, D, 4);                                                     \
+        else if (k < 512) GB_TBL(T, U, D, 8);                                                     \
+        else if (k < 1024) GB_TBL(T, U, D, 16);                                                   \
+        else GB_TBL(T, U, D, 0);                                                                  \
+    }
 #define GB_TBL2(T, U) { if (nounit) GB_TBL_G(T, U, true) else GB_TBL_G(T, U, false) }
         if (tr) { if (upper) GB_TBL2(true, true) else GB_TBL2(true, false) }
         else { if (upper) GB_TBL2(false, true) else GB_TBL2(false, false) }

@@ -633,7 +633,8 @@ def test_dtbmv(gpu, ora, uplo, trans, diag, n, k, incx):
 
 
 TBSV_CASES = [(1, 0, 1), (100, 1, 1), (5000, 5, 1), (3001, 63, -2),    # register window
-              (3000, 64, 1), (4000, 100, 3), (2500, 700, 1),           # LDS window, prefetched
+              (3000, 64, 1), (4000, 100, 3), (1200, 200, 1),           # LDS window, prefetched
+              (1100, 300, 2), (2500, 700, 1),
               (3000, 1500, 1)]                                          # LDS window, k >= 1024
 
 
