@@ -684,8 +684,7 @@ int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, con
         hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, s, n, t, d_x, (long)incx, x0);
         err = (int)hipGetLastError();
     }
-    (void)hipStreamSynchronize(s);
-    gpublas_dev_release(t);
+    dev_release_after(t, s);  // back to the pool once the stream's work on it is done
     return err;
 }
 
