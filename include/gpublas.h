@@ -172,6 +172,21 @@ void cblas_dsyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, i
 void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
                  CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
                  int ldb);
+/* FP32 Level-3 twins: cblas_ssymm cblas.h:744, cblas_ssyrk :762, cblas_ssyr2k :781,
+ * cblas_strmm :800, cblas_strsm :819 (same operand swaps as their FP64 twins) */
+void cblas_ssyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 float alpha, const float *a, int lda, float beta, float *c, int ldc);
+void cblas_strsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, float alpha, const float *a, int lda, float *b,
+                 int ldb);
+void cblas_ssymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n, float alpha,
+                 const float *a, int lda, const float *b, int ldb, float beta, float *c, int ldc);
+void cblas_ssyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  float alpha, const float *a, int lda, const float *b, int ldb, float beta,
+                  float *c, int ldc);
+void cblas_strmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, float alpha, const float *a, int lda, float *b,
+                 int ldb);
 void cblas_dgemv(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE trans, int m, int n, double alpha,
                  const double *a, int lda, const double *x, int incx, double beta, double *y,
                  int incy);

@@ -23,6 +23,10 @@
  *             (gpublas_sync) before taking the next arrival;
  *   serve  -- serving loop on (GPUBLAS_OPTIONS serve=..., serve_graph, ...
  *             are passed through --options).
+ * --verify k checks every k-th timed call of each thread against a host BLAS
+ * (--verify-blas) after its latency is taken; the per-kind counts, failures
+ * and worst normwise errors are added to the output, and the exit status is
+ * 4 if any check failed.  Throughput measured with checks on includes them.
  * Prints one JSON object.
  */
 #define _GNU_SOURCE
@@ -136,45 +140,174 @@ static void *resident(size_t elems, int single, uint64_t seed)
     return h;
 }
 
-static void one_call(worker_t *w, uint64_t *s, int *opout)
+/* One drawn call: everything needed to run it and to check it afterwards. */
+typedef struct {
+    int op, m, n, k, lenx;
+    char ta, tb;
+    size_t ia, ib, ix;
+} call_t;
+
+static void draw_call(uint64_t *s, call_t *c)
 {
     const double u = unif(s);
-    const double one = 1.0, zero = 0.0;
-    const float onef = 1.0f, zerof = 0.0f;
+    memset(c, 0, sizeof *c);
     if (u < 0.6) {
         const int single = unif(s) < 0.5;
-        const int m = logu(s, 32, 2048), n = logu(s, 32, 2048), k = logu(s, 32, 2048);
-        const char ta = unif(s) < 0.5 ? 'N' : 'T', tb = unif(s) < 0.5 ? 'N' : 'T';
-        const int ra = ta == 'N' ? m : k, ca = ta == 'N' ? k : m;
-        const int rb = tb == 'N' ? k : n, cb = tb == 'N' ? n : k;
-        const size_t ia = (size_t)(unif(s) * (GP - ra)) + (size_t)(unif(s) * (GP - ca)) * GP;
-        const size_t ib = (size_t)(unif(s) * (GP - rb)) + (size_t)(unif(s) * (GP - cb)) * GP;
-        const int ld = GP, ldc = m;
-        if (single) {
-            p_sgemm(&ta, &tb, &m, &n, &k, &onef, g_pools + ia, &ld, g_pools + ib, &ld, &zerof, w->cs,
-                    &ldc);
-            *opout = OP_SGEMM;
-        } else {
-            p_dgemm(&ta, &tb, &m, &n, &k, &one, g_poold + ia, &ld, g_poold + ib, &ld, &zero, w->cd,
-                    &ldc);
-            *opout = OP_DGEMM;
-        }
+        c->m = logu(s, 32, 2048), c->n = logu(s, 32, 2048), c->k = logu(s, 32, 2048);
+        c->ta = unif(s) < 0.5 ? 'N' : 'T', c->tb = unif(s) < 0.5 ? 'N' : 'T';
+        const int ra = c->ta == 'N' ? c->m : c->k, ca = c->ta == 'N' ? c->k : c->m;
+        const int rb = c->tb == 'N' ? c->k : c->n, cb = c->tb == 'N' ? c->n : c->k;
+        c->ia = (size_t)(unif(s) * (GP - ra)) + (size_t)(unif(s) * (GP - ca)) * GP;
+        c->ib = (size_t)(unif(s) * (GP - rb)) + (size_t)(unif(s) * (GP - cb)) * GP;
+        c->op = single ? OP_SGEMM : OP_DGEMM;
     } else if (u < 0.9) {
-        const int m = logu(s, 256, VP), n = logu(s, 256, VP);
-        const char tr = unif(s) < 0.5 ? 'N' : 'T';
-        const size_t ia = (size_t)(unif(s) * (VP - m)) + (size_t)(unif(s) * (VP - n)) * VP;
-        const int ld = VP, inc = 1;
-        const int lenx = tr == 'N' ? n : m;
-        const size_t ix = (size_t)(unif(s) * (VP - lenx));
-        p_dgemv(&tr, &m, &n, &one, g_poolv + ia, &ld, g_poolx + ix, &inc, &zero, w->y, &inc);
-        *opout = OP_DGEMV;
+        c->m = logu(s, 256, VP), c->n = logu(s, 256, VP);
+        c->ta = unif(s) < 0.5 ? 'N' : 'T';
+        c->ia = (size_t)(unif(s) * (VP - c->m)) + (size_t)(unif(s) * (VP - c->n)) * VP;
+        c->lenx = c->ta == 'N' ? c->n : c->m;
+        c->ix = (size_t)(unif(s) * (VP - c->lenx));
+        c->op = OP_DGEMV;
     } else {
-        const int m = logu(s, 64, TP), n = NRHS, ld = TP;
-        const double alpha = 1024.0;
-        p_dtrsm("L", "L", "N", "N", &m, &n, &alpha, g_poolt, &ld, w->bt, &ld);
-        *opout = OP_DTRSM;
+        c->m = logu(s, 64, TP), c->n = NRHS;
+        c->op = OP_DTRSM;
+    }
+}
+
+static const double TRSM_ALPHA = 1024.0;
+
+static void run_call(worker_t *w, const call_t *c)
+{
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    const int ld = c->op == OP_DGEMV ? VP : c->op == OP_DTRSM ? TP : GP, inc = 1, ldc = c->m;
+    switch (c->op) {
+    case OP_SGEMM:
+        p_sgemm(&c->ta, &c->tb, &c->m, &c->n, &c->k, &onef, g_pools + c->ia, &ld, g_pools + c->ib, &ld,
+                &zerof, w->cs, &ldc);
+        break;
+    case OP_DGEMM:
+        p_dgemm(&c->ta, &c->tb, &c->m, &c->n, &c->k, &one, g_poold + c->ia, &ld, g_poold + c->ib, &ld,
+                &zero, w->cd, &ldc);
+        break;
+    case OP_DGEMV:
+        p_dgemv(&c->ta, &c->m, &c->n, &one, g_poolv + c->ia, &ld, g_poolx + c->ix, &inc, &zero, w->y,
+                &inc);
+        break;
+    default:
+        p_dtrsm("L", "L", "N", "N", &c->m, &c->n, &TRSM_ALPHA, g_poolt, &ld, w->bt, &ld);
+        break;
     }
     if (g_plain) p_sync();
+}
+
+/* ---- verification against a host BLAS (--verify k --verify-blas lib) ----
+ * Every k-th timed call of each thread is checked once it has returned,
+ * before the thread's next call reuses its output buffer:
+ *   GEMM  ||C - Cref||_F / (||op(A)||_F ||op(B)||_F k eps) <= 4, Cref from
+ *         the host DGEMM in FP64 on the same (FP32-widened) operands;
+ *   GEMV  ||y - yref|| / (||A||_F ||x|| len eps) <= 4, yref from host DGEMV;
+ *   TRSM  ||A X - alpha B0||_F / (||A||_F ||X||_F m eps) <= 4, A X from host
+ *         DTRMM; B0 is copied before the call.
+ * eps is that of the call's precision.  Reading the tracked outputs and pool
+ * windows from the host goes through the library's coherence (a download). */
+typedef void (*hgemm_t)(const char *, const char *, const int *, const int *, const int *,
+                        const double *, const double *, const int *, const double *, const int *,
+                        const double *, double *, const int *);
+typedef void (*htrmm_t)(const char *, const char *, const char *, const char *, const int *,
+                        const int *, const double *, const double *, const int *, double *,
+                        const int *);
+static int g_verify = 0;
+static hgemm_t h_dgemm;
+static gemv_t h_dgemv;
+static htrmm_t h_dtrmm;
+static const double VERIFY_BAR = 4.0;
+
+typedef struct {
+    double *a, *b, *r; /* scratch: widened operands / saved B0, reference result */
+    int checked[NOPS], failed[NOPS];
+    double worst[NOPS];
+} verify_t;
+static verify_t *g_ver;
+
+static double fro(const double *a, int rows, int cols, int ld)
+{
+    double s = 0;
+    for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i) s += a[i + (size_t)j * ld] * a[i + (size_t)j * ld];
+    return sqrt(s);
+}
+
+static void widen(const float *src, int rows, int cols, int ld, double *dst)
+{
+    for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i) dst[i + (size_t)j * rows] = src[i + (size_t)j * ld];
+}
+
+static void verify_call(worker_t *w, const call_t *c, const double *b0, verify_t *v)
+{
+    const double one = 1.0, zero = 0.0;
+    double err = 0;
+    if (c->op == OP_DGEMM || c->op == OP_SGEMM) {
+        const int ra = c->ta == 'N' ? c->m : c->k, ca = c->ta == 'N' ? c->k : c->m;
+        const int rb = c->tb == 'N' ? c->k : c->n, cb = c->tb == 'N' ? c->n : c->k;
+        const double *A, *B;
+        int lda, ldb;
+        double diff = 0;
+        if (c->op == OP_DGEMM) {
+            A = g_poold + c->ia, B = g_poold + c->ib, lda = ldb = GP;
+        } else {
+            widen(g_pools + c->ia, ra, ca, GP, v->a);
+            widen(g_pools + c->ib, rb, cb, GP, v->b);
+            A = v->a, B = v->b, lda = ra, ldb = rb;
+        }
+        h_dgemm(&c->ta, &c->tb, &c->m, &c->n, &c->k, &one, A, &lda, B, &ldb, &zero, v->r, &c->m);
+        for (size_t i = 0; i < (size_t)c->m * c->n; ++i) {
+            const double g = c->op == OP_DGEMM ? w->cd[i] : (double)w->cs[i];
+            diff += (g - v->r[i]) * (g - v->r[i]);
+        }
+        const double eps = c->op == OP_DGEMM ? 0x1.0p-52 : 0x1.0p-23;
+        const double den = fro(A, ra, ca, lda) * fro(B, rb, cb, ldb) * c->k * eps;
+        err = den > 0 ? sqrt(diff) / den : sqrt(diff);
+    } else if (c->op == OP_DGEMV) {
+        const int inc = 1, ld = VP, leny = c->ta == 'N' ? c->m : c->n;
+        h_dgemv(&c->ta, &c->m, &c->n, &one, g_poolv + c->ia, &ld, g_poolx + c->ix, &inc, &zero,
+                v->r, &inc);
+        double diff = 0;
+        for (int i = 0; i < leny; ++i) diff += (w->y[i] - v->r[i]) * (w->y[i] - v->r[i]);
+        const double den = fro(g_poolv + c->ia, c->m, c->n, VP) * fro(g_poolx + c->ix, c->lenx, 1, 1) *
+                           c->lenx * 0x1.0p-52;
+        err = den > 0 ? sqrt(diff) / den : sqrt(diff);
+    } else {
+        const int m = c->m, n = c->n, ld = TP;
+        for (int j = 0; j < n; ++j)
+            memcpy(v->r + (size_t)j * m, w->bt + (size_t)j * TP, sizeof(double) * (size_t)m);
+        h_dtrmm("L", "L", "N", "N", &m, &n, &one, g_poolt, &ld, v->r, &m);
+        double diff = 0, ta = 0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i) {
+                const double d = v->r[i + (size_t)j * m] - TRSM_ALPHA * b0[i + (size_t)j * m];
+                diff += d * d;
+            }
+        for (int j = 0; j < m; ++j)
+            for (int i = j; i < m; ++i) ta += g_poolt[i + (size_t)j * TP] * g_poolt[i + (size_t)j * TP];
+        const double den = sqrt(ta) * fro(w->bt, m, n, TP) * m * 0x1.0p-52;
+        err = den > 0 ? sqrt(diff) / den : sqrt(diff);
+    }
+    v->checked[c->op]++;
+    if (!(err <= VERIFY_BAR)) v->failed[c->op]++;
+    if (!(err <= v->worst[c->op])) v->worst[c->op] = err; /* NaN sticks */
+}
+
+/* Runs one drawn call; with `check` it also saves what the check needs
+ * (the right-hand sides a TRSM overwrites) into the thread's scratch. */
+static void one_call(worker_t *w, uint64_t *s, call_t *c, int check)
+{
+    draw_call(s, c);
+    if (check && c->op == OP_DTRSM)
+        for (int j = 0; j < c->n; ++j)
+            memcpy(g_ver[w->id].b + (size_t)j * c->m, w->bt + (size_t)j * TP,
+                   sizeof(double) * (size_t)c->m);
+    run_call(w, c);
 }
 
 static void *worker_main(void *arg)
@@ -182,8 +315,8 @@ static void *worker_main(void *arg)
     worker_t *w = arg;
     uint64_t s = g_seed * 1000003ull + (uint64_t)w->id;
     const int per = g_calls / g_threads;
-    int dummy;
-    for (int i = 0; i < g_warmup; ++i) one_call(w, &s, &dummy);
+    call_t c;
+    for (int i = 0; i < g_warmup; ++i) one_call(w, &s, &c, 0);
     const double rate_t = g_rate > 0 ? g_rate / g_threads : 0;
     double t_sched = now();
     for (int i = 0; i < per; ++i) {
@@ -200,8 +333,11 @@ static void *worker_main(void *arg)
         } else {
             t_sched = now();
         }
-        one_call(w, &s, &w->op[i]);
+        const int check = g_verify > 0 && i % g_verify == 0;
+        one_call(w, &s, &c, check);
         w->lat[i] = now() - t_sched;
+        w->op[i] = c.op;
+        if (check) verify_call(w, &c, g_ver[w->id].b, &g_ver[w->id]);
     }
     w->t_end = now();
     return NULL;
@@ -224,7 +360,7 @@ static double pct(const double *v, int n, double p)
 
 int main(int argc, char **argv)
 {
-    const char *lib = NULL, *opts = "";
+    const char *lib = NULL, *opts = "", *vblas = NULL, *vprefix = "";
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--lib") && i + 1 < argc) lib = argv[++i];
         else if (!strcmp(argv[i], "--threads") && i + 1 < argc) g_threads = atoi(argv[++i]);
@@ -234,10 +370,14 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--seed") && i + 1 < argc) g_seed = strtoull(argv[++i], 0, 10);
         else if (!strcmp(argv[i], "--plain")) g_plain = 1;
         else if (!strcmp(argv[i], "--options") && i + 1 < argc) opts = argv[++i];
+        else if (!strcmp(argv[i], "--verify") && i + 1 < argc) g_verify = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--verify-blas") && i + 1 < argc) vblas = argv[++i];
+        else if (!strcmp(argv[i], "--verify-prefix") && i + 1 < argc) vprefix = argv[++i];
     }
     if (!lib) {
         fprintf(stderr, "usage: serve_bench --lib libgpublas.so [--plain] [--options o] "
-                        "[--rate calls/s] [--threads t] [--calls n]\n");
+                        "[--rate calls/s] [--threads t] [--calls n] "
+                        "[--verify k --verify-blas host_blas.so [--verify-prefix p]]\n");
         return 2;
     }
     void *h = dlopen(lib, RTLD_NOW | RTLD_GLOBAL);
@@ -259,6 +399,29 @@ int main(int argc, char **argv)
     p_fill32 = sym(h, "gpublas_dev_fill_uniform_f32");
     p_devsync = sym(h, "gpublas_dev_sync");
     ((void (*)(int))sym(h, "gpublas_set_cutoff"))(0);
+    if (g_verify > 0) {
+        if (!vblas) {
+            fprintf(stderr, "serve_bench: --verify needs --verify-blas <host BLAS library>\n");
+            return 2;
+        }
+        void *hb = dlopen(vblas, RTLD_NOW | RTLD_LOCAL);
+        if (!hb) {
+            fprintf(stderr, "serve_bench: %s\n", dlerror());
+            return 2;
+        }
+        char nm[128];
+        snprintf(nm, sizeof nm, "%sdgemm_", vprefix);
+        h_dgemm = (hgemm_t)sym(hb, nm);
+        snprintf(nm, sizeof nm, "%sdgemv_", vprefix);
+        h_dgemv = (gemv_t)sym(hb, nm);
+        snprintf(nm, sizeof nm, "%sdtrmm_", vprefix);
+        h_dtrmm = (htrmm_t)sym(hb, nm);
+        /* one host thread per caller: 16 callers each checking on a
+         * multi-threaded host BLAS would oversubscribe the cores */
+        snprintf(nm, sizeof nm, "%sopenblas_set_num_threads", vprefix);
+        void (*setnt)(int) = (void (*)(int))dlsym(hb, nm);
+        if (setnt) setnt(1);
+    }
     if (g_plain) p_setopt("serve=0");
     else if (p_setopt(*opts ? opts : "serve") != 0) {
         fprintf(stderr, "serve_bench: bad options '%s'\n", opts);
@@ -283,6 +446,17 @@ int main(int argc, char **argv)
         w[t].y = resident((size_t)VP, 0, 300 + t);
         w[t].bt = resident((size_t)TP * NRHS, 0, 400 + t);
         w[t].lat = calloc((size_t)per, sizeof(double));
+        if (g_verify > 0) {
+            if (!g_ver) g_ver = calloc((size_t)g_threads, sizeof *g_ver);
+            const size_t big = (size_t)2048 * 2048;
+            g_ver[t].a = malloc(big * sizeof(double));
+            g_ver[t].b = malloc(big * sizeof(double));
+            g_ver[t].r = malloc((big > VP ? big : VP) * sizeof(double));
+            if (!g_ver[t].a || !g_ver[t].b || !g_ver[t].r) {
+                fprintf(stderr, "serve_bench: out of host memory for --verify\n");
+                return 3;
+            }
+        }
         w[t].op = calloc((size_t)per, sizeof(int));
     }
     const double t_setup = now() - t0;
@@ -327,11 +501,29 @@ int main(int argc, char **argv)
            st.serve_wait_ns / ns / 1e3, st.serve_run_ns / ns / 1e3,
            st.batches ? st.serve_launch_ns / (double)st.batches / 1e3 : 0.0);
     printf(", \"served\": %llu, \"batches\": %llu, \"mean_batch\": %.2f, \"max_batch\": %llu, "
-           "\"deferred\": %llu, \"evictions\": %llu, \"resident_bytes\": %llu, \"setup_s\": %.2f}\n",
+           "\"deferred\": %llu, \"evictions\": %llu, \"resident_bytes\": %llu, \"setup_s\": %.2f",
            (unsigned long long)st.served, (unsigned long long)st.batches,
            st.batches ? (double)st.served / (double)st.batches : 0.0,
            (unsigned long long)st.max_batch, (unsigned long long)st.deferred,
            (unsigned long long)st.evictions, (unsigned long long)st.resident_bytes, t_setup);
+    int bad = 0;
+    if (g_verify > 0) {
+        /* the closing brace of the object moves behind the verify summary */
+        printf(", \"verify_every\": %d, \"verify_bar\": %.1f", g_verify, VERIFY_BAR);
+        for (int o = 0; o < NOPS; ++o) {
+            int ck = 0, fl = 0;
+            double worst = 0;
+            for (int t = 0; t < g_threads; ++t) {
+                ck += g_ver[t].checked[o];
+                fl += g_ver[t].failed[o];
+                if (!(g_ver[t].worst[o] <= worst)) worst = g_ver[t].worst[o];
+            }
+            bad += fl;
+            printf(", \"verify_%s\": {\"checked\": %d, \"failed\": %d, \"max_err\": %.4g}", op_name[o],
+                   ck, fl, worst);
+        }
+    }
+    printf("}\n");
     fflush(stdout);
-    return 0;
+    return bad ? 4 : 0;
 }

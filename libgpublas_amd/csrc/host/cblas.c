@@ -81,95 +81,93 @@ DEFINE_CBLAS_GEMM(s, float, "SGEMM ")
 /* ---- SYRK: 1 layout 2 uplo 3 trans 4 n 5 k 6 alpha 7 A 8 lda 9 beta 10 C 11 ldc ---- */
 static const int syrk_map[11] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 0, 11};
 
-void cblas_dsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
-                 double alpha, const double *a, int lda, double beta, double *c, int ldc)
-{
-    if (!lay_ok(layout)) { cbl_err("DSYRK ", 1); return; }
-    if (!up_ok(uplo)) { cbl_err("DSYRK ", 2); return; }
-    if (!tr_ok(trans)) { cbl_err("DSYRK ", 3); return; }
-    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
-    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
-    WITH_MAP(syrk_map, dsyrk_(&u, &t, &n, &k, &alpha, a, &lda, &beta, c, &ldc));
-}
+#define DEFINE_CBLAS_SYRK(PRE, T, NAME6)                                                         \
+    void cblas_##PRE##syrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n,   \
+                           int k, T alpha, const T *a, int lda, T beta, T *c, int ldc)           \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 2); return; }                                         \
+        if (!tr_ok(trans)) { cbl_err(NAME6, 3); return; }                                        \
+        char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);                              \
+        char t = layout == CblasColMajor ? tr(trans) : trflip(trans);                            \
+        WITH_MAP(syrk_map, PRE##syrk_(&u, &t, &n, &k, &alpha, a, &lda, &beta, c, &ldc));         \
+    }
+DEFINE_CBLAS_SYRK(d, double, "DSYRK ")
+DEFINE_CBLAS_SYRK(s, float, "SSYRK ")
 
-/* ---- TRSM: 1 layout 2 side 3 uplo 4 ta 5 diag 6 m 7 n 8 alpha 9 A 10 lda 11 B 12 ldb ---- */
+/* ---- TRSM / TRMM: 1 layout 2 side 3 uplo 4 ta 5 diag 6 m 7 n 8 alpha 9 A 10 lda 11 B
+ *      12 ldb (row major: the transposed problem, side and uplo flipped) ---- */
 static const int trsm_col[12] = {0, 2, 3, 4, 5, 6, 7, 0, 0, 10, 0, 12};
 static const int trsm_row[12] = {0, 2, 3, 4, 5, 7, 6, 0, 0, 10, 0, 12};
 
-void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
-                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
-                 int ldb)
-{
-    if (!lay_ok(layout)) { cbl_err("DTRSM ", 1); return; }
-    if (!side_ok(side)) { cbl_err("DTRSM ", 2); return; }
-    if (!up_ok(uplo)) { cbl_err("DTRSM ", 3); return; }
-    if (!tr_ok(ta)) { cbl_err("DTRSM ", 4); return; }
-    if (!diag_ok(diag)) { cbl_err("DTRSM ", 5); return; }
-    char t = tr(ta), d = dg(diag);
-    if (layout == CblasColMajor) {
-        char s = sd(side), u = up(uplo);
-        WITH_MAP(trsm_col, dtrsm_(&s, &u, &t, &d, &m, &n, &alpha, a, &lda, b, &ldb));
-    } else {
-        char s = sdflip(side), u = upflip(uplo);
-        WITH_MAP(trsm_row, dtrsm_(&s, &u, &t, &d, &n, &m, &alpha, a, &lda, b, &ldb));
+#define DEFINE_CBLAS_TRIANGULAR(PRE, OP, T, NAME6)                                               \
+    void cblas_##PRE##OP(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo,                  \
+                         CBLAS_TRANSPOSE ta, CBLAS_DIAG diag, int m, int n, T alpha, const T *a, \
+                         int lda, T *b, int ldb)                                                 \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!side_ok(side)) { cbl_err(NAME6, 2); return; }                                       \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 3); return; }                                         \
+        if (!tr_ok(ta)) { cbl_err(NAME6, 4); return; }                                           \
+        if (!diag_ok(diag)) { cbl_err(NAME6, 5); return; }                                       \
+        char t = tr(ta), d = dg(diag);                                                           \
+        if (layout == CblasColMajor) {                                                           \
+            char s = sd(side), u = up(uplo);                                                     \
+            WITH_MAP(trsm_col, PRE##OP##_(&s, &u, &t, &d, &m, &n, &alpha, a, &lda, b, &ldb));    \
+        } else {                                                                                 \
+            char s = sdflip(side), u = upflip(uplo);                                             \
+            WITH_MAP(trsm_row, PRE##OP##_(&s, &u, &t, &d, &n, &m, &alpha, a, &lda, b, &ldb));    \
+        }                                                                                        \
     }
-}
+DEFINE_CBLAS_TRIANGULAR(d, trsm, double, "DTRSM ")
+DEFINE_CBLAS_TRIANGULAR(s, trsm, float, "STRSM ")
+DEFINE_CBLAS_TRIANGULAR(d, trmm, double, "DTRMM ")
+DEFINE_CBLAS_TRIANGULAR(s, trmm, float, "STRMM ")
 
 /* ---- SYMM: 1 layout 2 side 3 uplo 4 m 5 n 6 alpha 7 A 8 lda 9 B 10 ldb 11 beta 12 C 13 ldc ---- */
 static const int symm_col[13] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 10, 0, 0, 13};
 static const int symm_row[13] = {0, 2, 3, 5, 4, 0, 0, 8, 0, 10, 0, 0, 13};
 
-void cblas_dsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n, double alpha,
-                 const double *a, int lda, const double *b, int ldb, double beta, double *c,
-                 int ldc)
-{
-    if (!lay_ok(layout)) { cbl_err("DSYMM ", 1); return; }
-    if (!side_ok(side)) { cbl_err("DSYMM ", 2); return; }
-    if (!up_ok(uplo)) { cbl_err("DSYMM ", 3); return; }
-    if (layout == CblasColMajor) {
-        char s = sd(side), u = up(uplo);
-        WITH_MAP(symm_col, dsymm_(&s, &u, &m, &n, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
-    } else {
-        char s = sdflip(side), u = upflip(uplo);
-        WITH_MAP(symm_row, dsymm_(&s, &u, &n, &m, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
+#define DEFINE_CBLAS_SYMM(PRE, T, NAME6)                                                         \
+    void cblas_##PRE##symm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,  \
+                           T alpha, const T *a, int lda, const T *b, int ldb, T beta, T *c,      \
+                           int ldc)                                                              \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!side_ok(side)) { cbl_err(NAME6, 2); return; }                                       \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 3); return; }                                         \
+        if (layout == CblasColMajor) {                                                           \
+            char s = sd(side), u = up(uplo);                                                     \
+            WITH_MAP(symm_col,                                                                   \
+                     PRE##symm_(&s, &u, &m, &n, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));      \
+        } else {                                                                                 \
+            char s = sdflip(side), u = upflip(uplo);                                             \
+            WITH_MAP(symm_row,                                                                   \
+                     PRE##symm_(&s, &u, &n, &m, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));      \
+        }                                                                                        \
     }
-}
+DEFINE_CBLAS_SYMM(d, double, "DSYMM ")
+DEFINE_CBLAS_SYMM(s, float, "SSYMM ")
 
 /* ---- SYR2K: 1 layout 2 uplo 3 trans 4 n 5 k 6 alpha 7 A 8 lda 9 B 10 ldb 11 beta 12 C
  *      13 ldc (row major: the same update of C^T, uplo and trans flipped) ---- */
 static const int syr2k_map[13] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 10, 0, 0, 13};
 
-void cblas_dsyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
-                  double alpha, const double *a, int lda, const double *b, int ldb, double beta,
-                  double *c, int ldc)
-{
-    if (!lay_ok(layout)) { cbl_err("DSYR2K", 1); return; }
-    if (!up_ok(uplo)) { cbl_err("DSYR2K", 2); return; }
-    if (!tr_ok(trans)) { cbl_err("DSYR2K", 3); return; }
-    char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);
-    char t = layout == CblasColMajor ? tr(trans) : trflip(trans);
-    WITH_MAP(syr2k_map, dsyr2k_(&u, &t, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));
-}
-
-/* ---- TRMM: 1 layout 2 side 3 uplo 4 ta 5 diag 6 m 7 n 8 alpha 9 A 10 lda 11 B 12 ldb ---- */
-void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
-                 CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
-                 int ldb)
-{
-    if (!lay_ok(layout)) { cbl_err("DTRMM ", 1); return; }
-    if (!side_ok(side)) { cbl_err("DTRMM ", 2); return; }
-    if (!up_ok(uplo)) { cbl_err("DTRMM ", 3); return; }
-    if (!tr_ok(ta)) { cbl_err("DTRMM ", 4); return; }
-    if (!diag_ok(diag)) { cbl_err("DTRMM ", 5); return; }
-    char t = tr(ta), d = dg(diag);
-    if (layout == CblasColMajor) {
-        char s = sd(side), u = up(uplo);
-        WITH_MAP(trsm_col, dtrmm_(&s, &u, &t, &d, &m, &n, &alpha, a, &lda, b, &ldb));
-    } else {
-        char s = sdflip(side), u = upflip(uplo);
-        WITH_MAP(trsm_row, dtrmm_(&s, &u, &t, &d, &n, &m, &alpha, a, &lda, b, &ldb));
+#define DEFINE_CBLAS_SYR2K(PRE, T, NAME6)                                                        \
+    void cblas_##PRE##syr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n,  \
+                            int k, T alpha, const T *a, int lda, const T *b, int ldb, T beta,    \
+                            T *c, int ldc)                                                       \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 2); return; }                                         \
+        if (!tr_ok(trans)) { cbl_err(NAME6, 3); return; }                                        \
+        char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);                              \
+        char t = layout == CblasColMajor ? tr(trans) : trflip(trans);                            \
+        WITH_MAP(syr2k_map,                                                                      \
+                 PRE##syr2k_(&u, &t, &n, &k, &alpha, a, &lda, b, &ldb, &beta, c, &ldc));         \
     }
-}
+DEFINE_CBLAS_SYR2K(d, double, "DSYR2K")
+DEFINE_CBLAS_SYR2K(s, float, "SSYR2K")
 
 /* ---- GEMV: 1 layout 2 trans 3 m 4 n 5 alpha 6 A 7 lda 8 x 9 incx 10 beta 11 y 12 incy ---- */
 static const int gemv_col[12] = {0, 2, 3, 4, 0, 0, 7, 0, 9, 0, 0, 12};

@@ -529,3 +529,31 @@ def test_idamin_drotm_fall_through(ora):
     X = xs.copy()
     gb.drotm(100, X, 1, Y, 1, np.array([-2.0, 9, 9, 9, 9]))  # identity flag: nothing happens
     assert np.array_equal(X, xs)
+
+
+# ----------------------------------------------------------------------------- errors
+def test_cblas_error_positions_fp32_level3():
+    """The FP32 Level-3 twins and a row-major DGEMV report CBLAS positions
+    (checks run before any device work, so no GPU is needed)."""
+    from test_gpu_cblas import (COL, LEFT, NONUNIT, NOTRANS, ROW, UPPER, _fn, _ptr)
+    lib = gb.library()
+    d = np.zeros(16, np.float32)
+    _fn(gb, "syrk", "s")(999, UPPER, NOTRANS, 2, 2, 1.0, _ptr(d), 2, 0.0, _ptr(d), 2)
+    assert gb.last_xerbla() == ("SSYRK ", 1)
+    _fn(gb, "syrk", "s")(COL, 7, NOTRANS, 2, 2, 1.0, _ptr(d), 2, 0.0, _ptr(d), 2)
+    assert gb.last_xerbla() == ("SSYRK ", 2)
+    _fn(gb, "syrk", "s")(ROW, UPPER, NOTRANS, 2, -1, 1.0, _ptr(d), 2, 0.0, _ptr(d), 2)
+    assert gb.last_xerbla() == ("SSYRK ", 5)
+    _fn(gb, "trsm", "s")(ROW, LEFT, UPPER, NOTRANS, 5, 2, 2, 1.0, _ptr(d), 2, _ptr(d), 2)
+    assert gb.last_xerbla() == ("STRSM ", 5)
+    _fn(gb, "trsm", "s")(ROW, LEFT, UPPER, NOTRANS, NONUNIT, 3, 2, 1.0, _ptr(d), 2, _ptr(d), 2)
+    assert gb.last_xerbla() == ("STRSM ", 10)  # row-major: lda < m for the left side
+    _fn(gb, "trmm", "s")(COL, 9, UPPER, NOTRANS, NONUNIT, 2, 2, 1.0, _ptr(d), 2, _ptr(d), 2)
+    assert gb.last_xerbla() == ("STRMM ", 2)
+    _fn(gb, "symm", "s")(ROW, LEFT, UPPER, 2, 3, 1.0, _ptr(d), 2, _ptr(d), 3, 0.0, _ptr(d), 2)
+    assert gb.last_xerbla() == ("SSYMM ", 13)  # row-major: ldc < n
+    _fn(gb, "syr2k", "s")(COL, UPPER, NOTRANS, 3, 2, 1.0, _ptr(d), 3, _ptr(d), 2, 0.0, _ptr(d), 3)
+    assert gb.last_xerbla() == ("SSYR2K", 10)
+    dd = np.zeros(16)
+    lib.cblas_dgemv(ROW, NOTRANS, 3, 4, 1.0, _ptr(dd), 3, _ptr(dd), 1, 0.0, _ptr(dd), 1)
+    assert gb.last_xerbla() == ("DGEMV ", 7)  # row-major: lda < n

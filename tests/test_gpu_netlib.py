@@ -1,13 +1,15 @@
-"""The reference's own Level-3 conformance tester on the GPU path.
+"""The reference's own Level-3 conformance testers on the GPU path.
 
-oracle/_ref/dblat3 is tests/netlib/dblat3.f of the reference compiled from the
-reference tree (oracle/Makefile, target ref) and linked against
-libgpublas.so.  With cutoff=0 every DGEMM/DSYMM/DTRSM/DSYRK call it makes
-(N in {0,1,2,3,5,9}, alpha in {0,1,0.7}, beta in {0,1,1.3}, every trans /
-side / uplo / diag, ld padding filled with ROGUE) runs on the gfx950 kernels;
-dblat3 checks each result with its DMMCH gauge (threshold 16) and the
-untouched regions bit-for-bit, and checks every error exit through XERBLA.
-DTRMM and DSYR2K are not on the hot path and are switched off in the input.
+oracle/_ref/dblat3 and oracle/_ref/sblat3 are tests/netlib/{d,s}blat3.f of the
+reference compiled from the reference tree (oracle/Makefile, target ref) and
+linked against libgpublas.so; they run with the reference's own input files
+(tests/netlib/input.{d,s}blat3, committed unchanged under tests/golden/), as
+the reference's tests/netlib/meson.build:1-21 runs them.  With cutoff=0 every
+GEMM / SYMM / TRMM / TRSM / SYRK / SYR2K call they make (N in {0,1,2,3,5,9},
+alpha in {0,1,0.7}, beta in {0,1,1.3}, every trans / side / uplo / diag, ld
+padding filled with ROGUE) runs on the gfx950 kernels; the testers check each
+result with their gauge (threshold 16) and the untouched regions bit-for-bit,
+and check every error exit through XERBLA.
 """
 import os
 import subprocess
@@ -21,22 +23,24 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_dblat3_all_hot_path_routines_on_gpu(gpu):
-    exe = os.path.join(ROOT, "oracle", "_ref", "dblat3")
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_netlib_level3_tester_on_gpu(gpu, prec):
+    exe = os.path.join(ROOT, "oracle", "_ref", f"{prec}blat3")
     if not os.path.exists(exe):
-        pytest.skip("oracle/_ref/dblat3 was not built (needs the reference tree at build time)")
-    inp = open(os.path.join(ROOT, "tests", "golden", "input.dblat3.gpu")).read()
+        pytest.skip(f"oracle/_ref/{prec}blat3 was not built (needs the reference tree at build time)")
+    inp = open(os.path.join(ROOT, "tests", "golden", f"input.{prec}blat3")).read()
     with tempfile.TemporaryDirectory() as td:
         env = dict(os.environ)
         env["GPUBLAS_OPTIONS"] = "cutoff=0;stats=" + os.path.join(td, "stats.csv")
         out = subprocess.run([exe], input=inp, cwd=td, env=env, capture_output=True, text=True,
                              timeout=600)
-        summ = open(os.path.join(td, "DBLAT3.SUMM")).read()
+        summ = open(os.path.join(td, f"{prec.upper()}BLAT3.SUMM")).read()
         stats = open(os.path.join(td, "stats.csv")).read().splitlines()
     assert out.returncode == 0, out.stderr[-3000:]
-    for r in ("DGEMM ", "DSYMM ", "DTRSM ", "DSYRK "):
-        assert f"{r} PASSED THE TESTS OF ERROR-EXITS" in summ, summ
-        assert f"{r} PASSED THE COMPUTATIONAL TESTS" in summ, summ
+    p = prec.upper()
+    for r in ("GEMM ", "SYMM ", "TRMM ", "TRSM ", "SYRK ", "SYR2K"):
+        assert f"{p}{r} PASSED THE TESTS OF ERROR-EXITS" in summ, summ
+        assert f"{p}{r} PASSED THE COMPUTATIONAL TESTS" in summ, summ
     assert "FAIL" not in summ and "**" not in summ
     hdr = [h.strip() for h in stats[0].split(",")]
     vals = dict(zip(hdr, (int(v) for v in stats[1].split(","))))

@@ -369,3 +369,38 @@ def test_tuner_writes_and_reloads_the_table(gpu):
         choice = gb.tune_choice(False, 2048, 2048, 2048)
         assert gb.tune_gemm(path, force=False) == 0
         assert gb.tune_choice(False, 2048, 2048, 2048) == choice
+
+
+SCIPY_OPENBLAS = "/usr/local/lib/python3.10/dist-packages/scipy.libs/libscipy_openblas-68440149.so"
+
+
+@pytest.mark.parametrize("mode", ["serve", "plain"])
+def test_serving_trace_verified(gb, mode):
+    """The serving trace itself (serve_bench: 16 threads, the 60/30/10
+    GEMM / DGEMV / DTRSM mix with M, N, K log-uniform in [32, 2048], DGEMV
+    m, n in [256, 16384], DTRSM orders in [64, 1024]), 1,600 calls with every
+    call checked against the host BLAS: GEMM and GEMV normwise error
+    ||C - Cref|| / (||A|| ||B|| k eps) <= 4, TRSM residual
+    ||A X - alpha B0|| / (||A|| ||X|| m eps) <= 4."""
+    import json
+    import subprocess
+    if not os.path.exists(SCIPY_OPENBLAS):
+        pytest.skip("the host BLAS (scipy's OpenBLAS) is not installed here")
+    cmd = [gb.SERVE_BENCH, "--lib", gb.LIB_PATH, "--threads", "16", "--calls", "1600",
+           "--warmup", "2", "--verify", "1", "--verify-blas", SCIPY_OPENBLAS,
+           "--verify-prefix", "scipy_"]
+    if mode == "plain":
+        cmd.append("--plain")
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert out.returncode in (0, 4), out.stderr[-2000:]
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    total = 0
+    for op in ("dgemm", "sgemm", "dgemv", "dtrsm"):
+        v = res[f"verify_{op}"]
+        assert v["failed"] == 0 and v["max_err"] <= 4.0, (op, v)
+        assert v["checked"] > 0, op
+        total += v["checked"]
+    assert total == 1600
+    assert out.returncode == 0
+    if mode == "serve":
+        assert res["served"] >= 1600
