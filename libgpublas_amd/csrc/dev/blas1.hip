@@ -26,17 +26,40 @@ constexpr int kMaxBlocks = 1024;        // partials per slot
 constexpr size_t kSlotBytes = kMaxBlocks * 16;
 thread_local unsigned t_slot = 0;
 
+__global__ void zero_ticket(unsigned *t)
+{
+    if (threadIdx.x == 0) *t = 0u;
+}
+
+// Partials and ticket of one reduction on stream st.  On the calling thread's
+// own stream: one of its kSlots rotating slots (tickets kept zero between
+// launches, calls ordered by the stream).  On another stream: a pool block of
+// its own, whose ticket a one-thread kernel clears first (a fill kernel rather
+// than a memset, so the call can be captured into a graph).
 struct RedSlot {
-    void *partials;
-    unsigned *ticket;
+    Scratch scr;
+    void *partials = nullptr;
+    unsigned *ticket = nullptr;
 };
 
-RedSlot next_slot()
+RedSlot next_slot(hipStream_t st)
 {
     DevCtx *c = dev_ctx();
-    char *ws = (char *)dev_workspace(kSlots * kSlotBytes);
-    unsigned s = t_slot++ % kSlots;
-    return RedSlot{ws + s * kSlotBytes, c->tickets + s};
+    RedSlot r;
+    if (st == c->stream) {
+        r.scr = dev_scratch(kSlots * kSlotBytes, st);
+        if (!r.scr.p) return r;
+        unsigned s = t_slot++ % kSlots;
+        r.partials = (char *)r.scr.p + s * kSlotBytes;
+        r.ticket = c->tickets + s;
+        return r;
+    }
+    r.scr = dev_scratch(kSlotBytes + 256, st);
+    if (!r.scr.p) return r;
+    r.partials = r.scr.p;
+    r.ticket = (unsigned *)((char *)r.scr.p + kSlotBytes);
+    hipLaunchKernelGGL(zero_ticket, dim3(1), dim3(64), 0, st, r.ticket);
+    return r;
 }
 
 int red_blocks(int n)
@@ -634,7 +657,8 @@ int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const dou
         PROF_END(s);
         return (int)hipGetLastError();
     }
-    RedSlot r = next_slot();
+    RedSlot r = next_slot(s);
+    if (!r.partials) return (int)hipErrorOutOfMemory;
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "ddot");
     hipLaunchKernelGGL(dot_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
@@ -660,7 +684,8 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
         PROF_END(s);
         return (int)hipGetLastError();
     }
-    RedSlot r = next_slot();
+    RedSlot r = next_slot(s);
+    if (!r.partials) return (int)hipErrorOutOfMemory;
     int blocks = red_blocks(n);
     if (blocks > kMaxBlocks / 2) blocks = kMaxBlocks / 2;  // 3 partials per block
     PROF_BEGIN(s, "dnrm2");
@@ -693,7 +718,8 @@ int launch_iamax(void *stream, int n, const double *d_x, int incx, int *h_result
         PROF_END(s);
         return (int)hipGetLastError();
     }
-    RedSlot r = next_slot();
+    RedSlot r = next_slot(s);
+    if (!r.partials) return (int)hipErrorOutOfMemory;
     int blocks = red_blocks(n);
     PROF_BEGIN(s, name);
     hipLaunchKernelGGL(iamax_kernel<MIN>, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,
@@ -779,7 +805,8 @@ int gpublas_dev_dasum(void *stream, int n, const double *d_x, int incx, double *
         PROF_END(s);
         return (int)hipGetLastError();
     }
-    RedSlot r = next_slot();
+    RedSlot r = next_slot(s);
+    if (!r.partials) return (int)hipErrorOutOfMemory;
     int blocks = red_blocks(n);
     PROF_BEGIN(s, "dasum");
     hipLaunchKernelGGL(asum_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx,

@@ -480,15 +480,32 @@ __global__ __launch_bounds__(256) void spr_kernel(int n, double alpha, const dou
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
 inline bool isN(char c) { return c == 'N' || c == 'n'; }
 
-}  // namespace
-
-extern "C" {
-
-int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
-                      int lda, const double *d_x, int incx, double beta, double *d_y, int incy)
+// Column-panel plan of DGEMV 'N': nrb row blocks of GN_ROWS rows times ncb
+// panels of cbw columns (~2 workgroups per CU, panels at least 16 columns
+// wide).  With ncb > 1 the panels' partial sums (ncb x m doubles) are folded
+// in panel order by a second kernel.
+struct GemvNPlan {
+    int nrb, ncb, cbw;
+};
+GemvNPlan gemv_n_plan(int m, int n)
 {
-    hipStream_t s = pick_stream(stream);
+    GemvNPlan p;
+    p.nrb = ceil_div(m, GN_ROWS);
+    p.ncb = ceil_div(512, p.nrb);
+    const int maxcb = ceil_div(n, 16);
+    if (p.ncb > maxcb) p.ncb = maxcb;
+    if (p.ncb < 1) p.ncb = 1;
+    p.cbw = ceil_div(n, p.ncb);
+    p.ncb = ceil_div(n, p.cbw);
+    return p;
+}
+
+// One DGEMV on stream s; the partials come from the stream's scratch.
+int gemv_run(hipStream_t s, char trans, int m, int n, double alpha, const double *d_A, int lda,
+             const double *d_x, int incx, double beta, double *d_y, int incy)
+{
     if (m <= 0 || n <= 0) return 0;
+    const bool prof = g_prof_on;
     const bool nt = isN(trans);
     const int lenx = nt ? n : m, leny = nt ? m : n;
     const long x0 = start_of(lenx, incx), y0 = start_of(leny, incy);
@@ -501,41 +518,46 @@ int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, cons
         return (int)hipGetLastError();
     }
     if (nt) {
-        // ~2 workgroups per CU, each panel at least 16 columns wide
-        const int nrb = ceil_div(m, GN_ROWS);
-        int ncb = ceil_div(512, nrb);
-        const int maxcb = ceil_div(n, 16);
-        if (ncb > maxcb) ncb = maxcb;
-        if (ncb < 1) ncb = 1;
-        const int cbw = ceil_div(n, ncb);
-        ncb = ceil_div(n, cbw);
+        const GemvNPlan pl = gemv_n_plan(m, n);
         double *partials = nullptr;
-        if (ncb > 1) {
-            partials = (double *)dev_workspace((size_t)ncb * m * sizeof(double));
+        Scratch scr;
+        if (pl.ncb > 1) {
+            scr = dev_scratch((size_t)pl.ncb * m * sizeof(double), s);
+            partials = (double *)scr.p;
             if (!partials) return (int)hipErrorOutOfMemory;
         }
         const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
-        PROF_BEGIN(s, "dgemv_n");
-        hipLaunchKernelGGL(gemv_n_stream, dim3((unsigned)nrb * ncb), dim3(256), 0, s, m, n, alpha,
-                           d_A, (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0, nrb,
-                           cbw, ncb == 1, vec, partials);
-        PROF_END(s);
-        if (ncb > 1) {
-            PROF_BEGIN(s, "dgemv_n_fold");
+        if (prof) prof_begin(s, "dgemv_n");
+        hipLaunchKernelGGL(gemv_n_stream, dim3((unsigned)pl.nrb * pl.ncb), dim3(256), 0, s, m, n,
+                           alpha, d_A, (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0,
+                           pl.nrb, pl.cbw, pl.ncb == 1, vec, partials);
+        if (prof) prof_end(s);
+        if (pl.ncb > 1) {
+            if (prof) prof_begin(s, "dgemv_n_fold");
             hipLaunchKernelGGL(gemv_n_fold, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, m,
-                               ncb, partials, alpha, beta, d_y, (long)incy, y0);
-            PROF_END(s);
+                               pl.ncb, partials, alpha, beta, d_y, (long)incy, y0);
+            if (prof) prof_end(s);
         }
         return (int)hipGetLastError();
     }
     const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0) && (((uintptr_t)d_x & 15) == 0);
     int blocks = ceil_div(n, 4);
     if (blocks > 2048) blocks = 2048;
-    PROF_BEGIN(s, "dgemv_t");
+    if (prof) prof_begin(s, "dgemv_t");
     hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)blocks), dim3(256), 0, s, m, n, alpha, d_A,
                        (long)lda, d_x, (long)incx, x0, beta, d_y, (long)incy, y0, vec);
-    PROF_END(s);
+    if (prof) prof_end(s);
     return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
+                      int lda, const double *d_x, int incx, double beta, double *d_y, int incy)
+{
+    return gemv_run(pick_stream(stream), trans, m, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy);
 }
 
 int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x, int incx,
@@ -568,7 +590,8 @@ int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double
     const int lower = uplo == 'L' || uplo == 'l';
     const int nb = ceil_div(n, SY_T);
     const long tiles = (long)nb * (nb + 1) / 2;
-    double *part = (double *)dev_workspace(2 * (size_t)nb * n * sizeof(double));
+    Scratch scr = dev_scratch(2 * (size_t)nb * n * sizeof(double), s);
+    double *part = (double *)scr.p;
     if (!part) return (int)hipErrorOutOfMemory;
     double *rowP = part, *colP = part + (size_t)nb * n;
     PROF_BEGIN(s, "dsymv");

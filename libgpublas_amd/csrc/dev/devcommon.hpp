@@ -36,12 +36,40 @@ DevCtx *ctx_of(hipStream_t stream);
 // first use; returns it to the pool at thread exit -- streams are never
 // destroyed, so a stream handle recorded by the host layer stays valid).  The
 // stream argument of the C ABI may be NULL (use this thread's stream) or a
-// stream of the caller's; workspace and tickets always come from the calling
-// thread's context, so two threads never share a workspace.
+// stream of the caller's.
 DevCtx *dev_ctx();
-// Workspace of at least `bytes` on the calling thread's context (grows).
-void *dev_workspace(size_t bytes);
 void dev_release_after(void *d_ptr, hipStream_t s);  // pool release ordered behind stream s
+
+// Scratch memory (partial sums, tickets) of one call whose kernels run on
+// stream s.  On the calling thread's own stream it is the thread's workspace:
+// only that stream ever uses it, so growing it after a sync of that stream is
+// safe.  On any other stream it is a pool block, released behind s (once the
+// work queued on s so far completes) when the Scratch goes out of scope --
+// after the call has enqueued its launches.  Two calls on two streams thus
+// never share partials, and no queued kernel loses its workspace.
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    bool pooled = false;
+    Scratch() = default;
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    Scratch(Scratch &&o) noexcept : p(o.p), s(o.s), pooled(o.pooled) { o.pooled = false; }
+    Scratch &operator=(Scratch &&o) noexcept
+    {
+        if (this != &o) {
+            if (pooled && p) dev_release_after(p, s);
+            p = o.p, s = o.s, pooled = o.pooled;
+            o.pooled = false;
+        }
+        return *this;
+    }
+    ~Scratch()
+    {
+        if (pooled && p) dev_release_after(p, s);
+    }
+};
+Scratch dev_scratch(size_t bytes, hipStream_t s);
 
 static constexpr int kTicketSlots = 4096;  // ticket counters per context
 

@@ -462,15 +462,20 @@ int exp_chunks(int R, int K)
 // Workspace of one split call: the non-finite flag, then room for the
 // partial line maxima of the strided exponent pass.
 struct SplitWs {
-    int *bad;
-    double *part;
+    Scratch scr;
+    int *bad = nullptr;
+    double *part = nullptr;
 };
-SplitWs split_workspace(int R1, int R2, int K)
+SplitWs split_workspace(hipStream_t st, int R1, int R2, int K)
 {
     const size_t part = (size_t)std::max((long)exp_chunks(R1, K) * R1, (long)exp_chunks(R2, K) * R2);
-    char *w = (char *)dev_workspace(256 + part * sizeof(double));
-    if (!w) return {nullptr, nullptr};
-    return {(int *)w, (double *)(w + 256)};
+    SplitWs ws;
+    ws.scr = dev_scratch(256 + part * sizeof(double), st);
+    char *w = (char *)ws.scr.p;
+    if (!w) return ws;
+    ws.bad = (int *)w;
+    ws.part = (double *)(w + 256);
+    return ws;
 }
 
 // Exponents and slices of an R x K operand given by (X, ld, rcontig).
@@ -575,7 +580,7 @@ int gpublas_dev_dgemm_split_fmt(void *stream, int slices, int fmt, char transa, 
         return gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
     const int f16 = fmt == GPUBLAS_SLICE_FP16;
     const int KT = ceil_div(k, 16);
-    const SplitWs ws = split_workspace(m, n, k);
+    SplitWs ws = split_workspace(st, m, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
     int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
@@ -622,7 +627,7 @@ int gpublas_dev_dsyrk_split_fmt(void *stream, int slices, int fmt, char uplo, ch
         return gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
     const int f16 = fmt == GPUBLAS_SLICE_FP16;
     const int KT = ceil_div(k, 16);
-    const SplitWs ws = split_workspace(n, n, k);
+    SplitWs ws = split_workspace(st, n, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
     int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);

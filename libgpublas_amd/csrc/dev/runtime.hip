@@ -263,11 +263,20 @@ DevCtx *ctx_of(hipStream_t stream)
     return nullptr;
 }
 
-void *dev_workspace(size_t bytes)
+Scratch dev_scratch(size_t bytes, hipStream_t s)
 {
+    Scratch r;
     DevCtx *c = dev_ctx();
+    r.s = s ? s : c->stream;
+    if (r.s != c->stream) {
+        r.p = gpublas_dev_alloc(bytes);
+        r.pooled = r.p != nullptr;
+        return r;
+    }
     if (c->ws_bytes < bytes) {
         if (c->ws) {
+            // the workspace is only ever used on c->stream: once that stream
+            // has drained, no kernel can still be using it
             (void)hipStreamSynchronize(c->stream);
             gpublas_dev_release(c->ws);
         }
@@ -275,7 +284,8 @@ void *dev_workspace(size_t bytes)
         c->ws = gpublas_dev_alloc(want);
         c->ws_bytes = c->ws ? want : 0;
     }
-    return c->ws;
+    r.p = c->ws;
+    return r;
 }
 
 void prof_begin(hipStream_t s, const char *name)

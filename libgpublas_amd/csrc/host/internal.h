@@ -162,6 +162,14 @@ void gb_acquire_matrix(gb_operand *op, const void *host, size_t ld, size_t rows,
 /* Vector of n elements with stride inc (span 1+(n-1)|inc|). */
 void gb_acquire_vector(gb_operand *op, const void *host, int n, int inc, size_t es, int mode,
                        void *stream);
+/* Elements of a packed triangle of order n; n(n+1)/2 passes INT_MAX from
+ * n = 65536, so it is computed in size_t. */
+static inline size_t gb_packed_len(int n)
+{
+    return n > 0 ? (size_t)n * ((size_t)n + 1) / 2 : 0;
+}
+/* Packed triangle of order n: one contiguous span of gb_packed_len(n) elements. */
+void gb_acquire_packed(gb_operand *op, const void *host, int n, size_t es, int mode, void *stream);
 /* Copy outputs back (untracked) or mark them device-resident (tracked).
  * Call after the kernel launch. */
 void gb_release(gb_operand *op, void *stream);

@@ -719,7 +719,7 @@ static void spr_common(int two, const char *uplo, const int *n, const double *al
     memset(ops, 0, sizeof ops);
     gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
     if (two) gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
-    gb_acquire_vector(&ops[2], ap, (int)len, 1, 8, GB_INOUT, s);
+    gb_acquire_packed(&ops[2], ap, *n, 8, GB_INOUT, s);
     int e = two ? gpublas_dev_dspr2(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
                                     (const double *)ops[1].dev, *incy, (double *)ops[2].dev)
                 : gpublas_dev_dspr(s, *uplo, *n, *alpha, (const double *)ops[0].dev, *incx,
@@ -790,7 +790,7 @@ static void tri_common(int kind, const char *uplo, const char *trans, const char
     gb_operand ops[2];
     memset(ops, 0, sizeof ops);
     if (kind == TRI_MV) gb_acquire_matrix(&ops[0], a, *lda, *n, *n, 8, GB_IN, s);
-    else gb_acquire_vector(&ops[0], a, (int)tri, 1, 8, GB_IN, s);
+    else gb_acquire_packed(&ops[0], a, *n, 8, GB_IN, s);
     gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_INOUT, s);
     int e;
     if (kind == TRI_MV)
@@ -863,7 +863,7 @@ void dspmv_(const char *uplo, const int *n, const double *alpha, const double *a
     gb_operand ops[3];
     memset(ops, 0, sizeof ops);
     if (*alpha != 0.0) {
-        gb_acquire_vector(&ops[0], ap, (int)tri, 1, 8, GB_IN, s);
+        gb_acquire_packed(&ops[0], ap, *n, 8, GB_IN, s);
         gb_acquire_vector(&ops[1], x, *n, *incx, 8, GB_IN, s);
     }
     gb_acquire_vector(&ops[2], y, *n, *incy, 8, *beta == 0.0 ? GB_OUT : GB_INOUT, s);

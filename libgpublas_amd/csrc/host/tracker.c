@@ -490,6 +490,16 @@ void gb_acquire_vector(gb_operand *op, const void *host, int n, int inc, size_t 
     gb_acquire_matrix(op, host, span, span, 1, es, mode, stream);
 }
 
+void gb_acquire_packed(gb_operand *op, const void *host, int n, size_t es, int mode, void *stream)
+{
+    const size_t len = gb_packed_len(n);
+    if (!len) {
+        memset(op, 0, sizeof *op);
+        return;
+    }
+    gb_acquire_matrix(op, host, len, len, 1, es, mode, stream);
+}
+
 void gb_release(gb_operand *op, void *stream)
 {
     if (!op->dev) return;

@@ -557,3 +557,23 @@ def test_cblas_error_positions_fp32_level3():
     dd = np.zeros(16)
     lib.cblas_dgemv(ROW, NOTRANS, 3, 4, 1.0, _ptr(dd), 3, _ptr(dd), 1, 0.0, _ptr(dd), 1)
     assert gb.last_xerbla() == ("DGEMV ", 7)  # row-major: lda < n
+
+
+def test_packed_triangle_length_beyond_int_range(tmp_path):
+    """Packed routines acquire n(n+1)/2 elements; from n = 65536 that count
+    passes INT_MAX, so the host layer computes it in size_t (a 32-bit cast
+    would hand the device a negative length and a missing operand)."""
+    src = tmp_path / "packed_len.c"
+    src.write_text('#include "internal.h"\n#include <stdio.h>\n'
+                   'int main(void) { printf("%zu %zu %zu %zu %zu\\n", gb_packed_len(-1), '
+                   'gb_packed_len(1), gb_packed_len(65535), gb_packed_len(65536), '
+                   'gb_packed_len(2000000000)); return 0; }\n')
+    exe = tmp_path / "packed_len"
+    inc = [os.path.join(ROOT, "include"), os.path.join(ROOT, "libgpublas_amd", "csrc", "host")]
+    subprocess.run(["gcc", "-std=gnu11", "-I", inc[0], "-I", inc[1], "-o", str(exe), str(src)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    n = 2000000000
+    assert got == [0, 1, 65535 * 65536 // 2, 65536 * 65537 // 2, n * (n + 1) // 2]
+    assert got[3] > 2**31 - 1
