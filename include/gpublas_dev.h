@@ -149,6 +149,23 @@ int gpublas_dev_dasum(void *stream, int n, const double *d_x, int incx, double *
 /* ---- Level 2 ---- */
 int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
                       int lda, const double *d_x, int incx, double beta, double *d_y, int incy);
+/* Allocation-free DGEMV.  No counterpart in the reference: it is for callers
+ * that cannot allocate -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments, kernels and results (bit
+ * for bit) as gpublas_dev_dgemv, but it allocates nothing: the column-panel
+ * partials of trans 'N' live in a fixed 4 MiB device array per GPU (2^19
+ * doubles, enough for every shape; held until the process ends), whose
+ * address gpublas_dev_init() takes once per device.  Differences: `stream`
+ * is used as given (NULL = the device's null stream, not the calling
+ * thread's stream), no kernel timing is recorded, and it returns
+ * hipErrorNotInitialized if gpublas_dev_init() has not run (call it once
+ * first) and hipErrorInvalidValue if a shape ever needed more partials than
+ * the array holds.  Every call on one GPU shares that array: calls that
+ * overlap in time on one GPU (two streams, two threads) race and must be
+ * ordered by the caller. */
+int gpublas_dev_dgemv_noalloc(void *stream, char trans, int m, int n, double alpha,
+                              const double *d_A, int lda, const double *d_x, int incx, double beta,
+                              double *d_y, int incy);
 int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x, int incx,
                      const double *d_y, int incy, double *d_A, int lda);
 int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,

@@ -500,12 +500,29 @@ GemvNPlan gemv_n_plan(int m, int n)
     return p;
 }
 
-// One DGEMV on stream s; the partials come from the stream's scratch.
+// ---- scratch of the allocation-free DGEMV entry point ------------------------
+// The panel partials of gpublas_dev_dgemv_noalloc live in this fixed array
+// instead of a pool block.  ncb > 1 needs nrb <= 511, and then
+// ncb * m <= ceil(512 / nrb) * GN_ROWS * nrb <= 512 * (512 + 511) = 523,776
+// doubles, so 2^19 doubles (4 MiB per GPU, held until the process ends) fit
+// every shape.  Host code reaches the array only through the address that
+// noalloc_scratch_setup() takes with hipGetSymbolAddress on each device; the
+// array's own name is a host-side stand-in that kernels cannot use.
+constexpr size_t kNoallocPartials = size_t(1) << 19;
+__device__ __align__(256) double scratch[kNoallocPartials];
+constexpr int kScratchDevices = 64;
+double *g_scratch_ptr[kScratchDevices];
+
+// One DGEMV on stream s.  fixed_partials = nullptr: the partials come from
+// the stream's scratch (dev_scratch, which may allocate); otherwise they are
+// fixed_partials (kNoallocPartials doubles, no allocation, no profiling
+// events).
 int gemv_run(hipStream_t s, char trans, int m, int n, double alpha, const double *d_A, int lda,
-             const double *d_x, int incx, double beta, double *d_y, int incy)
+             const double *d_x, int incx, double beta, double *d_y, int incy,
+             double *fixed_partials)
 {
     if (m <= 0 || n <= 0) return 0;
-    const bool prof = g_prof_on;
+    const bool prof = g_prof_on && !fixed_partials;
     const bool nt = isN(trans);
     const int lenx = nt ? n : m, leny = nt ? m : n;
     const long x0 = start_of(lenx, incx), y0 = start_of(leny, incy);
@@ -522,9 +539,15 @@ int gemv_run(hipStream_t s, char trans, int m, int n, double alpha, const double
         double *partials = nullptr;
         Scratch scr;
         if (pl.ncb > 1) {
-            scr = dev_scratch((size_t)pl.ncb * m * sizeof(double), s);
-            partials = (double *)scr.p;
-            if (!partials) return (int)hipErrorOutOfMemory;
+            const size_t need = (size_t)pl.ncb * m;
+            if (fixed_partials) {
+                if (need > kNoallocPartials) return (int)hipErrorInvalidValue;
+                partials = fixed_partials;
+            } else {
+                scr = dev_scratch(need * sizeof(double), s);
+                partials = (double *)scr.p;
+                if (!partials) return (int)hipErrorOutOfMemory;
+            }
         }
         const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
         if (prof) prof_begin(s, "dgemv_n");
@@ -552,12 +575,39 @@ int gemv_run(hipStream_t s, char trans, int m, int n, double alpha, const double
 
 }  // namespace
 
+// Takes the device address of `scratch` on the current device (called by the
+// library's init with that device current).  Returns 0 or the HIP error.
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= kScratchDevices) return (int)hipErrorInvalidDevice;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+    __atomic_store_n(&g_scratch_ptr[device], (double *)ptr, __ATOMIC_RELEASE);
+    return 0;
+}
+
 extern "C" {
 
 int gpublas_dev_dgemv(void *stream, char trans, int m, int n, double alpha, const double *d_A,
                       int lda, const double *d_x, int incx, double beta, double *d_y, int incy)
 {
-    return gemv_run(pick_stream(stream), trans, m, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy);
+    return gemv_run(pick_stream(stream), trans, m, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy,
+                    nullptr);
+}
+
+int gpublas_dev_dgemv_noalloc(void *stream, char trans, int m, int n, double alpha,
+                              const double *d_A, int lda, const double *d_x, int incx, double beta,
+                              double *d_y, int incy)
+{
+    hipDevice_t dev = -1;
+    if (hipStreamGetDevice((hipStream_t)stream, &dev) != hipSuccess || dev < 0 ||
+        dev >= kScratchDevices)
+        return (int)hipErrorInvalidDevice;
+    double *partials = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
+    if (!partials) return (int)hipErrorNotInitialized;  // gpublas_dev_init() has not run here
+    return gemv_run((hipStream_t)stream, trans, m, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy,
+                    partials);
 }
 
 int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x, int incx,

@@ -73,6 +73,11 @@ Scratch dev_scratch(size_t bytes, hipStream_t s);
 
 static constexpr int kTicketSlots = 4096;  // ticket counters per context
 
+// Records, per device, the address of the fixed scratch array of the
+// allocation-free DGEMV (blas2.hip); the library's init calls it once with
+// the device current.  Returns 0 or a HIP error code.
+int noalloc_scratch_setup(int device);
+
 // Kernel timing (HIP events on the launch stream), enabled by
 // gpublas_dev_prof_enable.  Wrap each launch as
 //   PROF_BEGIN(stream, "name"); hipLaunchKernelGGL(...); PROF_END(stream);

@@ -190,6 +190,10 @@ void do_init()
         g_init_status = -3;  // code objects are built for gfx950 only
         return;
     }
+    if (noalloc_scratch_setup(0) != 0) {  // device 0 is current (hipSetDevice above)
+        g_init_status = -4;
+        return;
+    }
     g_init_status = 0;
 }
 
