@@ -224,7 +224,12 @@ def level3_suite(reps):
                       "sgemm_tflops_min": min(gemm32), "sgemm_tflops_max": max(gemm32)}), flush=True)
 
 
-def split_suite(n=8192, slices=(3, 4, 5, 6)):
+SPLIT_VARIANTS = [("bf16", s) for s in (3, 4, 5, 6)] + [("fp16", 6)] + \
+    [("int8", s) for s in (4, 5, 6, 7, 8)]
+SLICE_FMT = {"bf16": 0, "fp16": 1, "int8": 2}
+
+
+def split_suite(n=8192, variants=SPLIT_VARIANTS):
     """Split-precision DGEMM M=N=K=n against plain FP64 DGEMM: (a) uniform[-1,1];
     (b) uniform entries with every row of A and column of B scaled by
     2^U[-20,20].  Errors are measured against the host BLAS (OpenBLAS, all
@@ -268,18 +273,19 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
         host_s = time.perf_counter() - t0
         normR = np.linalg.norm(R)
         maxR = np.max(np.abs(R))
-        variants = [("fp64", 0, None)] + [("split", s, f) for f in ("bf16", "fp16") for s in slices]
-        for kind, s, fmt in variants:
+        runs = [("fp64", 0, None)] + [("split", s, f) for f, s in variants]
+        for kind, s, fmt in runs:
             for rep in range(3):
                 if rep == 1:
                     lib.gpublas_dev_prof_reset()
                 if kind == "fp64":
                     lib.gpublas_dev_dgemm(None, b"N", b"N", n, n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
                 else:
-                    lib.gpublas_dev_dgemm_split_fmt(None, s, 1 if fmt == "fp16" else 0, b"N", b"N", n,
-                                                    n, n, 1.0, dA, n, dB, n, 0.0, dC, n)
+                    lib.gpublas_dev_dgemm_split_fmt(None, s, SLICE_FMT[fmt], b"N", b"N", n, n, n,
+                                                    1.0, dA, n, dB, n, 0.0, dC, n)
             lib.gpublas_dev_sync(None)
-            ms = ktime(["dgemm_fast_128x128"] if kind == "fp64" else ["ozaki_split", "ozaki_gemm"]) / 2
+            ms = ktime(["dgemm_fast_128x128"] if kind == "fp64" else
+                       ["ozaki_split", "ozaki_i8" if fmt == "int8" else "ozaki_gemm"]) / 2
             lib.gpublas_dev_d2h(host.ctypes.data, dC, bytes_, None)
             lib.gpublas_dev_sync(None)
             err_norm = float(np.linalg.norm(host - R) / normR)
@@ -288,7 +294,8 @@ def split_suite(n=8192, slices=(3, 4, 5, 6)):
             rec = {"suite": "split", "dist": dist, "n": n, "kind": kind, "slices": s, "slice_format": fmt,
                    "slice_products": npairs, "kernel_ms": round(ms, 3),
                    "effective_tflops": round(2.0 * n ** 3 / (ms / 1e3) / 1e12, 2),
-                   "bf16_mfma_tflops": round(npairs * 2.0 * n ** 3 / (ms / 1e3) / 1e12, 1) if s else None,
+                   ("int8_mfma_tops" if fmt == "int8" else "bf16_mfma_tflops"):
+                       round(npairs * 2.0 * n ** 3 / (ms / 1e3) / 1e12, 1) if s else None,
                    "normwise_rel_err": err_norm, "max_rel_err": err_max,
                    "reference": f"OpenBLAS dgemm on the host ({host_s:.1f} s)"}
             print(json.dumps(rec), flush=True)

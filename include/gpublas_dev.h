@@ -124,6 +124,10 @@ int gpublas_dev_strmm(void *stream, char side, char uplo, char transa, char diag
  * points without _fmt use BF16. */
 #define GPUBLAS_SLICE_BF16 0
 #define GPUBLAS_SLICE_FP16 1
+/* INT8: signed 7-bit integer slices (1..8), products on
+ * v_mfma_i32_32x32x32_i8 with exact int32 accumulation per diagonal p + q
+ * (Ootomo, Ozaki & Yokota, IJHPCA 2024); s slices carry 7s bits. */
+#define GPUBLAS_SLICE_INT8 2
 int gpublas_dev_dgemm_split(void *stream, int slices, char transa, char transb, int m, int n,
                             int k, double alpha, const double *d_A, int lda, const double *d_B,
                             int ldb, double beta, double *d_C, int ldc);

@@ -159,6 +159,54 @@ __global__ __launch_bounds__(256) void split_kernel(int R, int Rp, int K, int KT
     }
 }
 
+// INT8 slices (Ootomo, Ozaki & Yokota, "DGEMM on integer matrix
+// multiplication unit", IJHPCA 2024, in the same integer-slice form): the
+// scaled entry x (|x| < 1) is cut into s signed 7-bit integers
+//   t = 128 x;  q_p = trunc(t);  x = t - q_p   (all exact, |q_p| <= 127)
+// so a = 2^(ea-7) sum_p q_p 2^-7p + r with |r| < 2^(ea-7s).  Layout
+// S8[p][k/32][row][32] (rows padded to 128, k to 32): a 32-deep k slab of 32
+// rows is one contiguous kilobyte, the unit the product kernel moves from
+// HBM into LDS.  Same 32 x 64 FP64 tile through LDS as split_kernel; 8
+// threads write a row's 32 bytes of a slab, four slices digits per store.
+__global__ __launch_bounds__(256) void split_i8_kernel(int R, int Rp, int K, int KT, int s,
+                                                       const double *X, long ld, int rcontig,
+                                                       const int *e, int8_t *S)
+{
+    __shared__ double t[32][65];
+    const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 64;
+    for (int idx = threadIdx.x; idx < 32 * 64; idx += 256) {
+        int rr, kk;
+        if (rcontig) { rr = idx % 32; kk = idx / 32; }  // r fastest: X[r + k*ld]
+        else { kk = idx % 64; rr = idx / 64; }          // k fastest: X[k + r*ld]
+        const long r = r0 + rr, k = k0 + kk;
+        double v = 0;
+        if (r < R && k < K) v = rcontig ? X[r + k * ld] : X[k + r * ld];
+        t[rr][kk] = v;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 2 * 32 * 8; idx += 256) {
+        const int kb = idx / 256, rr = (idx % 256) / 8, kg = idx % 8;
+        const int kt = blockIdx.y * 2 + kb;
+        const long r = r0 + rr;
+        if (kt >= KT || r >= Rp) continue;
+        const int kk = kb * 32 + 4 * kg;
+        const int er = r < R ? e[r] : 0;
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = ldexp(t[rr][kk + j], -er);
+        for (int p = 0; p < s; ++p) {
+            unsigned w = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double u = x[j] * 128.0, q = trunc(u);
+                x[j] = u - q;
+                w |= ((unsigned)(int)q & 0xffu) << (8 * j);
+            }
+            *(unsigned *)(S + (((long)p * KT + kt) * Rp + r) * 32 + 4 * kg) = w;
+        }
+    }
+}
+
 // ---- slice products ----------------------------------------------------------
 typedef const __attribute__((address_space(1))) void *gptr_t;
 typedef __attribute__((address_space(3))) void *lptr_t;
@@ -444,10 +492,215 @@ __global__ __launch_bounds__(64 * NW, OCC) void ozaki_gemm_kernel(OzArgs g)
     }
 }
 
+// ---- INT8 slice products ------------------------------------------------------
+// v_mfma_i32_32x32x32_i8 sums 32 products of 7-bit integers exactly in
+// int32, so every diagonal d = p + q has one exact integer accumulator over
+// the whole k range of a launch: diagonal d holds d + 1 <= s pairs, each
+// adding at most 32 * 127^2 per 32-deep slab, so a launch covers at most
+// i8_max_slabs(s) slabs (k up to 2^31 / (127^2 s), 16k-18k for s = 7, 8;
+// longer k runs as several launches accumulating into C).  The epilogue
+// weighs diagonal d by 2^-7d in FP64, smallest first, and applies
+// 2^(ea_i + eb_j - 14), alpha and beta.
+constexpr int i8_max_slabs(int s) { return (int)(2147483647L / ((long)s * 32 * 127 * 127)); }
+
+struct OzI8Args {
+    int M, N, Mp, Np, KT;   // KT: 32-deep k slabs stored per slice
+    int kt0, nkt;           // slabs [kt0, kt0 + nkt) in this launch
+    const int8_t *SA, *SB;  // [s][KT][Mp][32], [s][KT][Np][32]
+    const int *ea, *eb;
+    double alpha, beta;
+    double *C;
+    long ldc;
+    int tiles_m, tiles_n;
+    int tri;
+};
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+template <int NSL, int TM, int TN> struct OzStage8 {
+    __attribute__((aligned(16))) int8_t r[NSL][TM + TN][32];
+};
+
+// Same pipeline as ozaki_gemm_kernel: one workgroup per TM x 64 tile of C,
+// one 32 x 32 MFMA tile per wave, 32-deep slabs of all S slices of the A and
+// B tiles moved HBM -> LDS by global_load_lds through a ring of up to four
+// stages, the next slab's fragments read between the MFMA rows of the
+// current one.
+template <int S, int TM, int NW, int OCC>
+__global__ __launch_bounds__(64 * NW, OCC) void ozaki_i8_kernel(OzI8Args g)
+{
+    constexpr int TN = 64;
+    constexpr int MW = NW / 2;
+    constexpr int MI = TM / (32 * MW);
+    constexpr int D = oz_stages(S, TM, OCC);
+    constexpr int P = (TM + TN) / 32;
+    constexpr int PW = (P + NW - 1) / NW;
+    static_assert((TM == 64 || TM == 128) && MI >= 1 && D >= 2 && D <= 4 && PW * S * D < 64,
+                  "slab pipeline shape");
+    __shared__ OzStage8<S, TM, TN> L0, L1;
+    __shared__ OzStage8<(D > 2 ? S : 1), TM, TN> L2;
+    __shared__ OzStage8<(D > 3 ? S : 1), TM, TN> L3;
+    auto stage = [&](auto jc) -> int8_t(*)[TM + TN][32] {
+        constexpr int J = decltype(jc)::value;
+        if constexpr (J == 0) return L0.r;
+        else if constexpr (J == 1) return L1.r;
+        else if constexpr (J == 2) return L2.r;
+        else return L3.r;
+    };
+
+    int tm, tn;
+    tile_of(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
+    const int m0 = tm * TM, n0 = tn * TN;
+    if (g.tri == 1 && n0 > m0 + TM - 1) return;
+    if (g.tri == 2 && n0 + TN - 1 < m0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % MW, wn = wave / MW;
+
+    v16i acc[S][MI];
+#pragma unroll
+    for (int c = 0; c < S; ++c)
+#pragma unroll
+        for (int i = 0; i < MI; ++i) acc[c][i] = v16i{};
+
+    const int KT = g.KT, kt0 = g.kt0, NK = g.nkt;
+    // piece x (32 rows x 32 B of one slice) is moved by wave x % NW, 16 B
+    // per lane; the two halves of a row are swapped when bit 3 of the row is
+    // set so that fragment reads are bank-conflict free (as for BF16)
+    const int half_src = (lane & 1) ^ ((lane >> 4) & 1);
+    long src0[PW];
+    bool mine[PW], isa[PW];
+#pragma unroll
+    for (int u = 0; u < PW; ++u) {
+        const int x = wave + NW * u;
+        mine[u] = x < P;
+        isa[u] = 32 * x < TM;
+        src0[u] = (isa[u] ? (long)(m0 + 32 * x) : (long)(n0 + 32 * x - TM)) * 32 +
+                  (lane >> 1) * 32 + half_src * 16;
+    }
+    int nmine = 0;
+#pragma unroll
+    for (int u = 0; u < PW; ++u) nmine += mine[u] ? 1 : 0;
+    auto issue_slice = [&](int kt, int p, auto jc) {
+        int8_t(*ls)[TM + TN][32] = stage(jc);
+        const long slab = (long)p * KT + kt0 + kt;
+#pragma unroll
+        for (int u = 0; u < PW; ++u) {
+            if (!mine[u]) continue;
+            const int8_t *src = isa[u] ? g.SA + slab * g.Mp * 32 + src0[u]
+                                       : g.SB + slab * g.Np * 32 + src0[u];
+            __builtin_amdgcn_global_load_lds((gptr_t)src, (lptr_t)&ls[p][32 * (wave + NW * u)][0],
+                                             16, 0, 0);
+        }
+    };
+    auto issue = [&](int kt, auto jc) {
+#pragma unroll
+        for (int p = 0; p < S; ++p) issue_slice(kt, p, jc);
+    };
+    struct Frag {
+        v4i b[S];
+        v4i a[S][MI];
+    };
+    const int hr = 16 * ((lane >> 5) ^ ((lane >> 3) & 1));
+    auto read_b = [&](Frag &f, int q, auto jc) {
+        int8_t(*ls)[TM + TN][32] = stage(jc);
+        f.b[q] = *(const v4i *)&ls[q][TM + wn * 32 + (lane & 31)][hr];
+    };
+    auto read_a = [&](Frag &f, int p, auto jc) {
+        int8_t(*ls)[TM + TN][32] = stage(jc);
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+            f.a[p][i] = *(const v4i *)&ls[p][wm * 32 * MI + i * 32 + (lane & 31)][hr];
+    };
+    auto wait_slab = [&](bool drain, auto later_c) {
+        constexpr int LATER = decltype(later_c)::value;
+        if (drain) __builtin_amdgcn_s_waitcnt(vm_wait(0));
+        else if (nmine == PW) __builtin_amdgcn_s_waitcnt(vm_wait(PW * S * LATER));
+        else __builtin_amdgcn_s_waitcnt(vm_wait((PW - 1) * S * LATER));
+    };
+    auto barrier = [&]() {
+        __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));  // lgkmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    };
+    Frag cur, nxt;
+    auto step = [&](int kt, auto jc) {
+        constexpr int J = decltype(jc)::value;
+        constexpr int J1 = (J + 1) % D;
+        const bool more = kt + 1 < NK;
+        if (more) {
+            wait_slab(kt + D - 1 >= NK, std::integral_constant<int, D - 2>{});
+            barrier();
+        }
+        const bool refill = kt + D < NK;
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+#pragma unroll
+            for (int q = 0; q + p < S; ++q)
+#pragma unroll
+                for (int i = 0; i < MI; ++i)
+                    acc[p + q][i] = __builtin_amdgcn_mfma_i32_32x32x32_i8(cur.b[q], cur.a[p][i],
+                                                                          acc[p + q][i], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (more) {
+                read_a(nxt, p, std::integral_constant<int, J1>{});
+                read_b(nxt, S - 1 - p, std::integral_constant<int, J1>{});
+            }
+            if (refill) issue_slice(kt + D, p, jc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        cur = nxt;
+    };
+
+    issue(0, std::integral_constant<int, 0>{});
+    if (1 < NK) issue(1, std::integral_constant<int, 1 % D>{});
+    if constexpr (D > 2) if (2 < NK) issue(2, std::integral_constant<int, 2 % D>{});
+    if constexpr (D > 3) if (3 < NK) issue(3, std::integral_constant<int, 3 % D>{});
+    wait_slab(D - 1 >= NK, std::integral_constant<int, D - 1>{});
+    barrier();
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        read_a(cur, p, std::integral_constant<int, 0>{});
+        read_b(cur, p, std::integral_constant<int, 0>{});
+    }
+    step(0, std::integral_constant<int, 0>{});
+    for (int kt = 1; kt < NK; kt += D) {
+        step(kt, std::integral_constant<int, 1 % D>{});
+        if constexpr (D > 2) if (kt + 1 < NK) step(kt + 1, std::integral_constant<int, 2 % D>{});
+        if constexpr (D > 3) if (kt + 2 < NK) step(kt + 2, std::integral_constant<int, 3 % D>{});
+        if (kt + D - 1 < NK) step(kt + D - 1, std::integral_constant<int, 0>{});
+    }
+
+    // epilogue (the 32 x 32 accumulator layout of the FP32 kernels):
+    //   row = m0 + wm*32*MI + i*32 + lane%32,
+    //   col = n0 + wn*32 + 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const long gi = m0 + wm * 32 * MI + i * 32 + (lane & 31);
+        if (gi >= g.M) continue;
+        const int ei = g.ea[gi];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const long gj = n0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (gj >= g.N) continue;
+            if (g.tri == 1 && gi < gj) continue;
+            if (g.tri == 2 && gi > gj) continue;
+            double v = 0.0;
+#pragma unroll
+            for (int d = S - 1; d >= 0; --d) v += ldexp((double)acc[d][i][r], -7 * d);
+            double *cp = g.C + gi + gj * g.ldc;
+            v = g.alpha * ldexp(v, ei + g.eb[gj] - 14);
+            *cp = g.beta == 0.0 ? v : fma(g.beta, *cp, v);
+        }
+    }
+}
+
 inline bool isN(char c) { return c == 'N' || c == 'n'; }
 
 struct Split {
-    bf16 *S;
+    bf16 *S;      // BF16 / FP16 slices
+    int8_t *S8;   // INT8 slices
     int *e;
     int Rp;
 };
@@ -478,14 +731,19 @@ SplitWs split_workspace(hipStream_t st, int R1, int R2, int K)
     return ws;
 }
 
-// Exponents and slices of an R x K operand given by (X, ld, rcontig).
+// Exponents and slices of an R x K operand given by (X, ld, rcontig); KT is
+// the number of k slabs (16 deep for BF16 / FP16, 32 deep for INT8).
 int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, long ld, int rcontig,
-               int f16, Split &out, int *bad, double *part)
+               int fmt, Split &out, int *bad, double *part)
 {
+    const int f16 = fmt == GPUBLAS_SLICE_FP16, i8 = fmt == GPUBLAS_SLICE_INT8;
     out.Rp = ceil_div(R, 128) * 128;
     out.e = (int *)gpublas_dev_alloc((size_t)R * sizeof(int));
-    out.S = (bf16 *)gpublas_dev_alloc((size_t)s * KT * out.Rp * 16 * sizeof(bf16));
-    if (!out.e || !out.S) return (int)hipErrorOutOfMemory;
+    // both formats: 32 bytes per row of a slab
+    void *sl = gpublas_dev_alloc((size_t)s * KT * out.Rp * 32);
+    out.S = i8 ? nullptr : (bf16 *)sl;
+    out.S8 = i8 ? (int8_t *)sl : nullptr;
+    if (!out.e || !sl) return (int)hipErrorOutOfMemory;
     PROF_BEGIN(st, "ozaki_split");
     if (rcontig) {
         const int chunks = exp_chunks(R, K);
@@ -496,8 +754,69 @@ int make_split(hipStream_t st, int R, int K, int KT, int s, const double *X, lon
     } else
         hipLaunchKernelGGL(exp_rows_contig, dim3((unsigned)ceil_div(R, 4)), dim3(256), 0, st, R, K,
                            X, ld, out.e, bad);
-    hipLaunchKernelGGL(split_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 4)),
-                       dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, f16, out.S);
+    if (i8)
+        hipLaunchKernelGGL(split_i8_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 2)),
+                           dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, out.S8);
+    else
+        hipLaunchKernelGGL(split_kernel, dim3((unsigned)(out.Rp / 32), (unsigned)ceil_div(KT, 4)),
+                           dim3(256), 0, st, R, out.Rp, K, KT, s, X, ld, rcontig, out.e, f16,
+                           out.S);
+    PROF_END(st);
+    return (int)hipGetLastError();
+}
+
+void free_split(Split &x)
+{
+    gpublas_dev_release(x.S ? (void *)x.S : (void *)x.S8);
+    gpublas_dev_release(x.e);
+}
+
+template <int S>
+void launch_i8(hipStream_t st, unsigned blocks, const OzI8Args &g)
+{
+    constexpr int TM = oz_tile_rows(S), NW = TM / 16, OCC = 8 / NW;
+    hipLaunchKernelGGL((ozaki_i8_kernel<S, TM, NW, OCC>), dim3(blocks), dim3(64 * NW), 0, st, g);
+}
+
+// INT8 slice products over all KT slabs: one launch per i8_max_slabs(s)
+// slabs, the later ones accumulating into C (beta 1).
+int run_products_i8(hipStream_t st, int M, int N, int KT, int s, const Split &a, const Split &b,
+                    double alpha, double beta, double *C, int ldc, int tri)
+{
+    OzI8Args g;
+    g.M = M;
+    g.N = N;
+    g.Mp = a.Rp;
+    g.Np = b.Rp;
+    g.KT = KT;
+    g.SA = a.S8;
+    g.SB = b.S8;
+    g.ea = a.e;
+    g.eb = b.e;
+    g.alpha = alpha;
+    g.C = C;
+    g.ldc = ldc;
+    g.tri = tri;
+    g.tiles_m = ceil_div(M, oz_tile_rows(s));
+    g.tiles_n = ceil_div(N, 64);
+    const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
+    const int per = i8_max_slabs(s);
+    PROF_BEGIN(st, "ozaki_i8");
+    for (int kt0 = 0; kt0 < KT; kt0 += per) {
+        g.kt0 = kt0;
+        g.nkt = std::min(per, KT - kt0);
+        g.beta = kt0 == 0 ? beta : 1.0;
+        switch (s) {
+        case 1: launch_i8<1>(st, blocks, g); break;
+        case 2: launch_i8<2>(st, blocks, g); break;
+        case 3: launch_i8<3>(st, blocks, g); break;
+        case 4: launch_i8<4>(st, blocks, g); break;
+        case 5: launch_i8<5>(st, blocks, g); break;
+        case 6: launch_i8<6>(st, blocks, g); break;
+        case 7: launch_i8<7>(st, blocks, g); break;
+        default: launch_i8<8>(st, blocks, g); break;
+        }
+    }
     PROF_END(st);
     return (int)hipGetLastError();
 }
@@ -573,36 +892,38 @@ int gpublas_dev_dgemm_split_fmt(void *stream, int slices, int fmt, char transa, 
 {
     hipStream_t st = pick_stream(stream);
     if (m <= 0 || n <= 0) return 0;
-    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16) return (int)hipErrorInvalidValue;
+    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16 && fmt != GPUBLAS_SLICE_INT8)
+        return (int)hipErrorInvalidValue;
     const int maxs = fmt == GPUBLAS_SLICE_FP16 ? 6 : OZ_MAXS;
     if (slices < 1 || slices > maxs) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
-    const int f16 = fmt == GPUBLAS_SLICE_FP16;
-    const int KT = ceil_div(k, 16);
+    const int i8 = fmt == GPUBLAS_SLICE_INT8;
+    const int KT = ceil_div(k, i8 ? 32 : 16);
     SplitWs ws = split_workspace(st, m, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
     int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{}, b{};
     // op(A) rows: A 'N' -> X[r + k*lda] (rows contiguous); 'T' -> X[k + r*lda]
-    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), f16, a, bad, ws.part);
+    int err = make_split(st, m, k, KT, slices, d_A, lda, isN(transa), fmt, a, bad, ws.part);
     // op(B) columns: B 'N' -> element (k, j) at B[k + j*ldb] (k contiguous); 'T' -> B[j + k*ldb]
-    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), f16, b, bad, ws.part);
+    if (!err) err = make_split(st, n, k, KT, slices, d_B, ldb, !isN(transb), fmt, b, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
     if (!err) {
         if (hbad)  // Inf / NaN input: the splitting is undefined, use the FP64 kernel
             err = gpublas_gemm_d(st, transa, transb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+        else if (i8)
+            err = run_products_i8(st, m, n, KT, slices, a, b, alpha, beta, d_C, ldc, 0);
         else
-            err = run_products(st, m, n, KT, slices, f16, a, b, alpha, beta, d_C, ldc, 0);
+            err = run_products(st, m, n, KT, slices, fmt == GPUBLAS_SLICE_FP16, a, b, alpha, beta,
+                               d_C, ldc, 0);
     }
     (void)hipStreamSynchronize(st);
-    gpublas_dev_release(a.S);
-    gpublas_dev_release(a.e);
-    gpublas_dev_release(b.S);
-    gpublas_dev_release(b.e);
+    free_split(a);
+    free_split(b);
     return err;
 }
 
@@ -620,31 +941,33 @@ int gpublas_dev_dsyrk_split_fmt(void *stream, int slices, int fmt, char uplo, ch
 {
     hipStream_t st = pick_stream(stream);
     if (n <= 0) return 0;
-    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16) return (int)hipErrorInvalidValue;
+    if (fmt != GPUBLAS_SLICE_BF16 && fmt != GPUBLAS_SLICE_FP16 && fmt != GPUBLAS_SLICE_INT8)
+        return (int)hipErrorInvalidValue;
     const int maxs = fmt == GPUBLAS_SLICE_FP16 ? 6 : OZ_MAXS;
     if (slices < 1 || slices > maxs) return (int)hipErrorInvalidValue;
     if (alpha == 0.0 || k <= 0)
         return gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
-    const int f16 = fmt == GPUBLAS_SLICE_FP16;
-    const int KT = ceil_div(k, 16);
+    const int i8 = fmt == GPUBLAS_SLICE_INT8;
+    const int KT = ceil_div(k, i8 ? 32 : 16);
     SplitWs ws = split_workspace(st, n, n, k);
     if (!ws.bad) return (int)hipErrorOutOfMemory;
     int *bad = ws.bad;
     (void)hipMemsetAsync(bad, 0, sizeof(int), st);
     Split a{};
     // op(A) = A (trans N: rows contiguous) or A^T; op(B) = op(A)^T has the same lines
-    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), f16, a, bad, ws.part);
+    int err = make_split(st, n, k, KT, slices, d_A, lda, isN(trans), fmt, a, bad, ws.part);
     int hbad = 0;
     if (!err) err = (int)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
     if (!err) err = (int)hipStreamSynchronize(st);
     if (!err) {
+        const int tri = (uplo == 'U' || uplo == 'u') ? 2 : 1;
         if (hbad) err = gpublas_dev_dsyrk(st, uplo, trans, n, k, alpha, d_A, lda, beta, d_C, ldc);
-        else err = run_products(st, n, n, KT, slices, f16, a, a, alpha, beta, d_C, ldc,
-                                (uplo == 'U' || uplo == 'u') ? 2 : 1);
+        else if (i8) err = run_products_i8(st, n, n, KT, slices, a, a, alpha, beta, d_C, ldc, tri);
+        else err = run_products(st, n, n, KT, slices, fmt == GPUBLAS_SLICE_FP16, a, a, alpha, beta,
+                                d_C, ldc, tri);
     }
     (void)hipStreamSynchronize(st);
-    gpublas_dev_release(a.S);
-    gpublas_dev_release(a.e);
+    free_split(a);
     return err;
 }
 

@@ -85,7 +85,8 @@ static void print_help(void)
            "  profile              time every kernel with HIP events\n"
            "  split=<s>            DGEMM/DSYRK as split-precision slice products\n"
            "                       (s = 1..8 slices per operand; 0 = plain FP64, default)\n"
-           "  slice=<bf16|fp16>    slice format of split (bf16; fp16 takes s <= 6)\n"
+           "  slice=<bf16|fp16|int8> slice format of split (bf16; fp16 takes s <= 6; int8:\n"
+           "                       7-bit slices on integer MFMA, s = 7 for FP64-level accuracy)\n"
            "  serve[=<bytes>]      run DGEMM/SGEMM/DGEMV/left DTRSM calls whose operands are\n"
            "                       all tracked through the batching serving loop, with an\n"
            "                       HBM admission budget (64G); serve=0 turns it off\n"
@@ -134,6 +135,7 @@ int gb_parse_options(const char *s)
         }
         else if (!strcmp(opt, "slice=bf16")) gb_opt.split_fmt = GPUBLAS_SLICE_BF16;
         else if (!strcmp(opt, "slice=fp16")) gb_opt.split_fmt = GPUBLAS_SLICE_FP16;
+        else if (!strcmp(opt, "slice=int8")) gb_opt.split_fmt = GPUBLAS_SLICE_INT8;
         else if (!strcmp(opt, "serve")) gb_opt.serve = 1;
         else if (!strncmp(opt, "serve=", 6)) {
             const size_t b = parse_size(opt + 6);
