@@ -12,6 +12,9 @@
  *   blas_bench --lib ... --op symv --n N --uplo L   (DSYMV, bytes = the triangle once)
  *   blas_bench --lib ... --op syr --n N --uplo L    (DSYR, the triangle read and written)
  *   blas_bench --lib ... --op trmv --n N --uplo L --ta N --diag N  (DTRMV, the triangle once)
+ *   blas_bench --lib ... --op trsv --n N --uplo L --ta N --diag N  (DTRSV, the triangle once;
+ *                                                   diagonally dominant triangle)
+ *   blas_bench --lib ... --op ger --m M --n N        (DGER, A read and written once)
  *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
  *   blas_bench --lib ... --op gbmv --m M --n N --k KL=KU --ta N   (DGBMV, the band once)
  *   blas_bench --lib ... --op sbmv|tbmv|tbsv --n N --k K --uplo L [--ta N --diag N]
@@ -212,6 +215,10 @@ int main(int argc, char **argv)
     typedef void (*trmv_t)(const char *, const char *, const char *, const int *, const double *,
                            const int *, double *, const int *);
     trmv_t trmv = NULL;
+    trmv_t trsv = NULL; /* same argument list as DTRMV */
+    typedef void (*ger_t)(const int *, const int *, const double *, const double *, const int *,
+                          const double *, const int *, double *, const int *);
+    ger_t ger = NULL;
     typedef void (*spmv_t)(const char *, const int *, const double *, const double *,
                            const double *, const int *, const double *, double *, const int *);
     spmv_t spmv = NULL;
@@ -310,6 +317,25 @@ int main(int argc, char **argv)
         B = operand((size_t)n, 'd', seed + 1);
         bytes = 8.0 * ((double)n * (n + 1) / 2 + 2.0 * n); /* triangle once, x in/out */
         flops = (double)n * n;
+    } else if (!strcmp(op, "trsv")) {
+        trsv = (trmv_t)sym("dtrsv_");
+        A = operand((size_t)n * n, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        /* diagonally dominant: diag = n + uniform[0,1] (host touch of the diagonal) */
+        uint64_t s = seed + 5;
+        for (int i = 0; i < n; ++i) {
+            s = s * 6364136223846793005ull + 1442695040888963407ull;
+            ((double *)A)[(size_t)i * n + i] = n + (double)(s >> 11) * 0x1.0p-53;
+        }
+        bytes = 8.0 * ((double)n * (n + 1) / 2 + 2.0 * n); /* triangle once, x in/out */
+        flops = (double)n * n;
+    } else if (!strcmp(op, "ger")) {
+        ger = (ger_t)sym("dger_");
+        A = operand((size_t)m * n, 'd', seed);
+        B = operand((size_t)m, 'd', seed + 1);
+        C = operand((size_t)n, 'd', seed + 2);
+        bytes = 8.0 * (2.0 * m * n + m + n); /* A read and written, x, y */
+        flops = 2.0 * m * n;
     } else if (!strcmp(op, "spmv")) {
         spmv = (spmv_t)sym("dspmv_");
         A = operand((size_t)n * (n + 1) / 2, 'd', seed);
@@ -374,6 +400,13 @@ int main(int argc, char **argv)
         } else if (trmv) {
             const int one = 1;
             trmv(&uplo, &ta, &diag, &n, (const double *)A, &n, (double *)B, &one);
+        } else if (trsv) {
+            const int one = 1;
+            trsv(&uplo, &ta, &diag, &n, (const double *)A, &n, (double *)B, &one);
+        } else if (ger) {
+            const double a = 1e-3;
+            const int one = 1;
+            ger(&m, &n, &a, (const double *)B, &one, (const double *)C, &one, (double *)A, &m);
         } else if (spmv) {
             const double a = 1.0, b = 0.5;
             const int one = 1;

@@ -1,9 +1,8 @@
-// blas2.hip -- Level-2 kernels on gfx950: DGEMV (N and T), DGER, DTRSV,
-// DSYMV, DSYR, DSYR2, and the packed rank updates DSPR, DSPR2.
+// blas2.hip -- Level-2 kernels on gfx950: DGEMV (N and T), DGER, DSYMV,
+// DSYR, DSYR2, and the packed rank updates DSPR, DSPR2 (DTRSV: trsv.hip).
 //
 // Replaces the vendor calls of the reference's unbuilt CBLAS wrappers:
-// cublasDgemv (blas_level2/gemv.cc:97), cublasDger (ger.cc:92),
-// cublasDtrsv (trsv.cc:86).
+// cublasDgemv (blas_level2/gemv.cc:97), cublasDger (ger.cc:92).
 //
 // DGEMV 'N' is the HBM stream of the Level-1/2 chain: y = alpha*A*x + beta*y
 // with A column-major.  Two launches:
@@ -23,9 +22,6 @@
 // 16-B loads in flight per lane and a wave64 shuffle tree.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
-
-int gpublas_dev_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
-                      double alpha, const double *d_A, int lda, double *d_B, int ldb);
 
 namespace {
 
@@ -161,33 +157,65 @@ __global__ void scal_vec_kernel(int n, double beta, double *y, long incy, long y
     }
 }
 
-__global__ __launch_bounds__(256) void ger_kernel(int m, int n, double alpha, const double *x,
+// DGER: A := alpha x y^T + A, read and written once (16 mn bytes).  The grid
+// is the DGEMV 'N' panel plan: a workgroup owns GN_ROWS rows (two per lane,
+// one 16-B load and store per column) of a panel of cbw columns and keeps
+// GR_U columns of loads in flight per lane before their stores.  As netlib,
+// a column with y(j) = 0 is left untouched and the update is
+// A(i,j) + x(i) * (alpha y(j)) with one rounding (FMA).
+constexpr int GR_U = 8;
+__global__ __launch_bounds__(256) void ger_stream(int m, int n, double alpha, const double *x,
                                                   long incx, long x0, const double *y, long incy,
-                                                  long y0, double *A, long lda)
+                                                  long y0, double *A, long lda, int nrb, int cbw,
+                                                  int vec)
 {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    const double xi = x[x0 + i * incx];
-    const int j0 = blockIdx.y * 16, j1 = min(n, j0 + 16);
-    for (int j = j0; j < j1; ++j) {
-        const double yj = y[y0 + (long)j * incy];
-        if (yj != 0.0) {
-            double *p = A + i + (long)j * lda;
-            *p = fma(xi, alpha * yj, *p);
+    const int rb = blockIdx.x % nrb, cb = blockIdx.x / nrb;
+    const long row = (long)rb * GN_ROWS + 2 * threadIdx.x;
+    const int c0 = cb * cbw, c1 = min(n, c0 + cbw);
+    if (row >= m) return;
+    const double xa = x[x0 + row * incx];
+    const double xb = row + 1 < m ? x[x0 + (row + 1) * incx] : 0.0;
+    if (vec && row + 1 < m) {
+        double *a = A + row;
+        int c = c0;
+        for (; c + GR_U <= c1; c += GR_U) {
+            d2 v[GR_U];
+            double t[GR_U];
+#pragma unroll
+            for (int u = 0; u < GR_U; ++u) {
+                v[u] = __builtin_nontemporal_load((const d2 *)(a + (long)(c + u) * lda));
+                t[u] = y[y0 + (long)(c + u) * incy];
+            }
+#pragma unroll
+            for (int u = 0; u < GR_U; ++u) {
+                if (t[u] == 0.0) continue;
+                const double ay = alpha * t[u];
+                d2 w;
+                w.x = fma(xa, ay, v[u].x);
+                w.y = fma(xb, ay, v[u].y);
+                __builtin_nontemporal_store(w, (d2 *)(a + (long)(c + u) * lda));
+            }
         }
+        for (; c < c1; ++c) {
+            const double t = y[y0 + (long)c * incy];
+            if (t == 0.0) continue;
+            const double ay = alpha * t;
+            d2 *p = (d2 *)(a + (long)c * lda);
+            d2 w = *p;
+            w.x = fma(xa, ay, w.x);
+            w.y = fma(xb, ay, w.y);
+            *p = w;
+        }
+        return;
     }
-}
-
-__global__ void gather_kernel(int n, const double *x, long incx, long x0, double *t)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (long)gridDim.x * blockDim.x) t[i] = x[x0 + i * incx];
-}
-
-__global__ void scatter_kernel(int n, const double *t, double *x, long incx, long x0)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (long)gridDim.x * blockDim.x) x[x0 + i * incx] = t[i];
+    for (int c = c0; c < c1; ++c) {
+        const double t = y[y0 + (long)c * incy];
+        if (t == 0.0) continue;
+        const double ay = alpha * t;
+        double *p = A + row + (long)c * lda;
+        p[0] = fma(xa, ay, p[0]);
+        if (row + 1 < m) p[1] = fma(xb, ay, p[1]);
+    }
 }
 
 // DSYMV: y = alpha*A*x + beta*y with only the uplo triangle of A stored.
@@ -605,7 +633,16 @@ int gpublas_dev_dgemv_noalloc(void *stream, char trans, int m, int n, double alp
         dev >= kScratchDevices)
         return (int)hipErrorInvalidDevice;
     double *partials = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
-    if (!partials) return (int)hipErrorNotInitialized;  // gpublas_dev_init() has not run here
+    if (!partials) {
+        // gpublas_dev_init() takes the address; if it has not run, or could
+        // not take it, try once more here with the stream's device current
+        int cur = -1;
+        if (!dev_ready() || hipGetDevice(&cur) != hipSuccess || cur != dev)
+            return (int)hipErrorNotInitialized;
+        const int e = noalloc_scratch_setup(dev);
+        if (e) return e;
+        partials = __atomic_load_n(&g_scratch_ptr[dev], __ATOMIC_ACQUIRE);
+    }
     return gemv_run((hipStream_t)stream, trans, m, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy,
                     partials);
 }
@@ -615,10 +652,12 @@ int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x
 {
     hipStream_t s = pick_stream(stream);
     if (m <= 0 || n <= 0 || alpha == 0.0) return 0;
-    dim3 grid((unsigned)ceil_div(m, 256), (unsigned)ceil_div(n, 16));
+    const GemvNPlan pl = gemv_n_plan(m, n);
+    const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
     PROF_BEGIN(s, "dger");
-    hipLaunchKernelGGL(ger_kernel, grid, dim3(256), 0, s, m, n, alpha, d_x, (long)incx,
-                       start_of(m, incx), d_y, (long)incy, start_of(n, incy), d_A, (long)lda);
+    hipLaunchKernelGGL(ger_stream, dim3((unsigned)pl.nrb * pl.ncb), dim3(256), 0, s, m, n, alpha,
+                       d_x, (long)incx, start_of(m, incx), d_y, (long)incy, start_of(n, incy), d_A,
+                       (long)lda, pl.nrb, pl.cbw, vec);
     PROF_END(s);
     return (int)hipGetLastError();
 }
@@ -738,27 +777,6 @@ int gpublas_dev_dspr2(void *stream, char uplo, int n, double alpha, const double
                       const double *d_y, int incy, double *d_AP)
 {
     return spr_launch(pick_stream(stream), true, uplo, n, alpha, d_x, incx, d_y, incy, d_AP);
-}
-
-int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,
-                      int lda, double *d_x, int incx)
-{
-    hipStream_t s = pick_stream(stream);
-    if (n <= 0) return 0;
-    if (incx == 1) return gpublas_dev_dtrsm(s, 'L', uplo, trans, diag, n, 1, 1.0, d_A, lda, d_x, n);
-    double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
-    if (!t) return (int)hipErrorOutOfMemory;
-    const long x0 = start_of(n, incx);
-    int blocks = ceil_div(n, 256);
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(gather_kernel, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx, x0, t);
-    int err = gpublas_dev_dtrsm(s, 'L', uplo, trans, diag, n, 1, 1.0, d_A, lda, t, n);
-    if (!err) {
-        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, s, n, t, d_x, (long)incx, x0);
-        err = (int)hipGetLastError();
-    }
-    dev_release_after(t, s);  // back to the pool once the stream's work on it is done
-    return err;
 }
 
 }  // extern "C"

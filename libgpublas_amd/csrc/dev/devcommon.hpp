@@ -77,6 +77,7 @@ static constexpr int kTicketSlots = 4096;  // ticket counters per context
 // allocation-free DGEMV (blas2.hip); the library's init calls it once with
 // the device current.  Returns 0 or a HIP error code.
 int noalloc_scratch_setup(int device);
+bool dev_ready();  // gpublas_dev_init() has run and found a gfx950 device (no init here)
 
 // Kernel timing (HIP events on the launch stream), enabled by
 // gpublas_dev_prof_enable.  Wrap each launch as

@@ -190,10 +190,10 @@ void do_init()
         g_init_status = -3;  // code objects are built for gfx950 only
         return;
     }
-    if (noalloc_scratch_setup(0) != 0) {  // device 0 is current (hipSetDevice above)
-        g_init_status = -4;
-        return;
-    }
+    // the allocation-free DGEMV's fixed scratch array: its address on device 0
+    // (current since hipSetDevice above); if it cannot be taken here, that
+    // entry point's first call tries again and reports the error
+    (void)noalloc_scratch_setup(0);
     g_init_status = 0;
 }
 
@@ -309,6 +309,8 @@ void prof_end(hipStream_t s)
     g_pending->push_back(Pending{t_begin_ev, b, t_begin_name});
     t_begin_ev = nullptr;
 }
+
+bool dev_ready() { return __atomic_load_n(&g_init_status, __ATOMIC_ACQUIRE) == 0; }
 
 extern "C" {
 
@@ -558,7 +560,8 @@ void *gpublas_dev_result_slot(void) { return dev_ctx()->pinned; }
 
 void gpublas_dev_prof_enable(int on)
 {
-    gpublas_dev_init();
+    // no device work here: the library's constructor calls this before every
+    // object's kernels are registered with the HIP runtime
     g_prof_on = on;
 }
 

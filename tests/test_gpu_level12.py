@@ -731,3 +731,69 @@ def test_drotm(gpu, ora, flag, n, incx, incy):
         gaps = np.ones(len(got), bool)
         gaps[:: abs(inc)][:n] = False
         assert np.array_equal(got[gaps], orig[gaps])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_dtrsv_blocked_sizes(gpu, ora, uplo, trans, diag):
+    """n = 1000: seven full 128-blocks and a partial one, solved in either
+    direction by one launch per block; the other triangle (and a unit
+    diagonal) NaN, so any read of them shows."""
+    n = 1000
+    A = uniform(26, (n, n))
+    if diag == "N":
+        A[np.diag_indices(n)] = n + np.abs(uniform(27, (n,)))
+    else:
+        A *= 1.0 / n
+        A[np.diag_indices(n)] = np.nan
+    A[_other_triangle(n, uplo)] = np.nan
+    A = np.asfortranarray(A)
+    x = uniform(28, (n,))
+    r = x.copy()
+    gpu.dtrsv(uplo, trans, diag, n, A, n, x, 1)
+    ora.ora_dtrsv(c(uplo), c(trans), c(diag), n, p(A), n, p(r), 1)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T"), ("U", "N"), ("L", "T")])
+def test_dtrsv_integer_exact_8192(gpu, uplo, trans):
+    """n = 8192, unit triangle with entries in {-1, 0, 1} and an integer
+    solution: every partial sum of the blocked substitution is an integer
+    below 2^53, so x is recovered exactly (64 launches, each block's panel
+    applied to every open unknown)."""
+    n = 8192
+    rng = np.random.default_rng(29)
+    T = rng.integers(-1, 2, size=(n, n)).astype(float)
+    T = np.tril(T, -1) if uplo == "L" else np.triu(T, 1)
+    T[np.diag_indices(n)] = 1.0
+    xs = rng.integers(-3, 4, size=n).astype(float)
+    b = (T if trans == "N" else T.T) @ xs
+    A = np.asfortranarray(T)
+    A[np.diag_indices(n)] = np.nan  # unit diagonal: not read
+    gpu.dtrsv(uplo, trans, "U", n, A, n, b, 1)
+    assert np.array_equal(b, xs)
+
+
+def test_dger_8192(gpu, ora):
+    """DGER at 8192 x 8192 (16-B loads and stores along columns): against the
+    oracle within one rounding per element, and bit-exact on integer data;
+    a column with y(j) = 0 is left untouched (netlib)."""
+    m = n = 8192
+    A0 = uniform(31, (m, n))
+    x, y = uniform(32, (m,)), uniform(33, (n,))
+    y[77] = 0.0
+    x[5] = 0.5
+    A0[5, 77] = -0.0  # -0.0 + 0.5 * 0.0 would be +0.0 if the zero column were updated
+    A, R = A0.copy(order="F"), A0.copy(order="F")
+    gpu.dger(m, n, 0.9, x, 1, y, 1, A, m)
+    ora.ora_dger(m, n, 0.9, p(x), 1, p(y), 1, p(R), m)
+    scale = np.maximum(np.abs(R) + np.abs(np.outer(x, 0.9 * y)), np.finfo(float).tiny)
+    assert np.max(np.abs(A - R) / scale) <= 2 * EPS
+    assert np.array_equal(A[:, 77], A0[:, 77]) and np.signbit(A[5, 77])
+    rng = np.random.default_rng(34)
+    Ai = np.asfortranarray(rng.integers(-8, 9, size=(m, n)).astype(float))
+    xi, yi = rng.integers(-8, 9, size=m).astype(float), rng.integers(-8, 9, size=n).astype(float)
+    want = Ai + 2.0 * np.outer(xi, yi)
+    gpu.dger(m, n, 2.0, xi, 1, yi, 1, Ai, m)
+    assert np.array_equal(Ai, want)
