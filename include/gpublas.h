@@ -259,6 +259,7 @@ typedef struct {
     uint64_t serve_wait_ns;  /* served calls: summed submission -> launch time */
     uint64_t serve_run_ns;   /* served calls: summed launch -> completion seen */
     uint64_t serve_launch_ns;/* engine time spent forming and launching batches */
+    uint64_t resident;       /* reductions served by the resident server (option resident) */
 } gpublas_stats_t;
 void gpublas_get_stats(gpublas_stats_t *out);
 void gpublas_reset_stats(void);

@@ -33,6 +33,11 @@ int gpublas_dev_sync(void *stream);
 uint64_t gpublas_dev_stream_mark(void *stream);
 int gpublas_dev_stream_wait(void *stream, uint64_t seq);
 int gpublas_dev_stream_done(void *stream, uint64_t seq);  /* non-blocking: 1 if complete */
+/* The caller has seen work up to `seq` of the stream end (e.g. a reduction
+ * published after it): later done/wait queries for it return at once. */
+void gpublas_dev_stream_observed(void *stream, uint64_t seq);
+/* 1 if a synchronisation or observation has already seen `seq` complete (no query). */
+int gpublas_dev_stream_known_done(void *stream, uint64_t seq);
 /* Scalar results: the next sequence number of the calling thread's result
  * slot, and a poll for the reduction kernel's publication of that number. */
 uint64_t gpublas_dev_scalar_next(void);
@@ -227,6 +232,26 @@ int gpublas_dev_drotm(void *stream, int n, double *d_x, int incx, double *d_y, i
                       const double *h_param);
 /* Pinned result slot of the calling thread (256 bytes). */
 void *gpublas_dev_result_slot(void);
+/* Resident reduction server (opt-in; no counterpart in the reference, which
+ * launches one vendor call per intercepted call, runtime-blas.c:14-20): a
+ * kernel that stays on the device (two 1024-thread workgroups) and serves
+ * short reductions (n <= 65536) from a ring of request slots in pinned host
+ * memory, so a scalar-returning call pays no kernel launch.  reduce() runs
+ * op 0 DDOT, 1 DNRM2, 2 IDAMAX, 3 IDAMIN, 4 DASUM (1-based index in *ires
+ * for 2 and 3, netlib rules) and returns once the result is in *dres /
+ * *ires.  The operands must not be pending writes: with operands_ready = 0
+ * the call is served only if `stream` has no work queued; operands_ready = 1
+ * asserts that the kernels that wrote them have ended.  It returns -1
+ * (nothing done) when the server is not running, another request is in
+ * flight or the call is not one it serves, so the caller can take the
+ * launched path.  Results are bit-identical to the
+ * launched kernels'.  The server exits at process exit or on stop(). */
+int gpublas_dev_resident_start(void);
+void gpublas_dev_resident_stop(void);
+int gpublas_dev_resident_active(void);
+int gpublas_dev_resident_reduce(void *stream, int op, int n, const double *d_x, int incx,
+                                const double *d_y, int incy, int operands_ready, double *dres,
+                                int *ires);
 
 /* ---- Batch slots of the serving loop (no counterpart in the reference,
  * which issues one cuBLAS call per intercepted call: blas_level3/gemm.cc:62-82,

@@ -54,7 +54,8 @@ class Stats(ctypes.Structure):
         "calls", "device_calls", "fallthrough", "hits", "misses", "h2d_bytes", "d2h_bytes",
         "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen",
         "async_returns", "evictions", "served", "batches", "max_batch", "deferred",
-        "over_budget", "resident_bytes", "serve_wait_ns", "serve_run_ns", "serve_launch_ns")]
+        "over_budget", "resident_bytes", "serve_wait_ns", "serve_run_ns", "serve_launch_ns",
+        "resident")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -53,7 +53,7 @@ typedef struct {
     uint64_t calls, device_calls, fallthrough, hits, misses, h2d_bytes, d2h_bytes;
     uint64_t tracked_objects, tracked_bytes, faults, uploads, allocs_seen, async_returns;
     uint64_t evictions, served, batches, max_batch, deferred, over_budget, resident_bytes;
-    uint64_t serve_wait_ns, serve_run_ns, serve_launch_ns;
+    uint64_t serve_wait_ns, serve_run_ns, serve_launch_ns, resident;
 } stats_t;
 
 static gemm_t p_dgemm, p_sgemm;

@@ -18,6 +18,10 @@
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 #include <math.h>
+#include <mutex>
+#include <time.h>
+#include <stdlib.h>
+#include <string.h>
 
 namespace {
 
@@ -380,9 +384,9 @@ __device__ __forceinline__ double block_sum16(double v, double *red)
     return r;
 }
 
-__global__ __launch_bounds__(1024) void dot_small(int n, const double *x, long incx, long x0,
-                                                  const double *y, long incy, long y0, int vec,
-                                                  double *result, uint64_t *flag, uint64_t seq)
+__device__ __forceinline__ void dot_wg(int n, const double *x, long incx, long x0, const double *y,
+                                       long incy, long y0, int vec, double *result, uint64_t *flag,
+                                       uint64_t seq)
 {
     __shared__ double red[16];
     double s = 0;
@@ -415,9 +419,8 @@ __global__ __launch_bounds__(1024) void dot_small(int n, const double *x, long i
     }
 }
 
-__global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long incx, long x0,
-                                                   int vec, double *result, uint64_t *flag,
-                                                   uint64_t seq)
+__device__ __forceinline__ void nrm2_wg(int n, const double *x, long incx, long x0, int vec,
+                                        double *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ double red[16];
     Acc3 a{0, 0, 0};
@@ -450,8 +453,8 @@ __global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long 
 }
 
 template <bool MIN>
-__global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long incx, int vec,
-                                                    int *result, uint64_t *flag, uint64_t seq)
+__device__ __forceinline__ void iamax_wg(int n, const double *x, long incx, int vec, int *result,
+                                         uint64_t *flag, uint64_t seq)
 {
     __shared__ Cand red[16];
     Cand c = cand_none<MIN>();
@@ -486,6 +489,27 @@ __global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long
         *result = (int)cand_result<MIN>(t, x);
         publish(flag, seq);
     }
+}
+
+__global__ __launch_bounds__(1024) void dot_small(int n, const double *x, long incx, long x0,
+                                                  const double *y, long incy, long y0, int vec,
+                                                  double *result, uint64_t *flag, uint64_t seq)
+{
+    dot_wg(n, x, incx, x0, y, incy, y0, vec, result, flag, seq);
+}
+
+__global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long incx, long x0,
+                                                   int vec, double *result, uint64_t *flag,
+                                                   uint64_t seq)
+{
+    nrm2_wg(n, x, incx, x0, vec, result, flag, seq);
+}
+
+template <bool MIN>
+__global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long incx, int vec,
+                                                    int *result, uint64_t *flag, uint64_t seq)
+{
+    iamax_wg<MIN>(n, x, incx, vec, result, flag, seq);
 }
 
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
@@ -559,8 +583,8 @@ __global__ __launch_bounds__(256) void rotm_kernel(int n, double *x, long incx, 
 }
 
 // ---- DASUM: the DDOT reductions with |x_i| in place of x_i*y_i -----------------
-__global__ __launch_bounds__(1024) void asum_small(int n, const double *x, long incx, int vec,
-                                                   double *result, uint64_t *flag, uint64_t seq)
+__device__ __forceinline__ void asum_wg(int n, const double *x, long incx, int vec,
+                                        double *result, uint64_t *flag, uint64_t seq)
 {
     __shared__ double red[16];
     double s = 0;
@@ -590,6 +614,12 @@ __global__ __launch_bounds__(1024) void asum_small(int n, const double *x, long 
     }
 }
 
+__global__ __launch_bounds__(1024) void asum_small(int n, const double *x, long incx, int vec,
+                                                   double *result, uint64_t *flag, uint64_t seq)
+{
+    asum_wg(n, x, incx, vec, result, flag, seq);
+}
+
 __global__ __launch_bounds__(256) void asum_kernel(int n, const double *x, long incx,
                                                    double *partials, unsigned *ticket,
                                                    double *result, uint64_t *flag, uint64_t seq)
@@ -611,9 +641,237 @@ __global__ __launch_bounds__(256) void asum_kernel(int n, const double *x, long 
     }
 }
 
+// ---- resident reduction server -------------------------------------------------
+// A kernel that stays on the device and serves the short (n <= kSmallN)
+// reductions DDOT / DNRM2 / IDAMAX / IDAMIN / DASUM from a ring of request
+// slots in pinned host memory, so a scalar-returning call costs no kernel
+// launch: the caller fills a slot and publishes its sequence number; the
+// server workgroup (1024 threads), already waiting on that slot, sees it,
+// reads the descriptor (one word per lane, one trip over the link), runs the
+// same single-workgroup reduction as the launched kernel (identical bits)
+// and publishes the result in the slot.  It serves a call only when the
+// kernels that wrote its operands have ended (the caller's stream has no work
+// queued, or the caller knows those writers completed): a launch queued
+// behind such work starts as soon as it ends, which a hand-over cannot beat,
+// and stream order needs no device-side wait.  Every wait of the server is bounded:
+// after `idle` steady-clock ticks with nothing to do it exits, and a later
+// request starts it again; a caller whose request was not served (the server
+// exited first) takes the launched path.  So a server that shares a hardware
+// queue with other work delays that work by at most the idle time, and
+// nothing waits on it forever.  Descriptor fields and flags are read with
+// atomic loads (never through the scalar cache, which a persistent kernel
+// would not see refreshed); results are written with vector stores and
+// published with a system-scope release.
+enum { RS_DOT = 0, RS_NRM2, RS_IAMAX, RS_IAMIN, RS_ASUM };
+constexpr int kResSlots = 64;
+
+struct alignas(128) ResSlot {
+    uint64_t seq;            // request id + 1, written last by the host
+    uint64_t op, n;
+    int64_t incx, incy, x0, y0, vec;
+    uint64_t x, y;           // device addresses
+    double dres;             // results, written by the server
+    int64_t ires;
+    uint64_t done;           // = seq once the result is valid
+};
+static_assert(sizeof(ResSlot) == 128, "one slot per 128 B");
+constexpr int kResWords = 10;  // descriptor words read by the server (seq .. y)
+
+__global__ __launch_bounds__(1024) void res_server(ResSlot *ring, unsigned long long first,
+                                                   const int *stop, long long idle)
+{
+    __shared__ uint64_t d[kResWords];
+    __shared__ int quit;
+    for (unsigned long long id = first;; ++id) {
+        ResSlot *sl = ring + id % kResSlots;
+        if (threadIdx.x == 0) {
+            int q = 0;
+            const long long t0 = wall_clock64();
+            for (unsigned spin = 0;; ++spin) {
+                if (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == id + 1)
+                    break;
+                if ((spin & 15) == 0 &&
+                    (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+                     wall_clock64() - t0 > idle)) {
+                    q = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            quit = q;
+        }
+        __syncthreads();
+        if (quit) return;
+        // the descriptor: one word per lane, all read at once (one trip over the link)
+        if (threadIdx.x < kResWords)
+            d[threadIdx.x] = __hip_atomic_load((const uint64_t *)sl + threadIdx.x, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
+        // see the operands' latest contents (their writers ended before the
+        // caller's stream went idle)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int op = (int)d[1], n = (int)d[2];
+        const long incx = (long)d[3], incy = (long)d[4], x0 = (long)d[5], y0 = (long)d[6];
+        const int vec = (int)d[7];
+        const double *x = (const double *)d[8], *y = (const double *)d[9];
+        int *ires = (int *)&sl->ires;
+        switch (op) {
+        case RS_DOT: dot_wg(n, x, incx, x0, y, incy, y0, vec, &sl->dres, &sl->done, id + 1); break;
+        case RS_NRM2: nrm2_wg(n, x, incx, x0, vec, &sl->dres, &sl->done, id + 1); break;
+        case RS_IAMAX: iamax_wg<false>(n, x, incx, vec, ires, &sl->done, id + 1); break;
+        case RS_IAMIN: iamax_wg<true>(n, x, incx, vec, ires, &sl->done, id + 1); break;
+        default: asum_wg(n, x, incx, vec, &sl->dres, &sl->done, id + 1); break;
+        }
+        __syncthreads();
+    }
+}
+
+// Host side.  One request in flight at a time (a caller that finds another
+// request in flight takes the launched path); ids never repeat, so a slot's
+// `done` names exactly one request.
+struct ResServer {
+    std::mutex mu;
+    bool enabled = false;   // gpublas_dev_resident_start() called, not stopped
+    bool running = false;   // a server kernel is on the device
+    ResSlot *ring = nullptr;  // pinned host memory
+    int *stop = nullptr;      // pinned host memory
+    hipStream_t stream = nullptr;
+    uint64_t next = 0;        // next request id
+    long long idle = 0;       // steady-clock ticks of idle before the server exits
+    uint64_t idle_ns = 0;     // the same in host nanoseconds
+    uint64_t last_ns = 0;     // host time the last request ended
+};
+ResServer g_res;
+
+void res_stop_at_exit() { gpublas_dev_resident_stop(); }
+
+uint64_t res_now_ns()
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (uint64_t)t.tv_sec * 1000000000ull + (uint64_t)t.tv_nsec;
+}
+
+// (mu held) make sure a server is running.  While requests come faster than
+// half its idle time it is known to be alive; after a longer gap the stream
+// is queried, and an exited server is launched again.  (One that exits in
+// the moment between the query and the request is noticed by that request's
+// wait, which then takes the launched path.)
+int res_ensure_locked()
+{
+    if (g_res.running && res_now_ns() - g_res.last_ns < g_res.idle_ns / 2) return 0;
+    if (g_res.running) {
+        const hipError_t q = hipStreamQuery(g_res.stream);
+        if (q == hipErrorNotReady) return 0;
+        if (q != hipSuccess) return (int)q;
+        g_res.running = false;
+    }
+    const hipError_t q = hipStreamSynchronize(g_res.stream);
+    if (q != hipSuccess) return (int)q;
+    *(volatile int *)g_res.stop = 0;
+    hipLaunchKernelGGL(res_server, dim3(1), dim3(1024), 0, g_res.stream, g_res.ring,
+                       (unsigned long long)g_res.next, (const int *)g_res.stop, g_res.idle);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    g_res.running = true;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpublas_dev_resident_start(void)
+{
+    if (gpublas_dev_init() != 0) return (int)hipErrorNoDevice;
+    std::lock_guard<std::mutex> g(g_res.mu);
+    if (g_res.enabled) return 0;
+    if (!g_res.ring) {
+        // a queue of its own where the runtime provides one (CU-masked streams)
+        uint32_t mask[8];
+        for (uint32_t &m : mask) m = 0xffffffffu;
+        if (hipExtStreamCreateWithCUMask(&g_res.stream, 8, mask) != hipSuccess &&
+            hipStreamCreateWithFlags(&g_res.stream, hipStreamNonBlocking) != hipSuccess)
+            return (int)hipErrorOutOfMemory;
+        if (hipHostMalloc((void **)&g_res.ring, kResSlots * sizeof(ResSlot), hipHostMallocCoherent) !=
+                hipSuccess ||
+            hipHostMalloc((void **)&g_res.stop, 64, hipHostMallocCoherent) != hipSuccess)
+            return (int)hipErrorOutOfMemory;
+        memset((void *)g_res.ring, 0, kResSlots * sizeof(ResSlot));
+        int rate_khz = 0;
+        if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess ||
+            rate_khz <= 0)
+            rate_khz = 100000;
+        g_res.idle = (long long)rate_khz * 2;  // 2 ms
+        g_res.idle_ns = 2000000;
+        atexit(res_stop_at_exit);
+    }
+    g_res.enabled = true;
+    return 0;
+}
+
+void gpublas_dev_resident_stop(void)
+{
+    std::lock_guard<std::mutex> g(g_res.mu);
+    g_res.enabled = false;
+    if (!g_res.running) return;
+    __atomic_store_n(g_res.stop, 1, __ATOMIC_RELEASE);
+    (void)hipStreamSynchronize(g_res.stream);
+    g_res.running = false;
+}
+
+int gpublas_dev_resident_active(void) { return __atomic_load_n(&g_res.enabled, __ATOMIC_ACQUIRE); }
+
+int gpublas_dev_resident_reduce(void *stream, int op, int n, const double *d_x, int incx,
+                                const double *d_y, int incy, int operands_ready, double *dres,
+                                int *ires)
+{
+    if (!__atomic_load_n(&g_res.enabled, __ATOMIC_ACQUIRE)) return -1;
+    if (n < 1 || n > kSmallN || op < RS_DOT || op > RS_ASUM) return -1;
+    if ((op == RS_IAMAX || op == RS_IAMIN || op == RS_ASUM) && incx <= 0) return -1;
+    hipStream_t s = pick_stream(stream);
+    DevCtx *c = ctx_of(s);
+    if (!c) return -1;
+    std::unique_lock<std::mutex> lk(g_res.mu, std::try_to_lock);
+    if (!lk.owns_lock() || !g_res.enabled) return -1;  // another request in flight
+    if (res_ensure_locked() != 0) return -1;
+    // Unless the caller knows the operands' writers have ended, only when the
+    // caller's stream has drained: behind queued work a launch starts as soon
+    // as that work ends, which no hand-over to the server beats.
+    if (!operands_ready) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipErrorNotReady) return -1;
+        if (q != hipSuccess) return (int)q;
+    }
+    const uint64_t id = g_res.next++;
+    ResSlot *sl = g_res.ring + id % kResSlots;
+    const int vy = op == RS_DOT ? (((uintptr_t)d_y & 15) == 0 && incy == 1) : 1;
+    sl->op = (uint64_t)op;
+    sl->n = (uint64_t)n;
+    sl->incx = incx;
+    sl->incy = incy;
+    sl->x0 = incx < 0 ? -(long)(n - 1) * incx : 0;
+    sl->y0 = incy < 0 ? -(long)(n - 1) * incy : 0;
+    sl->vec = incx == 1 && ((uintptr_t)d_x & 15) == 0 && vy;
+    sl->x = (uint64_t)(uintptr_t)d_x;
+    sl->y = (uint64_t)(uintptr_t)d_y;
+    __atomic_store_n(&sl->seq, id + 1, __ATOMIC_RELEASE);
+    for (unsigned spin = 1;; ++spin) {
+        if (__atomic_load_n(&sl->done, __ATOMIC_ACQUIRE) == id + 1) break;
+        if ((spin & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(g_res.stream);
+            if (e == hipErrorNotReady) continue;
+            // the server exited (idle time, or an error): served after all?
+            g_res.running = false;
+            if (__atomic_load_n(&sl->done, __ATOMIC_ACQUIRE) == id + 1) break;
+            return -1;  // not served: the caller takes the launched path
+        }
+    }
+    if (dres) *dres = *(volatile double *)&sl->dres;
+    if (ires) *ires = (int)*(volatile int64_t *)&sl->ires;
+    g_res.last_ns = res_now_ns();
+    return 0;
+}
 
 int gpublas_dev_daxpy(void *stream, int n, double alpha, const double *d_x, int incx,
                       double *d_y, int incy)

@@ -378,6 +378,22 @@ int gpublas_dev_stream_done(void *stream, uint64_t seq)
     return 1;
 }
 
+int gpublas_dev_stream_known_done(void *stream, uint64_t seq)
+{
+    DevCtx *c = ctx_of(pick_stream(stream));
+    return c && __atomic_load_n(&c->completed, __ATOMIC_ACQUIRE) >= seq;
+}
+
+void gpublas_dev_stream_observed(void *stream, uint64_t seq)
+{
+    DevCtx *c = ctx_of(pick_stream(stream));
+    if (!c) return;
+    uint64_t cur = __atomic_load_n(&c->completed, __ATOMIC_RELAXED);
+    while (cur < seq && !__atomic_compare_exchange_n(&c->completed, &cur, seq, false,
+                                                     __ATOMIC_RELEASE, __ATOMIC_RELAXED)) {
+    }
+}
+
 int gpublas_dev_stream_wait(void *stream, uint64_t seq)
 {
     DevCtx *c = ctx_of(pick_stream(stream));

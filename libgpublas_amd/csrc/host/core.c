@@ -92,6 +92,8 @@ static void print_help(void)
            "                       HBM admission budget (64G); serve=0 turns it off\n"
            "  serve_slots=<n>      batches in flight (8)\n"
            "  serve_graph=<0|1>    replay captured hipGraphs (1) or launch eagerly (0, default)\n"
+           "  resident[=0|1]       DDOT/DNRM2/IDAMAX/IDAMIN/DASUM (n <= 65536) on tracked operands\n"
+           "                       through a kernel resident on the device (no launch per call)\n"
            "  serve_batch=<n>      most calls per batch (256; 1 = one launch per call)\n"
            "  tune=<file>          tile-shape table of the batch GEMM kernels: loaded\n"
            "                       from <file>, measured and written there if absent\n"
@@ -137,6 +139,8 @@ int gb_parse_options(const char *s)
         else if (!strcmp(opt, "slice=fp16")) gb_opt.split_fmt = GPUBLAS_SLICE_FP16;
         else if (!strcmp(opt, "slice=int8")) gb_opt.split_fmt = GPUBLAS_SLICE_INT8;
         else if (!strcmp(opt, "serve")) gb_opt.serve = 1;
+        else if (!strcmp(opt, "resident") || !strcmp(opt, "resident=1")) gb_opt.resident = 1;
+        else if (!strcmp(opt, "resident=0")) gb_opt.resident = 0;
         else if (!strncmp(opt, "serve=", 6)) {
             const size_t b = parse_size(opt + 6);
             gb_opt.serve = b != 0;

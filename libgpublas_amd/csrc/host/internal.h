@@ -36,6 +36,7 @@ struct gb_options {
     int split;               /* >0: DGEMM/DSYRK as split-precision with this many slices */
     int split_fmt;           /* GPUBLAS_SLICE_BF16 (default) or GPUBLAS_SLICE_FP16 */
     int serve;               /* route calls on tracked operands through the serving loop */
+    int resident;            /* short reductions on tracked operands through the resident server */
     size_t serve_budget;     /* HBM admission budget of the serving loop (bytes) */
     int serve_slots;         /* batches in flight */
     int serve_graph;         /* replay captured graphs (else launch eagerly) */
@@ -182,6 +183,10 @@ void gb_finish(void *stream, gb_operand *ops, int nops);
  * in the result slot, then release staging buffers. */
 void gb_finish_scalar(void *stream, gb_operand *ops, int nops, uint64_t seq);
 bool gb_any_tracked(const gb_operand *ops, int nops);
+/* Every operand is a tracked object whose last writing kernel has completed. */
+bool gb_writers_done(const gb_operand *ops, int nops);
+/* End of a reduction served by the resident server (result already in hand). */
+void gb_finish_served(void *stream, gb_operand *ops, int nops);
 
 /* ---- serving loop (serve.c) ---- */
 /* Each returns 1 if the call was served (all operands tracked, shape

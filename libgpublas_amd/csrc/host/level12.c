@@ -182,6 +182,27 @@ void drot_(const int *n, double *x, const int *incx, double *y, const int *incy,
     gb_leave();
 }
 
+/* Option `resident`: a short reduction whose operands are all tracked objects
+ * goes to the resident server (no kernel launch).  Returns 1 when served; 0
+ * leaves the call to the launched path (operands stay acquired). */
+static int served_reduce(void *s, int op, int n, gb_operand *ops, int nops, int incx, int incy,
+                         double *dr, int *ir)
+{
+    if (!gb_opt.resident) return 0;
+    /* the server reads the operands at once: their writers (and uploads) must
+     * be known to have ended; otherwise a launch queued behind them is as quick */
+    if (!gb_writers_done(ops, nops)) return 0;
+    if (!gpublas_dev_resident_active() && gpublas_dev_resident_start() != 0) return 0;
+    if (gpublas_dev_resident_reduce(s, op, n, (const double *)ops[0].dev, incx,
+                                    nops > 1 ? (const double *)ops[1].dev : NULL, incy, 1, dr,
+                                    ir) != 0)
+        return 0;
+    gb_finish_served(s, ops, nops);
+    GB_COUNT(device_calls, 1);
+    GB_COUNT(resident, 1);
+    return 1;
+}
+
 double dasum_(const int *n, const double *x, const int *incx)
 {
     gb_enter();
@@ -201,6 +222,11 @@ double dasum_(const int *n, const double *x, const int *incx)
     void *s = gpublas_dev_stream();
     gb_operand ops[1];
     gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
+    double served;
+    if (served_reduce(s, 4, *n, ops, 1, *incx, 0, &served, NULL)) {
+        gb_leave();
+        return served;
+    }
     double *slot = (double *)gpublas_dev_result_slot();
     const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_dasum(s, *n, (const double *)ops[0].dev, *incx, slot), "dasum launch");
@@ -232,6 +258,11 @@ double ddot_(const int *n, const double *x, const int *incx, const double *y, co
     gb_operand ops[2];
     gb_acquire_vector(&ops[0], x, *n, *incx, 8, GB_IN, s);
     gb_acquire_vector(&ops[1], y, *n, *incy, 8, GB_IN, s);
+    double served;
+    if (served_reduce(s, 0, *n, ops, 2, *incx, *incy, &served, NULL)) {
+        gb_leave();
+        return served;
+    }
     double *slot = (double *)gpublas_dev_result_slot();
     const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_ddot(s, *n, (const double *)ops[0].dev, *incx,
@@ -264,6 +295,11 @@ double dnrm2_(const int *n, const double *x, const int *incx)
     void *s = gpublas_dev_stream();
     gb_operand op;
     gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    double served;
+    if (served_reduce(s, 1, *n, &op, 1, *incx, 0, &served, NULL)) {
+        gb_leave();
+        return served;
+    }
     double *slot = (double *)gpublas_dev_result_slot();
     const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_dnrm2(s, *n, (const double *)op.dev, *incx, slot), "dnrm2 launch");
@@ -298,6 +334,11 @@ int idamax_(const int *n, const double *x, const int *incx)
     void *s = gpublas_dev_stream();
     gb_operand op;
     gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    int served;
+    if (served_reduce(s, 2, *n, &op, 1, *incx, 0, NULL, &served)) {
+        gb_leave();
+        return served;
+    }
     int *slot = (int *)gpublas_dev_result_slot();
     const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_idamax(s, *n, (const double *)op.dev, *incx, slot), "idamax launch");
@@ -333,6 +374,11 @@ int idamin_(const int *n, const double *x, const int *incx)
     void *s = gpublas_dev_stream();
     gb_operand op;
     gb_acquire_vector(&op, x, *n, *incx, 8, GB_IN, s);
+    int served;
+    if (served_reduce(s, 3, *n, &op, 1, *incx, 0, NULL, &served)) {
+        gb_leave();
+        return served;
+    }
     int *slot = (int *)gpublas_dev_result_slot();
     const uint64_t seq = gpublas_dev_scalar_next();
     gb_check(gpublas_dev_idamin(s, *n, (const double *)op.dev, *incx, slot), "idamin launch");

@@ -517,15 +517,16 @@ void gb_release(gb_operand *op, void *stream)
 
 /* Record the call's kernels as the objects' latest writer / readers and
  * drop the references taken at acquire. */
-static void mark_uses(void *stream, gb_operand *ops, int nops)
+static uint64_t mark_uses(void *stream, gb_operand *ops, int nops)
 {
     const uint64_t seq = gpublas_dev_stream_mark(stream);
     for (int i = 0; i < nops; ++i) {
         struct gb_obj *o = ops[i].obj;
         if (!o) continue;
         pthread_mutex_lock(&o->mu);
-        if (ops[i].mode & GB_OUT) {
-            /* readers on other streams were waited for at acquire */
+        if ((ops[i].mode & GB_OUT) || ops[i].uploaded) {
+            /* a write or this call's upload of the mirror (queued on `stream`):
+             * readers on other streams were waited for at acquire */
             o->wr_stream = stream;
             o->wr_seq = seq;
             o->nrd = 0;
@@ -550,6 +551,7 @@ static void mark_uses(void *stream, gb_operand *ops, int nops)
         o->refs--;
         pthread_mutex_unlock(&o->mu);
     }
+    return seq;
 }
 
 void gb_finish(void *stream, gb_operand *ops, int nops)
@@ -571,9 +573,36 @@ void gb_finish(void *stream, gb_operand *ops, int nops)
 
 void gb_finish_scalar(void *stream, gb_operand *ops, int nops, uint64_t seq)
 {
-    mark_uses(stream, ops, nops);
+    const uint64_t mark = mark_uses(stream, ops, nops);
     gb_check(gpublas_dev_scalar_wait(stream, seq), "BLAS reduction kernel");
+    /* the reduction has published, so the stream ran everything before it and
+     * the reduction itself has read its operands */
+    gpublas_dev_stream_observed(stream, mark);
     /* the kernel has consumed its inputs: staging blocks can be reused */
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].tmp) gpublas_dev_release(ops[i].tmp);
+}
+
+bool gb_writers_done(const gb_operand *ops, int nops)
+{
+    for (int i = 0; i < nops; ++i) {
+        struct gb_obj *o = ops[i].obj;
+        if (!o || ops[i].uploaded) return false;  /* this call's upload is still queued */
+        pthread_mutex_lock(&o->mu);
+        /* known complete only (no stream query: that costs more than it saves) */
+        const bool ok = !o->wr_stream || gpublas_dev_stream_known_done(o->wr_stream, o->wr_seq);
+        pthread_mutex_unlock(&o->mu);
+        if (!ok) return false;
+    }
+    return true;
+}
+
+void gb_finish_served(void *stream, gb_operand *ops, int nops)
+{
+    /* the server has read the operands and the caller has its result:
+     * record the uses (ordering later writers on other streams) and drop
+     * the references taken at acquire */
+    mark_uses(stream, ops, nops);
     for (int i = 0; i < nops; ++i)
         if (ops[i].tmp) gpublas_dev_release(ops[i].tmp);
 }
