@@ -10,11 +10,15 @@ operand allocations, so the measured path is the drop-in path: Fortran-ABI
 entry points, interposed malloc, HBM-resident operands, gfx950 kernels.
 
 value = algorithmic bytes of the chain (537,460,736 B per step) / wall time of
-the K timed steps, inputs already resident in HBM.  The roofline object is
+the K timed steps, inputs already resident in HBM, with the resident
+reduction server answering DDOT/DNRM2/IDAMAX (option `resident`; the line also
+gives the chain with those launched as kernels).  The roofline object is
 for the dominant kernel (dgemv 'N'): 8*(m*n + m + n) bytes per launch over its
 average duration measured with HIP events on its launch stream.  The line also
 carries the DGEMM figures the composite metric names (1024^3 and 8192^3 NN,
-kernel TFLOP/s vs the 78.6 TFLOP/s FP64 peak) and the CPU baselines.
+kernel TFLOP/s vs the 78.6 TFLOP/s FP64 peak) and the CPU baselines: the host
+BLAS on all cores (cpu_baseline) and the oracle's C loops on one
+(cpu_baseline_port).
 """
 import argparse
 import ctypes
@@ -42,7 +46,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_chain(steps, warmup, profile, seed=20240601):
+def run_chain(steps, warmup, profile, resident=True, seed=20240601):
     import libgpublas_amd as gb
     with tempfile.TemporaryDirectory() as td:
         objtrack = os.path.join(td, "chain.objtrack")
@@ -52,8 +56,7 @@ def run_chain(steps, warmup, profile, seed=20240601):
         env = dict(os.environ)
         pre = env.get("LD_PRELOAD", "").strip()
         env["LD_PRELOAD"] = (pre + " " if pre else "") + gb.PRELOAD_PATH
-        opts = f"heuristic=objtrack:{objtrack};cutoff=0"
-        env["GPUBLAS_OPTIONS"] = opts + (";profile" if profile else "")
+        env["GPUBLAS_OPTIONS"] = chain_options(objtrack, resident) + (";profile" if profile else "")
         cmd = [gb.CHAIN_BENCH, "--n", str(N), "--iters", str(steps), "--warmup", str(warmup),
                "--seed", str(seed)]
         log("chain: " + " ".join(cmd))
@@ -66,6 +69,13 @@ def run_chain(steps, warmup, profile, seed=20240601):
     return res
 
 
+def chain_options(objtrack, resident):
+    """GPUBLAS_OPTIONS of the chain: the tracking file, no size cutoff (every
+    call of the chain goes to the device) and, with `resident`, the resident
+    reduction server for DDOT/DNRM2/IDAMAX."""
+    return f"heuristic=objtrack:{objtrack};cutoff=0" + (";resident" if resident else "")
+
+
 def run_host_blas_chain(steps, threads):
     if not os.path.exists(SCIPY_OPENBLAS):
         return None
@@ -73,11 +83,31 @@ def run_host_blas_chain(steps, threads):
     env = dict(os.environ)
     env["OPENBLAS_NUM_THREADS"] = str(threads)
     cmd = [gb.CHAIN_BENCH, "--n", str(N), "--iters", str(steps), "--warmup", "1",
-           "--blas", SCIPY_OPENBLAS, "--prefix", "scipy_"]
+           "--blas", SCIPY_OPENBLAS, "--prefix", "scipy_", "--threads", str(threads)]
     out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
     if out.returncode != 0:
         return None
     return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def host_blas_chain_baseline(budget_s=12.0):
+    """The reference's CPU path on the same chain: libgpublas forwards every
+    call it does not run on the device to the host BLAS (dlsym RTLD_NEXT), so
+    the CPU baseline is that host BLAS (scipy's OpenBLAS 0.3.29) on every core
+    this process may run on, over a bounded sample of about `budget_s`."""
+    cores = len(os.sched_getaffinity(0))
+    probe = run_host_blas_chain(8, cores)
+    if not probe:
+        return None
+    iters = int(min(3000, max(20, budget_s * 1e3 / max(probe["ms_per_iter"], 1e-3))))
+    hb = run_host_blas_chain(iters, cores)
+    if not hb:
+        return None
+    return {"value": round(hb["GBps"], 3), "unit": "GB/s", "cores": hb.get("blas_threads", cores),
+            "kind": "reference", "nproc": os.cpu_count(), "affinity_cpus": cores,
+            "library": "OpenBLAS 0.3.29 (scipy), the host BLAS the reference forwards to",
+            "sample": f"{hb['iters']} chain steps at n={N} ({hb['seconds']:.1f} s), "
+                      f"p50 {hb.get('iter_ms_p50', 0):.3f} ms/step"}
 
 
 def host_dgemm_1024(cores):
@@ -373,6 +403,8 @@ def main():
     # run with HIP events around every kernel on its launch stream
     chain = run_chain(args.steps, args.warmup, profile=False)
     prof = run_chain(args.steps, args.warmup, profile=True)
+    # the same chain with every reduction launched as a kernel (no server)
+    plain_chain = run_chain(args.steps, args.warmup, profile=False, resident=False)
     secs = chain["seconds"]
     value = BYTES_PER_STEP * args.steps / secs / 1e9
     kcount = prof.get("k_dgemv_n_count", 0)
@@ -416,7 +448,10 @@ def main():
         "data": "synthetic (SplitMix64 uniform[-1,1], seed 20240601)",
         "config": {"workload": "level-1/2 chain on tracked buffers: dgemv N 8192x8192 + daxpy + "
                                "ddot + dnrm2 + idamax, FP64, LD_PRELOAD drop-in",
-                   "n": N, "bytes_per_step": BYTES_PER_STEP},
+                   "n": N, "bytes_per_step": BYTES_PER_STEP,
+                   "options": chain_options("<file>", True)},
+        "value_without_resident": round(BYTES_PER_STEP * args.steps / plain_chain["seconds"] / 1e9, 3),
+        "ms_per_step_p50_without_resident": round(plain_chain.get("iter_ms_p50", 0), 4),
         "roofline": roofline,
         "kernel_avg_us": kernels,
         "chain_results": {"ddot": chain["ddot"], "dnrm2": chain["dnrm2"], "idamax": chain["idamax"]},
@@ -424,17 +459,17 @@ def main():
         "serving": srv,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_oracle_chain()
         cores = len(os.sched_getaffinity(0))
-        cores = min(cores, 16)
-        hb = run_host_blas_chain(20, cores)
-        hb1 = run_host_blas_chain(5, 1)
+        port = cpu_oracle_chain()
+        hb = host_blas_chain_baseline()
         if hb:
-            line["host_blas_baseline"] = {"value": round(hb["GBps"], 3), "unit": "GB/s",
-                                          "cores": cores, "library": "OpenBLAS 0.3.29 (scipy)",
-                                          "sample": f"{hb['iters']} chain steps at n={N}"}
+            hb1 = run_host_blas_chain(10, 1)
             if hb1:
-                line["host_blas_baseline"]["value_1_thread"] = round(hb1["GBps"], 3)
+                hb["value_1_thread"] = round(hb1["GBps"], 3)
+            line["cpu_baseline"] = hb
+            line["cpu_baseline_port"] = port
+        else:  # no host BLAS on this machine: the oracle port is the baseline
+            line["cpu_baseline"] = port
         hg = host_dgemm_1024(cores)
         if hg:
             line["dgemm"]["host_blas_1024"] = hg

@@ -4,8 +4,9 @@
  * symbols, knowing nothing of the GPU.  Run it with
  *   LD_PRELOAD=libgpublas_preload.so GPUBLAS_OPTIONS="heuristic=objtrack:<f>"
  * and the allocations listed in <f> (its first four: A, x, y, z) become
- * HBM-resident tracked objects; run it with --blas <lib> [--prefix p] and the
- * same loop runs on that host BLAS instead.
+ * HBM-resident tracked objects; run it with --blas <lib> [--prefix p]
+ * [--threads T] and the same loop runs on that host BLAS instead (T threads
+ * through OpenBLAS's <p>openblas_set_num_threads where the library has it).
  *
  * Per iteration (n = 8192 by default, FP64, entries uniform[-1,1]):
  *   y = A*x (dgemv 'N', alpha 1, beta 0); z = 0.3*y + z (daxpy);
@@ -72,6 +73,7 @@ int main(int argc, char **argv)
     int n = 8192, iters = 200, warmup = 5;
     uint64_t seed = 20240601;
     const char *blas = NULL, *prefix = "", *direct = NULL;
+    int threads = 0; /* --blas: OpenBLAS threads (0: its default) */
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--n") && i + 1 < argc) n = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--iters") && i + 1 < argc) iters = atoi(argv[++i]);
@@ -80,6 +82,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--blas") && i + 1 < argc) blas = argv[++i];
         else if (!strcmp(argv[i], "--prefix") && i + 1 < argc) prefix = argv[++i];
         else if (!strcmp(argv[i], "--lib") && i + 1 < argc) direct = argv[++i];
+        else if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
     }
     void *(*alloc)(size_t) = malloc;
     void (*release)(void *) = free;
@@ -114,6 +117,17 @@ int main(int argc, char **argv)
             fprintf(stderr, "chain_bench: %s\n", dlerror());
             return 2;
         }
+    }
+    int blas_threads = -1;
+    if (blas) {
+        /* OpenBLAS's thread controls, when the library has them */
+        char nm[128];
+        snprintf(nm, sizeof nm, "%sopenblas_set_num_threads", prefix);
+        void (*set_threads)(int) = (void (*)(int))dlsym(h, nm);
+        snprintf(nm, sizeof nm, "%sopenblas_get_num_threads", prefix);
+        int (*get_threads)(void) = (int (*)(void))dlsym(h, nm);
+        if (set_threads && threads > 0) set_threads(threads);
+        if (get_threads) blas_threads = get_threads();
     }
     gemv_t gemv = (gemv_t)sym(h, prefix, "dgemv_");
     axpy_t axpy = (axpy_t)sym(h, prefix, "daxpy_");
@@ -169,6 +183,7 @@ int main(int argc, char **argv)
                1e3 * p50, 1e3 * p90);
         free(it_s);
     }
+    if (blas) printf(", \"blas_threads\": %d", blas_threads);
     if (ktime) {
         const char *ks[] = {"dgemv_n", "daxpy", "ddot", "dnrm2", "idamax"};
         for (int i = 0; i < 5; ++i) {

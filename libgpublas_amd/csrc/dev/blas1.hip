@@ -6,7 +6,7 @@
 // (nrm2.cc:39), cublasIdamax (amax.cc:40).
 //
 // The reductions are single launches.  Vectors of up to 65536 elements are
-// reduced by one 1024-thread workgroup (one memory round trip, no partials).
+// reduced by one 512-thread workgroup (no partials, no tickets).
 // Longer ones: every workgroup reduces its grid-stride share in a fixed
 // order (per-lane sequential, then a wave64 shuffle tree, then the 4 waves
 // in order), writes one partial, and the workgroup that draws the last
@@ -353,11 +353,18 @@ __global__ __launch_bounds__(256) void iamax_kernel(int n, const double *x, long
 }
 
 // ---- single-workgroup reductions for short vectors ---------------------------
-// Below kSmallN one 1024-thread workgroup reduces the whole vector: thread t
-// takes elements 2t, 2t+1, 2t+2048, ... (16-B loads, four per operand in
-// flight), then a wave64 shuffle tree and the 16 waves in order.  No
-// tickets, no partials: one memory round trip and the result publication.
+// Below kSmallN one 512-thread workgroup reduces the whole vector: thread t
+// takes elements 2t, 2t+1, 2t+1024, ... (16-B loads, eight per operand in
+// flight), then a wave64 shuffle tree and the 8 waves in order.  No
+// tickets, no partials: one memory round trip per 8192 elements and the
+// result publication.  512 threads at <= 128 VGPRs take half of one CU's
+// registers, so the resident server (the same code) leaves room on its CU
+// for other kernels' waves.
 constexpr int kSmallN = 1 << 16;
+constexpr int kWgThreads = 512;              // threads of a single-workgroup reduction
+constexpr int kWgWaves = kWgThreads / 64;
+constexpr int kWgStride = 2 * kWgThreads;    // elements per sweep of the workgroup
+constexpr int kWgU = 8;                      // 16-B loads in flight per operand
 typedef double d2v __attribute__((ext_vector_type(2)));
 
 template <typename F>
@@ -365,13 +372,13 @@ __device__ __forceinline__ void for_pairs(int n, F &&f)
 {
     // calls f(i) for the element indices of this thread in increasing order
     int i = 2 * (int)threadIdx.x;
-    for (; i < n; i += 2048) {
+    for (; i < n; i += kWgStride) {
         f(i);
         if (i + 1 < n) f(i + 1);
     }
 }
 
-__device__ __forceinline__ double block_sum16(double v, double *red)
+__device__ __forceinline__ double block_sum_wg(double v, double *red)
 {
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -379,7 +386,7 @@ __device__ __forceinline__ double block_sum16(double v, double *red)
     __syncthreads();
     double r = 0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) r += red[w];
+    for (int w = 0; w < kWgWaves; ++w) r += red[w];
     __syncthreads();
     return r;
 }
@@ -388,31 +395,31 @@ __device__ __forceinline__ void dot_wg(int n, const double *x, long incx, long x
                                        long incy, long y0, int vec, double *result, uint64_t *flag,
                                        uint64_t seq)
 {
-    __shared__ double red[16];
+    __shared__ double red[kWgWaves];
     double s = 0;
     if (vec) {
         int i = 2 * (int)threadIdx.x;
-        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
-            d2v xv[4], yv[4];
+        for (; i + (kWgU - 1) * kWgStride + 1 < n; i += kWgU * kWgStride) {
+            d2v xv[kWgU], yv[kWgU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                xv[u] = *(const d2v *)(x + i + 2048 * u);
-                yv[u] = *(const d2v *)(y + i + 2048 * u);
+            for (int u = 0; u < kWgU; ++u) {
+                xv[u] = *(const d2v *)(x + i + kWgStride * u);
+                yv[u] = *(const d2v *)(y + i + kWgStride * u);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kWgU; ++u) {
                 s = fma(xv[u].x, yv[u].x, s);
                 s = fma(xv[u].y, yv[u].y, s);
             }
         }
-        for (; i < n; i += 2048) {
+        for (; i < n; i += kWgStride) {
             s = fma(x[i], y[i], s);
             if (i + 1 < n) s = fma(x[i + 1], y[i + 1], s);
         }
     } else {
         for_pairs(n, [&](int i) { s = fma(x[x0 + i * incx], y[y0 + i * incy], s); });
     }
-    s = block_sum16(s, red);
+    s = block_sum_wg(s, red);
     if (threadIdx.x == 0) {
         *result = s;
         publish(flag, seq);
@@ -422,30 +429,30 @@ __device__ __forceinline__ void dot_wg(int n, const double *x, long incx, long x
 __device__ __forceinline__ void nrm2_wg(int n, const double *x, long incx, long x0, int vec,
                                         double *result, uint64_t *flag, uint64_t seq)
 {
-    __shared__ double red[16];
+    __shared__ double red[kWgWaves];
     Acc3 a{0, 0, 0};
     if (vec) {
         int i = 2 * (int)threadIdx.x;
-        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
-            d2v xv[4];
+        for (; i + (kWgU - 1) * kWgStride + 1 < n; i += kWgU * kWgStride) {
+            d2v xv[kWgU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+            for (int u = 0; u < kWgU; ++u) xv[u] = *(const d2v *)(x + i + kWgStride * u);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kWgU; ++u) {
                 acc3_add(a, xv[u].x);
                 acc3_add(a, xv[u].y);
             }
         }
-        for (; i < n; i += 2048) {
+        for (; i < n; i += kWgStride) {
             acc3_add(a, x[i]);
             if (i + 1 < n) acc3_add(a, x[i + 1]);
         }
     } else {
         for_pairs(n, [&](int i) { acc3_add(a, x[x0 + i * incx]); });
     }
-    a.sml = block_sum16(a.sml, red);
-    a.med = block_sum16(a.med, red);
-    a.big = block_sum16(a.big, red);
+    a.sml = block_sum_wg(a.sml, red);
+    a.med = block_sum_wg(a.med, red);
+    a.big = block_sum_wg(a.big, red);
     if (threadIdx.x == 0) {
         *result = nrm2_finish(a.sml, a.med, a.big);
         publish(flag, seq);
@@ -456,22 +463,22 @@ template <bool MIN>
 __device__ __forceinline__ void iamax_wg(int n, const double *x, long incx, int vec, int *result,
                                          uint64_t *flag, uint64_t seq)
 {
-    __shared__ Cand red[16];
+    __shared__ Cand red[kWgWaves];
     Cand c = cand_none<MIN>();
     auto take = [&](long i, double v) { c = better<MIN>(c, cand_of<MIN>(i, v)); };
     if (vec) {
         int i = 2 * (int)threadIdx.x;
-        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
-            d2v xv[4];
+        for (; i + (kWgU - 1) * kWgStride + 1 < n; i += kWgU * kWgStride) {
+            d2v xv[kWgU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+            for (int u = 0; u < kWgU; ++u) xv[u] = *(const d2v *)(x + i + kWgStride * u);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                take(i + 2048 * u, xv[u].x);
-                take(i + 2048 * u + 1, xv[u].y);
+            for (int u = 0; u < kWgU; ++u) {
+                take(i + kWgStride * u, xv[u].x);
+                take(i + kWgStride * u + 1, xv[u].y);
             }
         }
-        for (; i < n; i += 2048) {
+        for (; i < n; i += kWgStride) {
             take(i, x[i]);
             if (i + 1 < n) take(i + 1, x[i + 1]);
         }
@@ -485,20 +492,20 @@ __device__ __forceinline__ void iamax_wg(int n, const double *x, long incx, int 
     if (threadIdx.x == 0) {
         Cand t = red[0];
 #pragma unroll
-        for (int w = 1; w < 16; ++w) t = better<MIN>(t, red[w]);
+        for (int w = 1; w < kWgWaves; ++w) t = better<MIN>(t, red[w]);
         *result = (int)cand_result<MIN>(t, x);
         publish(flag, seq);
     }
 }
 
-__global__ __launch_bounds__(1024) void dot_small(int n, const double *x, long incx, long x0,
+__global__ __launch_bounds__(kWgThreads) void dot_small(int n, const double *x, long incx, long x0,
                                                   const double *y, long incy, long y0, int vec,
                                                   double *result, uint64_t *flag, uint64_t seq)
 {
     dot_wg(n, x, incx, x0, y, incy, y0, vec, result, flag, seq);
 }
 
-__global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long incx, long x0,
+__global__ __launch_bounds__(kWgThreads) void nrm2_small(int n, const double *x, long incx, long x0,
                                                    int vec, double *result, uint64_t *flag,
                                                    uint64_t seq)
 {
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(1024) void nrm2_small(int n, const double *x, long 
 }
 
 template <bool MIN>
-__global__ __launch_bounds__(1024) void iamax_small(int n, const double *x, long incx, int vec,
+__global__ __launch_bounds__(kWgThreads) void iamax_small(int n, const double *x, long incx, int vec,
                                                     int *result, uint64_t *flag, uint64_t seq)
 {
     iamax_wg<MIN>(n, x, incx, vec, result, flag, seq);
@@ -586,35 +593,35 @@ __global__ __launch_bounds__(256) void rotm_kernel(int n, double *x, long incx, 
 __device__ __forceinline__ void asum_wg(int n, const double *x, long incx, int vec,
                                         double *result, uint64_t *flag, uint64_t seq)
 {
-    __shared__ double red[16];
+    __shared__ double red[kWgWaves];
     double s = 0;
     if (vec) {
         int i = 2 * (int)threadIdx.x;
-        for (; i + 3 * 2048 + 1 < n; i += 4 * 2048) {
-            d2v xv[4];
+        for (; i + (kWgU - 1) * kWgStride + 1 < n; i += kWgU * kWgStride) {
+            d2v xv[kWgU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xv[u] = *(const d2v *)(x + i + 2048 * u);
+            for (int u = 0; u < kWgU; ++u) xv[u] = *(const d2v *)(x + i + kWgStride * u);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kWgU; ++u) {
                 s += fabs(xv[u].x);
                 s += fabs(xv[u].y);
             }
         }
-        for (; i < n; i += 2048) {
+        for (; i < n; i += kWgStride) {
             s += fabs(x[i]);
             if (i + 1 < n) s += fabs(x[i + 1]);
         }
     } else {
         for_pairs(n, [&](int i) { s += fabs(x[i * incx]); });
     }
-    s = block_sum16(s, red);
+    s = block_sum_wg(s, red);
     if (threadIdx.x == 0) {
         *result = s;
         publish(flag, seq);
     }
 }
 
-__global__ __launch_bounds__(1024) void asum_small(int n, const double *x, long incx, int vec,
+__global__ __launch_bounds__(kWgThreads) void asum_small(int n, const double *x, long incx, int vec,
                                                    double *result, uint64_t *flag, uint64_t seq)
 {
     asum_wg(n, x, incx, vec, result, flag, seq);
@@ -646,10 +653,11 @@ __global__ __launch_bounds__(256) void asum_kernel(int n, const double *x, long 
 // reductions DDOT / DNRM2 / IDAMAX / IDAMIN / DASUM from a ring of request
 // slots in pinned host memory, so a scalar-returning call costs no kernel
 // launch: the caller fills a slot and publishes its sequence number; the
-// server workgroup (1024 threads), already waiting on that slot, sees it,
-// reads the descriptor (one word per lane, one trip over the link), runs the
-// same single-workgroup reduction as the launched kernel (identical bits)
-// and publishes the result in the slot.  It serves a call only when the
+// server workgroup (512 threads), already polling that slot, sees it --
+// the whole descriptor in the same read, each word tagged with the request's
+// id, so one trip over the link both finds the request and brings it in --
+// runs the same single-workgroup reduction as the launched kernel (identical
+// bits) and publishes the result in the slot.  It serves a call only when the
 // kernels that wrote its operands have ended (the caller's stream has no work
 // queued, or the caller knows those writers completed): a launch queued
 // behind such work starts as soon as it ends, which a hand-over cannot beat,
@@ -658,62 +666,81 @@ __global__ __launch_bounds__(256) void asum_kernel(int n, const double *x, long 
 // request starts it again; a caller whose request was not served (the server
 // exited first) takes the launched path.  So a server that shares a hardware
 // queue with other work delays that work by at most the idle time, and
-// nothing waits on it forever.  Descriptor fields and flags are read with
+// nothing waits on it forever.  Descriptor words and flags are read with
 // atomic loads (never through the scalar cache, which a persistent kernel
 // would not see refreshed); results are written with vector stores and
 // published with a system-scope release.
 enum { RS_DOT = 0, RS_NRM2, RS_IAMAX, RS_IAMIN, RS_ASUM };
 constexpr int kResSlots = 64;
 
+// Descriptor words: bits 63..48 carry the request's tag, bits 47..0 the
+// field.  The host stores each word whole (8-byte aligned: single-copy
+// atomic on both sides), so a read that finds the expected tag in all five
+// words has the request's whole descriptor, whatever the order in which the
+// link delivered the words.  A slot is reused every kResSlots requests, and
+// tags of requests less than 2^15 apart differ, so stale words never match.
+// Device addresses fit in 48 bits (the host checks each one).
+constexpr int kResWords = 5;
 struct alignas(128) ResSlot {
-    uint64_t seq;            // request id + 1, written last by the host
-    uint64_t op, n;
-    int64_t incx, incy, x0, y0, vec;
-    uint64_t x, y;           // device addresses
-    double dres;             // results, written by the server
+    uint64_t w[8];           // [0] op | vec << 3 | n << 4, [1] x, [2] y, [3] incx, [4] incy
+    double dres;             // results, written by the server (their own 64 B line)
     int64_t ires;
-    uint64_t done;           // = seq once the result is valid
+    uint64_t done;           // = request id + 1 once the result is valid
+    uint64_t pad[5];
 };
 static_assert(sizeof(ResSlot) == 128, "one slot per 128 B");
-constexpr int kResWords = 10;  // descriptor words read by the server (seq .. y)
 
-__global__ __launch_bounds__(1024) void res_server(ResSlot *ring, unsigned long long first,
+__host__ __device__ inline uint64_t res_tag(unsigned long long id)
+{
+    return (uint64_t)(((id + 1) & 0x7fffu) | 0x8000u) << 48;  // never zero: a cleared slot never matches
+}
+constexpr uint64_t kResField = (uint64_t(1) << 48) - 1;
+
+__global__ __launch_bounds__(kWgThreads, 4) void res_server(ResSlot *ring, unsigned long long first,
                                                    const int *stop, long long idle)
 {
     __shared__ uint64_t d[kResWords];
     __shared__ int quit;
     for (unsigned long long id = first;; ++id) {
         ResSlot *sl = ring + id % kResSlots;
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {  // wave 0 polls: lane i reads word i
+            const int lane = threadIdx.x;
+            const uint64_t tag = res_tag(id);
             int q = 0;
+            uint64_t v = 0;
             const long long t0 = wall_clock64();
             for (unsigned spin = 0;; ++spin) {
-                if (__hip_atomic_load(&sl->seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == id + 1)
-                    break;
-                if ((spin & 15) == 0 &&
-                    (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
-                     wall_clock64() - t0 > idle)) {
-                    q = 1;
-                    break;
+                bool ok = true;
+                if (lane < kResWords) {
+                    v = __hip_atomic_load(sl->w + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    ok = (v & ~kResField) == tag;
+                }
+                if (__ballot(ok) == __ballot(1)) break;  // all five words carry the tag
+                if ((spin & 15) == 0) {
+                    int sv = 0;
+                    if (lane == 0)
+                        sv = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+                             wall_clock64() - t0 > idle;
+                    if (__builtin_amdgcn_readfirstlane(sv)) {
+                        q = 1;
+                        break;
+                    }
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
-            quit = q;
+            if (lane < kResWords) d[lane] = v & kResField;
+            if (lane == 0) quit = q;
         }
         __syncthreads();
         if (quit) return;
-        // the descriptor: one word per lane, all read at once (one trip over the link)
-        if (threadIdx.x < kResWords)
-            d[threadIdx.x] = __hip_atomic_load((const uint64_t *)sl + threadIdx.x, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_SYSTEM);
-        __syncthreads();
         // see the operands' latest contents (their writers ended before the
         // caller's stream went idle)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const int op = (int)d[1], n = (int)d[2];
-        const long incx = (long)d[3], incy = (long)d[4], x0 = (long)d[5], y0 = (long)d[6];
-        const int vec = (int)d[7];
-        const double *x = (const double *)d[8], *y = (const double *)d[9];
+        const int op = (int)(d[0] & 7), vec = (int)((d[0] >> 3) & 1), n = (int)(d[0] >> 4);
+        const double *x = (const double *)d[1], *y = (const double *)d[2];
+        const long incx = (long)(int)(uint32_t)d[3], incy = (long)(int)(uint32_t)d[4];
+        const long x0 = incx < 0 ? -(long)(n - 1) * incx : 0;
+        const long y0 = incy < 0 ? -(long)(n - 1) * incy : 0;
         int *ires = (int *)&sl->ires;
         switch (op) {
         case RS_DOT: dot_wg(n, x, incx, x0, y, incy, y0, vec, &sl->dres, &sl->done, id + 1); break;
@@ -740,10 +767,19 @@ struct ResServer {
     long long idle = 0;       // steady-clock ticks of idle before the server exits
     uint64_t idle_ns = 0;     // the same in host nanoseconds
     uint64_t last_ns = 0;     // host time the last request ended
+    bool closed = false;      // process exit has begun: no new start
 };
 ResServer g_res;
 
-void res_stop_at_exit() { gpublas_dev_resident_stop(); }
+// At exit: stop the server and release its queue while the runtime is whole.
+void res_stop_at_exit()
+{
+    gpublas_dev_resident_stop();
+    std::lock_guard<std::mutex> g(g_res.mu);
+    g_res.closed = true;
+    if (g_res.stream) (void)hipStreamDestroy(g_res.stream);
+    g_res.stream = nullptr;
+}
 
 uint64_t res_now_ns()
 {
@@ -769,7 +805,7 @@ int res_ensure_locked()
     const hipError_t q = hipStreamSynchronize(g_res.stream);
     if (q != hipSuccess) return (int)q;
     *(volatile int *)g_res.stop = 0;
-    hipLaunchKernelGGL(res_server, dim3(1), dim3(1024), 0, g_res.stream, g_res.ring,
+    hipLaunchKernelGGL(res_server, dim3(1), dim3(kWgThreads), 0, g_res.stream, g_res.ring,
                        (unsigned long long)g_res.next, (const int *)g_res.stop, g_res.idle);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
@@ -786,6 +822,7 @@ int gpublas_dev_resident_start(void)
     if (gpublas_dev_init() != 0) return (int)hipErrorNoDevice;
     std::lock_guard<std::mutex> g(g_res.mu);
     if (g_res.enabled) return 0;
+    if (g_res.closed) return (int)hipErrorNotReady;
     if (!g_res.ring) {
         // a queue of its own where the runtime provides one (CU-masked streams)
         uint32_t mask[8];
@@ -829,6 +866,7 @@ int gpublas_dev_resident_reduce(void *stream, int op, int n, const double *d_x, 
     if (!__atomic_load_n(&g_res.enabled, __ATOMIC_ACQUIRE)) return -1;
     if (n < 1 || n > kSmallN || op < RS_DOT || op > RS_ASUM) return -1;
     if ((op == RS_IAMAX || op == RS_IAMIN || op == RS_ASUM) && incx <= 0) return -1;
+    if (((uintptr_t)d_x | (uintptr_t)d_y) & ~(uintptr_t)kResField) return -1;  // beyond 48 bits
     hipStream_t s = pick_stream(stream);
     DevCtx *c = ctx_of(s);
     if (!c) return -1;
@@ -846,16 +884,14 @@ int gpublas_dev_resident_reduce(void *stream, int op, int n, const double *d_x, 
     const uint64_t id = g_res.next++;
     ResSlot *sl = g_res.ring + id % kResSlots;
     const int vy = op == RS_DOT ? (((uintptr_t)d_y & 15) == 0 && incy == 1) : 1;
-    sl->op = (uint64_t)op;
-    sl->n = (uint64_t)n;
-    sl->incx = incx;
-    sl->incy = incy;
-    sl->x0 = incx < 0 ? -(long)(n - 1) * incx : 0;
-    sl->y0 = incy < 0 ? -(long)(n - 1) * incy : 0;
-    sl->vec = incx == 1 && ((uintptr_t)d_x & 15) == 0 && vy;
-    sl->x = (uint64_t)(uintptr_t)d_x;
-    sl->y = (uint64_t)(uintptr_t)d_y;
-    __atomic_store_n(&sl->seq, id + 1, __ATOMIC_RELEASE);
+    const int vec = incx == 1 && ((uintptr_t)d_x & 15) == 0 && vy;
+    const uint64_t tag = res_tag(id);
+    __atomic_store_n(&sl->w[1], tag | (uint64_t)(uintptr_t)d_x, __ATOMIC_RELAXED);
+    __atomic_store_n(&sl->w[2], tag | (uint64_t)(uintptr_t)d_y, __ATOMIC_RELAXED);
+    __atomic_store_n(&sl->w[3], tag | (uint32_t)incx, __ATOMIC_RELAXED);
+    __atomic_store_n(&sl->w[4], tag | (uint32_t)incy, __ATOMIC_RELAXED);
+    __atomic_store_n(&sl->w[0], tag | (uint64_t)op | (uint64_t)vec << 3 | (uint64_t)n << 4,
+                     __ATOMIC_RELEASE);
     for (unsigned spin = 1;; ++spin) {
         if (__atomic_load_n(&sl->done, __ATOMIC_ACQUIRE) == id + 1) break;
         if ((spin & 1023) == 0) {
@@ -909,7 +945,7 @@ int gpublas_dev_ddot(void *stream, int n, const double *d_x, int incx, const dou
     if (n <= kSmallN) {
         const int vec = incx == 1 && incy == 1 && aligned16(d_x) && aligned16(d_y);
         PROF_BEGIN(s, "ddot");
-        hipLaunchKernelGGL(dot_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx,
+        hipLaunchKernelGGL(dot_small, dim3(1), dim3(kWgThreads), 0, s, n, d_x, (long)incx,
                            start_of(n, incx), d_y, (long)incy, start_of(n, incy), vec, h_result,
                            pb.flag, pb.seq);
         PROF_END(s);
@@ -937,7 +973,7 @@ int gpublas_dev_dnrm2(void *stream, int n, const double *d_x, int incx, double *
     if (n <= kSmallN) {
         const int vec = incx == 1 && aligned16(d_x);
         PROF_BEGIN(s, "dnrm2");
-        hipLaunchKernelGGL(nrm2_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx,
+        hipLaunchKernelGGL(nrm2_small, dim3(1), dim3(kWgThreads), 0, s, n, d_x, (long)incx,
                            start_of(n, incx), vec, h_result, pb.flag, pb.seq);
         PROF_END(s);
         return (int)hipGetLastError();
@@ -971,7 +1007,7 @@ int launch_iamax(void *stream, int n, const double *d_x, int incx, int *h_result
     if (n <= kSmallN) {
         const int vec = incx == 1 && aligned16(d_x);
         PROF_BEGIN(s, name);
-        hipLaunchKernelGGL(iamax_small<MIN>, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
+        hipLaunchKernelGGL(iamax_small<MIN>, dim3(1), dim3(kWgThreads), 0, s, n, d_x, (long)incx, vec,
                            h_result, pb.flag, pb.seq);
         PROF_END(s);
         return (int)hipGetLastError();
@@ -1058,7 +1094,7 @@ int gpublas_dev_dasum(void *stream, int n, const double *d_x, int incx, double *
     if (n <= kSmallN) {
         const int vec = incx == 1 && aligned16(d_x);
         PROF_BEGIN(s, "dasum");
-        hipLaunchKernelGGL(asum_small, dim3(1), dim3(1024), 0, s, n, d_x, (long)incx, vec,
+        hipLaunchKernelGGL(asum_small, dim3(1), dim3(kWgThreads), 0, s, n, d_x, (long)incx, vec,
                            h_result, pb.flag, pb.seq);
         PROF_END(s);
         return (int)hipGetLastError();

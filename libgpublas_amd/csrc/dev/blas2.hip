@@ -91,15 +91,17 @@ __global__ __launch_bounds__(256) void gemv_n_fold(int m, int ncb, const double 
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
+    // all of a group's loads are issued before its sums (one memory round
+    // trip per 16 panels); the sum runs in panel order
+    constexpr int G = 16;
     double s = 0;
     int b = 0;
-    for (; b + 4 <= ncb; b += 4) {
-        const double p0 = partials[(long)b * m + i], p1 = partials[(long)(b + 1) * m + i];
-        const double p2 = partials[(long)(b + 2) * m + i], p3 = partials[(long)(b + 3) * m + i];
-        s += p0;
-        s += p1;
-        s += p2;
-        s += p3;
+    for (; b + G <= ncb; b += G) {
+        double p[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) p[u] = partials[(long)(b + u) * m + i];
+#pragma unroll
+        for (int u = 0; u < G; ++u) s += p[u];
     }
     for (; b < ncb; ++b) s += partials[(long)b * m + i];
     double *yp = y + y0 + i * incy;
