@@ -260,8 +260,20 @@ typedef struct {
     uint64_t serve_run_ns;   /* served calls: summed launch -> completion seen */
     uint64_t serve_launch_ns;/* engine time spent forming and launching batches */
     uint64_t resident;       /* reductions served by the resident server (option resident) */
+    uint64_t io_syncs;       /* tracked objects brought to the host for a read/write/fread...
+                                call on their memory (device-newer download or view opened) */
+    uint64_t handler_reinstalls; /* SIGSEGV handler found replaced (not through
+                                sigaction/signal) and installed again */
 } gpublas_stats_t;
 void gpublas_get_stats(gpublas_stats_t *out);
+/* Test hooks for the coherence machinery without a device: set the
+ * residency state of a tracked allocation that has no device copy yet
+ * (0 HOST: view read-write, 1 SHARED: read-only, 2 DEVICE: inaccessible;
+ * returns 0, or -1 if p does not start such an allocation), and read it
+ * back (-1 if p is not tracked).  No counterpart in the reference, whose
+ * managed memory has no such states. */
+int gpublas_debug_set_residency(void *p, int state);
+int gpublas_debug_residency(const void *p);
 void gpublas_reset_stats(void);
 /* Wait for every kernel the calling thread has launched (calls on resident
  * operands may return before their kernel ends; see async_returns). */

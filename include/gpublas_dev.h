@@ -36,6 +36,10 @@ int gpublas_dev_stream_done(void *stream, uint64_t seq);  /* non-blocking: 1 if 
 /* The caller has seen work up to `seq` of the stream end (e.g. a reduction
  * published after it): later done/wait queries for it return at once. */
 void gpublas_dev_stream_observed(void *stream, uint64_t seq);
+/* In a child process after fork(): the GPU runtime is unusable there; mark
+ * the device layer uninitialisable and forget the resident server without
+ * any runtime call. */
+void gpublas_dev_forked_child(void);
 /* 1 if a synchronisation or observation has already seen `seq` complete (no query). */
 int gpublas_dev_stream_known_done(void *stream, uint64_t seq);
 /* Scalar results: the next sequence number of the calling thread's result

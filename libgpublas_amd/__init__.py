@@ -55,7 +55,7 @@ class Stats(ctypes.Structure):
         "tracked_objects", "tracked_bytes", "faults", "uploads", "allocs_seen",
         "async_returns", "evictions", "served", "batches", "max_batch", "deferred",
         "over_budget", "resident_bytes", "serve_wait_ns", "serve_run_ns", "serve_launch_ns",
-        "resident")]
+        "resident", "io_syncs", "handler_reinstalls")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -38,6 +38,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include "gpublas.h"
 
 typedef void (*gemm_t)(const char *, const char *, const int *, const int *, const int *,
                        const void *, const void *, const int *, const void *, const int *,
@@ -49,12 +50,7 @@ typedef void (*trsm_t)(const char *, const char *, const char *, const char *, c
                        const int *, const double *, const double *, const int *, double *,
                        const int *);
 
-typedef struct {
-    uint64_t calls, device_calls, fallthrough, hits, misses, h2d_bytes, d2h_bytes;
-    uint64_t tracked_objects, tracked_bytes, faults, uploads, allocs_seen, async_returns;
-    uint64_t evictions, served, batches, max_batch, deferred, over_budget, resident_bytes;
-    uint64_t serve_wait_ns, serve_run_ns, serve_launch_ns, resident;
-} stats_t;
+typedef gpublas_stats_t stats_t; /* the library fills the whole struct */
 
 static gemm_t p_dgemm, p_sgemm;
 static gemv_t p_dgemv;

@@ -19,6 +19,7 @@
 #include "gpublas_dev.h"
 #include <math.h>
 #include <mutex>
+#include <new>
 #include <time.h>
 #include <stdlib.h>
 #include <string.h>
@@ -768,12 +769,15 @@ struct ResServer {
     uint64_t idle_ns = 0;     // the same in host nanoseconds
     uint64_t last_ns = 0;     // host time the last request ended
     bool closed = false;      // process exit has begun: no new start
+    bool forked = false;      // this is a fork child: the server is the parent's
 };
 ResServer g_res;
 
-// At exit: stop the server and release its queue while the runtime is whole.
+// At exit: stop the server and release its queue while the runtime is whole
+// (not in a fork child, whose runtime is not the parent's).
 void res_stop_at_exit()
 {
+    if (g_res.forked) return;
     gpublas_dev_resident_stop();
     std::lock_guard<std::mutex> g(g_res.mu);
     g_res.closed = true;
@@ -814,6 +818,15 @@ int res_ensure_locked()
 }
 
 }  // namespace
+
+void res_forget_after_fork()
+{
+    new (&g_res.mu) std::mutex();  // possibly held by a parent thread at the fork
+    g_res.forked = true;
+    g_res.closed = true;
+    g_res.enabled = false;
+    g_res.running = false;
+}
 
 extern "C" {
 

@@ -78,6 +78,8 @@ static constexpr int kTicketSlots = 4096;  // ticket counters per context
 // the device current.  Returns 0 or a HIP error code.
 int noalloc_scratch_setup(int device);
 bool dev_ready();  // gpublas_dev_init() has run and found a gfx950 device (no init here)
+// (blas1.hip) drop the resident server's state in a fork child, no runtime call
+void res_forget_after_fork();
 
 // Kernel timing (HIP events on the launch stream), enabled by
 // gpublas_dev_prof_enable.  Wrap each launch as

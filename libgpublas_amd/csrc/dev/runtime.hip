@@ -172,6 +172,7 @@ __global__ void fill_uniform_f32_kernel(uint64_t seed, int64_t n, float *out)
 
 void do_init()
 {
+    if (__atomic_load_n(&g_init_status, __ATOMIC_ACQUIRE) == -5) return;  // a fork child
     g_free = new std::unordered_map<size_t, std::vector<void *>>();
     g_live = new std::unordered_map<void *, size_t>();
     g_deferred = new std::vector<Deferred>();
@@ -376,6 +377,12 @@ int gpublas_dev_stream_done(void *stream, uint64_t seq)
         }
     }
     return 1;
+}
+
+void gpublas_dev_forked_child(void)
+{
+    __atomic_store_n(&g_init_status, -5, __ATOMIC_RELEASE);  // -5: after fork
+    res_forget_after_fork();
 }
 
 int gpublas_dev_stream_known_done(void *stream, uint64_t seq)

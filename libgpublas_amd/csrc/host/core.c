@@ -238,6 +238,10 @@ __attribute__((destructor)) static void gb_fini(void)
 {
     gb_enter();
     gb_tracking = 0;
+    if (gb_forked_child) { /* the serving thread, the files and the GPU are the parent's */
+        gb_leave();
+        return;
+    }
     gb_serve_stop();
     if (gb_opt.objtrack_out[0]) gb_objtrack_write();
     if (gb_opt.stats_file[0]) {
@@ -363,6 +367,10 @@ void *gb_host_fn(const char *f77name)
 /* ------------------------------------------------------------------ */
 void gb_require_device(void)
 {
+    if (gb_forked_child)
+        gb_fatal("BLAS on the GPU is not available in a child process after fork(); the "
+                 "child's calls run on the host BLAS");
+    gb_check_fault_handler();
     int st = gpublas_dev_init();
     if (st != 0)
         gb_fatal("no usable gfx950 GPU (device init status %d); libgpublas runs its BLAS "

@@ -8,6 +8,7 @@
 #include <stdint.h>
 #include <stdbool.h>
 #include <pthread.h>
+#include <signal.h>
 #include "gpublas.h"
 #include "gpublas_dev.h"
 
@@ -109,15 +110,37 @@ struct gb_obj {
     int nrd;
     int refs;            /* calls between acquire and finish, served requests in flight */
     uint64_t last_use;   /* acquire tick, for least-recently-used eviction */
+    int detached;        /* in a fork child: plain private memory, no longer tracked */
     pthread_mutex_t mu;
 };
 void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero);
 int gb_obj_destroy(void *p);                 /* 1 if p was a tracked object */
 void *gb_obj_realloc(void *p, size_t size);  /* p tracked */
-struct gb_obj *gb_obj_find(const void *p);   /* object containing p, or NULL */
+struct gb_obj *gb_obj_find(const void *p);   /* tracked object containing p, or NULL */
+int gb_obj_owns(const void *p);              /* p starts an allocation of this library */
 size_t gb_obj_usable_size(const void *p);    /* 0 if untracked */
 int gb_tracked_count(void);
 void gb_install_fault_handler(void);
+/* Make [p, p+n) usable by a system call or a library routine that does not
+ * fault into the SIGSEGV handler: every tracked object the range overlaps is
+ * brought to the host (its device copy downloaded if newer) and its view
+ * opened for reading, or for writing (demoting it to HOST) when `write`. */
+void gb_host_access(const void *p, size_t n, int write);
+/* The application's sigaction() / signal() on SIGSEGV once the library's
+ * handler is installed: `act` becomes the handler the library chains to
+ * for faults that are not its own, `old` receives the previous one.
+ * Returns 0 if the library's handler is not installed (the caller then
+ * performs the call itself). */
+int gb_segv_app_action(const struct sigaction *act, struct sigaction *old);
+/* Reinstall the library's SIGSEGV handler if something replaced it without
+ * going through sigaction()/signal() (checked at most once a millisecond). */
+void gb_check_fault_handler(void);
+/* The libc sigaction, bypassing the interposer. */
+int gb_real_sigaction(int sig, const struct sigaction *act, struct sigaction *old);
+/* A libc function by name, bypassing any interposer (NULL if absent). */
+void *gb_libc_sym(const char *name);
+/* Set in a child process after fork(): every BLAS call runs on the host. */
+extern int gb_forked_child;
 /* Make the device copy valid (upload if the host copy is newer); returns 1 if
  * it uploaded.  Takes a reference (released by gb_finish / gb_obj_unref).
  * Orders `stream` after conflicting kernels on other streams: the last

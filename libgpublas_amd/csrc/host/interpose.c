@@ -64,7 +64,7 @@ void *realloc(void *p, size_t n)
         }
         return __libc_realloc(NULL, n);
     }
-    if (gb_tracked_count() && gb_obj_find(p)) {
+    if (gb_tracked_count() && gb_obj_owns(p)) {
         if (!gb_guard) __atomic_fetch_add(&gb_nallocs, 1, __ATOMIC_RELAXED);
         if (n == 0) {
             gb_obj_destroy(p);

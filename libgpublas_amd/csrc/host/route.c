@@ -105,6 +105,7 @@ static int gpu_wins(int kind, double flops, double bytes)
 int gb_host_route(int kind, int below_fixed, double flops, double bytes, const void *p,
                   const void *q, const void *r)
 {
+    if (gb_forked_child) return 1; /* no GPU runtime in a fork child */
     if (gb_route_gpu) return 0;
     const int cut = gb_opt.cutoff;
     if (cut == 0) return 0;

@@ -34,9 +34,15 @@
 #include <string.h>
 #include <ucontext.h>
 #include <unistd.h>
+#include <dlfcn.h>
+#include <fcntl.h>
+#include <linux/futex.h>
+#include <poll.h>
+#include <time.h>
 #include <sys/mman.h>
 #include <sys/syscall.h>
 
+extern void *__libc_malloc(size_t);
 extern void *__libc_calloc(size_t, size_t);
 extern void __libc_free(void *);
 
@@ -101,7 +107,8 @@ static struct gb_obj *tab_remove(const void *p)
     return o;
 }
 
-struct gb_obj *gb_obj_find(const void *p)
+/* object whose mapping contains p, detached or not */
+static struct gb_obj *tab_find(const void *p)
 {
     if (!__atomic_load_n(&g_tab_n, __ATOMIC_RELAXED) || !p) return NULL;
     struct gb_obj *o = NULL;
@@ -113,6 +120,18 @@ struct gb_obj *gb_obj_find(const void *p)
     }
     pthread_rwlock_unlock(&g_tab_lock);
     return o;
+}
+
+struct gb_obj *gb_obj_find(const void *p)
+{
+    struct gb_obj *o = tab_find(p);
+    return o && !o->detached ? o : NULL;
+}
+
+int gb_obj_owns(const void *p)
+{
+    struct gb_obj *o = tab_find(p);
+    return o && o->host == (const char *)p;
 }
 
 int gb_tracked_count(void) { return (int)g_tab_n; }
@@ -128,7 +147,7 @@ bool gb_ptr_tracked(const void *p) { return gb_obj_find(p) != NULL; }
 
 size_t gb_obj_usable_size(const void *p)
 {
-    struct gb_obj *o = gb_obj_find(p);
+    struct gb_obj *o = tab_find(p);
     if (!o || o->host != (const char *)p) return 0;
     return o->size;
 }
@@ -179,6 +198,7 @@ void *gb_obj_create(size_t size, uint64_t nth, void *callsite, int zero)
 
 size_t gb_resident_bytes = 0;
 static uint64_t g_tick = 0;
+static void fault_service_start(void);
 
 static void wait_use(void *stream, uint64_t seq)
 {
@@ -281,9 +301,10 @@ static void obj_free(struct gb_obj *o)
 {
     gb_trace_alloc('U', o->nth, o->callsite, o->size, o->host);
     if (o->used && gb_opt.objtrack_out[0]) gb_objtrack_note(o->nth, o->callsite, o->size);
-    gb_obj_quiesce(o); /* a kernel launched by a call that returned early may still use it */
+    if (!o->detached)
+        gb_obj_quiesce(o); /* a kernel launched by a call that returned early may still use it */
     munmap(o->host, o->map_size);
-    munmap(o->alias, o->map_size);
+    if (o->alias) munmap(o->alias, o->map_size);
     if (o->dev) {
         gpublas_dev_release(o->dev);
         __atomic_fetch_sub(&gb_resident_bytes, o->map_size, __ATOMIC_RELAXED);
@@ -305,8 +326,8 @@ int gb_obj_destroy(void *p)
 
 static void obj_pull_to_host(struct gb_obj *o)
 {
-    /* caller holds o->mu */
-    if (o->state == ST_DEVICE) {
+    /* caller holds o->mu; an object without a mirror has no newer copy to fetch */
+    if (o->state == ST_DEVICE && o->dev) {
         gb_obj_quiesce(o);
         void *s = gpublas_dev_stream();
         gb_check(gpublas_dev_d2h(o->alias, o->dev, o->size, s), "download of a tracked object");
@@ -318,10 +339,20 @@ static void obj_pull_to_host(struct gb_obj *o)
 void *gb_obj_realloc(void *p, size_t size)
 {
     gb_enter();
-    struct gb_obj *o = gb_obj_find(p);
+    struct gb_obj *o = tab_find(p);
     if (!o || o->host != (char *)p) {
         gb_leave();
         return NULL;
+    }
+    if (o->detached) {
+        /* a fork child's copy: becomes ordinary heap memory */
+        void *np = __libc_malloc(size);
+        if (np) {
+            memcpy(np, o->host, size < o->size ? size : o->size);
+            gb_obj_destroy(p);
+        }
+        gb_leave();
+        return np;
     }
     void *np = gb_obj_create(size, o->nth, o->callsite, 0);
     if (np) {
@@ -344,6 +375,7 @@ int gb_obj_to_device(struct gb_obj *o, void *stream, int write)
     o->refs++;
     o->last_use = __atomic_add_fetch(&g_tick, 1, __ATOMIC_RELAXED);
     if (!o->dev) {
+        fault_service_start(); /* host touches of device-newer objects are served there */
         o->dev = gpublas_dev_alloc(o->map_size);
         if (!o->dev) gb_fatal("HBM exhausted allocating a %zu-byte tracked object", o->map_size);
         __atomic_fetch_add(&gb_resident_bytes, o->map_size, __ATOMIC_RELAXED);
@@ -375,12 +407,11 @@ void gb_obj_device_written(struct gb_obj *o)
     pthread_mutex_unlock(&o->mu);
 }
 
-static int handle_fault(void *addr, int write)
+/* Caller holds o->mu: make the application view usable for a host access
+ * (download a DEVICE object first).  A read leaves the object SHARED (view
+ * read-only), a write HOST (view read-write; the next BLAS call uploads). */
+static void host_open_locked(struct gb_obj *o, int write)
 {
-    struct gb_obj *o = gb_obj_find(addr);
-    if (!o) return 0;
-    gb_enter();
-    pthread_mutex_lock(&o->mu);
     if (o->state == ST_DEVICE) {
         obj_pull_to_host(o);
         if (write) {
@@ -390,56 +421,393 @@ static int handle_fault(void *addr, int write)
             mprotect(o->host, o->map_size, PROT_READ);
             o->state = ST_SHARED;
         }
-        GB_COUNT(faults, 1);
-    } else if (o->state == ST_SHARED) {
+    } else if (o->state == ST_SHARED && write) {
         mprotect(o->host, o->map_size, PROT_READ | PROT_WRITE);
         o->state = ST_HOST;
-        GB_COUNT(faults, 1);
     }
-    /* ST_HOST: another thread already opened the view; retry the access */
+}
+
+static int handle_fault(void *addr, int write)
+{
+    struct gb_obj *o = gb_obj_find(addr);
+    if (!o) return 0;
+    gb_enter();
+    pthread_mutex_lock(&o->mu);
+    /* ST_HOST (or SHARED on a read): another thread already opened the view;
+     * the access is retried */
+    if (o->state == ST_DEVICE || (o->state == ST_SHARED && write)) GB_COUNT(faults, 1);
+    host_open_locked(o, write);
     pthread_mutex_unlock(&o->mu);
     gb_leave();
     return 1;
 }
 
-static struct sigaction g_old_segv;
+void gb_host_access(const void *p, size_t n, int write)
+{
+    if (!n || !p || gb_guard || !__atomic_load_n(&g_tab_n, __ATOMIC_RELAXED)) return;
+    const char *lo = p, *hi = lo + n < lo ? (const char *)UINTPTR_MAX : lo + n;
+    gb_enter();
+    /* objects overlapping [lo, hi), a batch at a time: the table lock is not
+     * held across downloads (the runtime may allocate meanwhile) */
+    for (;;) {
+        struct gb_obj *batch[16];
+        int nb = 0;
+        pthread_rwlock_rdlock(&g_tab_lock);
+        size_t k = tab_upper(lo);
+        if (k > 0 && g_tab[k - 1]->host + g_tab[k - 1]->map_size > lo) --k;
+        for (; k < g_tab_n && g_tab[k]->host < hi && nb < 16; ++k)
+            if (!g_tab[k]->detached) batch[nb++] = g_tab[k];
+        const char *next = k < g_tab_n ? g_tab[k]->host : hi;
+        pthread_rwlock_unlock(&g_tab_lock);
+        for (int i = 0; i < nb; ++i) {
+            pthread_mutex_lock(&batch[i]->mu);
+            if (batch[i]->state == ST_DEVICE || (batch[i]->state == ST_SHARED && write))
+                GB_COUNT(io_syncs, 1);
+            host_open_locked(batch[i], write);
+            pthread_mutex_unlock(&batch[i]->mu);
+        }
+        if (nb < 16 || next >= hi) break;
+        lo = next;
+    }
+    gb_leave();
+}
+
+/* ---- fault service -------------------------------------------------------
+ * The SIGSEGV handler does not download in signal context: it hands the
+ * fault to a helper thread (a request on its own stack, pushed on a
+ * lock-free list) and sleeps on a futex until the helper has opened the
+ * view.  The GPU runtime, locks and allocation then run on an ordinary
+ * thread.  Before the helper exists (no object has reached the device yet)
+ * and on the helper itself the fault is handled in place. */
+struct fault_req {
+    void *addr;
+    int write;
+    int done;     /* futex word: 1 once handled */
+    int handled;
+    struct fault_req *next;
+};
+static struct fault_req *g_fault_head = NULL;
+static int g_fault_bell = 0;       /* futex word: bumped per request */
+static pthread_t g_fault_thread;
+static int g_fault_up = 0;         /* 1 while the helper runs (0 again in a fork child) */
+static long g_fault_pid = 0;       /* the process the helper runs in */
+static pthread_mutex_t g_fault_start_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static long futex_op(int *w, int op, int val)
+{
+    return syscall(SYS_futex, w, op, val, NULL, NULL, 0);
+}
+
+static void *fault_service(void *arg)
+{
+    (void)arg;
+    sigset_t all;
+    sigfillset(&all);
+    pthread_sigmask(SIG_BLOCK, &all, NULL);
+    gb_enter(); /* nothing this thread allocates is tracked */
+    for (;;) {
+        const int bell = __atomic_load_n(&g_fault_bell, __ATOMIC_ACQUIRE);
+        struct fault_req *r = __atomic_exchange_n(&g_fault_head, NULL, __ATOMIC_ACQ_REL);
+        if (!r) {
+            futex_op(&g_fault_bell, FUTEX_WAIT_PRIVATE, bell);
+            continue;
+        }
+        while (r) {
+            struct fault_req *next = r->next; /* r lives on the requester's stack */
+            r->handled = handle_fault(r->addr, r->write);
+            __atomic_store_n(&r->done, 1, __ATOMIC_RELEASE);
+            futex_op(&r->done, FUTEX_WAKE_PRIVATE, 1);
+            r = next;
+        }
+    }
+    return NULL;
+}
+
+static void fault_service_start(void)
+{
+    if (__atomic_load_n(&g_fault_up, __ATOMIC_ACQUIRE)) return;
+    pthread_mutex_lock(&g_fault_start_mu);
+    if (!g_fault_up) {
+        pthread_attr_t at;
+        pthread_attr_init(&at);
+        pthread_attr_setstacksize(&at, 256 << 10);
+        g_fault_pid = syscall(SYS_getpid);
+        if (pthread_create(&g_fault_thread, &at, fault_service, NULL) == 0)
+            __atomic_store_n(&g_fault_up, 1, __ATOMIC_RELEASE);
+        pthread_attr_destroy(&at);
+    }
+    pthread_mutex_unlock(&g_fault_start_mu);
+}
+
+/* signal context: async-signal-safe operations only (atomics, futex) */
+static int request_fault(void *addr, int write)
+{
+    /* no helper yet, this is the helper, or this is a fork child whose fault
+     * comes before the library's child handler ran (the helper is the
+     * parent's): handle it here */
+    if (!__atomic_load_n(&g_fault_up, __ATOMIC_ACQUIRE) ||
+        syscall(SYS_getpid) != g_fault_pid || pthread_equal(pthread_self(), g_fault_thread))
+        return handle_fault(addr, write);
+    struct fault_req r = {addr, write, 0, 0, NULL};
+    struct fault_req *h = __atomic_load_n(&g_fault_head, __ATOMIC_RELAXED);
+    do {
+        r.next = h;
+    } while (!__atomic_compare_exchange_n(&g_fault_head, &h, &r, 1, __ATOMIC_RELEASE,
+                                          __ATOMIC_RELAXED));
+    __atomic_add_fetch(&g_fault_bell, 1, __ATOMIC_RELEASE);
+    futex_op(&g_fault_bell, FUTEX_WAKE_PRIVATE, 1);
+    while (!__atomic_load_n(&r.done, __ATOMIC_ACQUIRE)) futex_op(&r.done, FUTEX_WAIT_PRIVATE, 0);
+    return r.handled;
+}
+
+/* ---- the SIGSEGV handler and the application's own ---------------------- */
+static struct sigaction g_app_segv; /* what the application installed (or had before us) */
 static int g_handler_installed = 0;
+int gb_forked_child = 0;
+
+/* libc's own definitions, whatever interposes them (the preload library's
+ * sigaction, read, ...; RTLD_NEXT would miss libc when this library is
+ * itself loaded with dlopen) */
+void *gb_libc_sym(const char *name)
+{
+    static void *libc = NULL;
+    if (!libc) {
+        gb_enter();
+        libc = dlopen("libc.so.6", RTLD_LAZY | RTLD_NOLOAD);
+        gb_leave();
+        if (!libc) return NULL;
+    }
+    return dlsym(libc, name);
+}
+
+extern int __sigaction(int, const struct sigaction *, struct sigaction *);
+
+int gb_real_sigaction(int sig, const struct sigaction *act, struct sigaction *old)
+{
+    return __sigaction(sig, act, old);
+}
 
 static void on_segv(int sig, siginfo_t *si, void *ucv)
 {
+    const int saved_errno = errno;
     if (si && si->si_code == SEGV_ACCERR) {
         int write = 1;
 #if defined(__x86_64__)
         const ucontext_t *uc = (const ucontext_t *)ucv;
         write = (uc->uc_mcontext.gregs[REG_ERR] & 2) != 0;
 #endif
-        if (handle_fault(si->si_addr, write)) return;
-    }
-    if (g_old_segv.sa_flags & SA_SIGINFO) {
-        if (g_old_segv.sa_sigaction) {
-            g_old_segv.sa_sigaction(sig, si, ucv);
+        if (request_fault(si->si_addr, write)) {
+            errno = saved_errno;
             return;
         }
-    } else if (g_old_segv.sa_handler != SIG_DFL && g_old_segv.sa_handler != SIG_IGN) {
-        g_old_segv.sa_handler(sig);
+    }
+    errno = saved_errno;
+    /* not a tracked object: the application's handler, if any */
+    const struct sigaction app = g_app_segv;
+    if (app.sa_flags & SA_SIGINFO) {
+        if (app.sa_sigaction) {
+            app.sa_sigaction(sig, si, ucv);
+            return;
+        }
+    } else if (app.sa_handler != SIG_DFL && app.sa_handler != SIG_IGN) {
+        app.sa_handler(sig);
         return;
     }
     /* default action: re-executing the access faults again with SIG_DFL */
     struct sigaction dfl;
     memset(&dfl, 0, sizeof dfl);
     dfl.sa_handler = SIG_DFL;
-    sigaction(SIGSEGV, &dfl, NULL);
+    gb_real_sigaction(SIGSEGV, &dfl, NULL);
 }
+
+static int install_ours(struct sigaction *prev)
+{
+    struct sigaction sa;
+    memset(&sa, 0, sizeof sa);
+    sa.sa_sigaction = on_segv;
+    sa.sa_flags = SA_SIGINFO | SA_NODEFER | SA_RESTART | SA_ONSTACK;
+    sigemptyset(&sa.sa_mask);
+    return gb_real_sigaction(SIGSEGV, &sa, prev);
+}
+
+static void fork_prepare(void);
+static void fork_parent(void);
+static void fork_child(void);
 
 void gb_install_fault_handler(void)
 {
     if (g_handler_installed) return;
-    struct sigaction sa;
-    memset(&sa, 0, sizeof sa);
-    sa.sa_sigaction = on_segv;
-    sa.sa_flags = SA_SIGINFO | SA_NODEFER | SA_RESTART;
-    sigemptyset(&sa.sa_mask);
-    if (sigaction(SIGSEGV, &sa, &g_old_segv) == 0) g_handler_installed = 1;
+    if (install_ours(&g_app_segv) == 0) g_handler_installed = 1;
+    pthread_atfork(fork_prepare, fork_parent, fork_child);
+}
+
+int gb_segv_app_action(const struct sigaction *act, struct sigaction *old)
+{
+    if (!__atomic_load_n(&g_handler_installed, __ATOMIC_ACQUIRE)) return 0;
+    struct sigaction prev = g_app_segv;
+    if (act) g_app_segv = *act;
+    if (old) *old = prev;
+    return 1;
+}
+
+void gb_check_fault_handler(void)
+{
+    static uint64_t last_ns = 0;
+    if (!g_handler_installed) return;
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC_COARSE, &t);
+    const uint64_t now = (uint64_t)t.tv_sec * 1000000000ull + (uint64_t)t.tv_nsec;
+    if (now - __atomic_load_n(&last_ns, __ATOMIC_RELAXED) < 1000000) return;
+    __atomic_store_n(&last_ns, now, __ATOMIC_RELAXED);
+    struct sigaction cur;
+    if (gb_real_sigaction(SIGSEGV, NULL, &cur) != 0) return;
+    if ((cur.sa_flags & SA_SIGINFO) && cur.sa_sigaction == on_segv) return;
+    /* replaced behind the library's back (a raw syscall, another library's
+     * static sigaction): chain to the newcomer and take the signal back */
+    g_app_segv = cur;
+    install_ours(NULL);
+    GB_COUNT(handler_reinstalls, 1);
+}
+
+/* ---- fork -----------------------------------------------------------------
+ * A child must not share the parent's tracked pages (a memfd mapped shared
+ * would make the child's writes visible to the parent and the reverse).
+ * Device-newer objects are downloaded before the fork (by the preload
+ * library's fork(), else in the prepare handler).  The child copies every
+ * object into private memory at the same address and detaches it: it is then
+ * ordinary memory that free() releases, and every BLAS call of the child
+ * runs on the host BLAS (the GPU runtime does not survive fork).  The
+ * forking thread waits in the parent until the child has copied (bounded:
+ * a child that dies first closes the pipe, one that stalls is given up on
+ * with a warning), so the parent's own writes after fork() returns never
+ * reach the child.  No lock of the library is held across the fork: other
+ * libraries' fork handlers run in between, and threads they depend on may
+ * be inside the library's interposers. */
+static int g_fork_pipe[2] = {-1, -1};
+
+void gb_fork_sync_device(void)
+{
+    if (!__atomic_load_n(&g_tab_n, __ATOMIC_RELAXED) || gb_forked_child) return;
+    gb_enter();
+    for (size_t k = 0;; ++k) {
+        pthread_rwlock_rdlock(&g_tab_lock);
+        struct gb_obj *o = k < g_tab_n ? g_tab[k] : NULL;
+        pthread_rwlock_unlock(&g_tab_lock);
+        if (!o) break;
+        pthread_mutex_lock(&o->mu);
+        if (o->state == ST_DEVICE && o->dev) host_open_locked(o, 0);
+        pthread_mutex_unlock(&o->mu);
+    }
+    gb_leave();
+}
+
+static void fork_prepare(void)
+{
+    if (!__atomic_load_n(&g_tab_n, __ATOMIC_RELAXED) || gb_forked_child) return;
+    gb_fork_sync_device();
+    if (syscall(SYS_pipe2, g_fork_pipe, O_CLOEXEC) != 0) g_fork_pipe[0] = g_fork_pipe[1] = -1;
+}
+
+static void fork_parent(void)
+{
+    if (g_fork_pipe[0] < 0) return;
+    syscall(SYS_close, g_fork_pipe[1]);
+    /* the child writes one byte once it has copied (EOF if it died first);
+     * allow 10 s plus 1 s per GB of tracked memory */
+    const long budget_ms =
+        10000 + (long)(__atomic_load_n(&gb_stats.tracked_bytes, __ATOMIC_RELAXED) >> 20);
+    struct pollfd pf = {g_fork_pipe[0], POLLIN, 0};
+    long waited = 0;
+    int ready = 0;
+    while (waited < budget_ms) {
+        const int r = (int)syscall(SYS_poll, &pf, 1, 100);
+        if (r > 0) {
+            ready = 1;
+            break;
+        }
+        if (r < 0 && errno != EINTR) break;
+        waited += 100;
+    }
+    if (ready) {
+        char c;
+        (void)syscall(SYS_read, g_fork_pipe[0], &c, 1);
+    } else {
+        gb_log(2, "libgpublas: fork: the child did not finish copying its tracked objects in "
+                  "time; continuing (it may see later writes of this process)\n");
+    }
+    syscall(SYS_close, g_fork_pipe[0]);
+    g_fork_pipe[0] = g_fork_pipe[1] = -1;
+}
+
+static void fork_child(void)
+{
+    gb_forked_child = 1;
+    gb_tracking = 0;
+    gpublas_dev_forked_child();
+    __atomic_store_n(&g_fault_up, 0, __ATOMIC_RELEASE);
+    g_fault_head = NULL;
+    /* locks possibly held by other threads of the parent at the fork */
+    pthread_mutex_init(&g_fault_start_mu, NULL);
+    pthread_rwlock_init(&g_tab_lock, NULL);
+    for (size_t k = 0; k < g_tab_n; ++k) {
+        struct gb_obj *o = g_tab[k];
+        pthread_mutex_init(&o->mu, NULL);
+        if (o->detached) continue;
+        char *copy = mmap(NULL, o->map_size, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS,
+                          -1, 0);
+        if (copy == MAP_FAILED) gb_fatal("fork: no memory to copy a tracked object");
+        memcpy(copy, o->alias, o->map_size);
+        if (mremap(copy, o->map_size, o->map_size, MREMAP_MAYMOVE | MREMAP_FIXED, o->host) ==
+            MAP_FAILED)
+            gb_fatal("fork: cannot replace a tracked mapping");
+        munmap(o->alias, o->map_size);
+        o->alias = NULL;
+        o->dev = NULL; /* the parent's HBM, not this process's */
+        o->wr_stream = NULL;
+        o->nrd = 0;
+        o->refs = 0;
+        o->state = ST_HOST;
+        o->detached = 1;
+    }
+    if (g_fork_pipe[1] >= 0) {
+        const char c = 1;
+        syscall(SYS_write, g_fork_pipe[1], &c, 1);
+        syscall(SYS_close, g_fork_pipe[1]);
+        syscall(SYS_close, g_fork_pipe[0]);
+        g_fork_pipe[0] = g_fork_pipe[1] = -1;
+    }
+}
+
+/* ---- test hook ---- */
+int gpublas_debug_set_residency(void *p, int state)
+{
+    struct gb_obj *o = gb_obj_find(p);
+    if (!o || o->host != (char *)p || state < 0 || state > 2) return -1;
+    int rc = 0;
+    gb_enter();
+    fault_service_start();
+    pthread_mutex_lock(&o->mu);
+    if (o->dev) {
+        rc = -1; /* only for objects that never reached the device */
+    } else {
+        static const int prot[3] = {PROT_READ | PROT_WRITE, PROT_READ, PROT_NONE};
+        static const int st[3] = {ST_HOST, ST_SHARED, ST_DEVICE};
+        mprotect(o->host, o->map_size, prot[state]);
+        o->state = st[state];
+    }
+    pthread_mutex_unlock(&o->mu);
+    gb_leave();
+    return rc;
+}
+
+int gpublas_debug_residency(const void *p)
+{
+    struct gb_obj *o = gb_obj_find(p);
+    if (!o) return -1;
+    pthread_mutex_lock(&o->mu);
+    const int st = o->state == ST_HOST ? 0 : o->state == ST_SHARED ? 1 : 2;
+    pthread_mutex_unlock(&o->mu);
+    return st;
 }
 
 /* ---- operand residency ---- */

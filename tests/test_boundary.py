@@ -36,6 +36,19 @@ def _exports(path):
     return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
 
 
+def test_stats_struct_mirror_matches_header():
+    """The Python mirror of gpublas_stats_t has the header's fields in order
+    (gpublas_get_stats fills the whole C struct: a shorter mirror is a heap
+    overrun in the caller)."""
+    src = open(os.path.join(ROOT, "include", "gpublas.h")).read()
+    body = re.search(r"typedef struct \{(.*?)\} gpublas_stats_t;", src, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in re.findall(r"uint64_t\s+([^;]+);", body):
+        fields += [f.strip() for f in decl.split(",")]
+    assert [n for n, _ in gb.Stats._fields_] == fields
+
+
 @pytest.mark.parametrize("header", ["gpublas.h", "gpublas_dev.h"])
 def test_library_exports_every_declared_symbol(header):
     gb.library()
