@@ -238,6 +238,218 @@ __global__ __launch_bounds__(256) void trsm_leaf_big(int nb, int n, T alpha,
         }
 }
 
+// FP64 base solve on the matrix cores: op(T) X = alpha B for an order of at
+// most 256, one 512-thread workgroup per 16 right-hand sides, the 16-column
+// panel of B held in LDS, 16-row blocks.
+//  * Prologue: the inverses of the 16 diagonal blocks are formed in LDS,
+//    four blocks per wave, one lane per column (a substitution with the unit
+//    vector; reciprocal of each diagonal entry, computed off the chain),
+//    while the panel's loads are still in flight.
+//  * Wave w keeps its blocks w and w + 8 in registers in accumulator layout.
+//    Step s: X_I (I the s-th block in solve order) is in the panel; every
+//    wave subtracts op(T)[J, I] X_I from its blocks J beyond I (two
+//    independent chains of two v_mfma_f64_16x16x4), the owner of the next
+//    block first, and that owner then forms X = inv(T_JJ) B_J with four
+//    independent MFMAs that take the update's accumulator registers directly
+//    as their B operand (the accumulator layout of one is the operand layout
+//    of the other) and stores X into the panel; one LDS-only barrier per step.
+//    The op(T) fragments are fetched two steps ahead (unrolled case).
+// The sequential part per block is a few MFMAs and a barrier instead of a
+// 16-step substitution (multiplying by the inverse of a 16 x 16 diagonal
+// block is the usual GPU TRSM; exact for integer data, and within the
+// usual triangular-solve error bound for well-conditioned diagonal blocks).
+// Measured at order 256 with 4096 right-hand sides: 26 us per launch against
+// 68 us for the substitution kernel (trsm_leaf_big, still the FP32 base).
+constexpr int MB_MAX = 256;   // orders up to 256: 16 blocks
+constexpr int MB_WAVES = 8;   // two blocks per wave
+// FULL = 16: the order is 241..256 and the steps are unrolled (with no loop
+// back edge the compiler waits for each prefetch only where it is used);
+// FULL = 0: a loop over the steps for other orders.
+template <bool LOWER, bool NOUNIT, int FULL>
+__global__ __launch_bounds__(512) void trsm_base_mfma(int nbr, int n, double alpha,
+                                                      const double *__restrict__ A, long lda,
+                                                      int transN, double *__restrict__ B,
+                                                      long sstride, long rstride)
+{
+    __shared__ double Bs[MB_MAX][17];      // Bs[row][rhs]
+    __shared__ double Ls[16][16][17];      // diagonal blocks, then their inverses
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q0 = lane >> 4, cl = lane & 15;
+    const int c0 = blockIdx.x * 16;
+    const int nb = FULL ? FULL : (nbr + 15) / 16;
+    const int rows = nb * 16;
+    // Prologue.  The diagonal blocks' loads are issued first and the panel's
+    // after them, so the inversion waits only for the former (loads complete
+    // in order) and runs while the panel is still in flight.
+    constexpr int PER = MB_MAX * 16 / 512;
+    double v[PER], d[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = threadIdx.x + 512 * e;
+        const int bk = idx >> 8, i = (idx >> 4) & 15, k = idx & 15;
+        const int gi = 16 * bk + i, gk = 16 * bk + k;
+        const bool tri = bk < nb && gi < nbr && gk < nbr && (LOWER ? k < i : k > i);
+        d[e] = A[tri ? (transN ? gi + (long)gk * lda : gk + (long)gi * lda)
+                     : (NOUNIT && bk < nb && gi < nbr && i == k ? gi + (long)gi * lda : 0)];
+    }
+    // alpha B: contiguous index fastest (rows on the left side, right-hand
+    // sides on the right)
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = threadIdx.x + 512 * e;
+        const int r = sstride == 1 ? idx % rows : idx / 16;
+        const int c = sstride == 1 ? idx / rows : idx % 16;
+        const bool in = idx < rows * 16 && r < nbr && c0 + c < n;
+        v[e] = B[in ? r * sstride + (long)(c0 + c) * rstride : 0];
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = threadIdx.x + 512 * e;
+        const int bk = idx >> 8, i = (idx >> 4) & 15, k = idx & 15;
+        const int gi = 16 * bk + i, gk = 16 * bk + k;
+        const bool tri = bk < nb && gi < nbr && gk < nbr && (LOWER ? k < i : k > i);
+        // identity past the order, unit diagonal when asked
+        Ls[bk][i][k] = tri ? d[e] : i == k ? ((NOUNIT && bk < nb && gi < nbr) ? d[e] : 1.0) : 0.0;
+    }
+    lds_barrier();  // the diagonal blocks are in LDS (the panel's loads still in flight)
+    if (wave < 4) {
+        // inverse of diagonal block 4 * wave + q0, column cl of it in this lane
+        // (the reciprocals of the diagonal are off the substitution's chain)
+        const int bk = 4 * wave + q0;
+        if (bk < nb) {
+            double y[16];
+#pragma unroll
+            for (int ii = 0; ii < 16; ++ii) {
+                const int i = LOWER ? ii : 15 - ii;
+                const double rc = 1.0 / Ls[bk][i][i];  // independent of the chain
+                double sacc = i == cl ? 1.0 : 0.0;
+#pragma unroll
+                for (int kk = 0; kk < ii; ++kk) {
+                    const int k = LOWER ? kk : 15 - kk;
+                    sacc = fma(-Ls[bk][i][k], y[k], sacc);
+                }
+                y[i] = sacc * rc;
+            }
+            // every lane's reads of the block are done before it is overwritten
+            __builtin_amdgcn_s_waitcnt(vm_wait(63) & ~(15 << 8));
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Ls[bk][i][cl] = y[i];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int idx = threadIdx.x + 512 * e;
+        const int r = sstride == 1 ? idx % rows : idx / 16;
+        const int c = sstride == 1 ? idx / rows : idx % 16;
+        const bool in = idx < rows * 16 && r < nbr && c0 + c < n;
+        if (idx < rows * 16) Bs[r][c] = in ? alpha * v[e] : 0.0;
+    }
+    // op(T)[J, I] in MFMA A-operand layout for the wave's blocks J = wave + 8 s:
+    // frag[s][u] = op(T)(16 J + cl, 16 I + 4 u + q0) where it lies in the block's
+    // part of the triangle (in_tri).  Indices are clamped into the stored
+    // nbr x nbr square, so every load has a valid address and no condition (a
+    // branch here makes the compiler wait for the loads where it joins); the
+    // mask is applied where the value is used.
+    auto in_tri = [&](int I, int sb, int u) {
+        const int J = wave + MB_WAVES * sb;
+        const int i = 16 * J + cl, k = 16 * I + 4 * u + q0;
+        return J < nb && (LOWER ? J > I : J < I) && i < nbr && k < nbr;
+    };
+    auto fetch = [&](int I, double (&frag)[2][4]) {
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = min(16 * (wave + MB_WAVES * sb) + cl, nbr - 1);
+                const int k = min(16 * I + 4 * u + q0, nbr - 1);
+                frag[sb][u] = A[transN ? i + (long)k * lda : k + (long)i * lda];
+            }
+    };
+    // The wave's two blocks stay in registers in accumulator layout (bacc[s]:
+    // rows 16 J + q0 + 4 r, column cl); only solved blocks go to the panel.
+    v4d bacc[2];
+    // X_J = inv(T_JJ) B_J into the panel
+    auto solve = [&](int sb) {
+        const int J = wave + MB_WAVES * sb;
+        // four independent k-chunk products (no accumulator chain), then summed
+        const v4d z = {0.0, 0.0, 0.0, 0.0};
+        v4d p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            p[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[J][cl][4 * u + q0], bacc[sb][u], z, 0,
+                                                        0, 0);
+        const v4d x = (p[0] + p[1]) + (p[2] + p[3]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Bs[16 * J + q0 + 4 * r][cl] = x[r];
+    };
+    // step st: X_I is in the panel (I = the st-th block in solve order);
+    // updates with it, Inext solved.  `ahead` (fragments of the step `dist`
+    // steps later) is fetched now, a step's duration being shorter than the
+    // loads' latency.
+    auto step = [&](int st, double (&cur)[2][4], double (&ahead)[2][4], int dist) {
+        const int I = LOWER ? st : nb - 1 - st;
+        const int Inext = LOWER ? I + 1 : I - 1;
+        if (st + dist < nb - 1) fetch(LOWER ? st + dist : nb - 1 - (st + dist), ahead);
+        double xo[4];  // X_I in B-operand layout: row 16 I + 4 u + q0, column cl
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xo[u] = Bs[16 * I + 4 * u + q0][cl];
+        const int first = (wave + MB_WAVES == Inext) ? 1 : 0;  // the next block first
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int sb = q ? 1 - first : first;
+            const int J = wave + MB_WAVES * sb;
+            if (J < nb && (LOWER ? J > I : J < I)) {
+                // B_J -= op(T)[J, I] X_I: two independent chains of two k chunks
+                const v4d z = {0.0, 0.0, 0.0, 0.0};
+                v4d p0 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    in_tri(I, sb, 0) ? -cur[sb][0] : 0.0, xo[0], bacc[sb], 0, 0, 0);
+                v4d p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    in_tri(I, sb, 2) ? -cur[sb][2] : 0.0, xo[2], z, 0, 0, 0);
+                p0 = __builtin_amdgcn_mfma_f64_16x16x4f64(in_tri(I, sb, 1) ? -cur[sb][1] : 0.0,
+                                                         xo[1], p0, 0, 0, 0);
+                p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(in_tri(I, sb, 3) ? -cur[sb][3] : 0.0,
+                                                         xo[3], p1, 0, 0, 0);
+                bacc[sb] = p0 + p1;
+                if (J == Inext) solve(sb);
+            }
+        }
+        lds_barrier();  // X_Inext in the panel
+    };
+    // fragment buffers: a ring of three (two steps ahead) when unrolled, two otherwise
+    double fr[3][2][4];
+    fetch(LOWER ? 0 : nb - 1, fr[0]);
+    if constexpr (FULL > 0) fetch(LOWER ? 1 : nb - 2, fr[1]);
+    __syncthreads();  // the inverses are in LDS
+#pragma unroll
+    for (int sb = 0; sb < 2; ++sb) {
+        const int J = min(wave + MB_WAVES * sb, 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bacc[sb][r] = Bs[16 * J + q0 + 4 * r][cl];
+    }
+    {
+        const int I = LOWER ? 0 : nb - 1;
+        if (wave == I % MB_WAVES) solve(I / MB_WAVES);
+        lds_barrier();
+    }
+    if constexpr (FULL > 0) {
+#pragma unroll
+        for (int st = 0; st < FULL - 1; ++st) step(st, fr[st % 3], fr[(st + 2) % 3], 2);
+    } else {
+        for (int st = 0; st < nb - 1; st += 2) {
+            step(st, fr[0], fr[1], 1);
+            if (st + 1 < nb - 1) step(st + 1, fr[1], fr[0], 1);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < rows * 16; idx += 512) {
+        const int r = sstride == 1 ? idx % rows : idx / 16;
+        const int c = sstride == 1 ? idx / rows : idx % 16;
+        if (r < nbr && c0 + c < n) B[r * sstride + (long)(c0 + c) * rstride] = Bs[r][c];
+    }
+}
+
 template <typename T>
 __global__ void zero_kernel(int m, int n, T *B, long ldb)
 {
@@ -303,10 +515,36 @@ struct Solve {
 
     // Solve on the sub-problem whose triangle starts at a (order `t`), with
     // right-hand sides at b (m x n, m = t for left, n = t for right).
+    // FP64 orders up to MB_MAX in one launch on the matrix cores (trsm_base_mfma)
+    int base_mfma(int m, int n, T alpha, const T *a, T *b)
+    {
+        const bool lower = left ? effLower : !effLower;
+        const int tn = left ? transN : !transN;
+        const int nrhs = left ? n : m, order = left ? m : n;
+        const long ss = left ? 1 : ldb, rs = left ? ldb : 1;
+        const unsigned blocks = (unsigned)ceil_div(nrhs, 16);
+        const bool full = order > MB_MAX - 16;
+        auto k = full ? (lower ? (nounit ? trsm_base_mfma<true, true, 16>
+                                         : trsm_base_mfma<true, false, 16>)
+                               : (nounit ? trsm_base_mfma<false, true, 16>
+                                         : trsm_base_mfma<false, false, 16>))
+                      : (lower ? (nounit ? trsm_base_mfma<true, true, 0>
+                                         : trsm_base_mfma<true, false, 0>)
+                               : (nounit ? trsm_base_mfma<false, true, 0>
+                                         : trsm_base_mfma<false, false, 0>));
+        PROF_BEGIN(s, "dtrsm_base");
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(512), 0, s, order, nrhs, (double)alpha,
+                           (const double *)a, (long)lda, tn, (double *)b, ss, rs);
+        PROF_END(s);
+        return (int)hipGetLastError();
+    }
+
     int rec(int m, int n, T alpha, const T *a, T *b)
     {
         int err;
         const int t = left ? m : n;
+        if constexpr (sizeof(T) == 8)
+            if (t <= MB_MAX) return base_mfma(m, n, alpha, a, b);
         if (t <= NB) return leaf(m, n, alpha, a, b);
         if (t <= NB * BIG_R) return leaf_big<TRSM_LPWB>(m, n, alpha, a, b);
         const int t1 = split(t), t2 = t - t1;
