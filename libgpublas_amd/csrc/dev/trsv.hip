@@ -410,19 +410,22 @@ __device__ __forceinline__ void panel_apply(const PanelRegs &pr, const double *x
     }
 }
 
-// thread 0 waits for block j's flag (acquire); false if the wait expired
+// thread 0 waits for block j's flag; false if the wait expired.  The polls
+// are relaxed loads (an acquire per poll would invalidate the cache for every
+// other workgroup streaming its panel); one acquire fence follows.
 __device__ __forceinline__ bool wait_block(const TrsvLb &g, int j, int *ok)
 {
     if (threadIdx.x == 0) {
         const long long t0 = wall_clock64();
         int good = 1;
-        while (__hip_atomic_load(g.flags + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        while (__hip_atomic_load(g.flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
             if (wall_clock64() - t0 > g.timeout) {
                 good = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *ok = good;
     }
     __syncthreads();
