@@ -16,7 +16,9 @@ gives the chain with those launched as kernels).  The roofline object is
 for the dominant kernel (dgemv 'N'): 8*(m*n + m + n) bytes per launch over its
 average duration measured with HIP events on its launch stream.  The line also
 carries the DGEMM figures the composite metric names (1024^3 and 8192^3 NN,
-kernel TFLOP/s vs the 78.6 TFLOP/s FP64 peak) and the CPU baselines: the host
+kernel TFLOP/s vs the 78.6 TFLOP/s FP64 peak), the serving trace (20,000 calls
+from 16 threads, closed loop and three Poisson rates, batching loop vs one
+launch per call) and the CPU baselines: the host
 BLAS on all cores (cpu_baseline) and the oracle's C loops on one
 (cpu_baseline_port).
 """
@@ -40,6 +42,8 @@ DGEMV_BYTES = 8 * (N * N + N + N)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 SCIPY_OPENBLAS = "/usr/local/lib/python3.10/dist-packages/scipy.libs/libscipy_openblas-68440149.so"
+SERVE_CALLS = 20000
+SERVE_RATES = (0, 10000, 20000, 30000)  # offered calls/s; 0 = closed loop
 
 
 def log(msg):
@@ -420,20 +424,31 @@ def main():
                for k in prof if k.startswith("k_") and k.endswith("_ms")}
 
     gemm = gemm_figures()
-    # serving trace (closed loop, 16 threads, 8000 calls): batching loop vs
+    # the serving trace itself: 20,000 calls from 16 threads, closed loop and
+    # Poisson arrivals at 10k / 20k / 30k calls/s, the batching loop against
     # one launch per call
     srv = {}
     try:
-        plain = serve_run("plain", 0, 8000)
-        batched = serve_run("serve", 0, 8000)
-        srv = {"workload": "16 threads, closed loop, 8000 calls: 60% D/SGEMM logU[32,2048]^3, "
-                           "30% DGEMV logU[256,16384]^2, 10% DTRSM m logU[64,1024] x 256",
-               "calls_per_s": round(batched["calls_per_s"], 1),
-               "plain_calls_per_s": round(plain["calls_per_s"], 1),
-               "speedup_vs_plain": round(batched["calls_per_s"] / plain["calls_per_s"], 3),
-               "p50_us": batched["p50_us"], "p99_us": batched["p99_us"],
-               "plain_p50_us": plain["p50_us"], "plain_p99_us": plain["p99_us"],
-               "mean_batch": batched["mean_batch"]}
+        rates = []
+        for rate in SERVE_RATES:
+            plain = serve_run("plain", rate, SERVE_CALLS)
+            batched = serve_run("serve", rate, SERVE_CALLS)
+            rates.append({"offered_calls_per_s": rate or "closed loop",
+                          "calls_per_s": round(batched["calls_per_s"], 1),
+                          "plain_calls_per_s": round(plain["calls_per_s"], 1),
+                          "p50_us": batched["p50_us"], "p99_us": batched["p99_us"],
+                          "plain_p50_us": plain["p50_us"], "plain_p99_us": plain["p99_us"],
+                          "mean_batch": batched["mean_batch"]})
+        closed = rates[0]
+        srv = {"workload": f"16 threads, {SERVE_CALLS} calls: 60% D/SGEMM logU[32,2048]^3, "
+                           "30% DGEMV logU[256,16384]^2, 10% DTRSM m logU[64,1024] x 256; "
+                           "tracked operands",
+               "calls_per_s": closed["calls_per_s"],
+               "plain_calls_per_s": closed["plain_calls_per_s"],
+               "speedup_vs_plain": round(closed["calls_per_s"] / closed["plain_calls_per_s"], 3),
+               "p50_us": closed["p50_us"], "p99_us": closed["p99_us"],
+               "plain_p50_us": closed["plain_p50_us"], "plain_p99_us": closed["plain_p99_us"],
+               "mean_batch": closed["mean_batch"], "rates": rates}
     except Exception as e:  # reported, not fatal for the headline line
         srv = {"error": str(e)[:300]}
     line = {
