@@ -36,6 +36,23 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmArgs<T> g)
     gemm_tile<T, BM, BN, BK>(g, tm, tn, As, Bs);
 }
 
+// A rectangular grid with both full and ragged tiles in one launch: full
+// tiles take the fast tile routine (patterns PA / PB), ragged ones its
+// guarded variant.  One launch instead of a fast grid followed by an edge
+// grid, whose few tiles would otherwise make a round of their own.
+template <typename T, int BM, int PA, int PB>
+__global__ __launch_bounds__(256, 2) void gemm_mixed_kernel(GemmArgs<T> g)
+{
+    constexpr int PAD = 128 / (int)sizeof(T);
+    __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
+    const int tm = blockIdx.x % g.tiles_m, tn = blockIdx.x / g.tiles_m;
+    if (tm < g.full_m && tn < g.full_n)
+        gemm_fast_tile<T, BM, BM, 16, PA, PB, SYM_NONE>(g, tm, tn, As, Bs);
+    else
+        gemm_fast_tile<T, BM, BM, 16, PA, PB, SYM_NONE, true>(g, tm, tn, As, Bs);
+}
+
 // Fast kernel: the grid holds only tiles that are full in M and N, both
 // operands are 16-B aligned with leading dimensions a multiple of the vector
 // width, and their access patterns are compile-time constants (PA / PB).
@@ -107,8 +124,8 @@ __global__ __launch_bounds__(256, 2) void gemm_fast_splitk_kernel(GemmArgs<T> g,
     const long koff = (long)kc * kchunk;
     GemmArgs<T> h = g;
     // element (o, k) of an operand is p[o + k*ld] (outer) or p[k + o*ld]
-    h.a.p += PA == PAT_OUTER ? koff * g.a.ld : koff;
-    h.b.p += PB == PAT_OUTER ? koff * g.b.ld : koff;
+    h.a.p += pat_base(PA) == PAT_OUTER ? koff * g.a.ld : koff;
+    h.b.p += pat_base(PB) == PAT_OUTER ? koff * g.b.ld : koff;
     h.k = kchunk;
     h.alpha = T(1);
     h.beta = T(0);
@@ -180,7 +197,13 @@ void launch_fast(hipStream_t s, const GemmArgs<T> &g, unsigned blocks)
     hipLaunchKernelGGL((gemm_fast_kernel<T, BM, BM, 16, PA_, PB_, TRI_>), dim3(blocks),         \
                        dim3(256), 0, s, g)
     const int pa = g.a.form, pb = g.b.form;
-    if (g.tri == SYM_NONE) {
+    const bool ua = !(g.a.vec_ok && g.b.vec_ok);  // element loads (rectangular grids only)
+    if (g.tri == SYM_NONE && ua) {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_FAST(PAT_OUTER_U, PAT_KCONT_U, SYM_NONE);
+        else if (pa == PAT_OUTER) GB_FAST(PAT_OUTER_U, PAT_OUTER_U, SYM_NONE);
+        else if (pb == PAT_KCONT) GB_FAST(PAT_KCONT_U, PAT_KCONT_U, SYM_NONE);
+        else GB_FAST(PAT_KCONT_U, PAT_OUTER_U, SYM_NONE);
+    } else if (g.tri == SYM_NONE) {
         if (pa == PAT_OUTER && pb == PAT_KCONT) GB_FAST(PAT_OUTER, PAT_KCONT, SYM_NONE);      // NN
         else if (pa == PAT_OUTER) GB_FAST(PAT_OUTER, PAT_OUTER, SYM_NONE);                    // NT
         else if (pb == PAT_KCONT) GB_FAST(PAT_KCONT, PAT_KCONT, SYM_NONE);                    // TN
@@ -202,7 +225,13 @@ void launch_fast_sub(hipStream_t s, const GemmArgs<T> &g, int base, unsigned blo
     hipLaunchKernelGGL((gemm_fast_sub_kernel<T, PA_, PB_, TRI_>), dim3(blocks), dim3(256), 0, s, \
                        g, base)
     const int pa = g.a.form, pb = g.b.form;
-    if (g.tri == SYM_NONE) {
+    const bool ua = !(g.a.vec_ok && g.b.vec_ok);
+    if (g.tri == SYM_NONE && ua) {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SUB(PAT_OUTER_U, PAT_KCONT_U, SYM_NONE);
+        else if (pa == PAT_OUTER) GB_SUB(PAT_OUTER_U, PAT_OUTER_U, SYM_NONE);
+        else if (pb == PAT_KCONT) GB_SUB(PAT_KCONT_U, PAT_KCONT_U, SYM_NONE);
+        else GB_SUB(PAT_KCONT_U, PAT_OUTER_U, SYM_NONE);
+    } else if (g.tri == SYM_NONE) {
         if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SUB(PAT_OUTER, PAT_KCONT, SYM_NONE);
         else if (pa == PAT_OUTER) GB_SUB(PAT_OUTER, PAT_OUTER, SYM_NONE);
         else if (pb == PAT_KCONT) GB_SUB(PAT_KCONT, PAT_KCONT, SYM_NONE);
@@ -257,11 +286,19 @@ int launch_splitk(hipStream_t s, const GemmArgs<T> &g, int m, int n, int k)
     hipLaunchKernelGGL((gemm_fast_splitk_kernel<T, BM, PA_, PB_>), dim3(blocks), dim3(256), 0, s, \
                        f, k / ks, P)
     const int pa = g.a.form, pb = g.b.form;
+    const bool ua = !(g.a.vec_ok && g.b.vec_ok);
     PROF_BEGIN(s, sizeof(T) == 8 ? "dgemm_splitk" : "sgemm_splitk");
-    if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SK(PAT_OUTER, PAT_KCONT);
-    else if (pa == PAT_OUTER) GB_SK(PAT_OUTER, PAT_OUTER);
-    else if (pb == PAT_KCONT) GB_SK(PAT_KCONT, PAT_KCONT);
-    else GB_SK(PAT_KCONT, PAT_OUTER);
+    if (ua) {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SK(PAT_OUTER_U, PAT_KCONT_U);
+        else if (pa == PAT_OUTER) GB_SK(PAT_OUTER_U, PAT_OUTER_U);
+        else if (pb == PAT_KCONT) GB_SK(PAT_KCONT_U, PAT_KCONT_U);
+        else GB_SK(PAT_KCONT_U, PAT_OUTER_U);
+    } else {
+        if (pa == PAT_OUTER && pb == PAT_KCONT) GB_SK(PAT_OUTER, PAT_KCONT);
+        else if (pa == PAT_OUTER) GB_SK(PAT_OUTER, PAT_OUTER);
+        else if (pb == PAT_KCONT) GB_SK(PAT_KCONT, PAT_KCONT);
+        else GB_SK(PAT_KCONT, PAT_OUTER);
+    }
 #undef GB_SK
     const long mn = (long)m * n;
     const unsigned fb = (unsigned)std::min<long>((mn + 255) / 256, 4096);
@@ -274,9 +311,10 @@ int launch_splitk(hipStream_t s, const GemmArgs<T> &g, int m, int n, int k)
 
 // Core launcher: C = alpha*op(A)*op(B) + beta*C on an (optionally
 // triangular) tile grid.  k > 0, alpha != 0 assumed.  Tiles that are full in
-// M and N go to the fast kernel when both operands allow it; the generic
-// kernel takes the ragged edge (and everything, for symmetric or unaligned
-// operands).
+// M and N go to the fast kernel (vector loads when both operands are
+// aligned, element loads otherwise; a triangular grid needs aligned
+// operands); the generic kernel takes the ragged edge (and everything, for
+// symmetric operands).
 template <typename T>
 int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a,
                 const Operand<T> &b, T beta, T *C, long ldc, int tri)
@@ -292,7 +330,10 @@ int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a
     const long t128 = (long)ceil_div(m, 128) * ceil_div(n, 128);
     const bool big = t128 >= 256;
     const int bm = big ? 128 : 64;
-    const bool fast_ok = a.sym == SYM_NONE && b.sym == SYM_NONE && a.vec_ok && b.vec_ok;
+    // full tiles take the fast kernels: with vector loads when both operands
+    // are aligned, else (rectangular grids) with element loads
+    const bool fast_ok = a.sym == SYM_NONE && b.sym == SYM_NONE &&
+                         ((a.vec_ok && b.vec_ok) || tri == SYM_NONE);
     if (fast_ok && !big && tri == SYM_NONE) {
         // few 64 x 64 tiles, long k: split k to fill the workgroup slots
         // (measured: 1024^3 32.8 -> 35.1 TFLOP/s, 512 x 512 x 4096 -> 37.4;
@@ -302,6 +343,28 @@ int launch_gemm(hipStream_t s, int m, int n, int k, T alpha, const Operand<T> &a
     }
     const int full_m = fast_ok ? m / bm : 0, full_n = fast_ok ? n / bm : 0;
     const long fast_blocks = tri ? (long)full_m * (full_m + 1) / 2 : (long)full_m * full_n;
+    if (tri == SYM_NONE && fast_blocks > 0 && (m % bm || n % bm)) {
+        g.tiles_m = ceil_div(m, bm);
+        g.tiles_n = ceil_div(n, bm);
+        g.full_m = full_m;
+        g.full_n = full_n;
+        const unsigned blocks = (unsigned)((long)g.tiles_m * g.tiles_n);
+        const bool ua = !(a.vec_ok && b.vec_ok);
+#define GB_MIX(BM_, PA_, PB_)                                                                            hipLaunchKernelGGL((gemm_mixed_kernel<T, BM_, PA_, PB_>), dim3(blocks), dim3(256), 0, s, g)
+#define GB_MIX4(BM_, O, K)                                                                               if (a.form == PAT_OUTER && b.form == PAT_KCONT) GB_MIX(BM_, O, K);                               else if (a.form == PAT_OUTER) GB_MIX(BM_, O, O);                                                 else if (b.form == PAT_KCONT) GB_MIX(BM_, K, K);                                                 else GB_MIX(BM_, K, O)
+        PROF_BEGIN(s, kname<T>(big, 1));
+        if (big) {
+            if (ua) { GB_MIX4(128, PAT_OUTER_U, PAT_KCONT_U); }
+            else { GB_MIX4(128, PAT_OUTER, PAT_KCONT); }
+        } else {
+            if (ua) { GB_MIX4(64, PAT_OUTER_U, PAT_KCONT_U); }
+            else { GB_MIX4(64, PAT_OUTER, PAT_KCONT); }
+        }
+        PROF_END(s);
+#undef GB_MIX4
+#undef GB_MIX
+        return (int)hipGetLastError();
+    }
     if (fast_blocks > 0) {
         GemmArgs<T> f = g;
         f.tiles_m = full_m;

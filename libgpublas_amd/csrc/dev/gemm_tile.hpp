@@ -23,6 +23,13 @@
 namespace {
 
 enum { PAT_OUTER = 0, PAT_KCONT = 1, PAT_ELEM = 2 };
+// Template-only patterns of the fast tile: full tiles of an operand that is
+// not 16-B aligned (or whose leading dimension is odd), read in the OUTER /
+// KCONT layout with one element per load and no bounds checks.  Same values
+// in the same LDS places as the aligned patterns, hence the same bits.
+enum { PAT_OUTER_U = 3, PAT_KCONT_U = 4 };
+__host__ __device__ constexpr int pat_base(int p) { return p >= PAT_OUTER_U ? p - PAT_OUTER_U : p; }
+__host__ __device__ constexpr bool pat_unaligned(int p) { return p >= PAT_OUTER_U; }
 enum { SYM_NONE = 0, SYM_LOWER = 1, SYM_UPPER = 2 };
 
 // One operand of the product as the loader sees it: element (o, k) of the
@@ -133,8 +140,10 @@ struct TileLoader {
         }
     }
 
-    // Fast path: full tile, aligned operand, pattern fixed at compile time.
-    template <int PAT>
+    // Fast path: full tile, pattern fixed at compile time; a vector load per
+    // VEC elements when the operand is aligned (UA false), one load per
+    // element otherwise.
+    template <int PAT, bool UA = false>
     __device__ __forceinline__ static void load_pat(const Operand<T> &op, int o0, int k0,
                                                     Vec (&st)[NV])
     {
@@ -144,10 +153,48 @@ struct TileLoader {
             const int v = t + r * 256;
             if (PAT == PAT_OUTER) {
                 const int o = (v % (R / VEC)) * VEC, kk = v / (R / VEC);
-                st[r] = *(const Vec *)(op.p + (long)(o0 + o) + (long)(k0 + kk) * op.ld);
+                const T *src = op.p + (long)(o0 + o) + (long)(k0 + kk) * op.ld;
+                if (UA) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) st[r][e] = src[e];
+                } else {
+                    st[r] = *(const Vec *)src;
+                }
             } else {
                 const int kk = (v % (BK / VEC)) * VEC, o = v / (BK / VEC);
-                st[r] = *(const Vec *)(op.p + (long)(k0 + kk) + (long)(o0 + o) * op.ld);
+                const T *src = op.p + (long)(k0 + kk) + (long)(o0 + o) * op.ld;
+                if (UA) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) st[r][e] = src[e];
+                } else {
+                    st[r] = *(const Vec *)src;
+                }
+            }
+        }
+    }
+
+    // A ragged tile with the pattern fixed at compile time: element loads,
+    // zero outside the operand (o >= olim or k >= K; the address is clamped
+    // to the operand's first element there, so no load is skipped by a
+    // branch), vectors laid out as load_pat's.
+    template <int PAT>
+    __device__ __forceinline__ static void load_guard(const Operand<T> &op, int o0, int k0, int K,
+                                                      Vec (&st)[NV])
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            const int v = t + r * 256;
+            const int o = PAT == PAT_OUTER ? (v % (R / VEC)) * VEC : v / (BK / VEC);
+            const int kk = PAT == PAT_OUTER ? v / (R / VEC) : (v % (BK / VEC)) * VEC;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const long oe = o0 + o + (PAT == PAT_OUTER ? e : 0);
+                const long ke = k0 + kk + (PAT == PAT_OUTER ? 0 : e);
+                const bool in = oe < op.olim && ke < K;
+                const long off = PAT == PAT_OUTER ? oe + ke * op.ld : ke + oe * op.ld;
+                const T x = op.p[in ? off : 0];
+                st[r][e] = in ? x : T(0);
             }
         }
     }
@@ -319,16 +366,23 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T> &g, int tm, int tn,
     }
 }
 
-// Full C tile (tm, tn) of g with both operands 16-B aligned, leading
-// dimensions a multiple of the vector width and access patterns PA / PB
-// fixed at compile time: no per-element selects in the main loop (only a
-// partial last k-tile takes guarded element loads).  Same k order and
-// epilogue as gemm_tile, hence the same bits.
-template <typename T, int BM, int BN, int BK, int PA, int PB, int TRI>
+// Full C tile (tm, tn) of g with access patterns PA / PB fixed at compile
+// time: no per-element selects or bounds checks in the main loop (only a
+// partial last k-tile takes guarded element loads).  PAT_OUTER / PAT_KCONT:
+// both operands 16-B aligned with leading dimensions a multiple of the
+// vector width (vector loads); PAT_OUTER_U / PAT_KCONT_U: any alignment (one
+// load per element).  Same k order and epilogue as gemm_tile, hence the same
+// bits.
+// EDGE: a ragged tile (rows past m or columns past n, any k): every load
+// guarded (load_guard) and the epilogue bounded; elements past an operand's
+// end read as zero, as in gemm_tile, so the bits are still the same.
+template <typename T, int BM, int BN, int BK, int PA_, int PB_, int TRI, bool EDGE = false>
 __device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int tn,
                                                T (*As)[BK][BM + 128 / sizeof(T)],
                                                T (*Bs)[BK][BN + 128 / sizeof(T)])
 {
+    constexpr int PA = pat_base(PA_), PB = pat_base(PB_);
+    constexpr bool UA = pat_unaligned(PA_), UB = pat_unaligned(PB_);
     constexpr int PAD = 128 / (int)sizeof(T);
     constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
     typedef TileLoader<T, BM, BK> LA;
@@ -350,9 +404,14 @@ __device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int
     typename LA::Vec sa[LA::NV];
     typename LB::Vec sb[LB::NV];
     auto load = [&](int k0) -> bool {
+        if (EDGE) {
+            LA::template load_guard<PA>(g.a, m0, k0, K, sa);
+            LB::template load_guard<PB>(g.b, n0, k0, K, sb);
+            return false;  // laid out as the patterns
+        }
         if (k0 + BK <= K) {
-            LA::template load_pat<PA>(g.a, m0, k0, sa);
-            LB::template load_pat<PB>(g.b, n0, k0, sb);
+            LA::template load_pat<PA, UA>(g.a, m0, k0, sa);
+            LB::template load_pat<PB, UB>(g.b, n0, k0, sb);
             return false;
         }
         LA::load(g.a, m0, k0, K, PAT_ELEM, sa);
@@ -410,6 +469,7 @@ __device__ __forceinline__ void gemm_fast_tile(const GemmArgs<T> &g, int tm, int
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const long gj = n0 + wn * WN + j * 16 + Mfma<T>::drow(lane, r);
+                if (EDGE && (gi >= g.m || gj >= g.n)) continue;
                 if (TRI == SYM_LOWER && gi < gj) continue;
                 if (TRI == SYM_UPPER && gi > gj) continue;
                 T *c = g.C + gi + gj * g.ldc;

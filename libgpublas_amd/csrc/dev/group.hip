@@ -61,7 +61,7 @@ struct GGemm {
     GemmArgs<T> g;
     T *part;
     int fast_begin, edge_begin, fold_begin;
-    int pat;  // 0 NN, 1 NT, 2 TN, 3 TT
+    int pat;  // 0 NN, 1 NT, 2 TN, 3 TT; + 4: an operand not aligned (element loads)
     int ksplit, kchunk;
 };
 
@@ -167,7 +167,12 @@ __device__ __forceinline__ void gemm_fast_group_body(const GemmGroup<T> &G)
         case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
         case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
         case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
-        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+        case 3: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+        // operands not both aligned: element loads
+        case 4: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+        case 5: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+        case 6: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
         }
     }
 }
@@ -178,8 +183,8 @@ template <typename T, int BM>
 __device__ __forceinline__ void gemm_edge_group_body(const GemmGroup<T> &G)
 {
     constexpr int PAD = 128 / (int)sizeof(T);
-    __shared__ T As[2][16][BM + PAD];
-    __shared__ T Bs[2][16][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
+    __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
     const int np = min(G.np, kCapGemm);
     const int total = G.edge_tiles;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
@@ -197,7 +202,14 @@ __device__ __forceinline__ void gemm_edge_group_body(const GemmGroup<T> &G)
             tm = g.full_m + e2 % rows;
             tn = e2 / rows;
         }
-        gemm_tile<T, BM, BM, 16>(g, tm, tn, As, Bs);
+        // the guarded fast tile with the problem's access patterns (element
+        // loads; same bits as the generic tile)
+        switch (G.p[p].pat & 3) {
+        case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+        case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+        case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+        }
     }
 }
 
@@ -688,13 +700,15 @@ int add_gemm(gpublas_batch *b, char ta, char tb, int m, int n, int k, T alpha, c
     g.tiles_m = ceil_div(m, bm);
     g.tiles_n = ceil_div(n, bm);
     g.tri = SYM_NONE;
-    // interior tiles take the fast kernel when both operands allow vector
-    // loads (the 32-wide shape has no fast variant)
-    const bool fast = cfg != CFG_32 && g.a.vec_ok && g.b.vec_ok;
+    // interior tiles take the fast tile: vector loads when both operands
+    // allow them, element loads otherwise (the 32-wide shape has no fast
+    // variant)
+    const bool fast = cfg != CFG_32;
     g.full_m = fast ? m / bm : 0;
     g.full_n = fast ? n / bm : 0;
     if (g.full_m == 0 || g.full_n == 0) g.full_m = g.full_n = 0;
-    e.pat = (g.a.form == PAT_OUTER ? 0 : 2) + (g.b.form == PAT_KCONT ? 0 : 1);
+    e.pat = (g.a.form == PAT_OUTER ? 0 : 2) + (g.b.form == PAT_KCONT ? 0 : 1) +
+            (g.a.vec_ok && g.b.vec_ok ? 0 : 4);
     // split along k when the tiles alone cannot spread over the CUs: about
     // 256 units, chunks of at least 256, partials of at most 16 MiB.  Counted
     // in 64 x 64 tiles whatever the tile shape, so every shape splits alike
