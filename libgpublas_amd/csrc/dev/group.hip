@@ -18,10 +18,12 @@
 //
 // Kernels (all persistent: a fixed grid loops over the batch's work units,
 // so one captured graph serves every batch of the slot):
-//  * group_gemm_kernel<T, BM, BK>: every tile of every GEMM problem that the
-//    tile-shape table assigns to BM x BM; tile t -> problem by binary search
-//    over the prefix sums; the tile itself is the per-call generic routine
-//    (gemm_tile.hpp), so a product's bits do not depend on the batch;
+//  * group_gemm_arg / _tab<T, BM>: every tile of every GEMM problem that the
+//    tile-shape table assigns to BM x BM, full tiles first, in one grid;
+//    tile t -> problem by binary search over the prefix sums; the tile
+//    itself is the per-call fast tile routine (gemm_tile.hpp; its guarded
+//    variant for ragged tiles), so a product's bits do not depend on the
+//    batch;
 //  * group_gemv_part_kernel / group_gemv_fold_kernel: y = alpha*op(A)*x +
 //    beta*y as column-block (N) or row-chunk (T) partial sums followed by a
 //    fixed-order fold; the split depends only on (m, n);
@@ -149,66 +151,76 @@ __device__ __forceinline__ GemmArgs<T> chunk_args(const GGemm<T> &e, int s)
     return g;
 }
 
+// Full tile t of the group (numbered over its problems' fast tiles).
 template <typename T, int BM>
-__device__ __forceinline__ void gemm_fast_group_body(const GemmGroup<T> &G)
+__device__ __forceinline__ void fast_group_tile(const GemmGroup<T> &G, int np, int t,
+                                                T (*As)[16][BM + 128 / sizeof(T)],
+                                                T (*Bs)[16][BM + 128 / sizeof(T)])
 {
-    constexpr int PAD = 128 / (int)sizeof(T);
-    __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
-    __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
-    const int np = min(G.np, kCapGemm);
-    const int total = G.fast_tiles;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int p = find_unit(&G.p[0].fast_begin, sizeof(GGemm<T>), np, t);
-        const int per = G.p[p].g.full_m * G.p[p].g.full_n;
-        const int tl0 = t - G.p[p].fast_begin, tl = tl0 % per;
-        const GemmArgs<T> g = chunk_args(G.p[p], tl0 / per);
-        const int tm = tl % g.full_m, tn = tl / g.full_m;
-        switch (G.p[p].pat) {
-        case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
-        case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
-        case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
-        case 3: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
-        // operands not both aligned: element loads
-        case 4: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
-        case 5: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
-        case 6: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
-        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
-        }
+    const int p = find_unit(&G.p[0].fast_begin, sizeof(GGemm<T>), np, t);
+    const int per = G.p[p].g.full_m * G.p[p].g.full_n;
+    const int tl0 = t - G.p[p].fast_begin, tl = tl0 % per;
+    const GemmArgs<T> g = chunk_args(G.p[p], tl0 / per);
+    const int tm = tl % g.full_m, tn = tl / g.full_m;
+    switch (G.p[p].pat) {
+    case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
+    case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+    case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE>(g, tm, tn, As, Bs); break;
+    case 3: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE>(g, tm, tn, As, Bs); break;
+    // operands not both aligned: element loads
+    case 4: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+    case 5: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+    case 6: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_KCONT_U, SYM_NONE>(g, tm, tn, As, Bs); break;
+    default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT_U, PAT_OUTER_U, SYM_NONE>(g, tm, tn, As, Bs); break;
     }
 }
 
-// Edge tiles of a problem, enumerated as the column strip past the full
-// columns (all rows) followed by the row strip past the full rows.
+// Ragged tile t of the group (numbered over its problems' edge tiles; a
+// problem's are the column strip past the full columns, all rows, followed
+// by the row strip past the full rows), on the guarded fast tile with the
+// problem's access patterns (element loads; same bits as the generic tile).
 template <typename T, int BM>
-__device__ __forceinline__ void gemm_edge_group_body(const GemmGroup<T> &G)
+__device__ __forceinline__ void edge_group_tile(const GemmGroup<T> &G, int np, int t,
+                                                T (*As)[16][BM + 128 / sizeof(T)],
+                                                T (*Bs)[16][BM + 128 / sizeof(T)])
+{
+    const int p = find_unit(&G.p[0].edge_begin, sizeof(GGemm<T>), np, t);
+    const int per = G.p[p].g.tiles_m * G.p[p].g.tiles_n - G.p[p].g.full_m * G.p[p].g.full_n;
+    const int e0 = t - G.p[p].edge_begin, e = e0 % per;
+    const GemmArgs<T> g = chunk_args(G.p[p], e0 / per);
+    const int strip = g.tiles_m * (g.tiles_n - g.full_n);
+    int tm, tn;
+    if (e < strip) {
+        tm = e % g.tiles_m;
+        tn = g.full_n + e / g.tiles_m;
+    } else {
+        const int e2 = e - strip, rows = g.tiles_m - g.full_m;
+        tm = g.full_m + e2 % rows;
+        tn = e2 / rows;
+    }
+    switch (G.p[p].pat & 3) {
+    case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+    case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+    case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+    default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+    }
+}
+
+// Every tile of the group, full ones first, in one grid: the ragged tiles
+// of a batch do not make a round of their own after the full ones.
+template <typename T, int BM>
+__device__ __forceinline__ void gemm_group_body(const GemmGroup<T> &G)
 {
     constexpr int PAD = 128 / (int)sizeof(T);
     __shared__ __attribute__((aligned(16))) T As[2][16][BM + PAD];
     __shared__ __attribute__((aligned(16))) T Bs[2][16][BM + PAD];
     const int np = min(G.np, kCapGemm);
-    const int total = G.edge_tiles;
+    const int nf = BM == 32 ? 0 : G.fast_tiles, total = nf + G.edge_tiles;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int p = find_unit(&G.p[0].edge_begin, sizeof(GGemm<T>), np, t);
-        const int per = G.p[p].g.tiles_m * G.p[p].g.tiles_n - G.p[p].g.full_m * G.p[p].g.full_n;
-        const int e0 = t - G.p[p].edge_begin, e = e0 % per;
-        const GemmArgs<T> g = chunk_args(G.p[p], e0 / per);
-        const int strip = g.tiles_m * (g.tiles_n - g.full_n);
-        int tm, tn;
-        if (e < strip) {
-            tm = e % g.tiles_m;
-            tn = g.full_n + e / g.tiles_m;
+        if (t < nf) {
+            if constexpr (BM != 32) fast_group_tile<T, BM>(G, np, t, As, Bs);
         } else {
-            const int e2 = e - strip, rows = g.tiles_m - g.full_m;
-            tm = g.full_m + e2 % rows;
-            tn = e2 / rows;
-        }
-        // the guarded fast tile with the problem's access patterns (element
-        // loads; same bits as the generic tile)
-        switch (G.p[p].pat & 3) {
-        case 0: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
-        case 1: gemm_fast_tile<T, BM, BM, 16, PAT_OUTER, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
-        case 2: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_KCONT, SYM_NONE, true>(g, tm, tn, As, Bs); break;
-        default: gemm_fast_tile<T, BM, BM, 16, PAT_KCONT, PAT_OUTER, SYM_NONE, true>(g, tm, tn, As, Bs); break;
+            edge_group_tile<T, BM>(G, np, t - nf, As, Bs);
         }
     }
 }
@@ -246,24 +258,14 @@ __global__ __launch_bounds__(256) void group_gemm_fold_tab(const GemmGroup<T> *G
 }
 
 template <typename T, int BM>
-__global__ __launch_bounds__(256, 2) void group_gemm_fast_arg(const GemmGroup<T> G)
+__global__ __launch_bounds__(256, 2) void group_gemm_arg(const GemmGroup<T> G)
 {
-    gemm_fast_group_body<T, BM>(G);
+    gemm_group_body<T, BM>(G);
 }
 template <typename T, int BM>
-__global__ __launch_bounds__(256, 2) void group_gemm_fast_tab(const GemmGroup<T> *G)
+__global__ __launch_bounds__(256, 2) void group_gemm_tab(const GemmGroup<T> *G)
 {
-    gemm_fast_group_body<T, BM>(*G);
-}
-template <typename T, int BM>
-__global__ __launch_bounds__(256) void group_gemm_edge_arg(const GemmGroup<T> G)
-{
-    gemm_edge_group_body<T, BM>(G);
-}
-template <typename T, int BM>
-__global__ __launch_bounds__(256) void group_gemm_edge_tab(const GemmGroup<T> *G)
-{
-    gemm_edge_group_body<T, BM>(*G);
+    gemm_group_body<T, BM>(*G);
 }
 
 // ---- grouped GEMV -----------------------------------------------------------
@@ -630,8 +632,7 @@ int cfg_bm(int cfg) { return cfg == CFG_32 ? 32 : cfg == CFG_64 ? 64 : 128; }
 
 // grid of each persistent kernel: enough workgroups to fill 256 CUs at the
 // kernel's occupancy (double / single), capped by the work when eager
-constexpr int kGridGemm[2][kCfgs] = {{1024, 768, 256}, {1024, 1024, 512}};
-constexpr int kGridGemmFast[2][kCfgs] = {{512, 512, 512}, {512, 512, 512}};  // 2 per CU
+constexpr int kGridGemm[2][kCfgs] = {{1024, 512, 512}, {1024, 512, 512}};  // 2 per CU from 64
 constexpr int kGridGemv = 1024, kGridFold = 512, kGridTrsm = 256;  // TRSM: 1024-thread groups
 constexpr size_t kWsDoubles = (size_t)8 << 20;  // 64 MiB of GEMV partial sums
 
@@ -757,21 +758,17 @@ void launch_gemm_group(gpublas_batch *b, hipStream_t s, int cfg, bool capture)
 {
     constexpr int ty = sizeof(T) == 8 ? TY_D : TY_S;
     const GemmGroup<T> &G = group_of<T>(b->h_tab, cfg);
-    const int ffix = kGridGemmFast[ty][cfg], efix = kGridGemm[ty][cfg];
+    const int fix = kGridGemm[ty][cfg];
     if (capture) {
         const GemmGroup<T> *dG = &group_of<T>(b->d_tab, cfg);
-        if (BM != 32)
-            hipLaunchKernelGGL((group_gemm_fast_tab<T, BM>), dim3(ffix), dim3(256), 0, s, dG);
-        hipLaunchKernelGGL((group_gemm_edge_tab<T, BM>), dim3(efix), dim3(256), 0, s, dG);
+        hipLaunchKernelGGL((group_gemm_tab<T, BM>), dim3(fix), dim3(256), 0, s, dG);
         hipLaunchKernelGGL((group_gemm_fold_tab<T>), dim3(512), dim3(256), 0, s, dG);
         return;
     }
-    if (BM != 32 && G.fast_tiles)
-        hipLaunchKernelGGL((group_gemm_fast_arg<T, BM>), dim3(G.fast_tiles < ffix ? G.fast_tiles : ffix),
-                           dim3(256), 0, s, G);
-    if (G.edge_tiles)
-        hipLaunchKernelGGL((group_gemm_edge_arg<T, BM>), dim3(G.edge_tiles < efix ? G.edge_tiles : efix),
-                           dim3(256), 0, s, G);
+    const int tiles = (BM == 32 ? 0 : G.fast_tiles) + G.edge_tiles;
+    if (tiles)
+        hipLaunchKernelGGL((group_gemm_arg<T, BM>), dim3(tiles < fix ? tiles : fix), dim3(256), 0,
+                           s, G);
     if (G.fold_units)
         hipLaunchKernelGGL((group_gemm_fold_arg<T>), dim3(G.fold_units < 512 ? G.fold_units : 512),
                            dim3(256), 0, s, G);
