@@ -756,6 +756,56 @@ def test_dtrsv_blocked_sizes(gpu, ora, uplo, trans, diag):
     np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,lda", [(1537, 1537), (2176, 2180), (640, 640)])
+def test_dtrsv_lookback_shapes(gpu, ora, uplo, trans, n, lda):
+    """Single-launch solve with helper workgroups sharing each block's earlier
+    tiles: 13 blocks with a 1-wide last block and an odd leading dimension
+    (element loads), 17 full blocks with padded columns, 5 blocks (one helper
+    per block).  Padding rows and the other triangle are NaN, so any read of
+    them shows; a second call gives the same bits."""
+    A = np.full((lda, n), np.nan)
+    T = uniform(35, (n, n))
+    T[np.diag_indices(n)] = n + np.abs(uniform(36, (n,)))
+    T[_other_triangle(n, uplo)] = np.nan
+    A[:n, :] = T
+    A = np.asfortranarray(A)
+    b = uniform(37, (n,))
+    x, x2, r = b.copy(), b.copy(), b.copy()
+    gpu.dtrsv(uplo, trans, "N", n, A, lda, x, 1)
+    ora.ora_dtrsv(c(uplo), c(trans), c("N"), n, p(A), lda, p(r), 1)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+    gpu.dtrsv(uplo, trans, "N", n, A, lda, x2, 1)
+    assert np.array_equal(x, x2)
+
+
+@pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T"), ("U", "N"), ("L", "T")])
+def test_dtrsv_nonfinite_reaches_only_later_unknowns(gpu, ora, uplo, trans):
+    """n = 1000 with one Inf in the triangle, coupling unknown 300 into 600
+    (forward solves) or 600 into 300 (backward): the unknowns solved before
+    the Inf appears, including those of the same 128-block, stay finite and
+    match the oracle; the rest are non-finite exactly where the oracle's
+    are (0 * Inf must not reach a row that was already solved)."""
+    n = 1000
+    A = uniform(38, (n, n))
+    A[np.diag_indices(n)] = n + 1.0
+    A[_other_triangle(n, uplo)] = 0.0
+    fwd = (uplo == "L") == (trans == "N")
+    early, late = (300, 600) if fwd else (600, 300)
+    # op(A)[late, early] is the coefficient of the earlier unknown in the later row
+    A[(late, early) if trans == "N" else (early, late)] = np.inf
+    A = np.asfortranarray(A)
+    b = uniform(39, (n,))
+    x, r = b.copy(), b.copy()
+    gpu.dtrsv(uplo, trans, "N", n, A, n, x, 1)
+    ora.ora_dtrsv(c(uplo), c(trans), c("N"), n, p(A), n, p(r), 1)
+    solved_before = np.arange(n) < late if fwd else np.arange(n) > late
+    assert np.all(np.isfinite(r[solved_before])) and not np.any(np.isfinite(r[~solved_before]))
+    assert np.array_equal(np.isfinite(x), np.isfinite(r))
+    np.testing.assert_allclose(x[solved_before], r[solved_before], rtol=1e-12, atol=1e-14)
+
+
 @pytest.mark.parametrize("uplo,trans", [("L", "N"), ("U", "T"), ("U", "N"), ("L", "T")])
 def test_dtrsv_integer_exact_8192(gpu, uplo, trans):
     """n = 8192, unit triangle with entries in {-1, 0, 1} and an integer
