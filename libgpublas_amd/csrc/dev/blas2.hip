@@ -160,7 +160,10 @@ __global__ void scal_vec_kernel(int n, double beta, double *y, long incy, long y
 }
 
 // DGER: A := alpha x y^T + A, read and written once (16 mn bytes).  The grid
-// is the DGEMV 'N' panel plan: a workgroup owns GN_ROWS rows (two per lane,
+// is the DGEMV 'N' panel plan at one workgroup per CU (a read-modify-write
+// stream measured 6.2 TB/s with one workgroup per CU, 5.7 with two; a sweep
+// of 4 KB column runs by all workgroups together, as a plain copy does it,
+// measured slower here): a workgroup owns GN_ROWS rows (two per lane,
 // one 16-B load and store per column) of a panel of cbw columns and keeps
 // GR_U columns of loads in flight per lane before their stores.  As netlib,
 // a column with y(j) = 0 is left untouched and the update is
@@ -510,18 +513,20 @@ __global__ __launch_bounds__(256) void spr_kernel(int n, double alpha, const dou
 inline long start_of(int n, int inc) { return inc < 0 ? -(long)(n - 1) * inc : 0; }
 inline bool isN(char c) { return c == 'N' || c == 'n'; }
 
-// Column-panel plan of DGEMV 'N': nrb row blocks of GN_ROWS rows times ncb
-// panels of cbw columns (~2 workgroups per CU, panels at least 16 columns
-// wide).  With ncb > 1 the panels' partial sums (ncb x m doubles) are folded
-// in panel order by a second kernel.
+// Column-panel plan of DGEMV 'N' (and DGER): nrb row blocks of GN_ROWS rows
+// times ncb panels of cbw columns, about `target` workgroups (DGEMV: 512,
+// two per CU; DGER: 256, one per CU, where its read-modify-write stream
+// measured 2% faster), panels at least 16 columns wide.  With ncb > 1 the
+// DGEMV panels' partial sums (ncb x m doubles) are folded in panel order by
+// a second kernel.
 struct GemvNPlan {
     int nrb, ncb, cbw;
 };
-GemvNPlan gemv_n_plan(int m, int n)
+GemvNPlan gemv_n_plan(int m, int n, int target = 512)
 {
     GemvNPlan p;
     p.nrb = ceil_div(m, GN_ROWS);
-    p.ncb = ceil_div(512, p.nrb);
+    p.ncb = ceil_div(target, p.nrb);
     const int maxcb = ceil_div(n, 16);
     if (p.ncb > maxcb) p.ncb = maxcb;
     if (p.ncb < 1) p.ncb = 1;
@@ -654,7 +659,7 @@ int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x
 {
     hipStream_t s = pick_stream(stream);
     if (m <= 0 || n <= 0 || alpha == 0.0) return 0;
-    const GemvNPlan pl = gemv_n_plan(m, n);
+    const GemvNPlan pl = gemv_n_plan(m, n, 256);
     const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
     PROF_BEGIN(s, "dger");
     hipLaunchKernelGGL(ger_stream, dim3((unsigned)pl.nrb * pl.ncb), dim3(256), 0, s, m, n, alpha,
