@@ -66,6 +66,15 @@ void dgemm_(const char *transa, const char *transb, const int *m, const int *n, 
 void sgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
             const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
             const float *beta, float *c, const int *ldc);
+/* complex GEMM (gemm.cc:181-217, F77_gemm(z) / F77_gemm(c)): a, b, c, alpha
+ * and beta point to (re, im) pairs, leading dimensions count complex
+ * elements; trans 'C' is the conjugate transpose */
+void zgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+            const double *beta, double *c, const int *ldc);
+void cgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
+            const float *beta, float *c, const int *ldc);
 void dsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
             const double *a, const int *lda, const double *beta, double *c, const int *ldc);
 void ssyrk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
@@ -158,6 +167,13 @@ void cblas_dgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, in
 void cblas_sgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
                  float alpha, const float *a, int lda, const float *b, int ldb, float beta,
                  float *c, int ldc);
+/* complex twins (cblas.h cblas_zgemm / cblas_cgemm): alpha, beta by address */
+void cblas_zgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
+void cblas_cgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
 void cblas_dsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
                  double alpha, const double *a, int lda, double beta, double *c, int ldc);
 void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,

@@ -93,6 +93,16 @@ int gpublas_dev_dgemm(void *stream, char transa, char transb, int m, int n, int 
 int gpublas_dev_sgemm(void *stream, char transa, char transb, int m, int n, int k, float alpha,
                       const float *d_A, int lda, const float *d_B, int ldb, float beta,
                       float *d_C, int ldc);
+/* complex GEMM on interleaved (re, im) operands (replaces cublasZgemm /
+ * cublasCgemm behind gemm.cc:181-217); ld counts complex elements; runs as one
+ * real MFMA product of m x 2n x 2k plus pack and combine passes */
+int gpublas_dev_zgemm(void *stream, char transa, char transb, int m, int n, int k,
+                      double alpha_re, double alpha_im, const double *d_A, int lda,
+                      const double *d_B, int ldb, double beta_re, double beta_im, double *d_C,
+                      int ldc);
+int gpublas_dev_cgemm(void *stream, char transa, char transb, int m, int n, int k,
+                      float alpha_re, float alpha_im, const float *d_A, int lda, const float *d_B,
+                      int ldb, float beta_re, float beta_im, float *d_C, int ldc);
 int gpublas_dev_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha,
                       const double *d_A, int lda, double beta, double *d_C, int ldc);
 int gpublas_dev_ssyrk(void *stream, char uplo, char trans, int n, int k, float alpha,

@@ -98,6 +98,8 @@ def _declare(lib):
     lib.cblas_daxpy.argtypes = [ci, cd, vp, ci, vp, ci]
     lib.cblas_dgemm.argtypes = [ci, ci, ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.cblas_sgemm.argtypes = [ci, ci, ci, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci]
+    lib.cblas_zgemm.argtypes = [ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
+    lib.cblas_cgemm.argtypes = [ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
     lib.cblas_dgemv.argtypes = [ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.cblas_dger.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, vp, ci]
     lib.cblas_dsymv.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
@@ -167,6 +169,24 @@ def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
 def sgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     library().sgemm_(_c(transa), _c(transb), _i(m), _i(n), _i(k), _f(alpha), _p(a), _i(lda),
                      _p(b), _i(ldb), _f(beta), _p(c), _i(ldc))
+
+
+def _z(v, dt=ctypes.c_double):
+    """A complex scalar as the (re, im) pair the Fortran ABI passes by address."""
+    v = complex(v)
+    return (dt * 2)(v.real, v.imag)
+
+
+def zgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """ZGEMM on complex128 arrays (leading dimensions in complex elements)."""
+    library().zgemm_(_c(transa), _c(transb), _i(m), _i(n), _i(k), _z(alpha), _p(a), _i(lda),
+                     _p(b), _i(ldb), _z(beta), _p(c), _i(ldc))
+
+
+def cgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """CGEMM on complex64 arrays."""
+    library().cgemm_(_c(transa), _c(transb), _i(m), _i(n), _i(k), _z(alpha, ctypes.c_float),
+                     _p(a), _i(lda), _p(b), _i(ldb), _z(beta, ctypes.c_float), _p(c), _i(ldc))
 
 
 def dsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc):

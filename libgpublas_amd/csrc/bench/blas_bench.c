@@ -3,6 +3,8 @@
  * through the Fortran-ABI entry points of libgpublas on tracked operands.
  *
  *   blas_bench --lib libgpublas.so --op gemm --prec d --m M --n N --k K --ta N --tb N
+ *                                    (--prec s, z or c: SGEMM, ZGEMM, CGEMM; complex flops
+ *                                     counted as 8mnk real flops)
  *   blas_bench --lib ... --op syrk --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trsm --m M --n N --side L --uplo L --ta N --diag N
  *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
@@ -70,6 +72,10 @@ static int (*g_sync)(void *);
 
 static void *operand(size_t elems, int prec, uint64_t seed)
 {
+    if (prec == 'z' || prec == 'c') { /* complex: (re, im) pairs of the real fill */
+        elems *= 2;
+        prec = prec == 'z' ? 'd' : 's';
+    }
     size_t es = prec == 'd' ? 8 : 4;
     void *h = g_malloc(elems * es);
     if (!h) {
@@ -174,10 +180,11 @@ int main(int argc, char **argv)
 
     double flops = 0, bytes = 0;
     void *A = NULL, *B = NULL, *C = NULL;
-    double alpha_d = 1.0, beta_d = 0.5;
-    float alpha_s = 1.0f, beta_s = 0.5f;
-    void *alpha = prec == 'd' ? (void *)&alpha_d : (void *)&alpha_s;
-    void *beta = prec == 'd' ? (void *)&beta_d : (void *)&beta_s;
+    double alpha_d[2] = {1.0, 0.0}, beta_d[2] = {0.5, 0.0};
+    float alpha_s[2] = {1.0f, 0.0f}, beta_s[2] = {0.5f, 0.0f};
+    const int dbl = prec == 'd' || prec == 'z';
+    void *alpha = dbl ? (void *)alpha_d : (void *)alpha_s;
+    void *beta = dbl ? (void *)beta_d : (void *)beta_s;
     int lda = 0, ldb = 0, ldc = 0;
     typedef void (*gemm_t)(const char *, const char *, const int *, const int *, const int *,
                            const void *, const void *, const int *, const void *, const int *,
@@ -235,14 +242,14 @@ int main(int argc, char **argv)
     tbmv_t tbmv = NULL;
     int bw = 0; /* band array rows */
     if (!strcmp(op, "gemm")) {
-        gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : "sgemm_");
+        gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : prec == 's' ? "sgemm_" : prec == 'z' ? "zgemm_" : "cgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
         ldb = (tb == 'N' || tb == 'n') ? k : n;
         ldc = m;
         A = operand((size_t)lda * ((ta == 'N' || ta == 'n') ? k : m), prec, seed);
         B = operand((size_t)ldb * ((tb == 'N' || tb == 'n') ? n : k), prec, seed + 1);
         C = operand((size_t)m * n, prec, seed + 2);
-        flops = 2.0 * m * n * k;
+        flops = (prec == 'z' || prec == 'c' ? 8.0 : 2.0) * m * n * k; /* real flops */
     } else if (!strcmp(op, "syrk")) {
         syrk = (syrk_t)sym(prec == 'd' ? "dsyrk_" : "ssyrk_");
         lda = (trans == 'N' || trans == 'n') ? n : k;

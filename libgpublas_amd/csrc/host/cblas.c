@@ -78,6 +78,27 @@ static const int gemm_row[14] = {0, 3, 2, 5, 4, 6, 0, 0, 11, 0, 9, 0, 0, 14};
 DEFINE_CBLAS_GEMM(d, double, "DGEMM ")
 DEFINE_CBLAS_GEMM(s, float, "SGEMM ")
 
+/* complex twins: alpha and beta by address, (re, im) pairs */
+#define DEFINE_CBLAS_CGEMM(PRE, R, NAME6)                                                        \
+    void cblas_##PRE##gemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m,    \
+                           int n, int k, const void *alpha, const void *a, int lda,              \
+                           const void *b, int ldb, const void *beta, void *c, int ldc)           \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!tr_ok(ta)) { cbl_err(NAME6, 2); return; }                                           \
+        if (!tr_ok(tb)) { cbl_err(NAME6, 3); return; }                                           \
+        char cta = tr(ta), ctb = tr(tb);                                                         \
+        const R *al = (const R *)alpha, *be = (const R *)beta;                                   \
+        if (layout == CblasColMajor)                                                             \
+            WITH_MAP(gemm_col, PRE##gemm_(&cta, &ctb, &m, &n, &k, al, (const R *)a, &lda,        \
+                                          (const R *)b, &ldb, be, (R *)c, &ldc));                \
+        else                                                                                     \
+            WITH_MAP(gemm_row, PRE##gemm_(&ctb, &cta, &n, &m, &k, al, (const R *)b, &ldb,        \
+                                          (const R *)a, &lda, be, (R *)c, &ldc));                \
+    }
+DEFINE_CBLAS_CGEMM(z, double, "ZGEMM ")
+DEFINE_CBLAS_CGEMM(c, float, "CGEMM ")
+
 /* ---- SYRK: 1 layout 2 uplo 3 trans 4 n 5 k 6 alpha 7 A 8 lda 9 beta 10 C 11 ldc ---- */
 static const int syrk_map[11] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 0, 11};
 

@@ -1,6 +1,6 @@
 /*
  * level3.c -- Fortran-ABI Level-3 entry points: ?gemm_, ?syrk_, ?syr2k_, ?trmm_, ?trsm_,
- * ?symm_ for real single and double precision.
+ * ?symm_ for real single and double precision, and zgemm_ / cgemm_.
  *
  * Each entry follows the reference's three steps (blas_level3/gemm.cc:87-179
  * and the syrk/trsm/symm twins): netlib argument check with xerbla, quick
@@ -120,6 +120,86 @@ static int dsyrk_dev(void *s, char uplo, char trans, int n, int k, double alpha,
 
 DEFINE_GEMM(d, double, "DGEMM ", dgemm_dev, gb_serve_dgemm, GB_RK_DGEMM)
 DEFINE_GEMM(s, float, "SGEMM ", gpublas_dev_sgemm, gb_serve_sgemm, GB_RK_SGEMM)
+
+/* ------------------------------------------------------------------ */
+/* complex GEMM (zgemm_ / cgemm_, gemm.cc:181-217): operands, alpha and    */
+/* beta are (re, im) pairs; 'C' is the conjugate transpose.  Routing uses  */
+/* the real GEMM's crossover row with the complex flop and byte counts.    */
+/* ------------------------------------------------------------------ */
+#define DEFINE_CGEMM(PRE, R, NAME6, DEVFN, RK)                                                   \
+    void PRE##gemm_(const char *transa, const char *transb, const int *m, const int *n,          \
+                    const int *k, const R *alpha, const R *a, const int *lda, const R *b,        \
+                    const int *ldb, const R *beta, R *c, const int *ldc)                          \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int nota = gb_lsame(*transa, 'N'), notb = gb_lsame(*transb, 'N');                  \
+        const int nrowa = nota ? *m : *k, nrowb = notb ? *k : *n;                                \
+        int info = 0;                                                                            \
+        if (!nota && !gb_lsame(*transa, 'C') && !gb_lsame(*transa, 'T')) info = 1;               \
+        else if (!notb && !gb_lsame(*transb, 'C') && !gb_lsame(*transb, 'T')) info = 2;          \
+        else if (*m < 0) info = 3;                                                               \
+        else if (*n < 0) info = 4;                                                               \
+        else if (*k < 0) info = 5;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 8;                                                \
+        else if (*ldb < MAXI(1, nrowb)) info = 10;                                               \
+        else if (*ldc < MAXI(1, *m)) info = 13;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int a0 = alpha[0] == 0 && alpha[1] == 0, b1 = beta[0] == 1 && beta[1] == 0;        \
+        const int b0 = beta[0] == 0 && beta[1] == 0;                                             \
+        if (*m == 0 || *n == 0 || ((a0 || *k == 0) && b1)) {                                     \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int cut = gb_opt.cutoff;                                                           \
+        const double es_ = 2 * sizeof(R), mm_ = *m, nn_ = *n, kk_ = *k;                          \
+        if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldb < cut || *ldc < cut),             \
+                          8 * mm_ * nn_ * kk_, es_ * (mm_ * kk_ + kk_ * nn_ + 2 * mm_ * nn_), a,  \
+                          b, c)) {                                                               \
+            typedef void (*fn_t)(const char *, const char *, const int *, const int *,           \
+                                 const int *, const R *, const R *, const int *, const R *,      \
+                                 const int *, const R *, R *, const int *);                      \
+            fn_t f = (fn_t)gb_host_fn(#PRE "gemm_");                                             \
+            GB_COUNT(fallthrough, 1);                                                            \
+            f(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);                     \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int scale_only = (a0 || *k == 0);                                                  \
+        if (scale_only && !gb_ptr_tracked(c)) {                                                  \
+            for (int j = 0; j < *n; ++j)                                                         \
+                for (int i = 0; i < *m; ++i) {                                                   \
+                    R *e = c + 2 * IDX(i, j, *ldc);                                              \
+                    const R cr = e[0], ci = e[1];                                                \
+                    e[0] = b0 ? (R)0 : beta[0] * cr - beta[1] * ci;                              \
+                    e[1] = b0 ? (R)0 : beta[0] * ci + beta[1] * cr;                              \
+                }                                                                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (!scale_only) {                                                                       \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nota ? *k : *m, 2 * sizeof(R), GB_IN, s); \
+            gb_acquire_matrix(&ops[1], b, *ldb, nrowb, notb ? *n : *k, 2 * sizeof(R), GB_IN, s); \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *m, *n, 2 * sizeof(R), b0 ? GB_OUT : GB_INOUT, s);   \
+        gb_check(DEVFN(s, *transa, *transb, *m, *n, *k, alpha[0], alpha[1],                      \
+                       (const R *)ops[0].dev, *lda, (const R *)ops[1].dev, *ldb, beta[0],        \
+                       beta[1], (R *)ops[2].dev, *ldc),                                          \
+                 #PRE "gemm launch");                                                            \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+
+DEFINE_CGEMM(z, double, "ZGEMM ", gpublas_dev_zgemm, GB_RK_DGEMM)
+DEFINE_CGEMM(c, float, "CGEMM ", gpublas_dev_cgemm, GB_RK_SGEMM)
 
 /* ------------------------------------------------------------------ */
 /* SYRK                                                                */

@@ -23,6 +23,7 @@
  * is what the fall-through host BLAS computes up to summation order.
  */
 #include "refblas.h"
+#include <complex.h>
 #include <math.h>
 #include <stddef.h>
 #include <ctype.h>
@@ -259,6 +260,58 @@ void NAME(char ta, char tb, int m, int n, int k, T alpha, const T *A, int lda, c
 }
 DEFINE_GEMM(ora_dgemm, double)
 DEFINE_GEMM(ora_sgemm, float)
+
+/* Complex GEMM, netlib ZGEMM / CGEMM (the routines behind the reference's
+ * F77_gemm(z) / F77_gemm(c), gemm.cc:181-217): 'C' conjugates; the 'N' form
+ * of A runs column axpys, the others dot products, as netlib does.  Operands
+ * and scalars are (re, im) pairs. */
+#define DEFINE_CGEMM(NAME, R, CT)                                                             \
+void NAME(char ta, char tb, int m, int n, int k, const R *alpha_, const R *A_, int lda,       \
+          const R *B_, int ldb, const R *beta_, R *C_, int ldc)                               \
+{                                                                                             \
+    const CT *A = (const CT *)A_, *B = (const CT *)B_;                                        \
+    CT *C = (CT *)C_;                                                                         \
+    const CT alpha = *(const CT *)alpha_, beta = *(const CT *)beta_;                          \
+    int nota = ora_lsame(ta, 'N'), notb = ora_lsame(tb, 'N');                                 \
+    int conja = ora_lsame(ta, 'C'), conjb = ora_lsame(tb, 'C');                               \
+    if (ora_gemm_info(ta, tb, m, n, k, lda, ldb, ldc)) return;                                \
+    if (m == 0 || n == 0 || ((alpha == 0 || k == 0) && beta == 1)) return;                    \
+    if (alpha == 0) {                                                                         \
+        for (int j = 0; j < n; ++j)                                                           \
+            for (int i = 0; i < m; ++i)                                                       \
+                C[IX(i, j, ldc)] = beta == 0 ? (CT)0 : beta * C[IX(i, j, ldc)];               \
+        return;                                                                               \
+    }                                                                                         \
+    for (int j = 0; j < n; ++j) {                                                             \
+        if (nota) {                                                                           \
+            if (beta == 0)                                                                    \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] = 0;                             \
+            else if (beta != 1)                                                               \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] *= beta;                         \
+            for (int l = 0; l < k; ++l) {                                                     \
+                CT b = notb ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)];                            \
+                if (conjb) b = conj(b);                                                       \
+                CT temp = alpha * b;                                                          \
+                for (int i = 0; i < m; ++i) C[IX(i, j, ldc)] += temp * A[IX(i, l, lda)];      \
+            }                                                                                 \
+        } else {                                                                              \
+            for (int i = 0; i < m; ++i) {                                                     \
+                CT temp = 0;                                                                  \
+                for (int l = 0; l < k; ++l) {                                                 \
+                    CT a = A[IX(l, i, lda)];                                                  \
+                    CT b = notb ? B[IX(l, j, ldb)] : B[IX(j, l, ldb)];                        \
+                    if (conja) a = conj(a);                                                   \
+                    if (conjb) b = conj(b);                                                   \
+                    temp += a * b;                                                            \
+                }                                                                             \
+                C[IX(i, j, ldc)] = beta == 0 ? alpha * temp                                   \
+                                             : alpha * temp + beta * C[IX(i, j, ldc)];        \
+            }                                                                                 \
+        }                                                                                     \
+    }                                                                                         \
+}
+DEFINE_CGEMM(ora_zgemm, double, double _Complex)
+DEFINE_CGEMM(ora_cgemm, float, float _Complex)
 
 #define DEFINE_SYRK(NAME, T)                                                                  \
 void NAME(char uplo, char trans, int n, int k, T alpha, const T *A, int lda, T beta, T *C,    \

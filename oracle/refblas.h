@@ -56,6 +56,11 @@ void ora_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double
                const double *B, int ldb, double beta, double *C, int ldc);
 void ora_sgemm(char ta, char tb, int m, int n, int k, float alpha, const float *A, int lda,
                const float *B, int ldb, float beta, float *C, int ldc);
+/* complex GEMM (netlib ZGEMM / CGEMM): operands and scalars are (re, im) pairs */
+void ora_zgemm(char ta, char tb, int m, int n, int k, const double *alpha, const double *A, int lda,
+               const double *B, int ldb, const double *beta, double *C, int ldc);
+void ora_cgemm(char ta, char tb, int m, int n, int k, const float *alpha, const float *A, int lda,
+               const float *B, int ldb, const float *beta, float *C, int ldc);
 void ora_dsyrk(char uplo, char trans, int n, int k, double alpha, const double *A, int lda,
                double beta, double *C, int ldc);
 void ora_ssyrk(char uplo, char trans, int n, int k, float alpha, const float *A, int lda,

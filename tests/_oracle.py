@@ -54,6 +54,8 @@ def load():
     L.ora_dsbmv.argtypes = [C, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_dgemm.argtypes = [C, C, I, I, I, D, VP, I, VP, I, D, VP, I]
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
+    L.ora_zgemm.argtypes = [C, C, I, I, I, VP, VP, I, VP, I, VP, VP, I]
+    L.ora_cgemm.argtypes = [C, C, I, I, I, VP, VP, I, VP, I, VP, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]
     L.ora_ssyrk.argtypes = [C, C, I, I, F, VP, I, F, VP, I]
     L.ora_dtrsm.argtypes = [C, C, C, C, I, I, D, VP, I, VP, I]

@@ -779,3 +779,63 @@ def test_drotm_vs_host_blas_and_closed_form(L, flag, incx, incy):
     hx, hy = sb.drotm(xs.copy(), ys.copy(), param)
     np.testing.assert_allclose(get(X, incx), hx, rtol=1e-15, atol=1e-16)
     np.testing.assert_allclose(get(Y, incy), hy, rtol=1e-15, atol=1e-16)
+
+
+# ---- complex GEMM (netlib ZGEMM / CGEMM) ----
+def _cplx(seed, shape, dtype=np.complex128):
+    """Deterministic complex input: real and imaginary parts from two streams."""
+    return np.asfortranarray((uniform(seed, shape) + 1j * uniform(seed + 1000, shape)).astype(dtype))
+
+
+def _op(X, t):
+    return X if t == "N" else (X.T if t == "T" else X.conj().T)
+
+
+@pytest.mark.parametrize("ta", ["N", "T", "C"])
+@pytest.mark.parametrize("tb", ["N", "T", "C"])
+@pytest.mark.parametrize("prec", ["z", "c"])
+def test_complex_gemm_vs_closed_form_and_host_blas(L, ta, tb, prec):
+    """op(A) op(B) against numpy's product of the same conjugated /
+    transposed arrays (the definition), and against the host BLAS; a form
+    whose conjugation is wrong misses by the size of the imaginary parts."""
+    dt = np.complex128 if prec == "z" else np.complex64
+    eps = np.finfo(np.float64 if prec == "z" else np.float32).eps
+    m, n, k = 7, 5, 9
+    A = _cplx(40, (m, k) if ta == "N" else (k, m), dt)
+    B = _cplx(42, (k, n) if tb == "N" else (n, k), dt)
+    C0 = _cplx(44, (m, n), dt)
+    alpha, beta = np.array([0.7 - 0.2j], dt), np.array([1.3 + 0.4j], dt)
+    R = C0.copy(order="F")
+    fn = L.ora_zgemm if prec == "z" else L.ora_cgemm
+    fn(c(ta), c(tb), m, n, k, p(alpha), p(A), A.shape[0], p(B), B.shape[0], p(beta), p(R), m)
+    oa, ob = _op(A.astype(np.complex128), ta), _op(B.astype(np.complex128), tb)
+    want = alpha[0] * (oa @ ob) + beta[0] * C0
+    gauge = abs(alpha[0]) * (np.abs(oa) @ np.abs(ob)) + abs(beta[0]) * np.abs(C0)
+    assert np.all(np.abs(R - want) <= 8 * (k + 2) * eps * gauge)
+    # the conjugation is not a no-op on these data
+    other = {"C": "T", "T": "C", "N": "N"}
+    if ta != "N" or tb != "N":
+        wrong = alpha[0] * (_op(A, other[ta]) @ _op(B, other[tb])) + beta[0] * C0
+        assert np.max(np.abs(R - wrong)) > 1e3 * 8 * (k + 2) * eps * np.max(gauge)
+    hb = (sb.zgemm if prec == "z" else sb.cgemm)(alpha[0], A, B, beta[0], C0.copy(order="F"),
+                                                  trans_a="NTC".index(ta), trans_b="NTC".index(tb))
+    assert np.all(np.abs(R - hb) <= 16 * (k + 2) * eps * gauge)
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+def test_complex_gemm_integer_exact_and_netlib_quick_paths(L, prec):
+    dt = np.complex128 if prec == "z" else np.complex64
+    fn = L.ora_zgemm if prec == "z" else L.ora_cgemm
+    rng = np.random.default_rng(45)
+    A = np.asfortranarray((rng.integers(-4, 5, (6, 8)) + 1j * rng.integers(-4, 5, (6, 8))).astype(dt))
+    B = np.asfortranarray((rng.integers(-4, 5, (6, 5)) + 1j * rng.integers(-4, 5, (6, 5))).astype(dt))
+    C = np.zeros((8, 5), dt, order="F")
+    one, zero = np.array([1], dt), np.array([0], dt)
+    fn(b"C", b"N", 8, 5, 6, p(one), p(A), 6, p(B), 6, p(zero), p(C), 8)
+    assert np.array_equal(C, A.conj().T @ B)  # every partial sum an exact integer
+    C = np.full((8, 5), np.nan, dt, order="F")
+    fn(b"C", b"N", 8, 5, 6, p(one), p(A), 6, p(B), 6, p(zero), p(C), 8)
+    assert np.all(np.isfinite(C))  # beta == 0: C is not read
+    C = np.full((8, 5), np.nan, dt, order="F")
+    fn(b"N", b"N", 8, 5, 6, p(zero), p(A), 8, p(B), 6, p(zero), p(C), 8)
+    assert np.all(C == 0)  # alpha == beta == 0: exact zeros
