@@ -75,6 +75,49 @@ void zgemm_(const char *transa, const char *transb, const int *m, const int *n, 
 void cgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
             const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
             const float *beta, float *c, const int *ldc);
+/* the rest of the complex Level-3 (syrk.cc, herk.cc, syr2k.cc, her2k.cc, symm.cc,
+ * hemm.cc, trsm.cc: F77_*(c / z); ztrmm_ / ctrmm_ as netlib): complex scalars by
+ * address as (re, im) pairs; HERK's alpha and beta and HER2K's beta are real */
+void zsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+            const double *a, const int *lda, const double *beta, double *c, const int *ldc);
+void zherk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+            const double *a, const int *lda, const double *beta, double *c, const int *ldc);
+void zsyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+             const double *a, const int *lda, const double *b, const int *ldb, const double *beta, double *c,
+             const int *ldc);
+void zher2k_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
+             const double *a, const int *lda, const double *b, const int *ldb, const double *beta, double *c,
+             const int *ldc);
+void zsymm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *a, const int *lda, const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+void zhemm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *a, const int *lda, const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+void ztrmm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const double *alpha, const double *a, const int *lda, double *b, const int *ldb);
+void ztrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const double *alpha, const double *a, const int *lda, double *b, const int *ldb);
+void csyrk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+            const float *a, const int *lda, const float *beta, float *c, const int *ldc);
+void cherk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+            const float *a, const int *lda, const float *beta, float *c, const int *ldc);
+void csyr2k_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+             const float *a, const int *lda, const float *b, const int *ldb, const float *beta, float *c,
+             const int *ldc);
+void cher2k_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,
+             const float *a, const int *lda, const float *b, const int *ldb, const float *beta, float *c,
+             const int *ldc);
+void csymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *a, const int *lda, const float *b, const int *ldb, const float *beta, float *c,
+            const int *ldc);
+void chemm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *a, const int *lda, const float *b, const int *ldb, const float *beta, float *c,
+            const int *ldc);
+void ctrmm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const float *alpha, const float *a, const int *lda, float *b, const int *ldb);
+void ctrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m,
+            const int *n, const float *alpha, const float *a, const int *lda, float *b, const int *ldb);
 void dsyrk_(const char *uplo, const char *trans, const int *n, const int *k, const double *alpha,
             const double *a, const int *lda, const double *beta, double *c, const int *ldc);
 void ssyrk_(const char *uplo, const char *trans, const int *n, const int *k, const float *alpha,

@@ -103,6 +103,55 @@ int gpublas_dev_zgemm(void *stream, char transa, char transb, int m, int n, int 
 int gpublas_dev_cgemm(void *stream, char transa, char transb, int m, int n, int k,
                       float alpha_re, float alpha_im, const float *d_A, int lda, const float *d_B,
                       int ldb, float beta_re, float beta_im, float *d_C, int ldc);
+/* the rest of the complex Level-3 (replaces the cuBLAS complex calls of
+ * syrk.cc:182-215, herk.cc:131-162, syr2k.cc:182-215, her2k.cc:141-178,
+ * symm.cc:158-207, hemm.cc:120-153, trsm.cc:175-215; ZTRMM / CTRMM follow netlib,
+ * trmm.cc being unbuilt there): interleaved (re, im) operands, ld in complex
+ * elements.  SYRK-type products run as two stacked real products on the
+ * triangular tile grid; SYMM / HEMM and TRMM expand the triangle and use the
+ * complex GEMM; TRSM is recursive with complex GEMM updates.  herm = 1 in
+ * *symm selects HEMM. */
+int gpublas_dev_zsyrk(void *stream, char uplo, char trans, int n, int k, double alpha_re,
+                      double alpha_im, const double *d_A, int lda, double beta_re, double beta_im,
+                      double *d_C, int ldc);
+int gpublas_dev_csyrk(void *stream, char uplo, char trans, int n, int k, float alpha_re,
+                      float alpha_im, const float *d_A, int lda, float beta_re, float beta_im,
+                      float *d_C, int ldc);
+int gpublas_dev_zherk(void *stream, char uplo, char trans, int n, int k, double alpha,
+                      const double *d_A, int lda, double beta, double *d_C, int ldc);
+int gpublas_dev_cherk(void *stream, char uplo, char trans, int n, int k, float alpha,
+                      const float *d_A, int lda, float beta, float *d_C, int ldc);
+int gpublas_dev_zsyr2k(void *stream, char uplo, char trans, int n, int k, double alpha_re,
+                       double alpha_im, const double *d_A, int lda, const double *d_B, int ldb,
+                       double beta_re, double beta_im, double *d_C, int ldc);
+int gpublas_dev_csyr2k(void *stream, char uplo, char trans, int n, int k, float alpha_re,
+                       float alpha_im, const float *d_A, int lda, const float *d_B, int ldb,
+                       float beta_re, float beta_im, float *d_C, int ldc);
+int gpublas_dev_zher2k(void *stream, char uplo, char trans, int n, int k, double alpha_re,
+                       double alpha_im, const double *d_A, int lda, const double *d_B, int ldb,
+                       double beta, double *d_C, int ldc);
+int gpublas_dev_cher2k(void *stream, char uplo, char trans, int n, int k, float alpha_re,
+                       float alpha_im, const float *d_A, int lda, const float *d_B, int ldb,
+                       float beta, float *d_C, int ldc);
+int gpublas_dev_zsymm(void *stream, int herm, char side, char uplo, int m, int n,
+                      double alpha_re, double alpha_im, const double *d_A, int lda,
+                      const double *d_B, int ldb, double beta_re, double beta_im, double *d_C,
+                      int ldc);
+int gpublas_dev_csymm(void *stream, int herm, char side, char uplo, int m, int n, float alpha_re,
+                      float alpha_im, const float *d_A, int lda, const float *d_B, int ldb,
+                      float beta_re, float beta_im, float *d_C, int ldc);
+int gpublas_dev_ztrmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      double alpha_re, double alpha_im, const double *d_A, int lda, double *d_B,
+                      int ldb);
+int gpublas_dev_ctrmm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      float alpha_re, float alpha_im, const float *d_A, int lda, float *d_B,
+                      int ldb);
+int gpublas_dev_ztrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      double alpha_re, double alpha_im, const double *d_A, int lda, double *d_B,
+                      int ldb);
+int gpublas_dev_ctrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                      float alpha_re, float alpha_im, const float *d_A, int lda, float *d_B,
+                      int ldb);
 int gpublas_dev_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha,
                       const double *d_A, int lda, double beta, double *d_C, int ldc);
 int gpublas_dev_ssyrk(void *stream, char uplo, char trans, int n, int k, float alpha,

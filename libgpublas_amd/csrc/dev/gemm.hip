@@ -718,6 +718,36 @@ int gpublas_gemm_s(hipStream_t s, char ta, char tb, int m, int n, int k, float a
     return gemm_entry<float>(s, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+// Shared with cplx.hip: the uplo triangle of alpha op(A) op(B) + beta C on
+// the triangular tile grid (op(A) n x k, op(B) k x n); the rest of C is not
+// touched.
+template <typename T>
+int gemm_tri(hipStream_t s, char uplo, char ta, char tb, int n, int k, T alpha, const T *A,
+             int lda, const T *B, int ldb, T beta, T *C, int ldc)
+{
+    const int tri = is_u(uplo) ? SYM_UPPER : SYM_LOWER;
+    if (n <= 0) return 0;
+    if (alpha == T(0) || k <= 0) {
+        if (beta == T(1)) return 0;
+        return launch_scale<T>(s, n, n, beta, C, ldc, tri);
+    }
+    Operand<T> a = make_op<T>(A, lda, n, is_n(ta) ? PAT_OUTER : PAT_KCONT, SYM_NONE);
+    Operand<T> b = make_op<T>(B, ldb, n, is_n(tb) ? PAT_KCONT : PAT_OUTER, SYM_NONE);
+    return launch_gemm<T>(s, n, n, k, alpha, a, b, beta, C, ldc, tri);
+}
+int gpublas_gemm_tri_d(hipStream_t s, char uplo, char ta, char tb, int n, int k, double alpha,
+                       const double *A, int lda, const double *B, int ldb, double beta, double *C,
+                       int ldc)
+{
+    return gemm_tri<double>(s, uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+int gpublas_gemm_tri_s(hipStream_t s, char uplo, char ta, char tb, int n, int k, float alpha,
+                       const float *A, int lda, const float *B, int ldb, float beta, float *C,
+                       int ldc)
+{
+    return gemm_tri<float>(s, uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 extern "C" {
 
 int gpublas_dev_dgemm(void *stream, char transa, char transb, int m, int n, int k, double alpha,

@@ -1,6 +1,7 @@
 /*
  * level3.c -- Fortran-ABI Level-3 entry points: ?gemm_, ?syrk_, ?syr2k_, ?trmm_, ?trsm_,
- * ?symm_ for real single and double precision, and zgemm_ / cgemm_.
+ * ?symm_ for real single and double precision, and the complex Level-3 (z / c: gemm,
+ * syrk, herk, syr2k, her2k, symm, hemm, trmm, trsm).
  *
  * Each entry follows the reference's three steps (blas_level3/gemm.cc:87-179
  * and the syrk/trsm/symm twins): netlib argument check with xerbla, quick
@@ -569,6 +570,246 @@ DEFINE_TRSM(s, float, "STRSM ", gpublas_dev_strsm, no_serve_strsm, GB_RK_STRSM, 
 
 DEFINE_SYMM(d, double, "DSYMM ", gpublas_dev_dsymm, GB_RK_DSYMM)
 DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm, GB_RK_SSYMM)
+
+/* ------------------------------------------------------------------ */
+/* complex SYRK / HERK / SYR2K / HER2K / SYMM / HEMM / TRMM / TRSM       */
+/* (syrk.cc, herk.cc, syr2k.cc, her2k.cc, symm.cc, hemm.cc, trsm.cc of   */
+/* the reference; netlib semantics; trmm.cc is not built there).         */
+/* Scalars are (re, im) pairs, except the real alpha / beta of HERK and  */
+/* the real beta of HER2K.  A call that only scales C (alpha = 0 or k =  */
+/* 0) runs the scaling kernel on the device.                             */
+/* ------------------------------------------------------------------ */
+#define CZERO(p) ((p)[0] == 0 && (p)[1] == 0)
+#define CONE(p) ((p)[0] == 1 && (p)[1] == 0)
+
+#define GB_CPLX_HOST(PRE, NAME, ARGS_T, ARGS)                                                    \
+    do {                                                                                         \
+        typedef void(*fn_t) ARGS_T;                                                              \
+        fn_t f = (fn_t)gb_host_fn(#PRE #NAME "_");                                               \
+        GB_COUNT(fallthrough, 1);                                                                \
+        f ARGS;                                                                                  \
+        gb_leave();                                                                              \
+        return;                                                                                  \
+    } while (0)
+
+/* RK: SYRK-like (one operand) or SYR2K-like (two); HERM: real alpha/beta (HERK) or real beta
+ * (HER2K) */
+#define DEFINE_CRK(PRE, NAME, R, NAME6, HERM, TWO, RK, AL_T, BE_T, DEVCALL)                       \
+    void PRE##NAME##_(const char *uplo, const char *trans, const int *n, const int *k,           \
+                      const AL_T *alpha, const R *a, const int *lda TWO##_PARAM(R),              \
+                      const BE_T *beta, R *c, const int *ldc)                                    \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int upper = gb_lsame(*uplo, 'U'), nt = gb_lsame(*trans, 'N');                     \
+        const int nrowa = nt ? *n : *k;                                                          \
+        int info = 0;                                                                            \
+        if (!upper && !gb_lsame(*uplo, 'L')) info = 1;                                           \
+        else if (!nt && !gb_lsame(*trans, HERM ? 'C' : 'T')) info = 2;                           \
+        else if (*n < 0) info = 3;                                                               \
+        else if (*k < 0) info = 4;                                                               \
+        else if (*lda < MAXI(1, nrowa)) info = 7;                                                \
+        TWO##_LDB_CHECK                                                                          \
+        else if (*ldc < MAXI(1, *n)) info = TWO##_LDC_INFO;                                      \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int a0 = AL_T##_ZERO(alpha), b1 = BE_T##_ONE(beta);                                \
+        if (*n == 0 || ((a0 || *k == 0) && b1)) {                                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int cut = gb_opt.cutoff;                                                           \
+        const double es_ = 2 * sizeof(R), nn_ = *n, kk_ = *k;                                    \
+        const int scale_only = a0 || *k == 0;                                                    \
+        if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldc < cut), 4 * (TWO##_F) * nn_ * nn_ * kk_, \
+                          es_ * ((TWO##_F) * nn_ * kk_ + 2 * nn_ * nn_), a, NULL, c))            \
+            GB_CPLX_HOST(PRE, NAME,                                                              \
+                         (const char *, const char *, const int *, const int *, const AL_T *,    \
+                          const R *, const int *TWO##_PT(R), const BE_T *, R *, const int *),    \
+                         (uplo, trans, n, k, alpha, a, lda TWO##_PA, beta, c, ldc));             \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (!scale_only) {                                                                       \
+            gb_acquire_matrix(&ops[0], a, *lda, nrowa, nt ? *k : *n, 2 * sizeof(R), GB_IN, s);   \
+            TWO##_ACQ(R)                                                                         \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *n, *n, 2 * sizeof(R), GB_INOUT, s);                 \
+        gb_check(DEVCALL, #PRE #NAME " launch");                                                 \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+/* the pieces that differ between the one- and two-operand forms */
+#define ONE_PARAM(R)
+#define TWO_PARAM(R) , const R *b, const int *ldb
+#define ONE_PT(R)
+#define TWO_PT(R) , const R *, const int *
+#define ONE_PA
+#define TWO_PA , b, ldb
+#define ONE_LDB_CHECK
+#define TWO_LDB_CHECK else if (*ldb < MAXI(1, nrowa)) info = 9;
+#define ONE_LDC_INFO 10
+#define TWO_LDC_INFO 12
+#define ONE_F 1
+#define TWO_F 2
+#define ONE_ACQ(R)
+#define TWO_ACQ(R) gb_acquire_matrix(&ops[1], b, *ldb, nrowa, nt ? *k : *n, 2 * sizeof(R), GB_IN, s);
+#define double_ZERO(p) (*(p) == 0)
+#define double_ONE(p) (*(p) == 1)
+#define float_ZERO(p) (*(p) == 0)
+#define float_ONE(p) (*(p) == 1)
+/* complex scalars are spelled CD / CF here: (re, im) pairs */
+typedef double CD;
+typedef float CF;
+#define CD_ZERO(p) CZERO(p)
+#define CD_ONE(p) CONE(p)
+#define CF_ZERO(p) CZERO(p)
+#define CF_ONE(p) CONE(p)
+
+#define DEV_A ((const R_ *)ops[0].dev)
+#define DEV_B ((const R_ *)ops[1].dev)
+#define DEV_C ((R_ *)ops[2].dev)
+#define R_ double
+DEFINE_CRK(z, syrk, double, "ZSYRK ", 0, ONE, GB_RK_DSYRK, CD, CD,
+           gpublas_dev_zsyrk(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, beta[0],
+                             beta[1], DEV_C, *ldc))
+DEFINE_CRK(z, herk, double, "ZHERK ", 1, ONE, GB_RK_DSYRK, double, double,
+           gpublas_dev_zherk(s, *uplo, *trans, *n, *k, *alpha, DEV_A, *lda, *beta, DEV_C, *ldc))
+DEFINE_CRK(z, syr2k, double, "ZSYR2K", 0, TWO, GB_RK_DSYR2K, CD, CD,
+           gpublas_dev_zsyr2k(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, DEV_B,
+                              *ldb, beta[0], beta[1], DEV_C, *ldc))
+DEFINE_CRK(z, her2k, double, "ZHER2K", 1, TWO, GB_RK_DSYR2K, CD, double,
+           gpublas_dev_zher2k(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, DEV_B,
+                              *ldb, *beta, DEV_C, *ldc))
+#undef R_
+#define R_ float
+DEFINE_CRK(c, syrk, float, "CSYRK ", 0, ONE, GB_RK_SSYRK, CF, CF,
+           gpublas_dev_csyrk(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, beta[0],
+                             beta[1], DEV_C, *ldc))
+DEFINE_CRK(c, herk, float, "CHERK ", 1, ONE, GB_RK_SSYRK, float, float,
+           gpublas_dev_cherk(s, *uplo, *trans, *n, *k, *alpha, DEV_A, *lda, *beta, DEV_C, *ldc))
+DEFINE_CRK(c, syr2k, float, "CSYR2K", 0, TWO, GB_RK_SSYR2K, CF, CF,
+           gpublas_dev_csyr2k(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, DEV_B,
+                              *ldb, beta[0], beta[1], DEV_C, *ldc))
+DEFINE_CRK(c, her2k, float, "CHER2K", 1, TWO, GB_RK_SSYR2K, CF, float,
+           gpublas_dev_cher2k(s, *uplo, *trans, *n, *k, alpha[0], alpha[1], DEV_A, *lda, DEV_B,
+                              *ldb, *beta, DEV_C, *ldc))
+#undef R_
+
+#define DEFINE_CSYMM(PRE, NAME, R, NAME6, HERM, RK, DEVFN)                                        \
+    void PRE##NAME##_(const char *side, const char *uplo, const int *m, const int *n,            \
+                      const R *alpha, const R *a, const int *lda, const R *b, const int *ldb,    \
+                      const R *beta, R *c, const int *ldc)                                       \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int left = gb_lsame(*side, 'L'), upper = gb_lsame(*uplo, 'U');                     \
+        const int na = left ? *m : *n;                                                           \
+        int info = 0;                                                                            \
+        if (!left && !gb_lsame(*side, 'R')) info = 1;                                            \
+        else if (!upper && !gb_lsame(*uplo, 'L')) info = 2;                                      \
+        else if (*m < 0) info = 3;                                                               \
+        else if (*n < 0) info = 4;                                                               \
+        else if (*lda < MAXI(1, na)) info = 7;                                                   \
+        else if (*ldb < MAXI(1, *m)) info = 9;                                                   \
+        else if (*ldc < MAXI(1, *m)) info = 12;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0 || (CZERO(alpha) && CONE(beta))) {                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int cut = gb_opt.cutoff;                                                           \
+        const double es_ = 2 * sizeof(R), mm_ = *m, nn_ = *n, aa_ = na;                          \
+        if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldb < cut || *ldc < cut),             \
+                          8 * mm_ * nn_ * aa_, es_ * (aa_ * aa_ + 3 * mm_ * nn_), a, b, c))      \
+            GB_CPLX_HOST(PRE, NAME,                                                              \
+                         (const char *, const char *, const int *, const int *, const R *,       \
+                          const R *, const int *, const R *, const int *, const R *, R *,        \
+                          const int *),                                                          \
+                         (side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc));               \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[3] = {{0}};                                                               \
+        if (!CZERO(alpha)) {                                                                     \
+            gb_acquire_matrix(&ops[0], a, *lda, na, na, 2 * sizeof(R), GB_IN, s);                \
+            gb_acquire_matrix(&ops[1], b, *ldb, *m, *n, 2 * sizeof(R), GB_IN, s);                \
+        }                                                                                        \
+        gb_acquire_matrix(&ops[2], c, *ldc, *m, *n, 2 * sizeof(R),                               \
+                          CZERO(beta) ? GB_OUT : GB_INOUT, s);                                   \
+        gb_check(DEVFN(s, HERM, *side, *uplo, *m, *n, alpha[0], alpha[1], (const R *)ops[0].dev, \
+                       *lda, (const R *)ops[1].dev, *ldb, beta[0], beta[1], (R *)ops[2].dev,     \
+                       *ldc),                                                                    \
+                 #PRE #NAME " launch");                                                          \
+        gb_finish(s, ops, 3);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+DEFINE_CSYMM(z, symm, double, "ZSYMM ", 0, GB_RK_DSYMM, gpublas_dev_zsymm)
+DEFINE_CSYMM(z, hemm, double, "ZHEMM ", 1, GB_RK_DSYMM, gpublas_dev_zsymm)
+DEFINE_CSYMM(c, symm, float, "CSYMM ", 0, GB_RK_SSYMM, gpublas_dev_csymm)
+DEFINE_CSYMM(c, hemm, float, "CHEMM ", 1, GB_RK_SSYMM, gpublas_dev_csymm)
+
+#define DEFINE_CTRI(PRE, NAME, R, NAME6, RK, DEVFN)                                               \
+    void PRE##NAME##_(const char *side, const char *uplo, const char *transa, const char *diag,  \
+                      const int *m, const int *n, const R *alpha, const R *a, const int *lda,    \
+                      R *b, const int *ldb)                                                      \
+    {                                                                                            \
+        gb_enter();                                                                              \
+        GB_COUNT(calls, 1);                                                                      \
+        const int left = gb_lsame(*side, 'L'), upper = gb_lsame(*uplo, 'U');                     \
+        const int na = left ? *m : *n;                                                           \
+        int info = 0;                                                                            \
+        if (!left && !gb_lsame(*side, 'R')) info = 1;                                            \
+        else if (!upper && !gb_lsame(*uplo, 'L')) info = 2;                                      \
+        else if (!gb_lsame(*transa, 'N') && !gb_lsame(*transa, 'T') && !gb_lsame(*transa, 'C')) \
+            info = 3;                                                                            \
+        else if (!gb_lsame(*diag, 'U') && !gb_lsame(*diag, 'N')) info = 4;                       \
+        else if (*m < 0) info = 5;                                                               \
+        else if (*n < 0) info = 6;                                                               \
+        else if (*lda < MAXI(1, na)) info = 9;                                                   \
+        else if (*ldb < MAXI(1, *m)) info = 11;                                                  \
+        if (info) {                                                                              \
+            gb_xerbla(NAME6, info);                                                              \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        if (*m == 0 || *n == 0) {                                                                \
+            gb_leave();                                                                          \
+            return;                                                                              \
+        }                                                                                        \
+        const int cut = gb_opt.cutoff;                                                           \
+        const double es_ = 2 * sizeof(R), mm_ = *m, nn_ = *n, aa_ = na;                          \
+        if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldb < cut), 4 * mm_ * nn_ * aa_,       \
+                          es_ * (aa_ * aa_ / 2 + 2 * mm_ * nn_), a, b, NULL))                    \
+            GB_CPLX_HOST(PRE, NAME,                                                              \
+                         (const char *, const char *, const char *, const char *, const int *,   \
+                          const int *, const R *, const R *, const int *, R *, const int *),     \
+                         (side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb));               \
+        gb_require_device();                                                                     \
+        void *s = gpublas_dev_stream();                                                          \
+        gb_operand ops[2] = {{0}};                                                               \
+        if (!CZERO(alpha)) gb_acquire_matrix(&ops[0], a, *lda, na, na, 2 * sizeof(R), GB_IN, s); \
+        gb_acquire_matrix(&ops[1], b, *ldb, *m, *n, 2 * sizeof(R),                               \
+                          CZERO(alpha) ? GB_OUT : GB_INOUT, s);                                  \
+        gb_check(DEVFN(s, *side, *uplo, *transa, *diag, *m, *n, alpha[0], alpha[1],              \
+                       (const R *)ops[0].dev, *lda, (R *)ops[1].dev, *ldb),                      \
+                 #PRE #NAME " launch");                                                          \
+        gb_finish(s, ops, 2);                                                                    \
+        GB_COUNT(device_calls, 1);                                                               \
+        gb_leave();                                                                              \
+    }
+DEFINE_CTRI(z, trsm, double, "ZTRSM ", GB_RK_DTRSM, gpublas_dev_ztrsm)
+DEFINE_CTRI(c, trsm, float, "CTRSM ", GB_RK_STRSM, gpublas_dev_ctrsm)
+DEFINE_CTRI(z, trmm, double, "ZTRMM ", GB_RK_DTRMM, gpublas_dev_ztrmm)
+DEFINE_CTRI(c, trmm, float, "CTRMM ", GB_RK_STRMM, gpublas_dev_ctrmm)
 
 /* Library-internal names of the entry points above, bound inside the library
  * (the crossover measurement calls them; a process may hold another BLAS

@@ -150,3 +150,129 @@ def test_cblas_zgemm_row_major(gpu, ora):
     assert np.all(np.abs(C - R) <= 16 * (k + 2) * eps * G)
     want = alpha[0] * (A @ B.conj().T) + beta[0] * C0
     assert np.all(np.abs(C - want) <= 16 * (k + 2) * eps * G)
+
+
+# ---- the rest of the complex Level-3, at shapes past the netlib testers' N <= 9 ----
+def _tri(X, uplo):
+    return np.triu(X) if uplo == "U" else np.tril(X)
+
+
+def _herm_full(A, uplo, herm):
+    """The full matrix a SYMM / HEMM reads from the stored triangle."""
+    T = _tri(A, uplo)
+    off = T - np.diag(np.diag(T))
+    F = T + (off.conj().T if herm else off.T)
+    if herm:
+        F[np.diag_indices(len(F))] = F.diagonal().real  # a Hermitian diagonal is real
+    return F
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("kind", ["syrk", "herk", "syr2k", "her2k"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "X"])  # X: T for syrk / syr2k, C for herk / her2k
+def test_complex_rank_k_updates(gpu, prec, kind, uplo, trans):
+    dt = np.complex128 if prec == "z" else np.complex64
+    eps = np.finfo(np.float64 if prec == "z" else np.float32).eps
+    herm = kind.startswith("he")
+    tr = "N" if trans == "N" else ("C" if herm else "T")
+    n, k = 200, 150
+    shape = (n, k) if tr == "N" else (k, n)
+    A, B = _cplx(70, shape, dt), _cplx(72, shape, dt)
+    C0 = _cplx(74, (n, n), dt)
+    alpha = 0.6 if kind == "herk" else 0.6 - 0.7j
+    beta = 0.8 if herm else 0.8 + 0.3j
+    C = C0.copy(order="F")
+    fn = getattr(gpu.library(), prec + kind + "_")
+    rdt = np.float64 if prec == "z" else np.float32
+    al = np.array([alpha], rdt) if kind == "herk" else np.array([alpha], dt)
+    be = np.array([beta], rdt) if herm else np.array([beta], dt)
+    before = gpu.stats()["device_calls"]
+    args = [gpu._c(uplo), gpu._c(tr), gpu._i(n), gpu._i(k), p(al), p(A), gpu._i(shape[0])]
+    if kind.endswith("2k"):
+        args += [p(B), gpu._i(shape[0])]
+    fn(*args, p(be), p(C), gpu._i(n))
+    assert gpu.stats()["device_calls"] == before + 1
+    X = A.astype(np.complex128) if tr == "N" else (A.T if tr == "T" else A.conj().T).astype(np.complex128)
+    Y = B.astype(np.complex128) if tr == "N" else (B.T if tr == "T" else B.conj().T).astype(np.complex128)
+    if kind == "syrk":
+        P, G = alpha * X @ X.T, abs(alpha) * np.abs(X) @ np.abs(X).T
+    elif kind == "herk":
+        P, G = alpha * X @ X.conj().T, abs(alpha) * np.abs(X) @ np.abs(X).T
+    elif kind == "syr2k":
+        P = alpha * X @ Y.T + alpha * Y @ X.T
+        G = abs(alpha) * (np.abs(X) @ np.abs(Y).T + np.abs(Y) @ np.abs(X).T)
+    else:
+        P = alpha * X @ Y.conj().T + np.conj(alpha) * Y @ X.conj().T
+        G = abs(alpha) * (np.abs(X) @ np.abs(Y).T + np.abs(Y) @ np.abs(X).T)
+    C0d = C0.astype(np.complex128)
+    if herm:
+        C0d = C0d - 1j * np.diag(np.diag(C0d).imag)  # the diagonal's imaginary part is not read
+    want = P + beta * C0d
+    G = G + abs(beta) * np.abs(C0)
+    m = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    assert np.all(np.abs(C - want)[m] <= 16 * (k + 2) * eps * G[m])
+    assert np.array_equal(C[~m], C0[~m])  # the other triangle untouched
+    if herm:
+        assert np.all(np.diag(C).imag == 0)
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("herm", [0, 1])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_complex_symm_hemm(gpu, prec, herm, side, uplo):
+    dt = np.complex128 if prec == "z" else np.complex64
+    eps = np.finfo(np.float64 if prec == "z" else np.float32).eps
+    m, n = 150, 170
+    na = m if side == "L" else n
+    A = _cplx(76, (na, na), dt)
+    B, C0 = _cplx(77, (m, n), dt), _cplx(78, (m, n), dt)
+    alpha, beta = 0.9 + 0.2j, -0.4 + 0.6j
+    C = C0.copy(order="F")
+    fn = getattr(gpu.library(), prec + ("hemm_" if herm else "symm_"))
+    al, be = np.array([alpha], dt), np.array([beta], dt)
+    fn(gpu._c(side), gpu._c(uplo), gpu._i(m), gpu._i(n), p(al), p(A), gpu._i(na), p(B),
+       gpu._i(m), p(be), p(C), gpu._i(m))
+    F = _herm_full(A.astype(np.complex128), uplo, herm)
+    P = F @ B if side == "L" else B @ F
+    G = abs(alpha) * (np.abs(F) @ np.abs(B) if side == "L" else np.abs(B) @ np.abs(F)) + abs(beta) * np.abs(C0)
+    assert np.all(np.abs(C - (alpha * P + beta * C0)) <= 16 * (na + 2) * eps * G)
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("ta", ["N", "T", "C"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_complex_trsm_trmm(gpu, prec, side, uplo, ta, diag):
+    """TRSM at orders past the substitution base (recursive halving with
+    complex GEMM updates): the residual op(A) X - alpha B against the
+    triangular-solve error bound; TRMM against the product of the expanded
+    triangle."""
+    dt = np.complex128 if prec == "z" else np.complex64
+    eps = np.finfo(np.float64 if prec == "z" else np.float32).eps
+    m, n = 230, 170
+    na = m if side == "L" else n
+    A = np.asfortranarray(_cplx(80, (na, na), dt) / na)  # small off-diagonal: well conditioned
+    A[np.diag_indices(na)] = 1.0 + 0.5j  # (a unit diagonal is not read)
+    B0 = _cplx(81, (m, n), dt)
+    alpha = 0.8 - 0.5j
+    al = np.array([alpha], dt)
+    T = _tri(A.astype(np.complex128), uplo)
+    if diag == "U":
+        T = T - np.diag(np.diag(T)) + np.eye(na)
+    opT = T if ta == "N" else (T.T if ta == "T" else T.conj().T)
+    lib = gpu.library()
+    args = lambda Bx: (gpu._c(side), gpu._c(uplo), gpu._c(ta), gpu._c(diag), gpu._i(m), gpu._i(n),
+                       p(al), p(A), gpu._i(na), p(Bx), gpu._i(m))
+    X = B0.copy(order="F")
+    getattr(lib, prec + "trsm_")(*args(X))
+    R = opT @ X - alpha * B0 if side == "L" else X @ opT - alpha * B0
+    scale = np.linalg.norm(opT) * np.linalg.norm(X) + abs(alpha) * np.linalg.norm(B0)
+    assert np.linalg.norm(R) <= 8 * na * eps * scale
+    Y = B0.copy(order="F")
+    getattr(lib, prec + "trmm_")(*args(Y))
+    want = alpha * (opT @ B0 if side == "L" else B0 @ opT)
+    G = abs(alpha) * (np.abs(opT) @ np.abs(B0) if side == "L" else np.abs(B0) @ np.abs(opT))
+    assert np.all(np.abs(Y - want) <= 16 * (na + 2) * eps * G)
