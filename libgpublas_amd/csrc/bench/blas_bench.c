@@ -4,7 +4,8 @@
  *
  *   blas_bench --lib libgpublas.so --op gemm --prec d --m M --n N --k K --ta N --tb N
  *                                    (--prec s, z or c: SGEMM, ZGEMM, CGEMM; complex flops
- *                                     counted as 8mnk real flops)
+ *                                     counted as 8mnk real flops; syrk, syr2k, trsm, trmm,
+ *                                     symm take z / c as well, plus herk, her2k, hemm)
  *   blas_bench --lib ... --op syrk --n N --k K --uplo L --trans N
  *   blas_bench --lib ... --op trsm --m M --n N --side L --uplo L --ta N --diag N
  *   blas_bench --lib ... --op symm --m M --n N --side L --uplo L
@@ -87,6 +88,16 @@ static void *operand(size_t elems, int prec, uint64_t seed)
     else g_fill32(NULL, seed, (int64_t)elems, d);
     g_sync(NULL);
     return h;
+}
+
+/* "<prec><op>_": the Fortran entry of a routine in the chosen precision */
+static const char *fname(int prec, const char *op)
+{
+    static char buf[8][16];
+    static int i;
+    i = (i + 1) & 7;
+    snprintf(buf[i], sizeof buf[i], "%c%s_", prec, op);
+    return buf[i];
 }
 
 int main(int argc, char **argv)
@@ -241,6 +252,7 @@ int main(int argc, char **argv)
                            const double *, const int *, double *, const int *);
     tbmv_t tbmv = NULL;
     int bw = 0; /* band array rows */
+    const double cx = (prec == 'z' || prec == 'c') ? 4.0 : 1.0; /* real flops per FMA over 2 */
     if (!strcmp(op, "gemm")) {
         gemm = (gemm_t)sym(prec == 'd' ? "dgemm_" : prec == 's' ? "sgemm_" : prec == 'z' ? "zgemm_" : "cgemm_");
         lda = (ta == 'N' || ta == 'n') ? m : k;
@@ -250,15 +262,15 @@ int main(int argc, char **argv)
         B = operand((size_t)ldb * ((tb == 'N' || tb == 'n') ? n : k), prec, seed + 1);
         C = operand((size_t)m * n, prec, seed + 2);
         flops = (prec == 'z' || prec == 'c' ? 8.0 : 2.0) * m * n * k; /* real flops */
-    } else if (!strcmp(op, "syrk")) {
-        syrk = (syrk_t)sym(prec == 'd' ? "dsyrk_" : "ssyrk_");
+    } else if (!strcmp(op, "syrk") || !strcmp(op, "herk")) {
+        syrk = (syrk_t)sym(fname(prec, op));
         lda = (trans == 'N' || trans == 'n') ? n : k;
         ldc = n;
         A = operand((size_t)lda * ((trans == 'N' || trans == 'n') ? k : n), prec, seed);
         C = operand((size_t)n * n, prec, seed + 2);
-        flops = (double)n * (n + 1) * k;
+        flops = cx * n * (n + 1) * k;
     } else if (!strcmp(op, "trsm")) {
-        trsm = (trsm_t)sym(prec == 'd' ? "dtrsm_" : "strsm_");
+        trsm = (trsm_t)sym(fname(prec, "trsm"));
         const int na = (side == 'L' || side == 'l') ? m : n;
         lda = na;
         ldb = m;
@@ -269,35 +281,36 @@ int main(int argc, char **argv)
         for (int i = 0; i < na; ++i) {
             s = s * 6364136223846793005ull + 1442695040888963407ull;
             double u = (double)(s >> 11) * 0x1.0p-53;
-            if (prec == 'd') ((double *)A)[(size_t)i * na + i] = na + u;
-            else ((float *)A)[(size_t)i * na + i] = (float)(na + u);
+            const size_t d = (size_t)i * na + i, e = (prec == 'z' || prec == 'c') ? 2 * d : d;
+            if (prec == 'd' || prec == 'z') ((double *)A)[e] = na + u;
+            else ((float *)A)[e] = (float)(na + u);
         }
-        flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
-    } else if (!strcmp(op, "syr2k")) {
-        syr2k = (syr2k_t)sym(prec == 'd' ? "dsyr2k_" : "ssyr2k_");
+        flops = cx * ((side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m);
+    } else if (!strcmp(op, "syr2k") || !strcmp(op, "her2k")) {
+        syr2k = (syr2k_t)sym(fname(prec, op));
         lda = ldb = (trans == 'N' || trans == 'n') ? n : k;
         ldc = n;
         A = operand((size_t)lda * ((trans == 'N' || trans == 'n') ? k : n), prec, seed);
         B = operand((size_t)ldb * ((trans == 'N' || trans == 'n') ? k : n), prec, seed + 1);
         C = operand((size_t)n * n, prec, seed + 2);
-        flops = 2.0 * n * (n + 1) * k;
+        flops = cx * 2.0 * n * (n + 1) * k;
     } else if (!strcmp(op, "trmm")) {
-        trmm = (trsm_t)sym(prec == 'd' ? "dtrmm_" : "strmm_");
+        trmm = (trsm_t)sym(fname(prec, "trmm"));
         const int na = (side == 'L' || side == 'l') ? m : n;
         lda = na;
         ldb = m;
         A = operand((size_t)na * na, prec, seed);
         B = operand((size_t)m * n, prec, seed + 1);
-        flops = (side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m;
-    } else if (!strcmp(op, "symm")) {
-        symm = (symm_t)sym(prec == 'd' ? "dsymm_" : "ssymm_");
+        flops = cx * ((side == 'L' || side == 'l') ? (double)m * m * n : (double)n * n * m);
+    } else if (!strcmp(op, "symm") || !strcmp(op, "hemm")) {
+        symm = (symm_t)sym(fname(prec, op));
         const int na = (side == 'L' || side == 'l') ? m : n;
         lda = na;
         ldb = ldc = m;
         A = operand((size_t)na * na, prec, seed);
         B = operand((size_t)m * n, prec, seed + 1);
         C = operand((size_t)m * n, prec, seed + 2);
-        flops = 2.0 * m * n * na;
+        flops = cx * 2.0 * m * n * na;
     } else if (!strcmp(op, "gemv")) {
         gemv = (gemv_t)sym("dgemv_");
         A = operand((size_t)m * n, 'd', seed);

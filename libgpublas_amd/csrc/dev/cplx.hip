@@ -468,11 +468,14 @@ int ctrmm_entry(void *stream, char side, char uplo, char transa, char diag, int 
 
 // ---- complex TRSM (trsm.cc:175-215 of the reference; netlib ZTRSM) -------
 // Recursive halving with the complex GEMM for the off-diagonal updates; a
-// triangle of order <= CT_BASE is solved by substitution, one thread per
-// right-hand side (left side: a column of B; right side: a row), in the
-// netlib order (x_i = (b_i - sum of the solved terms) / a_ii).  op(A)(r, c)
-// reads A(r, c) ('N'), A(c, r) ('T') or conj(A(c, r)) ('C').
+// triangle of order <= CT_BASE is solved by substitution in the netlib order
+// (x_t = (b_t - sum of the solved terms) / a_tt), one thread per right-hand
+// side (left side: a column of B; right side: a row) holding its CT_BASE
+// unknowns in registers, op(A) in LDS (padded with the identity up to
+// CT_BASE, so a smaller triangle solves the same way with zeros after it).
+// op(A)(r, c) reads A(r, c) ('N'), A(c, r) ('T') or conj(A(c, r)) ('C').
 constexpr int CT_BASE = 32;
+constexpr int CT_THREADS = 128;
 
 template <typename T>
 __device__ __forceinline__ void opa(const T *A, long lda, int tr, long r, long c, T &re, T &im)
@@ -482,46 +485,67 @@ __device__ __forceinline__ void opa(const T *A, long lda, int tr, long r, long c
     im = tr == 2 ? -A[2 * e + 1] : A[2 * e + 1];
 }
 
-// op(A) (nb x nb, "lower" in solve order when fwd) X = alpha B (left) or
-// X op(A) = alpha B (right) for nrhs right-hand sides of B
-template <typename T>
-__global__ __launch_bounds__(64) void ctrsm_base(int nb, int nrhs, int left, int fwd, int tr,
-                                                 int unit, T ar, T ai, const T *A, long lda, T *B,
-                                                 long ldb)
+// op(A) (nb x nb, lower in solve order when FWD) X = alpha B (LEFT) or
+// X op(A) = alpha B, for nrhs right-hand sides of B
+template <typename T, bool LEFT, bool FWD>
+__global__ __launch_bounds__(CT_THREADS) void ctrsm_base(int nb, int nrhs, int tr, int unit, T ar,
+                                                         T ai, const T *A, long lda, T *B,
+                                                         long ldb)
 {
-    const long r = (long)blockIdx.x * 64 + threadIdx.x;  // the right-hand side
-    if (r >= nrhs) return;
-    // element t of this right-hand side
-    auto at = [&](int t) -> T * { return left ? B + 2 * (t + r * ldb) : B + 2 * (r + (long)t * ldb); };
-    for (int t = 0; t < nb; ++t) {
-        T *x = at(t);
-        const T xr = x[0], xi = x[1];
-        x[0] = ar * xr - ai * xi;
-        x[1] = ar * xi + ai * xr;
+    __shared__ T Sr[CT_BASE][CT_BASE + 1], Si[CT_BASE][CT_BASE + 1];  // op(A)(r, c) at [r][c]
+    for (int e = threadIdx.x; e < CT_BASE * CT_BASE; e += CT_THREADS) {
+        const int r = e % CT_BASE, c = e / CT_BASE;
+        T re = r == c ? T(1) : T(0), im = T(0);
+        if (r < nb && c < nb && !(unit && r == c)) opa(A, lda, tr, r, c, re, im);
+        Sr[r][c] = re;
+        Si[r][c] = im;
     }
-    for (int q = 0; q < nb; ++q) {
-        const int t = fwd ? q : nb - 1 - q;
-        T *x = at(t);
-        T sr = x[0], si = x[1];
-        for (int u = 0; u < q; ++u) {
-            const int v = fwd ? u : nb - 1 - u;  // solved before t
-            const T *y = at(v);
-            T cr, ci;
-            if (left) opa(A, lda, tr, t, v, cr, ci);   // op(A)(t, v) x_v
-            else opa(A, lda, tr, v, t, cr, ci);        // x_v op(A)(v, t)
-            sr -= cr * y[0] - ci * y[1];
-            si -= cr * y[1] + ci * y[0];
+    __syncthreads();
+    const long q = (long)blockIdx.x * CT_THREADS + threadIdx.x;  // the right-hand side
+    if (q >= nrhs) return;
+    // register t holds unknown row(t); the solve always walks t downwards, so
+    // a forward solve keeps its unknowns in reverse order (one loop shape for
+    // both directions)
+    auto row = [](int t) { return FWD ? CT_BASE - 1 - t : t; };
+    T xr[CT_BASE], xi[CT_BASE];
+#pragma unroll
+    for (int t = 0; t < CT_BASE; ++t) {
+        const int r = row(t);
+        T br = T(0), bi = T(0);
+        if (r < nb) {
+            const T *b = LEFT ? B + 2 * (r + q * ldb) : B + 2 * (q + (long)r * ldb);
+            br = b[0];
+            bi = b[1];
         }
-        if (!unit) {
-            T dr, di;
-            opa(A, lda, tr, t, t, dr, di);
-            const T den = dr * dr + di * di;  // (s / d) = s conj(d) / |d|^2
-            const T qr = (sr * dr + si * di) / den, qi = (si * dr - sr * di) / den;
-            sr = qr;
-            si = qi;
+        xr[t] = ar * br - ai * bi;
+        xi[t] = ar * bi + ai * br;
+    }
+#pragma unroll
+    for (int s = 0; s < CT_BASE; ++s) {
+        const int t = CT_BASE - 1 - s, rt = row(t);
+        T sr = xr[t], si = xi[t];
+#pragma unroll
+        for (int u = 0; u < s; ++u) {
+            const int v = CT_BASE - 1 - u, rv = row(v);  // solved before t
+            const T cr = LEFT ? Sr[rt][rv] : Sr[rv][rt], ci = LEFT ? Si[rt][rv] : Si[rv][rt];
+            sr -= cr * xr[v] - ci * xi[v];
+            si -= cr * xi[v] + ci * xr[v];
+            if ((u & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 8 coefficients at a time
         }
-        x[0] = sr;
-        x[1] = si;
+        const T dr = Sr[rt][rt], di = Si[rt][rt];  // s / d = s conj(d) / |d|^2 (1 on a unit diagonal)
+        const T den = dr * dr + di * di;
+        xr[t] = (sr * dr + si * di) / den;
+        xi[t] = (si * dr - sr * di) / den;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t < CT_BASE; ++t) {
+        const int r = row(t);
+        if (r < nb) {
+            T *b = LEFT ? B + 2 * (r + q * ldb) : B + 2 * (q + (long)r * ldb);
+            b[0] = xr[t];
+            b[1] = xi[t];
+        }
     }
 }
 
@@ -536,9 +560,19 @@ int ctrsm_rec(hipStream_t s, bool left, bool lower, char tr, bool unit, int m, i
     const int trc = (tr == 'N' || tr == 'n') ? 0 : ((tr == 'T' || tr == 't') ? 1 : 2);
     if (na <= CT_BASE) {
         const int nrhs = left ? n : m;
-        hipLaunchKernelGGL(ctrsm_base<T>, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, s, na,
-                           nrhs, left ? 1 : 0, (left == lower) ? 1 : 0, trc, unit ? 1 : 0, ar, ai,
-                           A, (long)lda, B, (long)ldb);
+        const dim3 grid((unsigned)((nrhs + CT_THREADS - 1) / CT_THREADS));
+#define GB_CTB(L, F)                                                                             \
+    hipLaunchKernelGGL((ctrsm_base<T, L, F>), grid, dim3(CT_THREADS), 0, s, na, nrhs, trc,       \
+                       unit ? 1 : 0, ar, ai, A, (long)lda, B, (long)ldb)
+        // forward order for a lower op(A) on the left and an upper one on the right
+        if (left) {
+            if (lower) GB_CTB(true, true);
+            else GB_CTB(true, false);
+        } else {
+            if (lower) GB_CTB(false, false);
+            else GB_CTB(false, true);
+        }
+#undef GB_CTB
         return (int)hipGetLastError();
     }
     const int n1 = na / 2 / 16 * 16 > 0 ? na / 2 / 16 * 16 : na / 2, n2 = na - n1;
