@@ -217,6 +217,50 @@ void cblas_zgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, in
 void cblas_cgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE ta, CBLAS_TRANSPOSE tb, int m, int n, int k,
                  const void *alpha, const void *a, int lda, const void *b, int ldb,
                  const void *beta, void *c, int ldc);
+void cblas_zsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 const void *alpha, const void *a, int lda, const void *beta, void *c, int ldc);
+void cblas_zherk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 double alpha, const void *a, int lda, double beta, void *c, int ldc);
+void cblas_zsyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  const void *alpha, const void *a, int lda, const void *b, int ldb,
+                  const void *beta, void *c, int ldc);
+void cblas_zher2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  const void *alpha, const void *a, int lda, const void *b, int ldb, double beta,
+                  void *c, int ldc);
+void cblas_zsymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
+void cblas_zhemm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
+void cblas_ztrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, const void *alpha, const void *a, int lda,
+                 void *b, int ldb);
+void cblas_ztrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, const void *alpha, const void *a, int lda,
+                 void *b, int ldb);
+void cblas_csyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 const void *alpha, const void *a, int lda, const void *beta, void *c, int ldc);
+void cblas_cherk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                 float alpha, const void *a, int lda, float beta, void *c, int ldc);
+void cblas_csyr2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  const void *alpha, const void *a, int lda, const void *b, int ldb,
+                  const void *beta, void *c, int ldc);
+void cblas_cher2k(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
+                  const void *alpha, const void *a, int lda, const void *b, int ldb, float beta,
+                  void *c, int ldc);
+void cblas_csymm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
+void cblas_chemm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,
+                 const void *alpha, const void *a, int lda, const void *b, int ldb,
+                 const void *beta, void *c, int ldc);
+void cblas_ctrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, const void *alpha, const void *a, int lda,
+                 void *b, int ldb);
+void cblas_ctrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,
+                 CBLAS_DIAG diag, int m, int n, const void *alpha, const void *a, int lda,
+                 void *b, int ldb);
 void cblas_dsyrk(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n, int k,
                  double alpha, const double *a, int lda, double beta, double *c, int ldc);
 void cblas_dtrsm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TRANSPOSE ta,

@@ -100,6 +100,15 @@ def _declare(lib):
     lib.cblas_sgemm.argtypes = [ci, ci, ci, ci, ci, ci, cf, vp, ci, vp, ci, cf, vp, ci]
     lib.cblas_zgemm.argtypes = [ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
     lib.cblas_cgemm.argtypes = [ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
+    for p_, r_ in (("z", cd), ("c", cf)):
+        getattr(lib, f"cblas_{p_}syrk").argtypes = [ci, ci, ci, ci, ci, vp, vp, ci, vp, vp, ci]
+        getattr(lib, f"cblas_{p_}herk").argtypes = [ci, ci, ci, ci, ci, r_, vp, ci, r_, vp, ci]
+        getattr(lib, f"cblas_{p_}syr2k").argtypes = [ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
+        getattr(lib, f"cblas_{p_}her2k").argtypes = [ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, r_, vp, ci]
+        for op in ("symm", "hemm"):
+            getattr(lib, f"cblas_{p_}{op}").argtypes = [ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
+        for op in ("trmm", "trsm"):
+            getattr(lib, f"cblas_{p_}{op}").argtypes = [ci, ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci]
     lib.cblas_dgemv.argtypes = [ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.cblas_dger.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, vp, ci]
     lib.cblas_dsymv.argtypes = [ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]

@@ -190,6 +190,107 @@ static const int syr2k_map[13] = {0, 2, 3, 4, 5, 0, 0, 8, 0, 10, 0, 0, 13};
 DEFINE_CBLAS_SYR2K(d, double, "DSYR2K")
 DEFINE_CBLAS_SYR2K(s, float, "SSYR2K")
 
+/* ---- complex twins of SYRK / HERK / SYR2K / HER2K / SYMM / HEMM / TRMM / TRSM: complex
+ *      scalars by address; row major is the column-major problem on the transposed operands:
+ *      uplo flipped and trans N <-> T (N <-> C for the Hermitian forms; HER2K then takes
+ *      conj(alpha), since C^T = conj(alpha) conj(A) B^T + alpha conj(B) A^T), side flipped and
+ *      m / n swapped for SYMM / HEMM / TRMM / TRSM.  SYRK / SYR2K accept no ConjTrans,
+ *      HERK / HER2K no Trans (netlib CBLAS). ---- */
+static char trflipc(CBLAS_TRANSPOSE t) { return t == CblasNoTrans ? 'C' : 'N'; }
+static int tr_ok_sym(CBLAS_TRANSPOSE t) { return t == CblasNoTrans || t == CblasTrans; }
+static int tr_ok_herm(CBLAS_TRANSPOSE t) { return t == CblasNoTrans || t == CblasConjTrans; }
+
+#define DEFINE_CBLAS_CRK(PRE, NAME, R, NAME6, HERM, AT, BT, AARG, BARG)                          \
+    void cblas_##PRE##NAME(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n,   \
+                           int k, AT alpha, const void *a, int lda, BT beta, void *c, int ldc)   \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 2); return; }                                         \
+        if (HERM ? !tr_ok_herm(trans) : !tr_ok_sym(trans)) { cbl_err(NAME6, 3); return; }         \
+        char u = layout == CblasColMajor ? up(uplo) : upflip(uplo);                              \
+        char t = layout == CblasColMajor ? tr(trans) : (HERM ? trflipc(trans) : trflip(trans));  \
+        WITH_MAP(syrk_map, PRE##NAME##_(&u, &t, &n, &k, AARG, (const R *)a, &lda, BARG, (R *)c,  \
+                                        &ldc));                                                  \
+    }
+DEFINE_CBLAS_CRK(z, syrk, double, "ZSYRK ", 0, const void *, const void *, (const double *)alpha,
+                 (const double *)beta)
+DEFINE_CBLAS_CRK(c, syrk, float, "CSYRK ", 0, const void *, const void *, (const float *)alpha,
+                 (const float *)beta)
+DEFINE_CBLAS_CRK(z, herk, double, "ZHERK ", 1, double, double, &alpha, &beta)
+DEFINE_CBLAS_CRK(c, herk, float, "CHERK ", 1, float, float, &alpha, &beta)
+
+#define DEFINE_CBLAS_CR2K(PRE, NAME, R, NAME6, HERM, BT, BARG)                                    \
+    void cblas_##PRE##NAME(CBLAS_LAYOUT layout, CBLAS_UPLO uplo, CBLAS_TRANSPOSE trans, int n,   \
+                           int k, const void *alpha, const void *a, int lda, const void *b,      \
+                           int ldb, BT beta, void *c, int ldc)                                   \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 2); return; }                                         \
+        if (HERM ? !tr_ok_herm(trans) : !tr_ok_sym(trans)) { cbl_err(NAME6, 3); return; }         \
+        const int row = layout == CblasRowMajor;                                                 \
+        char u = row ? upflip(uplo) : up(uplo);                                                  \
+        char t = !row ? tr(trans) : (HERM ? trflipc(trans) : trflip(trans));                     \
+        const R *al = (const R *)alpha;                                                          \
+        R alc[2] = {al[0], HERM && row ? -al[1] : al[1]};                                        \
+        WITH_MAP(syr2k_map, PRE##NAME##_(&u, &t, &n, &k, alc, (const R *)a, &lda, (const R *)b,  \
+                                         &ldb, BARG, (R *)c, &ldc));                             \
+    }
+DEFINE_CBLAS_CR2K(z, syr2k, double, "ZSYR2K", 0, const void *, (const double *)beta)
+DEFINE_CBLAS_CR2K(c, syr2k, float, "CSYR2K", 0, const void *, (const float *)beta)
+DEFINE_CBLAS_CR2K(z, her2k, double, "ZHER2K", 1, double, &beta)
+DEFINE_CBLAS_CR2K(c, her2k, float, "CHER2K", 1, float, &beta)
+
+#define DEFINE_CBLAS_CSYMM(PRE, NAME, R, NAME6)                                                  \
+    void cblas_##PRE##NAME(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, int m, int n,  \
+                           const void *alpha, const void *a, int lda, const void *b, int ldb,    \
+                           const void *beta, void *c, int ldc)                                   \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!side_ok(side)) { cbl_err(NAME6, 2); return; }                                       \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 3); return; }                                         \
+        const R *al = (const R *)alpha, *be = (const R *)beta;                                   \
+        if (layout == CblasColMajor) {                                                           \
+            char s = sd(side), u = up(uplo);                                                     \
+            WITH_MAP(symm_col, PRE##NAME##_(&s, &u, &m, &n, al, (const R *)a, &lda,              \
+                                            (const R *)b, &ldb, be, (R *)c, &ldc));              \
+        } else {                                                                                 \
+            char s = sdflip(side), u = upflip(uplo);                                             \
+            WITH_MAP(symm_row, PRE##NAME##_(&s, &u, &n, &m, al, (const R *)a, &lda,              \
+                                            (const R *)b, &ldb, be, (R *)c, &ldc));              \
+        }                                                                                        \
+    }
+DEFINE_CBLAS_CSYMM(z, symm, double, "ZSYMM ")
+DEFINE_CBLAS_CSYMM(c, symm, float, "CSYMM ")
+DEFINE_CBLAS_CSYMM(z, hemm, double, "ZHEMM ")
+DEFINE_CBLAS_CSYMM(c, hemm, float, "CHEMM ")
+
+#define DEFINE_CBLAS_CTRI(PRE, OP, R, NAME6)                                                     \
+    void cblas_##PRE##OP(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo,                  \
+                         CBLAS_TRANSPOSE ta, CBLAS_DIAG diag, int m, int n, const void *alpha,   \
+                         const void *a, int lda, void *b, int ldb)                               \
+    {                                                                                            \
+        if (!lay_ok(layout)) { cbl_err(NAME6, 1); return; }                                      \
+        if (!side_ok(side)) { cbl_err(NAME6, 2); return; }                                       \
+        if (!up_ok(uplo)) { cbl_err(NAME6, 3); return; }                                         \
+        if (!tr_ok(ta)) { cbl_err(NAME6, 4); return; }                                           \
+        if (!diag_ok(diag)) { cbl_err(NAME6, 5); return; }                                       \
+        char t = tr(ta), d = dg(diag);                                                           \
+        const R *al = (const R *)alpha;                                                          \
+        if (layout == CblasColMajor) {                                                           \
+            char s = sd(side), u = up(uplo);                                                     \
+            WITH_MAP(trsm_col, PRE##OP##_(&s, &u, &t, &d, &m, &n, al, (const R *)a, &lda,        \
+                                          (R *)b, &ldb));                                        \
+        } else {                                                                                 \
+            char s = sdflip(side), u = upflip(uplo);                                             \
+            WITH_MAP(trsm_row, PRE##OP##_(&s, &u, &t, &d, &n, &m, al, (const R *)a, &lda,        \
+                                          (R *)b, &ldb));                                        \
+        }                                                                                        \
+    }
+DEFINE_CBLAS_CTRI(z, trsm, double, "ZTRSM ")
+DEFINE_CBLAS_CTRI(c, trsm, float, "CTRSM ")
+DEFINE_CBLAS_CTRI(z, trmm, double, "ZTRMM ")
+DEFINE_CBLAS_CTRI(c, trmm, float, "CTRMM ")
+
 /* ---- GEMV: 1 layout 2 trans 3 m 4 n 5 alpha 6 A 7 lda 8 x 9 incx 10 beta 11 y 12 incy ---- */
 static const int gemv_col[12] = {0, 2, 3, 4, 0, 0, 7, 0, 9, 0, 0, 12};
 static const int gemv_row[12] = {0, 2, 4, 3, 0, 0, 7, 0, 9, 0, 0, 12};

@@ -276,3 +276,57 @@ def test_complex_trsm_trmm(gpu, prec, side, uplo, ta, diag):
     want = alpha * (opT @ B0 if side == "L" else B0 @ opT)
     G = abs(alpha) * (np.abs(opT) @ np.abs(B0) if side == "L" else np.abs(B0) @ np.abs(opT))
     assert np.all(np.abs(Y - want) <= 16 * (na + 2) * eps * G)
+
+
+def test_cblas_complex_row_major_twins(gpu):
+    """Row-major cblas_zherk / cblas_zher2k (alpha conjugated on the way to
+    the column-major call) / cblas_zhemm / cblas_ztrsm against numpy's
+    closed forms on row-major arrays."""
+    import ctypes
+    lib = gpu.library()
+    RowMajor, NoTrans, ConjTrans = 101, 111, 113
+    Upper, Lower, Left, Right, NonUnit = 121, 122, 141, 142, 131
+    eps = np.finfo(float).eps
+    n, k = 90, 70
+    rm = lambda s, shape: np.ascontiguousarray(_cplx(s, shape, np.complex128))
+    A, B, C0 = rm(90, (n, k)), rm(91, (n, k)), rm(92, (n, n))
+    up = np.triu(np.ones((n, n), bool))
+    # ZHERK: C = alpha A A^H + beta C, upper triangle of the row-major C
+    C = C0.copy()
+    lib.cblas_zherk(RowMajor, Upper, NoTrans, n, k, ctypes.c_double(0.7), A.ctypes.data, k,
+                    ctypes.c_double(0.3), C.ctypes.data, n)
+    want = 0.7 * A @ A.conj().T + 0.3 * (C0 - 1j * np.diag(np.diag(C0).imag))
+    G = 0.7 * np.abs(A) @ np.abs(A).T + 0.3 * np.abs(C0)
+    assert np.all(np.abs(C - want)[up] <= 16 * (k + 2) * eps * G[up])
+    assert np.array_equal(C[~up], C0[~up])
+    # ZHER2K: C = alpha A B^H + conj(alpha) B A^H + beta C
+    al = np.array([0.4 - 0.9j])
+    C = C0.copy()
+    lib.cblas_zher2k(RowMajor, Upper, NoTrans, n, k, al.ctypes.data, A.ctypes.data, k,
+                     B.ctypes.data, k, ctypes.c_double(0.5), C.ctypes.data, n)
+    want = al[0] * A @ B.conj().T + np.conj(al[0]) * B @ A.conj().T + 0.5 * (C0 - 1j * np.diag(np.diag(C0).imag))
+    G = abs(al[0]) * (np.abs(A) @ np.abs(B).T + np.abs(B) @ np.abs(A).T) + 0.5 * np.abs(C0)
+    assert np.all(np.abs(C - want)[up] <= 16 * (k + 2) * eps * G[up])
+    # ZHEMM, left: C = alpha H B2 + beta C with H Hermitian from the lower triangle
+    m2, n2 = 60, 50
+    H = rm(93, (m2, m2))
+    B2, C2 = rm(94, (m2, n2)), rm(95, (m2, n2))
+    Lt = np.tril(H)
+    Hf = Lt + np.tril(H, -1).conj().T
+    Hf[np.diag_indices(m2)] = Hf.diagonal().real
+    be = np.array([0.2 + 0.1j])
+    C = C2.copy()
+    lib.cblas_zhemm(RowMajor, Left, Lower, m2, n2, al.ctypes.data, H.ctypes.data, m2,
+                    B2.ctypes.data, n2, be.ctypes.data, C.ctypes.data, n2)
+    want = al[0] * Hf @ B2 + be[0] * C2
+    G = abs(al[0]) * np.abs(Hf) @ np.abs(B2) + abs(be[0]) * np.abs(C2)
+    assert np.all(np.abs(C - want) <= 16 * (m2 + 2) * eps * G)
+    # ZTRSM, right, conjugate transpose: X U^H = alpha B2
+    U = rm(96, (n2, n2)) / n2
+    U[np.diag_indices(n2)] = 1.0 + 0.25j
+    X = B2.copy()
+    lib.cblas_ztrsm(RowMajor, Right, Upper, ConjTrans, NonUnit, m2, n2, al.ctypes.data,
+                    U.ctypes.data, n2, X.ctypes.data, n2)
+    opU = np.triu(U).conj().T
+    R = X @ opU - al[0] * B2
+    assert np.linalg.norm(R) <= 8 * n2 * eps * (np.linalg.norm(opU) * np.linalg.norm(X) + np.linalg.norm(B2))
