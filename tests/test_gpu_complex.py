@@ -330,3 +330,39 @@ def test_cblas_complex_row_major_twins(gpu):
     opU = np.triu(U).conj().T
     R = X @ opU - al[0] * B2
     assert np.linalg.norm(R) <= 8 * n2 * eps * (np.linalg.norm(opU) * np.linalg.norm(X) + np.linalg.norm(B2))
+
+
+def test_complex_on_tracked_operands(gpu):
+    """ZGEMM and ZHERK on tracked (HBM-resident) complex arrays: operands used
+    in place (hits, no misses), results equal to the untracked calls bit for
+    bit, and alpha = 0 on a tracked C scaled on the device (C := beta C)."""
+    gb = gpu
+    n, k = 160, 96
+    A0, B0, C0 = (_cplx(s, (n, n), np.complex128) for s in (100, 101, 102))
+    A, B, C = gb.tracked_copy(A0), gb.tracked_copy(B0), gb.tracked_copy(C0)
+    Cu = C0.copy(order="F")
+    gb.zgemm("C", "N", n, n, n, 0.5 + 0.5j, A0, n, B0, n, 1.5 - 0.5j, Cu, n)
+    gb.reset_stats()
+    gb.zgemm("C", "N", n, n, n, 0.5 + 0.5j, A, n, B, n, 1.5 - 0.5j, C, n)
+    s = gb.stats()
+    assert s["hits"] == 3 and s["misses"] == 0 and s["device_calls"] == 1
+    assert np.array_equal(np.array(C), Cu)
+    # alpha = 0: C := beta C on the device (C is tracked)
+    before = gb.stats()["device_calls"]
+    gb.zgemm("N", "N", n, n, n, 0, A, n, B, n, 0.25 + 1j, C, n)
+    assert gb.stats()["device_calls"] == before + 1
+    np.testing.assert_allclose(np.array(C), (0.25 + 1j) * Cu, rtol=4 * np.finfo(float).eps, atol=0)
+    # ZHERK on a tracked operand and C
+    X = gb.tracked_copy(_cplx(103, (n, k), np.complex128))
+    H = gb.tracked_copy(_cplx(104, (n, n), np.complex128))
+    H0 = np.array(H)
+    lib = gb.library()
+    al, be = np.array([0.9]), np.array([0.1])
+    lib.zherk_(gb._c("L"), gb._c("N"), gb._i(n), gb._i(k), p(al), p(X), gb._i(n), p(be), p(H),
+               gb._i(n))
+    Xh = np.array(X)
+    want = 0.9 * Xh @ Xh.conj().T + 0.1 * (H0 - 1j * np.diag(np.diag(H0).imag))
+    lo = np.tril(np.ones((n, n), bool))
+    G = 0.9 * np.abs(Xh) @ np.abs(Xh).T + 0.1 * np.abs(H0)
+    assert np.all(np.abs(np.array(H) - want)[lo] <= 16 * (k + 2) * np.finfo(float).eps * G[lo])
+    assert np.array_equal(np.array(H)[~lo], H0[~lo])
