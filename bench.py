@@ -209,6 +209,30 @@ def gemm_figures():
             "frac_of_fp64_peak": round(flops / kavg / 1e12 / FP64_PEAK_TFLOPS, 4) if cnt else None}
         for b in bufs:
             lib.gpublas_dev_release(b)
+    # ZGEMM 8192^3 (C := A^H B + 0.5 C): one real product of m x 2n x 2k plus
+    # the pack and combine passes; wall rate over the whole call, 8mnk flops
+    lib.gpublas_dev_zgemm.argtypes = [ctypes.c_void_p, ctypes.c_char, ctypes.c_char, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
+    n, reps = 8192, 3
+    bufs = [lib.gpublas_dev_alloc(2 * n * n * 8) for _ in range(3)]
+    for i, b in enumerate(bufs):
+        lib.gpublas_dev_fill_uniform(None, 201 + i, 2 * n * n, b)
+    call = lambda: lib.gpublas_dev_zgemm(None, b"C", b"N", n, n, n, 1.0, 0.0, bufs[0], n, bufs[1],
+                                         n, 0.5, 0.0, bufs[2], n)
+    call()
+    lib.gpublas_dev_sync(None)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    lib.gpublas_dev_sync(None)
+    wall = time.perf_counter() - t0
+    flops = 8.0 * n ** 3
+    out[f"zgemm_cn_{n}"] = {"wall_tflops": round(flops * reps / wall / 1e12, 3),
+                            "frac_of_fp64_peak": round(flops * reps / wall / 1e12 / FP64_PEAK_TFLOPS, 4)}
+    for b in bufs:
+        lib.gpublas_dev_release(b)
     return out
 
 
