@@ -625,7 +625,7 @@ DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm, GB_RK_SSYMM)
         const double es_ = 2 * sizeof(R), nn_ = *n, kk_ = *k;                                    \
         const int scale_only = a0 || *k == 0;                                                    \
         if (gb_host_route(RK, cut > 0 && (*lda < cut || *ldc < cut), 4 * (TWO##_F) * nn_ * nn_ * kk_, \
-                          es_ * ((TWO##_F) * nn_ * kk_ + 2 * nn_ * nn_), a, NULL, c))            \
+                          es_ * ((TWO##_F) * nn_ * kk_ + 2 * nn_ * nn_), a, TWO##_PB, c))        \
             GB_CPLX_HOST(PRE, NAME,                                                              \
                          (const char *, const char *, const int *, const int *, const AL_T *,    \
                           const R *, const int *TWO##_PT(R), const BE_T *, R *, const int *),    \
@@ -650,6 +650,8 @@ DEFINE_SYMM(s, float, "SSYMM ", gpublas_dev_ssymm, GB_RK_SSYMM)
 #define TWO_PT(R) , const R *, const int *
 #define ONE_PA
 #define TWO_PA , b, ldb
+#define ONE_PB NULL
+#define TWO_PB b
 #define ONE_LDB_CHECK
 #define TWO_LDB_CHECK else if (*ldb < MAXI(1, nrowa)) info = 9;
 #define ONE_LDC_INFO 10
