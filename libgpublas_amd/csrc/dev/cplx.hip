@@ -565,6 +565,7 @@ int ctrsm_rec(hipStream_t s, bool left, bool lower, char tr, bool unit, int m, i
     hipLaunchKernelGGL((ctrsm_base<T, L, F>), grid, dim3(CT_THREADS), 0, s, na, nrhs, trc,       \
                        unit ? 1 : 0, ar, ai, A, (long)lda, B, (long)ldb)
         // forward order for a lower op(A) on the left and an upper one on the right
+        PROF_BEGIN(s, sizeof(T) == 8 ? "ztrsm_base" : "ctrsm_base");
         if (left) {
             if (lower) GB_CTB(true, true);
             else GB_CTB(true, false);
@@ -573,6 +574,7 @@ int ctrsm_rec(hipStream_t s, bool left, bool lower, char tr, bool unit, int m, i
             else GB_CTB(false, true);
         }
 #undef GB_CTB
+        PROF_END(s);
         return (int)hipGetLastError();
     }
     const int n1 = na / 2 / 16 * 16 > 0 ? na / 2 / 16 * 16 : na / 2, n2 = na - n1;
