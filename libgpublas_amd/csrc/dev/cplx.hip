@@ -468,8 +468,8 @@ int ctrmm_entry(void *stream, char side, char uplo, char transa, char diag, int 
 
 // ---- complex TRSM (trsm.cc:175-215 of the reference; netlib ZTRSM) -------
 // Recursive halving with the complex GEMM for the off-diagonal updates; a
-// triangle of order <= CT_BASE is solved by substitution in the netlib order
-// (x_t = (b_t - sum of the solved terms) / a_tt), one thread per right-hand
+// triangle of order <= CT_BASE is solved by substitution (x_t = (b_t - the
+// solved terms, subtracted in solve order) / a_tt), one thread per right-hand
 // side (left side: a column of B; right side: a row) holding its CT_BASE
 // unknowns in registers, op(A) in LDS (padded with the identity up to
 // CT_BASE, so a smaller triangle solves the same way with zeros after it).
@@ -520,22 +520,26 @@ __global__ __launch_bounds__(CT_THREADS) void ctrsm_base(int nb, int nrhs, int t
         xr[t] = ar * br - ai * bi;
         xi[t] = ar * bi + ai * br;
     }
+    // right-looking: x_t is divided when every earlier update has reached it,
+    // then subtracted from each unknown after it -- independent updates, so
+    // the dependent path per step is the division and one update (each
+    // unknown still takes its subtractions in solve order)
 #pragma unroll
     for (int s = 0; s < CT_BASE; ++s) {
         const int t = CT_BASE - 1 - s, rt = row(t);
-        T sr = xr[t], si = xi[t];
-#pragma unroll
-        for (int u = 0; u < s; ++u) {
-            const int v = CT_BASE - 1 - u, rv = row(v);  // solved before t
-            const T cr = LEFT ? Sr[rt][rv] : Sr[rv][rt], ci = LEFT ? Si[rt][rv] : Si[rv][rt];
-            sr -= cr * xr[v] - ci * xi[v];
-            si -= cr * xi[v] + ci * xr[v];
-            if ((u & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 8 coefficients at a time
-        }
         const T dr = Sr[rt][rt], di = Si[rt][rt];  // s / d = s conj(d) / |d|^2 (1 on a unit diagonal)
         const T den = dr * dr + di * di;
-        xr[t] = (sr * dr + si * di) / den;
-        xi[t] = (si * dr - sr * di) / den;
+        const T tr_ = (xr[t] * dr + xi[t] * di) / den, ti_ = (xi[t] * dr - xr[t] * di) / den;
+        xr[t] = tr_;
+        xi[t] = ti_;
+#pragma unroll
+        for (int u = 0; u < t; ++u) {
+            const int v = t - 1 - u, rv = row(v);  // solved after t
+            const T cr = LEFT ? Sr[rv][rt] : Sr[rt][rv], ci = LEFT ? Si[rv][rt] : Si[rt][rv];
+            xr[v] -= cr * tr_ - ci * ti_;
+            xi[v] -= cr * ti_ + ci * tr_;
+            if ((u & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 8 coefficients at a time
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
