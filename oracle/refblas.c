@@ -24,6 +24,7 @@
  */
 #include "refblas.h"
 #include <complex.h>
+#include <stdlib.h>
 #include <math.h>
 #include <stddef.h>
 #include <ctype.h>
@@ -312,6 +313,183 @@ void NAME(char ta, char tb, int m, int n, int k, const R *alpha_, const R *A_, i
 }
 DEFINE_CGEMM(ora_zgemm, double, double _Complex)
 DEFINE_CGEMM(ora_cgemm, float, float _Complex)
+
+/* The rest of the complex Level-3 (netlib ZSYRK / ZHERK / ZSYR2K / ZHER2K /
+ * ZSYMM / ZHEMM / ZTRMM / ZTRSM, the routines behind the reference's
+ * syrk.cc:182-215, herk.cc:131-162, syr2k.cc:182-215, her2k.cc:141-178,
+ * symm.cc:158-207, hemm.cc:120-153, trsm.cc:175-215; ZTRMM as netlib, trmm.cc
+ * being unbuilt).  Each result element is its defining sum taken in index
+ * order; argument errors return without touching the outputs (the library's
+ * INFO codes are checked by the reference's own zblat3 / cblat3).  Scalars
+ * and operands are (re, im) pairs; HERK's alpha / beta and HER2K's beta are
+ * real. */
+#define DEFINE_CRK(NAME, R, CT, HERM, TWO)                                                     \
+void NAME(char uplo, char trans, int n, int k, const R *alpha_, const R *A_, int lda,         \
+          const R *B_, int ldb, const R *beta_, R *C_, int ldc)                               \
+{                                                                                             \
+    const CT *A = (const CT *)A_, *B = (const CT *)B_;                                        \
+    CT *C = (CT *)C_;                                                                         \
+    const CT alpha = HERM && !TWO ? (CT)alpha_[0] : *(const CT *)alpha_;                      \
+    const CT beta = HERM ? (CT)beta_[0] : *(const CT *)beta_;                                 \
+    const int upper = ora_lsame(uplo, 'U'), nt = ora_lsame(trans, 'N');                      \
+    if (n < 0 || k < 0) return;                                                               \
+    if (n == 0 || ((alpha == 0 || k == 0) && beta == 1)) return;                              \
+    for (int j = 0; j < n; ++j)                                                               \
+        for (int i = upper ? 0 : j; i < (upper ? j + 1 : n); ++i) {                           \
+            CT acc = 0;                                                                       \
+            if (alpha != 0)                                                                   \
+                for (int l = 0; l < k; ++l) {                                                 \
+                    /* x = op(A)(i, l), xj = op(A)(j, l); y likewise for B */                 \
+                    CT xi = nt ? A[IX(i, l, lda)] : A[IX(l, i, lda)];                         \
+                    CT xj = nt ? A[IX(j, l, lda)] : A[IX(l, j, lda)];                         \
+                    if (HERM && !nt) { xi = conj(xi); xj = conj(xj); }                        \
+                    if (!TWO) {                                                               \
+                        acc += xi * (HERM ? conj(xj) : xj);                                   \
+                    } else {                                                                  \
+                        CT yi = nt ? B[IX(i, l, ldb)] : B[IX(l, i, ldb)];                     \
+                        CT yj = nt ? B[IX(j, l, ldb)] : B[IX(l, j, ldb)];                     \
+                        if (HERM && !nt) { yi = conj(yi); yj = conj(yj); }                    \
+                        acc += HERM ? alpha * xi * conj(yj) + conj(alpha) * yi * conj(xj)     \
+                                    : alpha * (xi * yj + yi * xj);                            \
+                    }                                                                         \
+                }                                                                             \
+            CT v = TWO ? acc : alpha * acc;                                                   \
+            if (beta != 0) {                                                                  \
+                CT c = C[IX(i, j, ldc)];                                                      \
+                if (HERM && i == j) c = creal(c);                                             \
+                v += beta * c;                                                                \
+            }                                                                                 \
+            C[IX(i, j, ldc)] = (HERM && i == j) ? (CT)creal(v) : v;                           \
+        }                                                                                     \
+}
+DEFINE_CRK(ora_zsyrk_, double, double _Complex, 0, 0)
+DEFINE_CRK(ora_zherk_, double, double _Complex, 1, 0)
+DEFINE_CRK(ora_zsyr2k, double, double _Complex, 0, 1)
+DEFINE_CRK(ora_zher2k, double, double _Complex, 1, 1)
+DEFINE_CRK(ora_csyrk_, float, float _Complex, 0, 0)
+DEFINE_CRK(ora_cherk_, float, float _Complex, 1, 0)
+DEFINE_CRK(ora_csyr2k, float, float _Complex, 0, 1)
+DEFINE_CRK(ora_cher2k, float, float _Complex, 1, 1)
+
+#define DEFINE_CSYK_WRAP(NAME, INNER, R)                                                      \
+void NAME(char uplo, char trans, int n, int k, const R *alpha, const R *A, int lda,           \
+          const R *beta, R *C, int ldc)                                                       \
+{                                                                                             \
+    INNER(uplo, trans, n, k, alpha, A, lda, A, lda, beta, C, ldc);                            \
+}
+DEFINE_CSYK_WRAP(ora_zsyrk, ora_zsyrk_, double)
+DEFINE_CSYK_WRAP(ora_zherk, ora_zherk_, double)
+DEFINE_CSYK_WRAP(ora_csyrk, ora_csyrk_, float)
+DEFINE_CSYK_WRAP(ora_cherk, ora_cherk_, float)
+
+/* element (i, j) of the full matrix a SYMM / HEMM reads from the uplo triangle */
+#define CSYM_ELEM(A, i, j, lda, upper, herm)                                                  \
+    (((upper) ? (i) <= (j) : (i) >= (j))                                                      \
+         ? (((herm) && (i) == (j)) ? creal(A[IX(i, j, lda)]) : A[IX(i, j, lda)])              \
+         : ((herm) ? conj(A[IX(j, i, lda)]) : A[IX(j, i, lda)]))
+
+#define DEFINE_CSYMM(NAME, R, CT, HERM)                                                       \
+void NAME(char side, char uplo, int m, int n, const R *alpha_, const R *A_, int lda,          \
+          const R *B_, int ldb, const R *beta_, R *C_, int ldc)                               \
+{                                                                                             \
+    const CT *A = (const CT *)A_, *B = (const CT *)B_;                                        \
+    CT *C = (CT *)C_;                                                                         \
+    const CT alpha = *(const CT *)alpha_, beta = *(const CT *)beta_;                          \
+    const int left = ora_lsame(side, 'L'), upper = ora_lsame(uplo, 'U');                      \
+    if (m <= 0 || n <= 0 || (alpha == 0 && beta == 1)) return;                                \
+    for (int j = 0; j < n; ++j)                                                               \
+        for (int i = 0; i < m; ++i) {                                                         \
+            CT acc = 0;                                                                       \
+            if (alpha != 0) {                                                                 \
+                if (left)                                                                     \
+                    for (int l = 0; l < m; ++l)                                               \
+                        acc += CSYM_ELEM(A, i, l, lda, upper, HERM) * B[IX(l, j, ldb)];       \
+                else                                                                          \
+                    for (int l = 0; l < n; ++l)                                               \
+                        acc += B[IX(i, l, ldb)] * CSYM_ELEM(A, l, j, lda, upper, HERM);       \
+            }                                                                                 \
+            CT v = alpha * acc;                                                               \
+            if (beta != 0) v += beta * C[IX(i, j, ldc)];                                      \
+            C[IX(i, j, ldc)] = v;                                                             \
+        }                                                                                     \
+}
+DEFINE_CSYMM(ora_zsymm, double, double _Complex, 0)
+DEFINE_CSYMM(ora_zhemm, double, double _Complex, 1)
+DEFINE_CSYMM(ora_csymm, float, float _Complex, 0)
+DEFINE_CSYMM(ora_chemm, float, float _Complex, 1)
+
+/* op(T)(r, c) of the triangle: zero outside it, 1 on a unit diagonal */
+#define CTRI_OP(A, r, c, lda, upper, tr, unit)                                                \
+    ({                                                                                        \
+        const int r_ = (tr) ? (c) : (r), c_ = (tr) ? (r) : (c);                               \
+        const int in_ = (upper) ? r_ <= c_ : r_ >= c_;                                        \
+        __typeof__(A[0]) v_ = !in_ ? 0 : (r_ == c_ && (unit)) ? 1 : A[IX(r_, c_, lda)];       \
+        (tr) == 2 ? conj(v_) : v_;                                                            \
+    })
+
+#define DEFINE_CTRMM(NAME, R, CT)                                                             \
+void NAME(char side, char uplo, char transa, char diag, int m, int n, const R *alpha_,        \
+          const R *A_, int lda, R *B_, int ldb)                                               \
+{                                                                                             \
+    const CT *A = (const CT *)A_;                                                             \
+    CT *B = (CT *)B_;                                                                         \
+    const CT alpha = *(const CT *)alpha_;                                                     \
+    const int left = ora_lsame(side, 'L'), upper = ora_lsame(uplo, 'U');                      \
+    const int tr = ora_lsame(transa, 'N') ? 0 : ora_lsame(transa, 'T') ? 1 : 2;               \
+    const int unit = ora_lsame(diag, 'U');                                                    \
+    if (m <= 0 || n <= 0) return;                                                             \
+    const int na = left ? m : n;                                                              \
+    CT *W = (CT *)malloc((size_t)m * n * sizeof(CT));                                         \
+    if (!W) return;                                                                           \
+    for (int j = 0; j < n; ++j)                                                               \
+        for (int i = 0; i < m; ++i) {                                                         \
+            CT acc = 0;                                                                       \
+            if (alpha != 0)                                                                   \
+                for (int l = 0; l < na; ++l)                                                  \
+                    acc += left ? CTRI_OP(A, i, l, lda, upper, tr, unit) * B[IX(l, j, ldb)]   \
+                                : B[IX(i, l, ldb)] * CTRI_OP(A, l, j, lda, upper, tr, unit);  \
+            W[IX(i, j, m)] = alpha * acc;                                                     \
+        }                                                                                     \
+    for (int j = 0; j < n; ++j)                                                               \
+        for (int i = 0; i < m; ++i) B[IX(i, j, ldb)] = W[IX(i, j, m)];                        \
+    free(W);                                                                                  \
+}
+DEFINE_CTRMM(ora_ztrmm, double, double _Complex)
+DEFINE_CTRMM(ora_ctrmm, float, float _Complex)
+
+/* substitution in solve order: x_t = (alpha b_t - sum of the solved terms) / a_tt */
+#define DEFINE_CTRSM(NAME, R, CT)                                                             \
+void NAME(char side, char uplo, char transa, char diag, int m, int n, const R *alpha_,        \
+          const R *A_, int lda, R *B_, int ldb)                                               \
+{                                                                                             \
+    const CT *A = (const CT *)A_;                                                             \
+    CT *B = (CT *)B_;                                                                         \
+    const CT alpha = *(const CT *)alpha_;                                                     \
+    const int left = ora_lsame(side, 'L'), upper = ora_lsame(uplo, 'U');                      \
+    const int tr = ora_lsame(transa, 'N') ? 0 : ora_lsame(transa, 'T') ? 1 : 2;               \
+    const int unit = ora_lsame(diag, 'U');                                                    \
+    if (m <= 0 || n <= 0) return;                                                             \
+    const int na = left ? m : n, nrhs = left ? n : m;                                         \
+    /* op(A) lower: forward on the left, backward on the right */                             \
+    const int lower = upper == (tr != 0);                                                     \
+    const int fwd = left ? lower : !lower;                                                    \
+    for (int q = 0; q < nrhs; ++q)                                                            \
+        for (int s = 0; s < na; ++s) {                                                        \
+            const int t = fwd ? s : na - 1 - s;                                               \
+            CT *x = left ? &B[IX(t, q, ldb)] : &B[IX(q, t, ldb)];                             \
+            CT v = alpha * *x;                                                                \
+            for (int u = 0; u < s; ++u) {                                                     \
+                const int w = fwd ? u : na - 1 - u;                                           \
+                const CT xw = left ? B[IX(w, q, ldb)] : B[IX(q, w, ldb)];                     \
+                v -= (left ? CTRI_OP(A, t, w, lda, upper, tr, 0)                              \
+                           : CTRI_OP(A, w, t, lda, upper, tr, 0)) * xw;                       \
+            }                                                                                 \
+            if (!unit) v /= CTRI_OP(A, t, t, lda, upper, tr, 0);                              \
+            *x = v;                                                                           \
+        }                                                                                     \
+}
+DEFINE_CTRSM(ora_ztrsm, double, double _Complex)
+DEFINE_CTRSM(ora_ctrsm, float, float _Complex)
 
 #define DEFINE_SYRK(NAME, T)                                                                  \
 void NAME(char uplo, char trans, int n, int k, T alpha, const T *A, int lda, T beta, T *C,    \

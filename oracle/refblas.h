@@ -61,6 +61,27 @@ void ora_zgemm(char ta, char tb, int m, int n, int k, const double *alpha, const
                const double *B, int ldb, const double *beta, double *C, int ldc);
 void ora_cgemm(char ta, char tb, int m, int n, int k, const float *alpha, const float *A, int lda,
                const float *B, int ldb, const float *beta, float *C, int ldc);
+/* the rest of the complex Level-3 (netlib semantics; HERK alpha / beta and HER2K beta real) */
+#define ORA_CPLX_DECL(P, R)                                                                   \
+    void ora_##P##syrk(char uplo, char trans, int n, int k, const R *alpha, const R *A, int lda, \
+                       const R *beta, R *C, int ldc);                                          \
+    void ora_##P##herk(char uplo, char trans, int n, int k, const R *alpha, const R *A, int lda, \
+                       const R *beta, R *C, int ldc);                                          \
+    void ora_##P##syr2k(char uplo, char trans, int n, int k, const R *alpha, const R *A,       \
+                        int lda, const R *B, int ldb, const R *beta, R *C, int ldc);           \
+    void ora_##P##her2k(char uplo, char trans, int n, int k, const R *alpha, const R *A,       \
+                        int lda, const R *B, int ldb, const R *beta, R *C, int ldc);           \
+    void ora_##P##symm(char side, char uplo, int m, int n, const R *alpha, const R *A, int lda, \
+                       const R *B, int ldb, const R *beta, R *C, int ldc);                     \
+    void ora_##P##hemm(char side, char uplo, int m, int n, const R *alpha, const R *A, int lda, \
+                       const R *B, int ldb, const R *beta, R *C, int ldc);                     \
+    void ora_##P##trmm(char side, char uplo, char transa, char diag, int m, int n,             \
+                       const R *alpha, const R *A, int lda, R *B, int ldb);                    \
+    void ora_##P##trsm(char side, char uplo, char transa, char diag, int m, int n,             \
+                       const R *alpha, const R *A, int lda, R *B, int ldb);
+ORA_CPLX_DECL(z, double)
+ORA_CPLX_DECL(c, float)
+#undef ORA_CPLX_DECL
 void ora_dsyrk(char uplo, char trans, int n, int k, double alpha, const double *A, int lda,
                double beta, double *C, int ldc);
 void ora_ssyrk(char uplo, char trans, int n, int k, float alpha, const float *A, int lda,

@@ -56,6 +56,15 @@ def load():
     L.ora_sgemm.argtypes = [C, C, I, I, I, F, VP, I, VP, I, F, VP, I]
     L.ora_zgemm.argtypes = [C, C, I, I, I, VP, VP, I, VP, I, VP, VP, I]
     L.ora_cgemm.argtypes = [C, C, I, I, I, VP, VP, I, VP, I, VP, VP, I]
+    for pre in "zc":
+        for op in ("syrk", "herk"):
+            getattr(L, f"ora_{pre}{op}").argtypes = [C, C, I, I, VP, VP, I, VP, VP, I]
+        for op in ("syr2k", "her2k"):
+            getattr(L, f"ora_{pre}{op}").argtypes = [C, C, I, I, VP, VP, I, VP, I, VP, VP, I]
+        for op in ("symm", "hemm"):
+            getattr(L, f"ora_{pre}{op}").argtypes = [C, C, I, I, VP, VP, I, VP, I, VP, VP, I]
+        for op in ("trmm", "trsm"):
+            getattr(L, f"ora_{pre}{op}").argtypes = [C, C, C, C, I, I, VP, VP, I, VP, I]
     L.ora_dsyrk.argtypes = [C, C, I, I, D, VP, I, D, VP, I]
     L.ora_ssyrk.argtypes = [C, C, I, I, F, VP, I, F, VP, I]
     L.ora_dtrsm.argtypes = [C, C, C, C, I, I, D, VP, I, VP, I]

@@ -366,3 +366,66 @@ def test_complex_on_tracked_operands(gpu):
     G = 0.9 * np.abs(Xh) @ np.abs(Xh).T + 0.1 * np.abs(H0)
     assert np.all(np.abs(np.array(H) - want)[lo] <= 16 * (k + 2) * np.finfo(float).eps * G[lo])
     assert np.array_equal(np.array(H)[~lo], H0[~lo])
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("routine", ["syrk", "herk", "syr2k", "her2k", "symm", "hemm", "trmm", "trsm"])
+def test_complex_level3_against_oracle(gpu, ora, prec, routine):
+    """Every complex Level-3 routine against the oracle's restatement
+    (oracle/refblas.c) at shapes past the netlib testers' N <= 9, in two
+    forms each, within the ZMMCH-style gauge (TRSM: the oracle's own
+    substitution, relative to the size of the solution)."""
+    dt, rdt = (np.complex128, np.float64) if prec == "z" else (np.complex64, np.float32)
+    eps = np.finfo(rdt).eps
+    lib = gpu.library()
+    n, k, m = 130, 90, 110
+    for form in range(2):
+        if routine in ("syrk", "herk", "syr2k", "her2k"):
+            herm = routine.startswith("he")
+            uplo, tr = ("L", "N") if form == 0 else ("U", "C" if herm else "T")
+            shape = (n, k) if tr == "N" else (k, n)
+            A, B, C0 = _cplx(110, shape, dt), _cplx(111, shape, dt), _cplx(112, (n, n), dt)
+            al = np.array([0.8], rdt) if routine == "herk" else np.array([0.8 - 0.4j], dt)
+            be = np.array([0.6], rdt) if herm else np.array([0.6 + 0.2j], dt)
+            C, R = C0.copy(order="F"), C0.copy(order="F")
+            head = [gpu._c(uplo), gpu._c(tr), gpu._i(n), gpu._i(k), p(al), p(A), gpu._i(shape[0])]
+            two = [p(B), gpu._i(shape[0])] if routine.endswith("2k") else []
+            getattr(lib, prec + routine + "_")(*head, *two, p(be), p(C), gpu._i(n))
+            oargs = [c(uplo), c(tr), n, k, p(al), p(A), shape[0]]
+            otwo = [p(B), shape[0]] if routine.endswith("2k") else []
+            getattr(ora, f"ora_{prec}{routine}")(*oargs, *otwo, p(be), p(R), n)
+            Xa = np.abs(A.astype(np.complex128))
+            X = Xa if tr == "N" else Xa.T
+            Yb = np.abs(B.astype(np.complex128))
+            Y = Yb if tr == "N" else Yb.T
+            G = abs(al[0]) * ((X @ Y.T + Y @ X.T) if routine.endswith("2k") else X @ X.T) + abs(be[0]) * np.abs(C0)
+            assert np.all(np.abs(C - R) <= 16 * (k + 2) * eps * G), (routine, form)
+        elif routine in ("symm", "hemm"):
+            side, uplo = ("L", "U") if form == 0 else ("R", "L")
+            na = m if side == "L" else n
+            A, B, C0 = _cplx(113, (na, na), dt), _cplx(114, (m, n), dt), _cplx(115, (m, n), dt)
+            al, be = np.array([0.7 + 0.1j], dt), np.array([-0.3 + 0.9j], dt)
+            C, R = C0.copy(order="F"), C0.copy(order="F")
+            getattr(lib, prec + routine + "_")(gpu._c(side), gpu._c(uplo), gpu._i(m), gpu._i(n), p(al),
+                                               p(A), gpu._i(na), p(B), gpu._i(m), p(be), p(C), gpu._i(m))
+            getattr(ora, f"ora_{prec}{routine}")(c(side), c(uplo), m, n, p(al), p(A), na, p(B), m,
+                                                  p(be), p(R), m)
+            Aa, Ba = np.abs(A.astype(np.complex128)), np.abs(B.astype(np.complex128))
+            F = np.maximum(Aa, Aa.T)
+            G = abs(al[0]) * (F @ Ba if side == "L" else Ba @ F) + abs(be[0]) * np.abs(C0)
+            assert np.all(np.abs(C - R) <= 16 * (na + 2) * eps * G), (routine, form)
+        else:
+            side, uplo, ta, diag = ("L", "L", "C", "N") if form == 0 else ("R", "U", "T", "U")
+            na = m if side == "L" else n
+            A = np.asfortranarray(_cplx(116, (na, na), dt) / na)
+            A[np.diag_indices(na)] = 1.0 + 0.5j
+            B0 = _cplx(117, (m, n), dt)
+            al = np.array([0.6 - 0.8j], dt)
+            X, R = B0.copy(order="F"), B0.copy(order="F")
+            getattr(lib, prec + routine + "_")(gpu._c(side), gpu._c(uplo), gpu._c(ta), gpu._c(diag),
+                                               gpu._i(m), gpu._i(n), p(al), p(A), gpu._i(na), p(X),
+                                               gpu._i(m))
+            getattr(ora, f"ora_{prec}{routine}")(c(side), c(uplo), c(ta), c(diag), m, n, p(al), p(A), na,
+                                                  p(R), m)
+            scale = np.max(np.abs(R)) if routine == "trsm" else np.max(np.abs(B0)) * 2
+            assert np.max(np.abs(X - R)) <= 32 * (na + 2) * eps * scale, (routine, form)

@@ -839,3 +839,117 @@ def test_complex_gemm_integer_exact_and_netlib_quick_paths(L, prec):
     C = np.full((8, 5), np.nan, dt, order="F")
     fn(b"N", b"N", 8, 5, 6, p(zero), p(A), 8, p(B), 6, p(zero), p(C), 8)
     assert np.all(C == 0)  # alpha == beta == 0: exact zeros
+
+
+# ---- the rest of the complex Level-3 in the oracle ----
+def _tr(X, uplo):
+    return np.triu(X) if uplo == "U" else np.tril(X)
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("kind", ["syrk", "herk", "syr2k", "her2k"])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("nt", [True, False])
+def test_complex_rank_k_vs_closed_form_and_host_blas(L, prec, kind, uplo, nt):
+    dt, rdt = (np.complex128, np.float64) if prec == "z" else (np.complex64, np.float32)
+    eps = np.finfo(rdt).eps
+    herm = kind.startswith("he")
+    tr = "N" if nt else ("C" if herm else "T")
+    n, k = 7, 5
+    shape = (n, k) if nt else (k, n)
+    A, B, C0 = _cplx(46, shape, dt), _cplx(47, shape, dt), _cplx(48, (n, n), dt)
+    al = np.array([0.8], rdt) if kind == "herk" else np.array([0.8 - 0.4j], dt)
+    be = np.array([0.6], rdt) if herm else np.array([0.6 + 0.2j], dt)
+    R = C0.copy(order="F")
+    fn = getattr(L, f"ora_{prec}{kind}")
+    if kind.endswith("2k"):
+        fn(c(uplo), c(tr), n, k, p(al), p(A), shape[0], p(B), shape[0], p(be), p(R), n)
+    else:
+        fn(c(uplo), c(tr), n, k, p(al), p(A), shape[0], p(be), p(R), n)
+    X = _op(A.astype(np.complex128), tr) if not nt else A.astype(np.complex128)
+    Y = _op(B.astype(np.complex128), tr) if not nt else B.astype(np.complex128)
+    a = complex(al[0])
+    P = {"syrk": a * X @ X.T, "herk": a * X @ X.conj().T, "syr2k": a * (X @ Y.T + Y @ X.T),
+         "her2k": a * X @ Y.conj().T + np.conj(a) * Y @ X.conj().T}[kind]
+    Cin = C0.astype(np.complex128)
+    if herm:
+        Cin = Cin - 1j * np.diag(np.diag(Cin).imag)
+    want = P + complex(be[0]) * Cin
+    G = abs(a) * (np.abs(X) @ np.abs(Y if kind.endswith("2k") else X).T) * (2 if kind.endswith("2k") else 1) + abs(be[0]) * np.abs(C0)
+    msk = np.triu(np.ones((n, n), bool)) if uplo == "U" else np.tril(np.ones((n, n), bool))
+    assert np.all(np.abs(R - want)[msk] <= 8 * (k + 2) * eps * G[msk])
+    assert np.array_equal(R[~msk], C0[~msk])
+    if herm:
+        assert np.all(np.diag(R).imag == 0)
+    hb = getattr(sb, prec + kind)
+    lower = int(uplo == "L")
+    tflag = 0 if nt else (2 if herm else 1)
+    if kind.endswith("2k"):
+        H = hb(complex(al[0]), A, B, beta=complex(be[0]) if not herm else float(be[0]), c=C0.copy(order="F"), trans=tflag, lower=lower)
+    else:
+        H = hb(al[0] if herm else complex(al[0]), A, beta=be[0] if herm else complex(be[0]), c=C0.copy(order="F"), trans=tflag, lower=lower)
+    assert np.all(np.abs(R - H)[msk] <= 16 * (k + 2) * eps * G[msk])
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("herm", [0, 1])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_complex_symm_hemm_vs_closed_form(L, prec, herm, side, uplo):
+    dt, rdt = (np.complex128, np.float64) if prec == "z" else (np.complex64, np.float32)
+    eps = np.finfo(rdt).eps
+    m, n = 6, 5
+    na = m if side == "L" else n
+    A, B, C0 = _cplx(49, (na, na), dt), _cplx(50, (m, n), dt), _cplx(51, (m, n), dt)
+    al, be = np.array([0.7 + 0.1j], dt), np.array([-0.3 + 0.9j], dt)
+    R = C0.copy(order="F")
+    getattr(L, f"ora_{prec}{'hemm' if herm else 'symm'}")(c(side), c(uplo), m, n, p(al), p(A), na,
+                                                           p(B), m, p(be), p(R), m)
+    T = _tr(A.astype(np.complex128), uplo)
+    off = T - np.diag(np.diag(T))
+    F = T + (off.conj().T if herm else off.T)
+    if herm:
+        F[np.diag_indices(na)] = F.diagonal().real
+    Bd = B.astype(np.complex128)
+    want = complex(al[0]) * (F @ Bd if side == "L" else Bd @ F) + complex(be[0]) * C0
+    G = abs(al[0]) * (np.abs(F) @ np.abs(Bd) if side == "L" else np.abs(Bd) @ np.abs(F)) + abs(be[0]) * np.abs(C0)
+    assert np.all(np.abs(R - want) <= 8 * (na + 2) * eps * G)
+
+
+@pytest.mark.parametrize("prec", ["z", "c"])
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+@pytest.mark.parametrize("ta", ["N", "T", "C"])
+@pytest.mark.parametrize("diag", ["N", "U"])
+def test_complex_trmm_trsm_vs_closed_form(L, prec, side, uplo, ta, diag):
+    """TRMM against the product of the expanded triangle; TRSM solves the
+    system TRMM then undoes (a round trip to the original B)."""
+    dt, rdt = (np.complex128, np.float64) if prec == "z" else (np.complex64, np.float32)
+    eps = np.finfo(rdt).eps
+    m, n = 6, 5
+    na = m if side == "L" else n
+    A = np.asfortranarray(_cplx(52, (na, na), dt) / na)
+    A[np.diag_indices(na)] = 1.0 + 0.5j
+    B0 = _cplx(53, (m, n), dt)
+    al = np.array([0.6 - 0.8j], dt)
+    T = _tr(A.astype(np.complex128), uplo)
+    if diag == "U":
+        T = T - np.diag(np.diag(T)) + np.eye(na)
+    opT = _op(T, ta)
+    Y = B0.copy(order="F")
+    getattr(L, f"ora_{prec}trmm")(c(side), c(uplo), c(ta), c(diag), m, n, p(al), p(A), na, p(Y), m)
+    Bd = B0.astype(np.complex128)
+    want = complex(al[0]) * (opT @ Bd if side == "L" else Bd @ opT)
+    G = abs(al[0]) * (np.abs(opT) @ np.abs(Bd) if side == "L" else np.abs(Bd) @ np.abs(opT))
+    assert np.all(np.abs(Y - want) <= 8 * (na + 2) * eps * G)
+    X = Y.copy(order="F")
+    inv = np.array([1 / complex(al[0])], dt)
+    getattr(L, f"ora_{prec}trsm")(c(side), c(uplo), c(ta), c(diag), m, n, p(inv), p(A), na, p(X), m)
+    assert np.allclose(X, B0, rtol=0, atol=64 * na * eps)
+    # and against the host BLAS
+    H = getattr(sb, prec + "trsm")(complex(al[0]), A, B0.copy(order="F"), side=int(side == "R"),
+                                     lower=int(uplo == "L"), trans_a="NTC".index(ta),
+                                     diag=int(diag == "U"))
+    Z = B0.copy(order="F")
+    getattr(L, f"ora_{prec}trsm")(c(side), c(uplo), c(ta), c(diag), m, n, p(al), p(A), na, p(Z), m)
+    assert np.allclose(Z, H, rtol=64 * na * eps, atol=64 * na * eps)
