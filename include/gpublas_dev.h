@@ -279,6 +279,13 @@ int gpublas_dev_dtbmv(void *stream, char uplo, char trans, char diag, int n, int
                       const double *d_A, int lda, double *d_x, int incx);
 int gpublas_dev_dtbsv(void *stream, char uplo, char trans, char diag, int n, int k,
                       const double *d_A, int lda, double *d_x, int incx);
+/* DTBSV plan switch (no reference counterpart): on = 1 (the default) lets
+ * k <= 32, n >= 2048 solves run chunked (per-chunk responses, a propagation
+ * of the incoming unknowns, per-chunk re-solves; declined on the device, in
+ * favour of the one-wave recurrence, when a response leaves [-1, 1] or a
+ * value is not finite); on = 0 keeps every solve on the one-wave
+ * recurrence; on < 0 only queries.  Returns the previous setting. */
+int gpublas_dev_tbsv_chunked(int on);
 
 /* ---- Level 1 (scalar results land in *h_result, pinned or device-visible host
  * memory from gpublas_dev_host_alloc, valid after the stream syncs) ---- */

@@ -21,6 +21,7 @@
  *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
  *   blas_bench --lib ... --op gbmv --m M --n N --k KL=KU --ta N   (DGBMV, the band once)
  *   blas_bench --lib ... --op sbmv|tbmv|tbsv --n N --k K --uplo L [--ta N --diag N]
+ *              [--dominant]  (tbsv: diagonally dominant band)
  *   blas_bench --lib ... --op axpy --n N           (HBM traffic calibration)
  *   blas_bench --lib ... --op link --n BYTES        (host-link copies of pageable
  *                                                    memory: plain vs staged)
@@ -106,6 +107,7 @@ int main(int argc, char **argv)
     int prec = 'd', m = 1024, n = 1024, k = 1024, reps = 5, warmup = 1;
     char ta = 'N', tb = 'N', uplo = 'L', trans = 'N', side = 'L', diag = 'N';
     uint64_t seed = 7;
+    int dominant = 0; /* tbsv: diagonally dominant band (see below) */
     for (int i = 1; i < argc; ++i) {
         const char *a = argv[i], *v = i + 1 < argc ? argv[i + 1] : "";
         if (!strcmp(a, "--lib")) path = v, ++i;
@@ -123,6 +125,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--reps")) reps = atoi(v), ++i;
         else if (!strcmp(a, "--warmup")) warmup = atoi(v), ++i;
         else if (!strcmp(a, "--seed")) seed = strtoull(v, 0, 10), ++i;
+        else if (!strcmp(a, "--dominant")) dominant = 1;
     }
     if (!path) {
         fprintf(stderr, "blas_bench: --lib required\n");
@@ -382,6 +385,17 @@ int main(int argc, char **argv)
             flops = 4.0 * bw * n;
         } else {
             tbmv = (tbmv_t)sym(op[2] == 'm' ? "dtbmv_" : "dtbsv_");
+            if (dominant) {
+                /* off-diagonal entries u / (2 (k + 1)), diagonal 2 + u / 2 (u
+                 * the uniform fill in [-1, 1]): host writes, so the first call
+                 * uploads the band again */
+                double *ab = (double *)A;
+                const int dr = uplo == 'U' || uplo == 'u' ? k : 0;
+                for (size_t j = 0; j < (size_t)n; ++j)
+                    for (int r = 0; r < bw; ++r)
+                        ab[j * bw + r] = r == dr ? 2.0 + 0.5 * ab[j * bw + r]
+                                                 : ab[j * bw + r] * (0.5 / bw);
+            }
             bytes = 8.0 * ((double)bw * n + 2.0 * n);
             flops = 2.0 * bw * n;
         }

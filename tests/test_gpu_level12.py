@@ -13,6 +13,7 @@ import pytest
 
 import _oracle
 from _oracle import c, p, uniform
+from libgpublas_amd import library as gb_library
 
 pytestmark = pytest.mark.gpu
 EPS = np.finfo(np.float64).eps
@@ -680,6 +681,129 @@ def test_dtbsv_random(gpu, ora, uplo, trans, n, k):
     gpu.dtbsv(uplo, trans, "N", n, k, AB, k + 1, x, 1)
     ora.ora_dtbsv(c(uplo), c(trans), b"N", n, k, p(AB), k + 1, p(r), 1)
     np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14)
+
+
+def _tb_dominant(n, k, uplo, lda, seed, unit):
+    """Triangular band (k super- or sub-diagonals) in band storage with
+    off-diagonal entries in [-1, 1] / (2 (k + 1)) and a diagonal in [1.5, 2.5]
+    (or a unit diagonal, whose stored row holds NaN): every unit response of a
+    chunk stays far inside [-1, 1].  Band-array corners outside the matrix
+    hold NaN."""
+    ku = k if uplo == "U" else 0
+    AB = np.full((lda, n), np.nan, order="F")
+    AB[: k + 1, :] = uniform(seed, (k + 1, n)) * (0.5 / (k + 1))
+    AB[ku, :] = np.nan if unit else 2.0 + 0.5 * uniform(seed + 1, (n,))
+    for j in range(min(k, n)):
+        if uplo == "U":
+            AB[: k - j, j] = np.nan             # rows above the matrix
+        else:
+            AB[j + 1: k + 1, n - 1 - j] = np.nan  # rows below the matrix
+    return AB
+
+
+TB_CHUNK_CASES = [(2048, 0, 1), (2048, 1, 1), (4099, 7, -2), (100_003, 15, 1), (100_003, 16, 3),
+                  (1 << 20, 32, 1), (70_001, 32, -1)]
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("n,k,incx", TB_CHUNK_CASES)
+def test_dtbsv_chunked(gpu, ora, uplo, trans, n, k, incx):
+    """k <= 32, n >= 2048 runs chunked: per-chunk responses, the incoming
+    unknowns propagated chunk to chunk, per-chunk re-solves.  Against the
+    oracle's sequential solve to 1e-12 relative (diagonally dominant bands,
+    stored and unit diagonal); the first 64 unknowns in solve order (chunk 0 is the
+    whole-length recurrence's own first steps) bit-identical to the one-wave
+    recurrence.  (With responses this small the propagated unknowns often
+    come out bit-identical to the one-wave ones.)"""
+    lib = gb_library()
+    assert lib.gpublas_dev_tbsv_chunked(-1) == 1
+    for diag in "NU":
+        AB = _tb_dominant(n, k, uplo, k + 2, 120 + k, diag == "U")
+        b = _strided(uniform(121 + k, (n,)), incx)
+        x, r, q = b.copy(), b.copy(), b.copy()
+        gpu.dtbsv(uplo, trans, diag, n, k, AB, k + 2, x, incx)
+        ora.ora_dtbsv(c(uplo), c(trans), c(diag), n, k, p(AB), k + 2, p(r), incx)
+        xs, rs = _strided_get(x, n, incx), _strided_get(r, n, incx)
+        np.testing.assert_allclose(xs, rs, rtol=1e-12, atol=1e-14 * np.abs(rs).max(),
+                                   err_msg=str((uplo, trans, diag, n, k, incx)))
+        gaps = np.ones(len(b), bool)
+        gaps[:: abs(incx)][:n] = False
+        assert np.array_equal(x[gaps], b[gaps])
+        lib.gpublas_dev_tbsv_chunked(0)
+        try:
+            gpu.dtbsv(uplo, trans, diag, n, k, AB, k + 2, q, incx)
+        finally:
+            lib.gpublas_dev_tbsv_chunked(1)
+        qs = _strided_get(q, n, incx)
+        first = slice(0, 64) if (uplo == "L") == (trans == "N") else slice(n - 64, n)  # solve order
+        assert np.array_equal(xs[first], qs[first])
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dtbsv_chunked_slow_decay(gpu, ora, uplo, trans):
+    """x_j = b_j + 0.999 x_(j-1) (unit diagonal, one off-diagonal of -0.999):
+    a unit incoming unknown still weighs 0.999^L ~ 0.4 at a chunk's end, so
+    the propagated unknowns carry their own roundings -- the result differs
+    from the one-wave recurrence in the last bits (the chunked passes ran and
+    were not declined) and agrees with the oracle to 1e-12 relative."""
+    lib = gb_library()
+    n, k = 1 << 20, 1
+    ku = k if uplo == "U" else 0
+    AB = np.full((2, n), -0.999, order="F")
+    AB[ku, :] = np.nan
+    if uplo == "U":
+        AB[0, 0] = np.nan
+    else:
+        AB[1, n - 1] = np.nan
+    b = uniform(140, (n,))
+    x, r, q = b.copy(), b.copy(), b.copy()
+    gpu.dtbsv(uplo, trans, "U", n, k, AB, 2, x, 1)
+    ora.ora_dtbsv(c(uplo), c(trans), b"U", n, k, p(AB), 2, p(r), 1)
+    np.testing.assert_allclose(x, r, rtol=1e-12, atol=1e-14 * np.abs(r).max())
+    lib.gpublas_dev_tbsv_chunked(0)
+    try:
+        gpu.dtbsv(uplo, trans, "U", n, k, AB, 2, q, 1)
+    finally:
+        lib.gpublas_dev_tbsv_chunked(1)
+    assert not np.array_equal(x, q)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_dtbsv_chunked_declined(gpu, ora, uplo, trans):
+    """Bands the chunked passes must not take: (a) integer entries in
+    {-1, 0, 1} under a unit diagonal (responses grow past 1; the integer
+    solution must come back exactly); (b) a zero on the diagonal (infinities
+    and NaN from there on).  Both give the one-wave recurrence's bits."""
+    lib = gb_library()
+    n, k = 9000, 24
+    rng = np.random.default_rng(130)
+    kl, ku = (0, k) if uplo == "U" else (k, 0)
+    AB = np.full((k + 1, n), np.nan, order="F")
+    for j in range(n):
+        i0, i1 = max(0, j - ku), min(n, j + kl + 1)
+        AB[ku + i0 - j: ku + i1 - j, j] = rng.integers(-1, 2, i1 - i0)
+    xt = rng.integers(-3, 4, n).astype(float)
+    b = xt.copy()
+    ora.ora_dtbmv(c(uplo), c(trans), b"U", n, k, p(AB), k + 1, p(b), 1)
+    assert np.abs(b).max() < 2 ** 40
+    x = b.copy()
+    gpu.dtbsv(uplo, trans, "U", n, k, AB, k + 1, x, 1)
+    assert np.array_equal(x, xt)
+    A2 = _tb_dominant(n, k, uplo, k + 1, 131, False)
+    A2[ku, n // 2] = 0.0
+    b2 = uniform(132, (n,))
+    x2, q2 = b2.copy(), b2.copy()
+    gpu.dtbsv(uplo, trans, "N", n, k, A2, k + 1, x2, 1)
+    lib.gpublas_dev_tbsv_chunked(0)
+    try:
+        gpu.dtbsv(uplo, trans, "N", n, k, A2, k + 1, q2, 1)
+    finally:
+        lib.gpublas_dev_tbsv_chunked(1)
+    assert not np.isfinite(x2).all()
+    assert np.array_equal(x2, q2, equal_nan=True)
 
 
 @pytest.mark.parametrize("n,incx", [(2, 1), (1000, 1), (1001, 3), (65536, 1), (65537, 1),
