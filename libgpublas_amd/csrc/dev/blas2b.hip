@@ -51,6 +51,22 @@
 // the reciprocal, which gives the correctly rounded quotient except in rare
 // double-rounding cases.  One wave runs the whole solve: its instruction
 // issue per column (about 25 instructions) bounds it, not HBM.
+//
+// Chunked DTBSV (k <= 32, n >= 2048).  The solve positions are cut into C
+// chunks of L; chunk c depends on the rest only through its incoming state
+// z_c (the last k unknowns before it), and linearly: x_c = y_c + G_c z_c,
+// y_c its solve from a zero state, G_c its responses to unit states.
+//  1. tbsv_resp, one wave per chunk: lane 0 computes y_c, lane r = 1 .. k the
+//     response to a unit (r-1)-th incoming unknown, all lanes sharing each
+//     step's coefficients; keeps the last k values of each (h_c, H_c) and
+//     flags the chunk if a response leaves [-1, 1] or a value is not finite.
+//  2. tbsv_prop, one wave: z_{c+1} = h_c + H_c z_c over the chunks.
+//  3. tbsv_reg, one wave per chunk: the one-wave recurrence restarted at the
+//     chunk with z_c taken off its first k equations, in the order the
+//     whole-length recurrence takes them: given z_c, the same arithmetic.
+// A flagged chunk (errors in z_c would grow through it) or a non-finite
+// value makes step 3 run the whole-length recurrence on one wave instead, so
+// singular and non-dominant bands get exactly the one-wave result.
 #include <cfloat>
 #include <cmath>
 
