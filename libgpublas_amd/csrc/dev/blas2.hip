@@ -251,37 +251,64 @@ constexpr int SY_C = 8;    // columns per LDS pass
 // A: every element tested before it is loaded).
 enum { SY_INTERIOR = 0, SY_DIAG = 1, SY_EDGE = 2 };
 
-template <int KIND, bool LOWER>
-__device__ __forceinline__ void symv_load(d2 (&a)[SY_C], const double *A, long lda, long i,
-                                          long j0, int n, bool diag)
+// Packed storage (DSPMV, PACKED = true): column j of the triangle starts at
+// j (j + 1) / 2 (upper) or j (2n - j - 1) / 2 (lower, rows from j), at any
+// 8-byte offset, so a lane owns rows I*SY_T + l and + 64 (two 8-B loads per
+// column, each instruction 512 contiguous bytes) instead of a 16-B pair.
+// Inside a non-edge diagonal tile the unreferenced elements' addresses still
+// fall inside the packed array (those of the neighbouring columns); they are
+// loaded and replaced by zeros like the full-storage ones.
+template <bool PACKED, bool LOWER>
+__device__ __forceinline__ long sy_at(long i, long j, long lda, int n)
+{
+    if (!PACKED) return i + j * lda;
+    return LOWER ? i + j * (2 * (long)n - j - 1) / 2 : i + j * (j + 1) / 2;
+}
+
+template <int KIND, bool LOWER, bool PACKED>
+__device__ __forceinline__ void symv_load(d2 (&a)[SY_C], const double *A, long lda, long ra,
+                                          long rb, long j0, int n, bool diag)
 {
 #pragma unroll
     for (int u = 0; u < SY_C; ++u) {
         const long j = j0 + u;
         if (KIND == SY_INTERIOR) {
-            a[u] = __builtin_nontemporal_load((const d2 *)(A + i + j * lda));
+            if (PACKED) {
+                a[u].x = __builtin_nontemporal_load(A + sy_at<true, LOWER>(ra, j, lda, n));
+                a[u].y = __builtin_nontemporal_load(A + sy_at<true, LOWER>(rb, j, lda, n));
+            } else {
+                a[u] = __builtin_nontemporal_load((const d2 *)(A + ra + j * lda));
+            }
         } else if (KIND == SY_DIAG) {
-            const d2 v = __builtin_nontemporal_load((const d2 *)(A + i + j * lda));
-            a[u].x = (LOWER ? i >= j : i <= j) ? v.x : 0.0;
-            a[u].y = (LOWER ? i + 1 >= j : i + 1 <= j) ? v.y : 0.0;
+            d2 v;
+            if (PACKED) {
+                v.x = __builtin_nontemporal_load(A + sy_at<true, LOWER>(ra, j, lda, n));
+                v.y = __builtin_nontemporal_load(A + sy_at<true, LOWER>(rb, j, lda, n));
+            } else {
+                v = __builtin_nontemporal_load((const d2 *)(A + ra + j * lda));
+            }
+            a[u].x = (LOWER ? ra >= j : ra <= j) ? v.x : 0.0;
+            a[u].y = (LOWER ? rb >= j : rb <= j) ? v.y : 0.0;
         } else {
             const bool in = j < n;
-            const bool s0 = in && i < n && (!diag || (LOWER ? i >= j : i <= j));
-            const bool s1 = in && i + 1 < n && (!diag || (LOWER ? i + 1 >= j : i + 1 <= j));
-            a[u].x = s0 ? A[i + j * lda] : 0.0;
-            a[u].y = s1 ? A[i + 1 + j * lda] : 0.0;
+            const bool s0 = in && ra < n && (!diag || (LOWER ? ra >= j : ra <= j));
+            const bool s1 = in && rb < n && (!diag || (LOWER ? rb >= j : rb <= j));
+            a[u].x = s0 ? A[sy_at<PACKED, LOWER>(ra, j, lda, n)] : 0.0;
+            a[u].y = s1 ? A[sy_at<PACKED, LOWER>(rb, j, lda, n)] : 0.0;
         }
     }
 }
 
-template <int KIND, bool LOWER>
+template <int KIND, bool LOWER, bool PACKED>
 __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double *A,
                                                long lda, d2 xi, const double *xs, double *rowP,
                                                double *colP, double (*pr)[65])
 {
     constexpr bool EDGE = KIND == SY_EDGE;
     const int lane = threadIdx.x;
-    const long i = (long)I * SY_T + 2 * lane, j0 = (long)J * SY_T;
+    // this lane's two rows
+    const long ra = (long)I * SY_T + (PACKED ? lane : 2 * lane), rb = PACKED ? ra + 64 : ra + 1;
+    const long j0 = (long)J * SY_T;
     const bool diag = I == J;
     double r0 = 0, r1 = 0;
     // one pass: row sums and column products of `cur` (columns c0 ..), the
@@ -295,13 +322,14 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
             double px = cur[u].x * xi.x, py = cur[u].y * xi.y;
             if (KIND != SY_INTERIOR && diag) {  // the diagonal element: row sum only
                 const long j = j0 + c0 + u;
-                if (i == j) px = 0.0;
-                if (i + 1 == j) py = 0.0;
+                if (ra == j) px = 0.0;
+                if (rb == j) py = 0.0;
             }
             pr[u][lane] = px + py;
         }
         // the next pass's loads fly while this pass's column sums are formed
-        if (c0 + SY_C < SY_T) symv_load<KIND, LOWER>(nxt, A, lda, i, j0 + c0 + SY_C, n, diag);
+        if (c0 + SY_C < SY_T)
+            symv_load<KIND, LOWER, PACKED>(nxt, A, lda, ra, rb, j0 + c0 + SY_C, n, diag);
         lds_barrier();
         {
             const int c = lane % SY_C, part = lane / SY_C;
@@ -316,7 +344,7 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
         lds_barrier();  // pr is rewritten by the next pass
     };
     d2 a[SY_C], b[SY_C];
-    symv_load<KIND, LOWER>(a, A, lda, i, j0, n, diag);
+    symv_load<KIND, LOWER, PACKED>(a, A, lda, ra, rb, j0, n, diag);
     lds_barrier();  // xs is in LDS; the first pass's loads stay in flight
     // The copy at the end of a pass waits for the next pass's loads, after
     // they have overlapped the column sums.  Measured at n = 8192: 49.9 us
@@ -328,12 +356,12 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
 #pragma unroll
         for (int u = 0; u < SY_C; ++u) a[u] = b[u];
     }
-    if (!EDGE || i < n) rowP[(long)J * n + i] = r0;
-    if (!EDGE || i + 1 < n) rowP[(long)J * n + i + 1] = r1;
+    if (!EDGE || ra < n) rowP[(long)J * n + ra] = r0;
+    if (!EDGE || rb < n) rowP[(long)J * n + rb] = r1;
 }
 
-template <bool LOWER>  // a template parameter: as a run-time value the triangle
-                      // tests turn into branches that wait on every load
+template <bool LOWER, bool PACKED>  // template parameters: as run-time values the
+                                    // triangle tests turn into branches that wait on every load
 __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const double *A,
                                                 long lda, const double *x, long incx, long x0,
                                                 double *rowP, double *colP)
@@ -355,7 +383,8 @@ __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const do
     }
     const int I = LOWER ? p : q, J = LOWER ? q : p;
     const int lane = threadIdx.x;
-    const long i = (long)I * SY_T + 2 * lane, j0 = (long)J * SY_T;
+    const long ra = (long)I * SY_T + (PACKED ? lane : 2 * lane), rb = PACKED ? ra + 64 : ra + 1;
+    const long j0 = (long)J * SY_T;
 
     __shared__ double xs[SY_T];
     __shared__ double pr[SY_C][65];
@@ -365,15 +394,15 @@ __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const do
         xs[lane + 64 * h] = j < n ? x[x0 + j * incx] : 0.0;
     }
     d2 xi;
-    xi.x = i < n ? x[x0 + i * incx] : 0.0;
-    xi.y = i + 1 < n ? x[x0 + (i + 1) * incx] : 0.0;
+    xi.x = ra < n ? x[x0 + ra * incx] : 0.0;
+    xi.y = rb < n ? x[x0 + rb * incx] : 0.0;
     const bool edge = !vec || (long)(I + 1) * SY_T > n || (long)(J + 1) * SY_T > n;
     if (edge)
-        symv_tile_body<SY_EDGE, LOWER>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_EDGE, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
     else if (I == J)
-        symv_tile_body<SY_DIAG, LOWER>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_DIAG, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
     else
-        symv_tile_body<SY_INTERIOR, LOWER>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_INTERIOR, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
 }
 
 // Fold: 8 lanes per output.  Output k (block b) sums its row partials
@@ -669,6 +698,44 @@ int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x
     return (int)hipGetLastError();
 }
 
+}  // extern "C"
+
+namespace {
+// tiles + fold of DSYMV / DSPMV (full or packed storage)
+int symv_run(hipStream_t s, int lower, bool packed, int vec, int n, double alpha, const double *A,
+             int lda, const double *x, int incx, double beta, double *y, int incy, long y0)
+{
+    const int nb = ceil_div(n, SY_T);
+    const long tiles = (long)nb * (nb + 1) / 2;
+    Scratch scr = dev_scratch(2 * (size_t)nb * n * sizeof(double), s);
+    double *part = (double *)scr.p;
+    if (!part) return (int)hipErrorOutOfMemory;
+    double *rowP = part, *colP = part + (size_t)nb * n;
+    const char *name = packed ? "dspmv" : "dsymv";
+    PROF_BEGIN(s, name);
+#define GB_SYT(L, P)                                                                         \
+    hipLaunchKernelGGL((symv_tile<L, P>), dim3((unsigned)tiles), dim3(64), 0, s, n, nb, vec, A, \
+                       (long)lda, x, (long)incx, start_of(n, incx), rowP, colP)
+    if (packed) {
+        if (lower) GB_SYT(true, true);
+        else GB_SYT(false, true);
+    } else {
+        if (lower) GB_SYT(true, false);
+        else GB_SYT(false, false);
+    }
+#undef GB_SYT
+    PROF_END(s);
+    PROF_BEGIN(s, packed ? "dspmv_fold" : "dsymv_fold");
+    hipLaunchKernelGGL(symv_fold, dim3((unsigned)ceil_div((long)n * SY_FL, 256)), dim3(256), 0, s,
+                       n, lower, nb, (const double *)rowP, (const double *)colP, alpha, beta, y,
+                       (long)incy, y0);
+    PROF_END(s);
+    return (int)hipGetLastError();
+}
+}  // namespace
+
+extern "C" {
+
 int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double *d_A, int lda,
                       const double *d_x, int incx, double beta, double *d_y, int incy)
 {
@@ -684,28 +751,21 @@ int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double
         return (int)hipGetLastError();
     }
     const int lower = uplo == 'L' || uplo == 'l';
-    const int nb = ceil_div(n, SY_T);
-    const long tiles = (long)nb * (nb + 1) / 2;
-    Scratch scr = dev_scratch(2 * (size_t)nb * n * sizeof(double), s);
-    double *part = (double *)scr.p;
-    if (!part) return (int)hipErrorOutOfMemory;
-    double *rowP = part, *colP = part + (size_t)nb * n;
-    PROF_BEGIN(s, "dsymv");
     const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
-    if (lower)
-        hipLaunchKernelGGL(symv_tile<true>, dim3((unsigned)tiles), dim3(64), 0, s, n, nb, vec, d_A,
-                           (long)lda, d_x, (long)incx, start_of(n, incx), rowP, colP);
-    else
-        hipLaunchKernelGGL(symv_tile<false>, dim3((unsigned)tiles), dim3(64), 0, s, n, nb, vec,
-                           d_A, (long)lda, d_x, (long)incx, start_of(n, incx), rowP, colP);
-    PROF_END(s);
-    PROF_BEGIN(s, "dsymv_fold");
-    hipLaunchKernelGGL(symv_fold, dim3((unsigned)ceil_div((long)n * SY_FL, 256)), dim3(256), 0, s,
-                       n, lower, nb,
-                       (const double *)rowP, (const double *)colP, alpha, beta, d_y, (long)incy,
-                       y0);
-    PROF_END(s);
-    return (int)hipGetLastError();
+    return symv_run(s, lower, false, vec, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy, y0);
+}
+
+// DSPMV: the DSYMV tile kernel on packed storage (one read of the packed
+// triangle; replaces cublasDspmv, blas_level2/pmv.cc)
+int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double *d_AP,
+                      const double *d_x, int incx, double beta, double *d_y, int incy)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    if (alpha == 0.0) return gpublas_dev_dsymv(s, uplo, n, alpha, nullptr, n, d_x, incx, beta, d_y, incy);
+    const int lower = uplo == 'L' || uplo == 'l';
+    return symv_run(s, lower, true, 1, n, alpha, d_AP, 0, d_x, incx, beta, d_y, incy,
+                    start_of(n, incy));
 }
 
 int gpublas_dev_dsyr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,

@@ -1,5 +1,5 @@
 // blas2p.hip -- triangular and packed Level-2 routines on gfx950: DTRMV,
-// DTPMV, DTPSV, DSPMV.
+// DTPMV, DTPSV (DSPMV runs the DSYMV tile kernel on packed storage, blas2.hip).
 //
 // Replace the vendor calls of the reference's unbuilt CBLAS wrappers
 // (cublasDtrmv, blas_level2/trmv.cc; cublasDtpmv / cublasDtpsv, tpmv.cc,
@@ -17,10 +17,9 @@
 //    (gathered through incx) since x(j) is written while others read.
 // Packed storage (netlib column packing: column j of an upper triangle holds
 // rows 0..j at AP[j(j+1)/2 ..], of a lower one rows j..n-1 at
-// AP[j(2n-j+1)/2 ..]): DTPMV and DSPMV read it in place (pk_n_stream,
-// pk_t_kernel; DSPMV = the 'N' sweep with the diagonal + the 'T' sweep without
-// it, so the triangle is read twice); DTPSV unpacks it into a full-storage
-// block of a pool buffer and runs the DTRSV solve.
+// AP[j(2n-j+1)/2 ..]): DTPMV reads it in place (pk_n_stream, pk_t_kernel);
+// DTPSV unpacks it into a full-storage block of a pool buffer and runs the
+// DTRSV solve.
 #include "devcommon.hpp"
 #include "gpublas_dev.h"
 
@@ -189,7 +188,7 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(int n, const double *A, lon
     }
 }
 
-// ---- packed storage read in place (DSPMV, DTPMV) ----------------------------
+// ---- packed storage read in place (DTPMV) -----------------------------------
 // Column j of the packed triangle is contiguous: lower rows j .. n-1 from
 // AP[j (2n - j + 1) / 2], upper rows 0 .. j from AP[j (j + 1) / 2].  pcol(j)
 // is the address of element (0, j) of that column, so pcol(j)[i] is (i, j)
@@ -258,8 +257,8 @@ __global__ __launch_bounds__(256) void pk_n_stream(int n, const double *AP, cons
 }
 
 // op = T on the packed triangle: one wave per column, a dot product over the
-// column's contiguous stored run (without the diagonal when DIAG == 0, with a
-// unit one when DIAG == 2), four loads in flight per lane.
+// column's contiguous stored run (the stored diagonal when DIAG == 1, a unit
+// one when DIAG == 2), four loads in flight per lane.
 template <bool LOWER, int DIAG>
 __global__ __launch_bounds__(256) void pk_t_kernel(int n, const double *AP, const double *t,
                                                    long inct, long t0, double *out, long inco,
@@ -290,9 +289,8 @@ __global__ __launch_bounds__(256) void pk_t_kernel(int n, const double *AP, cons
     }
 }
 
-// out(i) := alpha * (panel partials of row i in panel order [+ the extra
-// partial P[ncb][i]]) + beta * out(i)
-template <bool LOWER, bool EXTRA>
+// out(i) := alpha * (panel partials of row i in panel order) + beta * out(i)
+template <bool LOWER>
 __global__ __launch_bounds__(64) void pk_fold(int n, int ncb, int cbw, const double *P,
                                               double alpha, double beta, double *y, long incy,
                                               long y0)
@@ -312,20 +310,17 @@ __global__ __launch_bounds__(64) void pk_fold(int n, int ncb, int cbw, const dou
         for (int u = 0; u < 8; ++u) s += v[u];
     }
     for (; c <= c_hi; ++c) s += P[(long)c * n + i];
-    if (EXTRA) s += P[(long)ncb * n + i];
     double *yo = y + y0 + i * incy;
     *yo = beta == 0.0 ? alpha * s : fma(alpha, s, beta * *yo);
 }
 
-// op(T) x on packed storage (DTPMV) or alpha A x + beta y with A the packed
-// symmetric matrix (DSPMV: the 'N' sweep with the diagonal plus the 'T' sweep
-// without it, added in one fold).
-int packed_mv(hipStream_t s, bool lower, bool sym, bool trans, bool unit, int n, const double *AP,
+// op(T) x on packed storage (DTPMV).
+int packed_mv(hipStream_t s, bool lower, bool trans, bool unit, int n, const double *AP,
               const double *x, int incx, double alpha, double beta, double *y, int incy)
 {
     const long x0 = start_of(n, incx), y0 = start_of(n, incy);
     const unsigned tblocks = (unsigned)std::min(ceil_div(n, 4), 8192);
-    if (!sym && trans) {
+    if (trans) {
         // x(j) is written while other waves read x(i): work from a copy
         double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
         if (!t) return (int)hipErrorOutOfMemory;
@@ -348,30 +343,21 @@ int packed_mv(hipStream_t s, bool lower, bool sym, bool trans, bool unit, int n,
     if (ncb > maxcb) ncb = maxcb;
     const int cbw = ceil_div(n, ncb);
     ncb = ceil_div(n, cbw);
-    double *P = (double *)gpublas_dev_alloc((size_t)(ncb + (sym ? 1 : 0)) * n * sizeof(double));
+    double *P = (double *)gpublas_dev_alloc((size_t)ncb * n * sizeof(double));
     if (!P) return (int)hipErrorOutOfMemory;
-    PROF_BEGIN(s, sym ? "dspmv" : "dtpmv");
+    PROF_BEGIN(s, "dtpmv");
     const dim3 grid((unsigned)nrb * ncb);
 #define GB_PN(L, U) \
     hipLaunchKernelGGL((pk_n_stream<L, U>), grid, dim3(256), 0, s, n, AP, x, (long)incx, x0, nrb, cbw, P)
     if (lower) { if (unit) GB_PN(true, true); else GB_PN(true, false); }
     else { if (unit) GB_PN(false, true); else GB_PN(false, false); }
 #undef GB_PN
-    if (sym) {
-        double *Pt = P + (size_t)ncb * n;
-        if (lower)
-            hipLaunchKernelGGL((pk_t_kernel<true, 0>), dim3(tblocks), dim3(256), 0, s, n, AP, x,
-                               (long)incx, x0, Pt, 1L, 0L);
-        else
-            hipLaunchKernelGGL((pk_t_kernel<false, 0>), dim3(tblocks), dim3(256), 0, s, n, AP, x,
-                               (long)incx, x0, Pt, 1L, 0L);
-    }
     const dim3 fg((unsigned)ceil_div(n, 64));
-#define GB_PF(L, E) \
-    hipLaunchKernelGGL((pk_fold<L, E>), fg, dim3(64), 0, s, n, ncb, cbw, P, alpha, beta, y, \
+#define GB_PF(L) \
+    hipLaunchKernelGGL((pk_fold<L>), fg, dim3(64), 0, s, n, ncb, cbw, P, alpha, beta, y, \
                        (long)incy, y0)
-    if (lower) { if (sym) GB_PF(true, true); else GB_PF(true, false); }
-    else { if (sym) GB_PF(false, true); else GB_PF(false, false); }
+    if (lower) GB_PF(true);
+    else GB_PF(false);
 #undef GB_PF
     PROF_END(s);
     dev_release_after(P, s);
@@ -477,7 +463,7 @@ int gpublas_dev_dtpmv(void *stream, char uplo, char trans, char diag, int n, con
 {
     hipStream_t s = pick_stream(stream);
     if (n <= 0) return 0;
-    return packed_mv(s, !up(uplo), false, !isN(trans), !isN(diag), n, d_AP, d_x, incx, 1.0, 0.0,
+    return packed_mv(s, !up(uplo), !isN(trans), !isN(diag), n, d_AP, d_x, incx, 1.0, 0.0,
                      d_x, incx);
 }
 
@@ -493,13 +479,5 @@ int gpublas_dev_dtpsv(void *stream, char uplo, char trans, char diag, int n, con
     return e;
 }
 
-int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double *d_AP,
-                      const double *d_x, int incx, double beta, double *d_y, int incy)
-{
-    hipStream_t s = pick_stream(stream);
-    if (n <= 0) return 0;
-    if (alpha == 0.0) return gpublas_dev_dsymv(s, uplo, n, alpha, nullptr, n, d_x, incx, beta, d_y, incy);
-    return packed_mv(s, !up(uplo), true, false, false, n, d_AP, d_x, incx, alpha, beta, d_y, incy);
-}
 
 }  // extern "C"

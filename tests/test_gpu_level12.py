@@ -482,7 +482,8 @@ def test_dtpmv_dtpsv(gpu, ora, uplo, trans, diag, n, incx):
 
 
 @pytest.mark.parametrize("uplo", ["L", "U"])
-@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (129, 2, -3), (1000, -1, 2), (4099, 1, 1)])
+@pytest.mark.parametrize("n,incx,incy", [(1, 1, 1), (129, 2, -3), (1000, -1, 2), (4099, 1, 1),
+                                         (1024, 1, 1), (8192, 1, -1)])
 def test_dspmv(gpu, ora, uplo, n, incx, incy):
     A = uniform(96, (n, n))
     AP = _pack(A, uplo)
@@ -501,16 +502,24 @@ def test_dspmv(gpu, ora, uplo, n, incx, incy):
 
 
 def test_dspmv_alpha_zero_and_integer_exact(gpu, ora):
-    n = 900
+    """Integer entries: every partial sum is exact, so the tile kernel's
+    summation order cannot show; n = 900 has ragged edge tiles, n = 1280 only
+    whole 128 x 128 tiles (the diagonal tiles load the packed neighbours of
+    the triangle and select them away)."""
     rng = np.random.default_rng(99)
-    A = rng.integers(-9, 10, (n, n)).astype(float)
-    AP = _pack(A, "L")
+    for n, uplo in ((900, "L"), (1280, "L"), (1280, "U")):
+        A = rng.integers(-9, 10, (n, n)).astype(float)
+        AP = _pack(A, uplo)
+        x = rng.integers(-9, 10, n).astype(float)
+        y0 = rng.integers(-9, 10, n).astype(float)
+        y, r = y0.copy(), y0.copy()
+        gpu.dspmv(uplo, n, 2.0, AP, x, 1, 3.0, y, 1)
+        ora.ora_dspmv(c(uplo), n, 2.0, p(AP), p(x), 1, 3.0, p(r), 1)
+        assert np.array_equal(y, r), (n, uplo)
+    n = 900
+    AP = _pack(rng.integers(-9, 10, (n, n)).astype(float), "U")
     x = rng.integers(-9, 10, n).astype(float)
     y0 = rng.integers(-9, 10, n).astype(float)
-    y, r = y0.copy(), y0.copy()
-    gpu.dspmv("L", n, 2.0, AP, x, 1, 3.0, y, 1)
-    ora.ora_dspmv(b"L", n, 2.0, p(AP), p(x), 1, 3.0, p(r), 1)
-    assert np.array_equal(y, r)
     y = y0.copy()
     gpu.dspmv("U", n, 0.0, AP, x, 1, 0.5, y, 1)  # alpha = 0: y := beta y, AP unread
     assert np.array_equal(y, 0.5 * y0)
