@@ -60,7 +60,10 @@
 //     response to a unit (r-1)-th incoming unknown, all lanes sharing each
 //     step's coefficients; keeps the last k values of each (h_c, H_c) and
 //     flags the chunk if a response leaves [-1, 1] or a value is not finite.
-//  2. tbsv_prop, one wave: z_{c+1} = h_c + H_c z_c over the chunks.
+//  2. z_{c+1} = h_c + H_c z_c over the chunks: one wave (tbsv_prop) for few
+//     chunks; else in two levels: the maps of each group of ~sqrt(C) chunks
+//     composed in parallel (tbsv_compose), one wave over the group maps
+//     (tbsv_groups), the groups' chunks filled in in parallel (tbsv_spread).
 //  3. tbsv_reg, one wave per chunk: the one-wave recurrence restarted at the
 //     chunk with z_c taken off its first k equations, in the order the
 //     whole-length recurrence takes them: given z_c, the same arithmetic.
@@ -728,32 +731,29 @@ __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
     if (r == 0) ch.flag[c] = any != 0;
 }
 
-// One wave: z_{c+1} = h_c + H_c z_c (lane p < k holds z[p]), z_0 = 0.  z is
-// shared through LDS into NC registers laid out per row of 16 lanes and
-// applied with the DPP broadcast FMA into four partial sums (q mod 4),
-// added as h + ((s0 + s1) + (s2 + s3)).  The rows of H are loaded D chunks
-// ahead (one buffer per chunk, named statically in a D-way unrolled loop);
-// one chunk ahead left every step waiting a full memory latency.
-// Declines (flag[C] = 1) when a chunk was flagged or a z is not finite.
+// The propagation z_{c+1} = h_c + H_c z_c (map c, c = 0 .. C - 2; z_0 = 0).
+// prop_walk runs one wave's z (lane p < k holds z[p], lanes p >= k zero)
+// through maps rb .. re - 1 of Hm (row stride k * kTbHRow per map): z <-
+// (aff ? h_r : 0) + H_r z.  z is shared through LDS into NC registers laid
+// out per row of 16 lanes and applied with the DPP broadcast FMA into four
+// partial sums (q mod 4), added as h + ((s0 + s1) + (s2 + s3)).  The maps'
+// rows are loaded D maps ahead (one buffer per map, named statically in a
+// D-way unrolled loop; one map ahead left every step waiting a full memory
+// latency), unconditionally with indices clamped to re - 1 (a load under a
+// branch makes the compiler wait for every outstanding load at the join),
+// with 16-byte loads (rows are 512-byte aligned; at most 63 loads can be in
+// flight, so D (W + 1) / 2 stays below that).  Entries q > k of a row meet
+// z = 0.  After map r, z goes to Zo[(r - rb) k + p] when Zo is given.
 template <int W>
-__global__ __launch_bounds__(64) void tbsv_prop(int k, TbChunk ch)
+__device__ __forceinline__ double prop_walk(int k, const double *Hm, int rb, int re, double z,
+                                            bool aff, double *Zo, bool &bad)
 {
     constexpr int D = 3, NC = W == 16 ? 1 : 2;
     __shared__ double zs[64];
-    const int p = threadIdx.x, j16 = p & 15;
-    unsigned fl = 0;
-    for (int c = p; c < ch.C; c += 64) fl |= ch.flag[c];
-    if (__ballot(fl != 0)) {
-        if (p == 0) ch.flag[ch.C] = 1;
-        return;
-    }
-    // 16-byte loads (rows are 512-byte aligned): at most 63 loads can be in
-    // flight, so D (W + 1) / 2 must stay below that.  Entries q > k are the
-    // idle lanes' zeros (a chunk where they are not finite is flagged: the
-    // same data made lane 0 non-finite) and meet z = 0 anyway; no select
-    // after the load (it would wait for the load at once).
-    auto load = [&](int c, double (&v)[W]) {
-        const double2 *hr = (const double2 *)(ch.H + ((long)c * k + (p < k ? p : 0)) * kTbHRow);
+    const int p = threadIdx.x & 63, j16 = p & 15;
+    if (re <= rb) return z;
+    auto load = [&](int r, double (&v)[W]) {
+        const double2 *hr = (const double2 *)(Hm + ((long)r * k + (p < k ? p : 0)) * kTbHRow);
 #pragma unroll
         for (int q2 = 0; q2 < (W + 1) / 2; ++q2) {
             const double2 h = hr[q2];
@@ -762,18 +762,14 @@ __global__ __launch_bounds__(64) void tbsv_prop(int k, TbChunk ch)
         }
     };
     double zr[NC], buf[D][W];
+    zs[p] = z;  // one wave: its LDS accesses complete in order
 #pragma unroll
-    for (int h = 0; h < NC; ++h) zr[h] = 0.0;
-    if (p < k) ch.Z[p] = 0.0;
-    bool bad = false;
-    // chunk rows are loaded unconditionally (indices clamped to the last
-    // row, C - 2): a load under a branch makes the compiler wait for every
-    // outstanding load at the join
-    const int last = ch.C - 2;
+    for (int h = 0; h < NC; ++h) zr[h] = zs[16 * h + j16];
+    const int last = re - 1;
 #pragma unroll
-    for (int j = 0; j < D; ++j) load(j < last ? j : last, buf[j]);
-    // z_c from the row of chunk c - 1, held in buf[u]
-    auto step = [&](int c, int u) {
+    for (int j = 0; j < D; ++j) load(rb + j < last ? rb + j : last, buf[j]);
+    // map r, its row held in buf[u]
+    auto step = [&](int r, int u) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         if constexpr (W == 16) {
             dpp_dot15(acc, zr[0], &buf[u][1]);
@@ -781,26 +777,105 @@ __global__ __launch_bounds__(64) void tbsv_prop(int k, TbChunk ch)
             dpp_dot16(acc, zr[0], &buf[u][1]);
             dpp_dot16(acc, zr[1], &buf[u][17]);
         }
-        double z = buf[u][0] + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-        z = p < k ? z : 0.0;
-        bad |= !(fabs(z) <= DBL_MAX);
-        if (p < k) ch.Z[(long)c * k + p] = z;
-        zs[p] = z;  // one wave: its LDS accesses complete in order
+        double nz = (aff ? buf[u][0] : 0.0) + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+        nz = p < k ? nz : 0.0;
+        bad |= !(fabs(nz) <= DBL_MAX);
+        if (Zo && p < k) Zo[(long)(r - rb) * k + p] = nz;
+        zs[p] = nz;
 #pragma unroll
         for (int h = 0; h < NC; ++h) zr[h] = zs[16 * h + j16];
-        const int nx = c - 1 + D;
-        load(nx < last ? nx : last, buf[u]);
+        z = nz;
+        load(r + D < last ? r + D : last, buf[u]);
     };
-    int c0 = 1;
-    for (; c0 + D <= ch.C; c0 += D) {
+    int r0 = rb;
+    for (; r0 + D <= re; r0 += D) {
 #pragma unroll
-        for (int u = 0; u < D; ++u) step(c0 + u, u);
+        for (int u = 0; u < D; ++u) step(r0 + u, u);
     }
 #pragma unroll
     for (int u = 0; u < D; ++u)
-        if (c0 + u < ch.C) step(c0 + u, u);
+        if (r0 + u < re) step(r0 + u, u);
+    return z;
+}
+
+// Any chunk flagged: the whole chunked result is declined (flag[C] = 1).
+__device__ __forceinline__ bool tb_any_flagged(const TbChunk &ch)
+{
+    unsigned fl = 0;
+    for (int c = threadIdx.x; c < ch.C; c += 64) fl |= ch.flag[c];
+    return __ballot(fl != 0) != 0;
+}
+
+// Few chunks: one wave walks all C - 1 maps.  Declines when a chunk was
+// flagged or a z is not finite.
+template <int W>
+__global__ __launch_bounds__(64) void tbsv_prop(int k, TbChunk ch)
+{
+    const int p = threadIdx.x;
+    if (tb_any_flagged(ch)) {
+        if (p == 0) ch.flag[ch.C] = 1;
+        return;
+    }
+    if (p < k) ch.Z[p] = 0.0;
+    bool bad = false;
+    prop_walk<W>(k, ch.H, 0, ch.C - 1, 0.0, true, ch.Z + k, bad);
     const unsigned long long any = __ballot(bad);
     if (p == 0) ch.flag[ch.C] = any != 0;
+}
+
+// Many chunks, in two levels: the C - 1 maps form NG groups of G.
+// tbsv_compose: workgroup (g, q) walks e_q (q < k; linear part only) or 0
+// (q = k; with the offsets) through group g's maps, giving column q + 1 (or
+// the offset, column 0) of the group's map in GH (same layout as H; the
+// unused entries k + 1 .. W - 1 of each row are zeroed).
+template <int W>
+__global__ __launch_bounds__(64) void tbsv_compose(int k, TbChunk ch, int G, double *GH)
+{
+    const int g = blockIdx.x, q = blockIdx.y, p = threadIdx.x;
+    const int rb = g * G, re = rb + G < ch.C - 1 ? rb + G : ch.C - 1;
+    const bool aff = q == k;
+    bool bad = false;
+    const double z = prop_walk<W>(k, ch.H, rb, re, !aff && p == q ? 1.0 : 0.0, aff, nullptr, bad);
+    if (p < k) {
+        double *row = GH + ((long)g * k + p) * kTbHRow;
+        row[aff ? 0 : q + 1] = z;
+        if (aff)
+            for (int e = k + 1; e < W; ++e) row[e] = 0.0;
+    }
+}
+
+// One wave over the group maps: ZG[g] = z at the first chunk of group g.
+// Takes the verdict (flagged chunk or non-finite z: declined).
+template <int W>
+__global__ __launch_bounds__(64) void tbsv_groups(int k, TbChunk ch, int NG, const double *GH,
+                                                  double *ZG)
+{
+    const int p = threadIdx.x;
+    if (tb_any_flagged(ch)) {
+        if (p == 0) ch.flag[ch.C] = 1;
+        return;
+    }
+    if (p < k) ZG[p] = 0.0;
+    bool bad = false;
+    prop_walk<W>(k, GH, 0, NG - 1, 0.0, true, ZG + k, bad);
+    const unsigned long long any = __ballot(bad);
+    if (p == 0) ch.flag[ch.C] = any != 0;
+}
+
+// One wave per group: from ZG[g], the z of each of the group's chunks into
+// Z.  The last map of a group is not walked (the next group's z comes from
+// ZG, so each Z row has one writer), except in the last group.
+template <int W>
+__global__ __launch_bounds__(64) void tbsv_spread(int k, TbChunk ch, int G, int NG, const double *ZG)
+{
+    if (ch.flag[ch.C]) return;
+    const int g = blockIdx.x, p = threadIdx.x;
+    const int rb = g * G, re = g == NG - 1 ? ch.C - 1 : rb + G - 1;
+    const double z = p < k ? ZG[(long)g * k + p] : 0.0;
+    if (p < k) ch.Z[(long)rb * k + p] = z;
+    bool bad = false;
+    prop_walk<W>(k, ch.H, rb, re, z, true, ch.Z + (long)(rb + 1) * k, bad);
+    if (__ballot(bad) != 0 && p == 0) ch.flag[ch.C] = 1;
 }
 
 // k >= 64: window of W (power of two > k) unknowns in LDS; lane l handles the
@@ -885,11 +960,25 @@ constexpr int kTbChunkMin = 2048;
 constexpr int kTbSimds = 1024;  // 256 CUs x 4 SIMDs
 int g_tb_chunk = 1;  // gpublas_dev_tbsv_chunked
 
+// Above this many maps the propagation runs in two levels (groups of about
+// sqrt(C) maps composed in parallel, one walk over the groups, the groups'
+// chunks filled in in parallel): three short chains instead of one of C.
+constexpr int kTbPropFlat = 64;
+
 template <bool TR, bool UPPER, bool NOUNIT, int W, int P>
-void tbsv_chunk_passes(hipStream_t s, const TbArgs &a, const TbChunk &ch)
+void tbsv_chunk_passes(hipStream_t s, const TbArgs &a, const TbChunk &ch, double *GH, double *ZG,
+                       int G, int NG)
 {
     hipLaunchKernelGGL((tbsv_resp<TR, UPPER, NOUNIT, W, P>), dim3(ch.C), dim3(64), 0, s, a, ch);
-    hipLaunchKernelGGL((tbsv_prop<W>), dim3(1), dim3(64), 0, s, a.k, ch);
+    if (NG == 0) {
+        hipLaunchKernelGGL((tbsv_prop<W>), dim3(1), dim3(64), 0, s, a.k, ch);
+        return;
+    }
+    hipLaunchKernelGGL((tbsv_compose<W>), dim3(NG, a.k + 1), dim3(64), 0, s, a.k, ch, G, GH);
+    hipLaunchKernelGGL((tbsv_groups<W>), dim3(1), dim3(64), 0, s, a.k, ch, NG, (const double *)GH,
+                       ZG);
+    hipLaunchKernelGGL((tbsv_spread<W>), dim3(NG), dim3(64), 0, s, a.k, ch, G, NG,
+                       (const double *)ZG);
 }
 
 // k <= 63; chunked for k <= 32 (a 64-register window would not unroll into
@@ -912,13 +1001,21 @@ int tbsv_short(hipStream_t s, const TbArgs &a)
     const int Lmin = 64 * ceil_div(ceil_div(n, kTbSimds), 64);
     if (L < Lmin) L = Lmin;
     const int C = ceil_div(n, L);
+    int G = 0, NG = 0;  // two-level propagation: NG groups of G maps
+    if (C - 1 > kTbPropFlat) {
+        G = (int)(std::sqrt((double)(C - 1)) + 0.5);
+        NG = ceil_div(C - 1, G);
+    }
     const size_t hb = (size_t)C * k * kTbHRow, zb = (size_t)C * k;
-    Scratch scr = dev_scratch((hb + zb) * sizeof(double) + (size_t)(C + 1) * sizeof(unsigned), s);
+    const size_t gb = (size_t)NG * k * kTbHRow, zgb = (size_t)NG * k;
+    Scratch scr = dev_scratch((hb + zb + gb + zgb) * sizeof(double) + (size_t)(C + 1) * sizeof(unsigned),
+                              s);
     if (!scr.p) return (int)hipErrorOutOfMemory;
-    double *H = (double *)scr.p;
-    const TbChunk ch{L, C, H, H + hb, (unsigned *)(H + hb + zb)};
-    if (k < 16) tbsv_chunk_passes<TR, UPPER, NOUNIT, 16, 16>(s, a, ch);
-    else tbsv_chunk_passes<TR, UPPER, NOUNIT, 33, 11>(s, a, ch);
+    // H and GH first: their 512-byte rows are read with 16-byte loads
+    double *H = (double *)scr.p, *GH = H + hb, *Z = GH + gb, *ZG = Z + zb;
+    const TbChunk ch{L, C, H, Z, (unsigned *)(ZG + zgb)};
+    if (k < 16) tbsv_chunk_passes<TR, UPPER, NOUNIT, 16, 16>(s, a, ch, GH, ZG, G, NG);
+    else tbsv_chunk_passes<TR, UPPER, NOUNIT, 33, 11>(s, a, ch, GH, ZG, G, NG);
     hipLaunchKernelGGL((tbsv_reg<TR, UPPER, NOUNIT>), dim3(C), dim3(64), 0, s, a, L,
                        (const double *)ch.Z, (const unsigned *)(ch.flag + C));
     return 0;
