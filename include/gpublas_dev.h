@@ -280,7 +280,7 @@ int gpublas_dev_dtbmv(void *stream, char uplo, char trans, char diag, int n, int
 int gpublas_dev_dtbsv(void *stream, char uplo, char trans, char diag, int n, int k,
                       const double *d_A, int lda, double *d_x, int incx);
 /* DTBSV plan switch (no reference counterpart): on = 1 (the default) lets
- * k <= 32, n >= 2048 solves run chunked (per-chunk responses, a propagation
+ * k <= 63, n >= 2048 solves run chunked (per-chunk responses, a propagation
  * of the incoming unknowns, per-chunk re-solves; declined on the device, in
  * favour of the one-wave recurrence, when a response leaves [-1, 1] or a
  * value is not finite); on = 0 keeps every solve on the one-wave

@@ -52,7 +52,7 @@
 // double-rounding cases.  One wave runs the whole solve: its instruction
 // issue per column (about 25 instructions) bounds it, not HBM.
 //
-// Chunked DTBSV (k <= 32, n >= 2048).  The solve positions are cut into C
+// Chunked DTBSV (k <= 63, n >= 2048).  The solve positions are cut into C
 // chunks of L; chunk c depends on the rest only through its incoming state
 // z_c (the last k unknowns before it), and linearly: x_c = y_c + G_c z_c,
 // y_c its solve from a zero state, G_c its responses to unit states.
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(64) void tbsv_reg(TbArgs a, int L, const double *Z,
     }
 }
 
-// Chunked solve, k <= 32 (see the file header).  Buffers of one call:
+// Chunked solve, k <= 63 (see the file header).  Buffers of one call:
 //   H[c][p][r]  p = 0 .. k-1, r = 0 .. 63 (row stride 64): value at solve
 //               position (c + 1) L - k + p of the chunk's solve from a zero
 //               state (r = 0, right-hand side b) and of its response to a
@@ -629,8 +629,8 @@ __device__ __forceinline__ void dpp_dot15(double (&acc)[4], double z, const doub
 template <bool TR, bool UPPER, bool NOUNIT, int W, int P>
 __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
 {
-    static_assert(W % P == 0 && (W == 16 || W == 33), "window and prefetch ring");
-    constexpr int NC = W == 16 ? 1 : 2;
+    static_assert(W % P == 0 && (W == 16 || W == 33 || W == 64), "window and prefetch ring");
+    constexpr int NC = (W - 1 + 15) / 16;
     constexpr bool FWD = TR ? UPPER : !UPPER;
     const int r = threadIdx.x, j16 = r & 15;
     const int n = a.n, k = a.k;
@@ -654,14 +654,15 @@ __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
     // right-hand sides, one block of W at a time: lane l < W holds b of
     // equation p0 + l (a per-step load under `lane 0 only` became a branch
     // with a full wait for the load)
-    auto bvec = [&](int p0) -> double {
-        const int p = p0 + (r < W ? r : 0);
+    // b of equation p0 + l in lane l < m
+    auto bvec = [&](int p0, int m) -> double {
+        const int p = p0 + (r < m ? r : 0);
         return *(p < n ? xg + a.x0 + (long)row(p) * a.incx : zp);
     };
     const bool lane0 = r == 0;
     double w[W];
     {
-        const double b0 = bvec(s);
+        const double b0 = bvec(s, W);
 #pragma unroll
         for (int t = 0; t < W; ++t) w[t] = lane0 ? rl(b0, t) : 0.0;
     }
@@ -679,7 +680,12 @@ __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
 #pragma unroll
         for (int h = 0; h < NC; ++h) cf[o][h] = coef(s + o, h);
     }
-    double dnx = diag(s + (r < W ? r : 0)), bnx = bvec(s + W);
+    // Steps go in blocks of S; with S < W (W = 64: a 64-step block does not
+    // unroll) the window is rotated by S registers after each block, so
+    // that slot t always holds equation (block start) + t.
+    constexpr int S = W == 64 ? 16 : W;
+    static_assert(S % P == 0, "prefetch ring");
+    double dnx = diag(s + (r < S ? r : 0)), bnx = bvec(s + W, S);
     // |v| bound per lane: finite for the right-hand side b, 1 for the
     // responses (lanes r > k are idle)
     const double lim = r == 0 ? DBL_MAX : 1.0;
@@ -687,32 +693,36 @@ __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
     bool bad = false;
     const int hs = ch.L - k;  // first kept step
     double *Hc = ch.H + (long)c * k * kTbHRow + r;
-    // one block of W steps; TAIL: steps past the chunk's end or kept in H
+    // one block of S steps; TAIL: steps past the chunk's end or kept in H
     // may occur (the other blocks are branch-free, so the compiler keeps the
     // loads P steps ahead of their use)
     auto block = [&](int i0, auto tail) {
         constexpr bool TAIL = decltype(tail)::value;
         const double dcur = dnx, rcur = 1.0 / dcur, bcur = bnx;
-        dnx = diag(s + i0 + W + (r < W ? r : 0));
-        bnx = bvec(s + i0 + 2 * W);
+        dnx = diag(s + i0 + S + (r < S ? r : 0));
+        bnx = bvec(s + i0 + S + W, S);
 #pragma unroll
-        for (int o = 0; o < W; ++o) {
+        for (int o = 0; o < S; ++o) {
             const int i = i0 + o;
             if (!TAIL || i < len) {
                 const double v = NOUNIT ? qdiv(w[o], rl(dcur, o), rl(rcur, o)) : w[o];
                 const double nv = -v;
+                // register h serves distances t0 = 16 h + 1 .. t0 + 15 (the last
+                // group of W = 16 and W = 64 has 15)
 #define GB_W(t) w[(o + (t)) % W]
-                if constexpr (W == 16) {
-                    dpp_win15(GB_W(1), GB_W(2), GB_W(3), GB_W(4), GB_W(5), GB_W(6), GB_W(7), GB_W(8),
-                              GB_W(9), GB_W(10), GB_W(11), GB_W(12), GB_W(13), GB_W(14), GB_W(15),
-                              cf[o % P][0], nv);
-                } else {
-                    dpp_win16(GB_W(1), GB_W(2), GB_W(3), GB_W(4), GB_W(5), GB_W(6), GB_W(7), GB_W(8),
-                              GB_W(9), GB_W(10), GB_W(11), GB_W(12), GB_W(13), GB_W(14), GB_W(15),
-                              GB_W(16), cf[o % P][0], nv);
-                    dpp_win16(GB_W(17), GB_W(18), GB_W(19), GB_W(20), GB_W(21), GB_W(22), GB_W(23),
-                              GB_W(24), GB_W(25), GB_W(26), GB_W(27), GB_W(28), GB_W(29), GB_W(30),
-                              GB_W(31), GB_W(32), cf[o % P][1], nv);
+#pragma unroll
+                for (int h = 0; h < NC; ++h) {
+                    const int t0 = 16 * h + 1;
+                    if (W - t0 >= 16)
+                        dpp_win16(GB_W(t0), GB_W(t0 + 1), GB_W(t0 + 2), GB_W(t0 + 3), GB_W(t0 + 4),
+                                  GB_W(t0 + 5), GB_W(t0 + 6), GB_W(t0 + 7), GB_W(t0 + 8), GB_W(t0 + 9),
+                                  GB_W(t0 + 10), GB_W(t0 + 11), GB_W(t0 + 12), GB_W(t0 + 13),
+                                  GB_W(t0 + 14), GB_W(t0 + 15), cf[o % P][h], nv);
+                    else
+                        dpp_win15(GB_W(t0), GB_W(t0 + 1), GB_W(t0 + 2), GB_W(t0 + 3), GB_W(t0 + 4),
+                                  GB_W(t0 + 5), GB_W(t0 + 6), GB_W(t0 + 7), GB_W(t0 + 8), GB_W(t0 + 9),
+                                  GB_W(t0 + 10), GB_W(t0 + 11), GB_W(t0 + 12), GB_W(t0 + 13),
+                                  GB_W(t0 + 14), cf[o % P][h], nv);
                 }
 #undef GB_W
                 w[o] = lane0 ? rl(bcur, o) : 0.0;  // equation s + i + W enters
@@ -722,11 +732,18 @@ __global__ __launch_bounds__(64) void tbsv_resp(TbArgs a, TbChunk ch)
                 for (int h = 0; h < NC; ++h) cf[o % P][h] = coef(s + i + P, h);
             }
         }
+        if constexpr (S < W) {
+            double t[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) t[q] = w[(q + S) % W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) w[q] = t[q];
+        }
     };
     const int plain = len < hs ? len : hs;
     int i0 = 0;
-    for (; i0 + W <= plain; i0 += W) block(i0, BoolTag<false>{});
-    for (; i0 < len; i0 += W) block(i0, BoolTag<true>{});
+    for (; i0 + S <= plain; i0 += S) block(i0, BoolTag<false>{});
+    for (; i0 < len; i0 += S) block(i0, BoolTag<true>{});
     const unsigned long long any = __ballot(bad);
     if (r == 0) ch.flag[c] = any != 0;
 }
@@ -748,7 +765,9 @@ template <int W>
 __device__ __forceinline__ double prop_walk(int k, const double *Hm, int rb, int re, double z,
                                             bool aff, double *Zo, bool &bad)
 {
-    constexpr int D = 3, NC = W == 16 ? 1 : 2;
+    // D (W + 1) / 2 loads in flight: 51 for W = 33, 64 (one over the
+    // counter's 63: the last issue waits) for W = 64
+    constexpr int D = W == 64 ? 2 : 3, NC = (W - 1 + 15) / 16;
     __shared__ double zs[64];
     const int p = threadIdx.x & 63, j16 = p & 15;
     if (re <= rb) return z;
@@ -771,11 +790,10 @@ __device__ __forceinline__ double prop_walk(int k, const double *Hm, int rb, int
     // map r, its row held in buf[u]
     auto step = [&](int r, int u) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        if constexpr (W == 16) {
-            dpp_dot15(acc, zr[0], &buf[u][1]);
-        } else {
-            dpp_dot16(acc, zr[0], &buf[u][1]);
-            dpp_dot16(acc, zr[1], &buf[u][17]);
+#pragma unroll
+        for (int h = 0; h < NC; ++h) {
+            if (W - 1 - 16 * h >= 16) dpp_dot16(acc, zr[h], &buf[u][1 + 16 * h]);
+            else dpp_dot15(acc, zr[h], &buf[u][1 + 16 * h]);
         }
         double nz = (aff ? buf[u][0] : 0.0) + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
         nz = p < k ? nz : 0.0;
@@ -954,7 +972,7 @@ __global__ __launch_bounds__(64) void tbsv_lds(TbArgs a, int W)
     }
 }
 
-// Chunked solve for k <= 32 from this length on (below it the one-wave
+// Chunked solve for k <= 63 from this length on (below it the one-wave
 // recurrence's few hundred microseconds are not worth three launches).
 constexpr int kTbChunkMin = 2048;
 constexpr int kTbSimds = 1024;  // 256 CUs x 4 SIMDs
@@ -981,15 +999,14 @@ void tbsv_chunk_passes(hipStream_t s, const TbArgs &a, const TbChunk &ch, double
                        (const double *)ZG);
 }
 
-// k <= 63; chunked for k <= 32 (a 64-register window would not unroll into
-// registers).  The chunk length L ~ sqrt(0.7 n) (multiple of 64) balances the
+// k <= 63 (windows of 16, 33 or 64 registers).  The chunk length L ~ sqrt(0.7 n) (multiple of 64) balances the
 // per-chunk passes (about L steps each) against the C = n / L steps of the
 // propagation, but gives at most one chunk per SIMD.
 template <bool TR, bool UPPER, bool NOUNIT>
 int tbsv_short(hipStream_t s, const TbArgs &a)
 {
     const int n = a.n, k = a.k;
-    if (!g_tb_chunk || n < kTbChunkMin || k > 32) {
+    if (!g_tb_chunk || n < kTbChunkMin) {
         hipLaunchKernelGGL((tbsv_reg<TR, UPPER, NOUNIT>), dim3(1), dim3(64), 0, s, a, n,
                            (const double *)nullptr, (const unsigned *)nullptr);
         return 0;
@@ -1015,7 +1032,8 @@ int tbsv_short(hipStream_t s, const TbArgs &a)
     double *H = (double *)scr.p, *GH = H + hb, *Z = GH + gb, *ZG = Z + zb;
     const TbChunk ch{L, C, H, Z, (unsigned *)(ZG + zgb)};
     if (k < 16) tbsv_chunk_passes<TR, UPPER, NOUNIT, 16, 16>(s, a, ch, GH, ZG, G, NG);
-    else tbsv_chunk_passes<TR, UPPER, NOUNIT, 33, 11>(s, a, ch, GH, ZG, G, NG);
+    else if (k < 33) tbsv_chunk_passes<TR, UPPER, NOUNIT, 33, 11>(s, a, ch, GH, ZG, G, NG);
+    else tbsv_chunk_passes<TR, UPPER, NOUNIT, 64, 8>(s, a, ch, GH, ZG, G, NG);
     hipLaunchKernelGGL((tbsv_reg<TR, UPPER, NOUNIT>), dim3(C), dim3(64), 0, s, a, L,
                        (const double *)ch.Z, (const unsigned *)(ch.flag + C));
     return 0;

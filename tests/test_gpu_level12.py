@@ -711,14 +711,14 @@ def _tb_dominant(n, k, uplo, lda, seed, unit):
 
 
 TB_CHUNK_CASES = [(2048, 0, 1), (2048, 1, 1), (4099, 7, -2), (100_003, 15, 1), (100_003, 16, 3),
-                  (1 << 20, 32, 1), (70_001, 32, -1)]
+                  (1 << 20, 32, 1), (70_001, 32, -1), (65_537, 33, 1), (50_000, 63, 2)]
 
 
 @pytest.mark.parametrize("uplo", ["L", "U"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("n,k,incx", TB_CHUNK_CASES)
 def test_dtbsv_chunked(gpu, ora, uplo, trans, n, k, incx):
-    """k <= 32, n >= 2048 runs chunked: per-chunk responses, the incoming
+    """k <= 63, n >= 2048 runs chunked: per-chunk responses, the incoming
     unknowns propagated chunk to chunk, per-chunk re-solves.  Against the
     oracle's sequential solve to 1e-12 relative (diagonally dominant bands,
     stored and unit diagonal); the first 64 unknowns in solve order (chunk 0 is the
