@@ -19,6 +19,7 @@
  *                                                   diagonally dominant triangle)
  *   blas_bench --lib ... --op ger --m M --n N        (DGER, A read and written once)
  *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
+ *   blas_bench --lib ... --op tpmv --n N --uplo L --ta N --diag N   (DTPMV, packed)
  *   blas_bench --lib ... --op gbmv --m M --n N --k KL=KU --ta N   (DGBMV, the band once)
  *   blas_bench --lib ... --op sbmv|tbmv|tbsv --n N --k K --uplo L [--ta N --diag N]
  *              [--dominant]  (tbsv: diagonally dominant band)
@@ -243,6 +244,9 @@ int main(int argc, char **argv)
     typedef void (*spmv_t)(const char *, const int *, const double *, const double *,
                            const double *, const int *, const double *, double *, const int *);
     spmv_t spmv = NULL;
+    typedef void (*tpmv_t)(const char *, const char *, const char *, const int *, const double *,
+                           double *, const int *);
+    tpmv_t tpmv = NULL;
     typedef void (*gbmv_t)(const char *, const int *, const int *, const int *, const int *,
                            const double *, const double *, const int *, const double *,
                            const int *, const double *, double *, const int *);
@@ -366,6 +370,12 @@ int main(int argc, char **argv)
         C = operand((size_t)n, 'd', seed + 2);
         bytes = 8.0 * ((double)n * (n + 1) / 2 + 3.0 * n); /* packed triangle once, x, y */
         flops = 2.0 * n * n;
+    } else if (!strcmp(op, "tpmv")) {
+        tpmv = (tpmv_t)sym("dtpmv_");
+        A = operand((size_t)n * (n + 1) / 2, 'd', seed);
+        B = operand((size_t)n, 'd', seed + 1);
+        bytes = 8.0 * ((double)n * (n + 1) / 2 + 2.0 * n); /* packed triangle once, x in and out */
+        flops = 1.0 * n * n;
     } else if (!strcmp(op, "gbmv")) {
         gbmv = (gbmv_t)sym("dgbmv_");
         bw = 2 * k + 1;
@@ -445,6 +455,9 @@ int main(int argc, char **argv)
             const double a = 1.0, b = 0.5;
             const int one = 1;
             spmv(&uplo, &n, &a, (const double *)A, (const double *)B, &one, &b, (double *)C, &one);
+        } else if (tpmv) {
+            const int one = 1;
+            tpmv(&uplo, &ta, &diag, &n, (const double *)A, (double *)B, &one);
         } else if (gbmv) {
             const double a = 1.0, b = 0.5;
             const int one = 1;

@@ -1,5 +1,7 @@
 // blas2.hip -- Level-2 kernels on gfx950: DGEMV (N and T), DGER, DSYMV,
-// DSYR, DSYR2, and the packed rank updates DSPR, DSPR2 (DTRSV: trsv.hip).
+// DSPMV, DTRMV, DTPMV (one tile kernel over a stored triangle, full or
+// packed), DSYR, DSYR2, and the packed rank updates DSPR, DSPR2 (DTRSV:
+// trsv.hip).
 //
 // Replaces the vendor calls of the reference's unbuilt CBLAS wrappers:
 // cublasDgemv (blas_level2/gemv.cc:97), cublasDger (ger.cc:92).
@@ -240,7 +242,9 @@ __global__ __launch_bounds__(256) void ger_stream(int m, int n, double alpha, co
 // each); symv_fold adds them per output in a fixed order and applies alpha /
 // beta.  Diagonal tiles count the diagonal once, in the row sum.  Ragged
 // edge tiles and a misaligned A (16-B loads need an even lda) take a
-// per-element tested path.
+// per-element tested path.  The same tiles give DTRMV / DTPMV x := op(T) x
+// with only the row sums ('N') or only the column sums ('T'): one read of the
+// triangle, no copy of x (the fold writes x after every tile has read it).
 constexpr int SY_T = 128;  // tile rows and columns
 constexpr int SY_C = 8;    // columns per LDS pass
 
@@ -250,6 +254,10 @@ constexpr int SY_C = 8;    // columns per LDS pass
 // reaches the arithmetic), and edge (ragged rows or columns, or a misaligned
 // A: every element tested before it is loaded).
 enum { SY_INTERIOR = 0, SY_DIAG = 1, SY_EDGE = 2 };
+// What a tile contributes: DSYMV / DSPMV both sums (the diagonal in the row
+// sum only); op(T) x for a stored triangle T (DTRMV / DTPMV) the row sums
+// ('N') or the column sums ('T'), the diagonal once (1 when unit).
+enum { SY_MV_SYM = 0, SY_MV_N = 1, SY_MV_T = 2 };
 
 // Packed storage (DSPMV, PACKED = true): column j of the triangle starts at
 // j (j + 1) / 2 (upper) or j (2n - j - 1) / 2 (lower, rows from j), at any
@@ -265,7 +273,7 @@ __device__ __forceinline__ long sy_at(long i, long j, long lda, int n)
     return LOWER ? i + j * (2 * (long)n - j - 1) / 2 : i + j * (j + 1) / 2;
 }
 
-template <int KIND, bool LOWER, bool PACKED>
+template <int KIND, bool LOWER, bool PACKED, bool UNIT = false>
 __device__ __forceinline__ void symv_load(d2 (&a)[SY_C], const double *A, long lda, long ra,
                                           long rb, long j0, int n, bool diag)
 {
@@ -287,19 +295,19 @@ __device__ __forceinline__ void symv_load(d2 (&a)[SY_C], const double *A, long l
             } else {
                 v = __builtin_nontemporal_load((const d2 *)(A + ra + j * lda));
             }
-            a[u].x = (LOWER ? ra >= j : ra <= j) ? v.x : 0.0;
-            a[u].y = (LOWER ? rb >= j : rb <= j) ? v.y : 0.0;
+            a[u].x = (LOWER ? ra >= j : ra <= j) ? (UNIT && ra == j ? 1.0 : v.x) : 0.0;
+            a[u].y = (LOWER ? rb >= j : rb <= j) ? (UNIT && rb == j ? 1.0 : v.y) : 0.0;
         } else {
             const bool in = j < n;
             const bool s0 = in && ra < n && (!diag || (LOWER ? ra >= j : ra <= j));
             const bool s1 = in && rb < n && (!diag || (LOWER ? rb >= j : rb <= j));
-            a[u].x = s0 ? A[sy_at<PACKED, LOWER>(ra, j, lda, n)] : 0.0;
-            a[u].y = s1 ? A[sy_at<PACKED, LOWER>(rb, j, lda, n)] : 0.0;
+            a[u].x = s0 ? (UNIT && ra == j ? 1.0 : A[sy_at<PACKED, LOWER>(ra, j, lda, n)]) : 0.0;
+            a[u].y = s1 ? (UNIT && rb == j ? 1.0 : A[sy_at<PACKED, LOWER>(rb, j, lda, n)]) : 0.0;
         }
     }
 }
 
-template <int KIND, bool LOWER, bool PACKED>
+template <int KIND, bool LOWER, bool PACKED, int MODE, bool UNIT>
 __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double *A,
                                                long lda, d2 xi, const double *xs, double *rowP,
                                                double *colP, double (*pr)[65])
@@ -316,20 +324,26 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
     auto pass = [&](int c0, d2 (&cur)[SY_C], d2 (&nxt)[SY_C]) {
 #pragma unroll
         for (int u = 0; u < SY_C; ++u) {
-            const double xj = xs[c0 + u];
-            r0 = fma(cur[u].x, xj, r0);
-            r1 = fma(cur[u].y, xj, r1);
-            double px = cur[u].x * xi.x, py = cur[u].y * xi.y;
-            if (KIND != SY_INTERIOR && diag) {  // the diagonal element: row sum only
-                const long j = j0 + c0 + u;
-                if (ra == j) px = 0.0;
-                if (rb == j) py = 0.0;
+            if (MODE != SY_MV_T) {
+                const double xj = xs[c0 + u];
+                r0 = fma(cur[u].x, xj, r0);
+                r1 = fma(cur[u].y, xj, r1);
             }
-            pr[u][lane] = px + py;
+            if (MODE != SY_MV_N) {
+                double px = cur[u].x * xi.x, py = cur[u].y * xi.y;
+                if (MODE == SY_MV_SYM && KIND != SY_INTERIOR && diag) {
+                    // the diagonal element: row sum only
+                    const long j = j0 + c0 + u;
+                    if (ra == j) px = 0.0;
+                    if (rb == j) py = 0.0;
+                }
+                pr[u][lane] = px + py;
+            }
         }
         // the next pass's loads fly while this pass's column sums are formed
         if (c0 + SY_C < SY_T)
-            symv_load<KIND, LOWER, PACKED>(nxt, A, lda, ra, rb, j0 + c0 + SY_C, n, diag);
+            symv_load<KIND, LOWER, PACKED, UNIT>(nxt, A, lda, ra, rb, j0 + c0 + SY_C, n, diag);
+        if constexpr (MODE == SY_MV_N) return;  // no column sums
         lds_barrier();
         {
             const int c = lane % SY_C, part = lane / SY_C;
@@ -344,7 +358,7 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
         lds_barrier();  // pr is rewritten by the next pass
     };
     d2 a[SY_C], b[SY_C];
-    symv_load<KIND, LOWER, PACKED>(a, A, lda, ra, rb, j0, n, diag);
+    symv_load<KIND, LOWER, PACKED, UNIT>(a, A, lda, ra, rb, j0, n, diag);
     lds_barrier();  // xs is in LDS; the first pass's loads stay in flight
     // The copy at the end of a pass waits for the next pass's loads, after
     // they have overlapped the column sums.  Measured at n = 8192: 49.9 us
@@ -356,12 +370,13 @@ __device__ __forceinline__ void symv_tile_body(int n, int I, int J, const double
 #pragma unroll
         for (int u = 0; u < SY_C; ++u) a[u] = b[u];
     }
+    if (MODE == SY_MV_T) return;
     if (!EDGE || ra < n) rowP[(long)J * n + ra] = r0;
     if (!EDGE || rb < n) rowP[(long)J * n + rb] = r1;
 }
 
-template <bool LOWER, bool PACKED>  // template parameters: as run-time values the
-                                    // triangle tests turn into branches that wait on every load
+template <bool LOWER, bool PACKED, int MODE, bool UNIT>  // template parameters: as run-time
+                                    // values the triangle tests turn into branches that wait on every load
 __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const double *A,
                                                 long lda, const double *x, long incx, long x0,
                                                 double *rowP, double *colP)
@@ -398,11 +413,12 @@ __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const do
     xi.y = rb < n ? x[x0 + rb * incx] : 0.0;
     const bool edge = !vec || (long)(I + 1) * SY_T > n || (long)(J + 1) * SY_T > n;
     if (edge)
-        symv_tile_body<SY_EDGE, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_EDGE, LOWER, PACKED, MODE, UNIT>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
     else if (I == J)
-        symv_tile_body<SY_DIAG, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_DIAG, LOWER, PACKED, MODE, UNIT>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
     else
-        symv_tile_body<SY_INTERIOR, LOWER, PACKED>(n, I, J, A, lda, xi, xs, rowP, colP, pr);
+        symv_tile_body<SY_INTERIOR, LOWER, PACKED, MODE, UNIT>(n, I, J, A, lda, xi, xs, rowP, colP,
+                                                              pr);
 }
 
 // Fold: 8 lanes per output.  Output k (block b) sums its row partials
@@ -411,9 +427,10 @@ __global__ __launch_bounds__(64) void symv_tile(int n, int nb, int vec, const do
 // tree joins the 8 lanes, so the summation order depends only on n.
 constexpr int SY_FL = 8;
 
-__global__ __launch_bounds__(256) void symv_fold(int n, int lower, int nb, const double *rowP,
-                                                 const double *colP, double alpha, double beta,
-                                                 double *y, long incy, long y0)
+__global__ __launch_bounds__(256) void symv_fold(int n, int lower, int nb, int mode,
+                                                 const double *rowP, const double *colP,
+                                                 double alpha, double beta, double *y, long incy,
+                                                 long y0)
 {
     const long gid = (long)blockIdx.x * 256 + threadIdx.x;
     const long k = gid / SY_FL;
@@ -421,8 +438,8 @@ __global__ __launch_bounds__(256) void symv_fold(int n, int lower, int nb, const
     double s = 0;
     if (k < n) {
         const int b = (int)(k / SY_T);
-        const int j_lo = lower ? 0 : b, nrow = lower ? b + 1 : nb - b;
-        const int i_lo = lower ? b : 0, ncol = lower ? nb - b : b + 1;
+        const int j_lo = lower ? 0 : b, nrow = mode == SY_MV_T ? 0 : lower ? b + 1 : nb - b;
+        const int i_lo = lower ? b : 0, ncol = mode == SY_MV_N ? 0 : lower ? nb - b : b + 1;
         const int tot = nrow + ncol;
         // 8 loads in flight per lane, then added in entry order
         for (int e0 = g; e0 < tot; e0 += 8 * SY_FL) {
@@ -701,9 +718,12 @@ int gpublas_dev_dger(void *stream, int m, int n, double alpha, const double *d_x
 }  // extern "C"
 
 namespace {
-// tiles + fold of DSYMV / DSPMV (full or packed storage)
-int symv_run(hipStream_t s, int lower, bool packed, int vec, int n, double alpha, const double *A,
-             int lda, const double *x, int incx, double beta, double *y, int incy, long y0)
+// tiles + fold of DSYMV / DSPMV (mode SY_MV_SYM) and of DTRMV / DTPMV
+// (SY_MV_N / SY_MV_T, the result written over x through the fold: every tile
+// has read x before the fold starts), full or packed storage
+int symv_run(hipStream_t s, int lower, bool packed, int mode, bool unit, int vec, int n,
+             double alpha, const double *A, int lda, const double *x, int incx, double beta,
+             double *y, int incy, long y0)
 {
     const int nb = ceil_div(n, SY_T);
     const long tiles = (long)nb * (nb + 1) / 2;
@@ -711,24 +731,31 @@ int symv_run(hipStream_t s, int lower, bool packed, int vec, int n, double alpha
     double *part = (double *)scr.p;
     if (!part) return (int)hipErrorOutOfMemory;
     double *rowP = part, *colP = part + (size_t)nb * n;
-    const char *name = packed ? "dspmv" : "dsymv";
-    PROF_BEGIN(s, name);
-#define GB_SYT(L, P)                                                                         \
-    hipLaunchKernelGGL((symv_tile<L, P>), dim3((unsigned)tiles), dim3(64), 0, s, n, nb, vec, A, \
-                       (long)lda, x, (long)incx, start_of(n, incx), rowP, colP)
-    if (packed) {
-        if (lower) GB_SYT(true, true);
-        else GB_SYT(false, true);
-    } else {
-        if (lower) GB_SYT(true, false);
-        else GB_SYT(false, false);
+    static const char *const names[2][3] = {{"dsymv", "dtrmv", "dtrmv"}, {"dspmv", "dtpmv", "dtpmv"}};
+    PROF_BEGIN(s, names[packed][mode]);
+#define GB_SYT(L, P, M, U)                                                                        \
+    hipLaunchKernelGGL((symv_tile<L, P, M, U>), dim3((unsigned)tiles), dim3(64), 0, s, n, nb, vec, \
+                       A, (long)lda, x, (long)incx, start_of(n, incx), rowP, colP)
+#define GB_SYT_M(L, P)                                                                            \
+    {                                                                                             \
+        if (mode == SY_MV_SYM) GB_SYT(L, P, SY_MV_SYM, false);                                    \
+        else if (mode == SY_MV_N) { if (unit) GB_SYT(L, P, SY_MV_N, true); else GB_SYT(L, P, SY_MV_N, false); } \
+        else { if (unit) GB_SYT(L, P, SY_MV_T, true); else GB_SYT(L, P, SY_MV_T, false); }       \
     }
+    if (packed) {
+        if (lower) GB_SYT_M(true, true) else GB_SYT_M(false, true)
+    } else {
+        if (lower) GB_SYT_M(true, false) else GB_SYT_M(false, false)
+    }
+#undef GB_SYT_M
 #undef GB_SYT
     PROF_END(s);
-    PROF_BEGIN(s, packed ? "dspmv_fold" : "dsymv_fold");
+    static const char *const fnames[2][3] = {{"dsymv_fold", "dtrmv_fold", "dtrmv_fold"},
+                                             {"dspmv_fold", "dtpmv_fold", "dtpmv_fold"}};
+    PROF_BEGIN(s, fnames[packed][mode]);
     hipLaunchKernelGGL(symv_fold, dim3((unsigned)ceil_div((long)n * SY_FL, 256)), dim3(256), 0, s,
-                       n, lower, nb, (const double *)rowP, (const double *)colP, alpha, beta, y,
-                       (long)incy, y0);
+                       n, lower, nb, mode, (const double *)rowP, (const double *)colP, alpha, beta,
+                       y, (long)incy, y0);
     PROF_END(s);
     return (int)hipGetLastError();
 }
@@ -752,7 +779,8 @@ int gpublas_dev_dsymv(void *stream, char uplo, int n, double alpha, const double
     }
     const int lower = uplo == 'L' || uplo == 'l';
     const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
-    return symv_run(s, lower, false, vec, n, alpha, d_A, lda, d_x, incx, beta, d_y, incy, y0);
+    return symv_run(s, lower, false, SY_MV_SYM, false, vec, n, alpha, d_A, lda, d_x, incx, beta, d_y,
+                    incy, y0);
 }
 
 // DSPMV: the DSYMV tile kernel on packed storage (one read of the packed
@@ -764,8 +792,36 @@ int gpublas_dev_dspmv(void *stream, char uplo, int n, double alpha, const double
     if (n <= 0) return 0;
     if (alpha == 0.0) return gpublas_dev_dsymv(s, uplo, n, alpha, nullptr, n, d_x, incx, beta, d_y, incy);
     const int lower = uplo == 'L' || uplo == 'l';
-    return symv_run(s, lower, true, 1, n, alpha, d_AP, 0, d_x, incx, beta, d_y, incy,
-                    start_of(n, incy));
+    return symv_run(s, lower, true, SY_MV_SYM, false, 1, n, alpha, d_AP, 0, d_x, incx, beta, d_y,
+                    incy, start_of(n, incy));
+}
+
+// DTRMV / DTPMV x := op(T) x: the tile kernel's row sums ('N') or column sums
+// ('T') over the stored triangle, read once; the fold writes x (replaces
+// cublasDtrmv / cublasDtpmv, blas_level2/trmv.cc, tpmv.cc)
+int gpublas_dev_dtrmv(void *stream, char uplo, char trans, char diag, int n, const double *d_A,
+                      int lda, double *d_x, int incx)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    const int lower = uplo == 'L' || uplo == 'l';
+    const int mode = trans == 'N' || trans == 'n' ? SY_MV_N : SY_MV_T;
+    const bool unit = diag == 'U' || diag == 'u';
+    const int vec = (((uintptr_t)d_A & 15) == 0) && (lda % 2 == 0);
+    return symv_run(s, lower, false, mode, unit, vec, n, 1.0, d_A, lda, d_x, incx, 0.0, d_x, incx,
+                    start_of(n, incx));
+}
+
+int gpublas_dev_dtpmv(void *stream, char uplo, char trans, char diag, int n, const double *d_AP,
+                      double *d_x, int incx)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    const int lower = uplo == 'L' || uplo == 'l';
+    const int mode = trans == 'N' || trans == 'n' ? SY_MV_N : SY_MV_T;
+    const bool unit = diag == 'U' || diag == 'u';
+    return symv_run(s, lower, true, mode, unit, 1, n, 1.0, d_AP, 0, d_x, incx, 0.0, d_x, incx,
+                    start_of(n, incx));
 }
 
 int gpublas_dev_dsyr(void *stream, char uplo, int n, double alpha, const double *d_x, int incx,
