@@ -19,7 +19,7 @@
  *                                                   diagonally dominant triangle)
  *   blas_bench --lib ... --op ger --m M --n N        (DGER, A read and written once)
  *   blas_bench --lib ... --op spmv --n N --uplo L   (DSPMV on packed storage)
- *   blas_bench --lib ... --op tpmv --n N --uplo L --ta N --diag N   (DTPMV, packed)
+ *   blas_bench --lib ... --op tpmv|tpsv --n N --uplo L --ta N --diag N   (DTPMV / DTPSV, packed)
  *   blas_bench --lib ... --op gbmv --m M --n N --k KL=KU --ta N   (DGBMV, the band once)
  *   blas_bench --lib ... --op sbmv|tbmv|tbsv --n N --k K --uplo L [--ta N --diag N]
  *              [--dominant]  (tbsv: diagonally dominant band)
@@ -370,10 +370,19 @@ int main(int argc, char **argv)
         C = operand((size_t)n, 'd', seed + 2);
         bytes = 8.0 * ((double)n * (n + 1) / 2 + 3.0 * n); /* packed triangle once, x, y */
         flops = 2.0 * n * n;
-    } else if (!strcmp(op, "tpmv")) {
-        tpmv = (tpmv_t)sym("dtpmv_");
+    } else if (!strcmp(op, "tpmv") || !strcmp(op, "tpsv")) {
+        tpmv = (tpmv_t)sym(op[2] == 'm' ? "dtpmv_" : "dtpsv_");
         A = operand((size_t)n * (n + 1) / 2, 'd', seed);
         B = operand((size_t)n, 'd', seed + 1);
+        if (op[2] == 's') { /* as for trsv: diagonal n + uniform[0,1] (packed positions) */
+            uint64_t s = seed + 5;
+            const int lo = uplo == 'L' || uplo == 'l';
+            for (size_t j = 0; j < (size_t)n; ++j) {
+                s = s * 6364136223846793005ull + 1442695040888963407ull;
+                const size_t d = lo ? j * (2 * (size_t)n - j + 1) / 2 : j * (j + 1) / 2 + j;
+                ((double *)A)[d] = n + (double)(s >> 11) * 0x1.0p-53;
+            }
+        }
         bytes = 8.0 * ((double)n * (n + 1) / 2 + 2.0 * n); /* packed triangle once, x in and out */
         flops = 1.0 * n * n;
     } else if (!strcmp(op, "gbmv")) {

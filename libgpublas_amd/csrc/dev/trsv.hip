@@ -1,5 +1,6 @@
 // trsv.hip -- DTRSV on gfx950: solve op(A) x = b in place (replaces
-// cublasDtrsv, blas_level2/trsv.cc:81-88 of the reference; netlib semantics).
+// cublasDtrsv, blas_level2/trsv.cc:81-88 of the reference; netlib semantics),
+// and DTPSV, the same solve on the packed triangle.
 //
 // Orders above 128: one launch (trsv_lookback, below).  The solver
 // workgroup of block t applies the last one or two earlier blocks' tiles and
@@ -188,7 +189,19 @@ struct TrsvLb {
     u64 *pmail;        // nb * nh * kMailWords words
     long long timeout; // steady-clock ticks a wait may last
     int vec;           // A 16-B aligned with even lda
+    int packed, plower;  // A is the packed triangle (DTPSV), lower or upper
 };
+
+// Column j of A: element (i, j) is colp(g, j)[i].  Packed storage (netlib
+// column packing) starts column j at j (j + 1) / 2 (upper) or, shifted by the
+// j rows it does not store, at j (2n - j - 1) / 2 (lower).  Inside a diagonal
+// block the unstored elements' addresses fall on neighbouring columns, still
+// inside the packed array; their values are discarded like full storage's.
+__device__ __forceinline__ const double *colp(const TrsvLb &g, long j)
+{
+    if (!g.packed) return g.A + j * g.lda;
+    return g.A + (g.plower ? j * (2 * (long)g.n - j - 1) / 2 : j * (j + 1) / 2);
+}
 
 template <bool FWD>
 __device__ __forceinline__ void block_of(const TrsvLb &g, int t, int &b0, int &bn)
@@ -274,20 +287,28 @@ __device__ __forceinline__ void tile_load(const TrsvLb &g, int p0, int pn, int c
                                           Tile &tl)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool full = g.vec && pn == TB && cn == TB;
+    // whole tiles load unguarded: 16-B loads when A allows them, else
+    // (packed columns at any 8-byte offset) two 8-B loads
+    const bool whole = pn == TB && cn == TB, full = g.vec && whole;
     if constexpr (!TRANS) {
         const long r = c0 + 2 * lane;
         const int k0 = 16 * wave;
         if (full) {
 #pragma unroll
             for (int u = 0; u < 16; ++u)
-                tl.v[u] = __builtin_nontemporal_load(
-                    (const d2 *)(g.A + r + (long)(p0 + k0 + u) * g.lda));
+                tl.v[u] = __builtin_nontemporal_load((const d2 *)(colp(g, p0 + k0 + u) + r));
+        } else if (whole) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const double *a = colp(g, p0 + k0 + u) + r;
+                tl.v[u].x = __builtin_nontemporal_load(a);
+                tl.v[u].y = __builtin_nontemporal_load(a + 1);
+            }
         } else {
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 const bool kk = k0 + u < pn;
-                const double *a = g.A + r + (long)(p0 + k0 + u) * g.lda;
+                const double *a = colp(g, p0 + k0 + u) + r;
                 tl.v[u].x = (kk && 2 * lane < cn) ? a[0] : 0.0;
                 tl.v[u].y = (kk && 2 * lane + 1 < cn) ? a[1] : 0.0;
             }
@@ -297,12 +318,15 @@ __device__ __forceinline__ void tile_load(const TrsvLb &g, int p0, int pn, int c
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int ii = 16 * wave + 8 * half + c;
-            const double *col = g.A + p0 + (long)(c0 + ii) * g.lda;
+            const double *col = colp(g, c0 + ii) + p0;
 #pragma unroll
             for (int q = 0; q < TB / 16; ++q) {
                 const int e = 16 * q + 2 * j;
                 if (full) {
                     tl.v[8 * half + q] = __builtin_nontemporal_load((const d2 *)(col + e));
+                } else if (whole) {
+                    tl.v[8 * half + q].x = __builtin_nontemporal_load(col + e);
+                    tl.v[8 * half + q].y = __builtin_nontemporal_load(col + e + 1);
                 } else {
                     tl.v[8 * half + q].x = (ii < cn && e < pn) ? col[e] : 0.0;
                     tl.v[8 * half + q].y = (ii < cn && e + 1 < pn) ? col[e + 1] : 0.0;
@@ -436,17 +460,23 @@ __device__ void lb_solver(const TrsvLb &g, int t, double *D, double (*part)[TB],
     for (int u = 0; u < PER; ++u) {
         const int idx = tid + u * TTHREADS;
         const int a = 2 * (idx % (TB / 2)), b = idx / (TB / 2);
-        const double *src = g.A + (c0 + a) + (long)(c0 + b) * g.lda;
+        const double *src = colp(g, c0 + b) + (c0 + a);
         if (vec && a + 1 < cn && b < cn) {
             const d2 v = *(const d2 *)src;
             va[u] = v.x;
             vb[u] = v.y;
+        } else if (cn == TB) {
+            // a whole block: every address is inside A (packed: the unstored
+            // half's addresses are the neighbouring columns'), and no select
+            // follows the loads (it would wait for each one in turn)
+            va[u] = src[0];
+            vb[u] = src[1];
         } else {
             va[u] = (a < cn && b < cn) ? src[0] : 0.0;
             vb[u] = (a + 1 < cn && b < cn) ? src[1] : 0.0;
         }
     }
-    const double dg = (!UNIT && tid < cn) ? g.A[(c0 + tid) + (long)(c0 + tid) * g.lda] : 1.0;
+    const double dg = (!UNIT && tid < cn) ? colp(g, c0 + tid)[c0 + tid] : 1.0;
     double acc0 = 0.0, acc1 = 0.0;  // wave 0: b of indices lane, lane + 64, minus the sums
     if (wave == 0) {
         acc0 = lane < cn ? g.x[c0 + lane] : 0.0;
@@ -550,7 +580,7 @@ __global__ void zero_words(u64 *f, long n)
 // to 128 (one block: no mailboxes) is the launch alone.  The mailboxes come
 // from the stream's scratch (released behind the stream's work).
 template <bool LOWER, bool TRANS, bool UNIT>
-int run(hipStream_t s, int n, const double *A, long lda, double *x)
+int run(hipStream_t s, int n, const double *A, long lda, bool packed, double *x)
 {
     TrsvLb g;
     g.n = n;
@@ -559,7 +589,10 @@ int run(hipStream_t s, int n, const double *A, long lda, double *x)
     g.A = A;
     g.lda = lda;
     g.x = x;
-    g.vec = (((uintptr_t)A & 15) == 0) && (lda % 2 == 0);
+    // packed columns start at any 8-byte offset: element loads only
+    g.vec = !packed && (((uintptr_t)A & 15) == 0) && (lda % 2 == 0);
+    g.packed = packed;
+    g.plower = LOWER;
     const long words = g.nb > 1 ? (long)g.nb * (1 + g.nh) * kMailWords : 0;
     g.xmail = g.pmail = nullptr;
     if (words) {
@@ -597,12 +630,12 @@ __global__ void scatter_x(int n, const double *t, double *x, long incx, long x0)
 }
 
 int solve(hipStream_t s, char uplo, char trans, char diag, int n, const double *A, long lda,
-          double *x)
+          bool packed, double *x)
 {
     const bool lo = uplo == 'L' || uplo == 'l';
     const bool tr = !(trans == 'N' || trans == 'n');
     const bool un = diag == 'U' || diag == 'u';
-#define GB_TRSV(L, T, U) return run<L, T, U>(s, n, A, lda, x)
+#define GB_TRSV(L, T, U) return run<L, T, U>(s, n, A, lda, packed, x)
     if (lo) {
         if (tr) { if (un) GB_TRSV(true, true, true); else GB_TRSV(true, true, false); }
         else { if (un) GB_TRSV(true, false, true); else GB_TRSV(true, false, false); }
@@ -613,6 +646,26 @@ int solve(hipStream_t s, char uplo, char trans, char diag, int n, const double *
 #undef GB_TRSV
 }
 
+// x := op(A)^-1 x on full (packed = false) or packed storage, strided x
+// through a contiguous copy
+int solve_strided(hipStream_t s, char uplo, char trans, char diag, int n, const double *A,
+                  long lda, bool packed, double *x, int incx)
+{
+    if (incx == 1) return solve(s, uplo, trans, diag, n, A, lda, packed, x);
+    double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
+    if (!t) return (int)hipErrorOutOfMemory;
+    const long x0 = incx < 0 ? -(long)(n - 1) * incx : 0;
+    const int blocks = std::min(ceil_div(n, 256), 4096);
+    hipLaunchKernelGGL(gather_x, dim3(blocks), dim3(256), 0, s, n, x, (long)incx, x0, t);
+    int err = solve(s, uplo, trans, diag, n, A, lda, packed, t);
+    if (!err) {
+        hipLaunchKernelGGL(scatter_x, dim3(blocks), dim3(256), 0, s, n, t, x, (long)incx, x0);
+        err = (int)hipGetLastError();
+    }
+    dev_release_after(t, s);  // back to the pool once the stream's work on it is done
+    return err;
+}
+
 }  // namespace
 
 extern "C" int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag, int n,
@@ -621,22 +674,20 @@ extern "C" int gpublas_dev_dtrsv(void *stream, char uplo, char trans, char diag,
     hipStream_t s = pick_stream(stream);
     if (n <= 0) return 0;
     PROF_BEGIN(s, "dtrsv");
-    int err;
-    if (incx == 1) {
-        err = solve(s, uplo, trans, diag, n, d_A, lda, d_x);
-    } else {
-        double *t = (double *)gpublas_dev_alloc((size_t)n * sizeof(double));
-        if (!t) return (int)hipErrorOutOfMemory;
-        const long x0 = incx < 0 ? -(long)(n - 1) * incx : 0;
-        const int blocks = std::min(ceil_div(n, 256), 4096);
-        hipLaunchKernelGGL(gather_x, dim3(blocks), dim3(256), 0, s, n, d_x, (long)incx, x0, t);
-        err = solve(s, uplo, trans, diag, n, d_A, lda, t);
-        if (!err) {
-            hipLaunchKernelGGL(scatter_x, dim3(blocks), dim3(256), 0, s, n, t, d_x, (long)incx, x0);
-            err = (int)hipGetLastError();
-        }
-        dev_release_after(t, s);  // back to the pool once the stream's work on it is done
-    }
+    const int err = solve_strided(s, uplo, trans, diag, n, d_A, lda, false, d_x, incx);
+    PROF_END(s);
+    return err;
+}
+
+// DTPSV (replaces cublasDtpsv behind blas_level2/tpsv.cc): the same one-launch
+// solve reading the packed triangle in place
+extern "C" int gpublas_dev_dtpsv(void *stream, char uplo, char trans, char diag, int n,
+                                 const double *d_AP, double *d_x, int incx)
+{
+    hipStream_t s = pick_stream(stream);
+    if (n <= 0) return 0;
+    PROF_BEGIN(s, "dtpsv");
+    const int err = solve_strided(s, uplo, trans, diag, n, d_AP, 0, true, d_x, incx);
     PROF_END(s);
     return err;
 }
