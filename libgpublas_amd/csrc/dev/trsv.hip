@@ -173,10 +173,7 @@ __device__ __forceinline__ void solve_block(const double *D, double r0, double r
 // wait ends; each wait is also bounded in time, and a result that depends on
 // an expired wait is NaN, never plausible.
 typedef unsigned long long u64;
-#ifndef GB_TRSV_HELPERS
-#define GB_TRSV_HELPERS 3
-#endif
-constexpr int kTrsvHelpers = GB_TRSV_HELPERS;  // helpers per block (orders of 6+ blocks)
+constexpr int kTrsvHelpers = 3;  // helpers per block (orders of 6+ blocks)
 constexpr int kMailWords = 2 * TB;             // words per mailbox block
 constexpr u64 kTag = 1ull << 32;
 
