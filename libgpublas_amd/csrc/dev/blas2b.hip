@@ -842,19 +842,90 @@ __global__ __launch_bounds__(64) void tbsv_prop(int k, TbChunk ch)
 }
 
 // Many chunks, in two levels: the C - 1 maps form NG groups of G.
-// tbsv_compose: workgroup (g, q) walks e_q (q < k; linear part only) or 0
-// (q = k; with the offsets) through group g's maps, giving column q + 1 (or
-// the offset, column 0) of the group's map in GH (same layout as H; the
-// unused entries k + 1 .. W - 1 of each row are zeroed).
+// tbsv_compose: wave w of workgroup (g, y) walks vector q = y VW + w -- e_q
+// (q < k; linear part only) or 0 (q = k; with the offsets) -- through group
+// g's maps, giving column q + 1 (or the offset, column 0) of the group's map
+// in GH (same layout as H; the unused entries k + 1 .. W - 1 of each row are
+// zeroed).  The VW waves share each map's rows through LDS (read from
+// memory once per workgroup, not once per vector: with one wave per vector
+// the reloads made this pass memory bound), the next map's rows fetched into
+// registers while the current one is applied.
+constexpr int kTbCompWaves = 8;
+
 template <int W>
-__global__ __launch_bounds__(64) void tbsv_compose(int k, TbChunk ch, int G, double *GH)
+__global__ __launch_bounds__(64 * kTbCompWaves) void tbsv_compose(int k, TbChunk ch, int G, double *GH)
 {
-    const int g = blockIdx.x, q = blockIdx.y, p = threadIdx.x;
+    constexpr int VW = kTbCompWaves, NT = 64 * VW, NC = (W - 1 + 15) / 16;
+    constexpr int RW = (W + 1) / 2;                // double2 per row in LDS
+    constexpr int PER = (64 * RW + NT - 1) / NT;   // a thread's share of a map (k <= 64 rows)
+    __shared__ double2 rows[64][RW];
+    __shared__ double zs[VW][64];
+    const int tid = threadIdx.x, wave = tid >> 6, p = tid & 63, j16 = p & 15;
+    const int g = blockIdx.x, q = (int)blockIdx.y * VW + wave;
     const int rb = g * G, re = rb + G < ch.C - 1 ? rb + G : ch.C - 1;
-    const bool aff = q == k;
-    bool bad = false;
-    const double z = prop_walk<W>(k, ch.H, rb, re, !aff && p == q ? 1.0 : 0.0, aff, nullptr, bad);
-    if (p < k) {
+    const bool aff = q == k, live = q <= k;
+    const int tot = k * RW;
+    // this thread's elements of a map: index (row, double2 column) into H's
+    // rows and into LDS, clamped to element 0 past the map (loads stay
+    // unconditional)
+    static_assert(PER <= 4, "a thread's share of a map");
+    int off0 = 0, off1 = 0, off2 = 0, off3 = 0, lds0 = 0, lds1 = 0, lds2 = 0, lds3 = 0;
+    auto place = [&](int i, int &off, int &lds) {
+        const int e = tid + NT * i, ee = e < tot ? e : 0;
+        off = (ee / RW) * (kTbHRow / 2) + ee % RW;
+        lds = e < tot ? e : 63 * RW;  // past the map: row 63, never a map row (k <= 63)
+    };
+    place(0, off0, lds0);
+    if constexpr (PER > 1) place(1, off1, lds1);
+    if constexpr (PER > 2) place(2, off2, lds2);
+    if constexpr (PER > 3) place(3, off3, lds3);
+    double2 pre0, pre1, pre2, pre3;
+#define GB_FETCH(r_)                                                                      \
+    {                                                                                     \
+        const double2 *src = (const double2 *)(ch.H + (long)(r_) * k * kTbHRow);          \
+        pre0 = src[off0];                                                                 \
+        if constexpr (PER > 1) pre1 = src[off1];                                          \
+        if constexpr (PER > 2) pre2 = src[off2];                                          \
+        if constexpr (PER > 3) pre3 = src[off3];                                          \
+    }
+    double z = live && !aff && p == q ? 1.0 : 0.0, zr[NC];
+    zs[wave][p] = z;  // one wave's own row of zs: its LDS accesses complete in order
+#pragma unroll
+    for (int h = 0; h < NC; ++h) zr[h] = zs[wave][16 * h + j16];
+    if (re > rb) GB_FETCH(rb);
+    double2 *rflat = &rows[0][0];
+    for (int r = rb; r < re; ++r) {
+        __syncthreads();  // the previous map's rows are no longer read
+        rflat[lds0] = pre0;
+        if constexpr (PER > 1) rflat[lds1] = pre1;
+        if constexpr (PER > 2) rflat[lds2] = pre2;
+        if constexpr (PER > 3) rflat[lds3] = pre3;
+        __syncthreads();
+        GB_FETCH(r + 1 < re ? r + 1 : r);
+        if (live) {
+            double hr[2 * RW];
+            const int pr = p < k ? p : 0;
+#pragma unroll
+            for (int c2 = 0; c2 < RW; ++c2) {
+                const double2 t = rows[pr][c2];
+                hr[2 * c2] = t.x;
+                hr[2 * c2 + 1] = t.y;
+            }
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int h = 0; h < NC; ++h) {
+                if (W - 1 - 16 * h >= 16) dpp_dot16(acc, zr[h], &hr[1 + 16 * h]);
+                else dpp_dot15(acc, zr[h], &hr[1 + 16 * h]);
+            }
+            double nz = (aff ? hr[0] : 0.0) + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+            z = p < k ? nz : 0.0;
+            zs[wave][p] = z;
+#pragma unroll
+            for (int h = 0; h < NC; ++h) zr[h] = zs[wave][16 * h + j16];
+        }
+    }
+#undef GB_FETCH
+    if (live && p < k) {
         double *row = GH + ((long)g * k + p) * kTbHRow;
         row[aff ? 0 : q + 1] = z;
         if (aff)
@@ -992,7 +1063,8 @@ void tbsv_chunk_passes(hipStream_t s, const TbArgs &a, const TbChunk &ch, double
         hipLaunchKernelGGL((tbsv_prop<W>), dim3(1), dim3(64), 0, s, a.k, ch);
         return;
     }
-    hipLaunchKernelGGL((tbsv_compose<W>), dim3(NG, a.k + 1), dim3(64), 0, s, a.k, ch, G, GH);
+    hipLaunchKernelGGL((tbsv_compose<W>), dim3(NG, ceil_div(a.k + 1, kTbCompWaves)),
+                       dim3(64 * kTbCompWaves), 0, s, a.k, ch, G, GH);
     hipLaunchKernelGGL((tbsv_groups<W>), dim3(1), dim3(64), 0, s, a.k, ch, NG, (const double *)GH,
                        ZG);
     hipLaunchKernelGGL((tbsv_spread<W>), dim3(NG), dim3(64), 0, s, a.k, ch, G, NG,
