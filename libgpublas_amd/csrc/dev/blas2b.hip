@@ -856,9 +856,13 @@ template <int W>
 __global__ __launch_bounds__(64 * kTbCompWaves) void tbsv_compose(int k, TbChunk ch, int G, double *GH)
 {
     constexpr int VW = kTbCompWaves, NT = 64 * VW, NC = (W - 1 + 15) / 16;
-    constexpr int RW = (W + 1) / 2;                // double2 per row in LDS
+    constexpr int RW = (W + 1) / 2;                // double2 per row
+    // LDS row stride: an odd number of double2, so that the 64 lanes reading
+    // one column of their rows spread over the banks (32 double2 put every
+    // lane in one bank)
+    constexpr int RWP = RW | 1;
     constexpr int PER = (64 * RW + NT - 1) / NT;   // a thread's share of a map (k <= 64 rows)
-    __shared__ double2 rows[64][RW];
+    __shared__ double2 rows[64][RWP];
     __shared__ double zs[VW][64];
     const int tid = threadIdx.x, wave = tid >> 6, p = tid & 63, j16 = p & 15;
     const int g = blockIdx.x, q = (int)blockIdx.y * VW + wave;
@@ -873,7 +877,7 @@ __global__ __launch_bounds__(64 * kTbCompWaves) void tbsv_compose(int k, TbChunk
     auto place = [&](int i, int &off, int &lds) {
         const int e = tid + NT * i, ee = e < tot ? e : 0;
         off = (ee / RW) * (kTbHRow / 2) + ee % RW;
-        lds = e < tot ? e : 63 * RW;  // past the map: row 63, never a map row (k <= 63)
+        lds = e < tot ? (e / RW) * RWP + e % RW : 63 * RWP;  // past the map: row 63 (k <= 63)
     };
     place(0, off0, lds0);
     if constexpr (PER > 1) place(1, off1, lds1);
