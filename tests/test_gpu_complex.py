@@ -8,7 +8,7 @@ Bars:
   * elementwise gauge, the complex form of netlib's DMMCH test
     (tests/netlib/zblat3.f ZMMCH): |C - Cref| <= 16 (k + 2) eps G with
     G = |alpha| |op(A)| |op(B)| + |beta| |C0|;
-  * the north star's normwise bound ||C - Cref|| / (||A|| ||B|| k eps) <= 4
+  * the normwise bound ||C - Cref|| / (||A|| ||B|| k eps) <= 4
     at the larger shape;
   * bit-exact results on small-integer data, where every summation order
     gives the same answer;
@@ -117,8 +117,8 @@ def test_complex_gemm_netlib_scalars(gpu, ora, prec):
 
 
 def test_zgemm_normwise_2048(gpu):
-    """ZGEMM 2048^3 in the conjugate-transpose form: the north star's
-    normwise bound against numpy's product of the same data."""
+    """ZGEMM 2048^3 in the conjugate-transpose form: the normwise bound
+    ||C - Cref|| / (||A|| ||B|| k eps) <= 4 against numpy's product of the same data."""
     n = 2048
     A, B, C0 = (_cplx(s, (n, n), np.complex128) for s in (60, 61, 62))
     alpha, beta = 0.9 + 0.1j, 0.5 - 0.5j
