@@ -414,7 +414,7 @@ def main():
                                                 "(calls/s; 0 = closed loop)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one MI355X: this build has no multi-GPU path "
+        print(f"bench.py measures one MI355X: the library drives one GPU per process "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     if args.suite == "level3":

@@ -1,6 +1,6 @@
 """The bench driver's command-line contract, checked without a GPU: the file
 compiles, and a request for more than one GPU is refused with a message
-before any device work (this build drives one GPU)."""
+before any device work (the library drives one GPU per process)."""
 import os
 import py_compile
 import subprocess
